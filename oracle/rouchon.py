@@ -1,0 +1,145 @@
+"""Rouchon1 (Kraus-Euler) with Cholesky normalisation, in complex128 (oracle).
+
+Restates ``dynamiqs/integrators/core/rouchon_integrator.py``:
+
+* ``KrausEuler.U`` :250-253: ``Hnh = H(t_mid) - 1/2 i sum L^dag L(t_mid)``,
+  ``M0 = I - i dt Hnh`` with ``t_mid = t + dt/2``;
+* ``KrausEuler.__call__`` :255-257: ``rho' = M0 rho M0^dag + dt sum L rho L^dag``
+  (jump operators at ``t_mid``);
+* ``KrausEuler.S`` :259-263: ``S = M0^dag M0 + dt sum L^dag L``;
+* ``cholesky_normalize`` :552-582: ``S = T T^dag`` (T lower), then
+  ``rho <- T^{-dag} rho T^{-1}`` by two triangular solves;
+* ``rouchon_step`` :394-407 (normalise, then apply the Kraus map), fixed steps
+  of ``method.dt`` (``ConstantStepSize``, ``diffrax_integrator.py:61-62, 74-76``)
+  with linear interpolation to ``tsave`` (``RouchonDXSolver.interpolation_cls =
+  LocalLinearInterpolation`` :44).
+
+Two implementations of the same step: :func:`kraus_step_dense` (dense LAPACK
+Cholesky / triangular solves, exactly the reference's calls) and
+:func:`kraus_step_banded` (banded Cholesky and banded solves for DIA operators).
+Pinned in ``tests/test_oracle_rouchon.py``: the two agree to rounding; trace
+preservation to machine precision; complete positivity (normalised Kraus
+operators satisfy ``sum A_k^dag A_k = I``); first-order convergence to the exact
+Lindblad solution (``expm`` of the superoperator) on the lossy oscillator.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import scipy.linalg as sla
+
+from .lindblad import dag
+
+
+def kraus_ops(H: np.ndarray, Ls, dt: float):
+    """Return ``(M0, [sqrt(dt) L_k], S)`` of one Kraus-Euler step."""
+    n = H.shape[-1]
+    LdL = sum((dag(L) @ L for L in Ls), np.zeros_like(H))
+    Hnh = H - 0.5j * LdL
+    M0 = np.eye(n) - 1j * dt * Hnh
+    S = dag(M0) @ M0 + dt * LdL
+    return M0, [np.sqrt(dt) * L for L in Ls], S
+
+
+def cholesky_normalize(S: np.ndarray, rho: np.ndarray) -> np.ndarray:
+    T = np.linalg.cholesky(S)  # lower triangular, S = T T^dag
+    x = sla.solve_triangular(T, rho, lower=True, trans='C')  # T^dag x = rho
+    # x T = rho'  <=>  T^T x^T = rho'^T
+    return sla.solve_triangular(T, x.T, lower=True, trans='T').T
+
+
+def kraus_step_dense(H: np.ndarray, Ls, rho: np.ndarray, dt: float, normalize: bool = True) -> np.ndarray:
+    M0, Ms, S = kraus_ops(H, Ls, dt)
+    if normalize:
+        rho = cholesky_normalize(S, rho)
+    out = M0 @ rho @ dag(M0)
+    for M in Ms:
+        out = out + M @ rho @ dag(M)
+    return out
+
+
+def normalized_kraus(H: np.ndarray, Ls, dt: float, normalize: bool = True):
+    """Kraus operators with the normalisation folded in: ``A_k = M_k T^{-dag}``,
+    so that one step is ``rho' = sum_k A_k rho A_k^dag`` and ``sum A_k^dag A_k = I``."""
+    M0, Ms, S = kraus_ops(H, Ls, dt)
+    if not normalize:
+        return [M0, *Ms]
+    T = np.linalg.cholesky(S)
+    Tinv_dag = dag(sla.solve_triangular(T, np.eye(T.shape[0]), lower=True))
+    return [M @ Tinv_dag for M in (M0, *Ms)]
+
+
+def _to_lower_band(S: np.ndarray, w: int) -> np.ndarray:
+    n = S.shape[0]
+    ab = np.zeros((w + 1, n), dtype=S.dtype)
+    for k in range(w + 1):
+        ab[k, : n - k] = np.diagonal(S, offset=-k)
+    return ab
+
+
+def banded_factor(S: np.ndarray, w: int):
+    """Lower banded Cholesky factor of a Hermitian matrix with half-bandwidth ``w``
+    in LAPACK lower-band storage (``ab[k, j] = T[j + k, j]``)."""
+    return sla.cholesky_banded(_to_lower_band(S, w), lower=True)
+
+
+def banded_normalize(Tab: np.ndarray, rho: np.ndarray) -> np.ndarray:
+    """``T^{-dag} rho T^{-1}`` with ``T`` given in lower-band storage."""
+    w = Tab.shape[0] - 1
+    n = Tab.shape[1]
+    # upper-band storage of T^dag: ab_u[w + i - j, j] = T^dag[i, j] = conj(T[j, i])
+    ab_u = np.zeros_like(Tab)
+    for k in range(w + 1):
+        ab_u[w - k, k:] = Tab[k, : n - k].conj()
+    x = sla.solve_banded((0, w), ab_u, rho)  # T^dag x = rho
+    # x T = y  <=>  T^dag x^dag = y^dag, and x^dag = x here up to rounding; solve exactly
+    y = sla.solve_banded((0, w), ab_u, dag(x))
+    return dag(y)
+
+
+def kraus_step_banded(H: np.ndarray, Ls, rho: np.ndarray, dt: float, w: int, Tab=None) -> np.ndarray:
+    M0, Ms, S = kraus_ops(H, Ls, dt)
+    if Tab is None:
+        Tab = banded_factor(S, w)
+    rho = banded_normalize(Tab, rho)
+    out = M0 @ rho @ dag(M0)
+    for M in Ms:
+        out = out + M @ rho @ dag(M)
+    return out
+
+
+def step_times(t0: float, t1: float, dt: float) -> np.ndarray:
+    """Step boundaries ``t0, t0+dt, ...`` with the last step clipped to ``t1``."""
+    n = (t1 - t0) / dt
+    nr = int(round(n))
+    if abs(nr - n) <= 1e-9 * max(1.0, n):
+        nsteps = max(nr, 1)
+    else:
+        nsteps = int(np.ceil(n))
+    ts = t0 + dt * np.arange(nsteps + 1, dtype=np.float64)
+    ts[-1] = t1
+    return ts
+
+
+def integrate(step, rho0: np.ndarray, tsave: np.ndarray, dt: float, *, t0=None, save=None):
+    """Fixed-step loop; ``step(t, h, rho) -> rho'``. Linear interpolation at tsave."""
+    tsave = np.asarray(tsave, dtype=np.float64)
+    t_start = float(tsave[0] if t0 is None else t0)
+    ts = step_times(t_start, float(tsave[-1]), dt)
+    save = save or (lambda y: y.copy())
+    rho = np.array(rho0, dtype=np.complex128)
+    saved = []
+    isave = 0
+    while isave < len(tsave) and tsave[isave] <= t_start:
+        saved.append(save(rho))
+        isave += 1
+    for i in range(len(ts) - 1):
+        t, tn = ts[i], ts[i + 1]
+        new = step(t, tn - t, rho)
+        while isave < len(tsave) and tsave[isave] <= tn + 1e-12 * max(1.0, abs(tn)):
+            theta = (tsave[isave] - t) / (tn - t)
+            theta = min(max(theta, 0.0), 1.0)
+            saved.append(save(rho + theta * (new - rho)))
+            isave += 1
+        rho = new
+    return saved, rho, len(ts) - 1

@@ -1,0 +1,405 @@
+"""Pins of the CPU oracle (``oracle/``) against the reference's KATs, closed forms,
+library routines and order conditions. CPU only."""
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import scipy.sparse as sp
+
+from oracle import dia, lindblad, operators as op, rouchon, tsit5
+from oracle import mesolve as om
+
+# ----------------------------------------------------------------- operators
+
+
+def test_operator_docstring_kats(kats):
+    k = kats['coherent_4_0.5']
+    np.testing.assert_allclose(op.coherent(4, 0.5)[:, 0].real, k['ket_real'], atol=k['atol'])
+    k = kats['coherent_dm_4_0.5']
+    np.testing.assert_allclose(op.coherent_dm(4, 0.5).real, k['real'], atol=k['atol'])
+    k = kats['displace_4_0.5']
+    np.testing.assert_allclose(op.displace(4, 0.5).real, k['real'], atol=k['atol'])
+    k = kats['position_3']
+    np.testing.assert_allclose(op.position(3).real, k['real'], atol=k['atol'])
+    k = kats['momentum_3']
+    np.testing.assert_allclose(op.momentum(3).imag, k['imag'], atol=k['atol'])
+
+
+def test_coherent_poisson_statistics():
+    # |<m|alpha>|^2 -> Poisson(|alpha|^2) as the truncation grows (closed form)
+    from math import factorial
+
+    alpha = 1.3 + 0.4j
+    psi = op.coherent(60, alpha)[:, 0]
+    m = np.arange(12)
+    lam = abs(alpha) ** 2
+    poisson = np.exp(-lam) * lam**m / np.array([factorial(int(i)) for i in m])
+    np.testing.assert_allclose(np.abs(psi[:12]) ** 2, poisson, atol=1e-12)
+
+
+def test_multimode_ordering():
+    a, b = op.destroy(3, 4)
+    np.testing.assert_allclose(a, np.kron(op.destroy(3), np.eye(4)))
+    np.testing.assert_allclose(b, np.kron(np.eye(3), op.destroy(4)))
+    # DIA offsets of the two-mode config: a -> +n_b, b -> +1, a^dag b -> -(n_b - 1)
+    offs, _ = dia.dia_from_dense(a)
+    assert offs == (4,)
+    offs, _ = dia.dia_from_dense(b)
+    assert offs == (1,)
+    offs, _ = dia.dia_from_dense(a.conj().T @ b)
+    assert offs == (-3,)
+
+
+# ----------------------------------------------------------------------- DIA
+
+
+def _rand_dia(rng, offsets, n, batch=()):
+    d = rng.normal(size=(*batch, len(offsets), n)) + 1j * rng.normal(size=(*batch, len(offsets), n))
+    # zero the out-of-bounds entries as the reference fixture does
+    for i, o in enumerate(offsets):
+        if o > 0:
+            d[..., i, :o] = 0
+        elif o < 0:
+            d[..., i, n + o :] = 0
+    return tuple(offsets), d
+
+
+@pytest.mark.parametrize('offsets', [(-2, 0, 2), (-2, -1, 1, 3)])
+def test_dia_matches_scipy_dia_matrix(kats, offsets):
+    n = kats['sparse_dia_fixture_offsets']['N']
+    rng = np.random.default_rng(0)
+    offs, d = _rand_dia(rng, offsets, n)
+    ref = sp.dia_matrix((d, np.array(offs)), shape=(n, n)).toarray()
+    np.testing.assert_array_equal(dia.dia_to_dense(offs, d), ref)
+    offs2, d2 = dia.dia_from_dense(ref, offs)
+    np.testing.assert_array_equal(d2, d)
+
+
+@pytest.mark.parametrize('lo,ro', [((-2, 0, 2), (-2, -1, 1, 3)), ((1,), (-3, 0)), ((0,), (0,))])
+def test_dia_products_vs_dense(lo, ro):
+    n = 10
+    rng = np.random.default_rng(1)
+    lo, ld = _rand_dia(rng, lo, n)
+    ro, rd = _rand_dia(rng, ro, n)
+    L = dia.dia_to_dense(lo, ld)
+    Rm = dia.dia_to_dense(ro, rd)
+    x = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    np.testing.assert_allclose(dia.dia_matmul_dense(lo, ld, x), L @ x, atol=1e-12)
+    np.testing.assert_allclose(dia.dense_matmul_dia(x, lo, ld), x @ L, atol=1e-12)
+    o, d = dia.dia_matmul_dia(lo, ld, ro, rd)
+    np.testing.assert_allclose(dia.dia_to_dense(o, d), L @ Rm, atol=1e-12)
+    o, d = dia.dia_dag(lo, ld)
+    np.testing.assert_allclose(dia.dia_to_dense(o, d), L.conj().T, atol=0)
+
+
+# ------------------------------------------------------------------ Lindblad
+
+
+def test_dissipator_lindbladian_kats(kats):
+    a = op.destroy(4)
+    out = lindblad.dissipator(a, op.fock_dm(4, 2))
+    np.testing.assert_allclose(out, np.diag(kats['dissipator_destroy4_fockdm4_2']['diag_real']), atol=1e-12)
+    N = a.conj().T @ a
+    out = lindblad.lindbladian(N, [a, N], op.fock_dm(4, 1))
+    np.testing.assert_allclose(out, np.diag(kats['lindbladian_num4_[a,num]_fockdm4_1']['diag_real']), atol=1e-12)
+
+
+def _rand_problem(rng, n, K):
+    H = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    H = H + H.conj().T
+    Ls = [rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)) for _ in range(K)]
+    psi = rng.normal(size=(n, 1)) + 1j * rng.normal(size=(n, 1))
+    rho = psi @ psi.conj().T
+    return H, Ls, rho / np.trace(rho)
+
+
+def test_rhs_forms_agree_and_invariants():
+    rng = np.random.default_rng(2)
+    H, Ls, rho = _rand_problem(rng, 5, 2)
+    a = lindblad.rhs_standard(H, Ls, rho)
+    b = lindblad.rhs_hermitian(H, Ls, rho)
+    c = lindblad.unvectorize(lindblad.slindbladian(H, Ls) @ lindblad.vectorize(rho))
+    d = lindblad.lindbladian(H, Ls, rho)
+    for x in (b, c, d):
+        np.testing.assert_allclose(x, a, atol=1e-10)
+    assert abs(np.trace(a)) < 1e-10
+    np.testing.assert_allclose(a, a.conj().T, atol=1e-10)
+
+
+def test_rhs_brute_force_index_sum():
+    rng = np.random.default_rng(3)
+    n = 3
+    H, Ls, rho = _rand_problem(rng, n, 1)
+    L = Ls[0]
+    out = np.zeros((n, n), dtype=complex)
+    for i in range(n):
+        for j in range(n):
+            s = 0
+            for k in range(n):
+                s += -1j * (H[i, k] * rho[k, j] - rho[i, k] * H[k, j])
+                for m in range(n):
+                    s += L[i, k] * rho[k, m] * np.conj(L[j, m])
+                    s -= 0.5 * np.conj(L[k, i]) * L[k, m] * rho[m, j]
+                    s -= 0.5 * rho[i, k] * np.conj(L[m, k]) * L[m, j]
+            out[i, j] = s
+    np.testing.assert_allclose(lindblad.rhs_hermitian(H, Ls, rho), out, atol=1e-12)
+
+
+def test_adjoint_lindbladian_is_adjoint():
+    rng = np.random.default_rng(4)
+    H, Ls, rho = _rand_problem(rng, 4, 2)
+    O = rng.normal(size=(4, 4)) + 1j * rng.normal(size=(4, 4))
+    lhs = np.trace(O.conj().T @ lindblad.lindbladian(H, Ls, rho))
+    rhs = np.trace(lindblad.adjoint_lindbladian(H, Ls, O).conj().T @ rho)
+    assert abs(lhs - rhs) < 1e-10
+
+
+# --------------------------------------------------------------------- Tsit5
+
+
+def _order_conditions(b, A, c):
+    Ac, Ac2, Ac3 = A @ c, A @ c**2, A @ c**3
+    AAc = A @ Ac
+    return {
+        (1, 'b'): (b.sum(), 1),
+        (2, 'bc'): (b @ c, 1 / 2),
+        (3, 'bc2'): (b @ c**2, 1 / 3),
+        (3, 'bAc'): (b @ Ac, 1 / 6),
+        (4, 'bc3'): (b @ c**3, 1 / 4),
+        (4, 'bcAc'): (b @ (c * Ac), 1 / 8),
+        (4, 'bAc2'): (b @ Ac2, 1 / 12),
+        (4, 'bAAc'): (b @ AAc, 1 / 24),
+        (5, 'bc4'): (b @ c**4, 1 / 5),
+        (5, 'bc2Ac'): (b @ (c**2 * Ac), 1 / 10),
+        (5, 'b(Ac)^2'): (b @ Ac**2, 1 / 20),
+        (5, 'bcAc2'): (b @ (c * Ac2), 1 / 15),
+        (5, 'bcAAc'): (b @ (c * AAc), 1 / 30),
+        (5, 'bAc3'): (b @ Ac3, 1 / 20),
+        (5, 'bA(cAc)'): (b @ (A @ (c * Ac)), 1 / 40),
+        (5, 'bAAc2'): (b @ (A @ Ac2), 1 / 60),
+        (5, 'bAAAc'): (b @ (A @ AAc), 1 / 120),
+    }
+
+
+def test_tsit5_tableau_order_conditions():
+    A, b, c, e = tsit5.A, tsit5.B, tsit5.C, tsit5.E
+    np.testing.assert_allclose(A.sum(axis=1), c, atol=1e-14)
+    conds = _order_conditions(b, A, c)
+    assert len(conds) == 17
+    for key, (val, want) in conds.items():
+        assert abs(val - want) < 1e-12, key
+    bhat = b - e
+    for key, (val, want) in _order_conditions(bhat, A, c).items():
+        if key[0] <= 4:
+            assert abs(val - want) < 1e-12, key
+    # the embedded pair is genuinely of order 4 (not 5)
+    assert any(abs(v - w) > 1e-6 for k, (v, w) in _order_conditions(bhat, A, c).items() if k[0] == 5)
+
+
+@pytest.mark.parametrize('theta', [0.1, 0.37, 0.5, 0.8, 1.0])
+def test_tsit5_interpolant_conditions(theta):
+    A, c = tsit5.A, tsit5.C
+    w = tsit5.interp_weights(theta)
+    Ac = A @ c
+    checks = [
+        (w.sum(), theta),
+        (w @ c, theta**2 / 2),
+        (w @ c**2, theta**3 / 3),
+        (w @ Ac, theta**3 / 6),
+        (w @ c**3, theta**4 / 4),
+        (w @ (c * Ac), theta**4 / 8),
+        (w @ (A @ c**2), theta**4 / 12),
+        (w @ (A @ Ac), theta**4 / 24),
+    ]
+    for val, want in checks:
+        assert abs(val - want) < 1e-12
+    if theta == 1.0:
+        np.testing.assert_allclose(w, tsit5.B, atol=1e-13)
+
+
+def test_tsit5_fifth_order_convergence():
+    # fixed steps on a linear test problem: error ratio ~ 2^5 when halving dt
+    lam = -1.0 + 3.0j
+
+    def run(h):
+        y = 1.0 + 0j
+        for _ in range(int(round(1.0 / h))):
+            ks = [h * lam * y]
+            for s in range(1, 7):
+                ks.append(h * lam * (y + sum(tsit5.A[s, j] * ks[j] for j in range(s))))
+            y = y + sum(tsit5.B[j] * ks[j] for j in range(6))
+        return abs(y - np.exp(lam))
+
+    r = run(0.05) / run(0.025)
+    assert 24 < r < 40
+
+
+def _ocavity(n, delta, kappa, alpha0):
+    a = op.destroy(n)
+    return delta * a.conj().T @ a, [np.sqrt(kappa) * a], op.coherent(n, alpha0)
+
+
+def test_tsit5_closed_form_ocavity(kats):
+    k = kats['ocavity']
+    H, Ls, psi0 = _ocavity(k['n'], k['delta'], k['kappa'], k['alpha0'])
+    tsave = np.linspace(0, k['t_end'], k['ntsave'])
+    Es = [op.position(k['n']), op.momentum(k['n'])]
+    r = om.mesolve(H, Ls, psi0, tsave, exp_ops=Es)
+    for i, t in enumerate(tsave):
+        al = k['alpha0'] * np.exp(-1j * k['delta'] * t - 0.5 * k['kappa'] * t)
+        err = np.linalg.norm(r['states'][i] - op.coherent_dm(k['n'], al))
+        assert err <= k['ysave_atol']
+        np.testing.assert_allclose(r['expects'][:, i], [al.real, al.imag], rtol=k['esave_rtol'], atol=k['esave_atol'])
+
+
+def test_tsit5_otdqubit_closed_form(kats):
+    k = kats['otdqubit']
+    eps, w, g = k['eps'], k['omega'], k['gamma']
+    sx = op.sigmax()
+    H = lambda t: eps * np.cos(w * t) * sx  # noqa: E731
+    # the oracle accepts callables through TimeOp-like objects
+    Hop = om.TimeOp(np.zeros((2, 2)))
+    Hop.__call__  # noqa: B018
+    tsave = np.linspace(0, k['t_end'], k['ntsave'])
+
+    class CallableOp(om.TimeOp):
+        def __call__(self, t):
+            return H(t)
+
+    r = om.mesolve(CallableOp(np.zeros((2, 2))), [np.sqrt(g) * sx], op.fock(2, 0), tsave)
+    for i, t in enumerate(tsave):
+        th = 2 * eps / w * np.sin(w * t)
+        eta = np.exp(-2 * g * t)
+        want = np.array(
+            [[0.5 * (1 + eta * np.cos(th)), 0.5j * eta * np.sin(th)], [-0.5j * eta * np.sin(th), 0.5 * (1 - eta * np.cos(th))]]
+        )
+        assert np.linalg.norm(r['states'][i] - want) < 1e-4
+
+
+def test_tsit5_vs_superoperator_expm():
+    # constant generator: exact solution expm(t L) vec(rho0) (the reference's Expm method)
+    rng = np.random.default_rng(5)
+    H, Ls, rho0 = _rand_problem(rng, 6, 2)
+    H /= np.linalg.norm(H, 2)
+    Ls = [0.3 * L / np.linalg.norm(L, 2) for L in Ls]
+    tsave = np.linspace(0, 3, 7)
+    r = om.mesolve(H, Ls, rho0, tsave, rtol=1e-10, atol=1e-12)
+    Lsup = lindblad.slindbladian(H, Ls)
+    for i, t in enumerate(tsave):
+        ex = lindblad.unvectorize(sla.expm(t * Lsup) @ lindblad.vectorize(rho0))
+        assert np.linalg.norm(r['states'][i] - ex) < 1e-8
+
+
+def test_tsit5_readme_kat(kats):
+    k = kats['readme_lossy_oscillator']
+    n = k['n']
+    a = op.destroy(n)
+    tsave = np.linspace(0, k['T'], k['ntsave'])
+    r = om.mesolve(k['omega'] * a.conj().T @ a, [np.sqrt(k['kappa']) * a], op.coherent(n, k['alpha0']), tsave)
+    # the state stays coherent: alpha(T) = alpha0 exp(-kappa T / 2) exp(-i omega T)
+    al = k['alpha0'] * np.exp(-k['kappa'] * k['T'] / 2 - 1j * k['omega'] * k['T'])
+    # Tsit5 global error at rtol = atol = 1e-6 over 40 steps is ~1e-4 in Frobenius norm
+    assert np.linalg.norm(r['final_state'] - op.coherent_dm(n, al)) < 2e-4
+    # printed step counts are matched to within one step (diffrax's initial-step rule is unpinned)
+    assert abs(r['infos']['nsteps'] - k['nsteps']) <= 1
+    assert r['infos']['nrejected'] == k['nrejected']
+    d = kats['mesolve_docstring']
+    r = om.mesolve(a.conj().T @ a, [a], op.coherent(n, d['alpha0']), np.linspace(0, d['T'], d['ntsave']))
+    assert abs(r['infos']['nsteps'] - d['nsteps']) <= 1
+
+
+def test_qubit_decay_doc_kat(kats):
+    r = om.mesolve(op.sigmaz(), [op.sigmam()], op.fock(2, 0), np.linspace(0, 1, 11))
+    np.testing.assert_allclose(r['final_state'].real, kats['qubit_decay']['final_real'], atol=kats['qubit_decay']['atol'])
+
+
+def test_max_steps_error():
+    a = op.destroy(4)
+    with pytest.raises(tsit5.MaxStepsError):
+        om.mesolve(a.conj().T @ a, [a], op.fock(4, 2), np.linspace(0, 10, 3), max_steps=3)
+
+
+def test_pwc_semantics_and_jumps():
+    # left-closed intervals, zero outside [times[0], times[-1]) (tests/core/test_time_qarray.py:285-292)
+    term = om.PWCTerm(np.array([0.0, 1.0, 2.0]), np.array([3.0, 5.0]), np.eye(2))
+    assert term.coeff(-0.1) == 0 and term.coeff(0.0) == 3 and term.coeff(0.999) == 3
+    assert term.coeff(1.0) == 5 and term.coeff(1.999) == 5 and term.coeff(2.0) == 0
+    # piecewise-constant drive: exact solution by products of segment propagators
+    n = 5
+    a = op.destroy(n)
+    times = np.array([0.0, 0.4, 0.7, 1.5])
+    vals = np.array([0.3, -0.8, 0.5])
+    H = om.TimeOp(a.conj().T @ a, [om.PWCTerm(times, vals, a + a.conj().T)])
+    Ls = [np.sqrt(0.2) * a]
+    rho0 = op.fock_dm(n, 1)
+    tsave = np.array([0.0, 0.5, 1.5])
+    r = om.mesolve(H, Ls, rho0, tsave, rtol=1e-10, atol=1e-12)
+    v = lindblad.vectorize(rho0)
+    out = [v]
+    bounds = [0.0, 0.4, 0.5, 0.7, 1.5]
+    for t0_, t1_ in zip(bounds[:-1], bounds[1:]):
+        Hs = H(t0_)
+        v = sla.expm((t1_ - t0_) * lindblad.slindbladian(Hs, Ls)) @ v
+        out.append(v)
+    np.testing.assert_allclose(r['states'][1], lindblad.unvectorize(out[2]), atol=1e-8)
+    np.testing.assert_allclose(r['states'][2], lindblad.unvectorize(out[4]), atol=1e-8)
+
+
+# ------------------------------------------------------------------ Rouchon1
+
+
+def test_rouchon_banded_matches_dense():
+    rng = np.random.default_rng(6)
+    n = 24
+    a, b = op.destroy(4, 6)
+    H = 1.3 * a.conj().T @ a + 0.7 * b.conj().T @ b + 0.2 * (a.conj().T @ b + a @ b.conj().T)
+    Ls = [np.sqrt(0.1) * a, np.sqrt(0.3) * b]
+    psi = rng.normal(size=(n, 1)) + 1j * rng.normal(size=(n, 1))
+    rho = psi @ psi.conj().T / np.vdot(psi, psi)
+    dt = 0.05
+    M0, Ms, S = rouchon.kraus_ops(H, Ls, dt)
+    offs, _ = dia.dia_from_dense(S)
+    w = max(abs(o) for o in offs)
+    x = rouchon.kraus_step_dense(H, Ls, rho, dt)
+    y = rouchon.kraus_step_banded(H, Ls, rho, dt, w)
+    np.testing.assert_allclose(y, x, atol=1e-13)
+    # folded form sum A rho A^dag, and completeness sum A^dag A = I
+    As = rouchon.normalized_kraus(H, Ls, dt)
+    z = sum(A @ rho @ A.conj().T for A in As)
+    np.testing.assert_allclose(z, x, atol=1e-13)
+    np.testing.assert_allclose(sum(A.conj().T @ A for A in As), np.eye(n), atol=1e-13)
+    assert abs(np.trace(x) - 1) < 1e-13
+
+
+def test_rouchon_cholesky_matches_reference_formula():
+    # cholesky_normalize(S, rho) == T^{-dag} rho T^{-1} with S = T T^dag
+    rng = np.random.default_rng(7)
+    n = 6
+    X = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    S = np.eye(n) + 0.1 * X.conj().T @ X
+    rho = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    T = np.linalg.cholesky(S)
+    Ti = np.linalg.inv(T)
+    np.testing.assert_allclose(rouchon.cholesky_normalize(S, rho), Ti.conj().T @ rho @ Ti, atol=1e-12)
+
+
+def test_rouchon1_first_order_and_reference_tolerance(kats):
+    k = kats['ocavity']
+    n = k['n']
+    H, Ls, psi0 = _ocavity(n, k['delta'], k['kappa'], k['alpha0'])
+    tsave = np.linspace(0, k['t_end'], k['ntsave'])
+    errs = []
+    for dt in (4e-3, 2e-3, 1e-3):
+        r = om.mesolve(H, Ls, psi0, tsave, method='rouchon1', dt=dt)
+        Lsup = lindblad.slindbladian(H, Ls)
+        ex = lindblad.unvectorize(sla.expm(tsave[-1] * Lsup) @ lindblad.vectorize(op.todm(psi0)))
+        errs.append(np.linalg.norm(r['final_state'] - ex))
+        assert abs(np.trace(r['final_state']) - 1) < 1e-12
+        assert r['infos']['nsteps'] == int(round(tsave[-1] / dt))
+    assert 1.7 < errs[0] / errs[1] < 2.3 and 1.7 < errs[1] / errs[2] < 2.3
+    # the reference's own test step dt = 1e-4 meets its ysave_atol = 1e-3
+    dt = kats['rouchon1_test_dt']['dt']
+    r = om.mesolve(H, Ls, psi0, tsave, method='rouchon1', dt=dt)
+    for i, t in enumerate(tsave):
+        al = k['alpha0'] * np.exp(-1j * k['delta'] * t - 0.5 * k['kappa'] * t)
+        assert np.linalg.norm(r['states'][i] - op.coherent_dm(n, al)) < kats['rouchon1_test_dt']['ysave_atol']
