@@ -1,0 +1,110 @@
+"""ctypes binding of ``libdqh.so`` (the HIP C-ABI declared in ``include/dqh.h``).
+
+The product path has no CPU fallback: if the library is missing or cannot be
+loaded, :func:`lib` raises ``RuntimeError``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'libdqh.so')
+
+# numpy mirror of ``dqh_me_inst`` (all fields 8 bytes, no padding)
+ME_INST_FIELDS = [
+    ('n', np.int64),
+    ('np', np.int64),
+    ('nL', np.int64),
+    ('nE', np.int64),
+    ('ntsave', np.int64),
+    ('npwc', np.int64),
+    ('nseg', np.int64),
+    ('flags', np.int64),
+    ('rtol', np.float64),
+    ('atol', np.float64),
+    ('safety', np.float64),
+    ('factormin', np.float64),
+    ('factormax', np.float64),
+    ('max_steps', np.int64),
+    ('t0', np.float64),
+    ('t1', np.float64),
+    ('tsave', np.uint64),
+    ('Hnh', np.uint64),
+    ('Dops', np.uint64),
+    ('Cops', np.uint64),
+    ('EopsT', np.uint64),
+    ('Hpwc', np.uint64),
+    ('pwc_times', np.uint64),
+    ('pwc_vals', np.uint64),
+    ('step_t', np.uint64),
+    ('step_set', np.uint64),
+    ('y', np.uint64),
+    ('work', np.uint64),
+    ('states', np.uint64),
+    ('expects', np.uint64),
+    ('final_state', np.uint64),
+    ('stats', np.uint64),
+]
+ME_INST_DTYPE = np.dtype(ME_INST_FIELDS)
+
+FLAG_SAVE_STATES = 1
+FLAG_HAS_HNH = 2
+
+STATUS_OK = 0
+STATUS_MAX_STEPS = -3
+STATUS_NONFINITE = -4
+
+# exported symbols (name, restype, argtypes); kept in sync with include/dqh.h
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+SYMBOLS = {
+    'dqh_sizeof_me_inst': (_I64, []),
+    'dqh_me_work_elems': (_I64, [_I64, _I64, _I64]),
+    'dqh_lindblad_rhs_dense': (ctypes.c_int, [_P, _I64, _P]),
+    'dqh_mesolve_tsit5_dense': (ctypes.c_int, [_P, _I64, _P]),
+    'dqh_mesolve_rouchon1_dense': (ctypes.c_int, [_P, _I64, _P]),
+    'dqh_last_error': (ctypes.c_char_p, []),
+    'dqh_device_info': (ctypes.c_int, [ctypes.c_char_p, _I64, ctypes.POINTER(_I64)]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load ``libdqh.so`` once; raise if it is missing (no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f'{LIB_PATH} is missing: build the HIP extension first '
+                '(python -c "import __graft_entry__ as g; g.build()")'
+            )
+        handle = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SYMBOLS.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        if handle.dqh_sizeof_me_inst() != ME_INST_DTYPE.itemsize:
+            raise RuntimeError('dqh_me_inst layout mismatch between include/dqh.h and dynamiqs_amd/_native.py')
+        _lib = handle
+    return _lib
+
+
+def check(code: int, what: str) -> None:
+    if code != 0:
+        msg = lib().dqh_last_error().decode(errors='replace')
+        raise RuntimeError(f'{what} failed with status {code}: {msg}')
+
+
+def work_elems(np_: int, nL: int, npwc: int) -> int:
+    return int(lib().dqh_me_work_elems(np_, nL, npwc))
+
+
+def current_stream_ptr():
+    import torch
+
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)

@@ -1,0 +1,619 @@
+// Dense master-equation kernels for gfx950: complex64 GEMMs on FP32 MFMA
+// (v_mfma_f32_16x16x4_f32), FP64 state accumulation and FP64 reductions.
+//
+// One workgroup owns one instance for the whole solve: instances are
+// independent, so a batch of B solves is a grid of B workgroups with no
+// inter-workgroup synchronisation, and each instance keeps its own t, dt and
+// accept/reject decision (per-instance adaptive stepping).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "dqh.h"
+#include "dqh_common.h"
+
+namespace dqh {
+
+// ---------------------------------------------------------------- constants
+// Tsit5 tableau (Tsitouras 2011), solution weights = last stage row (FSAL).
+__constant__ double kA[7][7] = {
+    {0, 0, 0, 0, 0, 0, 0},
+    {0.161, 0, 0, 0, 0, 0, 0},
+    {-0.008480655492356989, 0.335480655492357, 0, 0, 0, 0, 0},
+    {2.897153057105493, -6.359448489975075, 4.3622954328695815, 0, 0, 0, 0},
+    {5.325864828439257, -11.748883564062828, 7.4955393428898365, -0.09249506636175525, 0, 0, 0},
+    {5.86145544294642, -12.92096931784711, 8.159367898576159, -0.071584973281401, -0.028269050394068383, 0, 0},
+    {0.09646076681806523, 0.01, 0.4798896504144996, 1.379008574103742, -3.290069515436081, 2.324710524099774, 0},
+};
+__constant__ double kE[7] = {-0.00178001105222577714, -0.0008164344596567469, 0.007880878010261995,
+                             -0.1447110071732629,     0.5823571654525552,     -0.45808210592918697,
+                             0.015151515151515152};
+// continuous extension: b_i(theta) = sum_p kR[i][p] theta^(p+1)
+__constant__ double kR[7][4] = {
+    {1.0, -2.763706197274826, 2.9132554618219126, -1.0530884977290216},
+    {0.0, 0.13169999999999998, -0.2234, 0.1017},
+    {0.0, 3.9302962368947516, -5.941033872131505, 2.490627285651253},
+    {0.0, -12.411077166933676, 30.33818863028232, -16.548102889244902},
+    {0.0, 37.50931341651104, -88.1789048947664, 47.37952196281928},
+    {0.0, -27.896526289197286, 65.09189467479366, -34.87065786149661},
+    {0.0, 1.5, -4.0, 2.5},
+};
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+struct Acc {
+  f4 re, im;
+};
+
+__device__ __forceinline__ void acc_zero(Acc &a) {
+  a.re = f4{0.f, 0.f, 0.f, 0.f};
+  a.im = f4{0.f, 0.f, 0.f, 0.f};
+}
+
+// One complex k-slice: D += A * B with A = ar + i ai (lane's A element), B = b.
+__device__ __forceinline__ void cmfma(Acc &a, float ar, float ai, float2 b) {
+  a.re = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, b.x, a.re, 0, 0, 0);
+  a.re = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai, b.y, a.re, 0, 0, 0);
+  a.im = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, b.y, a.im, 0, 0, 0);
+  a.im = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, b.x, a.im, 0, 0, 0);
+}
+
+// acc += A[row0:row0+16, 0:K] @ B[0:K, col0:col0+16] for row-major complex64
+// matrices (K multiple of 16). Lane (r = lane%16, g = lane/16) loads
+// A[row0+r][k0+4g .. k0+4g+3] (32 contiguous bytes) and B[k0+4g+i][col0+r]; MFMA
+// slice i then pairs A column k0+4g+i with B row k0+4g+i, which covers the 16
+// k's of the block in the order {4g+i}.
+__device__ __forceinline__ void tile_mac(Acc &acc, const float2 *__restrict__ A, int lda,
+                                         const float2 *__restrict__ B, int ldb, int row0, int col0,
+                                         int K, int lane) {
+  const int r = lane & 15, g = lane >> 4;
+  const float2 *Ap = A + (size_t)(row0 + r) * lda + 4 * g;
+  const float2 *Bp = B + (size_t)(4 * g) * ldb + col0 + r;
+#pragma unroll 2
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    const float4 a01 = *reinterpret_cast<const float4 *>(Ap + k0);
+    const float4 a23 = *reinterpret_cast<const float4 *>(Ap + k0 + 2);
+    const float2 *bp = Bp + (size_t)k0 * ldb;
+    const float2 b0 = bp[0];
+    const float2 b1 = bp[ldb];
+    const float2 b2 = bp[2 * ldb];
+    const float2 b3 = bp[3 * ldb];
+    cmfma(acc, a01.x, a01.y, b0);
+    cmfma(acc, a01.z, a01.w, b1);
+    cmfma(acc, a23.x, a23.y, b2);
+    cmfma(acc, a23.z, a23.w, b3);
+  }
+}
+
+// Store a 16x16 accumulator tile: lane (r, g) holds D[4g+v][r], v = 0..3.
+__device__ __forceinline__ void tile_store(const Acc &acc, float2 *__restrict__ C, int ldc, int row0,
+                                           int col0, int lane) {
+  const int r = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) C[(size_t)(row0 + 4 * g + v) * ldc + col0 + r] = make_float2(acc.re[v], acc.im[v]);
+}
+
+// ----------------------------------------------------------- reductions
+template <int NW>
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) s += sh[w];
+  return s;
+}
+
+// ----------------------------------------------------------- instance view
+struct View {
+  int n, np, nL, nE, ntsave, npwc, nseg, flags;
+  size_t nn;
+  const float2 *Hnh;  // current Hnh (constant or PWC-updated copy)
+  const float2 *D, *C;
+  float2 *kbase;  // 7 stage buffers; logical stages 0 and 6 swap slots after an accepted step
+  int s0, s6;
+  float2 *ys, *X, *tmp, *Hcur;
+  double2 *y[2];
+};
+
+__device__ __forceinline__ void make_view(View &v, const dqh_me_inst &p) {
+  v.n = (int)p.n;
+  v.np = (int)p.np;
+  v.nL = (int)p.nL;
+  v.nE = (int)p.nE;
+  v.ntsave = (int)p.ntsave;
+  v.npwc = (int)p.npwc;
+  v.nseg = (int)p.nseg;
+  v.flags = (int)p.flags;
+  v.nn = (size_t)v.np * v.np;
+  float2 *w = reinterpret_cast<float2 *>(p.work);
+  v.kbase = w;
+  v.s0 = 0;
+  v.s6 = 6;
+  v.ys = w + 7 * v.nn;
+  v.tmp = w + 8 * v.nn;
+  v.X = w + 9 * v.nn;
+  v.Hcur = v.npwc > 0 ? w + (9 + v.nL) * v.nn : nullptr;
+  v.Hnh = v.npwc > 0 ? v.Hcur : reinterpret_cast<const float2 *>(p.Hnh);
+  v.D = reinterpret_cast<const float2 *>(p.Dops);
+  v.C = reinterpret_cast<const float2 *>(p.Cops);
+  double2 *y = reinterpret_cast<double2 *>(p.y);
+  v.y[0] = y;
+  v.y[1] = y + v.nn;
+}
+
+// out <- tmp + tmp^dag with tmp = [Hnh ys] + sum_k (D_k ys) C_k. ys is read from v.ys.
+// Uses v.X (nL * nn) and v.tmp as scratch. Ends with a barrier.
+template <int NW>
+__device__ void wg_rhs(const View &v, const float2 *D, const float2 *C, bool hasH, float2 *out) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int np = v.np, NT = np >> 4;
+  const int nt2 = NT * NT;
+  // phase 1: X_k = D_k ys
+  const int nt1 = v.nL * nt2;
+  for (int t = wave; t < nt1; t += NW) {
+    const int kk = t / nt2, ti = (t / NT) % NT, tj = t % NT;
+    Acc acc;
+    acc_zero(acc);
+    tile_mac(acc, D + kk * v.nn, np, v.ys, np, ti * 16, tj * 16, np, lane);
+    tile_store(acc, v.X + kk * v.nn, np, ti * 16, tj * 16, lane);
+  }
+  __syncthreads();
+  // phase 2: tmp = Hnh ys + sum_k X_k C_k
+  for (int t = wave; t < nt2; t += NW) {
+    const int ti = t / NT, tj = t % NT;
+    Acc acc;
+    acc_zero(acc);
+    if (hasH) tile_mac(acc, v.Hnh, np, v.ys, np, ti * 16, tj * 16, np, lane);
+    for (int kk = 0; kk < v.nL; ++kk)
+      tile_mac(acc, v.X + kk * v.nn, np, C + kk * v.nn, np, ti * 16, tj * 16, np, lane);
+    tile_store(acc, v.tmp, np, ti * 16, tj * 16, lane);
+  }
+  __syncthreads();
+  // phase 3: Hermitian symmetrisation
+  for (size_t e = threadIdx.x; e < v.nn; e += NW * 64) {
+    const int i = (int)(e / np), j = (int)(e % np);
+    const float2 a = v.tmp[e], b = v.tmp[(size_t)j * np + i];
+    out[e] = make_float2(a.x + b.x, a.y - b.y);
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ float2 *kp(const View &v, int j) {
+  const int slot = j == 0 ? v.s0 : (j == 6 ? v.s6 : j);
+  return v.kbase + (size_t)slot * v.nn;
+}
+
+__device__ __forceinline__ float2 to_f2(double2 a) { return make_float2((float)a.x, (float)a.y); }
+
+// ys <- (float) y
+template <int NW>
+__device__ void wg_round_state(const View &v, const double2 *y) {
+  for (size_t e = threadIdx.x; e < v.nn; e += NW * 64) v.ys[e] = to_f2(y[e]);
+  __syncthreads();
+}
+
+// Hcur <- Hnh0 + sum_j c_j Hpwc_j for the segment containing t (zero outside)
+template <int NW>
+__device__ void wg_update_pwc(const View &v, const dqh_me_inst &p, int seg) {
+  const float2 *H0 = reinterpret_cast<const float2 *>(p.Hnh);
+  const float2 *Hp = reinterpret_cast<const float2 *>(p.Hpwc);
+  const double *vals = reinterpret_cast<const double *>(p.pwc_vals);
+  const int nv = v.nseg - 1;
+  for (size_t e = threadIdx.x; e < v.nn; e += NW * 64) {
+    double re = H0[e].x, im = H0[e].y;
+    if (seg >= 0 && seg < nv) {
+      for (int j = 0; j < v.npwc; ++j) {
+        const double cr = vals[2 * (j * nv + seg)], ci = vals[2 * (j * nv + seg) + 1];
+        const float2 o = Hp[j * v.nn + e];
+        re += cr * o.x - ci * o.y;
+        im += cr * o.y + ci * o.x;
+      }
+    }
+    v.Hcur[e] = make_float2((float)re, (float)im);
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ int pwc_segment(const double *times, int nseg, double t) {
+  // searchsorted(times, t, 'right') - 1; -1 before the first breakpoint
+  int lo = 0, hi = nseg;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (times[mid] <= t)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo - 1;
+}
+
+// Save the state interpolated as y + h * sum_j w_j k_j (or y itself when h == 0)
+// into states[isave] (unpadded) and its expectation values.
+template <int NW>
+__device__ void wg_save(const View &v, const dqh_me_inst &p, const double2 *y, double h, const double *w,
+                        int nk, int isave, double *sh) {
+  float2 *states = reinterpret_cast<float2 *>(p.states);
+  const bool save_states = (v.flags & DQH_FLAG_SAVE_STATES) != 0;
+  const float2 *ET = reinterpret_cast<const float2 *>(p.EopsT);
+  double er[4] = {0, 0, 0, 0}, ei[4] = {0, 0, 0, 0};
+  for (size_t e = threadIdx.x; e < v.nn; e += NW * 64) {
+    double re = y[e].x, im = y[e].y;
+    for (int j = 0; j < nk; ++j) {
+      const float2 kj = kp(v, j)[e];
+      re += h * w[j] * kj.x;
+      im += h * w[j] * kj.y;
+    }
+    const int i = (int)(e / v.np), jj = (int)(e % v.np);
+    if (i < v.n && jj < v.n) {
+      if (save_states) states[((size_t)isave * v.n + i) * v.n + jj] = make_float2((float)re, (float)im);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (q < v.nE) {
+          const float2 o = ET[q * v.nn + e];
+          er[q] += o.x * re - o.y * im;
+          ei[q] += o.x * im + o.y * re;
+        }
+      }
+    }
+  }
+  float2 *ex = reinterpret_cast<float2 *>(p.expects);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (q >= v.nE) continue;
+    const double sr = block_sum<NW>(er[q], sh);
+    const double si = block_sum<NW>(ei[q], sh);
+    if (threadIdx.x == 0) ex[(size_t)q * v.ntsave + isave] = make_float2((float)sr, (float)si);
+  }
+  // more than 4 expectation operators: one extra pass per operator
+  for (int q = 4; q < v.nE; ++q) {
+    double a = 0, b = 0;
+    for (size_t e = threadIdx.x; e < v.nn; e += NW * 64) {
+      const int i = (int)(e / v.np), jj = (int)(e % v.np);
+      if (i >= v.n || jj >= v.n) continue;
+      double re = y[e].x, im = y[e].y;
+      for (int j = 0; j < nk; ++j) {
+        const float2 kj = kp(v, j)[e];
+        re += h * w[j] * kj.x;
+        im += h * w[j] * kj.y;
+      }
+      const float2 o = ET[q * v.nn + e];
+      a += o.x * re - o.y * im;
+      b += o.x * im + o.y * re;
+    }
+    const double sr = block_sum<NW>(a, sh), si = block_sum<NW>(b, sh);
+    if (threadIdx.x == 0) ex[(size_t)q * v.ntsave + isave] = make_float2((float)sr, (float)si);
+  }
+  __syncthreads();
+}
+
+template <int NW>
+__device__ void wg_write_final(const View &v, const dqh_me_inst &p, const double2 *y) {
+  float2 *fin = reinterpret_cast<float2 *>(p.final_state);
+  for (size_t e = threadIdx.x; e < v.nn; e += NW * 64) {
+    const int i = (int)(e / v.np), j = (int)(e % v.np);
+    if (i < v.n && j < v.n) fin[(size_t)i * v.n + j] = to_f2(y[e]);
+  }
+}
+
+// ------------------------------------------------------------ unit RHS kernel
+template <int NW>
+__global__ __launch_bounds__(NW * 64) void k_rhs(const dqh_me_inst *__restrict__ insts) {
+  const dqh_me_inst &p = insts[blockIdx.x];
+  View v;
+  make_view(v, p);
+  if (v.npwc > 0) {
+    const int seg = pwc_segment(reinterpret_cast<const double *>(p.pwc_times), v.nseg, p.t0);
+    wg_update_pwc<NW>(v, p, seg);
+  }
+  wg_round_state<NW>(v, v.y[0]);
+  wg_rhs<NW>(v, v.D, v.C, (v.flags & DQH_FLAG_HAS_HNH) != 0, kp(v, 0));
+  float2 *fin = reinterpret_cast<float2 *>(p.final_state);
+  for (size_t e = threadIdx.x; e < v.nn; e += NW * 64) {
+    const int i = (int)(e / v.np), j = (int)(e % v.np);
+    if (i < v.n && j < v.n) fin[(size_t)i * v.n + j] = kp(v, 0)[e];
+  }
+}
+
+// ----------------------------------------------------------- Tsit5 kernel
+template <int NW>
+__global__ __launch_bounds__(NW * 64) void k_tsit5(const dqh_me_inst *__restrict__ insts) {
+  __shared__ double sh[NW];
+  const dqh_me_inst &p = insts[blockIdx.x];
+  View v;
+  make_view(v, p);
+  const double *tsave = reinterpret_cast<const double *>(p.tsave);
+  const double *ptimes = reinterpret_cast<const double *>(p.pwc_times);
+  int64_t *stats = reinterpret_cast<int64_t *>(p.stats);
+  const double rtol = p.rtol, atol = p.atol;
+  const double inv_nn = 1.0 / ((double)v.n * v.n);
+  const double t_end = p.t1;
+  const bool hasH = (v.flags & DQH_FLAG_HAS_HNH) != 0;
+
+  double t = p.t0;
+  int cur = 0;
+  int isave = 0;
+  int64_t nsteps = 0, nacc = 0, nrej = 0, nrhs = 0;
+  int status = 0;
+  int seg = -2;
+  const double w1[1] = {0.0};
+
+  // tsave points at t0 store the initial state
+  while (isave < v.ntsave && tsave[isave] <= t) {
+    wg_save<NW>(v, p, v.y[cur], 0.0, w1, 0, isave, sh);
+    ++isave;
+  }
+  if (v.npwc > 0) {
+    seg = pwc_segment(ptimes, v.nseg, t);
+    wg_update_pwc<NW>(v, p, seg);
+  }
+  // k0 = f(y0)
+  wg_round_state<NW>(v, v.y[cur]);
+  wg_rhs<NW>(v, v.D, v.C, hasH, kp(v, 0));
+  ++nrhs;
+
+  // initial step (Hairer-Norsett-Wanner II.4)
+  double dt;
+  {
+    double s0 = 0, s1 = 0;
+    for (size_t e = threadIdx.x; e < v.nn; e += NW * 64) {
+      const double2 y0 = v.y[cur][e];
+      const double ay = sqrt(y0.x * y0.x + y0.y * y0.y);
+      const double sc = atol + rtol * ay;
+      const float2 f0 = kp(v, 0)[e];
+      s0 += (y0.x * y0.x + y0.y * y0.y) / (sc * sc);
+      s1 += ((double)f0.x * f0.x + (double)f0.y * f0.y) / (sc * sc);
+    }
+    const double d0 = sqrt(block_sum<NW>(s0, sh) * inv_nn);
+    const double d1 = sqrt(block_sum<NW>(s1, sh) * inv_nn);
+    const double h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
+    for (size_t e = threadIdx.x; e < v.nn; e += NW * 64) {
+      const double2 y0 = v.y[cur][e];
+      const float2 f0 = kp(v, 0)[e];
+      v.ys[e] = make_float2((float)(y0.x + h0 * f0.x), (float)(y0.y + h0 * f0.y));
+    }
+    __syncthreads();
+    wg_rhs<NW>(v, v.D, v.C, hasH, kp(v, 1));
+    ++nrhs;
+    double s2 = 0;
+    for (size_t e = threadIdx.x; e < v.nn; e += NW * 64) {
+      const double2 y0 = v.y[cur][e];
+      const double sc = atol + rtol * sqrt(y0.x * y0.x + y0.y * y0.y);
+      const double dr = (double)kp(v, 1)[e].x - kp(v, 0)[e].x, di = (double)kp(v, 1)[e].y - kp(v, 0)[e].y;
+      s2 += (dr * dr + di * di) / (sc * sc);
+    }
+    const double d2 = sqrt(block_sum<NW>(s2, sh) * inv_nn) / h0;
+    const double md = fmax(d1, d2);
+    const double h1 = md <= 1e-15 ? fmax(1e-6, h0 * 1e-3) : pow(0.01 / md, 0.2);
+    dt = fmin(100.0 * h0, h1);
+  }
+
+  bool need_k0 = false;
+  while (t < t_end) {
+    if (nsteps >= p.max_steps) {
+      status = DQH_MAX_STEPS;
+      break;
+    }
+    if (v.npwc > 0) {
+      const int s = pwc_segment(ptimes, v.nseg, t);
+      if (s != seg) {
+        seg = s;
+        wg_update_pwc<NW>(v, p, seg);
+      }
+    }
+    if (need_k0) {
+      wg_round_state<NW>(v, v.y[cur]);
+      wg_rhs<NW>(v, v.D, v.C, hasH, kp(v, 0));
+      ++nrhs;
+      need_k0 = false;
+    }
+    double h = fmin(dt, t_end - t);
+    bool made_jump = false;
+    double t_new = (h >= t_end - t) ? t_end : t + h;
+    if (v.npwc > 0) {
+      // next breakpoint strictly after t
+      const int m = pwc_segment(ptimes, v.nseg, t) + 1;
+      if (m < v.nseg && ptimes[m] < t_end && t + h >= ptimes[m]) {
+        h = ptimes[m] - t;
+        t_new = ptimes[m];
+        made_jump = true;
+      }
+    }
+    if (!(h > 0.0) || !(h > 1e-14 * fmax(1.0, fabs(t)))) {
+      status = DQH_NONFINITE;
+      break;
+    }
+    const double2 *y = v.y[cur];
+    double2 *ynew = v.y[1 - cur];
+    // stages 1..6 (stage 6 input is the candidate solution)
+    for (int s = 1; s < 7; ++s) {
+      for (size_t e = threadIdx.x; e < v.nn; e += NW * 64) {
+        double re = y[e].x, im = y[e].y;
+        for (int j = 0; j < s; ++j) {
+          const double c = h * kA[s][j];
+          const float2 kj = kp(v, j)[e];
+          re += c * kj.x;
+          im += c * kj.y;
+        }
+        if (s == 6) ynew[e] = make_double2(re, im);
+        v.ys[e] = make_float2((float)re, (float)im);
+      }
+      __syncthreads();
+      wg_rhs<NW>(v, v.D, v.C, hasH, kp(v, s));
+      ++nrhs;
+    }
+    // error norm (FP64 accumulation)
+    double s = 0;
+    for (size_t e = threadIdx.x; e < v.nn; e += NW * 64) {
+      double er = 0, ei = 0;
+#pragma unroll
+      for (int j = 0; j < 7; ++j) {
+        const float2 kj = kp(v, j)[e];
+        er += kE[j] * kj.x;
+        ei += kE[j] * kj.y;
+      }
+      er *= h;
+      ei *= h;
+      const double2 a = y[e], b = ynew[e];
+      const double sc = atol + rtol * fmax(sqrt(a.x * a.x + a.y * a.y), sqrt(b.x * b.x + b.y * b.y));
+      s += (er * er + ei * ei) / (sc * sc);
+    }
+    const double norm = sqrt(block_sum<NW>(s, sh) * inv_nn);
+    const bool finite = isfinite(norm);
+    const bool keep = finite && norm < 1.0;
+    double factor;
+    if (!finite)
+      factor = p.factormin;
+    else if (norm == 0.0)
+      factor = p.factormax;
+    else
+      factor = p.safety * pow(norm, -0.2);
+    factor = fmin(fmax(factor, keep ? 1.0 : p.factormin), p.factormax);
+    ++nsteps;
+    if (keep) {
+      ++nacc;
+      while (isave < v.ntsave && tsave[isave] <= t_new) {
+        const double th = (tsave[isave] - t) / h;
+        double w[7];
+        for (int j = 0; j < 7; ++j) w[j] = th * (kR[j][0] + th * (kR[j][1] + th * (kR[j][2] + th * kR[j][3])));
+        wg_save<NW>(v, p, y, h, w, 7, isave, sh);
+        ++isave;
+      }
+      t = t_new;
+      cur = 1 - cur;
+      const int ts = v.s0;
+      v.s0 = v.s6;
+      v.s6 = ts;
+      need_k0 = made_jump;
+    } else {
+      ++nrej;
+    }
+    dt = factor * h;
+  }
+  // points not reached (only after a failure): repeat the last state
+  {
+    const double w0[1] = {0.0};
+    while (isave < v.ntsave) {
+      wg_save<NW>(v, p, v.y[cur], 0.0, w0, 0, isave, sh);
+      ++isave;
+    }
+  }
+  wg_write_final<NW>(v, p, v.y[cur]);
+  if (!(v.flags & DQH_FLAG_SAVE_STATES)) {
+    float2 *st = reinterpret_cast<float2 *>(p.states);
+    for (size_t e = threadIdx.x; e < v.nn; e += NW * 64) {
+      const int i = (int)(e / v.np), j = (int)(e % v.np);
+      if (i < v.n && j < v.n) st[(size_t)i * v.n + j] = to_f2(v.y[cur][e]);
+    }
+  }
+  if (threadIdx.x == 0) {
+    stats[0] = nsteps;
+    stats[1] = nacc;
+    stats[2] = nrej;
+    stats[3] = status;
+    stats[4] = nrhs;
+  }
+}
+
+// -------------------------------------------------------- Rouchon1 kernel
+template <int NW>
+__global__ __launch_bounds__(NW * 64) void k_rouchon1(const dqh_me_inst *__restrict__ insts) {
+  __shared__ double sh[NW];
+  const dqh_me_inst &p = insts[blockIdx.x];
+  View v;
+  make_view(v, p);
+  const double *tsave = reinterpret_cast<const double *>(p.tsave);
+  const double *step_t = reinterpret_cast<const double *>(p.step_t);
+  const int *step_set = reinterpret_cast<const int *>(p.step_set);
+  int64_t *stats = reinterpret_cast<int64_t *>(p.stats);
+  const int64_t nsteps = p.max_steps;
+  const double w1[1] = {1.0};
+  double2 *y = v.y[0];
+  int isave = 0;
+  while (isave < v.ntsave && tsave[isave] <= step_t[0]) {
+    wg_save<NW>(v, p, y, 0.0, w1, 0, isave, sh);
+    ++isave;
+  }
+  for (int64_t m = 0; m < nsteps; ++m) {
+    const double t = step_t[m], tn = step_t[m + 1];
+    const size_t so = (size_t)step_set[m] * v.nL * v.nn;
+    wg_round_state<NW>(v, y);
+    wg_rhs<NW>(v, v.D + so, v.C + so, false, kp(v, 0));
+    // linear interpolation y + theta * increment at save points inside the step
+    const double tol = 1e-12 * fmax(1.0, fabs(tn));
+    while (isave < v.ntsave && tsave[isave] <= tn + tol) {
+      const double th = fmin(fmax((tsave[isave] - t) / (tn - t), 0.0), 1.0);
+      wg_save<NW>(v, p, y, th, w1, 1, isave, sh);
+      ++isave;
+    }
+    for (size_t e = threadIdx.x; e < v.nn; e += NW * 64) {
+      const float2 d = kp(v, 0)[e];
+      y[e].x += d.x;
+      y[e].y += d.y;
+    }
+    __syncthreads();
+  }
+  while (isave < v.ntsave) {
+    wg_save<NW>(v, p, y, 0.0, w1, 0, isave, sh);
+    ++isave;
+  }
+  wg_write_final<NW>(v, p, y);
+  if (!(v.flags & DQH_FLAG_SAVE_STATES)) {
+    float2 *st = reinterpret_cast<float2 *>(p.states);
+    for (size_t e = threadIdx.x; e < v.nn; e += NW * 64) {
+      const int i = (int)(e / v.np), j = (int)(e % v.np);
+      if (i < v.n && j < v.n) st[(size_t)i * v.n + j] = to_f2(y[e]);
+    }
+  }
+  if (threadIdx.x == 0) {
+    stats[0] = nsteps;
+    stats[1] = nsteps;
+    stats[2] = 0;
+    stats[3] = 0;
+    stats[4] = nsteps;
+  }
+}
+
+}  // namespace dqh
+
+using namespace dqh;
+
+static constexpr int kNW = 4;
+
+extern "C" int64_t dqh_sizeof_me_inst(void) { return (int64_t)sizeof(dqh_me_inst); }
+
+extern "C" int64_t dqh_me_work_elems(int64_t np, int64_t nL, int64_t npwc) {
+  return (9 + nL + (npwc > 0 ? 1 : 0)) * np * np;
+}
+
+static int launch_checked(const char *what, hipError_t err) {
+  if (err != hipSuccess) {
+    set_error(std::string(what) + ": " + hipGetErrorString(err));
+    return DQH_HIP_ERROR;
+  }
+  return DQH_OK;
+}
+
+extern "C" int dqh_lindblad_rhs_dense(const dqh_me_inst *d_insts, int64_t ninst, void *stream) {
+  if (!d_insts || ninst <= 0 || ninst > (1 << 30)) return fail(DQH_BAD_ARG, "dqh_lindblad_rhs_dense: bad arguments");
+  hipLaunchKernelGGL(k_rhs<kNW>, dim3((unsigned)ninst), dim3(kNW * 64), 0, (hipStream_t)stream, d_insts);
+  return launch_checked("dqh_lindblad_rhs_dense", hipGetLastError());
+}
+
+extern "C" int dqh_mesolve_tsit5_dense(const dqh_me_inst *d_insts, int64_t ninst, void *stream) {
+  if (!d_insts || ninst <= 0 || ninst > (1 << 30)) return fail(DQH_BAD_ARG, "dqh_mesolve_tsit5_dense: bad arguments");
+  hipLaunchKernelGGL(k_tsit5<kNW>, dim3((unsigned)ninst), dim3(kNW * 64), 0, (hipStream_t)stream, d_insts);
+  return launch_checked("dqh_mesolve_tsit5_dense", hipGetLastError());
+}
+
+extern "C" int dqh_mesolve_rouchon1_dense(const dqh_me_inst *d_insts, int64_t ninst, void *stream) {
+  if (!d_insts || ninst <= 0 || ninst > (1 << 30)) return fail(DQH_BAD_ARG, "dqh_mesolve_rouchon1_dense: bad arguments");
+  hipLaunchKernelGGL(k_rouchon1<kNW>, dim3((unsigned)ninst), dim3(kNW * 64), 0, (hipStream_t)stream, d_insts);
+  return launch_checked("dqh_mesolve_rouchon1_dense", hipGetLastError());
+}

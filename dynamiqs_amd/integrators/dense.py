@@ -1,0 +1,354 @@
+"""Packing of dense master-equation instances into ``dqh_me_inst`` descriptors and
+launch of the HIP kernels (``dqh_mesolve_tsit5_dense``,
+``dqh_mesolve_rouchon1_dense``, ``dqh_lindblad_rhs_dense``).
+
+Every instance of a batch gets its own descriptor; operators shared between
+instances (e.g. un-batched jump operators) are stored once on the device and
+referenced by every descriptor. Matrices are zero-padded to ``np = 16 *
+ceil(n / 16)`` so that MFMA tiles never need bounds checks.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+import scipy.linalg as sla
+
+from .. import _native
+from .operators import resample_pwc
+
+MAX_NP = 256
+
+
+def pad16(n: int) -> int:
+    return 16 * ((n + 15) // 16)
+
+
+def _pad(x: np.ndarray, np_: int) -> np.ndarray:
+    n = x.shape[-1]
+    if n == np_:
+        return x
+    out = np.zeros((*x.shape[:-2], np_, np_), dtype=x.dtype)
+    out[..., :n, :n] = x
+    return out
+
+
+def _dag(x: np.ndarray) -> np.ndarray:
+    return np.swapaxes(x.conj(), -1, -2)
+
+
+@dataclass
+class Instance:
+    """One solve: constant part of H, PWC terms of H, jump operators, initial state."""
+
+    H: np.ndarray  # (n, n) complex128
+    Ls: list  # [(n, n) complex128]
+    rho0: np.ndarray  # (n, n) complex128
+    pwc: list = field(default_factory=list)  # [(times, values (nv,), op (n, n))]
+    key_H: object = None  # sharing keys: equal keys share device copies
+    key_L: object = None
+    key_rho: object = None
+
+
+class _Arena:
+    """Host staging of device buffers; one torch tensor per buffer."""
+
+    def __init__(self, device):
+        self.device = device
+        self.tensors = []
+
+    def put(self, arr: np.ndarray) -> int:
+        import torch
+
+        a = np.ascontiguousarray(arr)
+        if a.dtype == np.complex64:
+            t = torch.from_numpy(a.view(np.float32)).to(self.device)
+        elif a.dtype == np.complex128:
+            t = torch.from_numpy(a.view(np.float64)).to(self.device)
+        else:
+            t = torch.from_numpy(a).to(self.device)
+        self.tensors.append(t)
+        return t.data_ptr()
+
+    def empty(self, nbytes: int, zero: bool = False):
+        import torch
+
+        t = (torch.zeros if zero else torch.empty)(max(int(nbytes), 16), dtype=torch.uint8, device=self.device)
+        self.tensors.append(t)
+        return t
+
+
+def _device():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise RuntimeError('dynamiqs_amd needs an MI355X GPU (torch.cuda.is_available() is False)')
+    return torch.device('cuda', torch.cuda.current_device())
+
+
+class DensePlan:
+    """A packed batch of instances of one kind, ready to launch (inputs resident in HBM)."""
+
+    def __init__(self, kind: str, n: int, ntsave: int, nE: int, ninst: int, save_states: bool):
+        self.kind = kind
+        self.n = n
+        self.np = pad16(n)
+        self.ntsave = ntsave
+        self.nE = nE
+        self.ninst = ninst
+        self.save_states = save_states
+        self.arena = None
+        self.descs = None
+        self.d_descs = None
+        self.states = None
+        self.expects = None
+        self.final = None
+        self.stats = None
+        self.y = None
+        self.y0 = None
+        self.nrhs_flops = 0.0
+
+    # ------------------------------------------------------------------ launch
+    def reset(self):
+        """Restore the initial states so that the plan can be run again."""
+        self.y[:, 0].copy_(self.y0)
+
+    def run(self, stream=None):
+        import torch
+
+        lib = _native.lib()
+        s = stream if stream is not None else torch.cuda.current_stream()
+        sp = _native.ctypes.c_void_p(s.cuda_stream)
+        fn = {
+            'tsit5': lib.dqh_mesolve_tsit5_dense,
+            'rouchon1': lib.dqh_mesolve_rouchon1_dense,
+            'rhs': lib.dqh_lindblad_rhs_dense,
+        }[self.kind]
+        _native.check(fn(_native.ctypes.c_void_p(self.d_descs.data_ptr()), self.ninst, sp), f'dqh {self.kind}')
+
+    def results(self) -> dict:
+        import torch
+
+        torch.cuda.synchronize()
+        n = self.n
+        out = {}
+        c64 = lambda t: t.cpu().numpy().view(np.complex64)  # noqa: E731
+        if self.states is not None:
+            shp = (self.ninst, self.ntsave, n, n) if self.save_states else (self.ninst, 1, n, n)
+            out['states'] = c64(self.states).reshape(shp)
+        out['final_state'] = c64(self.final).reshape(self.ninst, n, n)
+        if self.nE:
+            out['expects'] = c64(self.expects).reshape(self.ninst, self.nE, self.ntsave)
+        st = self.stats.cpu().numpy()
+        out['nsteps'], out['naccepted'], out['nrejected'] = st[:, 0], st[:, 1], st[:, 2]
+        out['status'], out['nrhs'] = st[:, 3], st[:, 4]
+        return out
+
+
+def _kraus_sets(H: np.ndarray, Ls, h: float, normalize: bool):
+    """Rouchon1 operators with the Cholesky normalisation folded in
+    (rouchon_integrator.py:250-263, 552-582): A_0 = M0 T^-dag, A_k = sqrt(h) L_k T^-dag.
+    One step adds tmp + tmp^dag with tmp = dA rho (I + dA^dag / 2) + sum_k A_k rho A_k^dag / 2,
+    dA = A_0 - I, i.e. D = [dA, A_k], C = [I + dA^dag / 2, A_k^dag / 2]."""
+    n = H.shape[-1]
+    LdL = sum((_dag(L) @ L for L in Ls), np.zeros_like(H))
+    Hnh = H - 0.5j * LdL
+    M0 = np.eye(n) - 1j * h * Hnh
+    Ms = [np.sqrt(h) * L for L in Ls]
+    if normalize:
+        S = _dag(M0) @ M0 + h * LdL
+        T = np.linalg.cholesky(S)
+        Tinv_dag = _dag(sla.solve_triangular(T, np.eye(n), lower=True))
+        M0 = M0 @ Tinv_dag
+        Ms = [M @ Tinv_dag for M in Ms]
+    dA = M0 - np.eye(n)
+    D = [dA, *Ms]
+    C = [np.eye(n) + 0.5 * _dag(dA), *[0.5 * _dag(M) for M in Ms]]
+    return D, C
+
+
+def rouchon_step_times(t0: float, t1: float, dt: float) -> np.ndarray:
+    """Boundaries ``t0 + k dt`` with the last step clipped to ``t1`` (diffrax
+    ``ConstantStepSize`` with ``dt0 = method.dt``)."""
+    nf = (t1 - t0) / dt
+    nr = int(round(nf))
+    if abs(nr - nf) <= 1e-9 * max(1.0, nf):
+        nsteps = max(nr, 1)
+    else:
+        nsteps = int(np.ceil(nf))
+    ts = t0 + dt * np.arange(nsteps + 1, dtype=np.float64)
+    ts[-1] = t1
+    return ts
+
+
+def build_plan(
+    kind: str,
+    instances: list,
+    tsave: np.ndarray,
+    *,
+    exp_ops=None,
+    t0: float | None = None,
+    rtol: float = 1e-6,
+    atol: float = 1e-6,
+    safety: float = 0.9,
+    factormin: float = 0.2,
+    factormax: float = 5.0,
+    max_steps: int = 100_000,
+    dt: float | None = None,
+    normalize: bool = True,
+    save_states: bool = True,
+) -> DensePlan:
+    """Pack ``instances`` (same Hilbert dimension) for ``kind`` in {'tsit5', 'rouchon1', 'rhs'}."""
+    import torch
+
+    dev = _device()
+    _native.lib()
+    n = instances[0].H.shape[-1]
+    if any(inst.H.shape[-1] != n for inst in instances):
+        raise ValueError('all instances of a dense plan must share the Hilbert dimension')
+    npad = pad16(n)
+    if npad > MAX_NP:
+        raise ValueError(f'dense MI355X path supports n <= {MAX_NP} (got n={n}); use DIA operators')
+    tsave = np.asarray(tsave, dtype=np.float64)
+    B = len(instances)
+    Es = [np.asarray(E, dtype=np.complex128) for E in (exp_ops or [])]
+    nE = len(Es)
+    t_start = float(tsave[0] if t0 is None else t0)
+    t_end = float(tsave[-1])
+    plan = DensePlan(kind, n, len(tsave), nE, B, save_states)
+    arena = _Arena(dev)
+    plan.arena = arena
+    nn = npad * npad
+
+    K = len(instances[0].Ls)
+    if any(len(inst.Ls) != K for inst in instances):
+        raise ValueError('all instances of a dense plan must have the same number of jump operators')
+    descs = np.zeros(B, dtype=_native.ME_INST_DTYPE)
+
+    p_tsave = arena.put(tsave)
+    p_E = arena.put(_pad(np.stack([E.T for E in Es]), npad).astype(np.complex64)) if nE else 0
+
+    # ---- operators, shared by key
+    cache = {}
+
+    def shared(key, build):
+        # instances share a device copy only when every component of the key is known
+        if key is None or any(k is None for k in key[1:]):
+            return arena.put(build())
+        if key not in cache:
+            cache[key] = arena.put(build())
+        return cache[key]
+
+    npwc_max = 0
+    if kind == 'rouchon1':
+        if dt is None:
+            raise ValueError('Rouchon1 needs a fixed step dt')
+        step_t = rouchon_step_times(t_start, t_end, float(dt))
+        nsteps = len(step_t) - 1
+        p_step_t = arena.put(step_t)
+        hs = np.diff(step_t)
+        mids = step_t[:-1] + 0.5 * hs
+        nLdev = K + 1
+    else:
+        nLdev = K
+    for b, inst in enumerate(instances):
+        d = descs[b]
+        d['n'], d['np'], d['nE'], d['ntsave'] = n, npad, nE, len(tsave)
+        d['rtol'], d['atol'], d['safety'] = rtol, atol, safety
+        d['factormin'], d['factormax'] = factormin, factormax
+        d['max_steps'] = max_steps
+        d['t0'], d['t1'] = t_start, t_end
+        d['tsave'] = p_tsave
+        d['EopsT'] = p_E
+        d['flags'] = _native.FLAG_SAVE_STATES if save_states else 0
+        if kind in ('tsit5', 'rhs'):
+            Ls = inst.Ls
+            d['nL'] = K
+            d['flags'] |= _native.FLAG_HAS_HNH
+
+            def bH(inst=inst):
+                LdL = sum((_dag(L) @ L for L in inst.Ls), np.zeros_like(inst.H))
+                return _pad(-1j * inst.H - 0.5 * LdL, npad).astype(np.complex64)
+
+            d['Hnh'] = shared(('Hnh', inst.key_H, inst.key_L), bH)
+            if K:
+                d['Dops'] = shared(('D', inst.key_L), lambda Ls=Ls: _pad(np.stack(Ls), npad).astype(np.complex64))
+                d['Cops'] = shared(
+                    ('C', inst.key_L), lambda Ls=Ls: _pad(0.5 * np.stack([_dag(L) for L in Ls]), npad).astype(np.complex64)
+                )
+            if inst.pwc:
+                grid = np.unique(np.concatenate([np.asarray(t, dtype=np.float64) for (t, _, _) in inst.pwc]))
+                vals = np.stack([resample_pwc(np.asarray(t), np.asarray(v)[None], grid)[0] for (t, v, _) in inst.pwc])
+                d['npwc'] = len(inst.pwc)
+                d['nseg'] = len(grid)
+                npwc_max = max(npwc_max, len(inst.pwc))
+                d['Hpwc'] = shared(
+                    ('Hpwc', inst.key_H), lambda inst=inst: _pad(np.stack([-1j * o for (_, _, o) in inst.pwc]), npad).astype(np.complex64)
+                )
+                d['pwc_times'] = shared(('ptimes', inst.key_H), lambda grid=grid: grid)
+                d['pwc_vals'] = arena.put(np.ascontiguousarray(vals).view(np.float64))
+        else:  # rouchon1: Kraus sets per (PWC segment, step length)
+            d['nL'] = nLdev
+            grid = None
+            if inst.pwc:
+                grid = np.unique(np.concatenate([np.asarray(t, dtype=np.float64) for (t, _, _) in inst.pwc]))
+            keys = []
+            for m in range(nsteps):
+                seg = -1 if grid is None else int(np.searchsorted(grid, mids[m], side='right')) - 1
+                keys.append((seg, float(hs[m])))
+            uniq = sorted(set(keys))
+            index = {k: i for i, k in enumerate(uniq)}
+            sets_D, sets_C = [], []
+            for seg, h in uniq:
+                tm = None
+                Hm = inst.H.copy()
+                if inst.pwc:
+                    tm = grid[seg] if seg >= 0 else -np.inf
+                    for (times, vals, op) in inst.pwc:
+                        if tm >= times[0] and tm < times[-1]:
+                            idx = int(np.searchsorted(times, tm, side='right')) - 1
+                            Hm = Hm + vals[idx] * op
+                Dk, Ck = _kraus_sets(Hm, inst.Ls, h, normalize)
+                sets_D.append(np.stack(Dk))
+                sets_C.append(np.stack(Ck))
+            key = ('kraus', inst.key_H, inst.key_L)
+            d['Dops'] = shared(key + ('D',), lambda: _pad(np.stack(sets_D), npad).astype(np.complex64))
+            d['Cops'] = shared(key + ('C',), lambda: _pad(np.stack(sets_C), npad).astype(np.complex64))
+            d['step_t'] = p_step_t
+            d['step_set'] = arena.put(np.asarray([index[k] for k in keys], dtype=np.int32))
+            d['max_steps'] = nsteps
+
+    # ---- state, scratch, outputs
+    welems = _native.work_elems(npad, nLdev, npwc_max)
+    work = arena.empty(B * welems * 8)
+    y = torch.zeros((B, 2, nn, 2), dtype=torch.float64, device=dev)
+    y0 = np.zeros((B, npad, npad), dtype=np.complex128)
+    rho_cache = {}
+    for b, inst in enumerate(instances):
+        key = inst.key_rho
+        y0[b, :n, :n] = inst.rho0
+        if key is not None:
+            rho_cache[key] = b
+    y0_t = torch.from_numpy(y0.reshape(B, nn).view(np.float64).reshape(B, nn, 2)).to(dev)
+    y[:, 0].copy_(y0_t)
+    plan.y, plan.y0 = y, y0_t
+    arena.tensors.append(y)
+    ns = len(tsave) if save_states else 1
+    states = torch.empty((B, ns * n * n * 2), dtype=torch.float32, device=dev)
+    final = torch.empty((B, n * n * 2), dtype=torch.float32, device=dev)
+    expects = torch.zeros((B, max(nE, 1) * len(tsave) * 2), dtype=torch.float32, device=dev)
+    stats = torch.zeros((B, 8), dtype=torch.int64, device=dev)
+    plan.states, plan.final, plan.expects, plan.stats = states, final, expects, stats
+    for b in range(B):
+        d = descs[b]
+        d['y'] = y.data_ptr() + b * 2 * nn * 16
+        d['work'] = work.data_ptr() + b * welems * 8
+        d['states'] = states.data_ptr() + b * ns * n * n * 8
+        d['final_state'] = final.data_ptr() + b * n * n * 8
+        d['expects'] = expects.data_ptr() + b * max(nE, 1) * len(tsave) * 8
+        d['stats'] = stats.data_ptr() + b * 64
+    plan.descs = descs
+    plan.d_descs = torch.from_numpy(descs.view(np.uint8).copy()).to(dev)
+    plan.K = K
+    return plan

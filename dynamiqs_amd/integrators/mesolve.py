@@ -1,0 +1,205 @@
+"""Drop-in ``mesolve`` running on the MI355X.
+
+Mirrors ``dynamiqs/integrators/apis/mesolve.py``: argument conversion (:280-285),
+checks (:288-291, :390-414), cartesian / flat batching (:307-336,
+``integrators/_utils.py:93-187``) and the method registry (:350-366). The
+integrator seam (``integrator_constructor(...).run()``, :372-384) is served by
+the HIP kernels through ``dynamiqs_amd.integrators.dense``: every batch element
+becomes one instance, solved by one persistent workgroup with its own adaptive
+step.
+"""
+
+from __future__ import annotations
+
+import warnings
+
+import numpy as np
+
+from .._checks import check_hermitian, check_qarray_is_dense, check_shape, check_times
+from ..gradient import Gradient
+from ..method import Method, Rouchon1, Tsit5, _DEAdaptiveStep
+from ..options import Options, check_options
+from ..qarrays import QArray, asqarray
+from ..result import AdaptiveStepInfos, FixedStepInfos, MESolveResult
+from ..time_qarray import TimeQArray, astimeqarray
+from . import dense
+from .operators import decompose
+
+SUPPORTED_METHODS = (Tsit5, Rouchon1)
+
+
+def assert_method_supported(method: Method, supported_methods) -> None:
+    """``integrators/_utils.py:84-90``."""
+    if not isinstance(method, tuple(supported_methods)):
+        supported_str = ', '.join(f'`{x.__name__}`' for x in supported_methods)
+        raise TypeError(
+            f'Method of type `{type(method).__name__}` is not supported (supported method types: {supported_str}).'
+        )
+
+
+def max_steps_error() -> RuntimeError:
+    """Message of ``integrators/_utils.py:66-72``."""
+    default_max_steps = _DEAdaptiveStep.max_steps
+    return RuntimeError(
+        'The maximum number of method steps has been reached (the default'
+        f' value is `max_steps={default_max_steps:_}`). Try increasing'
+        ' `max_steps` with the `method` argument, e.g.'
+        ' `method=dq.method.Tsit5(max_steps=1_000_000)`.'
+    )
+
+
+def _check_mesolve_args(H: TimeQArray, Ls, rho0: QArray, exp_ops) -> None:
+    check_shape(H, 'H', '(..., n, n)', subs={'...': '...H'})
+    for i, L in enumerate(Ls):
+        check_shape(L, f'jump_ops[{i}]', '(..., n, n)', subs={'...': f'...L{i}'})
+    if len(Ls) == 0 and rho0.isket():
+        warnings.warn(
+            'Argument `jump_ops` is an empty list and argument `rho0` is a ket,'
+            ' consider using `dq.sesolve()` to solve the Schrödinger equation.',
+            stacklevel=3,
+        )
+    check_shape(rho0, 'rho0', '(..., n, 1)', '(..., n, n)', subs={'...': '...rho0'})
+    check_qarray_is_dense(rho0, 'rho0')
+    if exp_ops is not None:
+        for i, E in enumerate(exp_ops):
+            check_shape(E, f'exp_ops[{i}]', '(n, n)')
+
+
+def _broadcast_index(idx: tuple, shape: tuple) -> tuple:
+    sub = idx[len(idx) - len(shape) :] if shape else ()
+    return tuple(i if d > 1 else 0 for i, d in zip(sub, shape))
+
+
+def enumerate_instances(H: TimeQArray, Ls, rho0: QArray, cartesian: bool):
+    """Return (out_shape, [Instance]) following the reference's batching rules."""
+    Ht = decompose(H)
+    Lts = [decompose(L) for L in Ls]
+    for i, Lt in enumerate(Lts):
+        if Lt.pwc:
+            raise NotImplementedError('time-dependent jump operators are not supported by the MI355X solver')
+    r = rho0.to_numpy().astype(np.complex128)
+    rb = r.shape[:-2]
+    shapes = [Ht.batch, *[Lt.batch for Lt in Lts], rb]
+    if cartesian:
+        out_shape = tuple(d for s in shapes for d in s)
+    else:
+        out_shape = tuple(np.broadcast_shapes(*shapes))
+    insts = []
+    for idx in np.ndindex(*out_shape) if out_shape else [()]:
+        if cartesian:
+            parts, o = [], 0
+            for s in shapes:
+                parts.append(tuple(idx[o : o + len(s)]))
+                o += len(s)
+        else:
+            parts = [_broadcast_index(idx, s) for s in shapes]
+        iH, iLs, ir = parts[0], parts[1:-1], parts[-1]
+        rho = r[ir]
+        if rho.shape[-1] == 1:
+            rho = rho @ rho.conj().T
+        insts.append(
+            dense.Instance(
+                H=Ht.const_at(iH),
+                Ls=[Lt.const_at(i) for Lt, i in zip(Lts, iLs)],
+                rho0=rho,
+                pwc=Ht.pwc_at(iH),
+                key_H=iH,
+                key_L=tuple(iLs),
+                key_rho=ir,
+            )
+        )
+    return out_shape, insts
+
+
+def mesolve(
+    H,
+    jump_ops,
+    rho0,
+    tsave,
+    *,
+    exp_ops=None,
+    method: Method = Tsit5(),  # noqa: B008
+    gradient: Gradient | None = None,
+    options: Options = Options(),  # noqa: B008
+) -> MESolveResult:
+    r"""Solve the Lindblad master equation on the MI355X.
+
+    Same signature, batching rules, errors and result as ``dq.mesolve``
+    (``dynamiqs/integrators/apis/mesolve.py:62-304``). Supported methods:
+    ``Tsit5`` and ``Rouchon1``; supported operators: constant (dense or DIA) and
+    piecewise-constant Hamiltonians (``dq.pwc``), constant jump operators.
+    """
+    # === convert arguments
+    H = astimeqarray(H)
+    Ls = [astimeqarray(L) for L in jump_ops]
+    rho0 = asqarray(rho0)
+    tsave = np.asarray(tsave, dtype=np.float64)
+    if exp_ops is not None:
+        exp_ops = [asqarray(E) for E in exp_ops] if len(exp_ops) > 0 else None
+
+    # === check arguments
+    _check_mesolve_args(H, Ls, rho0, exp_ops)
+    tsave = check_times(tsave, 'tsave')
+    check_options(options, 'mesolve')
+    options = options.initialise()
+    assert_method_supported(method, SUPPORTED_METHODS)
+    method.assert_supports_gradient(gradient)
+    if options.save_extra is not None:
+        raise NotImplementedError('`save_extra` is not supported by the MI355X solver')
+
+    out_shape, insts = enumerate_instances(H, Ls, rho0, options.cartesian_batching)
+    if options.assume_hermitian and isinstance(method, Tsit5):
+        for inst in insts:
+            check_hermitian(inst.rho0, 'y0')
+    Es = [E.to_numpy() for E in exp_ops] if exp_ops is not None else None
+    t0 = None if options.t0 is None else float(options.t0)
+
+    if isinstance(method, Tsit5):
+        plan = dense.build_plan(
+            'tsit5',
+            insts,
+            tsave,
+            exp_ops=Es,
+            t0=t0,
+            rtol=method.rtol,
+            atol=method.atol,
+            safety=method.safety_factor,
+            factormin=method.min_factor,
+            factormax=method.max_factor,
+            max_steps=method.max_steps,
+            save_states=options.save_states,
+        )
+    else:
+        plan = dense.build_plan(
+            'rouchon1',
+            insts,
+            tsave,
+            exp_ops=Es,
+            t0=t0,
+            dt=method.dt,
+            normalize=method.normalize,
+            save_states=options.save_states,
+        )
+    plan.run()
+    res = plan.results()
+    if np.any(res['status'] == -3):
+        raise max_steps_error()
+    if np.any(res['status'] == -4):
+        raise RuntimeError('The solver stopped on a non-finite error estimate or a vanishing step size.')
+
+    n = plan.n
+    states = res['states'].reshape(*out_shape, -1, n, n)
+    ysave = QArray(states, dims=rho0.dims)
+    Esave = None
+    if Es is not None:
+        Esave = res['expects'].reshape(*out_shape, len(Es), len(tsave))
+
+    def shaped(x):
+        x = np.asarray(x).reshape(out_shape)
+        return x if out_shape else x[()]
+
+    if isinstance(method, Tsit5):
+        infos = AdaptiveStepInfos(shaped(res['nsteps']), shaped(res['naccepted']), shaped(res['nrejected']))
+    else:
+        infos = FixedStepInfos(shaped(res['nsteps']))
+    return MESolveResult(tsave, method, gradient, options, ysave, Esave, None, infos)

@@ -1,0 +1,121 @@
+"""Integration methods, mirroring ``dynamiqs/method.py``.
+
+``Tsit5`` (:438-476, defaults of ``_DEAdaptiveStep`` :121-127) and ``Rouchon1``
+(:207-237) run on the GPU. The other method names of the reference exist so that
+``mesolve`` can raise the reference's ``TypeError`` for unsupported methods.
+"""
+
+from __future__ import annotations
+
+from .gradient import BackwardCheckpointed, Direct, Forward, Gradient
+
+
+class Method:
+    SUPPORTED_GRADIENT: tuple = ()
+
+    @classmethod
+    def supports_gradient(cls, gradient: Gradient | None) -> bool:
+        return isinstance(gradient, cls.SUPPORTED_GRADIENT)
+
+    @classmethod
+    def assert_supports_gradient(cls, gradient: Gradient | None) -> None:
+        if gradient is not None and not cls.supports_gradient(gradient):
+            support_str = ', '.join(f'`{x.__name__}`' for x in cls.SUPPORTED_GRADIENT)
+            raise ValueError(
+                f'Method `{cls.__name__}` does not support gradient'
+                f' `{type(gradient).__name__}` (supported gradient types:'
+                f' {support_str}).'
+            )
+
+    def _fields(self) -> dict:
+        return dict(self.__dict__)
+
+    def __str__(self) -> str:
+        inner = ', '.join(f'{k}={v}' for k, v in self._fields().items())
+        return f'{type(self).__name__}({inner})'
+
+    __repr__ = __str__
+
+    def __eq__(self, other) -> bool:
+        return type(self) is type(other) and self.__dict__ == other.__dict__
+
+    def __hash__(self) -> int:
+        return hash((type(self), tuple(sorted(self.__dict__.items()))))
+
+
+class _DEAdaptiveStep(Method):
+    max_steps = 100_000
+
+    def __init__(
+        self,
+        rtol: float = 1e-6,
+        atol: float = 1e-6,
+        safety_factor: float = 0.9,
+        min_factor: float = 0.2,
+        max_factor: float = 5.0,
+        max_steps: int = 100_000,
+    ):
+        self.rtol = rtol
+        self.atol = atol
+        self.safety_factor = safety_factor
+        self.min_factor = min_factor
+        self.max_factor = max_factor
+        self.max_steps = max_steps
+
+
+class Tsit5(_DEAdaptiveStep):
+    """Tsitouras 5(4) adaptive ODE method (FSAL, PID controller)."""
+
+    SUPPORTED_GRADIENT = (Direct, BackwardCheckpointed, Forward)
+
+
+class Dopri5(_DEAdaptiveStep):
+    SUPPORTED_GRADIENT = (Direct, BackwardCheckpointed, Forward)
+
+
+class Dopri8(_DEAdaptiveStep):
+    SUPPORTED_GRADIENT = (Direct, BackwardCheckpointed, Forward)
+
+
+class Kvaerno3(_DEAdaptiveStep):
+    SUPPORTED_GRADIENT = (Direct, BackwardCheckpointed, Forward)
+
+
+class Kvaerno5(_DEAdaptiveStep):
+    SUPPORTED_GRADIENT = (Direct, BackwardCheckpointed, Forward)
+
+
+class Euler(Method):
+    SUPPORTED_GRADIENT = (Direct, BackwardCheckpointed, Forward)
+
+    def __init__(self, dt: float):
+        self.dt = dt
+
+
+class Rouchon1(Method):
+    """First-order Rouchon method (fixed step, Kraus map with Cholesky normalisation)."""
+
+    SUPPORTED_GRADIENT = (Direct, BackwardCheckpointed, Forward)
+
+    def __init__(self, dt: float, normalize: bool = True):
+        self.dt = dt
+        self.normalize = normalize
+
+
+class Rouchon2(Method):
+    SUPPORTED_GRADIENT = (Direct, BackwardCheckpointed, Forward)
+
+    def __init__(self, dt: float | None = None, normalize: bool = True):
+        self.dt = dt
+        self.normalize = normalize
+
+
+class Rouchon3(Rouchon2):
+    pass
+
+
+class Expm(Method):
+    SUPPORTED_GRADIENT = (Direct,)
+
+    def __init__(self):
+        pass

@@ -1,0 +1,106 @@
+/*
+ * dqh -- MI355X-native (gfx950) kernels for the Lindblad master-equation solver
+ * behind a dynamiqs-compatible ``mesolve``.
+ *
+ * C ABI: plain pointers, sizes and POD descriptors. Every pointer inside a
+ * descriptor is a DEVICE pointer owned by the caller; the library never allocates
+ * or frees caller memory. Entry points return 0 on success or a negative
+ * ``dqh_status``; ``dqh_last_error()`` returns a thread-local message.
+ *
+ * Reference interface replaced (dynamiqs 0.3.5):
+ *   The integrator seam ``_mesolve`` -> ``integrator_constructor(...).run()``
+ *   (dynamiqs/integrators/apis/mesolve.py:350-384,
+ *    dynamiqs/integrators/core/abstract_integrator.py:16-54). The library runs
+ *   what ``MESolveDiffraxIntegrator.run`` (diffrax_integrator.py:135-147) and
+ *   ``MESolveFixedRouchon1Integrator`` (rouchon_integrator.py:385-432) run, for a
+ *   batch of independent instances per call.
+ */
+#ifndef DQH_H
+#define DQH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum dqh_status {
+  DQH_OK = 0,
+  DQH_BAD_ARG = -1,
+  DQH_BAD_SHAPE = -2,
+  DQH_MAX_STEPS = -3,  /* per-instance status: max_steps reached before t1 */
+  DQH_NONFINITE = -4,  /* per-instance status: NaN/Inf or dt underflow */
+  DQH_HIP_ERROR = -5,
+  DQH_UNSUPPORTED = -6,
+};
+
+/* Per-instance descriptor of one dense master-equation solve (all fields 8 bytes,
+ * no padding). Matrices are complex64 (float re, float im interleaved), row-major,
+ * zero-padded to np x np (np = 16 * ceil(n / 16)). */
+typedef struct dqh_me_inst {
+  int64_t n;          /* Hilbert-space dimension */
+  int64_t np;         /* padded dimension, multiple of 16, <= 256 */
+  int64_t nL;         /* jump-operator terms in the RHS (D/C pairs) */
+  int64_t nE;         /* expectation operators */
+  int64_t ntsave;     /* save times */
+  int64_t npwc;       /* piecewise-constant Hamiltonian terms (Tsit5) */
+  int64_t nseg;       /* PWC breakpoints (length of pwc_times) */
+  int64_t flags;      /* bit0: save states; bit1: generator has an Hnh term */
+  double rtol, atol, safety, factormin, factormax;
+  int64_t max_steps;  /* Tsit5: max steps; Rouchon1: number of fixed steps */
+  double t0;          /* start time (<= tsave[0]) */
+  double t1;          /* end time (= tsave[ntsave-1]) */
+  /* operators */
+  uint64_t tsave;     /* const double[ntsave] */
+  uint64_t Hnh;       /* const c64[np*np]   -iH0 - 1/2 sum L^dag L                 */
+  uint64_t Dops;      /* const c64[nsets*nL*np*np]  left factors  (L_k)            */
+  uint64_t Cops;      /* const c64[nsets*nL*np*np]  right factors (1/2 L_k^dag)    */
+  uint64_t EopsT;     /* const c64[nE*np*np] transposed exp ops                    */
+  uint64_t Hpwc;      /* const c64[npwc*np*np]  -i O_j of each PWC term            */
+  uint64_t pwc_times; /* const double[nseg]                                        */
+  uint64_t pwc_vals;  /* const double[npwc*(nseg-1)*2] complex coefficients        */
+  uint64_t step_t;    /* Rouchon1: const double[max_steps+1] step boundaries       */
+  uint64_t step_set;  /* Rouchon1: const int32[max_steps] Kraus-set index per step */
+  /* state, scratch and outputs */
+  uint64_t y;         /* double2[2*np*np]: FP64 state (buffer 0 holds rho0)        */
+  uint64_t work;      /* c64 scratch, dqh_me_work_elems(np, nL, npwc) elements     */
+  uint64_t states;    /* c64[ntsave*n*n] (or [n*n] when states are not saved)      */
+  uint64_t expects;   /* c64[nE*ntsave]                                             */
+  uint64_t final_state; /* c64[n*n]                                                 */
+  uint64_t stats;     /* int64[8]: nsteps, naccepted, nrejected, status, nrhs       */
+} dqh_me_inst;
+
+#define DQH_FLAG_SAVE_STATES 1
+#define DQH_FLAG_HAS_HNH 2
+
+/* Size checks for FFI mirrors. */
+int64_t dqh_sizeof_me_inst(void);
+/* Number of complex64 scratch elements one instance needs in ``work``. */
+int64_t dqh_me_work_elems(int64_t np, int64_t nL, int64_t npwc);
+
+/* Lindblad RHS in the Hermitian form (diffrax_integrator.py:319-323):
+ * final_state <- tmp + tmp^dag, tmp = Hnh rho + sum_k D_k rho C_k, rho = y buffer 0
+ * rounded to complex64. One workgroup per instance. */
+int dqh_lindblad_rhs_dense(const dqh_me_inst *d_insts, int64_t ninst, void *stream);
+
+/* Adaptive Tsit5 solve of every instance, one persistent workgroup per instance
+ * (replaces dx.diffeqsolve(..., dx.Tsit5(), dx.PIDController(...)) at
+ * diffrax_integrator.py:101-147 with the vector field of :319-323). */
+int dqh_mesolve_tsit5_dense(const dqh_me_inst *d_insts, int64_t ninst, void *stream);
+
+/* Fixed-step Rouchon1 with the Cholesky normalisation folded into the Kraus
+ * operators (rouchon_integrator.py:226-269, 394-407, 552-582): each step adds
+ * tmp + tmp^dag with tmp = sum_k D_k rho C_k to the FP64 state. */
+int dqh_mesolve_rouchon1_dense(const dqh_me_inst *d_insts, int64_t ninst, void *stream);
+
+/* Message of the last failing call on this thread. */
+const char *dqh_last_error(void);
+
+/* Device name / compute units of the current device (diagnostics). */
+int dqh_device_info(char *name, int64_t len, int64_t *cus);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DQH_H */

@@ -1,0 +1,180 @@
+"""GPU parity of the dense HIP path (C-ABI ``libdqh.so``) against the complex128
+oracle. Tolerances: complex64 MFMA arithmetic with FP64 state accumulation; every
+saved state must be within 1e-5 of the FP64 oracle in Frobenius norm and in
+trace distance unless a test states otherwise."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def trace_distance(a, b):
+    return 0.5 * np.abs(np.linalg.eigvalsh(0.5 * ((a - b) + (a - b).conj().T))).sum()
+
+
+def _rand_instance(rng, n, K, hermitian_scale=1.0, lscale=0.3):
+    H = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    H = hermitian_scale * (H + H.conj().T) / (2 * np.sqrt(n))
+    Ls = [lscale * (rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))) / np.sqrt(n) for _ in range(K)]
+    psi = rng.normal(size=(n, 1)) + 1j * rng.normal(size=(n, 1))
+    psi /= np.linalg.norm(psi)
+    return H, Ls, psi @ psi.conj().T
+
+
+def test_mfma_lane_layout_single_tile():
+    """16x16x16 complex product through one MFMA tile (checks the builtin's lane layout)."""
+    from dynamiqs_amd.integrators import dense
+    from oracle import lindblad
+
+    rng = np.random.default_rng(0)
+    n = 16
+    # K = 0, Hnh = -iH: rhs = -i[H, rho]; choose H real diagonal-free to exercise all lanes
+    H = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    H = H + H.conj().T
+    rho = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    rho = rho + rho.conj().T
+    inst = dense.Instance(H=H, Ls=[], rho0=rho)
+    plan = dense.build_plan('rhs', [inst], np.array([0.0, 1.0]))
+    plan.run()
+    out = plan.results()['final_state'][0]
+    ref = lindblad.rhs_hermitian(H, [], rho)
+    assert np.abs(out - ref).max() < 1e-4 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize('n,K', [(4, 1), (16, 1), (20, 2), (37, 3), (64, 2), (100, 1)])
+def test_rhs_matches_oracle(n, K):
+    from dynamiqs_amd.integrators import dense
+    from oracle import lindblad
+
+    rng = np.random.default_rng(n * 10 + K)
+    insts, refs = [], []
+    for _ in range(3):
+        H, Ls, rho = _rand_instance(rng, n, K)
+        insts.append(dense.Instance(H=H, Ls=Ls, rho0=rho))
+        refs.append(lindblad.rhs_hermitian(H, Ls, rho))
+    plan = dense.build_plan('rhs', insts, np.array([0.0, 1.0]))
+    plan.run()
+    out = plan.results()['final_state']
+    for b in range(3):
+        scale = np.abs(refs[b]).max()
+        assert np.abs(out[b] - refs[b]).max() < 2e-6 * np.sqrt(n) * scale + 1e-7
+
+
+def _c1():
+    from oracle import operators as op
+
+    n = 16
+    a = op.destroy(n)
+    return 1.0 * a.conj().T @ a, [np.sqrt(0.1) * a], op.coherent(n, 1.0), np.linspace(0, 2 * np.pi, 101)
+
+
+def test_tsit5_config1_parity_vs_fp64_oracle():
+    import dynamiqs_amd as dq
+    from oracle import mesolve as om
+    from oracle import operators as op
+
+    H, Ls, psi0, tsave = _c1()
+    ref = om.mesolve(H, Ls, psi0, tsave, rtol=1e-6, atol=1e-8)
+    a = dq.destroy(16, layout=dq.dense)
+    res = dq.mesolve(
+        1.0 * a.dag() @ a,
+        [np.sqrt(0.1) * a],
+        dq.coherent(16, 1.0),
+        tsave,
+        exp_ops=[dq.number(16)],
+        method=dq.method.Tsit5(rtol=1e-6, atol=1e-8),
+    )
+    st = res.states.to_numpy()
+    assert st.shape == (101, 16, 16) and st.dtype == np.complex64
+    errs = np.linalg.norm(st - ref['states'], axis=(-2, -1))
+    tds = [trace_distance(st[i].astype(np.complex128), ref['states'][i]) for i in range(101)]
+    assert errs.max() <= TOL, errs.max()
+    assert max(tds) <= TOL, max(tds)
+    # same controller: step counts agree with the FP64 oracle up to rounding-induced drift
+    assert abs(int(res.infos.nsteps) - ref['infos']['nsteps']) <= 2
+    # expectation values fused at save points
+    nbar = np.array([op.expect(op.number(16), s) for s in ref['states']])
+    np.testing.assert_allclose(res.expects[0], nbar, atol=TOL)
+    text = str(res)
+    assert text.startswith('==== MESolveResult ====\nMethod  : Tsit5\nInfos   : ')
+    assert 'States  : QArray complex64 (101, 16, 16) | 202.0 Kb' in text
+    assert 'Expects : Array complex64 (1, 101) | 0.8 Kb' in text
+
+
+def test_tsit5_batched_random_instances_alone_and_in_batch():
+    """Batch of uneven random problems; each instance equals its solo solve (batch invariance)."""
+    from dynamiqs_amd.integrators import dense
+    from oracle import mesolve as om
+
+    rng = np.random.default_rng(7)
+    n, K = 24, 2
+    tsave = np.linspace(0, 2.0, 9)
+    insts = []
+    for _ in range(5):
+        H, Ls, rho = _rand_instance(rng, n, K)
+        insts.append(dense.Instance(H=H, Ls=Ls, rho0=rho))
+    plan = dense.build_plan('tsit5', insts, tsave, rtol=1e-6, atol=1e-8)
+    plan.run()
+    out = plan.results()
+    for b, inst in enumerate(insts):
+        ref = om.mesolve(inst.H, inst.Ls, inst.rho0, tsave, rtol=1e-6, atol=1e-8)
+        err = np.linalg.norm(out['states'][b] - ref['states'], axis=(-2, -1)).max()
+        assert err <= TOL, (b, err)
+        solo = dense.build_plan('tsit5', [inst], tsave, rtol=1e-6, atol=1e-8)
+        solo.run()
+        so = solo.results()
+        np.testing.assert_array_equal(so['states'][0], out['states'][b])
+        assert so['nsteps'][0] == out['nsteps'][b]
+
+
+def test_rouchon1_dense_parity():
+    import dynamiqs_amd as dq
+    from oracle import mesolve as om
+    from oracle import operators as op
+
+    n = 8
+    delta = kappa = 2 * np.pi
+    a = op.destroy(n)
+    tsave = np.linspace(0, 0.3, 11)
+    ref = om.mesolve(delta * a.conj().T @ a, [np.sqrt(kappa) * a], op.coherent(n, 0.5), tsave, method='rouchon1', dt=1e-3)
+    A = dq.destroy(n)
+    res = dq.mesolve(delta * A.dag() @ A, [np.sqrt(kappa) * A], dq.coherent(n, 0.5), tsave, method=dq.method.Rouchon1(dt=1e-3))
+    st = res.states.to_numpy()
+    errs = np.linalg.norm(st - ref['states'], axis=(-2, -1))
+    assert errs.max() <= TOL, errs.max()
+    assert int(res.infos.nsteps) == ref['infos']['nsteps'] == 300
+    assert abs(np.trace(st[-1]) - 1) < 1e-6
+
+
+def test_tsit5_pwc_parity():
+    import dynamiqs_amd as dq
+    from oracle import mesolve as om
+    from oracle import operators as op
+
+    n = 12
+    rng = np.random.default_rng(3)
+    times = np.linspace(0, 2.0, 11)
+    u = rng.normal(0, 0.5, size=(3, 10))
+    a = op.destroy(n)
+    tsave = np.linspace(0, 2.0, 5)
+    A = dq.destroy(n, layout=dq.dense)
+    H = dq.pwc(times, u, A + A.dag()) + A.dag() @ A
+    res = dq.mesolve(H, [np.sqrt(0.05) * A], dq.fock(n, 0), tsave, method=dq.method.Tsit5(rtol=1e-7, atol=1e-9))
+    st = res.states.to_numpy()
+    assert st.shape == (3, 5, n, n)
+    for b in range(3):
+        Hop = om.TimeOp(a.conj().T @ a, [om.PWCTerm(times, u[b], a + a.conj().T)])
+        ref = om.mesolve(Hop, [np.sqrt(0.05) * a], op.fock(n, 0), tsave, rtol=1e-7, atol=1e-9)
+        err = np.linalg.norm(st[b] - ref['states'], axis=(-2, -1)).max()
+        assert err <= TOL, (b, err)
+
+
+def test_max_steps_raises_reference_error():
+    import dynamiqs_amd as dq
+
+    a = dq.destroy(4)
+    with pytest.raises(RuntimeError, match='maximum number of method steps'):
+        dq.mesolve(a.dag() @ a, [a], dq.fock(4, 2), np.linspace(0, 10, 3), method=dq.method.Tsit5(max_steps=3))
