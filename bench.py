@@ -1,0 +1,261 @@
+"""Benchmark of the MI355X mesolve hot path.
+
+Default workload (one "step" = one batched solve of the whole batch, inputs
+resident in HBM): batched dense Tsit5 on the two-photon driven Kerr oscillator,
+n = 64, H_b = -K a^dag^2 a^2 + eps2_b (a^dag^2 + a^2) with K = 1 and
+eps2_b ~ U[0.5, 4] (seeded), jump operators sqrt(0.01) a and sqrt(1.0) a^2,
+rho0 = vacuum, 201 save times on [0, 10], Tsit5 rtol = atol = 1e-6, complex64
+operators and states, B = 256 instances.
+
+Printed JSON line: solves/s of the batched path (``value``), the plain
+one-solve-per-call GPU path and their ratio, p50/p99 latencies, the FP32-MFMA
+roofline of the persistent Tsit5 kernel (HIP-event timing on the launch stream),
+the error of sampled instances against the complex128 oracle, and the oracle
+timed on the host CPU as the baseline.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+FP32_MFMA_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+
+
+def kerr_instances(B: int, n: int = 64, seed: int = 0):
+    """Two-photon driven Kerr oscillators (complex128 host operators)."""
+    from dynamiqs_amd.integrators.dense import Instance
+
+    rng = np.random.default_rng(seed)
+    eps2 = rng.uniform(0.5, 4.0, size=B)
+    a = np.diag(np.sqrt(np.arange(1, n, dtype=np.float64)), k=1).astype(np.complex128)
+    ad = a.conj().T
+    a2, ad2 = a @ a, ad @ ad
+    Ls = [np.sqrt(0.01) * a, np.sqrt(1.0) * a2]
+    rho0 = np.zeros((n, n), dtype=np.complex128)
+    rho0[0, 0] = 1.0
+    insts = []
+    for b in range(B):
+        H = -1.0 * ad2 @ a2 + eps2[b] * (ad2 + a2)
+        insts.append(Instance(H=H, Ls=Ls, rho0=rho0, key_H=(b,), key_L=(), key_rho=()))
+    return insts, eps2
+
+
+def trace_distance(a, b):
+    d = a - b
+    return 0.5 * np.abs(np.linalg.eigvalsh(0.5 * (d + d.conj().T))).sum()
+
+
+def cpu_oracle_solve(inst, tsave, rtol, atol):
+    """The complex128 oracle (CPU restatement) on one instance, 1 BLAS thread."""
+    from oracle import tsit5
+
+    H, Ls = inst.H, inst.Ls
+    LdL = sum(L.conj().T @ L for L in Ls)
+    Hnh = -1j * H - 0.5 * LdL
+    Lds = [0.5 * L.conj().T for L in Ls]
+
+    def f(t, y, tseg):
+        tmp = Hnh @ y
+        for L, Ld in zip(Ls, Lds):
+            tmp = tmp + (L @ y) @ Ld
+        return tmp + tmp.conj().T
+
+    saved, ylast, st = tsit5.integrate(f, inst.rho0, tsave, rtol=rtol, atol=atol)
+    return np.stack(saved), st
+
+
+class EventTimer:
+    """HIP events on the stream the kernels are launched on."""
+
+    def __init__(self, stream):
+        import torch
+
+        self.stream = stream
+        self.pairs = []
+        self.torch = torch
+
+    def start(self):
+        e = self.torch.cuda.Event(enable_timing=True)
+        e.record(self.stream)
+        return e
+
+    def stop(self, e0):
+        e1 = self.torch.cuda.Event(enable_timing=True)
+        e1.record(self.stream)
+        self.pairs.append((e0, e1))
+
+    def times_ms(self):
+        self.torch.cuda.synchronize()
+        return [a.elapsed_time(b) for a, b in self.pairs]
+
+
+def run_plan_timed(plan, reps, stream, timer=None):
+    import torch
+
+    walls = []
+    for _ in range(reps):
+        plan.reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e0 = timer.start() if timer else None
+        plan.run(stream)
+        if timer:
+            timer.stop(e0)
+        torch.cuda.synchronize()
+        walls.append(time.perf_counter() - t0)
+    return walls
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=3)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--batch', type=int, default=256)
+    ap.add_argument('--plain', type=int, default=2, help='number of one-solve-per-call solves timed')
+    ap.add_argument('--cpu-sample', type=int, default=1, help='instances solved by the CPU oracle (0: skip)')
+    args = ap.parse_args(argv)
+    if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
+        print(
+            'bench.py measures one MI355X: batched mesolve instances are independent, so a multi-GPU run '
+            'would shard instances across GPUs with no exchange; only --gpus 1 is measured here.',
+            file=sys.stderr,
+        )
+        return 2
+
+    import torch
+
+    from dynamiqs_amd.integrators import dense
+
+    n, K = 64, 2
+    rtol = atol = 1e-6
+    tsave = np.linspace(0.0, 10.0, 201)
+    insts, eps2 = kerr_instances(args.batch, n)
+    plan = dense.build_plan('tsit5', insts, tsave, rtol=rtol, atol=atol)
+    stream = torch.cuda.current_stream()
+
+    # warmup + timed batched solves (one step = one batched solve of B instances)
+    run_plan_timed(plan, args.warmup, stream)
+    torch.cuda.synchronize()
+    timer = EventTimer(stream)
+    t0 = time.perf_counter()
+    walls = run_plan_timed(plan, args.steps, stream, timer)
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    kern_ms = timer.times_ms()
+    res = plan.results()
+    if np.any(res['status'] != 0):
+        raise RuntimeError(f'solver status {np.unique(res["status"])}')
+    solves_per_s = args.batch * args.steps / sum(walls)
+    ms_per_step = 1e3 * sum(walls) / args.steps
+    f_rhs = (1 + 2 * K) * 8.0 * n**3
+    flops = float(np.sum(res['nrhs'])) * f_rhs
+    kern_avg_s = 1e-3 * float(np.mean(kern_ms))
+    achieved = flops / kern_avg_s / 1e12
+
+    # plain path: one solve per call, same kernel, B = 1
+    plain_lat = []
+    for i in range(args.plain):
+        p1 = dense.build_plan('tsit5', [insts[i]], tsave, rtol=rtol, atol=atol)
+        p1.run(stream)
+        torch.cuda.synchronize()
+        w = run_plan_timed(p1, 1, stream)
+        plain_lat.append(w[0])
+    plain_sps = len(plain_lat) / sum(plain_lat)
+
+    # error of sampled instances vs the complex128 oracle, and the CPU baseline timing
+    cpu = None
+    err = None
+    if args.cpu_sample > 0:
+        try:
+            from threadpoolctl import threadpool_limits
+        except ImportError:  # pragma: no cover
+            threadpool_limits = None
+        errs, tds, tcpu = [], [], 0.0
+        for i in range(args.cpu_sample):
+            if threadpool_limits is not None:
+                with threadpool_limits(limits=1):
+                    c0 = time.perf_counter()
+                    ref, st = cpu_oracle_solve(insts[i], tsave, rtol, atol)
+                    tcpu += time.perf_counter() - c0
+            else:
+                c0 = time.perf_counter()
+                ref, st = cpu_oracle_solve(insts[i], tsave, rtol, atol)
+                tcpu += time.perf_counter() - c0
+            got = res['states'][i].astype(np.complex128)
+            errs.append(float(np.linalg.norm(got - ref, axis=(-2, -1)).max()))
+            tds.append(max(trace_distance(got[j], ref[j]) for j in range(len(tsave))))
+        cpu = {
+            'value': args.cpu_sample / tcpu,
+            'unit': 'solves/s',
+            'cores': 1,
+            'kind': 'port',
+            'sample': f'{args.cpu_sample} of {args.batch} instances (complex128 Tsit5 oracle, 1 BLAS thread)',
+        }
+        err = {'max_frobenius': max(errs), 'max_trace_distance': max(tds), 'instances_checked': args.cpu_sample}
+
+    lat_sorted = sorted(plain_lat)
+    out = {
+        'metric': 'solves/s + p50/p99 latency, batched/served vs plain GPU mesolve at <=1e-5 err',
+        'value': solves_per_s,
+        'unit': 'solves/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': ms_per_step,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'c64 (fp32 MFMA), fp64 state/norms',
+        'data': 'synthetic',
+        'config': {
+            'workload': 'batched dense Tsit5, two-photon driven Kerr oscillator n=64, 2 jump ops, B=256, 201 tsave on [0,10], rtol=atol=1e-6',
+            'global_batch': args.batch,
+            'n': n,
+            'jump_ops': K,
+            'parallelism': 'one persistent workgroup per instance',
+        },
+        'plain_solves_per_s': plain_sps,
+        'batched_vs_plain': solves_per_s / plain_sps,
+        'latency_s': {
+            'batched_p50': float(np.percentile(walls, 50)),
+            'batched_p99': float(np.percentile(walls, 99)),
+            'plain_p50': float(np.percentile(lat_sorted, 50)),
+            'plain_p99': float(np.percentile(lat_sorted, 99)),
+        },
+        'steps_per_instance': {
+            'mean': float(np.mean(res['nsteps'])),
+            'max': int(np.max(res['nsteps'])),
+            'rejected_mean': float(np.mean(res['nrejected'])),
+        },
+        'roofline': {
+            'bound': 'mfma',
+            'achieved': achieved,
+            'peak': FP32_MFMA_PEAK_TFLOPS,
+            'unit': 'TFLOP/s',
+            'frac': achieved / FP32_MFMA_PEAK_TFLOPS,
+            'traffic': None,
+            'kernel': 'dqh::k_tsit5',
+            'kernel_avg_ms': float(np.mean(kern_ms)),
+            'flops_per_launch': flops,
+        },
+        'error_vs_fp64': err,
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == '__main__':
+    sys.exit(main())

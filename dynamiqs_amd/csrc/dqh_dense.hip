@@ -44,9 +44,28 @@ __constant__ double kR[7][4] = {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 
+constexpr int kWaves = 8;               // waves per workgroup (2 per SIMD)
+constexpr int kThreads = kWaves * 64;   // one workgroup = one instance
+constexpr int kResMaxL = 3;             // resident path: up to 3 jump-operator terms
+
 struct Acc {
   f4 re, im;
 };
+
+// Device buffers named in a descriptor are global memory: casting through
+// address_space(1) lets the compiler use global_load/store with an SGPR base and a
+// 32-bit VGPR offset instead of 64-bit flat addresses per access.
+template <typename T>
+__device__ __forceinline__ T *gp(uint64_t a) {
+  return (T *)((__attribute__((address_space(1))) T *)a);
+}
+
+// x^e for the step-size controller via the hardware exp2/log2 (single precision is
+// ample for a step factor and avoids the constant tables of the FP64 pow, which
+// would otherwise stay resident in VGPRs across the whole integration loop).
+__device__ __forceinline__ double ctrl_pow(double x, double e) {
+  return (double)__builtin_amdgcn_exp2f((float)e * __builtin_amdgcn_logf((float)x));
+}
 
 __device__ __forceinline__ void acc_zero(Acc &a) {
   a.re = f4{0.f, 0.f, 0.f, 0.f};
@@ -132,7 +151,7 @@ __device__ __forceinline__ void make_view(View &v, const dqh_me_inst &p) {
   v.nseg = (int)p.nseg;
   v.flags = (int)p.flags;
   v.nn = (size_t)v.np * v.np;
-  float2 *w = reinterpret_cast<float2 *>(p.work);
+  float2 *w = gp<float2>(p.work);
   v.kbase = w;
   v.s0 = 0;
   v.s6 = 6;
@@ -140,10 +159,10 @@ __device__ __forceinline__ void make_view(View &v, const dqh_me_inst &p) {
   v.tmp = w + 8 * v.nn;
   v.X = w + 9 * v.nn;
   v.Hcur = v.npwc > 0 ? w + (9 + v.nL) * v.nn : nullptr;
-  v.Hnh = v.npwc > 0 ? v.Hcur : reinterpret_cast<const float2 *>(p.Hnh);
-  v.D = reinterpret_cast<const float2 *>(p.Dops);
-  v.C = reinterpret_cast<const float2 *>(p.Cops);
-  double2 *y = reinterpret_cast<double2 *>(p.y);
+  v.Hnh = v.npwc > 0 ? v.Hcur : gp<const float2>(p.Hnh);
+  v.D = gp<const float2>(p.Dops);
+  v.C = gp<const float2>(p.Cops);
+  double2 *y = gp<double2>(p.y);
   v.y[0] = y;
   v.y[1] = y + v.nn;
 }
@@ -151,7 +170,7 @@ __device__ __forceinline__ void make_view(View &v, const dqh_me_inst &p) {
 // out <- tmp + tmp^dag with tmp = [Hnh ys] + sum_k (D_k ys) C_k. ys is read from v.ys.
 // Uses v.X (nL * nn) and v.tmp as scratch. Ends with a barrier.
 template <int NW>
-__device__ void wg_rhs(const View &v, const float2 *D, const float2 *C, bool hasH, float2 *out) {
+__device__ __forceinline__ void wg_rhs(const View &v, const float2 *D, const float2 *C, bool hasH, float2 *out) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int np = v.np, NT = np >> 4;
   const int nt2 = NT * NT;
@@ -194,17 +213,17 @@ __device__ __forceinline__ float2 to_f2(double2 a) { return make_float2((float)a
 
 // ys <- (float) y
 template <int NW>
-__device__ void wg_round_state(const View &v, const double2 *y) {
+__device__ __forceinline__ void wg_round_state(const View &v, const double2 *y) {
   for (size_t e = threadIdx.x; e < v.nn; e += NW * 64) v.ys[e] = to_f2(y[e]);
   __syncthreads();
 }
 
 // Hcur <- Hnh0 + sum_j c_j Hpwc_j for the segment containing t (zero outside)
 template <int NW>
-__device__ void wg_update_pwc(const View &v, const dqh_me_inst &p, int seg) {
-  const float2 *H0 = reinterpret_cast<const float2 *>(p.Hnh);
-  const float2 *Hp = reinterpret_cast<const float2 *>(p.Hpwc);
-  const double *vals = reinterpret_cast<const double *>(p.pwc_vals);
+__device__ __forceinline__ void wg_update_pwc(const View &v, const dqh_me_inst &p, int seg) {
+  const float2 *H0 = gp<const float2>(p.Hnh);
+  const float2 *Hp = gp<const float2>(p.Hpwc);
+  const double *vals = gp<const double>(p.pwc_vals);
   const int nv = v.nseg - 1;
   for (size_t e = threadIdx.x; e < v.nn; e += NW * 64) {
     double re = H0[e].x, im = H0[e].y;
@@ -237,11 +256,11 @@ __device__ __forceinline__ int pwc_segment(const double *times, int nseg, double
 // Save the state interpolated as y + h * sum_j w_j k_j (or y itself when h == 0)
 // into states[isave] (unpadded) and its expectation values.
 template <int NW>
-__device__ void wg_save(const View &v, const dqh_me_inst &p, const double2 *y, double h, const double *w,
+__device__ __forceinline__ void wg_save(const View &v, const dqh_me_inst &p, const double2 *y, double h, const double *w,
                         int nk, int isave, double *sh) {
-  float2 *states = reinterpret_cast<float2 *>(p.states);
+  float2 *states = gp<float2>(p.states);
   const bool save_states = (v.flags & DQH_FLAG_SAVE_STATES) != 0;
-  const float2 *ET = reinterpret_cast<const float2 *>(p.EopsT);
+  const float2 *ET = gp<const float2>(p.EopsT);
   double er[4] = {0, 0, 0, 0}, ei[4] = {0, 0, 0, 0};
   for (size_t e = threadIdx.x; e < v.nn; e += NW * 64) {
     double re = y[e].x, im = y[e].y;
@@ -263,7 +282,7 @@ __device__ void wg_save(const View &v, const dqh_me_inst &p, const double2 *y, d
       }
     }
   }
-  float2 *ex = reinterpret_cast<float2 *>(p.expects);
+  float2 *ex = gp<float2>(p.expects);
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     if (q >= v.nE) continue;
@@ -294,8 +313,8 @@ __device__ void wg_save(const View &v, const dqh_me_inst &p, const double2 *y, d
 }
 
 template <int NW>
-__device__ void wg_write_final(const View &v, const dqh_me_inst &p, const double2 *y) {
-  float2 *fin = reinterpret_cast<float2 *>(p.final_state);
+__device__ __forceinline__ void wg_write_final(const View &v, const dqh_me_inst &p, const double2 *y) {
+  float2 *fin = gp<float2>(p.final_state);
   for (size_t e = threadIdx.x; e < v.nn; e += NW * 64) {
     const int i = (int)(e / v.np), j = (int)(e % v.np);
     if (i < v.n && j < v.n) fin[(size_t)i * v.n + j] = to_f2(y[e]);
@@ -304,17 +323,16 @@ __device__ void wg_write_final(const View &v, const dqh_me_inst &p, const double
 
 // ------------------------------------------------------------ unit RHS kernel
 template <int NW>
-__global__ __launch_bounds__(NW * 64) void k_rhs(const dqh_me_inst *__restrict__ insts) {
-  const dqh_me_inst &p = insts[blockIdx.x];
+__device__ __forceinline__ void rhs_global(const dqh_me_inst &p) {
   View v;
   make_view(v, p);
   if (v.npwc > 0) {
-    const int seg = pwc_segment(reinterpret_cast<const double *>(p.pwc_times), v.nseg, p.t0);
+    const int seg = pwc_segment(gp<const double>(p.pwc_times), v.nseg, p.t0);
     wg_update_pwc<NW>(v, p, seg);
   }
   wg_round_state<NW>(v, v.y[0]);
   wg_rhs<NW>(v, v.D, v.C, (v.flags & DQH_FLAG_HAS_HNH) != 0, kp(v, 0));
-  float2 *fin = reinterpret_cast<float2 *>(p.final_state);
+  float2 *fin = gp<float2>(p.final_state);
   for (size_t e = threadIdx.x; e < v.nn; e += NW * 64) {
     const int i = (int)(e / v.np), j = (int)(e % v.np);
     if (i < v.n && j < v.n) fin[(size_t)i * v.n + j] = kp(v, 0)[e];
@@ -323,14 +341,12 @@ __global__ __launch_bounds__(NW * 64) void k_rhs(const dqh_me_inst *__restrict__
 
 // ----------------------------------------------------------- Tsit5 kernel
 template <int NW>
-__global__ __launch_bounds__(NW * 64) void k_tsit5(const dqh_me_inst *__restrict__ insts) {
-  __shared__ double sh[NW];
-  const dqh_me_inst &p = insts[blockIdx.x];
+__device__ __forceinline__ void tsit5_global(const dqh_me_inst &p, double *sh) {
   View v;
   make_view(v, p);
-  const double *tsave = reinterpret_cast<const double *>(p.tsave);
-  const double *ptimes = reinterpret_cast<const double *>(p.pwc_times);
-  int64_t *stats = reinterpret_cast<int64_t *>(p.stats);
+  const double *tsave = gp<const double>(p.tsave);
+  const double *ptimes = gp<const double>(p.pwc_times);
+  int64_t *stats = gp<int64_t>(p.stats);
   const double rtol = p.rtol, atol = p.atol;
   const double inv_nn = 1.0 / ((double)v.n * v.n);
   const double t_end = p.t1;
@@ -390,7 +406,7 @@ __global__ __launch_bounds__(NW * 64) void k_tsit5(const dqh_me_inst *__restrict
     }
     const double d2 = sqrt(block_sum<NW>(s2, sh) * inv_nn) / h0;
     const double md = fmax(d1, d2);
-    const double h1 = md <= 1e-15 ? fmax(1e-6, h0 * 1e-3) : pow(0.01 / md, 0.2);
+    const double h1 = md <= 1e-15 ? fmax(1e-6, h0 * 1e-3) : ctrl_pow(0.01 / md, 0.2);
     dt = fmin(100.0 * h0, h1);
   }
 
@@ -473,7 +489,7 @@ __global__ __launch_bounds__(NW * 64) void k_tsit5(const dqh_me_inst *__restrict
     else if (norm == 0.0)
       factor = p.factormax;
     else
-      factor = p.safety * pow(norm, -0.2);
+      factor = p.safety * ctrl_pow(norm, -0.2);
     factor = fmin(fmax(factor, keep ? 1.0 : p.factormin), p.factormax);
     ++nsteps;
     if (keep) {
@@ -506,7 +522,7 @@ __global__ __launch_bounds__(NW * 64) void k_tsit5(const dqh_me_inst *__restrict
   }
   wg_write_final<NW>(v, p, v.y[cur]);
   if (!(v.flags & DQH_FLAG_SAVE_STATES)) {
-    float2 *st = reinterpret_cast<float2 *>(p.states);
+    float2 *st = gp<float2>(p.states);
     for (size_t e = threadIdx.x; e < v.nn; e += NW * 64) {
       const int i = (int)(e / v.np), j = (int)(e % v.np);
       if (i < v.n && j < v.n) st[(size_t)i * v.n + j] = to_f2(v.y[cur][e]);
@@ -523,15 +539,13 @@ __global__ __launch_bounds__(NW * 64) void k_tsit5(const dqh_me_inst *__restrict
 
 // -------------------------------------------------------- Rouchon1 kernel
 template <int NW>
-__global__ __launch_bounds__(NW * 64) void k_rouchon1(const dqh_me_inst *__restrict__ insts) {
-  __shared__ double sh[NW];
-  const dqh_me_inst &p = insts[blockIdx.x];
+__device__ __forceinline__ void rouchon1_global(const dqh_me_inst &p, double *sh) {
   View v;
   make_view(v, p);
-  const double *tsave = reinterpret_cast<const double *>(p.tsave);
-  const double *step_t = reinterpret_cast<const double *>(p.step_t);
-  const int *step_set = reinterpret_cast<const int *>(p.step_set);
-  int64_t *stats = reinterpret_cast<int64_t *>(p.stats);
+  const double *tsave = gp<const double>(p.tsave);
+  const double *step_t = gp<const double>(p.step_t);
+  const int *step_set = gp<const int>(p.step_set);
+  int64_t *stats = gp<int64_t>(p.stats);
   const int64_t nsteps = p.max_steps;
   const double w1[1] = {1.0};
   double2 *y = v.y[0];
@@ -565,7 +579,7 @@ __global__ __launch_bounds__(NW * 64) void k_rouchon1(const dqh_me_inst *__restr
   }
   wg_write_final<NW>(v, p, y);
   if (!(v.flags & DQH_FLAG_SAVE_STATES)) {
-    float2 *st = reinterpret_cast<float2 *>(p.states);
+    float2 *st = gp<float2>(p.states);
     for (size_t e = threadIdx.x; e < v.nn; e += NW * 64) {
       const int i = (int)(e / v.np), j = (int)(e % v.np);
       if (i < v.n && j < v.n) st[(size_t)i * v.n + j] = to_f2(y[e]);
@@ -582,9 +596,48 @@ __global__ __launch_bounds__(NW * 64) void k_rouchon1(const dqh_me_inst *__restr
 
 }  // namespace dqh
 
-using namespace dqh;
+#include "dqh_resident.h"
 
-static constexpr int kNW = 4;
+namespace dqh {
+
+// One launch serves a heterogeneous batch: each workgroup picks the body that
+// fits its instance (register/LDS-resident for np <= 64 and at most 3 jump
+// operators, global-memory state otherwise).
+__global__ __launch_bounds__(kThreads) void k_tsit5(const dqh_me_inst *__restrict__ insts) {
+  __shared__ double sh[kWaves];
+  __shared__ float2 lds[kResLdsFloat2];
+  const dqh_me_inst &p = insts[blockIdx.x];
+  const int np = (int)p.np;
+  if (p.nL <= kResMaxL) {
+    if (np == 16) return tsit5_resident<16>(p, sh, lds);
+    if (np == 32) return tsit5_resident<32>(p, sh, lds);
+    if (np == 48) return tsit5_resident<48>(p, sh, lds);
+    if (np == 64) return tsit5_resident<64>(p, sh, lds);
+  }
+  tsit5_global<kWaves>(p, sh);
+}
+
+__global__ __launch_bounds__(kThreads) void k_rhs(const dqh_me_inst *__restrict__ insts) {
+  __shared__ float2 lds[kResLdsFloat2];
+  const dqh_me_inst &p = insts[blockIdx.x];
+  const int np = (int)p.np;
+  if (p.nL <= kResMaxL) {
+    if (np == 16) return rhs_resident<16>(p, lds);
+    if (np == 32) return rhs_resident<32>(p, lds);
+    if (np == 48) return rhs_resident<48>(p, lds);
+    if (np == 64) return rhs_resident<64>(p, lds);
+  }
+  rhs_global<kWaves>(p);
+}
+
+__global__ __launch_bounds__(kThreads) void k_rouchon1(const dqh_me_inst *__restrict__ insts) {
+  __shared__ double sh[kWaves];
+  rouchon1_global<kWaves>(insts[blockIdx.x], sh);
+}
+
+}  // namespace dqh
+
+using namespace dqh;
 
 extern "C" int64_t dqh_sizeof_me_inst(void) { return (int64_t)sizeof(dqh_me_inst); }
 
@@ -602,18 +655,18 @@ static int launch_checked(const char *what, hipError_t err) {
 
 extern "C" int dqh_lindblad_rhs_dense(const dqh_me_inst *d_insts, int64_t ninst, void *stream) {
   if (!d_insts || ninst <= 0 || ninst > (1 << 30)) return fail(DQH_BAD_ARG, "dqh_lindblad_rhs_dense: bad arguments");
-  hipLaunchKernelGGL(k_rhs<kNW>, dim3((unsigned)ninst), dim3(kNW * 64), 0, (hipStream_t)stream, d_insts);
+  hipLaunchKernelGGL(k_rhs, dim3((unsigned)ninst), dim3(kThreads), 0, (hipStream_t)stream, d_insts);
   return launch_checked("dqh_lindblad_rhs_dense", hipGetLastError());
 }
 
 extern "C" int dqh_mesolve_tsit5_dense(const dqh_me_inst *d_insts, int64_t ninst, void *stream) {
   if (!d_insts || ninst <= 0 || ninst > (1 << 30)) return fail(DQH_BAD_ARG, "dqh_mesolve_tsit5_dense: bad arguments");
-  hipLaunchKernelGGL(k_tsit5<kNW>, dim3((unsigned)ninst), dim3(kNW * 64), 0, (hipStream_t)stream, d_insts);
+  hipLaunchKernelGGL(k_tsit5, dim3((unsigned)ninst), dim3(kThreads), 0, (hipStream_t)stream, d_insts);
   return launch_checked("dqh_mesolve_tsit5_dense", hipGetLastError());
 }
 
 extern "C" int dqh_mesolve_rouchon1_dense(const dqh_me_inst *d_insts, int64_t ninst, void *stream) {
   if (!d_insts || ninst <= 0 || ninst > (1 << 30)) return fail(DQH_BAD_ARG, "dqh_mesolve_rouchon1_dense: bad arguments");
-  hipLaunchKernelGGL(k_rouchon1<kNW>, dim3((unsigned)ninst), dim3(kNW * 64), 0, (hipStream_t)stream, d_insts);
+  hipLaunchKernelGGL(k_rouchon1, dim3((unsigned)ninst), dim3(kThreads), 0, (hipStream_t)stream, d_insts);
   return launch_checked("dqh_mesolve_rouchon1_dense", hipGetLastError());
 }

@@ -1,0 +1,487 @@
+// Register/LDS-resident Tsit5 for padded dimensions np <= 64 (gfx950).
+//
+// One 512-thread workgroup integrates one instance. The seven FP32 Tsit5 stages
+// live in registers (thread t owns the row-major elements t, t + 512, ...); the
+// stage input rho_s, the products X_k = D_k rho_s and the pre-symmetrised tmp live
+// in LDS (padded row stride np + 2 keeps the MFMA fragment reads conflict-free).
+// The FP64 state y stays in its global buffer (L2-resident, each thread reads and
+// writes only its own elements, 16 B per element per stage); the operators are
+// L2-resident and shared by the batch.
+#pragma once
+
+namespace dqh {
+
+template <int NP>
+struct RCfg {
+  static constexpr int NT = NP / 16;
+  static constexpr int NN = NP * NP;
+  static constexpr int LDP = NP + 2;
+  static constexpr int MAT = NP * LDP;  // float2 per LDS matrix
+  static constexpr int NE = (NN + kThreads - 1) / kThreads;
+  static constexpr int TJ = (NT % 2 == 0) ? 2 : 1;  // output tiles sharing one A fragment
+  static constexpr int NJC = NT / TJ;
+  static constexpr int ITEMS2 = NT * NJC;
+  static constexpr int MAXI2 = (ITEMS2 + kWaves - 1) / kWaves;
+};
+
+// (1 + kResMaxL) matrices of 64 x 66 complex64: ys, then X_1..X_3 (X_1 reused for tmp)
+constexpr int kResLdsFloat2 = (1 + kResMaxL) * 64 * 66;
+
+// acc[t] += A[row0:row0+16, 0:K] @ B[0:K, col0+16t : col0+16t+16], t < TJ; the A
+// fragment of each 16-wide k block is loaded once and reused for the TJ tiles.
+template <int TJ>
+__device__ __forceinline__ void strip_mac(Acc (&acc)[TJ], const float2 *__restrict__ A, int lda,
+                                          const float2 *__restrict__ B, int ldb, int row0, int col0, int K,
+                                          int lane) {
+  const int r = lane & 15, g = lane >> 4;
+  const float2 *Ap = A + (size_t)(row0 + r) * lda + 4 * g;
+  const float2 *Bp = B + (size_t)(4 * g) * ldb + col0 + r;
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    const float4 a01 = *reinterpret_cast<const float4 *>(Ap + k0);
+    const float4 a23 = *reinterpret_cast<const float4 *>(Ap + k0 + 2);
+    const float2 *bp = Bp + (size_t)k0 * ldb;
+#pragma unroll
+    for (int t = 0; t < TJ; ++t) {
+      const float2 b0 = bp[16 * t];
+      const float2 b1 = bp[ldb + 16 * t];
+      const float2 b2 = bp[2 * ldb + 16 * t];
+      const float2 b3 = bp[3 * ldb + 16 * t];
+      cmfma(acc[t], a01.x, a01.y, b0);
+      cmfma(acc[t], a01.z, a01.w, b1);
+      cmfma(acc[t], a23.x, a23.y, b2);
+      cmfma(acc[t], a23.z, a23.w, b3);
+    }
+  }
+}
+
+template <int TJ>
+__device__ __forceinline__ void strip_store(const Acc (&acc)[TJ], float2 *C, int ldc, int row0, int col0, int lane) {
+#pragma unroll
+  for (int t = 0; t < TJ; ++t) tile_store(acc[t], C, ldc, row0, col0 + 16 * t, lane);
+}
+
+// out[q] <- (tmp + tmp^dag) at the owned elements, tmp = [Hnh ys] + sum_k (D_k ys) C_k.
+// Precondition: ys written to lds[0 .. MAT) by every thread (this routine starts
+// with the barrier that publishes it).
+template <int NP>
+__device__ __forceinline__ void res_rhs(const float2 *__restrict__ Hnh, const float2 *__restrict__ D, const float2 *__restrict__ C,
+                        int nL, bool hasH, float2 *lds, float2 (&out)[RCfg<NP>::NE]) {
+  using R = RCfg<NP>;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const float2 *ys = lds;
+  float2 *X = lds + R::MAT;
+  __syncthreads();
+  // phase 1: X_k = D_k ys (A from L2, B from LDS)
+  const int items1 = nL * R::NT * R::NJC;
+  for (int it = wave; it < items1; it += kWaves) {
+    const int kk = it / (R::NT * R::NJC), rem = it % (R::NT * R::NJC);
+    const int ti = rem / R::NJC, jc = rem % R::NJC;
+    Acc acc[R::TJ];
+#pragma unroll
+    for (int t = 0; t < R::TJ; ++t) acc_zero(acc[t]);
+    strip_mac<R::TJ>(acc, D + (size_t)kk * R::NN, NP, ys, R::LDP, ti * 16, jc * R::TJ * 16, NP, lane);
+    strip_store<R::TJ>(acc, X + kk * R::MAT, R::LDP, ti * 16, jc * R::TJ * 16, lane);
+  }
+  __syncthreads();
+  // phase 2: tmp = Hnh ys + sum_k X_k C_k, kept in registers across the barrier
+  Acc acc2[R::MAXI2][R::TJ];
+#pragma unroll
+  for (int m = 0; m < R::MAXI2; ++m) {
+    const int it = wave + kWaves * m;
+#pragma unroll
+    for (int t = 0; t < R::TJ; ++t) acc_zero(acc2[m][t]);
+    if (it < R::ITEMS2) {
+      const int ti = it / R::NJC, jc = it % R::NJC;
+      if (hasH) strip_mac<R::TJ>(acc2[m], Hnh, NP, ys, R::LDP, ti * 16, jc * R::TJ * 16, NP, lane);
+      for (int kk = 0; kk < nL; ++kk)
+        strip_mac<R::TJ>(acc2[m], X + kk * R::MAT, R::LDP, C + (size_t)kk * R::NN, NP, ti * 16, jc * R::TJ * 16, NP,
+                         lane);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < R::MAXI2; ++m) {
+    const int it = wave + kWaves * m;
+    if (it < R::ITEMS2) {
+      const int ti = it / R::NJC, jc = it % R::NJC;
+      strip_store<R::TJ>(acc2[m], X, R::LDP, ti * 16, jc * R::TJ * 16, lane);
+    }
+  }
+  __syncthreads();
+  // phase 3: Hermitian symmetrisation into the owned elements
+#pragma unroll
+  for (int q = 0; q < R::NE; ++q) {
+    const int e = threadIdx.x + kThreads * q;
+    if (e < R::NN) {
+      const int i = e / NP, j = e % NP;
+      const float2 a = X[i * R::LDP + j], b = X[j * R::LDP + i];
+      out[q] = make_float2(a.x + b.x, a.y - b.y);
+    } else {
+      out[q] = make_float2(0.f, 0.f);
+    }
+  }
+}
+
+template <int NP>
+__device__ __forceinline__ void res_write_ys(float2 *lds, const float2 (&v)[RCfg<NP>::NE]) {
+  using R = RCfg<NP>;
+#pragma unroll
+  for (int q = 0; q < R::NE; ++q) {
+    const int e = threadIdx.x + kThreads * q;
+    if (e < R::NN) lds[(e / NP) * R::LDP + (e % NP)] = v[q];
+  }
+}
+
+__device__ __forceinline__ const float2 *res_hnh(const dqh_me_inst &p) {
+  return p.npwc > 0 ? gp<const float2>(p.work) : gp<const float2>(p.Hnh);
+}
+
+// Hcur (first np*np of work) <- Hnh0 + sum_j c_j Hpwc_j for segment seg; global memory,
+// published to the workgroup by the barrier at the start of the next res_rhs.
+template <int NP>
+__device__ __forceinline__ void res_update_pwc(const dqh_me_inst &p, int seg) {
+  using R = RCfg<NP>;
+  const float2 *H0 = gp<const float2>(p.Hnh);
+  const float2 *Hp = gp<const float2>(p.Hpwc);
+  const double *vals = gp<const double>(p.pwc_vals);
+  float2 *Hc = gp<float2>(p.work);
+  const int nv = (int)p.nseg - 1, npwc = (int)p.npwc;
+  __syncthreads();  // no wave still reads the previous Hcur
+  for (int e = threadIdx.x; e < R::NN; e += kThreads) {
+    double re = H0[e].x, im = H0[e].y;
+    if (seg >= 0 && seg < nv) {
+      for (int j = 0; j < npwc; ++j) {
+        const double cr = vals[2 * (j * nv + seg)], ci = vals[2 * (j * nv + seg) + 1];
+        const float2 o = Hp[(size_t)j * R::NN + e];
+        re += cr * o.x - ci * o.y;
+        im += cr * o.y + ci * o.x;
+      }
+    }
+    Hc[e] = make_float2((float)re, (float)im);
+  }
+  __threadfence_block();
+}
+
+// save y + h sum_j w_j k_j at save index isave (states and expectation values)
+template <int NP>
+__device__ __forceinline__ void res_save(const dqh_me_inst &p, const double2 *__restrict__ y, const float2 (&k)[7][RCfg<NP>::NE],
+                         double h, const double (&w)[7], bool interp, int isave, double *sh) {
+  using R = RCfg<NP>;
+  const int n = (int)p.n, nE = (int)p.nE, ntsave = (int)p.ntsave;
+  const bool save_states = (p.flags & DQH_FLAG_SAVE_STATES) != 0;
+  float2 *states = gp<float2>(p.states);
+  const float2 *ET = gp<const float2>(p.EopsT);
+  double vr[R::NE], vi[R::NE];
+#pragma unroll
+  for (int q = 0; q < R::NE; ++q) {
+    const int e0 = threadIdx.x + kThreads * q;
+    const double2 yq = e0 < R::NN ? y[e0] : make_double2(0.0, 0.0);
+    double re = yq.x, im = yq.y;
+    if (interp) {
+#pragma unroll
+      for (int j = 0; j < 7; ++j) {
+        re += h * w[j] * k[j][q].x;
+        im += h * w[j] * k[j][q].y;
+      }
+    }
+    vr[q] = re;
+    vi[q] = im;
+    const int e = threadIdx.x + kThreads * q;
+    const int i = e / NP, jj = e % NP;
+    if (save_states && e < R::NN && i < n && jj < n)
+      states[((size_t)isave * n + i) * n + jj] = make_float2((float)re, (float)im);
+  }
+  float2 *ex = gp<float2>(p.expects);
+  for (int qe = 0; qe < nE; ++qe) {
+    double ar = 0, ai = 0;
+#pragma unroll
+    for (int q = 0; q < R::NE; ++q) {
+      const int e = threadIdx.x + kThreads * q;
+      if (e < R::NN) {
+        const float2 o = ET[(size_t)qe * R::NN + e];
+        ar += o.x * vr[q] - o.y * vi[q];
+        ai += o.x * vi[q] + o.y * vr[q];
+      }
+    }
+    const double sr = block_sum<kWaves>(ar, sh), si = block_sum<kWaves>(ai, sh);
+    if (threadIdx.x == 0) ex[(size_t)qe * ntsave + isave] = make_float2((float)sr, (float)si);
+  }
+}
+
+template <int NP>
+__device__ __forceinline__ void rhs_resident(const dqh_me_inst &p, float2 *lds) {
+  using R = RCfg<NP>;
+  if (p.npwc > 0) {
+    res_update_pwc<NP>(p, pwc_segment(gp<const double>(p.pwc_times), (int)p.nseg, p.t0));
+  }
+  const double2 *yg = gp<const double2>(p.y);
+  float2 v[R::NE];
+#pragma unroll
+  for (int q = 0; q < R::NE; ++q) {
+    const int e = threadIdx.x + kThreads * q;
+    v[q] = e < R::NN ? make_float2((float)yg[e].x, (float)yg[e].y) : make_float2(0.f, 0.f);
+  }
+  res_write_ys<NP>(lds, v);
+  float2 out[R::NE];
+  res_rhs<NP>(res_hnh(p), gp<const float2>(p.Dops), gp<const float2>(p.Cops),
+              (int)p.nL, (p.flags & DQH_FLAG_HAS_HNH) != 0, lds, out);
+  float2 *fin = gp<float2>(p.final_state);
+  const int n = (int)p.n;
+#pragma unroll
+  for (int q = 0; q < R::NE; ++q) {
+    const int e = threadIdx.x + kThreads * q;
+    const int i = e / NP, j = e % NP;
+    if (e < R::NN && i < n && j < n) fin[(size_t)i * n + j] = out[q];
+  }
+}
+
+template <int NP>
+__device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh, float2 *lds) {
+  using R = RCfg<NP>;
+  constexpr int NE = R::NE;
+  const int n = (int)p.n, nL = (int)p.nL, ntsave = (int)p.ntsave, nseg = (int)p.nseg;
+  const bool pwc = p.npwc > 0;
+  const bool hasH = (p.flags & DQH_FLAG_HAS_HNH) != 0;
+  const double *tsave = gp<const double>(p.tsave);
+  const double *ptimes = gp<const double>(p.pwc_times);
+  const float2 *D = gp<const float2>(p.Dops);
+  const float2 *C = gp<const float2>(p.Cops);
+  const float2 *Hnh = res_hnh(p);
+  int64_t *stats = gp<int64_t>(p.stats);
+  const double rtol = p.rtol, atol = p.atol;
+  const double inv_nn = 1.0 / ((double)n * n);
+  const double t_end = p.t1;
+
+  // y: FP64 state in global memory, element e of thread t at yg[e] (e = t + 512 q)
+  double2 *yg = gp<double2>(p.y);
+  float2 k[7][NE];
+  float2 tmp[NE];
+#pragma unroll
+  for (int q = 0; q < NE; ++q) {
+#pragma unroll
+    for (int j = 0; j < 7; ++j) k[j][q] = make_float2(0.f, 0.f);
+  }
+  auto yload = [&](int q) -> double2 {
+    const int e = threadIdx.x + kThreads * q;
+    return e < R::NN ? yg[e] : make_double2(0.0, 0.0);
+  };
+  double w0[7] = {0, 0, 0, 0, 0, 0, 0};
+
+  double t = p.t0;
+  int isave = 0;
+  int64_t nsteps = 0, nacc = 0, nrej = 0, nrhs = 0;
+  int status = 0;
+  int seg = -2;
+
+  while (isave < ntsave && tsave[isave] <= t) {
+    res_save<NP>(p, yg, k, 0.0, w0, false, isave, sh);
+    ++isave;
+  }
+  if (pwc) {
+    seg = pwc_segment(ptimes, nseg, t);
+    res_update_pwc<NP>(p, seg);
+  }
+  // k0 = f(y0)
+#pragma unroll
+  for (int q = 0; q < NE; ++q) {
+    const double2 yq = yload(q);
+    tmp[q] = make_float2((float)yq.x, (float)yq.y);
+  }
+  res_write_ys<NP>(lds, tmp);
+  res_rhs<NP>(Hnh, D, C, nL, hasH, lds, k[0]);
+  ++nrhs;
+
+  // initial step (Hairer-Norsett-Wanner II.4), scale atol + rtol |y0|
+  double dt;
+  {
+    double s0 = 0, s1 = 0;
+    double h0;
+#pragma unroll
+    for (int q = 0; q < NE; ++q) {
+      const double2 yq = yload(q);
+      const double ay2 = yq.x * yq.x + yq.y * yq.y;
+      const double sc = atol + rtol * sqrt(ay2);
+      s0 += ay2 / (sc * sc);
+      s1 += ((double)k[0][q].x * k[0][q].x + (double)k[0][q].y * k[0][q].y) / (sc * sc);
+    }
+    const double d0 = sqrt(block_sum<kWaves>(s0, sh) * inv_nn);
+    const double d1 = sqrt(block_sum<kWaves>(s1, sh) * inv_nn);
+    h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
+#pragma unroll
+    for (int q = 0; q < NE; ++q) {
+      const double2 yq = yload(q);
+      tmp[q] = make_float2((float)(yq.x + h0 * k[0][q].x), (float)(yq.y + h0 * k[0][q].y));
+    }
+    res_write_ys<NP>(lds, tmp);
+    res_rhs<NP>(Hnh, D, C, nL, hasH, lds, k[1]);
+    ++nrhs;
+    double s2 = 0;
+#pragma unroll
+    for (int q = 0; q < NE; ++q) {
+      const double2 yq = yload(q);
+      const double sc = atol + rtol * sqrt(yq.x * yq.x + yq.y * yq.y);
+      const double dr = (double)k[1][q].x - k[0][q].x, di = (double)k[1][q].y - k[0][q].y;
+      s2 += (dr * dr + di * di) / (sc * sc);
+    }
+    const double d2 = sqrt(block_sum<kWaves>(s2, sh) * inv_nn) / h0;
+    const double md = fmax(d1, d2);
+    const double h1 = md <= 1e-15 ? fmax(1e-6, h0 * 1e-3) : ctrl_pow(0.01 / md, 0.2);
+    dt = fmin(100.0 * h0, h1);
+  }
+
+  bool need_k0 = false;
+  while (t < t_end) {
+    if (nsteps >= p.max_steps) {
+      status = DQH_MAX_STEPS;
+      break;
+    }
+    if (pwc) {
+      const int s = pwc_segment(ptimes, nseg, t);
+      if (s != seg) {
+        seg = s;
+        res_update_pwc<NP>(p, seg);
+      }
+    }
+    if (need_k0) {
+#pragma unroll
+      for (int q = 0; q < NE; ++q) {
+        const double2 yq = yload(q);
+        tmp[q] = make_float2((float)yq.x, (float)yq.y);
+      }
+      res_write_ys<NP>(lds, tmp);
+      res_rhs<NP>(Hnh, D, C, nL, hasH, lds, k[0]);
+      ++nrhs;
+      need_k0 = false;
+    }
+    double h = fmin(dt, t_end - t);
+    bool made_jump = false;
+    double t_new = (h >= t_end - t) ? t_end : t + h;
+    if (pwc) {
+      const int m = pwc_segment(ptimes, nseg, t) + 1;
+      if (m < nseg && ptimes[m] < t_end && t + h >= ptimes[m]) {
+        h = ptimes[m] - t;
+        t_new = ptimes[m];
+        made_jump = true;
+      }
+    }
+    if (!(h > 0.0) || !(h > 1e-14 * fmax(1.0, fabs(t)))) {
+      status = DQH_NONFINITE;
+      break;
+    }
+    // stages 1..6; the stage loop is a runtime loop (one copy of the GEMM code), the
+    // register-resident stages are selected with compile-time indices
+#pragma unroll 1
+    for (int s = 1; s < 7; ++s) {
+      double c[6];
+#pragma unroll
+      for (int j = 0; j < 6; ++j) c[j] = (j < s) ? h * kA[s][j] : 0.0;
+#pragma unroll
+      for (int q = 0; q < NE; ++q) {
+        const double2 yq = yload(q);
+        double re = yq.x, im = yq.y;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+          re += c[j] * k[j][q].x;
+          im += c[j] * k[j][q].y;
+        }
+        tmp[q] = make_float2((float)re, (float)im);
+      }
+      res_write_ys<NP>(lds, tmp);
+      res_rhs<NP>(Hnh, D, C, nL, hasH, lds, tmp);
+      ++nrhs;
+#pragma unroll
+      for (int j = 1; j < 7; ++j) {
+        if (j == s) {
+#pragma unroll
+          for (int q = 0; q < NE; ++q) k[j][q] = tmp[q];
+        }
+      }
+    }
+    // error norm in FP64 with the candidate y1 = y + h sum b_j k_j
+    double ssum = 0;
+#pragma unroll
+    for (int q = 0; q < NE; ++q) {
+      const double2 yq = yload(q);
+      double er = 0, ei = 0, yr = yq.x, yi = yq.y;
+#pragma unroll
+      for (int j = 0; j < 7; ++j) {
+        er += kE[j] * k[j][q].x;
+        ei += kE[j] * k[j][q].y;
+        yr += h * kA[6][j] * k[j][q].x;
+        yi += h * kA[6][j] * k[j][q].y;
+      }
+      er *= h;
+      ei *= h;
+      const double sc = atol + rtol * fmax(sqrt(yq.x * yq.x + yq.y * yq.y), sqrt(yr * yr + yi * yi));
+      ssum += (er * er + ei * ei) / (sc * sc);
+    }
+    const double norm = sqrt(block_sum<kWaves>(ssum, sh) * inv_nn);
+    const bool finite = isfinite(norm);
+    const bool keep = finite && norm < 1.0;
+    double factor;
+    if (!finite)
+      factor = p.factormin;
+    else if (norm == 0.0)
+      factor = p.factormax;
+    else
+      factor = p.safety * ctrl_pow(norm, -0.2);
+    factor = fmin(fmax(factor, keep ? 1.0 : p.factormin), p.factormax);
+    ++nsteps;
+    if (keep) {
+      ++nacc;
+      while (isave < ntsave && tsave[isave] <= t_new) {
+        const double th = (tsave[isave] - t) / h;
+        double w[7];
+#pragma unroll
+        for (int j = 0; j < 7; ++j) w[j] = th * (kR[j][0] + th * (kR[j][1] + th * (kR[j][2] + th * kR[j][3])));
+        res_save<NP>(p, yg, k, h, w, true, isave, sh);
+        ++isave;
+      }
+#pragma unroll
+      for (int q = 0; q < NE; ++q) {
+        const int e = threadIdx.x + kThreads * q;
+        const double2 yq = yload(q);
+        double yr = yq.x, yi = yq.y;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+          yr += h * kA[6][j] * k[j][q].x;
+          yi += h * kA[6][j] * k[j][q].y;
+        }
+        if (e < R::NN) yg[e] = make_double2(yr, yi);
+        k[0][q] = k[6][q];
+      }
+      t = t_new;
+      need_k0 = made_jump;
+    } else {
+      ++nrej;
+    }
+    dt = factor * h;
+  }
+  while (isave < ntsave) {
+    res_save<NP>(p, yg, k, 0.0, w0, false, isave, sh);
+    ++isave;
+  }
+  float2 *fin = gp<float2>(p.final_state);
+  float2 *st = gp<float2>(p.states);
+  const bool save_states = (p.flags & DQH_FLAG_SAVE_STATES) != 0;
+#pragma unroll
+  for (int q = 0; q < NE; ++q) {
+    const int e = threadIdx.x + kThreads * q;
+    const int i = e / NP, j = e % NP;
+    if (e < R::NN && i < n && j < n) {
+      const double2 yq = yg[e];
+      const float2 v = make_float2((float)yq.x, (float)yq.y);
+      fin[(size_t)i * n + j] = v;
+      if (!save_states) st[(size_t)i * n + j] = v;
+    }
+  }
+  if (threadIdx.x == 0) {
+    stats[0] = nsteps;
+    stats[1] = nacc;
+    stats[2] = nrej;
+    stats[3] = status;
+    stats[4] = nrhs;
+  }
+}
+
+}  // namespace dqh
