@@ -67,6 +67,8 @@ SYMBOLS = {
     'dqh_lindblad_rhs_dense': (ctypes.c_int, [_P, _I64, _P]),
     'dqh_mesolve_tsit5_dense': (ctypes.c_int, [_P, _I64, _P]),
     'dqh_mesolve_rouchon1_dense': (ctypes.c_int, [_P, _I64, _P]),
+    'dqh_sizeof_dia_rouchon1': (_I64, []),
+    'dqh_mesolve_rouchon1_dia': (ctypes.c_int, [_P, _P]),
     'dqh_last_error': (ctypes.c_char_p, []),
     'dqh_device_info': (ctypes.c_int, [ctypes.c_char_p, _I64, ctypes.POINTER(_I64)]),
 }
@@ -90,6 +92,10 @@ def lib():
             fn.argtypes = args
         if handle.dqh_sizeof_me_inst() != ME_INST_DTYPE.itemsize:
             raise RuntimeError('dqh_me_inst layout mismatch between include/dqh.h and dynamiqs_amd/_native.py')
+        from .integrators.dia import DiaRouchon1
+
+        if handle.dqh_sizeof_dia_rouchon1() != ctypes.sizeof(DiaRouchon1):
+            raise RuntimeError('dqh_dia_rouchon1 layout mismatch between include/dqh.h and dynamiqs_amd/integrators/dia.py')
         _lib = handle
     return _lib
 

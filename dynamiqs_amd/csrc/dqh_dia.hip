@@ -1,0 +1,301 @@
+// Sparse-DIA Rouchon1 (gfx950): Kraus map with Cholesky normalisation for large
+// Hilbert spaces whose operators are stored by diagonals.
+//
+// One step (rouchon_integrator.py:250-263, 394-407, 552-582) is
+//   X = T^{-dag} rho T^{-1},   rho' = M0 X M0^dag + dt sum_k L_k X L_k^dag,
+// with S = M0^dag M0 + dt sum L^dag L = T T^dag banded (half-bandwidth w <= nb).
+// The state rho is kept in FP64 and every step adds an FP32 increment:
+//   K1  D1 = T^{-dag} rho - rho                          (banded left solve, MFMA)
+//   K2  E  = D1^dag + (T^{-dag} Z - Z),  Z = rho + D1^dag (same solve on Z)
+//       so that X = rho + E is the normalised state,
+//   K3  rho += E + dM X + X dM^dag + dM X dM^dag + sum_k A_k X A_k^dag
+//       (dM = M0 - I = -i dt Hnh, A_k = sqrt(dt) L_k; a DIA stencil sweep).
+// The banded solve T^dag x = s runs over blocks of nb rows from the bottom:
+//   x_i = V_i s_i - V_i U_{i,i+1} x_{i+1},  V_i = U_ii^{-1},  U = T^dag,
+// i.e. d_i = x_i - s_i = V'_i s_i + Wn_i x_{i+1} with V' = V - I, Wn = -V U_{i,i+1}
+// (nb x nb dense blocks precomputed in FP64 on the host). Columns are
+// independent, so one workgroup owns one panel of nb columns of one instance and
+// walks the block rows with its previous block kept in LDS.
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "dqh.h"
+#include "dqh_common.h"
+
+namespace dqh {
+namespace dia {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+__device__ __forceinline__ T *gp(uint64_t a) {
+  return (T *)((__attribute__((address_space(1))) T *)a);
+}
+
+struct Acc {
+  f4 re, im;
+};
+
+__device__ __forceinline__ void cmfma(Acc &a, float ar, float ai, float2 b) {
+  a.re = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, b.x, a.re, 0, 0, 0);
+  a.re = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai, b.y, a.re, 0, 0, 0);
+  a.im = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, b.y, a.im, 0, 0, 0);
+  a.im = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, b.x, a.im, 0, 0, 0);
+}
+
+constexpr int kNB = 64;          // block rows of the banded solve = panel width
+constexpr int kLD = kNB + 2;     // padded LDS row stride (float2)
+constexpr int kWaves = 8;
+constexpr int kThreads = kWaves * 64;
+
+// acc[t] += A[row0:+16, 0:64] (global, row-major lda=64) @ B[0:64, col0+16t:+16] (LDS, ld=kLD)
+__device__ __forceinline__ void mac_strip2(Acc (&acc)[2], const float2 *__restrict__ A, const float2 *B, int row0,
+                                           int col0, int lane) {
+  const int r = lane & 15, g = lane >> 4;
+  const float2 *Ap = A + (row0 + r) * kNB + 4 * g;
+  const float2 *Bp = B + (4 * g) * kLD + col0 + r;
+#pragma unroll
+  for (int k0 = 0; k0 < kNB; k0 += 16) {
+    const float4 a01 = *reinterpret_cast<const float4 *>(Ap + k0);
+    const float4 a23 = *reinterpret_cast<const float4 *>(Ap + k0 + 2);
+    const float2 *bp = Bp + k0 * kLD;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      cmfma(acc[t], a01.x, a01.y, bp[16 * t]);
+      cmfma(acc[t], a01.z, a01.w, bp[kLD + 16 * t]);
+      cmfma(acc[t], a23.x, a23.y, bp[2 * kLD + 16 * t]);
+      cmfma(acc[t], a23.z, a23.w, bp[3 * kLD + 16 * t]);
+    }
+  }
+}
+
+// Banded left solve of one column panel. mode 0: s = rho32, output D1 = x - s.
+// mode 1: s = rho32 + D1^dag, output E = D1^dag + (x - s).
+// grid: (n / kNB panels, ninst); block: kThreads.
+__global__ __launch_bounds__(kThreads) void k_lsolve(int mode, int64_t n, const float2 *__restrict__ Vp,
+                                                     const float2 *__restrict__ Wn, const float2 *__restrict__ rho32,
+                                                     const float2 *__restrict__ D1, float2 *__restrict__ out) {
+  __shared__ float2 s_lds[kNB * kLD];     // s_i panel
+  __shared__ float2 prev[kNB * kLD];      // x_{i+1} panel
+  __shared__ float2 tr[kNB * kLD];        // D1^dag tile (mode 1)
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int panel = blockIdx.x;
+  const size_t inst = blockIdx.y;
+  const size_t nn = (size_t)n * n;
+  const float2 *R = rho32 + inst * nn;
+  const float2 *Dt = D1 + inst * nn;
+  float2 *O = out + inst * nn;
+  const int c0 = panel * kNB;
+  const int NB = (int)(n / kNB);
+  // wave w owns output tiles (row tile w/2, col tiles 2*(w%2) and 2*(w%2)+1)
+  const int ti = wave >> 1, tj0 = (wave & 1) * 2;
+  for (int e = threadIdx.x; e < kNB * kLD; e += kThreads) prev[e] = make_float2(0.f, 0.f);
+  for (int bi = NB - 1; bi >= 0; --bi) {
+    const int r0 = bi * kNB;
+    // load s_i (64 x 64) and, in mode 1, the transposed conjugate tile of D1
+    for (int e = threadIdx.x; e < kNB * kNB; e += kThreads) {
+      const int i = e / kNB, j = e % kNB;
+      s_lds[i * kLD + j] = R[(size_t)(r0 + i) * n + c0 + j];
+      if (mode == 1) tr[i * kLD + j] = Dt[(size_t)(c0 + i) * n + r0 + j];  // D1[c0+i][r0+j]
+    }
+    __syncthreads();
+    if (mode == 1) {
+      for (int e = threadIdx.x; e < kNB * kNB; e += kThreads) {
+        const int i = e / kNB, j = e % kNB;
+        const float2 d = tr[j * kLD + i];  // D1[c0+j][r0+i] -> (D1^dag)[r0+i][c0+j] = conj
+        s_lds[i * kLD + j].x += d.x;
+        s_lds[i * kLD + j].y -= d.y;
+      }
+      __syncthreads();
+    }
+    Acc acc[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      acc[t].re = f4{0.f, 0.f, 0.f, 0.f};
+      acc[t].im = f4{0.f, 0.f, 0.f, 0.f};
+    }
+    mac_strip2(acc, Vp + (size_t)bi * kNB * kNB, s_lds, ti * 16, tj0 * 16, lane);
+    if (bi + 1 < NB) mac_strip2(acc, Wn + (size_t)bi * kNB * kNB, prev, ti * 16, tj0 * 16, lane);
+    __syncthreads();  // every wave is done reading prev
+    const int r = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = ti * 16 + 4 * g + v, j = (tj0 + t) * 16 + r;
+        const float2 d = make_float2(acc[t].re[v], acc[t].im[v]);
+        const float2 s = s_lds[i * kLD + j];
+        prev[i * kLD + j] = make_float2(s.x + d.x, s.y + d.y);
+        float2 o = d;
+        if (mode == 1) {
+          const float2 tt = tr[j * kLD + i];
+          o.x += tt.x;
+          o.y -= tt.y;
+        }
+        O[(size_t)(r0 + i) * n + c0 + j] = o;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Kraus increment + FP64 update + save. One thread per element.
+__global__ __launch_bounds__(256) void k_kraus(int64_t n, int64_t n_true, dqh_dia_ops ops,
+                                               const float2 *__restrict__ r32_in, const float2 *__restrict__ E,
+                                               double2 *__restrict__ r64, float2 *__restrict__ r32_out,
+                                               float2 *__restrict__ states, int64_t nsave, int64_t save_idx,
+                                               double theta) {
+  const size_t nn = (size_t)n * n;
+  const size_t inst = blockIdx.y;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)nn) return;
+  const int i = (int)(e / n), j = (int)(e % n);
+  const float2 *R = r32_in + inst * nn;
+  const float2 *Ei = E + inst * nn;
+  auto X = [&](int a, int b) -> float2 {
+    const size_t o = (size_t)a * n + b;
+    const float2 x = R[o], y = Ei[o];
+    return make_float2(x.x + y.x, x.y + y.y);
+  };
+  const float2 *dM = gp<const float2>(ops.diags[0]);
+  float ar = 0.f, ai = 0.f;
+  // one-sided terms dM X + X dM^dag
+  for (int u = 0; u < ops.noff[0]; ++u) {
+    const int p = ops.off[0][u];
+    const int a = i + p;
+    if (a >= 0 && a < n) {
+      const float2 m = dM[(size_t)u * n + a];  // dM[i][i+p]
+      const float2 x = X(a, j);
+      ar += m.x * x.x - m.y * x.y;
+      ai += m.x * x.y + m.y * x.x;
+    }
+    const int b = j + p;
+    if (b >= 0 && b < n) {
+      const float2 m = dM[(size_t)u * n + b];  // conj(dM[j][j+p])
+      const float2 x = X(i, b);
+      ar += x.x * m.x + x.y * m.y;
+      ai += x.y * m.x - x.x * m.y;
+    }
+  }
+  // sandwich terms A X A^dag (op 0 = dM, then sqrt(dt) L_k)
+  for (int s = 0; s < ops.nops; ++s) {
+    const float2 *A = gp<const float2>(ops.diags[s]);
+    const int no = ops.noff[s];
+    for (int u = 0; u < no; ++u) {
+      const int a = i + ops.off[s][u];
+      if (a < 0 || a >= n) continue;
+      const float2 ma = A[(size_t)u * n + a];
+      float zr = 0.f, zi = 0.f;  // (X A^dag)[a][j] = sum_v X[a][j+q_v] conj(A[j][j+q_v])
+      for (int v = 0; v < no; ++v) {
+        const int b = j + ops.off[s][v];
+        if (b < 0 || b >= n) continue;
+        const float2 mb = A[(size_t)v * n + b];
+        const float2 x = X(a, b);
+        zr += x.x * mb.x + x.y * mb.y;
+        zi += x.y * mb.x - x.x * mb.y;
+      }
+      ar += ma.x * zr - ma.y * zi;
+      ai += ma.x * zi + ma.y * zr;
+    }
+  }
+  const float2 eij = Ei[e];
+  const double dr = (double)eij.x + ar, di = (double)eij.y + ai;
+  double2 *y = r64 + inst * nn;
+  const double2 old = y[e];
+  const double2 nw = make_double2(old.x + dr, old.y + di);
+  y[e] = nw;
+  r32_out[inst * nn + e] = make_float2((float)nw.x, (float)nw.y);
+  if (save_idx >= 0 && i < n_true && j < n_true) {
+    const double sr = old.x + theta * dr, si = old.y + theta * di;
+    states[((inst * nsave + save_idx) * n_true + i) * n_true + j] = make_float2((float)sr, (float)si);
+  }
+}
+
+// states[inst][save_idx] <- rho (FP64 state) for saves at the start time
+__global__ __launch_bounds__(256) void k_save(int64_t n, int64_t n_true, const double2 *__restrict__ r64,
+                                              float2 *__restrict__ states, int64_t nsave, int64_t save_idx) {
+  const size_t nn = (size_t)n * n;
+  const size_t inst = blockIdx.y;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)nn) return;
+  const int i = (int)(e / n), j = (int)(e % n);
+  if (i < n_true && j < n_true) {
+    const double2 v = r64[inst * nn + e];
+    states[((inst * nsave + save_idx) * n_true + i) * n_true + j] = make_float2((float)v.x, (float)v.y);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_round(int64_t total, const double2 *__restrict__ r64, float2 *__restrict__ r32) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < total) r32[e] = make_float2((float)r64[e].x, (float)r64[e].y);
+}
+
+}  // namespace dia
+}  // namespace dqh
+
+using namespace dqh;
+
+extern "C" int64_t dqh_sizeof_dia_rouchon1(void) { return (int64_t)sizeof(dqh_dia_rouchon1); }
+
+extern "C" int dqh_mesolve_rouchon1_dia(const dqh_dia_rouchon1 *pb, void *stream) {
+  if (!pb) return fail(DQH_BAD_ARG, "dqh_mesolve_rouchon1_dia: null problem");
+  const dqh_dia_rouchon1 &p = *pb;
+  if (p.nb != dia::kNB || p.n <= 0 || p.n % dia::kNB != 0 || p.ninst <= 0 || p.n_true > p.n)
+    return fail(DQH_BAD_SHAPE, "dqh_mesolve_rouchon1_dia: n must be a positive multiple of 64 and nb == 64");
+  if (p.ops[0].nops < 1 || p.ops[0].nops > DQH_DIA_MAXOPS || p.nsets < 1 || p.nsets > 2)
+    return fail(DQH_BAD_ARG, "dqh_mesolve_rouchon1_dia: bad operator description");
+  for (int s = 0; s < p.nsets; ++s)
+    for (int k = 0; k < p.ops[s].nops; ++k)
+      if (p.ops[s].noff[k] < 1 || p.ops[s].noff[k] > DQH_DIA_MAXOFF)
+        return fail(DQH_BAD_ARG, "dqh_mesolve_rouchon1_dia: bad number of diagonals");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t n = p.n, nn = n * n;
+  float2 *r32[2] = {reinterpret_cast<float2 *>(p.rho32), reinterpret_cast<float2 *>(p.rho32) + p.ninst * nn};
+  double2 *r64 = reinterpret_cast<double2 *>(p.rho64);
+  float2 *D1 = reinterpret_cast<float2 *>(p.work);
+  float2 *E = D1 + p.ninst * nn;
+  float2 *states = reinterpret_cast<float2 *>(p.states);
+  const dim3 gel((unsigned)((nn + 255) / 256), (unsigned)p.ninst);
+  const dim3 gsolve((unsigned)(n / dia::kNB), (unsigned)p.ninst);
+  int cur = 0;
+  hipLaunchKernelGGL(dia::k_round, dim3((unsigned)((p.ninst * nn + 255) / 256)), dim3(256), 0, st, p.ninst * nn, r64,
+                     r32[0]);
+  const int64_t *hs_step = reinterpret_cast<const int64_t *>(p.save_step);
+  const double *hs_theta = reinterpret_cast<const double *>(p.save_theta);
+  int64_t isave = 0;
+  while (isave < p.nsave && hs_step[isave] < 0) {
+    hipLaunchKernelGGL(dia::k_save, gel, dim3(256), 0, st, n, p.n_true, r64, states, p.nsave, isave);
+    ++isave;
+  }
+  for (int64_t m = 0; m < p.nsteps; ++m) {
+    const int set = (m == p.nsteps - 1) ? p.nsets - 1 : 0;
+    const float2 *Vp = reinterpret_cast<const float2 *>(p.Vp[set]);
+    const float2 *Wn = reinterpret_cast<const float2 *>(p.Wn[set]);
+    hipLaunchKernelGGL(dia::k_lsolve, gsolve, dim3(dia::kThreads), 0, st, 0, n, Vp, Wn, r32[cur], D1, D1);
+    hipLaunchKernelGGL(dia::k_lsolve, gsolve, dim3(dia::kThreads), 0, st, 1, n, Vp, Wn, r32[cur], D1, E);
+    // saves inside this step: the kernel writes one interpolated state per launch
+    int64_t sidx = -1;
+    double th = 0.0;
+    if (isave < p.nsave && hs_step[isave] == m) {
+      sidx = isave;
+      th = hs_theta[isave];
+      ++isave;
+    }
+    hipLaunchKernelGGL(dia::k_kraus, gel, dim3(256), 0, st, n, p.n_true, p.ops[set], r32[cur], E, r64, r32[1 - cur],
+                       states, p.nsave, sidx, th);
+    // further saves in the same step (several save times inside one step): state at step end
+    while (isave < p.nsave && hs_step[isave] == m) {
+      hipLaunchKernelGGL(dia::k_save, gel, dim3(256), 0, st, n, p.n_true, r64, states, p.nsave, isave);
+      ++isave;
+    }
+    cur = 1 - cur;
+  }
+  while (isave < p.nsave) {
+    hipLaunchKernelGGL(dia::k_save, gel, dim3(256), 0, st, n, p.n_true, r64, states, p.nsave, isave);
+    ++isave;
+  }
+  const hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return fail(DQH_HIP_ERROR, std::string("dqh_mesolve_rouchon1_dia: ") + hipGetErrorString(err));
+  return DQH_OK;
+}

@@ -1,0 +1,214 @@
+"""Rouchon1 for sparse-DIA operators on the MI355X (``dqh_mesolve_rouchon1_dia``).
+
+Host part of the large-``n`` path: the normalisation ``S = M0^dag M0 + dt sum
+L^dag L = T T^dag`` (``rouchon_integrator.py:259-263, 573``) is factorised once in
+FP64 with a banded Cholesky, and the blocks of the banded solve ``T^dag x = s``
+(``:578-582``) are handed to the kernels as dense ``64 x 64`` matrices:
+``V_i - I`` with ``V_i = (T_ii^dag)^-1`` and ``Wn_i = -V_i T_{i+1,i}^dag``. The
+Kraus operators go to the device by diagonals: ``dM = M0 - I = -i dt Hnh`` and
+``A_k = sqrt(dt) L_k`` (``:250-257``).
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import scipy.linalg as sla
+
+from .. import _native
+from .dense import _Arena, _device, rouchon_step_times
+
+NB = 64
+MAXOPS = 8
+MAXOFF = 16
+
+
+class DiaOps(ctypes.Structure):
+    _fields_ = [
+        ('nops', ctypes.c_int32),
+        ('noff', ctypes.c_int32 * MAXOPS),
+        ('off', (ctypes.c_int32 * MAXOFF) * MAXOPS),
+        ('pad_', ctypes.c_int32),
+        ('diags', ctypes.c_uint64 * MAXOPS),
+    ]
+
+
+class DiaRouchon1(ctypes.Structure):
+    _fields_ = [
+        ('n', ctypes.c_int64),
+        ('n_true', ctypes.c_int64),
+        ('nb', ctypes.c_int64),
+        ('ninst', ctypes.c_int64),
+        ('nsteps', ctypes.c_int64),
+        ('nsave', ctypes.c_int64),
+        ('nsets', ctypes.c_int64),
+        ('pad_', ctypes.c_int64),
+        ('Vp', ctypes.c_uint64 * 2),
+        ('Wn', ctypes.c_uint64 * 2),
+        ('ops', DiaOps * 2),
+        ('rho64', ctypes.c_uint64),
+        ('rho32', ctypes.c_uint64),
+        ('work', ctypes.c_uint64),
+        ('states', ctypes.c_uint64),
+        ('save_step', ctypes.c_uint64),
+        ('save_theta', ctypes.c_uint64),
+    ]
+
+
+def _dia_of(x: np.ndarray, tol: float = 0.0):
+    n = x.shape[-1]
+    offs = [k for k in range(-(n - 1), n) if np.max(np.abs(np.diagonal(x, offset=k)), initial=0.0) > tol]
+    d = np.zeros((len(offs), n), dtype=np.complex128)
+    for i, k in enumerate(offs):
+        if k >= 0:
+            d[i, k:] = np.diagonal(x, offset=k)
+        else:
+            d[i, : n + k] = np.diagonal(x, offset=k)
+    return offs, d
+
+
+def half_bandwidth(S: np.ndarray) -> int:
+    offs, _ = _dia_of(S)
+    return max(abs(o) for o in offs) if offs else 0
+
+
+def block_solve_operators(H: np.ndarray, Ls, dt: float, n_pad: int):
+    """FP64 host precomputation of one step length: (Vp, Wn, dM diagonals, A_k diagonals)."""
+    n = H.shape[-1]
+    LdL = sum((L.conj().T @ L for L in Ls), np.zeros_like(H))
+    Hnh = H - 0.5j * LdL
+    M0 = np.eye(n) - 1j * dt * Hnh
+    S = M0.conj().T @ M0 + dt * LdL
+    w = half_bandwidth(S)
+    if w > NB:
+        raise NotImplementedError(f'DIA Rouchon1: half-bandwidth {w} of the normalisation exceeds {NB}')
+    Sp = np.eye(n_pad, dtype=np.complex128)
+    Sp[:n, :n] = S
+    ab = np.zeros((w + 1, n_pad), dtype=np.complex128)
+    for k in range(w + 1):
+        ab[k, : n_pad - k] = np.diagonal(Sp, offset=-k)
+    Tab = sla.cholesky_banded(ab, lower=True)  # T[j + k, j] = Tab[k, j]
+    T = np.zeros((n_pad, n_pad), dtype=np.complex128)
+    for k in range(w + 1):
+        idx = np.arange(n_pad - k)
+        T[idx + k, idx] = Tab[k, : n_pad - k]
+    nblk = n_pad // NB
+    Vp = np.zeros((nblk, NB, NB), dtype=np.complex128)
+    Wn = np.zeros((nblk, NB, NB), dtype=np.complex128)
+    for i in range(nblk):
+        s = slice(i * NB, (i + 1) * NB)
+        V = sla.solve_triangular(T[s, s].conj().T, np.eye(NB), lower=False)
+        Vp[i] = V - np.eye(NB)
+        if i + 1 < nblk:
+            s1 = slice((i + 1) * NB, (i + 2) * NB)
+            Wn[i] = -V @ T[s1, s].conj().T
+    dM_off, dM_d = _dia_of(-1j * dt * Hnh)
+    ops = [(dM_off, dM_d)]
+    for L in Ls:
+        ops.append(_dia_of(np.sqrt(dt) * L))
+    return Vp, Wn, ops
+
+
+def save_schedule(tsave: np.ndarray, step_t: np.ndarray):
+    steps, thetas = [], []
+    for ts in tsave:
+        if ts <= step_t[0]:
+            steps.append(-1)
+            thetas.append(0.0)
+            continue
+        m = int(np.searchsorted(step_t, ts, side='left')) - 1
+        m = min(max(m, 0), len(step_t) - 2)
+        h = step_t[m + 1] - step_t[m]
+        steps.append(m)
+        thetas.append(float(min(max((ts - step_t[m]) / h, 0.0), 1.0)))
+    return np.asarray(steps, dtype=np.int64), np.asarray(thetas, dtype=np.float64)
+
+
+class DiaPlan:
+    def __init__(self):
+        self.problem = None
+        self.arena = None
+        self.keep = []
+
+    def run(self, stream=None):
+        import torch
+
+        s = stream if stream is not None else torch.cuda.current_stream()
+        lib = _native.lib()
+        _native.check(lib.dqh_mesolve_rouchon1_dia(ctypes.byref(self.problem), ctypes.c_void_p(s.cuda_stream)), 'dqh_mesolve_rouchon1_dia')
+
+    def reset(self):
+        self.rho64.copy_(self.rho64_0)
+
+    def results(self) -> dict:
+        import torch
+
+        torch.cuda.synchronize()
+        st = self.states.cpu().numpy().view(np.complex64).reshape(self.ninst, self.nsave, self.n_true, self.n_true)
+        fin = self.rho64.cpu().numpy().view(np.complex128).reshape(self.ninst, self.n, self.n)[:, : self.n_true, : self.n_true]
+        return {'states': st, 'final_state': fin, 'nsteps': np.full(self.ninst, self.nsteps)}
+
+
+def build_plan(H: np.ndarray, Ls, rho0: np.ndarray, tsave: np.ndarray, dt: float, t0: float | None = None) -> DiaPlan:
+    """Pack a batch of Rouchon1 solves that share H and Ls (dense FP64 host arrays
+    whose nonzeros lie on few diagonals) and differ in rho0 (B, n, n)."""
+    import torch
+
+    dev = _device()
+    _native.lib()
+    tsave = np.asarray(tsave, dtype=np.float64)
+    rho0 = np.asarray(rho0, dtype=np.complex128)
+    if rho0.ndim == 2:
+        rho0 = rho0[None]
+    B, n = rho0.shape[0], H.shape[-1]
+    n_pad = NB * ((n + NB - 1) // NB)
+    t_start = float(tsave[0] if t0 is None else t0)
+    step_t = rouchon_step_times(t_start, float(tsave[-1]), float(dt))
+    hs = np.diff(step_t)
+    nsteps = len(hs)
+    sets = [float(hs[0])]
+    if abs(hs[-1] - hs[0]) > 1e-12 * max(1.0, abs(hs[0])):
+        sets.append(float(hs[-1]))
+    arena = _Arena(dev)
+    prob = DiaRouchon1()
+    prob.n, prob.n_true, prob.nb, prob.ninst = n_pad, n, NB, B
+    prob.nsteps, prob.nsets = nsteps, len(sets)
+    for si, h in enumerate(sets):
+        Vp, Wn, ops = block_solve_operators(H, Ls, h, n_pad)
+        prob.Vp[si] = arena.put(Vp.astype(np.complex64))
+        prob.Wn[si] = arena.put(Wn.astype(np.complex64))
+        if len(ops) > MAXOPS:
+            raise NotImplementedError(f'DIA Rouchon1 supports at most {MAXOPS - 1} jump operators')
+        o = prob.ops[si]
+        o.nops = len(ops)
+        for k, (offs, d) in enumerate(ops):
+            if len(offs) > MAXOFF:
+                raise NotImplementedError(f'DIA Rouchon1 supports at most {MAXOFF} diagonals per operator')
+            o.noff[k] = len(offs)
+            for u, off in enumerate(offs):
+                o.off[k][u] = off
+            dp = np.zeros((len(offs), n_pad), dtype=np.complex64)
+            dp[:, :n] = d
+            o.diags[k] = arena.put(dp)
+    r64 = np.zeros((B, n_pad, n_pad), dtype=np.complex128)
+    r64[:, :n, :n] = rho0
+    rho64_0 = torch.from_numpy(r64.view(np.float64)).to(dev)
+    rho64 = rho64_0.clone()
+    rho32 = torch.empty((2 * B * n_pad * n_pad * 2,), dtype=torch.float32, device=dev)
+    work = torch.empty((2 * B * n_pad * n_pad * 2,), dtype=torch.float32, device=dev)
+    steps_h, thetas_h = save_schedule(tsave, step_t)
+    states = torch.zeros((B * len(tsave) * n * n * 2,), dtype=torch.float32, device=dev)
+    prob.rho64, prob.rho32, prob.work, prob.states = rho64.data_ptr(), rho32.data_ptr(), work.data_ptr(), states.data_ptr()
+    prob.nsave = len(tsave)
+    prob.save_step = steps_h.ctypes.data
+    prob.save_theta = thetas_h.ctypes.data
+    plan = DiaPlan()
+    plan.problem, plan.arena = prob, arena
+    plan.keep = [steps_h, thetas_h, rho32, work]
+    plan.rho64, plan.rho64_0, plan.states = rho64, rho64_0, states
+    plan.ninst, plan.n, plan.n_true, plan.nsave, plan.nsteps = B, n_pad, n, len(tsave), nsteps
+    # roofline units: the Kraus sweep reads rho and writes rho' (2 n^2 complex64) per step and instance
+    plan.sweep_bytes_per_step = 2.0 * n * n * 8 * B
+    plan.solve_flops_per_step = 2 * 2 * 8.0 * NB * NB * n_pad * (n_pad // NB) * B
+    return plan

@@ -93,6 +93,55 @@ int dqh_mesolve_tsit5_dense(const dqh_me_inst *d_insts, int64_t ninst, void *str
  * tmp + tmp^dag with tmp = sum_k D_k rho C_k to the FP64 state. */
 int dqh_mesolve_rouchon1_dense(const dqh_me_inst *d_insts, int64_t ninst, void *stream);
 
+/* ------------------------------------------------------------------ sparse DIA */
+#define DQH_DIA_MAXOPS 8
+#define DQH_DIA_MAXOFF 16
+
+/* Operators of the Kraus sweep as diagonals. Operator 0 is dM = M0 - I = -i dt Hnh
+ * (used one-sided, dM X + X dM^dag, and as a sandwich dM X dM^dag); operators
+ * 1.. are sqrt(dt) L_k. diags[s] points to noff[s] device rows of n complex64 with
+ * A[i][i + off] stored at column i + off (scipy.sparse.dia_matrix convention,
+ * dynamiqs/qarrays/sparsedia_primitives.py:11-16, 232-261). */
+typedef struct dqh_dia_ops {
+  int32_t nops;
+  int32_t noff[DQH_DIA_MAXOPS];
+  int32_t off[DQH_DIA_MAXOPS][DQH_DIA_MAXOFF];
+  int32_t pad_;
+  uint64_t diags[DQH_DIA_MAXOPS];
+} dqh_dia_ops;
+
+/* A batch of Rouchon1 solves sharing their (constant, DIA) operators and differing
+ * in rho0. Set 0 is used for every step but the last, set nsets-1 for the last one
+ * (a shorter final step has its own normalisation). */
+typedef struct dqh_dia_rouchon1 {
+  int64_t n;       /* padded dimension, multiple of nb */
+  int64_t n_true;  /* Hilbert-space dimension */
+  int64_t nb;      /* block size of the banded solve (64; half-bandwidth of S <= nb) */
+  int64_t ninst;
+  int64_t nsteps;
+  int64_t nsave;
+  int64_t nsets;   /* 1 or 2 */
+  int64_t pad_;
+  uint64_t Vp[2];  /* device c64[n/nb][nb][nb]: V_i - I, V_i = (T_ii^dag)^-1 */
+  uint64_t Wn[2];  /* device c64[n/nb][nb][nb]: -V_i T_{i+1,i}^dag */
+  dqh_dia_ops ops[2];
+  uint64_t rho64;      /* device double2[ninst][n][n]: rho0 on entry, final state on exit */
+  uint64_t rho32;      /* device c64[2][ninst][n][n] scratch */
+  uint64_t work;       /* device c64[2][ninst][n][n] scratch */
+  uint64_t states;     /* device c64[ninst][nsave][n_true][n_true] */
+  uint64_t save_step;  /* HOST int64[nsave]: step index whose end state is saved (-1: initial) */
+  uint64_t save_theta; /* HOST double[nsave]: interpolation weight inside that step */
+} dqh_dia_rouchon1;
+
+int64_t dqh_sizeof_dia_rouchon1(void);
+
+/* Fixed-step Rouchon1 with Cholesky normalisation for DIA operators
+ * (replaces MESolveFixedRouchon1Integrator, rouchon_integrator.py:385-432, for
+ * large sparse problems): per step two banded block solves on MFMA and one
+ * bandwidth-bound stencil sweep, FP64 state with FP32 increments. Launches on
+ * ``stream`` and returns without synchronising. */
+int dqh_mesolve_rouchon1_dia(const dqh_dia_rouchon1 *problem, void *stream);
+
 /* Message of the last failing call on this thread. */
 const char *dqh_last_error(void);
 

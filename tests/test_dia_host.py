@@ -1,0 +1,103 @@
+"""CPU checks of the host precomputation of the DIA Rouchon1 path: a NumPy
+emulation of the three kernels (two banded block solves and the Kraus stencil
+sweep, increment form) must reproduce the oracle's dense Rouchon1 step
+(rouchon_integrator.py:226-269, 552-582)."""
+
+import numpy as np
+import pytest
+
+from dynamiqs_amd.integrators import dia
+from oracle import operators as op
+from oracle import rouchon
+
+
+def _two_mode(na, nb_):
+    a, b = op.destroy(na, nb_)
+    ad, bd = a.conj().T, b.conj().T
+    H = 5 * ad @ a + 6 * bd @ b + 0.1 * (ad @ b + a @ bd) - 0.01 * ad @ a @ bd @ b
+    Ls = [np.sqrt(0.02) * a, np.sqrt(0.05) * b]
+    return H, Ls
+
+
+def _emulate_step(rho, Vp, Wn, ops, n_pad):
+    """Increment-form step exactly as k_lsolve (x2) and k_kraus compute it."""
+    NB = dia.NB
+    nblk = n_pad // NB
+
+    def lsolve(s):
+        d = np.zeros_like(s)
+        prev = np.zeros((NB, n_pad), dtype=complex)
+        for i in reversed(range(nblk)):
+            r = slice(i * NB, (i + 1) * NB)
+            di = Vp[i] @ s[r]
+            if i + 1 < nblk:
+                di = di + Wn[i] @ prev
+            d[r] = di
+            prev = s[r] + di
+        return d
+
+    D1 = lsolve(rho)
+    Z = rho + D1.conj().T
+    E = D1.conj().T + lsolve(Z)
+    X = rho + E
+    n = n_pad
+
+    def dense(offs, d):
+        out = np.zeros((n, n), dtype=complex)
+        for u, o in enumerate(offs):
+            for c in range(n):
+                r = c - o
+                if 0 <= r < n:
+                    out[r, c] = d[u, c]
+        return out
+
+    dM = dense(*ops[0])
+    inc = E + dM @ X + X @ dM.conj().T
+    for offs, d in ops:
+        A = dense(offs, d)
+        inc = inc + A @ X @ A.conj().T
+    return rho + inc
+
+
+@pytest.mark.parametrize('na,nb_', [(4, 5), (8, 9), (12, 12)])
+def test_block_solve_emulation_matches_oracle_step(na, nb_):
+    H, Ls = _two_mode(na, nb_)
+    n = na * nb_
+    n_pad = dia.NB * ((n + dia.NB - 1) // dia.NB)
+    dt = 1e-3
+    rng = np.random.default_rng(0)
+    psi = rng.normal(size=(n, 1)) + 1j * rng.normal(size=(n, 1))
+    rho = psi @ psi.conj().T / np.vdot(psi, psi)
+    Vp, Wn, ops = dia.block_solve_operators(H, Ls, dt, n_pad)
+    ops_p = []
+    for offs, d in ops:
+        dp = np.zeros((len(offs), n_pad), dtype=complex)
+        dp[:, :n] = d
+        ops_p.append((offs, dp))
+    rp = np.zeros((n_pad, n_pad), dtype=complex)
+    rp[:n, :n] = rho
+    got = _emulate_step(rp, Vp, Wn, ops_p, n_pad)
+    want = rouchon.kraus_step_dense(H, Ls, rho, dt)
+    np.testing.assert_allclose(got[:n, :n], want, atol=1e-12)
+    assert np.abs(got[n:, :]).max() < 1e-14 and np.abs(got[:, n:]).max() < 1e-14
+
+
+def test_c3_operator_structure():
+    # the 32 x 32 two-mode config: H has offsets {-31, 0, 31}, L_a {+32}, L_b {+1}, w = 62
+    a, b = op.destroy(32, 32)
+    ad, bd = a.conj().T, b.conj().T
+    H = 5 * ad @ a + 6 * bd @ b + 0.1 * (ad @ b + a @ bd) - 0.01 * ad @ a @ bd @ b
+    assert dia._dia_of(H)[0] == [-31, 0, 31]
+    assert dia._dia_of(a)[0] == [32] and dia._dia_of(b)[0] == [1]
+    Ls = [np.sqrt(0.02) * a, np.sqrt(0.05) * b]
+    LdL = sum(L.conj().T @ L for L in Ls)
+    M0 = np.eye(1024) - 1e-3j * (H - 0.5j * LdL)
+    S = M0.conj().T @ M0 + 1e-3 * LdL
+    assert dia.half_bandwidth(S) == 62
+
+
+def test_save_schedule():
+    step_t = np.linspace(0, 1, 11)
+    steps, th = dia.save_schedule(np.array([0.0, 0.1, 0.25, 1.0]), step_t)
+    assert list(steps) == [-1, 0, 2, 9]
+    np.testing.assert_allclose(th, [0.0, 1.0, 0.5, 1.0])
