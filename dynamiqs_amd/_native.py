@@ -69,6 +69,7 @@ SYMBOLS = {
     'dqh_mesolve_rouchon1_dense': (ctypes.c_int, [_P, _I64, _P]),
     'dqh_sizeof_dia_rouchon1': (_I64, []),
     'dqh_mesolve_rouchon1_dia': (ctypes.c_int, [_P, _P]),
+    'dqh_adjoint_pwc_quadrature': (ctypes.c_int, [_I64, _I64, _I64, _I64, _I64, _P, _P, _P, _P, _P, _P, _P]),
     'dqh_last_error': (ctypes.c_char_p, []),
     'dqh_device_info': (ctypes.c_int, [ctypes.c_char_p, _I64, ctypes.POINTER(_I64)]),
 }

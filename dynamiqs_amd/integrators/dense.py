@@ -49,6 +49,9 @@ class Instance:
     key_H: object = None  # sharing keys: equal keys share device copies
     key_L: object = None
     key_rho: object = None
+    Hnh: np.ndarray | None = None  # explicit constant generator part (overrides -iH - 1/2 sum L^dag L)
+    tsave: np.ndarray | None = None  # per-instance save times (all instances of a plan: same length)
+    t0: float | None = None  # per-instance start time
 
 
 class _Arena:
@@ -216,6 +219,10 @@ def build_plan(
     nE = len(Es)
     t_start = float(tsave[0] if t0 is None else t0)
     t_end = float(tsave[-1])
+    per_inst_ts = instances[0].tsave is not None
+    if per_inst_ts:
+        if kind != 'tsit5' or any(inst.tsave is None or len(inst.tsave) != len(tsave) for inst in instances):
+            raise ValueError('per-instance save times need kind="tsit5" and equal lengths')
     plan = DensePlan(kind, n, len(tsave), nE, B, save_states)
     arena = _Arena(dev)
     plan.arena = arena
@@ -260,6 +267,11 @@ def build_plan(
         d['max_steps'] = max_steps
         d['t0'], d['t1'] = t_start, t_end
         d['tsave'] = p_tsave
+        if per_inst_ts:
+            ts_i = np.asarray(inst.tsave, dtype=np.float64)
+            d['tsave'] = arena.put(ts_i)
+            d['t0'] = float(ts_i[0] if inst.t0 is None else inst.t0)
+            d['t1'] = float(ts_i[-1])
         d['EopsT'] = p_E
         d['flags'] = _native.FLAG_SAVE_STATES if save_states else 0
         if kind in ('tsit5', 'rhs'):
@@ -268,6 +280,8 @@ def build_plan(
             d['flags'] |= _native.FLAG_HAS_HNH
 
             def bH(inst=inst):
+                if inst.Hnh is not None:
+                    return _pad(np.asarray(inst.Hnh), npad).astype(np.complex64)
                 LdL = sum((_dag(L) @ L for L in inst.Ls), np.zeros_like(inst.H))
                 return _pad(-1j * inst.H - 0.5 * LdL, npad).astype(np.complex64)
 

@@ -142,6 +142,18 @@ int64_t dqh_sizeof_dia_rouchon1(void);
  * ``stream`` and returns without synchronising. */
 int dqh_mesolve_rouchon1_dia(const dqh_dia_rouchon1 *problem, void *stream);
 
+/* ------------------------------------------------------------------ gradients */
+/* Gauss-Legendre quadrature of the adjoint gradient of piecewise-constant controls
+ * (replaces the reverse-mode path jax.grad -> dx.RecursiveCheckpointAdjoint,
+ * diffrax_integrator.py:88-99): for every instance and segment k,
+ *   grad[inst][j][k] = sum_{q in segment k} weights[q] Re Tr(lam_q (-i [O_j, rho_q])),
+ * rho_nodes / lam_nodes: device complex64 [ninst][nnodes][n][n], ops: device
+ * complex64 [nterms][n][n] (nterms <= 8), weights: device double[nnodes],
+ * seg_first: device int32[nseg + 1], grad: device double[ninst][nterms][nseg]. */
+int dqh_adjoint_pwc_quadrature(int64_t ninst, int64_t n, int64_t nnodes, int64_t nterms, int64_t nseg,
+                               const void *rho_nodes, const void *lam_nodes, const void *ops,
+                               const double *weights, const int32_t *seg_first, double *grad, void *stream);
+
 /* Message of the last failing call on this thread. */
 const char *dqh_last_error(void);
 
