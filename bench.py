@@ -188,9 +188,137 @@ def bench_dia(args):
     return 0
 
 
+def serving_requests(N: int, seed: int = 2):
+    """Random serving mix: n log-uniform in {8..256}, GUE H scaled to spectral norm 1,
+    1-3 jump ops sqrt(gamma) G (G complex Gaussian, entry variance 1/n, gamma ~ U[0.01, 0.5]),
+    random pure rho0, 50 save times on [0, T], T ~ U[1, 20], Tsit5 rtol = 1e-5."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for _ in range(N):
+        n = int(round(np.exp(rng.uniform(np.log(8), np.log(256)))))
+        A = (rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))) / np.sqrt(2)
+        H = (A + A.conj().T) / 2
+        H /= np.max(np.abs(np.linalg.eigvalsh(H)))
+        K = int(rng.integers(1, 4))
+        Ls = [np.sqrt(rng.uniform(0.01, 0.5)) * (rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))) / np.sqrt(2 * n) for _ in range(K)]
+        psi = rng.normal(size=(n, 1)) + 1j * rng.normal(size=(n, 1))
+        psi /= np.linalg.norm(psi)
+        T = rng.uniform(1.0, 20.0)
+        reqs.append((H, Ls, psi, np.linspace(0.0, T, 50)))
+    return reqs
+
+
+def bench_serving(args):
+    """Poisson arrivals into the continuous-batching server at several rates."""
+    import threading
+
+    from dynamiqs_amd.serving import Server
+
+    reqs = serving_requests(args.requests)
+    ns = np.array([r[0].shape[0] for r in reqs])
+    rows = []
+    for rate in args.rates:
+        rng = np.random.default_rng(3)
+        gaps = rng.exponential(1.0 / rate, size=len(reqs)) if rate > 0 else np.zeros(len(reqs))
+        with Server(memory_budget=int(args.budget_gb * (1 << 30))) as srv:
+            futs = [None] * len(reqs)
+
+            def client():
+                t_next = time.perf_counter()
+                for i, (H, Ls, psi, ts) in enumerate(reqs):
+                    t_next += gaps[i]
+                    dt = t_next - time.perf_counter()
+                    if dt > 0:
+                        time.sleep(dt)
+                    futs[i] = srv.submit(H, Ls, psi, ts, rtol=1e-5)
+
+            t0 = time.perf_counter()
+            th = threading.Thread(target=client)
+            th.start()
+            th.join()
+            res = [f.result() for f in futs]
+            wall = time.perf_counter() - t0
+            st = dict(srv.stats)
+        lat = np.array([r.latency_s for r in res])
+        rows.append({
+            'offered_rate': rate if rate > 0 else 'all-at-once',
+            'solves_per_s': len(res) / wall,
+            'p50_latency_s': float(np.percentile(lat, 50)),
+            'p99_latency_s': float(np.percentile(lat, 99)),
+            'launches': st['launches'],
+            'max_concurrent_launches': st['max_concurrent_launches'],
+            'mean_steps': float(np.mean([r.nsteps for r in res])),
+        })
+        print(json.dumps(rows[-1]), flush=True)
+    out = {
+        'metric': 'serving solves/s and p50/p99 latency',
+        'config': {'workload': f'serving mix, {len(reqs)} requests, n log-uniform 8..256 (median {int(np.median(ns))}), K 1-3, T U[1,20], 50 tsave, rtol 1e-5'},
+        'rates': rows,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_adjoint(args):
+    """Adjoint gradient vs forward cost: n = 32, H = a^dag a + u_k (a + a^dag) with 100 PWC
+    segments on [0, 10], u ~ N(0, 0.5), L = sqrt(0.05) a, rho0 = vacuum, loss Tr(a^dag a rho(T)),
+    B = 64 control vectors, checkpoints every 10 segments."""
+    import torch
+
+    from dynamiqs_amd.adjoint import mesolve_pwc_grad
+    from dynamiqs_amd.integrators import dense
+
+    n, nseg, B = 32, 100, 64
+    a = np.diag(np.sqrt(np.arange(1, n, dtype=np.float64)), k=1).astype(np.complex128)
+    H0 = a.conj().T @ a
+    X = a + a.conj().T
+    times = np.linspace(0.0, 10.0, nseg + 1)
+    u = np.random.default_rng(4).normal(0.0, 0.5, size=(B, 1, nseg))
+    Ls = [np.sqrt(0.05) * a]
+    rho0 = np.zeros((n, n), dtype=np.complex128)
+    rho0[0, 0] = 1
+    O = a.conj().T @ a
+    # forward alone (same solve the gradient starts with), timed as one launch
+    insts = [dense.Instance(H=H0, Ls=Ls, rho0=rho0, pwc=[(times, u[b, 0], X)], key_L=(), key_rho=()) for b in range(B)]
+    fwd = dense.build_plan('tsit5', insts, np.array([0.0, 10.0]), exp_ops=[O])
+    fwd.run()
+    torch.cuda.synchronize()
+    tf = []
+    for _ in range(max(args.steps, 1)):
+        fwd.reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fwd.run()
+        torch.cuda.synchronize()
+        tf.append(time.perf_counter() - t0)
+    mesolve_pwc_grad(H0, [X], times, u, Ls, rho0, O, checkpoint_every=10)  # warm
+    tg, tim = [], None
+    for _ in range(max(args.steps, 1)):
+        t0 = time.perf_counter()
+        J, g, tim = mesolve_pwc_grad(H0, [X], times, u, Ls, rho0, O, checkpoint_every=10, return_timing=True)
+        tg.append(time.perf_counter() - t0)
+    gpu_grad = tim['forward_run_s'] + tim['recompute_run_s'] + tim['backward_run_s'] + tim['quadrature_s']
+    out = {
+        'metric': 'adjoint gradient cost / forward solve',
+        'config': {'workload': 'PWC-driven oscillator n=32, 100 segments on [0,10], B=64, loss Tr(a^dag a rho(T)), checkpoints every 10 segments'},
+        'forward_s': float(np.median(tf)),
+        'gradient_wall_s': float(np.median(tg)),
+        'gradient_gpu_s': gpu_grad,
+        'gpu_ratio': gpu_grad / float(np.median(tf)),
+        'wall_ratio': float(np.median(tg)) / float(np.median(tf)),
+        'phases_s': tim,
+        'loss_mean': float(np.mean(J)),
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument('--config', default='kerr', choices=['kerr', 'dia'])
+    ap.add_argument('--config', default='kerr', choices=['kerr', 'dia', 'serving', 'adjoint'])
+    ap.add_argument('--requests', type=int, default=1000)
+    ap.add_argument('--rates', type=float, nargs='+', default=[100.0, 400.0, 0.0])
+    ap.add_argument('--budget-gb', type=float, default=32.0)
     ap.add_argument('--dia-T', type=float, default=5.0)
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=3)
@@ -209,6 +337,10 @@ def main(argv=None):
 
     if args.config == 'dia':
         return bench_dia(args)
+    if args.config == 'serving':
+        return bench_serving(args)
+    if args.config == 'adjoint':
+        return bench_adjoint(args)
 
     import torch
 

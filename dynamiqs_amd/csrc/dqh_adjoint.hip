@@ -21,54 +21,77 @@ namespace adj {
 constexpr int kThreads = 256;
 
 // rho, lam: [ninst][nnodes][n][n] complex64; ops: [nterms][n][n] complex64
-// weights: [nnodes] (already scaled by the segment length / 2); node_seg: [nnodes]
+// weights: [nnodes] (already scaled by the segment length / 2)
 // grad: [ninst][nterms][nseg] float64, nodes of one segment are contiguous
 // seg_first: [nseg + 1] first node of each segment
+//
+// With P = rho lam, Tr(O (P - P^dag)) = Tr(O P) - conj(Tr(O^dag P)) and
+// Tr(O P) = sum_cb (lam O)_cb rho_bc, Tr(O^dag P) = sum_cb (lam O^dag)_cb rho_bc.
+// For n <= 64 the node's lam, rho and the operator sit in LDS (FP32 operands,
+// FP64 accumulation); larger n read them from global memory.
+constexpr int kMaxLds = 64;
+
 __global__ __launch_bounds__(kThreads) void k_quad(int n, int nnodes, int nterms, int nseg,
                                                    const float2 *__restrict__ rho, const float2 *__restrict__ lam,
                                                    const float2 *__restrict__ ops, const double *__restrict__ weights,
                                                    const int *__restrict__ seg_first, double *__restrict__ grad) {
-  __shared__ double red[kThreads / 64][8];
+  __shared__ float2 sL[kMaxLds * (kMaxLds + 1)];
+  __shared__ float2 sR[kMaxLds * (kMaxLds + 1)];
+  __shared__ float2 sO[kMaxLds * (kMaxLds + 1)];
+  __shared__ double red[kThreads / 64][2];
   const int seg = blockIdx.x;
   const size_t inst = blockIdx.y;
   const size_t nn = (size_t)n * n;
-  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int q = seg_first[seg]; q < seg_first[seg + 1]; ++q) {
-    const float2 *R = rho + (inst * nnodes + q) * nn;
-    const float2 *Lm = lam + (inst * nnodes + q) * nn;
-    const double w = weights[q];
-    // element (b, a) of P - P^dag, P = R Lm, paired with O[a][b]
-    for (size_t e = threadIdx.x; e < nn; e += kThreads) {
-      const int b = (int)(e / n), a = (int)(e % n);
-      double pr = 0, pi = 0, qr = 0, qi = 0;  // P[b][a], P[a][b]
-      for (int m = 0; m < n; ++m) {
-        const float2 r1 = R[(size_t)b * n + m], l1 = Lm[(size_t)m * n + a];
-        pr += (double)r1.x * l1.x - (double)r1.y * l1.y;
-        pi += (double)r1.x * l1.y + (double)r1.y * l1.x;
-        const float2 r2 = R[(size_t)a * n + m], l2 = Lm[(size_t)m * n + b];
-        qr += (double)r2.x * l2.x - (double)r2.y * l2.y;
-        qi += (double)r2.x * l2.y + (double)r2.y * l2.x;
-      }
-      // (P - P^dag)[b][a] = P[b][a] - conj(P[a][b])
-      const double dr = pr - qr, di = pi + qi;
-      for (int j = 0; j < nterms && j < 8; ++j) {
-        const float2 o = ops[(size_t)j * nn + (size_t)a * n + b];
-        // Re(-i * o * d) = Im(o * d)
-        acc[j] += w * (o.x * di + o.y * dr);
-      }
-    }
-  }
+  const bool lds = n <= kMaxLds;
+  const int ld = kMaxLds + 1;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int j = 0; j < nterms && j < 8; ++j) {
-    double v = acc[j];
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    if (lane == 0) red[wave][j] = v;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int j = 0; j < nterms && j < 8; ++j) {
+  for (int j = 0; j < nterms; ++j) {
+    const float2 *Og = ops + (size_t)j * nn;
+    if (lds) {
+      __syncthreads();
+      for (size_t e = threadIdx.x; e < nn; e += kThreads) sO[(e / n) * ld + e % n] = Og[e];
+    }
+    double acc = 0.0;
+    for (int q = seg_first[seg]; q < seg_first[seg + 1]; ++q) {
+      const float2 *R = rho + (inst * nnodes + q) * nn;
+      const float2 *Lm = lam + (inst * nnodes + q) * nn;
+      if (lds) {
+        __syncthreads();
+        for (size_t e = threadIdx.x; e < nn; e += kThreads) {
+          sL[(e / n) * ld + e % n] = Lm[e];
+          sR[(e / n) * ld + e % n] = R[e];
+        }
+        __syncthreads();
+      }
+      double tr = 0.0, ti = 0.0, ur = 0.0, ui = 0.0;  // Tr(O P), Tr(O^dag P)
+      for (size_t e = threadIdx.x; e < nn; e += kThreads) {
+        const int c = (int)(e / n), b = (int)(e % n);
+        float wr = 0.f, wi = 0.f, vr = 0.f, vi = 0.f;  // (lam O)_cb, (lam O^dag)_cb
+        for (int a = 0; a < n; ++a) {
+          const float2 l = lds ? sL[c * ld + a] : Lm[(size_t)c * n + a];
+          const float2 o = lds ? sO[a * ld + b] : Og[(size_t)a * n + b];
+          const float2 od = lds ? sO[b * ld + a] : Og[(size_t)b * n + a];  // (O^dag)_ab = conj(O_ba)
+          wr += l.x * o.x - l.y * o.y;
+          wi += l.x * o.y + l.y * o.x;
+          vr += l.x * od.x + l.y * od.y;
+          vi += l.y * od.x - l.x * od.y;
+        }
+        const float2 r = lds ? sR[b * ld + c] : R[(size_t)b * n + c];
+        tr += (double)wr * r.x - (double)wi * r.y;
+        ti += (double)wr * r.y + (double)wi * r.x;
+        ur += (double)vr * r.x - (double)vi * r.y;
+        ui += (double)vr * r.y + (double)vi * r.x;
+      }
+      // T = Tr(O P) - conj(Tr(O^dag P));  Re(-i T) = Im(T) = ti + ui
+      acc += weights[q] * (ti + ui);
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    __syncthreads();
+    if (lane == 0) red[wave][0] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
       double s = 0;
-      for (int w = 0; w < kThreads / 64; ++w) s += red[w][j];
+      for (int w = 0; w < kThreads / 64; ++w) s += red[w][0];
       grad[(inst * nterms + j) * nseg + seg] = s;
     }
   }

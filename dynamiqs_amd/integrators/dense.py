@@ -55,13 +55,43 @@ class Instance:
 
 
 class _Arena:
-    """Host staging of device buffers; one torch tensor per buffer."""
+    """Device buffers of a plan. ``put`` stages host arrays into one pinned
+    buffer that ``upload`` copies to the device in a single transfer; it returns
+    ``offset + 1`` (0 stays the null pointer) and ``patch`` turns such values
+    into device addresses. ``put_now`` uploads immediately (one tensor each)."""
 
     def __init__(self, device):
         self.device = device
         self.tensors = []
+        self._chunks = []
+        self._size = 0
+        self.base = None
 
     def put(self, arr: np.ndarray) -> int:
+        a = np.ascontiguousarray(arr)
+        off = (self._size + 255) // 256 * 256
+        self._chunks.append((off, a.view(np.uint8).reshape(-1)))
+        self._size = off + a.nbytes
+        return off + 1
+
+    def upload(self) -> int:
+        import torch
+
+        host = torch.empty(max(self._size, 16), dtype=torch.uint8, pin_memory=True)
+        hn = host.numpy()
+        for off, b in self._chunks:
+            hn[off : off + b.size] = b
+        dev = host.to(self.device, non_blocking=True)
+        self.tensors.append(dev)
+        self.tensors.append(host)
+        self._chunks = []
+        self.base = dev.data_ptr()
+        return self.base
+
+    def patch(self, v: int) -> int:
+        return 0 if v == 0 else self.base + int(v) - 1
+
+    def put_now(self, arr: np.ndarray) -> int:
         import torch
 
         a = np.ascontiguousarray(arr)
@@ -80,6 +110,42 @@ class _Arena:
         t = (torch.zeros if zero else torch.empty)(max(int(nbytes), 16), dtype=torch.uint8, device=self.device)
         self.tensors.append(t)
         return t
+
+
+ARENA_FIELDS = ('tsave', 'Hnh', 'Dops', 'Cops', 'EopsT', 'Hpwc', 'pwc_times', 'pwc_vals', 'step_t', 'step_set')
+POINTER_FIELDS = ARENA_FIELDS + ('y', 'work', 'states', 'expects', 'final_state', 'stats')
+
+
+def check_descriptors(descs: np.ndarray, tensors) -> None:
+    """Host-side guard before a launch: every non-null pointer of every descriptor
+    must lie inside one of the device allocations of the plan, and the buffers
+    whose extent the descriptor fixes must end inside the same allocation."""
+    spans = sorted((t.data_ptr(), t.data_ptr() + t.numel() * t.element_size()) for t in tensors)
+    lo = np.array([s[0] for s in spans], dtype=np.uint64)
+    hi = np.array([s[1] for s in spans], dtype=np.uint64)
+    npd = descs['np'].astype(np.uint64)
+    n = descs['n'].astype(np.uint64)
+    nt = descs['ntsave'].astype(np.uint64)
+    nE = descs['nE'].astype(np.uint64)
+    one = np.uint64(1)
+    extent = {
+        'Hnh': npd * npd * np.uint64(8),
+        'EopsT': nE * npd * npd * np.uint64(8),
+        'y': np.uint64(2) * npd * npd * np.uint64(16),
+        'tsave': nt * np.uint64(8),
+        'final_state': n * n * np.uint64(8),
+        'expects': np.maximum(nE, one) * nt * np.uint64(8),
+        'stats': np.full(len(descs), 64, dtype=np.uint64),
+    }
+    for f in POINTER_FIELDS:
+        v = descs[f].astype(np.uint64)
+        nz = v != 0
+        idx = np.clip(np.searchsorted(lo, v, side='right') - 1, 0, len(hi) - 1)
+        end = v + extent.get(f, np.ones(len(descs), dtype=np.uint64))
+        ok = (v >= lo[idx]) & (end <= hi[idx])
+        if np.any(nz & ~ok):
+            bad = int(v[nz & ~ok][0])
+            raise RuntimeError(f'descriptor field {f} = {bad:#x} is outside the plan allocations')
 
 
 def _device():
@@ -333,6 +399,11 @@ def build_plan(
             d['step_set'] = arena.put(np.asarray([index[k] for k in keys], dtype=np.int32))
             d['max_steps'] = nsteps
 
+    # ---- one host-to-device transfer for every staged operator, then real addresses
+    arena.upload()
+    for f in ARENA_FIELDS:
+        descs[f] = [arena.patch(int(v)) for v in descs[f]]
+
     # ---- state, scratch, outputs
     welems = _native.work_elems(npad, nLdev, npwc_max)
     work = arena.empty(B * welems * 8)
@@ -362,6 +433,7 @@ def build_plan(
         d['final_state'] = final.data_ptr() + b * n * n * 8
         d['expects'] = expects.data_ptr() + b * max(nE, 1) * len(tsave) * 8
         d['stats'] = stats.data_ptr() + b * 64
+    check_descriptors(descs, [*arena.tensors, work, states, final, expects, stats])
     plan.descs = descs
     plan.d_descs = torch.from_numpy(descs.view(np.uint8).copy()).to(dev)
     plan.K = K

@@ -176,8 +176,8 @@ def build_plan(H: np.ndarray, Ls, rho0: np.ndarray, tsave: np.ndarray, dt: float
     prob.nsteps, prob.nsets = nsteps, len(sets)
     for si, h in enumerate(sets):
         Vp, Wn, ops = block_solve_operators(H, Ls, h, n_pad)
-        prob.Vp[si] = arena.put(Vp.astype(np.complex64))
-        prob.Wn[si] = arena.put(Wn.astype(np.complex64))
+        prob.Vp[si] = arena.put_now(Vp.astype(np.complex64))
+        prob.Wn[si] = arena.put_now(Wn.astype(np.complex64))
         if len(ops) > MAXOPS:
             raise NotImplementedError(f'DIA Rouchon1 supports at most {MAXOPS - 1} jump operators')
         o = prob.ops[si]
@@ -190,7 +190,7 @@ def build_plan(H: np.ndarray, Ls, rho0: np.ndarray, tsave: np.ndarray, dt: float
                 o.off[k][u] = off
             dp = np.zeros((len(offs), n_pad), dtype=np.complex64)
             dp[:, :n] = d
-            o.diags[k] = arena.put(dp)
+            o.diags[k] = arena.put_now(dp)
     r64 = np.zeros((B, n_pad, n_pad), dtype=np.complex128)
     r64[:, :n, :n] = rho0
     rho64_0 = torch.from_numpy(r64.view(np.float64)).to(dev)

@@ -87,11 +87,10 @@ def union_breakpoints(pwc_lists) -> np.ndarray:
 
 def resample_pwc(times: np.ndarray, values: np.ndarray, grid: np.ndarray) -> np.ndarray:
     """Coefficient of a PWC term on each interval ``[grid[i], grid[i+1])`` of a finer grid."""
-    out = np.zeros((*values.shape[:-1], len(grid) - 1), dtype=np.complex128)
-    for i in range(len(grid) - 1):
-        t = grid[i]
-        if t < times[0] or t >= times[-1]:
-            continue
-        idx = int(np.searchsorted(times, t, side='right')) - 1
-        out[..., i] = values[..., idx]
-    return out
+    times = np.asarray(times, dtype=np.float64)
+    t = np.asarray(grid[:-1], dtype=np.float64)
+    idx = np.searchsorted(times, t, side='right') - 1
+    inside = (t >= times[0]) & (t < times[-1])
+    idx = np.clip(idx, 0, values.shape[-1] - 1)
+    out = np.asarray(values, dtype=np.complex128)[..., idx]
+    return np.where(inside, out, 0.0)

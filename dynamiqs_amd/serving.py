@@ -194,6 +194,11 @@ def collect(oarena, outs) -> list:
     return res
 
 
+def _size_class(r) -> int:
+    npd = pad16(r.n)
+    return 0 if npd <= 32 else (1 if npd <= 64 else (2 if npd <= 128 else 3))
+
+
 class Server:
     """Continuous-batching request loop (see module docstring).
 
@@ -204,7 +209,7 @@ class Server:
         max_inflight: maximum concurrent launches.
     """
 
-    def __init__(self, memory_budget: int = 8 << 30, max_batch: int = 512, nstreams: int = 4, max_inflight: int = 8):
+    def __init__(self, memory_budget: int = 8 << 30, max_batch: int = 64, nstreams: int = 4, max_inflight: int = 8):
         import torch
 
         if not torch.cuda.is_available():
@@ -256,14 +261,21 @@ class Server:
     def _admit(self) -> None:
         if not self._pending or len(self._inflight) >= self.max_inflight:
             return
-        batch, nbytes = [], 0
-        while self._pending and len(batch) < self.max_batch:
-            r = self._pending[0]
+        # a launch lasts as long as its slowest request: fill it with requests of
+        # the same size class as the oldest pending one (FIFO inside the class)
+        cls = _size_class(self._pending[0])
+        batch, nbytes, keep = [], 0, []
+        for r in self._pending:
+            if len(batch) >= self.max_batch or _size_class(r) != cls:
+                keep.append(r)
+                continue
             fb = footprint_bytes(r.n, len(r.Ls), len(r.tsave), len(r.exp_ops))
             if self._used + nbytes + fb > self.memory_budget:
-                break
-            batch.append(self._pending.pop(0))
+                keep.append(r)
+                continue
+            batch.append(r)
             nbytes += fb
+        self._pending = keep
         if not batch:
             return
         stream = self.streams[self._next_stream % len(self.streams)]

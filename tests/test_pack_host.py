@@ -1,0 +1,58 @@
+"""CPU checks of the descriptor packing (``dynamiqs_amd.integrators.dense.build_plan``)
+with the device replaced by host memory: every pointer a kernel would dereference
+lies inside an allocation of the plan, staged operators land at their offsets, and
+the pre-launch guard rejects a descriptor that does not."""
+
+import numpy as np
+import pytest
+import torch
+
+from dynamiqs_amd.integrators import dense
+
+
+@pytest.fixture
+def host_device(monkeypatch):
+    monkeypatch.setattr(dense, '_device', lambda: torch.device('cpu'))
+    orig = torch.empty
+
+    def empty(*a, **k):
+        k.pop('pin_memory', None)
+        return orig(*a, **k)
+
+    monkeypatch.setattr(torch, 'empty', empty)
+
+
+def _inst(rng, n, K, pwc=False):
+    H = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    H = H + H.conj().T
+    rho = np.eye(n) / n
+    p = [(np.linspace(0, 1, 5), rng.normal(size=4), H)] if pwc else []
+    return dense.Instance(H=H, Ls=[0.1 * H] * K, rho0=rho, pwc=p)
+
+
+@pytest.mark.parametrize('kind,K,pwc', [('rhs', 0, False), ('tsit5', 2, False), ('tsit5', 1, True), ('rouchon1', 1, False)])
+def test_descriptors_point_into_plan_allocations(host_device, kind, K, pwc):
+    rng = np.random.default_rng(0)
+    n = 20
+    insts = [_inst(rng, n, K, pwc) for _ in range(3)]
+    plan = dense.build_plan(kind, insts, np.array([0.0, 0.5, 1.0]), dt=0.1 if kind == 'rouchon1' else None)
+    tensors = [*plan.arena.tensors, plan.states, plan.final, plan.expects, plan.stats]
+    dense.check_descriptors(plan.descs, tensors)
+    if kind in ('rhs', 'tsit5'):
+        # the staged Hnh of instance 1 holds -iH - 1/2 sum L^dag L
+        H, Ls = insts[1].H, insts[1].Ls
+        want = -1j * H - 0.5 * sum((L.conj().T @ L for L in Ls), np.zeros_like(H))
+        base = plan.arena.tensors[0].data_ptr()
+        off = int(plan.descs[1]['Hnh']) - base
+        host = plan.arena.tensors[0].numpy()
+        got = host[off : off + 32 * 32 * 8].view(np.complex64).reshape(32, 32)[:n, :n]
+        np.testing.assert_allclose(got, want, rtol=1e-6, atol=1e-5)
+
+
+def test_guard_rejects_unpatched_descriptor(host_device):
+    rng = np.random.default_rng(1)
+    plan = dense.build_plan('rhs', [_inst(rng, 8, 1)], np.array([0.0, 1.0]))
+    descs = plan.descs.copy()
+    descs['Hnh'] = 0x101  # a raw staging offset, never a device address
+    with pytest.raises(RuntimeError, match='outside the plan allocations'):
+        dense.check_descriptors(descs, [*plan.arena.tensors, plan.states, plan.final, plan.expects, plan.stats])
