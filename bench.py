@@ -151,7 +151,8 @@ def bench_dia(args):
         rho0.append(np.outer(v, v.conj()))
     rho0 = np.stack(rho0)
     tsave = np.linspace(0.0, args.dia_T, int(round(args.dia_T / 0.1)) + 1)
-    plan = dia.build_plan(H, Ls, rho0, tsave, 1e-3)
+    variant = {'auto': dia.SWEEP_AUTO, 'global': dia.SWEEP_GLOBAL, 'tiled': dia.SWEEP_TILED}[args.dia_variant]
+    plan = dia.build_plan(H, Ls, rho0, tsave, 1e-3, variant=variant)
     stream = torch.cuda.current_stream()
     for _ in range(args.warmup):
         plan.reset()
@@ -183,9 +184,30 @@ def bench_dia(args):
         'sweep_algorithmic_GBps': plan.sweep_bytes_per_step * plan.nsteps / (1e-3 * float(np.mean(kern))) / 1e9,
         'solve_TFLOPs': plan.solve_flops_per_step * plan.nsteps / (1e-3 * float(np.mean(kern))) / 1e12,
         'max_trace_error': float(tr),
+        'sweep_variant': args.dia_variant,
+        'rhs_sweep': dia_rhs_sweeps(H, Ls, rho0, stream),
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def dia_rhs_sweeps(H, Ls, rho0, stream, reps: int = 20):
+    """The DIA Lindblad vector field alone (dqh_lindblad_rhs_dia) on the same batch,
+    both sweep forms: mean kernel time and algorithmic GB/s (read rho + write drho)."""
+    from dynamiqs_amd.integrators import dia
+
+    out = {}
+    for name, v in (('tiled', dia.SWEEP_TILED), ('global', dia.SWEEP_GLOBAL)):
+        rp = dia.rhs_plan(H, Ls, rho0, variant=v)
+        rp.run(stream)
+        timer = EventTimer(stream)
+        for _ in range(reps):
+            e0 = timer.start()
+            rp.run(stream)
+            timer.stop(e0)
+        ms = float(np.mean(timer.times_ms()))
+        out[name] = {'us': 1e3 * ms, 'GBps': rp.bytes_per_call / (1e-3 * ms) / 1e9}
+    return out
 
 
 def serving_requests(N: int, seed: int = 2):
@@ -320,6 +342,7 @@ def main(argv=None):
     ap.add_argument('--rates', type=float, nargs='+', default=[100.0, 400.0, 0.0])
     ap.add_argument('--budget-gb', type=float, default=32.0)
     ap.add_argument('--dia-T', type=float, default=5.0)
+    ap.add_argument('--dia-variant', default='auto', choices=['auto', 'global', 'tiled'])
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=3)
     ap.add_argument('--warmup', type=int, default=1)

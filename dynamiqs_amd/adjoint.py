@@ -43,6 +43,21 @@ def _gl(nodes: int):
     return x, w
 
 
+def _check_status(name: str, plan) -> None:
+    """Raise if a solve of ``plan`` stopped before its end time (max_steps or a
+    non-finite / vanishing step), naming the first such instance."""
+    st = plan.stats.cpu().numpy()
+    bad = np.nonzero(st[:, 3] != 0)[0]
+    if len(bad):
+        i = int(bad[0])
+        d = plan.descs[i]
+        raise RuntimeError(
+            f'adjoint: {len(bad)} of {len(st)} {name} solves stopped early; first: instance {i}, status {int(st[i, 3])}'
+            f' (nsteps {int(st[i, 0])}, accepted {int(st[i, 1])}, rejected {int(st[i, 2])}),'
+            f' t0 {float(d["t0"])!r}, t1 {float(d["t1"])!r}'
+        )
+
+
 def mesolve_pwc_grad(
     H0: np.ndarray,
     ops,
@@ -175,9 +190,8 @@ def mesolve_pwc_grad(
     t8 = time.perf_counter()
     timing['backward_pack_s'] = t7 - t6
     timing['backward_run_s'] = t8 - t7
-    bstat = bwd.stats.cpu().numpy()
-    if np.any(bstat[:, 3] != 0) or np.any(rec.stats.cpu().numpy()[:, 3] != 0):
-        raise RuntimeError('adjoint: a recompute or backward solve did not reach its end time')
+    for name, plan in (('forward', fwd), ('recompute', rec), ('backward', bwd)):
+        _check_status(name, plan)
     # lambda at the nodes in increasing t, kept on the device
     lam_dev = torch.flip(bwd.states.view(B, 1 + Q, n * n * 2)[:, 1:], dims=[1]).contiguous()
 

@@ -22,6 +22,7 @@
 
 #include "dqh.h"
 #include "dqh_common.h"
+#include "dqh_dia_tile.h"
 
 namespace dqh {
 namespace dia {
@@ -140,7 +141,8 @@ __global__ __launch_bounds__(kThreads) void k_lsolve(int mode, int64_t n, const 
   }
 }
 
-// Kraus increment + FP64 update + save. One thread per element.
+// Kraus increment + FP64 update + save. One thread per element, neighbours of
+// X = rho32 + E read through the caches (any offsets, any n).
 __global__ __launch_bounds__(256) void k_kraus(int64_t n, int64_t n_true, dqh_dia_ops ops,
                                                const float2 *__restrict__ r32_in, const float2 *__restrict__ E,
                                                double2 *__restrict__ r64, float2 *__restrict__ r32_out,
@@ -151,56 +153,11 @@ __global__ __launch_bounds__(256) void k_kraus(int64_t n, int64_t n_true, dqh_di
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (int64_t)nn) return;
   const int i = (int)(e / n), j = (int)(e % n);
-  const float2 *R = r32_in + inst * nn;
   const float2 *Ei = E + inst * nn;
-  auto X = [&](int a, int b) -> float2 {
-    const size_t o = (size_t)a * n + b;
-    const float2 x = R[o], y = Ei[o];
-    return make_float2(x.x + y.x, x.y + y.y);
-  };
-  const float2 *dM = gp<const float2>(ops.diags[0]);
-  float ar = 0.f, ai = 0.f;
-  // one-sided terms dM X + X dM^dag
-  for (int u = 0; u < ops.noff[0]; ++u) {
-    const int p = ops.off[0][u];
-    const int a = i + p;
-    if (a >= 0 && a < n) {
-      const float2 m = dM[(size_t)u * n + a];  // dM[i][i+p]
-      const float2 x = X(a, j);
-      ar += m.x * x.x - m.y * x.y;
-      ai += m.x * x.y + m.y * x.x;
-    }
-    const int b = j + p;
-    if (b >= 0 && b < n) {
-      const float2 m = dM[(size_t)u * n + b];  // conj(dM[j][j+p])
-      const float2 x = X(i, b);
-      ar += x.x * m.x + x.y * m.y;
-      ai += x.y * m.x - x.x * m.y;
-    }
-  }
-  // sandwich terms A X A^dag (op 0 = dM, then sqrt(dt) L_k)
-  for (int s = 0; s < ops.nops; ++s) {
-    const float2 *A = gp<const float2>(ops.diags[s]);
-    const int no = ops.noff[s];
-    for (int u = 0; u < no; ++u) {
-      const int a = i + ops.off[s][u];
-      if (a < 0 || a >= n) continue;
-      const float2 ma = A[(size_t)u * n + a];
-      float zr = 0.f, zi = 0.f;  // (X A^dag)[a][j] = sum_v X[a][j+q_v] conj(A[j][j+q_v])
-      for (int v = 0; v < no; ++v) {
-        const int b = j + ops.off[s][v];
-        if (b < 0 || b >= n) continue;
-        const float2 mb = A[(size_t)v * n + b];
-        const float2 x = X(a, b);
-        zr += x.x * mb.x + x.y * mb.y;
-        zi += x.y * mb.x - x.x * mb.y;
-      }
-      ar += ma.x * zr - ma.y * zi;
-      ai += ma.x * zi + ma.y * zr;
-    }
-  }
+  const diat::GlobalX X{r32_in + inst * nn, Ei, n, i, j};
+  const float2 acc = diat::stencil(X, ops, true, n, i, j);
   const float2 eij = Ei[e];
-  const double dr = (double)eij.x + ar, di = (double)eij.y + ai;
+  const double dr = (double)eij.x + acc.x, di = (double)eij.y + acc.y;
   double2 *y = r64 + inst * nn;
   const double2 old = y[e];
   const double2 nw = make_double2(old.x + dr, old.y + di);
@@ -209,6 +166,44 @@ __global__ __launch_bounds__(256) void k_kraus(int64_t n, int64_t n_true, dqh_di
   if (save_idx >= 0 && i < n_true && j < n_true) {
     const double sr = old.x + theta * dr, si = old.y + theta * di;
     states[((inst * nsave + save_idx) * n_true + i) * n_true + j] = make_float2((float)sr, (float)si);
+  }
+}
+
+// LDS-tiled form of k_kraus (all offsets within the halo, n % 64 == 0): one
+// workgroup per 64 x 64 tile of one instance; X = rho32 + E staged with its halo.
+__global__ __launch_bounds__(diat::kThreads) void k_kraus_tiled(int64_t n, int64_t n_true, dqh_dia_ops ops,
+                                                               const float2 *__restrict__ r32_in,
+                                                               const float2 *__restrict__ E, double2 *__restrict__ r64,
+                                                               float2 *__restrict__ r32_out,
+                                                               float2 *__restrict__ states, int64_t nsave,
+                                                               int64_t save_idx, double theta) {
+  __shared__ float2 sx[diat::kXR * diat::kXC];
+  const size_t nn = (size_t)n * n;
+  const size_t inst = blockIdx.y;
+  int i0, j0;
+  diat::tile_of(n, i0, j0);
+  diat::load_halo(sx, r32_in + inst * nn, E + inst * nn, n, i0, j0);
+  __syncthreads();
+  const int jj = threadIdx.x & (diat::kTJ - 1);
+  const float2 *Ei = E + inst * nn;
+  double2 *y = r64 + inst * nn;
+#pragma unroll 1
+  for (int m = 0; m < diat::kRowsPerThread; ++m) {
+    const int ii = (threadIdx.x >> 6) + (diat::kThreads / diat::kTJ) * m;
+    const int i = i0 + ii, j = j0 + jj;
+    const diat::LdsX X{sx + (ii + diat::kHalo) * diat::kXC + (jj + diat::kHalo)};
+    const float2 acc = diat::stencil(X, ops, true, n, i, j);
+    const size_t e = (size_t)i * n + j;
+    const float2 eij = Ei[e];
+    const double dr = (double)eij.x + acc.x, di = (double)eij.y + acc.y;
+    const double2 old = y[e];
+    const double2 nw = make_double2(old.x + dr, old.y + di);
+    y[e] = nw;
+    r32_out[inst * nn + e] = make_float2((float)nw.x, (float)nw.y);
+    if (save_idx >= 0 && i < n_true && j < n_true) {
+      const double sr = old.x + theta * dr, si = old.y + theta * di;
+      states[((inst * nsave + save_idx) * n_true + i) * n_true + j] = make_float2((float)sr, (float)si);
+    }
   }
 }
 
@@ -223,6 +218,64 @@ __global__ __launch_bounds__(256) void k_save(int64_t n, int64_t n_true, const d
   if (i < n_true && j < n_true) {
     const double2 v = r64[inst * nn + e];
     states[((inst * nsave + save_idx) * n_true + i) * n_true + j] = make_float2((float)v.x, (float)v.y);
+  }
+}
+
+// states[inst][save_idx] <- old + theta (new - old) for a second save time inside one
+// step: the FP64 start-of-step state is gone, its FP32 copy r32_old is still there.
+__global__ __launch_bounds__(256) void k_save_interp(int64_t n, int64_t n_true, const float2 *__restrict__ r32_old,
+                                                     const double2 *__restrict__ r64, float2 *__restrict__ states,
+                                                     int64_t nsave, int64_t save_idx, double theta) {
+  const size_t nn = (size_t)n * n;
+  const size_t inst = blockIdx.y;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)nn) return;
+  const int i = (int)(e / n), j = (int)(e % n);
+  if (i < n_true && j < n_true) {
+    const double2 v = r64[inst * nn + e];
+    const float2 o = r32_old[inst * nn + e];
+    const double sr = o.x + theta * (v.x - o.x), si = o.y + theta * (v.y - o.y);
+    states[((inst * nsave + save_idx) * n_true + i) * n_true + j] = make_float2((float)sr, (float)si);
+  }
+}
+
+// expects[inst][k][s] = Tr(E_k rho_s) = sum_ab (E_k^T)_ab (rho_s)_ab over the saved
+// states, FP64 accumulation. grid: (nsave, ninst); block 256.
+__global__ __launch_bounds__(256) void k_expect(int64_t n_true, int64_t nsave, int64_t nE,
+                                                const float2 *__restrict__ states, const float2 *__restrict__ EopsT,
+                                                float2 *__restrict__ expects) {
+  __shared__ double red[4][2];
+  const int64_t s = blockIdx.x;
+  const size_t inst = blockIdx.y;
+  const size_t nn = (size_t)n_true * n_true;
+  const float2 *R = states + (inst * nsave + s) * nn;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int64_t k = 0; k < nE; ++k) {
+    const float2 *Ek = EopsT + k * nn;
+    double ar = 0.0, ai = 0.0;
+    for (size_t e = threadIdx.x; e < nn; e += 256) {
+      const float2 a = Ek[e], r = R[e];
+      ar += (double)a.x * r.x - (double)a.y * r.y;
+      ai += (double)a.x * r.y + (double)a.y * r.x;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      ar += __shfl_xor(ar, o, 64);
+      ai += __shfl_xor(ai, o, 64);
+    }
+    __syncthreads();
+    if (lane == 0) {
+      red[wave][0] = ar;
+      red[wave][1] = ai;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double sr = 0.0, si = 0.0;
+      for (int w = 0; w < 4; ++w) {
+        sr += red[w][0];
+        si += red[w][1];
+      }
+      expects[(inst * nE + k) * nsave + s] = make_float2((float)sr, (float)si);
+    }
   }
 }
 
@@ -243,6 +296,8 @@ extern "C" int dqh_mesolve_rouchon1_dia(const dqh_dia_rouchon1 *pb, void *stream
   const dqh_dia_rouchon1 &p = *pb;
   if (p.nb != dia::kNB || p.n <= 0 || p.n % dia::kNB != 0 || p.ninst <= 0 || p.n_true > p.n)
     return fail(DQH_BAD_SHAPE, "dqh_mesolve_rouchon1_dia: n must be a positive multiple of 64 and nb == 64");
+  if (p.nE < 0 || (p.nE > 0 && (!p.EopsT || !p.expects)))
+    return fail(DQH_BAD_ARG, "dqh_mesolve_rouchon1_dia: nE > 0 needs EopsT and expects");
   if (p.ops[0].nops < 1 || p.ops[0].nops > DQH_DIA_MAXOPS || p.nsets < 1 || p.nsets > 2)
     return fail(DQH_BAD_ARG, "dqh_mesolve_rouchon1_dia: bad operator description");
   for (int s = 0; s < p.nsets; ++s)
@@ -258,6 +313,13 @@ extern "C" int dqh_mesolve_rouchon1_dia(const dqh_dia_rouchon1 *pb, void *stream
   float2 *states = reinterpret_cast<float2 *>(p.states);
   const dim3 gel((unsigned)((nn + 255) / 256), (unsigned)p.ninst);
   const dim3 gsolve((unsigned)(n / dia::kNB), (unsigned)p.ninst);
+  const dim3 gtile((unsigned)((n / diat::kTI) * (n / diat::kTJ)), (unsigned)p.ninst);
+  int halo = 0;
+  for (int s = 0; s < p.nsets; ++s) halo = std::max(halo, diat::max_abs_offset(p.ops[s]));
+  const bool tiled_ok = halo <= diat::kHalo;  // n is a multiple of 64 here
+  if (p.variant == DQH_SWEEP_TILED && !tiled_ok)
+    return fail(DQH_BAD_ARG, "dqh_mesolve_rouchon1_dia: the tiled sweep needs every |offset| <= 32");
+  const bool tiled = p.variant == DQH_SWEEP_TILED || (p.variant == DQH_SWEEP_AUTO && tiled_ok);
   int cur = 0;
   hipLaunchKernelGGL(dia::k_round, dim3((unsigned)((p.ninst * nn + 255) / 256)), dim3(256), 0, st, p.ninst * nn, r64,
                      r32[0]);
@@ -282,11 +344,16 @@ extern "C" int dqh_mesolve_rouchon1_dia(const dqh_dia_rouchon1 *pb, void *stream
       th = hs_theta[isave];
       ++isave;
     }
-    hipLaunchKernelGGL(dia::k_kraus, gel, dim3(256), 0, st, n, p.n_true, p.ops[set], r32[cur], E, r64, r32[1 - cur],
-                       states, p.nsave, sidx, th);
-    // further saves in the same step (several save times inside one step): state at step end
+    if (tiled)
+      hipLaunchKernelGGL(dia::k_kraus_tiled, gtile, dim3(diat::kThreads), 0, st, n, p.n_true, p.ops[set], r32[cur], E,
+                         r64, r32[1 - cur], states, p.nsave, sidx, th);
+    else
+      hipLaunchKernelGGL(dia::k_kraus, gel, dim3(256), 0, st, n, p.n_true, p.ops[set], r32[cur], E, r64,
+                         r32[1 - cur], states, p.nsave, sidx, th);
+    // further save times inside the same step, each at its own interpolation weight
     while (isave < p.nsave && hs_step[isave] == m) {
-      hipLaunchKernelGGL(dia::k_save, gel, dim3(256), 0, st, n, p.n_true, r64, states, p.nsave, isave);
+      hipLaunchKernelGGL(dia::k_save_interp, gel, dim3(256), 0, st, n, p.n_true, r32[cur], r64, states, p.nsave,
+                         isave, hs_theta[isave]);
       ++isave;
     }
     cur = 1 - cur;
@@ -295,6 +362,10 @@ extern "C" int dqh_mesolve_rouchon1_dia(const dqh_dia_rouchon1 *pb, void *stream
     hipLaunchKernelGGL(dia::k_save, gel, dim3(256), 0, st, n, p.n_true, r64, states, p.nsave, isave);
     ++isave;
   }
+  if (p.nE > 0)
+    hipLaunchKernelGGL(dia::k_expect, dim3((unsigned)p.nsave, (unsigned)p.ninst), dim3(256), 0, st, p.n_true, p.nsave,
+                       p.nE, states, reinterpret_cast<const float2 *>(p.EopsT),
+                       reinterpret_cast<float2 *>(p.expects));
   const hipError_t err = hipGetLastError();
   if (err != hipSuccess) return fail(DQH_HIP_ERROR, std::string("dqh_mesolve_rouchon1_dia: ") + hipGetErrorString(err));
   return DQH_OK;

@@ -53,6 +53,25 @@ class DiaRouchon1(ctypes.Structure):
         ('states', ctypes.c_uint64),
         ('save_step', ctypes.c_uint64),
         ('save_theta', ctypes.c_uint64),
+        ('nE', ctypes.c_int64),
+        ('EopsT', ctypes.c_uint64),
+        ('expects', ctypes.c_uint64),
+        ('variant', ctypes.c_int64),
+    ]
+
+
+SWEEP_AUTO, SWEEP_GLOBAL, SWEEP_TILED = 0, 1, 2
+
+
+class DiaRhs(ctypes.Structure):
+    _fields_ = [
+        ('n', ctypes.c_int64),
+        ('ninst', ctypes.c_int64),
+        ('variant', ctypes.c_int64),
+        ('pad_', ctypes.c_int64),
+        ('ops', DiaOps),
+        ('rho', ctypes.c_uint64),
+        ('drho', ctypes.c_uint64),
     ]
 
 
@@ -147,12 +166,25 @@ class DiaPlan:
         torch.cuda.synchronize()
         st = self.states.cpu().numpy().view(np.complex64).reshape(self.ninst, self.nsave, self.n_true, self.n_true)
         fin = self.rho64.cpu().numpy().view(np.complex128).reshape(self.ninst, self.n, self.n)[:, : self.n_true, : self.n_true]
-        return {'states': st, 'final_state': fin, 'nsteps': np.full(self.ninst, self.nsteps)}
+        out = {'states': st, 'final_state': fin, 'nsteps': np.full(self.ninst, self.nsteps)}
+        if self.nE:
+            out['expects'] = self.expects.cpu().numpy().view(np.complex64).reshape(self.ninst, self.nE, self.nsave)
+        return out
 
 
-def build_plan(H: np.ndarray, Ls, rho0: np.ndarray, tsave: np.ndarray, dt: float, t0: float | None = None) -> DiaPlan:
+def build_plan(
+    H: np.ndarray,
+    Ls,
+    rho0: np.ndarray,
+    tsave: np.ndarray,
+    dt: float,
+    t0: float | None = None,
+    exp_ops=None,
+    variant: int = SWEEP_AUTO,
+) -> DiaPlan:
     """Pack a batch of Rouchon1 solves that share H and Ls (dense FP64 host arrays
-    whose nonzeros lie on few diagonals) and differ in rho0 (B, n, n)."""
+    whose nonzeros lie on few diagonals) and differ in rho0 (B, n, n). ``exp_ops``
+    (dense (n, n) arrays) are evaluated at every save time on the device."""
     import torch
 
     dev = _device()
@@ -173,24 +205,12 @@ def build_plan(H: np.ndarray, Ls, rho0: np.ndarray, tsave: np.ndarray, dt: float
     arena = _Arena(dev)
     prob = DiaRouchon1()
     prob.n, prob.n_true, prob.nb, prob.ninst = n_pad, n, NB, B
-    prob.nsteps, prob.nsets = nsteps, len(sets)
+    prob.nsteps, prob.nsets, prob.variant = nsteps, len(sets), variant
     for si, h in enumerate(sets):
         Vp, Wn, ops = block_solve_operators(H, Ls, h, n_pad)
         prob.Vp[si] = arena.put_now(Vp.astype(np.complex64))
         prob.Wn[si] = arena.put_now(Wn.astype(np.complex64))
-        if len(ops) > MAXOPS:
-            raise NotImplementedError(f'DIA Rouchon1 supports at most {MAXOPS - 1} jump operators')
-        o = prob.ops[si]
-        o.nops = len(ops)
-        for k, (offs, d) in enumerate(ops):
-            if len(offs) > MAXOFF:
-                raise NotImplementedError(f'DIA Rouchon1 supports at most {MAXOFF} diagonals per operator')
-            o.noff[k] = len(offs)
-            for u, off in enumerate(offs):
-                o.off[k][u] = off
-            dp = np.zeros((len(offs), n_pad), dtype=np.complex64)
-            dp[:, :n] = d
-            o.diags[k] = arena.put_now(dp)
+        _fill_ops(prob.ops[si], ops, n_pad, arena)
     r64 = np.zeros((B, n_pad, n_pad), dtype=np.complex128)
     r64[:, :n, :n] = rho0
     rho64_0 = torch.from_numpy(r64.view(np.float64)).to(dev)
@@ -203,12 +223,85 @@ def build_plan(H: np.ndarray, Ls, rho0: np.ndarray, tsave: np.ndarray, dt: float
     prob.nsave = len(tsave)
     prob.save_step = steps_h.ctypes.data
     prob.save_theta = thetas_h.ctypes.data
+    nE = 0 if exp_ops is None else len(exp_ops)
+    expects = torch.zeros((B * max(nE, 1) * len(tsave) * 2,), dtype=torch.float32, device=dev)
+    prob.nE = nE
+    if nE:
+        ET = np.stack([np.asarray(E, dtype=np.complex128).T for E in exp_ops]).astype(np.complex64)
+        prob.EopsT = arena.put_now(ET)
+        prob.expects = expects.data_ptr()
     plan = DiaPlan()
     plan.problem, plan.arena = prob, arena
     plan.keep = [steps_h, thetas_h, rho32, work]
-    plan.rho64, plan.rho64_0, plan.states = rho64, rho64_0, states
-    plan.ninst, plan.n, plan.n_true, plan.nsave, plan.nsteps = B, n_pad, n, len(tsave), nsteps
+    plan.rho64, plan.rho64_0, plan.states, plan.expects = rho64, rho64_0, states, expects
+    plan.ninst, plan.n, plan.n_true, plan.nsave, plan.nsteps, plan.nE = B, n_pad, n, len(tsave), nsteps, nE
     # roofline units: the Kraus sweep reads rho and writes rho' (2 n^2 complex64) per step and instance
     plan.sweep_bytes_per_step = 2.0 * n * n * 8 * B
     plan.solve_flops_per_step = 2 * 2 * 8.0 * NB * NB * n_pad * (n_pad // NB) * B
+    return plan
+
+
+def _fill_ops(o: DiaOps, ops, n_pad: int, arena: _Arena) -> None:
+    if len(ops) > MAXOPS:
+        raise NotImplementedError(f'DIA sweeps support at most {MAXOPS} operators (Hnh and {MAXOPS - 1} jump operators)')
+    o.nops = len(ops)
+    for k, (offs, d) in enumerate(ops):
+        if len(offs) > MAXOFF:
+            raise NotImplementedError(f'DIA sweeps support at most {MAXOFF} diagonals per operator')
+        if len(offs) == 0:  # an all-zero operator: one zero diagonal keeps the kernel loops uniform
+            offs, d = [0], np.zeros((1, d.shape[-1]), dtype=np.complex128)
+        o.noff[k] = len(offs)
+        for u, off in enumerate(offs):
+            o.off[k][u] = off
+        dp = np.zeros((len(offs), n_pad), dtype=np.complex64)
+        dp[:, : d.shape[-1]] = d
+        o.diags[k] = arena.put_now(dp)
+
+
+class DiaRhsPlan:
+    """A batch of DIA Lindblad vector fields (``dqh_lindblad_rhs_dia``) on
+    resident buffers: ``rho`` in, ``drho`` out, both (B, n_pad, n_pad) complex64."""
+
+    def run(self, stream=None):
+        import torch
+
+        s = stream if stream is not None else torch.cuda.current_stream()
+        _native.check(_native.lib().dqh_lindblad_rhs_dia(ctypes.byref(self.problem), ctypes.c_void_p(s.cuda_stream)), 'dqh_lindblad_rhs_dia')
+
+    def result(self) -> np.ndarray:
+        import torch
+
+        torch.cuda.synchronize()
+        out = self.drho.cpu().numpy().view(np.complex64).reshape(self.ninst, self.n, self.n)
+        return out[:, : self.n_true, : self.n_true]
+
+
+def rhs_plan(H: np.ndarray, Ls, rho: np.ndarray, variant: int = SWEEP_AUTO) -> DiaRhsPlan:
+    """Pack the DIA vector field of (H, Ls) (dense FP64 host arrays whose nonzeros lie
+    on few diagonals) for a batch of states ``rho`` (B, n, n)."""
+    import torch
+
+    dev = _device()
+    _native.lib()
+    rho = np.asarray(rho, dtype=np.complex128)
+    if rho.ndim == 2:
+        rho = rho[None]
+    B, n = rho.shape[0], H.shape[-1]
+    n_pad = NB * ((n + NB - 1) // NB)
+    LdL = sum((L.conj().T @ L for L in Ls), np.zeros_like(H))
+    Hnh = -1j * H - 0.5 * LdL
+    arena = _Arena(dev)
+    prob = DiaRhs()
+    prob.n, prob.ninst, prob.variant = n_pad, B, variant
+    _fill_ops(prob.ops, [_dia_of(Hnh)] + [_dia_of(L) for L in Ls], n_pad, arena)
+    r = np.zeros((B, n_pad, n_pad), dtype=np.complex64)
+    r[:, :n, :n] = rho
+    rho_d = torch.from_numpy(r.view(np.float32)).to(dev)
+    drho = torch.empty_like(rho_d)
+    prob.rho, prob.drho = rho_d.data_ptr(), drho.data_ptr()
+    plan = DiaRhsPlan()
+    plan.problem, plan.arena, plan.rho, plan.drho = prob, arena, rho_d, drho
+    plan.ninst, plan.n, plan.n_true = B, n_pad, n
+    # roofline unit: one vector field reads rho and writes drho (2 n^2 complex64) per instance
+    plan.bytes_per_call = 2.0 * n * n * 8 * B
     return plan

@@ -23,6 +23,7 @@ from ..qarrays import QArray, asqarray
 from ..result import AdaptiveStepInfos, FixedStepInfos, MESolveResult
 from ..time_qarray import TimeQArray, astimeqarray
 from . import dense
+from . import dia as dia_path
 from .operators import decompose
 
 SUPPORTED_METHODS = (Tsit5, Rouchon1)
@@ -68,6 +69,23 @@ def _check_mesolve_args(H: TimeQArray, Ls, rho0: QArray, exp_ops) -> None:
 def _broadcast_index(idx: tuple, shape: tuple) -> tuple:
     sub = idx[len(idx) - len(shape) :] if shape else ()
     return tuple(i if d > 1 else 0 for i, d in zip(sub, shape))
+
+
+# Rouchon1 runs on the banded DIA kernels from this dimension on (smaller problems
+# fit one workgroup of the dense kernel, which is faster there)
+DIA_MIN_N = 128
+
+
+def dia_route(method, H: TimeQArray, Ls, n: int):
+    """Operators for the DIA Rouchon1 kernels, or None when the dense path applies:
+    Rouchon1 with normalisation, constant un-batched DIA operators, n >= DIA_MIN_N."""
+    if not isinstance(method, Rouchon1) or not method.normalize or n < DIA_MIN_N:
+        return None
+    Ht = decompose(H)
+    Lts = [decompose(L) for L in Ls]
+    if Ht.dia is None or Ht.batch != () or any(Lt.dia is None or Lt.batch != () for Lt in Lts):
+        return None
+    return Ht.const_at(()), [Lt.const_at(()) for Lt in Lts]
 
 
 def enumerate_instances(H: TimeQArray, Ls, rho0: QArray, cartesian: bool):
@@ -147,6 +165,10 @@ def mesolve(
     if options.save_extra is not None:
         raise NotImplementedError('`save_extra` is not supported by the MI355X solver')
 
+    route = dia_route(method, H, Ls, H.shape[-1])
+    if route is not None:
+        return _mesolve_dia(route, rho0, tsave, exp_ops, method, gradient, options)
+
     out_shape, insts = enumerate_instances(H, Ls, rho0, options.cartesian_batching)
     if options.assume_hermitian and isinstance(method, Tsit5):
         for inst in insts:
@@ -202,4 +224,31 @@ def mesolve(
         infos = AdaptiveStepInfos(shaped(res['nsteps']), shaped(res['naccepted']), shaped(res['nrejected']))
     else:
         infos = FixedStepInfos(shaped(res['nsteps']))
+    return MESolveResult(tsave, method, gradient, options, ysave, Esave, None, infos)
+
+
+def _mesolve_dia(route, rho0: QArray, tsave, exp_ops, method, gradient, options) -> MESolveResult:
+    """Rouchon1 on the DIA kernels (``dqh_mesolve_rouchon1_dia``): the batch is the
+    batch of ``rho0`` (the operators are shared), every instance steps in lockstep."""
+    Hd, Ld = route
+    r = rho0.to_numpy().astype(np.complex128)
+    out_shape = r.shape[:-2]
+    n = r.shape[-1]
+    r = r.reshape(-1, *r.shape[-2:])
+    if r.shape[-1] == 1:
+        r = r @ np.conj(np.swapaxes(r, -1, -2))
+    Es = [E.to_numpy() for E in exp_ops] if exp_ops is not None else None
+    t0 = None if options.t0 is None else float(options.t0)
+    plan = dia_path.build_plan(Hd, Ld, r, tsave, method.dt, t0=t0, exp_ops=Es)
+    plan.run()
+    res = plan.results()
+    states = res['states']
+    if not options.save_states:
+        states = states[:, -1:]
+    ysave = QArray(states.reshape(*out_shape, -1, n, n), dims=rho0.dims)
+    Esave = None
+    if Es is not None:
+        Esave = res['expects'].reshape(*out_shape, len(Es), len(tsave))
+    nsteps = np.full(out_shape, plan.nsteps, dtype=np.int64)
+    infos = FixedStepInfos(nsteps if out_shape else nsteps[()])
     return MESolveResult(tsave, method, gradient, options, ysave, Esave, None, infos)

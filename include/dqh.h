@@ -94,6 +94,13 @@ int dqh_mesolve_tsit5_dense(const dqh_me_inst *d_insts, int64_t ninst, void *str
 int dqh_mesolve_rouchon1_dense(const dqh_me_inst *d_insts, int64_t ninst, void *stream);
 
 /* ------------------------------------------------------------------ sparse DIA */
+/* Forms of the DIA sweeps: AUTO picks TILED when every |offset| <= 32 and n is a
+ * multiple of 64 (64 x 64 output tiles staged in LDS with a 32-wide halo), else
+ * GLOBAL (one thread per element, neighbours read through the caches). */
+#define DQH_SWEEP_AUTO 0
+#define DQH_SWEEP_GLOBAL 1
+#define DQH_SWEEP_TILED 2
+
 #define DQH_DIA_MAXOPS 8
 #define DQH_DIA_MAXOFF 16
 
@@ -131,6 +138,10 @@ typedef struct dqh_dia_rouchon1 {
   uint64_t states;     /* device c64[ninst][nsave][n_true][n_true] */
   uint64_t save_step;  /* HOST int64[nsave]: step index whose end state is saved (-1: initial) */
   uint64_t save_theta; /* HOST double[nsave]: interpolation weight inside that step */
+  int64_t nE;          /* number of expectation operators (0: none)                  */
+  uint64_t EopsT;      /* device c64[nE][n_true][n_true]: transposed E_k              */
+  uint64_t expects;    /* device c64[ninst][nE][nsave]: Tr(E_k rho(tsave))            */
+  int64_t variant;     /* DQH_SWEEP_*: form of the Kraus sweep                        */
 } dqh_dia_rouchon1;
 
 int64_t dqh_sizeof_dia_rouchon1(void);
@@ -141,6 +152,26 @@ int64_t dqh_sizeof_dia_rouchon1(void);
  * bandwidth-bound stencil sweep, FP64 state with FP32 increments. Launches on
  * ``stream`` and returns without synchronising. */
 int dqh_mesolve_rouchon1_dia(const dqh_dia_rouchon1 *problem, void *stream);
+
+/* A batch of Lindblad vector fields for DIA operators shared by the batch:
+ *   drho = Hnh rho + rho Hnh^dag + sum_k L_k rho L_k^dag,  Hnh = -iH - 1/2 sum L^dag L,
+ * with ops.diags[0] = Hnh and ops.diags[k] = L_k (k = 1 .. nops-1) by diagonals. */
+typedef struct dqh_dia_rhs {
+  int64_t n;        /* dimension; the tiled sweep needs n % 64 == 0 */
+  int64_t ninst;
+  int64_t variant;  /* DQH_SWEEP_* */
+  int64_t pad_;
+  dqh_dia_ops ops;
+  uint64_t rho;     /* device c64[ninst][n][n] */
+  uint64_t drho;    /* device c64[ninst][n][n] */
+} dqh_dia_rhs;
+
+int64_t dqh_sizeof_dia_rhs(void);
+
+/* The Lindblad vector field for sparse-DIA operators as a bandwidth-bound sweep
+ * (replaces vector_field_unvec_hermitian, diffrax_integrator.py:319-323, with the
+ * DIA products of sparsedia_primitives.py:172-261). Launches on ``stream``. */
+int dqh_lindblad_rhs_dia(const dqh_dia_rhs *problem, void *stream);
 
 /* ------------------------------------------------------------------ gradients */
 /* Gauss-Legendre quadrature of the adjoint gradient of piecewise-constant controls

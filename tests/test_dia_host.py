@@ -101,3 +101,75 @@ def test_save_schedule():
     steps, th = dia.save_schedule(np.array([0.0, 0.1, 0.25, 1.0]), step_t)
     assert list(steps) == [-1, 0, 2, 9]
     np.testing.assert_allclose(th, [0.0, 1.0, 0.5, 1.0])
+
+
+def _emulate_tiled_sweep(X, ops, sand0, n):
+    """The LDS-tiled sweep as k_rhs_tiled / k_kraus_tiled index it: workgroup id ->
+    (XCD remap) -> 64 x 64 tile, 128 x 128 halo tile zero outside the matrix,
+    X(p, q) read at halo position (ii + 32 + p, jj + 32 + q), diagonal indices
+    clamped to [0, n). Every output element must be written exactly once."""
+    TI, H = 64, 32
+    tpr = n // TI
+    ntiles = tpr * tpr
+    out = np.zeros((n, n), dtype=complex)
+    hits = np.zeros((n, n), dtype=int)
+    for bid in range(ntiles):
+        b = (bid & 7) * (ntiles >> 3) + (bid >> 3) if ntiles % 8 == 0 else bid
+        assert 0 <= b < ntiles
+        i0, j0 = (b // tpr) * TI, (b % tpr) * TI
+        sx = np.zeros((TI + 2 * H, TI + 2 * H), dtype=complex)
+        gi = np.arange(i0 - H, i0 + TI + H)
+        gj = np.arange(j0 - H, j0 + TI + H)
+        mi, mj = (gi >= 0) & (gi < n), (gj >= 0) & (gj < n)
+        sx[np.ix_(mi, mj)] = X[np.ix_(gi[mi], gj[mj])]
+        ii, jj = np.meshgrid(np.arange(TI), np.arange(TI), indexing='ij')
+        i, j = i0 + ii, j0 + jj
+        acc = np.zeros((TI, TI), dtype=complex)
+
+        def Xat(p, q):
+            return sx[ii + H + p, jj + H + q]
+
+        offs0, D0 = ops[0]
+        for u, p in enumerate(offs0):
+            acc += D0[u][np.clip(i + p, 0, n - 1)] * Xat(p, 0)
+            acc += Xat(0, p) * np.conj(D0[u][np.clip(j + p, 0, n - 1)])
+        for s in range(0 if sand0 else 1, len(ops)):
+            offs, A = ops[s]
+            for u, p in enumerate(offs):
+                z = sum(Xat(p, q) * np.conj(A[v][np.clip(j + q, 0, n - 1)]) for v, q in enumerate(offs))
+                acc += A[u][np.clip(i + p, 0, n - 1)] * z
+        out[i0 : i0 + TI, j0 : j0 + TI] = acc
+        hits[i0 : i0 + TI, j0 : j0 + TI] += 1
+    assert np.all(hits == 1)
+    return out
+
+
+@pytest.mark.parametrize('na,nb_', [(12, 12), (32, 32)])
+def test_tiled_sweep_indexing_reproduces_dense_rhs(na, nb_):
+    """The tiled sweep's index arithmetic (n = 192 without and n = 1024 with the XCD
+    remap) gives the dense Hermitian-form Lindblad RHS."""
+    from oracle import lindblad
+
+    H, Ls = _two_mode(na, nb_)
+    n = H.shape[0]
+    n_pad = dia.NB * ((n + dia.NB - 1) // dia.NB)
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    rho = x @ x.conj().T
+    rho /= np.trace(rho).real
+    LdL = sum(L.conj().T @ L for L in Ls)
+    Hnh = -1j * H - 0.5 * LdL
+
+    def padded(opr):
+        offs, d = dia._dia_of(opr)
+        dp = np.zeros((len(offs), n_pad), dtype=complex)
+        dp[:, :n] = d
+        return offs, dp
+
+    assert max(abs(o) for o, _ in [(k, 0) for op_ in [Hnh, *Ls] for k in dia._dia_of(op_)[0]]) <= 32
+    X = np.zeros((n_pad, n_pad), dtype=complex)
+    X[:n, :n] = rho
+    out = _emulate_tiled_sweep(X, [padded(Hnh)] + [padded(L) for L in Ls], False, n_pad)
+    ref = lindblad.rhs_hermitian(H, Ls, rho)
+    np.testing.assert_allclose(out[:n, :n], ref, atol=1e-12)
+    assert np.abs(out[n:, :]).max(initial=0.0) == 0 and np.abs(out[:, n:]).max(initial=0.0) == 0

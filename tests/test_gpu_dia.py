@@ -67,3 +67,89 @@ def test_dia_rouchon1_config3_shape_short_run():
         assert err <= TOL, (b, err)
         # FP32 increments on an FP64 state: trace drift stays far below complex64 rounding
         assert abs(np.trace(out["final_state"][b]) - 1) < 1e-7
+
+
+def test_mesolve_routes_dia_rouchon1_with_expects_and_close_saves():
+    """``dq.mesolve`` with DIA operators and Rouchon1 at n = 144 runs the DIA kernels;
+    states and ``Tr(E rho)`` match the oracle, including two save times inside one
+    step (each interpolated with its own weight)."""
+    import dynamiqs_amd as dq
+    from dynamiqs_amd.integrators import mesolve as ms
+    from oracle import mesolve as om
+    from oracle import operators as op
+
+    a, b = dq.destroy(12, 12)
+    H = 5 * a.dag() @ a + 6 * b.dag() @ b + 0.1 * (a.dag() @ b + a @ b.dag()) - 0.01 * a.dag() @ a @ b.dag() @ b
+    Ls = [np.sqrt(0.02) * a, np.sqrt(0.05) * b]
+    assert ms.dia_route(dq.method.Rouchon1(dt=1e-3), dq.constant(H), [dq.constant(L) for L in Ls], 144) is not None
+    alphas = (0.5, 1.5)
+    kets = np.stack([op.coherent((12, 12), (al, 0.0)) for al in alphas])
+    E = (a.dag() @ a).to_numpy()
+    dt = 1e-3
+    tsave = np.array([0.0, 0.0502, 0.0504, 0.1, 0.2])
+    res = dq.mesolve(H, Ls, kets, tsave, exp_ops=[a.dag() @ a], method=dq.method.Rouchon1(dt=dt))
+    assert res.states.shape == (2, len(tsave), 144, 144)
+    Hn, Ln = H.to_numpy(), [L.to_numpy() for L in Ls]
+    for i in range(len(alphas)):
+        rho0 = kets[i] @ kets[i].conj().T
+        ref = om.mesolve(Hn, Ln, rho0, tsave, exp_ops=[E], method='rouchon1', dt=dt)
+        err = np.linalg.norm(res.states.to_numpy()[i] - ref['states'], axis=(-2, -1)).max()
+        assert err <= TOL, (i, err)
+        eerr = np.abs(res.expects[i, 0] - ref['expects'][0]).max()
+        assert eerr <= 1e-4, (i, eerr)
+    assert int(np.asarray(res.infos.nsteps).ravel()[0]) == 200
+
+
+def _random_dm(rng, n, B):
+    out = []
+    for _ in range(B):
+        x = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+        r = x @ x.conj().T
+        out.append(r / np.trace(r).real)
+    return np.stack(out)
+
+
+@pytest.mark.parametrize(
+    'dims,variant',
+    [((12, 12), 'auto'), ((12, 12), 'global'), ((32, 32), 'tiled'), ((32, 32), 'global'), ((6, 6, 6), 'auto')],
+)
+def test_dia_rhs_matches_oracle(dims, variant):
+    """``dqh_lindblad_rhs_dia`` against the FP64 Hermitian-form RHS of the oracle on
+    random density matrices: n = 144 (padded to 192), the full 1024 two-mode size
+    (tiled and global sweeps) and a 3-mode 216 space whose offsets (36) exceed the
+    LDS halo, so only the global sweep applies. Bar: 1e-5 relative Frobenius."""
+    from dynamiqs_amd.integrators import dia
+    from oracle import lindblad
+    from oracle import operators as op
+
+    rng = np.random.default_rng(7)
+    modes = op.destroy(*dims)
+    n = int(np.prod(dims))
+    H = sum((w + 1) * m.conj().T @ m for w, m in enumerate(modes))
+    H = H + 0.1 * (modes[0].conj().T @ modes[1] + modes[0] @ modes[1].conj().T)
+    H = H - 0.01 * modes[0].conj().T @ modes[0] @ modes[1].conj().T @ modes[1]
+    Ls = [np.sqrt(0.02 * (k + 1)) * m for k, m in enumerate(modes)]
+    rho = _random_dm(rng, n, 2)
+    v = {'auto': dia.SWEEP_AUTO, 'global': dia.SWEEP_GLOBAL, 'tiled': dia.SWEEP_TILED}[variant]
+    plan = dia.rhs_plan(H, Ls, rho, variant=v)
+    plan.run()
+    out = plan.result()
+    for b in range(2):
+        ref = lindblad.rhs_hermitian(H, Ls, rho[b])
+        err = np.linalg.norm(out[b] - ref) / max(1.0, np.linalg.norm(ref))
+        assert err <= 1e-5, (dims, variant, b, err)
+        # the vector field of a density matrix is traceless and Hermitian
+        assert abs(np.trace(out[b])) <= 1e-4 * max(1.0, np.linalg.norm(ref))
+
+
+def test_dia_rhs_tiled_rejects_wide_offsets():
+    """Offsets beyond the LDS halo (a 3-mode 6 x 6 x 6 space: offset 36) make the
+    tiled sweep refuse the problem on the host before any launch."""
+    from dynamiqs_amd.integrators import dia
+    from oracle import operators as op
+
+    modes = op.destroy(6, 6, 6)
+    H = sum(m.conj().T @ m for m in modes)
+    plan = dia.rhs_plan(H, [modes[0]], _random_dm(np.random.default_rng(0), 216, 1), variant=dia.SWEEP_TILED)
+    with pytest.raises(RuntimeError, match='tiled sweep'):
+        plan.run()
