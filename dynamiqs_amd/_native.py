@@ -71,6 +71,9 @@ SYMBOLS = {
     'dqh_mesolve_rouchon1_dia': (ctypes.c_int, [_P, _P]),
     'dqh_sizeof_dia_rhs': (_I64, []),
     'dqh_lindblad_rhs_dia': (ctypes.c_int, [_P, _P]),
+    'dqh_sizeof_dia_tsit5': (_I64, []),
+    'dqh_dia_tsit5_work_bytes': (_I64, [_I64, _I64]),
+    'dqh_mesolve_tsit5_dia': (ctypes.c_int, [_P, _P]),
     'dqh_adjoint_pwc_quadrature': (ctypes.c_int, [_I64, _I64, _I64, _I64, _I64, _P, _P, _P, _P, _P, _P, _P]),
     'dqh_last_error': (ctypes.c_char_p, []),
     'dqh_device_info': (ctypes.c_int, [ctypes.c_char_p, _I64, ctypes.POINTER(_I64)]),
@@ -95,12 +98,14 @@ def lib():
             fn.argtypes = args
         if handle.dqh_sizeof_me_inst() != ME_INST_DTYPE.itemsize:
             raise RuntimeError('dqh_me_inst layout mismatch between include/dqh.h and dynamiqs_amd/_native.py')
-        from .integrators.dia import DiaRhs, DiaRouchon1
+        from .integrators.dia import DiaRhs, DiaRouchon1, DiaTsit5
 
         if handle.dqh_sizeof_dia_rouchon1() != ctypes.sizeof(DiaRouchon1):
             raise RuntimeError('dqh_dia_rouchon1 layout mismatch between include/dqh.h and dynamiqs_amd/integrators/dia.py')
         if handle.dqh_sizeof_dia_rhs() != ctypes.sizeof(DiaRhs):
             raise RuntimeError('dqh_dia_rhs layout mismatch between include/dqh.h and dynamiqs_amd/integrators/dia.py')
+        if handle.dqh_sizeof_dia_tsit5() != ctypes.sizeof(DiaTsit5):
+            raise RuntimeError('dqh_dia_tsit5 layout mismatch between include/dqh.h and dynamiqs_amd/integrators/dia.py')
         _lib = handle
     return _lib
 

@@ -23,6 +23,7 @@
 #include "dqh.h"
 #include "dqh_common.h"
 #include "dqh_dia_tile.h"
+#include "dqh_expect.h"
 
 namespace dqh {
 namespace dia {
@@ -239,46 +240,6 @@ __global__ __launch_bounds__(256) void k_save_interp(int64_t n, int64_t n_true, 
   }
 }
 
-// expects[inst][k][s] = Tr(E_k rho_s) = sum_ab (E_k^T)_ab (rho_s)_ab over the saved
-// states, FP64 accumulation. grid: (nsave, ninst); block 256.
-__global__ __launch_bounds__(256) void k_expect(int64_t n_true, int64_t nsave, int64_t nE,
-                                                const float2 *__restrict__ states, const float2 *__restrict__ EopsT,
-                                                float2 *__restrict__ expects) {
-  __shared__ double red[4][2];
-  const int64_t s = blockIdx.x;
-  const size_t inst = blockIdx.y;
-  const size_t nn = (size_t)n_true * n_true;
-  const float2 *R = states + (inst * nsave + s) * nn;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int64_t k = 0; k < nE; ++k) {
-    const float2 *Ek = EopsT + k * nn;
-    double ar = 0.0, ai = 0.0;
-    for (size_t e = threadIdx.x; e < nn; e += 256) {
-      const float2 a = Ek[e], r = R[e];
-      ar += (double)a.x * r.x - (double)a.y * r.y;
-      ai += (double)a.x * r.y + (double)a.y * r.x;
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-      ar += __shfl_xor(ar, o, 64);
-      ai += __shfl_xor(ai, o, 64);
-    }
-    __syncthreads();
-    if (lane == 0) {
-      red[wave][0] = ar;
-      red[wave][1] = ai;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double sr = 0.0, si = 0.0;
-      for (int w = 0; w < 4; ++w) {
-        sr += red[w][0];
-        si += red[w][1];
-      }
-      expects[(inst * nE + k) * nsave + s] = make_float2((float)sr, (float)si);
-    }
-  }
-}
-
 __global__ __launch_bounds__(256) void k_round(int64_t total, const double2 *__restrict__ r64, float2 *__restrict__ r32) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e < total) r32[e] = make_float2((float)r64[e].x, (float)r64[e].y);
@@ -363,7 +324,7 @@ extern "C" int dqh_mesolve_rouchon1_dia(const dqh_dia_rouchon1 *pb, void *stream
     ++isave;
   }
   if (p.nE > 0)
-    hipLaunchKernelGGL(dia::k_expect, dim3((unsigned)p.nsave, (unsigned)p.ninst), dim3(256), 0, st, p.n_true, p.nsave,
+    hipLaunchKernelGGL(k_expect_states, dim3((unsigned)p.nsave, (unsigned)p.ninst), dim3(256), 0, st, p.n_true, p.nsave,
                        p.nE, states, reinterpret_cast<const float2 *>(p.EopsT),
                        reinterpret_cast<float2 *>(p.expects));
   const hipError_t err = hipGetLastError();

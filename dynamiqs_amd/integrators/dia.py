@@ -305,3 +305,134 @@ def rhs_plan(H: np.ndarray, Ls, rho: np.ndarray, variant: int = SWEEP_AUTO) -> D
     # roofline unit: one vector field reads rho and writes drho (2 n^2 complex64) per instance
     plan.bytes_per_call = 2.0 * n * n * 8 * B
     return plan
+
+
+class DiaTsit5(ctypes.Structure):
+    _fields_ = [
+        ('n', ctypes.c_int64),
+        ('n_true', ctypes.c_int64),
+        ('ninst', ctypes.c_int64),
+        ('nsave', ctypes.c_int64),
+        ('max_steps', ctypes.c_int64),
+        ('nE', ctypes.c_int64),
+        ('variant', ctypes.c_int64),
+        ('graph_steps', ctypes.c_int64),
+        ('rtol', ctypes.c_double),
+        ('atol', ctypes.c_double),
+        ('safety', ctypes.c_double),
+        ('factormin', ctypes.c_double),
+        ('factormax', ctypes.c_double),
+        ('t0', ctypes.c_double),
+        ('t1', ctypes.c_double),
+        ('pad_', ctypes.c_double),
+        ('ops', DiaOps),
+        ('y', ctypes.c_uint64),
+        ('work', ctypes.c_uint64),
+        ('tsave', ctypes.c_uint64),
+        ('states', ctypes.c_uint64),
+        ('EopsT', ctypes.c_uint64),
+        ('expects', ctypes.c_uint64),
+        ('stats', ctypes.c_uint64),
+        ('host_flags', ctypes.c_uint64),
+    ]
+
+
+class DiaTsit5Plan:
+    """A batch of adaptive Tsit5 solves on the DIA sweeps (``dqh_mesolve_tsit5_dia``)."""
+
+    def run(self, stream=None):
+        import torch
+
+        s = stream if stream is not None else torch.cuda.current_stream()
+        _native.check(_native.lib().dqh_mesolve_tsit5_dia(ctypes.byref(self.problem), ctypes.c_void_p(s.cuda_stream)), 'dqh_mesolve_tsit5_dia')
+
+    def reset(self):
+        self.y.copy_(self.y0)
+
+    def results(self) -> dict:
+        import torch
+
+        torch.cuda.synchronize()
+        st = self.states.cpu().numpy().view(np.complex64).reshape(self.ninst, self.nsave, self.n_true, self.n_true)
+        fin = self.y.cpu().numpy().view(np.complex128).reshape(self.ninst, self.n, self.n)[:, : self.n_true, : self.n_true]
+        stats = self.stats.cpu().numpy()
+        out = {
+            'states': st,
+            'final_state': fin,
+            'nsteps': stats[:, 0],
+            'naccepted': stats[:, 1],
+            'nrejected': stats[:, 2],
+            'status': stats[:, 3],
+            'nrhs': stats[:, 4],
+        }
+        if self.nE:
+            out['expects'] = self.expects.cpu().numpy().view(np.complex64).reshape(self.ninst, self.nE, self.nsave)
+        return out
+
+
+def tsit5_plan(
+    H: np.ndarray,
+    Ls,
+    rho0: np.ndarray,
+    tsave: np.ndarray,
+    *,
+    t0: float | None = None,
+    rtol: float = 1e-6,
+    atol: float = 1e-6,
+    safety: float = 0.9,
+    factormin: float = 0.2,
+    factormax: float = 5.0,
+    max_steps: int = 100_000,
+    exp_ops=None,
+    variant: int = SWEEP_AUTO,
+    graph_steps: int = 16,
+) -> DiaTsit5Plan:
+    """Pack a batch of Tsit5 solves that share H and Ls (dense FP64 host arrays whose
+    nonzeros lie on few diagonals) and differ in rho0 (B, n, n)."""
+    import torch
+
+    dev = _device()
+    lib = _native.lib()
+    tsave = np.asarray(tsave, dtype=np.float64)
+    rho0 = np.asarray(rho0, dtype=np.complex128)
+    if rho0.ndim == 2:
+        rho0 = rho0[None]
+    B, n = rho0.shape[0], H.shape[-1]
+    n_pad = NB * ((n + NB - 1) // NB)
+    LdL = sum((L.conj().T @ L for L in Ls), np.zeros_like(H))
+    Hnh = -1j * H - 0.5 * LdL
+    arena = _Arena(dev)
+    prob = DiaTsit5()
+    prob.n, prob.n_true, prob.ninst, prob.nsave = n_pad, n, B, len(tsave)
+    prob.max_steps, prob.variant, prob.graph_steps = int(max_steps), variant, int(graph_steps)
+    prob.rtol, prob.atol, prob.safety = float(rtol), float(atol), float(safety)
+    prob.factormin, prob.factormax = float(factormin), float(factormax)
+    prob.t0 = float(tsave[0] if t0 is None else t0)
+    prob.t1 = float(tsave[-1])
+    _fill_ops(prob.ops, [_dia_of(Hnh)] + [_dia_of(L) for L in Ls], n_pad, arena)
+    r64 = np.zeros((B, n_pad, n_pad), dtype=np.complex128)
+    r64[:, :n, :n] = rho0
+    y0 = torch.from_numpy(r64.view(np.float64)).to(dev)
+    y = y0.clone()
+    work = torch.empty((int(lib.dqh_dia_tsit5_work_bytes(n_pad, B)),), dtype=torch.uint8, device=dev)
+    ts_d = torch.from_numpy(tsave.copy()).to(dev)
+    states = torch.zeros((B * len(tsave) * n * n * 2,), dtype=torch.float32, device=dev)
+    stats = torch.zeros((B, 8), dtype=torch.int64, device=dev)
+    flags = torch.zeros((2,), dtype=torch.int64, pin_memory=True)
+    nE = 0 if exp_ops is None else len(exp_ops)
+    expects = torch.zeros((B * max(nE, 1) * len(tsave) * 2,), dtype=torch.float32, device=dev)
+    prob.nE = nE
+    if nE:
+        ET = np.stack([np.asarray(E, dtype=np.complex128).T for E in exp_ops]).astype(np.complex64)
+        prob.EopsT = arena.put_now(ET)
+        prob.expects = expects.data_ptr()
+    prob.y, prob.work, prob.tsave = y.data_ptr(), work.data_ptr(), ts_d.data_ptr()
+    prob.states, prob.stats, prob.host_flags = states.data_ptr(), stats.data_ptr(), flags.data_ptr()
+    plan = DiaTsit5Plan()
+    plan.problem, plan.arena = prob, arena
+    plan.y, plan.y0, plan.work, plan.tsave_d, plan.states, plan.stats, plan.flags = y, y0, work, ts_d, states, stats, flags
+    plan.expects, plan.nE = expects, nE
+    plan.ninst, plan.n, plan.n_true, plan.nsave = B, n_pad, n, len(tsave)
+    # roofline unit: one vector field reads rho and writes drho (2 n^2 complex64) per instance
+    plan.bytes_per_rhs = 2.0 * n * n * 8
+    return plan

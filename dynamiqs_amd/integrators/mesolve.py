@@ -71,15 +71,17 @@ def _broadcast_index(idx: tuple, shape: tuple) -> tuple:
     return tuple(i if d > 1 else 0 for i, d in zip(sub, shape))
 
 
-# Rouchon1 runs on the banded DIA kernels from this dimension on (smaller problems
-# fit one workgroup of the dense kernel, which is faster there)
+# Constant un-batched DIA operators run on the DIA sweeps from this dimension on
+# (smaller problems fit one workgroup of the dense kernel, which is faster there)
 DIA_MIN_N = 128
 
 
 def dia_route(method, H: TimeQArray, Ls, n: int):
-    """Operators for the DIA Rouchon1 kernels, or None when the dense path applies:
+    """Operators for the DIA kernels, or None when the dense path applies: Tsit5, or
     Rouchon1 with normalisation, constant un-batched DIA operators, n >= DIA_MIN_N."""
-    if not isinstance(method, Rouchon1) or not method.normalize or n < DIA_MIN_N:
+    if n < DIA_MIN_N:
+        return None
+    if not (isinstance(method, Tsit5) or (isinstance(method, Rouchon1) and method.normalize)):
         return None
     Ht = decompose(H)
     Lts = [decompose(L) for L in Ls]
@@ -228,20 +230,45 @@ def mesolve(
 
 
 def _mesolve_dia(route, rho0: QArray, tsave, exp_ops, method, gradient, options) -> MESolveResult:
-    """Rouchon1 on the DIA kernels (``dqh_mesolve_rouchon1_dia``): the batch is the
-    batch of ``rho0`` (the operators are shared), every instance steps in lockstep."""
+    """Tsit5 (``dqh_mesolve_tsit5_dia``, per-instance adaptive steps) or Rouchon1
+    (``dqh_mesolve_rouchon1_dia``, lockstep) on the DIA sweeps: the batch is the
+    batch of ``rho0``, the operators are shared."""
     Hd, Ld = route
     r = rho0.to_numpy().astype(np.complex128)
     out_shape = r.shape[:-2]
-    n = r.shape[-1]
+    n = r.shape[-2]
     r = r.reshape(-1, *r.shape[-2:])
     if r.shape[-1] == 1:
         r = r @ np.conj(np.swapaxes(r, -1, -2))
     Es = [E.to_numpy() for E in exp_ops] if exp_ops is not None else None
     t0 = None if options.t0 is None else float(options.t0)
-    plan = dia_path.build_plan(Hd, Ld, r, tsave, method.dt, t0=t0, exp_ops=Es)
+    if isinstance(method, Tsit5):
+        if options.assume_hermitian:
+            for x in r:
+                check_hermitian(x, 'y0')
+        plan = dia_path.tsit5_plan(
+            Hd,
+            Ld,
+            r,
+            tsave,
+            t0=t0,
+            rtol=method.rtol,
+            atol=method.atol,
+            safety=method.safety_factor,
+            factormin=method.min_factor,
+            factormax=method.max_factor,
+            max_steps=method.max_steps,
+            exp_ops=Es,
+        )
+    else:
+        plan = dia_path.build_plan(Hd, Ld, r, tsave, method.dt, t0=t0, exp_ops=Es)
     plan.run()
     res = plan.results()
+    if 'status' in res:
+        if np.any(res['status'] == -3):
+            raise max_steps_error()
+        if np.any(res['status'] == -4):
+            raise RuntimeError('The solver stopped on a non-finite error estimate or a vanishing step size.')
     states = res['states']
     if not options.save_states:
         states = states[:, -1:]
@@ -249,6 +276,12 @@ def _mesolve_dia(route, rho0: QArray, tsave, exp_ops, method, gradient, options)
     Esave = None
     if Es is not None:
         Esave = res['expects'].reshape(*out_shape, len(Es), len(tsave))
-    nsteps = np.full(out_shape, plan.nsteps, dtype=np.int64)
-    infos = FixedStepInfos(nsteps if out_shape else nsteps[()])
+    def shaped(x):
+        x = np.asarray(x).reshape(out_shape)
+        return x if out_shape else x[()]
+
+    if isinstance(method, Tsit5):
+        infos = AdaptiveStepInfos(shaped(res['nsteps']), shaped(res['naccepted']), shaped(res['nrejected']))
+    else:
+        infos = FixedStepInfos(shaped(np.full(len(r), plan.nsteps, dtype=np.int64)))
     return MESolveResult(tsave, method, gradient, options, ysave, Esave, None, infos)

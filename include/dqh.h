@@ -173,6 +173,41 @@ int64_t dqh_sizeof_dia_rhs(void);
  * DIA products of sparsedia_primitives.py:172-261). Launches on ``stream``. */
 int dqh_lindblad_rhs_dia(const dqh_dia_rhs *problem, void *stream);
 
+/* A batch of adaptive Tsit5 solves for DIA operators shared by the batch (the
+ * instances differ in rho0; each keeps its own t, dt and accept/reject). */
+typedef struct dqh_dia_tsit5 {
+  int64_t n;            /* padded dimension; the tiled sweep needs n % 64 == 0 */
+  int64_t n_true;       /* Hilbert-space dimension */
+  int64_t ninst;
+  int64_t nsave;
+  int64_t max_steps;
+  int64_t nE;
+  int64_t variant;      /* DQH_SWEEP_* */
+  int64_t graph_steps;  /* steps per captured hipGraph (even); 0: plain launches */
+  double rtol, atol, safety, factormin, factormax, t0, t1;
+  double pad_;
+  dqh_dia_ops ops;      /* diags[0] = Hnh = -iH - 1/2 sum L^dag L, diags[k] = L_k */
+  uint64_t y;           /* device double2[ninst][n][n]: rho0 on entry, final state on exit */
+  uint64_t work;        /* device scratch of dqh_dia_tsit5_work_bytes(n, ninst) bytes */
+  uint64_t tsave;       /* device double[nsave], increasing */
+  uint64_t states;      /* device c64[ninst][nsave][n_true][n_true] */
+  uint64_t EopsT;       /* device c64[nE][n_true][n_true] */
+  uint64_t expects;     /* device c64[ninst][nE][nsave] */
+  uint64_t stats;       /* device int64[ninst][8]: nsteps, naccepted, nrejected, status, nrhs */
+  uint64_t host_flags;  /* HOST pinned int64[2]: completion counts read back by the host loop */
+} dqh_dia_tsit5;
+
+int64_t dqh_sizeof_dia_tsit5(void);
+int64_t dqh_dia_tsit5_work_bytes(int64_t n, int64_t ninst);
+
+/* Adaptive Tsit5 with the DIA Lindblad sweep as vector field (replaces
+ * MESolveDiffraxIntegrator.run with Tsit5, diffrax_integrator.py:101-147,
+ * 373-378, for large sparse problems). Stage sweeps and the device-side
+ * controller run as a captured hipGraph of graph_steps steps, replayed until every
+ * instance reached t1 or stopped (status in stats). Returns when the batch is
+ * done; results are ready once ``stream`` is synchronised. */
+int dqh_mesolve_tsit5_dia(const dqh_dia_tsit5 *problem, void *stream);
+
 /* ------------------------------------------------------------------ gradients */
 /* Gauss-Legendre quadrature of the adjoint gradient of piecewise-constant controls
  * (replaces the reverse-mode path jax.grad -> dx.RecursiveCheckpointAdjoint,

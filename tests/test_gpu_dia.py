@@ -153,3 +153,109 @@ def test_dia_rhs_tiled_rejects_wide_offsets():
     plan = dia.rhs_plan(H, [modes[0]], _random_dm(np.random.default_rng(0), 216, 1), variant=dia.SWEEP_TILED)
     with pytest.raises(RuntimeError, match='tiled sweep'):
         plan.run()
+
+
+def _sparse_lindblad(H, Ls):
+    """FP64 vector field through scipy.sparse products (the oracle's Hermitian
+    form, rhs_hermitian, with sparse operators so n = 1024 runs in milliseconds)."""
+    import scipy.sparse as sp
+
+    Hnh = sp.csr_matrix(-1j * H - 0.5 * sum(L.conj().T @ L for L in Ls))
+    Hd = Hnh.conj().T.tocsr()
+    Lsp = [sp.csr_matrix(L) for L in Ls]
+    Ld = [L.conj().T.tocsr() for L in Lsp]
+
+    def f(t, y, tseg):
+        out = Hnh @ y + (Hd.T @ y.T).T
+        for L, D in zip(Lsp, Ld):
+            out = out + (D.T @ (L @ y).T).T
+        return out
+
+    return f
+
+
+@pytest.mark.parametrize('variant', ['tiled', 'global'])
+def test_dia_tsit5_matches_oracle(variant):
+    """Adaptive Tsit5 on the DIA sweeps (n = 144, padded to 192) against the oracle's
+    FP64 Tsit5 on the same problem and tolerances: states within 1e-5, identical
+    step counts up to a few steps (FP32 stages vs FP64 error estimate)."""
+    from dynamiqs_amd.integrators import dia
+    from oracle import mesolve as om
+    from oracle import operators as op
+
+    H, Ls, _ = _two_mode(12, 12)
+    alphas = (0.5, 1.5)
+    rho0 = np.stack([op.coherent_dm((12, 12), (al, 0.0)) for al in alphas])
+    tsave = np.linspace(0.0, 0.3, 7)
+    v = {'tiled': dia.SWEEP_TILED, 'global': dia.SWEEP_GLOBAL}[variant]
+    plan = dia.tsit5_plan(H, Ls, rho0, tsave, variant=v)
+    plan.run()
+    out = plan.results()
+    assert np.all(out['status'] == 0)
+    for b in range(len(alphas)):
+        ref = om.mesolve(H, Ls, rho0[b], tsave, method='tsit5', rtol=1e-6, atol=1e-6)
+        err = np.linalg.norm(out['states'][b] - ref['states'], axis=(-2, -1)).max()
+        assert err <= TOL, (variant, b, err)
+        nref = int(ref['infos']['nsteps'])
+        assert abs(int(out['nsteps'][b]) - nref) <= max(2, nref // 20), (out['nsteps'][b], nref)
+        assert abs(np.trace(out['final_state'][b]) - 1) < 1e-6
+
+
+def test_dia_tsit5_graph_replays_equal_plain_launches():
+    """The captured hipGraph (replayed many times, 4 steps each) and plain launches
+    of the same kernels give bit-identical states and counters."""
+    from dynamiqs_amd.integrators import dia
+    from oracle import operators as op
+
+    H, Ls, _ = _two_mode(12, 12)
+    rho0 = np.stack([op.coherent_dm((12, 12), (al, 0.0)) for al in (0.7, 1.9)])
+    tsave = np.linspace(0.0, 0.2, 5)
+    outs = []
+    for gs in (4, 0):
+        plan = dia.tsit5_plan(H, Ls, rho0, tsave, graph_steps=gs)
+        plan.run()
+        outs.append(plan.results())
+    assert np.array_equal(outs[0]['states'], outs[1]['states'])
+    assert np.array_equal(outs[0]['nsteps'], outs[1]['nsteps'])
+    assert np.array_equal(outs[0]['final_state'], outs[1]['final_state'])
+
+
+def test_dia_tsit5_full_size_short_run():
+    """The 32 x 32 two-mode size (n = 1024) for t in [0, 0.05] against the oracle's
+    Tsit5 driven by sparse FP64 products, plus the per-instance step counters."""
+    from dynamiqs_amd.integrators import dia
+    from oracle import operators as op
+    from oracle import tsit5
+
+    H, Ls, _ = _two_mode(32, 32)
+    rho0 = np.stack([op.coherent_dm((32, 32), (al, 0.0)) for al in (0.5, 2.0)])
+    tsave = np.linspace(0.0, 0.05, 3)
+    plan = dia.tsit5_plan(H, Ls, rho0, tsave)
+    plan.run()
+    out = plan.results()
+    assert np.all(out['status'] == 0)
+    f = _sparse_lindblad(H, Ls)
+    for b in range(2):
+        saved, _, st = tsit5.integrate(f, rho0[b], tsave, rtol=1e-6, atol=1e-6)
+        err = np.linalg.norm(out['states'][b] - np.stack(saved), axis=(-2, -1)).max()
+        assert err <= TOL, (b, err)
+        assert out['nrhs'][b] == 2 + 6 * out['nsteps'][b]
+
+
+def test_mesolve_routes_dia_tsit5():
+    """``dq.mesolve`` with DIA operators and Tsit5 at n = 144 runs the DIA Tsit5
+    kernels and returns adaptive-step infos and expectation values."""
+    import dynamiqs_amd as dq
+    from oracle import mesolve as om
+
+    a, b = dq.destroy(12, 12)
+    H = 5 * a.dag() @ a + 6 * b.dag() @ b + 0.1 * (a.dag() @ b + a @ b.dag())
+    Ls = [np.sqrt(0.02) * a, np.sqrt(0.05) * b]
+    psi = dq.coherent((12, 12), (1.0, 0.0))
+    tsave = np.linspace(0.0, 0.2, 5)
+    res = dq.mesolve(H, Ls, psi, tsave, exp_ops=[a.dag() @ a])
+    rho0 = psi.to_numpy() @ psi.to_numpy().conj().T
+    ref = om.mesolve(H.to_numpy(), [L.to_numpy() for L in Ls], rho0, tsave, exp_ops=[(a.dag() @ a).to_numpy()])
+    assert np.linalg.norm(res.states.to_numpy() - ref['states'], axis=(-2, -1)).max() <= TOL
+    assert np.abs(res.expects[0] - ref['expects'][0]).max() <= 1e-4
+    assert int(res.infos.nsteps) > 0
