@@ -584,7 +584,7 @@ __global__ __launch_bounds__(kThreads) void k_tsit5(const dqh_me_inst *__restric
   __shared__ float2 lds[kResLdsFloat2];
   const dqh_me_inst &p = insts[blockIdx.x];
   const int np = (int)p.np;
-  if (p.nL <= kResMaxL) {
+  if (p.nL <= kResMaxL && !(p.flags & DQH_FLAG_GLOBAL)) {
     if (np == 16) return tsit5_resident<16>(p, sh, lds);
     if (np == 32) return tsit5_resident<32>(p, sh, lds);
     if (np == 48) return tsit5_resident<48>(p, sh, lds);
@@ -597,7 +597,7 @@ __global__ __launch_bounds__(kThreads) void k_rhs(const dqh_me_inst *__restrict_
   __shared__ float2 lds[kResLdsFloat2];
   const dqh_me_inst &p = insts[blockIdx.x];
   const int np = (int)p.np;
-  if (p.nL <= kResMaxL) {
+  if (p.nL <= kResMaxL && !(p.flags & DQH_FLAG_GLOBAL)) {
     if (np == 16) return rhs_resident<16>(p, lds);
     if (np == 32) return rhs_resident<32>(p, lds);
     if (np == 48) return rhs_resident<48>(p, lds);

@@ -172,18 +172,19 @@ __global__ __launch_bounds__(256) void k_kraus(int64_t n, int64_t n_true, dqh_di
 
 // LDS-tiled form of k_kraus (all offsets within the halo, n % 64 == 0): one
 // workgroup per 64 x 64 tile of one instance; X = rho32 + E staged with its halo.
+template <int HALO>
 __global__ __launch_bounds__(diat::kThreads) void k_kraus_tiled(int64_t n, int64_t n_true, dqh_dia_ops ops,
                                                                const float2 *__restrict__ r32_in,
                                                                const float2 *__restrict__ E, double2 *__restrict__ r64,
                                                                float2 *__restrict__ r32_out,
                                                                float2 *__restrict__ states, int64_t nsave,
                                                                int64_t save_idx, double theta) {
-  __shared__ float2 sx[diat::kXR * diat::kXC];
+  __shared__ float2 sx[diat::Tile<HALO>::NX];
   const size_t nn = (size_t)n * n;
   const size_t inst = blockIdx.y;
   int i0, j0;
   diat::tile_of(n, i0, j0);
-  diat::load_halo(sx, r32_in + inst * nn, E + inst * nn, n, i0, j0);
+  diat::load_halo<HALO>(sx, r32_in + inst * nn, E + inst * nn, n, i0, j0);
   __syncthreads();
   const int jj = threadIdx.x & (diat::kTJ - 1);
   const float2 *Ei = E + inst * nn;
@@ -192,8 +193,7 @@ __global__ __launch_bounds__(diat::kThreads) void k_kraus_tiled(int64_t n, int64
   for (int m = 0; m < diat::kRowsPerThread; ++m) {
     const int ii = (threadIdx.x >> 6) + (diat::kThreads / diat::kTJ) * m;
     const int i = i0 + ii, j = j0 + jj;
-    const diat::LdsX X{sx + (ii + diat::kHalo) * diat::kXC + (jj + diat::kHalo)};
-    const float2 acc = diat::stencil(X, ops, true, n, i, j);
+    const float2 acc = diat::stencil(diat::LdsX<HALO>::at(sx, ii, jj), ops, true, n, i, j);
     const size_t e = (size_t)i * n + j;
     const float2 eij = Ei[e];
     const double dr = (double)eij.x + acc.x, di = (double)eij.y + acc.y;
@@ -275,12 +275,11 @@ extern "C" int dqh_mesolve_rouchon1_dia(const dqh_dia_rouchon1 *pb, void *stream
   const dim3 gel((unsigned)((nn + 255) / 256), (unsigned)p.ninst);
   const dim3 gsolve((unsigned)(n / dia::kNB), (unsigned)p.ninst);
   const dim3 gtile((unsigned)((n / diat::kTI) * (n / diat::kTJ)), (unsigned)p.ninst);
-  int halo = 0;
-  for (int s = 0; s < p.nsets; ++s) halo = std::max(halo, diat::max_abs_offset(p.ops[s]));
-  const bool tiled_ok = halo <= diat::kHalo;  // n is a multiple of 64 here
-  if (p.variant == DQH_SWEEP_TILED && !tiled_ok)
-    return fail(DQH_BAD_ARG, "dqh_mesolve_rouchon1_dia: the tiled sweep needs every |offset| <= 32");
-  const bool tiled = p.variant == DQH_SWEEP_TILED || (p.variant == DQH_SWEEP_AUTO && tiled_ok);
+  int maxoff = 0;
+  for (int s = 0; s < p.nsets; ++s) maxoff = std::max(maxoff, diat::max_abs_offset(p.ops[s]));
+  const int halo = diat::resolve_halo(p.variant, n, maxoff);
+  if (halo < 0)
+    return fail(DQH_BAD_ARG, "dqh_mesolve_rouchon1_dia: the tiled sweep needs every |offset| <= its halo");
   int cur = 0;
   hipLaunchKernelGGL(dia::k_round, dim3((unsigned)((p.ninst * nn + 255) / 256)), dim3(256), 0, st, p.ninst * nn, r64,
                      r32[0]);
@@ -305,9 +304,15 @@ extern "C" int dqh_mesolve_rouchon1_dia(const dqh_dia_rouchon1 *pb, void *stream
       th = hs_theta[isave];
       ++isave;
     }
-    if (tiled)
-      hipLaunchKernelGGL(dia::k_kraus_tiled, gtile, dim3(diat::kThreads), 0, st, n, p.n_true, p.ops[set], r32[cur], E,
-                         r64, r32[1 - cur], states, p.nsave, sidx, th);
+    if (halo == 8)
+      hipLaunchKernelGGL(dia::k_kraus_tiled<8>, gtile, dim3(diat::kThreads), 0, st, n, p.n_true, p.ops[set], r32[cur],
+                         E, r64, r32[1 - cur], states, p.nsave, sidx, th);
+    else if (halo == 16)
+      hipLaunchKernelGGL(dia::k_kraus_tiled<16>, gtile, dim3(diat::kThreads), 0, st, n, p.n_true, p.ops[set], r32[cur],
+                         E, r64, r32[1 - cur], states, p.nsave, sidx, th);
+    else if (halo == 32)
+      hipLaunchKernelGGL(dia::k_kraus_tiled<32>, gtile, dim3(diat::kThreads), 0, st, n, p.n_true, p.ops[set], r32[cur],
+                         E, r64, r32[1 - cur], states, p.nsave, sidx, th);
     else
       hipLaunchKernelGGL(dia::k_kraus, gel, dim3(256), 0, st, n, p.n_true, p.ops[set], r32[cur], E, r64,
                          r32[1 - cur], states, p.nsave, sidx, th);

@@ -7,8 +7,9 @@
 // which is the reference's tmp + tmp^dag with tmp = Hnh rho + 1/2 sum L rho L^dag,
 // evaluated element by element. Every output element reads a handful of
 // neighbours of rho, so the sweep is bound by HBM: rho is read once and drho
-// written once. The tiled form stages a 64 x 64 tile of rho with a 32-wide halo
-// in LDS; the global form reads neighbours through the caches (any offsets).
+// written once. The tiled forms stage a 64 x 64 tile of rho with an 8, 16 or
+// 32-wide halo in LDS; the global form reads neighbours through the caches (any
+// offsets).
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -20,14 +21,15 @@
 namespace dqh {
 namespace diar {
 
+template <int HALO>
 __global__ __launch_bounds__(diat::kThreads) void k_rhs_tiled(int64_t n, dqh_dia_ops ops, const float2 *__restrict__ rho,
                                                              float2 *__restrict__ drho) {
-  __shared__ float2 sx[diat::kXR * diat::kXC];
+  __shared__ float2 sx[diat::Tile<HALO>::NX];
   const size_t nn = (size_t)n * n;
   const size_t inst = blockIdx.y;
   int i0, j0;
   diat::tile_of(n, i0, j0);
-  diat::load_halo(sx, rho + inst * nn, nullptr, n, i0, j0);
+  diat::load_halo<HALO>(sx, rho + inst * nn, nullptr, n, i0, j0);
   __syncthreads();
   const int jj = threadIdx.x & (diat::kTJ - 1);
   float2 *out = drho + inst * nn;
@@ -35,8 +37,7 @@ __global__ __launch_bounds__(diat::kThreads) void k_rhs_tiled(int64_t n, dqh_dia
   for (int m = 0; m < diat::kRowsPerThread; ++m) {
     const int ii = (threadIdx.x >> 6) + (diat::kThreads / diat::kTJ) * m;
     const int i = i0 + ii, j = j0 + jj;
-    const diat::LdsX X{sx + (ii + diat::kHalo) * diat::kXC + (jj + diat::kHalo)};
-    out[(size_t)i * n + j] = diat::stencil(X, ops, false, n, i, j);
+    out[(size_t)i * n + j] = diat::stencil(diat::LdsX<HALO>::at(sx, ii, jj), ops, false, n, i, j);
   }
 }
 
@@ -72,20 +73,22 @@ extern "C" int dqh_lindblad_rhs_dia(const dqh_dia_rhs *pb, void *stream) {
       if (p.ops.off[k][u] <= -p.n || p.ops.off[k][u] >= p.n)
         return fail(DQH_BAD_ARG, "dqh_lindblad_rhs_dia: offset outside (-n, n)");
   }
-  const bool tiled_ok = p.n % diat::kTI == 0 && diat::max_abs_offset(p.ops) <= diat::kHalo;
-  if (p.variant == DQH_SWEEP_TILED && !tiled_ok)
-    return fail(DQH_BAD_ARG, "dqh_lindblad_rhs_dia: the tiled sweep needs n % 64 == 0 and every |offset| <= 32");
-  const bool tiled = p.variant == DQH_SWEEP_TILED || (p.variant == DQH_SWEEP_AUTO && tiled_ok);
+  const int halo = diat::resolve_halo(p.variant, p.n, diat::max_abs_offset(p.ops));
+  if (halo < 0)
+    return fail(DQH_BAD_ARG, "dqh_lindblad_rhs_dia: the tiled sweep needs n % 64 == 0 and every |offset| <= its halo");
   hipStream_t st = (hipStream_t)stream;
   const float2 *rho = reinterpret_cast<const float2 *>(p.rho);
   float2 *drho = reinterpret_cast<float2 *>(p.drho);
-  if (tiled) {
-    const dim3 grid((unsigned)((p.n / diat::kTI) * (p.n / diat::kTJ)), (unsigned)p.ninst);
-    hipLaunchKernelGGL(diar::k_rhs_tiled, grid, dim3(diat::kThreads), 0, st, p.n, p.ops, rho, drho);
-  } else {
-    const dim3 grid((unsigned)((p.n * p.n + 255) / 256), (unsigned)p.ninst);
-    hipLaunchKernelGGL(diar::k_rhs_global, grid, dim3(256), 0, st, p.n, p.ops, rho, drho);
-  }
+  const dim3 gtile((unsigned)((p.n / diat::kTI) * (p.n / diat::kTJ)), (unsigned)p.ninst);
+  if (halo == 8)
+    hipLaunchKernelGGL(diar::k_rhs_tiled<8>, gtile, dim3(diat::kThreads), 0, st, p.n, p.ops, rho, drho);
+  else if (halo == 16)
+    hipLaunchKernelGGL(diar::k_rhs_tiled<16>, gtile, dim3(diat::kThreads), 0, st, p.n, p.ops, rho, drho);
+  else if (halo == 32)
+    hipLaunchKernelGGL(diar::k_rhs_tiled<32>, gtile, dim3(diat::kThreads), 0, st, p.n, p.ops, rho, drho);
+  else
+    hipLaunchKernelGGL(diar::k_rhs_global, dim3((unsigned)((p.n * p.n + 255) / 256), (unsigned)p.ninst), dim3(256), 0,
+                       st, p.n, p.ops, rho, drho);
   const hipError_t err = hipGetLastError();
   if (err != hipSuccess) return fail(DQH_HIP_ERROR, std::string("dqh_lindblad_rhs_dia: ") + hipGetErrorString(err));
   return DQH_OK;

@@ -8,9 +8,9 @@
 //   (X A^dag)[i][j]   = sum_u X[i][j + p_u] conj(A[j][j + p_u])
 //   (A X A^dag)[i][j] = sum_u A[i][i + p_u] sum_v X[i + p_u][j + p_v] conj(A[j][j + p_v]).
 // One workgroup owns a kTI x kTJ tile of the output. It stages X over the tile
-// grown by kHalo on every side in LDS (zero outside the matrix), so every
+// grown by HALO on every side in LDS (zero outside the matrix), so every
 // stencil read hits LDS and X crosses HBM about once. Offsets must satisfy
-// |p| <= kHalo. The diagonal values are read from global memory (a few KB per
+// |p| <= HALO. The diagonal values are read from global memory (a few KB per
 // operator, cached): the row-side value is the same for all lanes of a wave,
 // the column-side one is coalesced.
 #pragma once
@@ -27,12 +27,19 @@ namespace diat {
 
 constexpr int kTI = 64;
 constexpr int kTJ = 64;
-constexpr int kHalo = 32;
-constexpr int kXR = kTI + 2 * kHalo;  // 128 staged rows
-constexpr int kXC = kTJ + 2 * kHalo;  // 128 staged columns (1 KiB per row)
 constexpr int kThreads = 512;
 constexpr int kRowsPerThread = kTI * kTJ / kThreads;  // 8
-constexpr size_t kLdsBytes = (size_t)kXR * kXC * sizeof(float2);  // 128 KiB
+
+// Staged tile of a sweep whose offsets satisfy |p| <= HALO: (64 + 2 HALO)^2
+// complex64 in LDS (HALO 8: 50 KiB, 3 workgroups per CU; 16: 72 KiB, 2 per CU;
+// 32: 128 KiB, 1 per CU). Smaller halos re-read less of X from L2 and keep more
+// waves in flight; the autotuner picks among the ones a problem admits.
+template <int HALO>
+struct Tile {
+  static constexpr int XR = kTI + 2 * HALO;
+  static constexpr int XC = kTJ + 2 * HALO;
+  static constexpr int NX = XR * XC;
+};
 
 // Largest |offset| over the operators of a description.
 inline int max_abs_offset(const dqh_dia_ops &ops) {
@@ -40,6 +47,27 @@ inline int max_abs_offset(const dqh_dia_ops &ops) {
   for (int s = 0; s < ops.nops; ++s)
     for (int u = 0; u < ops.noff[s]; ++u) m = std::max(m, std::abs(ops.off[s][u]));
   return m;
+}
+
+// Sweep form of a DQH_SWEEP_* request: the LDS halo (8, 16, 32), 0 for the
+// global form, -1 when a tiled form was asked for and the problem does not admit it.
+inline int resolve_halo(int64_t variant, int64_t n, int maxoff) {
+  const bool tileable = n % kTI == 0;
+  auto fits = [&](int h) { return tileable && maxoff <= h; };
+  switch (variant) {
+    case DQH_SWEEP_GLOBAL:
+      return 0;
+    case DQH_SWEEP_TILED_H8:
+      return fits(8) ? 8 : -1;
+    case DQH_SWEEP_TILED_H16:
+      return fits(16) ? 16 : -1;
+    case DQH_SWEEP_TILED_H32:
+      return fits(32) ? 32 : -1;
+    case DQH_SWEEP_TILED:
+      return fits(8) ? 8 : fits(16) ? 16 : fits(32) ? 32 : -1;
+    default:  // DQH_SWEEP_AUTO
+      return fits(8) ? 8 : fits(16) ? 16 : fits(32) ? 32 : 0;
+  }
 }
 
 template <typename T>
@@ -59,12 +87,14 @@ __device__ __forceinline__ void tile_of(int64_t n, int &i0, int &j0) {
   j0 = (b % tpr) * kTJ;
 }
 
-// sx[r][c] = A[i0 - kHalo + r][j0 - kHalo + c] (+ B[...]), zero outside [0, n)^2.
+// sx[r][c] = A[i0 - HALO + r][j0 - HALO + c] (+ B[...]), zero outside [0, n)^2.
+template <int HALO>
 __device__ __forceinline__ void load_halo(float2 *sx, const float2 *__restrict__ A, const float2 *__restrict__ B,
                                           int64_t n, int i0, int j0) {
-  for (int e = threadIdx.x; e < kXR * kXC; e += kThreads) {
-    const int r = e / kXC, c = e % kXC;
-    const int64_t gi = (int64_t)i0 - kHalo + r, gj = (int64_t)j0 - kHalo + c;
+  using T = Tile<HALO>;
+  for (int e = threadIdx.x; e < T::NX; e += kThreads) {
+    const int r = e / T::XC, c = e % T::XC;
+    const int64_t gi = (int64_t)i0 - HALO + r, gj = (int64_t)j0 - HALO + c;
     float2 v = make_float2(0.f, 0.f);
     if (gi >= 0 && gi < n && gj >= 0 && gj < n) {
       const size_t o = (size_t)gi * n + gj;
@@ -82,9 +112,13 @@ __device__ __forceinline__ void load_halo(float2 *sx, const float2 *__restrict__
 __device__ __forceinline__ int clampi(int64_t a, int64_t n) { return (int)(a < 0 ? 0 : (a >= n ? n - 1 : a)); }
 
 // X read from the LDS halo tile around element (ii, jj) of the tile
+template <int HALO>
 struct LdsX {
   const float2 *c;
-  __device__ __forceinline__ float2 operator()(int p, int q) const { return c[p * kXC + q]; }
+  __device__ __forceinline__ float2 operator()(int p, int q) const { return c[p * Tile<HALO>::XC + q]; }
+  static __device__ __forceinline__ LdsX at(const float2 *sx, int ii, int jj) {
+    return LdsX{sx + (ii + HALO) * Tile<HALO>::XC + (jj + HALO)};
+  }
 };
 
 // X = A (+ B) read from global memory with bounds checks (any offsets)

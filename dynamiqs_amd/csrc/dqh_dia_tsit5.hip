@@ -93,24 +93,24 @@ __device__ __forceinline__ double reduce_partials(const double *p, int64_t npart
 
 __device__ __forceinline__ double sq(double2 v) { return v.x * v.x + v.y * v.y; }
 
-// Visit the output elements of this workgroup: the tile (TILED, 512 threads, the
-// stage input staged in LDS) or one element per thread (256 threads, neighbours
-// read through the caches). fn(i, j, f) receives the vector field f at (i, j).
-template <bool TILED, class Fn>
+// Visit the output elements of this workgroup: the tile (HALO > 0, 512 threads,
+// the stage input staged in LDS with that halo) or one element per thread
+// (HALO == 0, 256 threads, neighbours read through the caches). fn(i, j, f)
+// receives the vector field f at (i, j).
+template <int HALO, class Fn>
 __device__ __forceinline__ void sweep(const V &v, const dqh_dia_ops &ops, const float2 *in, float2 *sx, Fn fn) {
   const int64_t n = v.n;
-  if constexpr (TILED) {
+  if constexpr (HALO > 0) {
     int i0, j0;
     diat::tile_of(n, i0, j0);
-    diat::load_halo(sx, in, nullptr, n, i0, j0);
+    diat::load_halo<HALO>(sx, in, nullptr, n, i0, j0);
     __syncthreads();
     const int jj = threadIdx.x & (diat::kTJ - 1);
 #pragma unroll 1
     for (int m = 0; m < diat::kRowsPerThread; ++m) {
       const int ii = (threadIdx.x >> 6) + (diat::kThreads / diat::kTJ) * m;
       const int i = i0 + ii, j = j0 + jj;
-      const diat::LdsX X{sx + (ii + diat::kHalo) * diat::kXC + (jj + diat::kHalo)};
-      fn(i, j, diat::stencil(X, ops, false, n, i, j));
+      fn(i, j, diat::stencil(diat::LdsX<HALO>::at(sx, ii, jj), ops, false, n, i, j));
     }
   } else {
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -122,19 +122,19 @@ __device__ __forceinline__ void sweep(const V &v, const dqh_dia_ops &ops, const 
   }
 }
 
-template <bool TILED>
+template <int HALO>
 constexpr int nthreads() {
-  return TILED ? diat::kThreads : 256;
+  return HALO > 0 ? diat::kThreads : 256;
 }
 
 // ------------------------------------------------------------------ one step
-template <bool TILED>
-__global__ __launch_bounds__(nthreads<TILED>()) void k_stage(V v, dqh_dia_ops ops, int s, int cur) {
-  constexpr int NT = nthreads<TILED>();
+template <int HALO>
+__global__ __launch_bounds__(nthreads<HALO>()) void k_stage(V v, dqh_dia_ops ops, int s, int cur) {
+  constexpr int NT = nthreads<HALO>();
   __shared__ double sh[NT / 64];
   float2 *sx = nullptr;
-  if constexpr (TILED) {
-    __shared__ float2 lds[diat::kXR * diat::kXC];
+  if constexpr (HALO > 0) {
+    __shared__ float2 lds[diat::Tile<HALO>::NX];
     sx = lds;
   }
   const size_t inst = blockIdx.y;
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(nthreads<TILED>()) void k_stage(V v, dqh_dia_ops op
   float2 *ks = kslot(v, s, c.rot, inst);
   float2 *nxt = ysbuf(v, s & 1, inst);
   double acc = 0.0;
-  sweep<TILED>(v, ops, ysbuf(v, (s + 1) & 1, inst), sx, [&](int i, int j, float2 f) {
+  sweep<HALO>(v, ops, ysbuf(v, (s + 1) & 1, inst), sx, [&](int i, int j, float2 f) {
     const size_t e = (size_t)i * n + j;
     ks[e] = f;
     const double2 yv = y[e];
@@ -310,13 +310,13 @@ __global__ __launch_bounds__(256) void k_init(V v) {
 // pass 0: k_0 = f(y0), partials |y0|^2 / sc^2 and |k_0|^2 / sc^2;
 // pass 1: f(y0 + h0 k_0), partials |f - k_0|^2 / sc^2 (sc = atol + rtol |y0|).
 // Hairer-Norsett-Wanner II.4 starting step as the dense kernel computes it.
-template <bool TILED>
-__global__ __launch_bounds__(nthreads<TILED>()) void k_init_rhs(V v, dqh_dia_ops ops, int pass) {
-  constexpr int NT = nthreads<TILED>();
+template <int HALO>
+__global__ __launch_bounds__(nthreads<HALO>()) void k_init_rhs(V v, dqh_dia_ops ops, int pass) {
+  constexpr int NT = nthreads<HALO>();
   __shared__ double sh[NT / 64];
   float2 *sx = nullptr;
-  if constexpr (TILED) {
-    __shared__ float2 lds[diat::kXR * diat::kXC];
+  if constexpr (HALO > 0) {
+    __shared__ float2 lds[diat::Tile<HALO>::NX];
     sx = lds;
   }
   const size_t inst = blockIdx.y;
@@ -326,7 +326,7 @@ __global__ __launch_bounds__(nthreads<TILED>()) void k_init_rhs(V v, dqh_dia_ops
   const double2 *y = v.y + inst * nn;
   float2 *k0 = kslot(v, 0, 0, inst);
   double a0 = 0.0, a1 = 0.0;
-  sweep<TILED>(v, ops, ysbuf(v, pass, inst), sx, [&](int i, int j, float2 f) {
+  sweep<HALO>(v, ops, ysbuf(v, pass, inst), sx, [&](int i, int j, float2 f) {
     const size_t e = (size_t)i * n + j;
     const double2 yv = y[e];
     const double sc = v.atol + v.rtol * sqrt(sq(yv));
@@ -457,20 +457,20 @@ extern "C" int64_t dqh_dia_tsit5_work_bytes(int64_t n, int64_t ninst) {
 
 namespace {
 
-template <bool TILED>
+template <int HALO>
 void launch_step(const diats::V &v, const dqh_dia_ops &ops, dim3 gstage, dim3 gflat, int cur, hipStream_t s) {
   for (int sg = 1; sg <= 6; ++sg)
-    hipLaunchKernelGGL(diats::k_stage<TILED>, gstage, dim3(diats::nthreads<TILED>()), 0, s, v, ops, sg, cur);
+    hipLaunchKernelGGL(diats::k_stage<HALO>, gstage, dim3(diats::nthreads<HALO>()), 0, s, v, ops, sg, cur);
   hipLaunchKernelGGL(diats::k_commit, gflat, dim3(256), 0, s, v, cur);
 }
 
-template <bool TILED>
+template <int HALO>
 int drive(const dqh_dia_tsit5 &p, const diats::V &v, hipStream_t st, dim3 gstage, dim3 gflat) {
   const dqh_dia_ops &ops = p.ops;
   hipLaunchKernelGGL(diats::k_init, gflat, dim3(256), 0, st, v);
-  hipLaunchKernelGGL(diats::k_init_rhs<TILED>, gstage, dim3(diats::nthreads<TILED>()), 0, st, v, ops, 0);
+  hipLaunchKernelGGL(diats::k_init_rhs<HALO>, gstage, dim3(diats::nthreads<HALO>()), 0, st, v, ops, 0);
   hipLaunchKernelGGL(diats::k_init_h0, gflat, dim3(256), 0, st, v);
-  hipLaunchKernelGGL(diats::k_init_rhs<TILED>, gstage, dim3(diats::nthreads<TILED>()), 0, st, v, ops, 1);
+  hipLaunchKernelGGL(diats::k_init_rhs<HALO>, gstage, dim3(diats::nthreads<HALO>()), 0, st, v, ops, 1);
   hipLaunchKernelGGL(diats::k_init_dt, gflat, dim3(256), 0, st, v);
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return fail(DQH_HIP_ERROR, std::string("dqh_mesolve_tsit5_dia init: ") + hipGetErrorString(err));
@@ -491,7 +491,7 @@ int drive(const dqh_dia_tsit5 &p, const diats::V &v, hipStream_t st, dim3 gstage
       cleanup();
       return fail(DQH_HIP_ERROR, std::string("dqh_mesolve_tsit5_dia capture: ") + hipGetErrorString(err));
     }
-    for (int64_t g = 0; g < G; ++g) launch_step<TILED>(v, ops, gstage, gflat, (int)(g & 1), cs);
+    for (int64_t g = 0; g < G; ++g) launch_step<HALO>(v, ops, gstage, gflat, (int)(g & 1), cs);
     hipLaunchKernelGGL(diats::k_count, dim3(1), dim3(256), 0, cs, v);
     err = hipStreamEndCapture(cs, &graph);
     if (err == hipSuccess) err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
@@ -515,7 +515,7 @@ int drive(const dqh_dia_tsit5 &p, const diats::V &v, hipStream_t st, dim3 gstage
     if (exec) {
       err = hipGraphLaunch(exec, st);
     } else {
-      for (int64_t g = 0; g < G; ++g) launch_step<TILED>(v, ops, gstage, gflat, (int)(g & 1), st);
+      for (int64_t g = 0; g < G; ++g) launch_step<HALO>(v, ops, gstage, gflat, (int)(g & 1), st);
       hipLaunchKernelGGL(diats::k_count, dim3(1), dim3(256), 0, st, v);
       err = hipGetLastError();
     }
@@ -575,10 +575,9 @@ extern "C" int dqh_mesolve_tsit5_dia(const dqh_dia_tsit5 *pb, void *stream) {
       if (p.ops.off[k][u] <= -p.n || p.ops.off[k][u] >= p.n)
         return fail(DQH_BAD_ARG, "dqh_mesolve_tsit5_dia: offset outside (-n, n)");
   }
-  const bool tiled_ok = p.n % diat::kTI == 0 && diat::max_abs_offset(p.ops) <= diat::kHalo;
-  if (p.variant == DQH_SWEEP_TILED && !tiled_ok)
-    return fail(DQH_BAD_ARG, "dqh_mesolve_tsit5_dia: the tiled sweep needs n % 64 == 0 and every |offset| <= 32");
-  const bool tiled = p.variant == DQH_SWEEP_TILED || (p.variant == DQH_SWEEP_AUTO && tiled_ok);
+  const int halo = diat::resolve_halo(p.variant, p.n, diat::max_abs_offset(p.ops));
+  if (halo < 0)
+    return fail(DQH_BAD_ARG, "dqh_mesolve_tsit5_dia: the tiled sweep needs n % 64 == 0 and every |offset| <= its halo");
   const size_t nn = (size_t)p.n * p.n;
   const diats::Layout L = diats::layout(p.n, p.ninst);
   char *w = reinterpret_cast<char *>(p.work);
@@ -607,11 +606,10 @@ extern "C" int dqh_mesolve_tsit5_dia(const dqh_dia_tsit5 *pb, void *stream) {
   v.stats = reinterpret_cast<int64_t *>(p.stats);
   const dim3 gflat((unsigned)((nn + 255) / 256), (unsigned)p.ninst);
   hipStream_t st = (hipStream_t)stream;
-  if (tiled) {
-    const dim3 gstage((unsigned)((p.n / diat::kTI) * (p.n / diat::kTJ)), (unsigned)p.ninst);
-    v.npart = gstage.x;
-    return drive<true>(p, v, st, gstage, gflat);
-  }
-  v.npart = gflat.x;
-  return drive<false>(p, v, st, gflat, gflat);
+  const dim3 gtile((unsigned)((p.n / diat::kTI) * (p.n / diat::kTJ)), (unsigned)p.ninst);
+  v.npart = halo > 0 ? gtile.x : gflat.x;
+  if (halo == 8) return drive<8>(p, v, st, gtile, gflat);
+  if (halo == 16) return drive<16>(p, v, st, gtile, gflat);
+  if (halo == 32) return drive<32>(p, v, st, gtile, gflat);
+  return drive<0>(p, v, st, gflat, gflat);
 }

@@ -267,8 +267,13 @@ def build_plan(
     dt: float | None = None,
     normalize: bool = True,
     save_states: bool = True,
+    form: str = 'auto',
 ) -> DensePlan:
-    """Pack ``instances`` (same Hilbert dimension) for ``kind`` in {'tsit5', 'rouchon1', 'rhs'}."""
+    """Pack ``instances`` (same Hilbert dimension) for ``kind`` in {'tsit5', 'rouchon1', 'rhs'}.
+
+    ``form`` picks the kernel form of the Tsit5 / RHS kernels: 'resident' (stages
+    and rho in registers / LDS, n <= 64 and at most 3 jump operators), 'global'
+    (stages in global memory) or 'auto' (the autotuner's choice for (n, K))."""
     import torch
 
     dev = _device()
@@ -297,6 +302,13 @@ def build_plan(
     K = len(instances[0].Ls)
     if any(len(inst.Ls) != K for inst in instances):
         raise ValueError('all instances of a dense plan must have the same number of jump operators')
+    if form not in ('auto', 'resident', 'global'):
+        raise ValueError("form must be 'auto', 'resident' or 'global'")
+    use_global = form == 'global'
+    if form == 'auto' and kind in ('tsit5', 'rhs'):
+        from .. import autotune
+
+        use_global = autotune.dense_global(npad, K, B)
     descs = np.zeros(B, dtype=_native.ME_INST_DTYPE)
 
     p_tsave = arena.put(tsave)
@@ -344,6 +356,8 @@ def build_plan(
             Ls = inst.Ls
             d['nL'] = K
             d['flags'] |= _native.FLAG_HAS_HNH
+            if use_global:
+                d['flags'] |= _native.FLAG_GLOBAL
 
             def bH(inst=inst):
                 if inst.Hnh is not None:

@@ -60,7 +60,8 @@ class DiaRouchon1(ctypes.Structure):
     ]
 
 
-SWEEP_AUTO, SWEEP_GLOBAL, SWEEP_TILED = 0, 1, 2
+SWEEP_AUTO, SWEEP_GLOBAL, SWEEP_TILED, SWEEP_TILED_H8, SWEEP_TILED_H16, SWEEP_TILED_H32 = 0, 1, 2, 3, 4, 5
+SWEEP_HALO = {SWEEP_TILED_H8: 8, SWEEP_TILED_H16: 16, SWEEP_TILED_H32: 32}
 
 
 class DiaRhs(ctypes.Structure):
@@ -205,6 +206,10 @@ def build_plan(
     arena = _Arena(dev)
     prob = DiaRouchon1()
     prob.n, prob.n_true, prob.nb, prob.ninst = n_pad, n, NB, B
+    if variant == SWEEP_AUTO:
+        from .. import autotune
+
+        variant = autotune.dia_sweep(H, Ls, B)
     prob.nsteps, prob.nsets, prob.variant = nsteps, len(sets), variant
     for si, h in enumerate(sets):
         Vp, Wn, ops = block_solve_operators(H, Ls, h, n_pad)
@@ -401,6 +406,10 @@ def tsit5_plan(
     n_pad = NB * ((n + NB - 1) // NB)
     LdL = sum((L.conj().T @ L for L in Ls), np.zeros_like(H))
     Hnh = -1j * H - 0.5 * LdL
+    if variant == SWEEP_AUTO:
+        from .. import autotune
+
+        variant = autotune.dia_sweep(H, Ls, B)
     arena = _Arena(dev)
     prob = DiaTsit5()
     prob.n, prob.n_true, prob.ninst, prob.nsave = n_pad, n, B, len(tsave)

@@ -72,6 +72,7 @@ typedef struct dqh_me_inst {
 
 #define DQH_FLAG_SAVE_STATES 1
 #define DQH_FLAG_HAS_HNH 2
+#define DQH_FLAG_GLOBAL 4 /* run the global-memory kernel form even where the LDS-resident one applies */
 
 /* Size checks for FFI mirrors. */
 int64_t dqh_sizeof_me_inst(void);
@@ -94,12 +95,17 @@ int dqh_mesolve_tsit5_dense(const dqh_me_inst *d_insts, int64_t ninst, void *str
 int dqh_mesolve_rouchon1_dense(const dqh_me_inst *d_insts, int64_t ninst, void *stream);
 
 /* ------------------------------------------------------------------ sparse DIA */
-/* Forms of the DIA sweeps: AUTO picks TILED when every |offset| <= 32 and n is a
- * multiple of 64 (64 x 64 output tiles staged in LDS with a 32-wide halo), else
- * GLOBAL (one thread per element, neighbours read through the caches). */
+/* Forms of the DIA sweeps. TILED_Hh: 64 x 64 output tiles per workgroup, the input
+ * staged in LDS with an h-wide halo (needs n % 64 == 0 and every |offset| <= h).
+ * GLOBAL: one thread per element, neighbours read through the caches (any
+ * problem). TILED: the smallest halo that fits. AUTO: TILED when it fits, else
+ * GLOBAL. */
 #define DQH_SWEEP_AUTO 0
 #define DQH_SWEEP_GLOBAL 1
 #define DQH_SWEEP_TILED 2
+#define DQH_SWEEP_TILED_H8 3
+#define DQH_SWEEP_TILED_H16 4
+#define DQH_SWEEP_TILED_H32 5
 
 #define DQH_DIA_MAXOPS 8
 #define DQH_DIA_MAXOFF 16

@@ -259,3 +259,25 @@ def test_mesolve_routes_dia_tsit5():
     assert np.linalg.norm(res.states.to_numpy() - ref['states'], axis=(-2, -1)).max() <= TOL
     assert np.abs(res.expects[0] - ref['expects'][0]).max() <= 1e-4
     assert int(res.infos.nsteps) > 0
+
+
+def test_autotuner_times_and_caches_dia_and_dense_forms(tmp_path, monkeypatch):
+    """The autotuner times every admissible form on the GPU, keeps the fastest and
+    answers the next request for the same key from its cache."""
+    import importlib
+
+    monkeypatch.setenv('DQH_AUTOTUNE_CACHE', str(tmp_path / 'tune.json'))
+    from dynamiqs_amd import autotune
+    from dynamiqs_amd.integrators import dia
+
+    importlib.reload(autotune)
+    H, Ls, _ = _two_mode(12, 12)
+    v = autotune.dia_sweep(H, Ls, 2)
+    assert v in (dia.SWEEP_GLOBAL, dia.SWEEP_TILED_H16, dia.SWEEP_TILED_H32)
+    entries = [e for k, e in autotune._memo.items() if k.startswith('dia|')]
+    assert len(entries) == 1 and len(entries[0]['us']) == 3
+    assert autotune.dia_sweep(H, Ls, 2) == v
+    g = autotune.dense_global(32, 2, 64)
+    assert isinstance(g, bool)
+    assert any(k.startswith('dense|') for k in autotune._memo)
+    importlib.reload(autotune)

@@ -35,7 +35,7 @@ def test_descriptors_point_into_plan_allocations(host_device, kind, K, pwc):
     rng = np.random.default_rng(0)
     n = 20
     insts = [_inst(rng, n, K, pwc) for _ in range(3)]
-    plan = dense.build_plan(kind, insts, np.array([0.0, 0.5, 1.0]), dt=0.1 if kind == 'rouchon1' else None)
+    plan = dense.build_plan(kind, insts, np.array([0.0, 0.5, 1.0]), dt=0.1 if kind == 'rouchon1' else None, form='resident')
     tensors = [*plan.arena.tensors, plan.states, plan.final, plan.expects, plan.stats]
     dense.check_descriptors(plan.descs, tensors)
     if kind in ('rhs', 'tsit5'):
@@ -51,7 +51,8 @@ def test_descriptors_point_into_plan_allocations(host_device, kind, K, pwc):
 
 def test_guard_rejects_unpatched_descriptor(host_device):
     rng = np.random.default_rng(1)
-    plan = dense.build_plan('rhs', [_inst(rng, 8, 1)], np.array([0.0, 1.0]))
+    plan = dense.build_plan('rhs', [_inst(rng, 8, 1)], np.array([0.0, 1.0]), form='global')
+    assert plan.descs['flags'][0] & 4  # the global kernel form is requested in the descriptor
     descs = plan.descs.copy()
     descs['Hnh'] = 0x101  # a raw staging offset, never a device address
     with pytest.raises(RuntimeError, match='outside the plan allocations'):
