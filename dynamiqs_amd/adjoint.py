@@ -31,6 +31,7 @@ function rather than through ``jax.custom_vjp``.
 from __future__ import annotations
 
 import ctypes
+from dataclasses import dataclass
 
 import numpy as np
 
@@ -58,7 +59,33 @@ def _check_status(name: str, plan) -> None:
         )
 
 
-def mesolve_pwc_grad(
+@dataclass
+class AdjointResult:
+    """``J`` (B,): loss ``Tr(O rho(T))``; ``grad_u`` (B, nterms, nseg): dJ/du;
+    ``grad_rates`` (B, nrates): dJ/dkappa_r for jump operators sqrt(kappa_r) A_r;
+    ``lambda0`` (B, n, n): the adjoint state at the start time, dJ/drho0 (J is linear
+    in rho0: J = Tr(lambda0 rho0)); ``timing``: seconds per phase."""
+
+    J: np.ndarray
+    grad_u: np.ndarray
+    grad_rates: np.ndarray | None
+    lambda0: np.ndarray
+    timing: dict
+
+
+def mesolve_pwc_grad(H0, ops, times, u, Ls, rho0, O, *, return_timing: bool = False, **kw):
+    """Loss ``J_b = Tr(O rho_b(T))`` and gradient ``dJ_b/du_b`` for a batch of control
+    arrays ``u`` of shape ``(B, nterms, nseg)`` (``T = times[-1]``).
+
+    Returns ``(J (B,), grad (B, nterms, nseg))`` (and a timing dict if requested).
+    """
+    r = mesolve_adjoint(H0, ops, times, u, Ls, rho0, O, **kw)
+    if return_timing:
+        return r.J, r.grad_u, r.timing
+    return r.J, r.grad_u
+
+
+def mesolve_adjoint(
     H0: np.ndarray,
     ops,
     times: np.ndarray,
@@ -67,16 +94,19 @@ def mesolve_pwc_grad(
     rho0: np.ndarray,
     O: np.ndarray,
     *,
+    rate_ops=(),
     rtol: float = 1e-6,
     atol: float = 1e-6,
     checkpoint_every: int = 10,
     nodes: int = 5,
-    return_timing: bool = False,
-):
-    """Loss ``J_b = Tr(O rho_b(T))`` and gradient ``dJ_b/du_b`` for a batch of control
-    arrays ``u`` of shape ``(B, nterms, nseg)`` (``T = times[-1]``).
+) -> AdjointResult:
+    """Loss, control gradient, jump-rate gradients and adjoint initial state of
+    ``J = Tr(O rho(T))`` for a batch of control arrays ``u`` (B, nterms, nseg).
 
-    Returns ``(J (B,), grad (B, nterms, nseg))`` (and a timing dict if requested).
+    ``rate_ops``: operators A_r whose rates are differentiated; the corresponding
+    jump operators in ``Ls`` are sqrt(kappa_r) A_r (dJ/dkappa_r is summed over the
+    whole interval). ``ops`` may be empty (then ``u`` has shape (B, 0, nseg) and
+    ``times`` only sets the quadrature segments).
     """
     import time
 
@@ -93,6 +123,9 @@ def mesolve_pwc_grad(
         raise ValueError('u must have shape (B, len(ops), len(times) - 1)')
     if nterms > 8:
         raise ValueError('at most 8 control operators')
+    rate_ops = [np.asarray(A, dtype=np.complex128) for A in rate_ops]
+    if len(rate_ops) > 8:
+        raise ValueError('at most 8 rate operators')
     Ls = [np.asarray(L, dtype=np.complex128) for L in Ls]
     rho0 = np.asarray(rho0, dtype=np.complex128)
     if rho0.shape[-1] == 1:
@@ -178,7 +211,8 @@ def mesolve_pwc_grad(
         )
         for b in range(B)
     ]
-    bwd_ts = np.concatenate([[0.0], s_nodes])
+    # saves: s = 0 (lambda = O), the nodes, and s = T - t0 (lambda at the start time)
+    bwd_ts = np.concatenate([[0.0], s_nodes, [T - t_start]])
     # lambda starts at O: scale the absolute tolerance by |O| so that the backward
     # solve is held to the same relative accuracy as the forward one (|rho| <= 1)
     atol_b = atol * max(1.0, float(np.linalg.norm(O, 2)))
@@ -193,39 +227,45 @@ def mesolve_pwc_grad(
     for name, plan in (('forward', fwd), ('recompute', rec), ('backward', bwd)):
         _check_status(name, plan)
     # lambda at the nodes in increasing t, kept on the device
-    lam_dev = torch.flip(bwd.states.view(B, 1 + Q, n * n * 2)[:, 1:], dims=[1]).contiguous()
+    bst = bwd.states.view(B, 2 + Q, n * n * 2)
+    lam_dev = torch.flip(bst[:, 1 : 1 + Q], dims=[1]).contiguous()
+    lambda0 = bst[:, 1 + Q].cpu().numpy().view(np.complex64).reshape(B, n, n)
 
     # ------------------------------------------------------------- quadrature
     dev = torch.device('cuda', torch.cuda.current_device())
-    r_t = rho_nodes_dev
-    l_t = lam_dev
-    o_t = torch.from_numpy(np.ascontiguousarray(np.stack(ops).astype(np.complex64)).view(np.float32)).to(dev)
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
     w_t = torch.from_numpy(node_w).to(dev)
     sf_t = torch.from_numpy(seg_first).to(dev)
-    g_t = torch.zeros((B, nterms, nseg), dtype=torch.float64, device=dev)
     lib = _native.lib()
+    P = ctypes.c_void_p
     t9 = time.perf_counter()
-    _native.check(
-        lib.dqh_adjoint_pwc_quadrature(
-            B,
-            n,
-            Q,
-            nterms,
-            nseg,
-            ctypes.c_void_p(r_t.data_ptr()),
-            ctypes.c_void_p(l_t.data_ptr()),
-            ctypes.c_void_p(o_t.data_ptr()),
-            ctypes.c_void_p(w_t.data_ptr()),
-            ctypes.c_void_p(sf_t.data_ptr()),
-            ctypes.c_void_p(g_t.data_ptr()),
-            ctypes.c_void_p(torch.cuda.current_stream().cuda_stream),
-        ),
-        'dqh_adjoint_pwc_quadrature',
-    )
+    grad = np.zeros((B, nterms, nseg))
+    if nterms:
+        o_t = torch.from_numpy(np.ascontiguousarray(np.stack(ops).astype(np.complex64)).view(np.float32)).to(dev)
+        g_t = torch.zeros((B, nterms, nseg), dtype=torch.float64, device=dev)
+        _native.check(
+            lib.dqh_adjoint_pwc_quadrature(
+                B, n, Q, nterms, nseg, P(rho_nodes_dev.data_ptr()), P(lam_dev.data_ptr()), P(o_t.data_ptr()),
+                P(w_t.data_ptr()), P(sf_t.data_ptr()), P(g_t.data_ptr()), stream,
+            ),
+            'dqh_adjoint_pwc_quadrature',
+        )
+        grad = g_t.cpu().numpy()
+    grad_rates = None
+    if rate_ops:
+        c64 = lambda xs: torch.from_numpy(np.ascontiguousarray(np.stack(xs).astype(np.complex64)).view(np.float32)).to(dev)  # noqa: E731
+        a_t = c64(rate_ops)
+        n_t = c64([A.conj().T @ A for A in rate_ops])
+        gr_t = torch.zeros((B, len(rate_ops), nseg), dtype=torch.float64, device=dev)
+        _native.check(
+            lib.dqh_adjoint_rate_quadrature(
+                B, n, Q, len(rate_ops), nseg, P(rho_nodes_dev.data_ptr()), P(lam_dev.data_ptr()), P(a_t.data_ptr()),
+                P(n_t.data_ptr()), P(w_t.data_ptr()), P(sf_t.data_ptr()), P(gr_t.data_ptr()), stream,
+            ),
+            'dqh_adjoint_rate_quadrature',
+        )
+        grad_rates = gr_t.cpu().numpy().sum(axis=-1)
     torch.cuda.synchronize()
     timing['quadrature_s'] = time.perf_counter() - t9
-    grad = g_t.cpu().numpy()
     timing['forward_nrhs'] = float(np.sum(fres['nrhs']))
-    if return_timing:
-        return J, grad, timing
-    return J, grad
+    return AdjointResult(J=J, grad_u=grad, grad_rates=grad_rates, lambda0=lambda0, timing=timing)

@@ -97,6 +97,82 @@ __global__ __launch_bounds__(kThreads) void k_quad(int n, int nnodes, int nterms
   }
 }
 
+// Gradient with respect to a jump-rate parameter: for L = sqrt(kappa) A,
+// d/dkappa of the Lindbladian is the dissipator D[A], so
+//   dJ/dkappa = int Tr(lam D[A] rho) dt,
+//   Tr(lam D[A] rho) = sum_cb (A^dag lam A - (lam N + N lam) / 2)_cb rho_bc,  N = A^dag A.
+// One workgroup per (instance, segment); lam, A, N and P = lam A sit in LDS
+// (n <= 64), rho is read from global memory once per node; FP64 accumulation.
+__global__ __launch_bounds__(kThreads) void k_rate_quad(int n, int nnodes, int nrates, int nseg,
+                                                        const float2 *__restrict__ rho, const float2 *__restrict__ lam,
+                                                        const float2 *__restrict__ Aops, const float2 *__restrict__ Nops,
+                                                        const double *__restrict__ weights,
+                                                        const int *__restrict__ seg_first, double *__restrict__ grad) {
+  __shared__ float2 sL[kMaxLds * (kMaxLds + 1)];
+  __shared__ float2 sA[kMaxLds * (kMaxLds + 1)];
+  __shared__ float2 sN[kMaxLds * (kMaxLds + 1)];
+  __shared__ float2 sP[kMaxLds * (kMaxLds + 1)];
+  __shared__ double red[kThreads / 64];
+  const int seg = blockIdx.x;
+  const size_t inst = blockIdx.y;
+  const size_t nn = (size_t)n * n;
+  const int ld = kMaxLds + 1;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int r = 0; r < nrates; ++r) {
+    __syncthreads();
+    for (size_t e = threadIdx.x; e < nn; e += kThreads) {
+      sA[(e / n) * ld + e % n] = Aops[(size_t)r * nn + e];
+      sN[(e / n) * ld + e % n] = Nops[(size_t)r * nn + e];
+    }
+    double acc = 0.0;
+    for (int q = seg_first[seg]; q < seg_first[seg + 1]; ++q) {
+      const float2 *R = rho + (inst * nnodes + q) * nn;
+      const float2 *Lm = lam + (inst * nnodes + q) * nn;
+      __syncthreads();
+      for (size_t e = threadIdx.x; e < nn; e += kThreads) sL[(e / n) * ld + e % n] = Lm[e];
+      __syncthreads();
+      for (size_t e = threadIdx.x; e < nn; e += kThreads) {  // P = lam A
+        const int c = (int)(e / n), b = (int)(e % n);
+        float pr = 0.f, pi = 0.f;
+        for (int a = 0; a < n; ++a) {
+          const float2 l = sL[c * ld + a], x = sA[a * ld + b];
+          pr += l.x * x.x - l.y * x.y;
+          pi += l.x * x.y + l.y * x.x;
+        }
+        sP[c * ld + b] = make_float2(pr, pi);
+      }
+      __syncthreads();
+      double tr = 0.0;  // Re Tr(M rho), M = A^dag P - (lam N + N lam) / 2
+      for (size_t e = threadIdx.x; e < nn; e += kThreads) {
+        const int b = (int)(e / n), c = (int)(e % n);  // rho_bc is contiguous in e
+        float mr = 0.f, mi = 0.f;
+        for (int a = 0; a < n; ++a) {
+          const float2 ad = sA[a * ld + c];  // (A^dag)_ca = conj(A_ac)
+          const float2 p = sP[a * ld + b];
+          mr += ad.x * p.x + ad.y * p.y;
+          mi += ad.x * p.y - ad.y * p.x;
+          const float2 l1 = sL[c * ld + a], n1 = sN[a * ld + b];  // (lam N)_cb
+          const float2 n2 = sN[c * ld + a], l2 = sL[a * ld + b];  // (N lam)_cb
+          mr -= 0.5f * ((l1.x * n1.x - l1.y * n1.y) + (n2.x * l2.x - n2.y * l2.y));
+          mi -= 0.5f * ((l1.x * n1.y + l1.y * n1.x) + (n2.x * l2.y + n2.y * l2.x));
+        }
+        const float2 rv = R[e];
+        tr += (double)mr * rv.x - (double)mi * rv.y;
+      }
+      acc += weights[q] * tr;
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    __syncthreads();
+    if (lane == 0) red[wave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double s = 0;
+      for (int w = 0; w < kThreads / 64; ++w) s += red[w];
+      grad[(inst * nrates + r) * nseg + seg] = s;
+    }
+  }
+}
+
 }  // namespace adj
 }  // namespace dqh
 
@@ -114,5 +190,23 @@ extern "C" int dqh_adjoint_pwc_quadrature(int64_t ninst, int64_t n, int64_t nnod
   const hipError_t err = hipGetLastError();
   if (err != hipSuccess)
     return dqh::fail(DQH_HIP_ERROR, std::string("dqh_adjoint_pwc_quadrature: ") + hipGetErrorString(err));
+  return DQH_OK;
+}
+
+extern "C" int dqh_adjoint_rate_quadrature(int64_t ninst, int64_t n, int64_t nnodes, int64_t nrates, int64_t nseg,
+                                           const void *rho_nodes, const void *lam_nodes, const void *Aops,
+                                           const void *Nops, const double *weights, const int32_t *seg_first,
+                                           double *grad, void *stream) {
+  if (ninst <= 0 || n <= 0 || n > dqh::adj::kMaxLds || nnodes <= 0 || nseg <= 0 || nrates <= 0 || nrates > 8 ||
+      !rho_nodes || !lam_nodes || !Aops || !Nops || !weights || !seg_first || !grad)
+    return dqh::fail(DQH_BAD_ARG, "dqh_adjoint_rate_quadrature: bad arguments (n <= 64, 1 <= nrates <= 8)");
+  hipLaunchKernelGGL(dqh::adj::k_rate_quad, dim3((unsigned)nseg, (unsigned)ninst), dim3(dqh::adj::kThreads), 0,
+                     (hipStream_t)stream, (int)n, (int)nnodes, (int)nrates, (int)nseg,
+                     reinterpret_cast<const float2 *>(rho_nodes), reinterpret_cast<const float2 *>(lam_nodes),
+                     reinterpret_cast<const float2 *>(Aops), reinterpret_cast<const float2 *>(Nops), weights,
+                     reinterpret_cast<const int *>(seg_first), grad);
+  const hipError_t err = hipGetLastError();
+  if (err != hipSuccess)
+    return dqh::fail(DQH_HIP_ERROR, std::string("dqh_adjoint_rate_quadrature: ") + hipGetErrorString(err));
   return DQH_OK;
 }

@@ -226,6 +226,16 @@ int dqh_adjoint_pwc_quadrature(int64_t ninst, int64_t n, int64_t nnodes, int64_t
                                const void *rho_nodes, const void *lam_nodes, const void *ops,
                                const double *weights, const int32_t *seg_first, double *grad, void *stream);
 
+/* Gauss-Legendre quadrature of the adjoint gradient with respect to jump rates:
+ * for L_r = sqrt(kappa_r) A_r,
+ *   grad[inst][r][k] = sum_{q in segment k} weights[q] Tr(lam_q D[A_r] rho_q),
+ * D[A] rho = A rho A^dag - (N rho + rho N) / 2 with N = A^dag A. Aops, Nops: device
+ * complex64 [nrates][n][n] (n <= 64, nrates <= 8); other arguments as in
+ * dqh_adjoint_pwc_quadrature. */
+int dqh_adjoint_rate_quadrature(int64_t ninst, int64_t n, int64_t nnodes, int64_t nrates, int64_t nseg,
+                                const void *rho_nodes, const void *lam_nodes, const void *Aops, const void *Nops,
+                                const double *weights, const int32_t *seg_first, double *grad, void *stream);
+
 /* Message of the last failing call on this thread. */
 const char *dqh_last_error(void);
 

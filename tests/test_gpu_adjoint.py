@@ -77,3 +77,39 @@ def test_adjoint_closed_form_constant_drive():
         integ = (np.exp(-gam * (T - t1)) - np.exp(-gam * (T - t0))) / gam
         want.append(2 * np.real(np.conj(alT) * (-1j) * integ))
     np.testing.assert_allclose(g[0, 0], want, atol=2e-5)
+
+
+def test_readme_gradient_kat():
+    """The reference README's gradient example (README.md:85-121): n = 16, omega = 1,
+    kappa = 0.1, alpha0 = 1, T = 2 pi, J = <a^dag a>(T). Printed gradients:
+    d/domega 0.0000, d/dkappa -3.3520, d/dalpha0 1.0670; closed form
+    (0, -alpha0^2 T e^{-kappa T}, 2 alpha0 e^{-kappa T}).
+
+    omega enters as a piecewise-constant control held at 1 over 40 segments (dJ/domega
+    is the sum over segments), kappa through the rate gradient of L = sqrt(kappa) a,
+    alpha0 through lambda(0) = dJ/drho0 contracted with d rho0 / d alpha0."""
+    from dynamiqs_amd.adjoint import mesolve_adjoint
+    from oracle import operators as op
+
+    n, omega, kappa, alpha0 = 16, 1.0, 0.1, 1.0
+    T = 2 * np.pi
+    a = op.destroy(n)
+    N = a.conj().T @ a
+    times = np.linspace(0.0, T, 41)
+    u = np.full((1, 1, 40), omega)
+    res = mesolve_adjoint(
+        np.zeros((n, n), dtype=complex), [N], times, u, [np.sqrt(kappa) * a], op.coherent_dm(n, alpha0), N,
+        rate_ops=[a], checkpoint_every=4,
+    )
+    h = 1e-5
+    drho = (op.coherent_dm(n, alpha0 + h) - op.coherent_dm(n, alpha0 - h)) / (2 * h)
+    g_omega = float(res.grad_u[0, 0].sum())
+    g_kappa = float(res.grad_rates[0, 0])
+    g_alpha = float(np.real(np.trace(res.lambda0[0].astype(complex) @ drho)))
+    decay = np.exp(-kappa * T)
+    assert abs(res.J[0] - alpha0**2 * decay) <= 1e-5
+    assert abs(g_omega) <= 1e-4, g_omega
+    assert abs(g_kappa - (-alpha0**2 * T * decay)) <= 2e-4, g_kappa
+    assert abs(g_alpha - 2 * alpha0 * decay) <= 2e-4, g_alpha
+    # the README's printed values (4 decimals)
+    assert abs(g_kappa - (-3.3520)) <= 1.5e-4 and abs(g_alpha - 1.0670) <= 1.5e-4
