@@ -56,7 +56,10 @@ struct V {
 };
 
 __device__ __forceinline__ bool is_done(const Ctrl &c, double t1) { return c.status != 0 || !(c.t < t1); }
-__device__ __forceinline__ double step_h(const Ctrl &c, double t1) { return fmin(c.dt, t1 - c.t); }
+// step of the attempt described by c: the proposed dt, or exactly the rest of the
+// interval when it would end within the snapping tolerance of t1
+__device__ __forceinline__ double step_h(const Ctrl &c, double t1) { return snaps_to(c.t, c.dt, t1) ? t1 - c.t : c.dt; }
+__device__ __forceinline__ double step_end(const Ctrl &c, double t1) { return snaps_to(c.t, c.dt, t1) ? t1 : c.t + c.dt; }
 
 __device__ __forceinline__ float2 *kslot(const V &v, int j, int64_t rot, size_t inst) {
   const size_t nn = (size_t)v.n * v.n;
@@ -192,7 +195,7 @@ __global__ __launch_bounds__(256) void k_commit(V v, int cur) {
     return;
   }
   const double h = step_h(c, v.t1);
-  const double t_new = (h >= v.t1 - c.t) ? v.t1 : c.t + h;
+  const double t_new = step_end(c, v.t1);
   const double norm = sqrt(reduce_partials<256>(partial(v, inst, 0), v.npart, sh) * v.inv_nn);
   const bool finite = isfinite(norm);
   const bool keep = finite && norm < 1.0;
@@ -221,7 +224,7 @@ __global__ __launch_bounds__(256) void k_commit(V v, int cur) {
     if (o.nsteps >= v.max_steps) {
       o.status = DQH_MAX_STEPS;
     } else {
-      const double hn = fmin(o.dt, v.t1 - o.t);
+      const double hn = step_h(o, v.t1);
       if (!(hn > 0.0) || !(hn > 1e-14 * fmax(1.0, fabs(o.t)))) o.status = DQH_NONFINITE;
     }
   }
@@ -260,7 +263,7 @@ __global__ __launch_bounds__(256) void k_commit(V v, int cur) {
       k0n = kk[6];
     }
     if (!is_done(o, v.t1)) {
-      const double hn = fmin(o.dt, v.t1 - o.t);
+      const double hn = step_h(o, v.t1);
       ysbuf(v, 0, inst)[e] = make_float2((float)(y1.x + hn * kA[1][0] * k0n.x), (float)(y1.y + hn * kA[1][0] * k0n.y));
     }
   }
@@ -388,7 +391,7 @@ __global__ __launch_bounds__(256) void k_init_dt(V v) {
   const double h1 = md <= 1e-15 ? fmax(1e-6, h0 * 1e-3) : pow(0.01 / md, 0.2);
   Ctrl o = c;
   o.dt = fmin(100.0 * h0, h1);
-  const double hn = fmin(o.dt, v.t1 - o.t);
+  const double hn = step_h(o, v.t1);
   if (!(hn > 0.0) || !(hn > 1e-14 * fmax(1.0, fabs(o.t)))) o.status = DQH_NONFINITE;
   const int64_t n = v.n;
   const size_t nn = (size_t)n * n;

@@ -353,16 +353,23 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
       ++nrhs;
       need_k0 = false;
     }
-    double h = fmin(dt, t_end - t);
-    bool made_jump = false;
-    double t_new = (h >= t_end - t) ? t_end : t + h;
+    // clip the step to the end time or the next breakpoint strictly after t
+    double h = dt;
+    double target = t_end;
+    bool target_jump = false;
     if (pwc) {
       const int m = pwc_segment(ptimes, nseg, t) + 1;
-      if (m < nseg && ptimes[m] < t_end && t + h >= ptimes[m]) {
-        h = ptimes[m] - t;
-        t_new = ptimes[m];
-        made_jump = true;
+      if (m < nseg && ptimes[m] < t_end) {
+        target = ptimes[m];
+        target_jump = true;
       }
+    }
+    bool made_jump = false;
+    double t_new = t + h;
+    if (snaps_to(t, h, target)) {
+      h = target - t;
+      t_new = target;
+      made_jump = target_jump;
     }
     if (!(h > 0.0) || !(h > 1e-14 * fmax(1.0, fabs(t)))) {
       status = DQH_NONFINITE;

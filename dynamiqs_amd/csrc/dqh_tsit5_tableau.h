@@ -1,4 +1,4 @@
-// Tsit5 tableau (Ch. Tsitouras, Comput. Math. Appl. 62 (2011) 770-775): stage
+// Tsit5 tableau and step clipping. Tableau (Ch. Tsitouras, Comput. Math. Appl. 62 (2011) 770-775): stage
 // coefficients kA (the last row holds the solution weights b, FSAL), error
 // weights kE = b - b_hat and the continuous extension kR. Published constants
 // of the method; each translation unit gets its own __constant__ copy.
@@ -30,5 +30,14 @@ static __constant__ double kR[7][4] = {
     {0.0, -27.896526289197286, 65.09189467479366, -34.87065786149661},
     {0.0, 1.5, -4.0, 2.5},
 };
+
+// A step that would stop within kSnap * max(1, |target|) short of its target (the
+// end time or the next discontinuity) ends on the target instead: otherwise a
+// step landing a few ulps before a breakpoint leaves a sliver step to it, far
+// below any meaningful step size (and below the vanishing-step guard).
+constexpr double kSnap = 1e-12;
+__device__ __forceinline__ bool snaps_to(double t, double h, double target) {
+  return t + h >= target - kSnap * fmax(1.0, fabs(target));
+}
 
 }  // namespace dqh

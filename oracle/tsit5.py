@@ -20,7 +20,11 @@ reference, so this module restates its published algorithm:
   routine), scale ``atol + rtol |y0|``, exponent ``1/5``.
 * ``jump_ts``: a step never crosses a discontinuity time; all stages of a step
   use the generator of the segment in which the step starts, and the FSAL stage
-  is recomputed after landing on a jump.
+  is recomputed after landing on a jump. A step that would stop within
+  ``SNAP * max(1, |target|)`` before the next discontinuity or the end time ends
+  on it: floating-point breakpoints (``linspace``) are not equally spaced, and a
+  step proposed equal to the previous segment length can stop an ulp short of
+  the next breakpoint, which would leave a step of ~1e-15 to it.
 * Saving: states at ``tsave`` by the continuous extension of the step that
   contains them; ``tsave[0] == t0`` saves the initial state.
 
@@ -74,6 +78,7 @@ R = np.array(
         [0.0, 1.5, -4.0, 2.5],
     ]
 )
+SNAP = 1e-12
 ORDER = 5  # exponent 1/ORDER in the controller and in the initial step
 
 
@@ -171,14 +176,17 @@ def integrate(
             k1 = f(t, y, t)
             stats.nrhs += 1
             need_k1 = False
-        # clip the step to the end time and to the next discontinuity
-        h = min(dt, t_end - t)
-        made_jump = False
+        # clip the step to the end time or the next discontinuity; a step that would
+        # stop within SNAP * max(1, |target|) short of it ends on it (no sliver step)
+        h = dt
         nj = jumps[jumps > t]
-        if nj.size and t + h >= nj[0]:
-            h = nj[0] - t
-            made_jump = True
-        t_new = t + h if (t + h < t_end and not made_jump) else (nj[0] if made_jump else t_end)
+        target, target_jump = (float(nj[0]), True) if nj.size else (t_end, False)
+        made_jump = False
+        t_new = t + h
+        if t + h >= target - SNAP * max(1.0, abs(target)):
+            h = target - t
+            t_new = target
+            made_jump = target_jump
 
         ks = [h * k1]
         for s in range(1, 7):

@@ -403,3 +403,29 @@ def test_rouchon1_first_order_and_reference_tolerance(kats):
     for i, t in enumerate(tsave):
         al = k['alpha0'] * np.exp(-1j * k['delta'] * t - 0.5 * k['kappa'] * t)
         assert np.linalg.norm(r['states'][i] - op.coherent_dm(n, al)) < kats['rouchon1_test_dt']['ysave_atol']
+
+
+def test_tsit5_no_sliver_steps_at_breakpoints():
+    """A step proposed equal to the previous segment length can stop an ulp short of
+    the next linspace breakpoint; the controller then ends it on the breakpoint
+    instead of taking a ~1e-15 step. Seeded driven oscillator (n = 32, 100 control
+    segments on [0, 10]) that produced such a sliver at t = 7.8."""
+    from oracle import lindblad, tsit5
+
+    n, nseg = 32, 100
+    a = np.diag(np.sqrt(np.arange(1, n, dtype=np.float64)), k=1).astype(np.complex128)
+    H0, X = a.conj().T @ a, a + a.conj().T
+    times = np.linspace(0.0, 10.0, nseg + 1)
+    u = np.random.default_rng(4).normal(0.0, 0.5, size=(64, 1, nseg))[34, 0]
+    Ls = [np.sqrt(0.05) * a]
+    rho0 = np.zeros((n, n), dtype=np.complex128)
+    rho0[0, 0] = 1
+
+    def f(t, y, tseg):
+        k = np.searchsorted(times, tseg, side='right') - 1
+        return lindblad.rhs_hermitian(H0 + (u[k] if 0 <= k < nseg else 0.0) * X, Ls, y)
+
+    tsave = np.array([0.0, 8.0])
+    _, _, st = tsit5.integrate(f, rho0, tsave, rtol=1e-6, atol=1e-6, jump_ts=times)
+    assert min(st.dts) > 1e-6, min(st.dts)
+    assert st.nrejected == 0 and st.naccepted <= 82
