@@ -76,6 +76,7 @@ SYMBOLS = {
     'dqh_dia_tsit5_work_bytes': (_I64, [_I64, _I64]),
     'dqh_mesolve_tsit5_dia': (ctypes.c_int, [_P, _P]),
     'dqh_adjoint_pwc_quadrature': (ctypes.c_int, [_I64, _I64, _I64, _I64, _I64, _P, _P, _P, _P, _P, _P, _P]),
+    'dqh_adjoint_pwc_quadrature_dia': (ctypes.c_int, [_I64, _I64, _I64, _I64, _P, _P, _P, _P, _P, _P, _P]),
     'dqh_adjoint_rate_quadrature': (ctypes.c_int, [_I64, _I64, _I64, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P]),
     'dqh_last_error': (ctypes.c_char_p, []),
     'dqh_device_info': (ctypes.c_int, [ctypes.c_char_p, _I64, ctypes.POINTER(_I64)]),

@@ -44,6 +44,33 @@ def _gl(nodes: int):
     return x, w
 
 
+def _sparse_controls(ops, n: int):
+    """Control operators by diagonals for ``dqh_adjoint_pwc_quadrature_dia`` when each
+    has at most 16 nonzero diagonals and fewer than n / 4 (else None: dense path).
+    Returns (DiaOps, device buffers to keep alive until the launch is done)."""
+    import torch
+
+    from .integrators.dia import MAXOFF, DiaOps, _dia_of
+
+    if n > 64:
+        return None
+    forms = [_dia_of(o) for o in ops]
+    if any(len(offs) == 0 or len(offs) > MAXOFF or 4 * len(offs) >= n for offs, _ in forms):
+        return None
+    dev = torch.device('cuda', torch.cuda.current_device())
+    d = DiaOps()
+    d.nops = len(forms)
+    keep = []
+    for k, (offs, diags) in enumerate(forms):
+        d.noff[k] = len(offs)
+        for u, off in enumerate(offs):
+            d.off[k][u] = off
+        t = torch.from_numpy(np.ascontiguousarray(diags.astype(np.complex64)).view(np.float32)).to(dev)
+        keep.append(t)
+        d.diags[k] = t.data_ptr()
+    return d, keep
+
+
 def _check_status(name: str, plan) -> None:
     """Raise if a solve of ``plan`` stopped before its end time (max_steps or a
     non-finite / vanishing step), naming the first such instance."""
@@ -140,7 +167,9 @@ def mesolve_adjoint(
     ck_idx = list(range(0, nseg, checkpoint_every)) + [nseg]
     ck_t = times[ck_idx]
     fwd_insts = [
-        dense.Instance(H=H0, Ls=Ls, rho0=rho0, pwc=[(times, u[b, j], ops[j]) for j in range(nterms)], key_L=(), key_rho=())
+        dense.Instance(
+            H=H0, Ls=Ls, rho0=rho0, pwc=[(times, u[b, j], ops[j]) for j in range(nterms)], key_H=(), key_L=(), key_rho=()
+        )
         for b in range(B)
     ]
     torch.cuda.synchronize()
@@ -179,6 +208,7 @@ def mesolve_adjoint(
                     pwc=[(times, u[b, j], ops[j]) for j in range(nterms)],
                     tsave=ts,
                     t0=float(times[k0]),
+                    key_H=(),
                     key_L=(),
                 )
             )
@@ -206,6 +236,7 @@ def mesolve_adjoint(
             rho0=O,
             Hnh=Hnh_adj,
             pwc=[(rtimes, u[b, j][::-1].copy(), -ops[j]) for j in range(nterms)],
+            key_H=(),
             key_L=(),
             key_rho=(),
         )
@@ -241,15 +272,29 @@ def mesolve_adjoint(
     t9 = time.perf_counter()
     grad = np.zeros((B, nterms, nseg))
     if nterms:
-        o_t = torch.from_numpy(np.ascontiguousarray(np.stack(ops).astype(np.complex64)).view(np.float32)).to(dev)
         g_t = torch.zeros((B, nterms, nseg), dtype=torch.float64, device=dev)
-        _native.check(
-            lib.dqh_adjoint_pwc_quadrature(
-                B, n, Q, nterms, nseg, P(rho_nodes_dev.data_ptr()), P(lam_dev.data_ptr()), P(o_t.data_ptr()),
-                P(w_t.data_ptr()), P(sf_t.data_ptr()), P(g_t.data_ptr()), stream,
-            ),
-            'dqh_adjoint_pwc_quadrature',
-        )
+        dia_ops = _sparse_controls(ops, n)
+        if dia_ops is not None:
+            # control operators with few diagonals: 2 d n^2 products per node
+            dops, keep = dia_ops
+            _native.check(
+                lib.dqh_adjoint_pwc_quadrature_dia(
+                    B, n, Q, nseg, P(rho_nodes_dev.data_ptr()), P(lam_dev.data_ptr()), ctypes.byref(dops),
+                    P(w_t.data_ptr()), P(sf_t.data_ptr()), P(g_t.data_ptr()), stream,
+                ),
+                'dqh_adjoint_pwc_quadrature_dia',
+            )
+            torch.cuda.synchronize()
+            del keep
+        else:
+            o_t = torch.from_numpy(np.ascontiguousarray(np.stack(ops).astype(np.complex64)).view(np.float32)).to(dev)
+            _native.check(
+                lib.dqh_adjoint_pwc_quadrature(
+                    B, n, Q, nterms, nseg, P(rho_nodes_dev.data_ptr()), P(lam_dev.data_ptr()), P(o_t.data_ptr()),
+                    P(w_t.data_ptr()), P(sf_t.data_ptr()), P(g_t.data_ptr()), stream,
+                ),
+                'dqh_adjoint_pwc_quadrature',
+            )
         grad = g_t.cpu().numpy()
     grad_rates = None
     if rate_ops:

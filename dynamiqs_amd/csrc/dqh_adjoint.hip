@@ -97,6 +97,80 @@ __global__ __launch_bounds__(kThreads) void k_quad(int n, int nnodes, int nterms
   }
 }
 
+// Same integrand for control operators stored by diagonals (O[a][a + p] =
+// diags[u][a + p]): with the integrand Im(Tr(lam O rho) - Tr(lam rho O)),
+//   Tr(lam O rho) = sum_u sum_{a,c} lam[c][a] O[a][a+p] rho[a+p][c],
+//   Tr(lam rho O) = sum_u sum_{a,b} lam[b+p][a] rho[a][b] O[b][b+p],
+// 2 d n^2 complex products per node instead of 2 n^3 (d diagonals). lam is
+// staged in LDS (n <= 64), rho is read from global memory along its rows.
+__global__ __launch_bounds__(kThreads) void k_quad_dia(int n, int nnodes, int nseg, dqh_dia_ops ops,
+                                                       const float2 *__restrict__ rho, const float2 *__restrict__ lam,
+                                                       const double *__restrict__ weights,
+                                                       const int *__restrict__ seg_first, double *__restrict__ grad) {
+  __shared__ float2 sL[kMaxLds * (kMaxLds + 1)];
+  __shared__ double red[kThreads / 64];
+  const int seg = blockIdx.x;
+  const size_t inst = blockIdx.y;
+  const size_t nn = (size_t)n * n;
+  const int ld = kMaxLds + 1;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nterms = ops.nops;
+  double acc[DQH_DIA_MAXOPS];
+  for (int j = 0; j < DQH_DIA_MAXOPS; ++j) acc[j] = 0.0;
+  for (int q = seg_first[seg]; q < seg_first[seg + 1]; ++q) {
+    const float2 *R = rho + (inst * nnodes + q) * nn;
+    const float2 *Lm = lam + (inst * nnodes + q) * nn;
+    __syncthreads();
+    for (size_t e = threadIdx.x; e < nn; e += kThreads) sL[(e / n) * ld + e % n] = Lm[e];
+    __syncthreads();
+    const double wq = weights[q];
+    for (int j = 0; j < nterms; ++j) {
+      const float2 *D = reinterpret_cast<const float2 *>(ops.diags[j]);
+      double im = 0.0;
+      for (size_t e = threadIdx.x; e < nn; e += kThreads) {
+        const int a = (int)(e / n), c = (int)(e % n);  // rho[a][c] is contiguous in e
+        const float2 r = R[e];
+        float sr = 0.f, si = 0.f;
+        for (int u = 0; u < ops.noff[j]; ++u) {
+          const int p = ops.off[j][u];
+          // Tr(lam O rho): rho[a][c] pairs with lam[c][a - p] O[a - p][a]
+          const int a2 = a - p;
+          if (a2 >= 0 && a2 < n) {
+            const float2 o = D[(size_t)u * n + a];
+            const float2 l = sL[c * ld + a2];
+            const float lr = l.x * o.x - l.y * o.y, li = l.x * o.y + l.y * o.x;
+            sr += lr * r.x - li * r.y;
+            si += lr * r.y + li * r.x;
+          }
+          // Tr(lam rho O): rho[a][c] pairs with lam[c + p][a] O[c][c + p]
+          const int c2 = c + p;
+          if (c2 >= 0 && c2 < n) {
+            const float2 o = D[(size_t)u * n + c2];
+            const float2 l = sL[c2 * ld + a];
+            const float lr = l.x * o.x - l.y * o.y, li = l.x * o.y + l.y * o.x;
+            sr -= lr * r.x - li * r.y;
+            si -= lr * r.y + li * r.x;
+          }
+        }
+        im += (double)si;
+      }
+      acc[j] += wq * im;
+    }
+  }
+  for (int j = 0; j < nterms; ++j) {
+    double v = acc[j];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double s = 0;
+      for (int w = 0; w < kThreads / 64; ++w) s += red[w];
+      grad[(inst * nterms + j) * nseg + seg] = s;
+    }
+  }
+}
+
 // Gradient with respect to a jump-rate parameter: for L = sqrt(kappa) A,
 // d/dkappa of the Lindbladian is the dissipator D[A], so
 //   dJ/dkappa = int Tr(lam D[A] rho) dt,
@@ -208,5 +282,31 @@ extern "C" int dqh_adjoint_rate_quadrature(int64_t ninst, int64_t n, int64_t nno
   const hipError_t err = hipGetLastError();
   if (err != hipSuccess)
     return dqh::fail(DQH_HIP_ERROR, std::string("dqh_adjoint_rate_quadrature: ") + hipGetErrorString(err));
+  return DQH_OK;
+}
+
+extern "C" int dqh_adjoint_pwc_quadrature_dia(int64_t ninst, int64_t n, int64_t nnodes, int64_t nseg,
+                                              const void *rho_nodes, const void *lam_nodes, const dqh_dia_ops *ops,
+                                              const double *weights, const int32_t *seg_first, double *grad,
+                                              void *stream) {
+  if (ninst <= 0 || n <= 0 || n > dqh::adj::kMaxLds || nnodes <= 0 || nseg <= 0 || !ops || !rho_nodes || !lam_nodes ||
+      !weights || !seg_first || !grad)
+    return dqh::fail(DQH_BAD_ARG, "dqh_adjoint_pwc_quadrature_dia: bad arguments (n <= 64)");
+  if (ops->nops < 1 || ops->nops > DQH_DIA_MAXOPS)
+    return dqh::fail(DQH_BAD_ARG, "dqh_adjoint_pwc_quadrature_dia: 1 <= number of control operators <= 8");
+  for (int k = 0; k < ops->nops; ++k) {
+    if (ops->noff[k] < 1 || ops->noff[k] > DQH_DIA_MAXOFF || !ops->diags[k])
+      return dqh::fail(DQH_BAD_ARG, "dqh_adjoint_pwc_quadrature_dia: bad number of diagonals or null diagonals");
+    for (int u = 0; u < ops->noff[k]; ++u)
+      if (ops->off[k][u] <= -n || ops->off[k][u] >= n)
+        return dqh::fail(DQH_BAD_ARG, "dqh_adjoint_pwc_quadrature_dia: offset outside (-n, n)");
+  }
+  hipLaunchKernelGGL(dqh::adj::k_quad_dia, dim3((unsigned)nseg, (unsigned)ninst), dim3(dqh::adj::kThreads), 0,
+                     (hipStream_t)stream, (int)n, (int)nnodes, (int)nseg, *ops,
+                     reinterpret_cast<const float2 *>(rho_nodes), reinterpret_cast<const float2 *>(lam_nodes), weights,
+                     reinterpret_cast<const int *>(seg_first), grad);
+  const hipError_t err = hipGetLastError();
+  if (err != hipSuccess)
+    return dqh::fail(DQH_HIP_ERROR, std::string("dqh_adjoint_pwc_quadrature_dia: ") + hipGetErrorString(err));
   return DQH_OK;
 }
