@@ -4,13 +4,14 @@ The sweep kernels have several admissible forms for one problem, and which is
 fastest depends on the problem's shape, so they are timed once per key and the
 fastest is kept:
 
-* DIA sweeps, keyed on (n, number of operators, total DIA diagonal count, largest
-  |offset|): the global form (one thread per element, neighbours through the
-  caches) and the LDS-tiled forms with halo 8, 16 and 32 that the offsets admit
-  (64 x 64 output tiles; a smaller halo stages less and fits more workgroups per
-  CU). The candidate is timed on the vector field ``dqh_lindblad_rhs_dia`` of the
-  actual operators over a synthetic batch; the DIA Tsit5 stages and the Rouchon1
-  Kraus sweep use the same staging and take the same choice.
+* DIA sweeps, keyed on (sweep kind, n, number of operators, total DIA diagonal
+  count, largest |offset|): the global form (one thread per element, neighbours
+  through the caches) and the LDS-tiled forms with halo 8, 16 and 32 that the
+  offsets admit (64 x 64 output tiles; a smaller halo stages less and fits more
+  workgroups per CU). For the Lindblad vector field (DIA Tsit5) a candidate is
+  timed on ``dqh_lindblad_rhs_dia`` of the actual operators over a synthetic
+  batch; for the Rouchon1 Kraus sweep on three steps of
+  ``dqh_mesolve_rouchon1_dia``, where the two-pass split form is a candidate too.
 * Dense kernels, keyed on (padded n, number of jump operators) where the
   LDS-resident form applies (n <= 64, at most 3 jump operators): resident against
   global, timed on ``dqh_lindblad_rhs_dense`` over a batch of random instances.
@@ -123,8 +124,10 @@ def dia_candidates(n_pad: int, maxoff: int) -> list[int]:
     return out
 
 
-def dia_sweep(H: np.ndarray, Ls, batch: int) -> int:
-    """Fastest sweep form (a ``SWEEP_*`` value) for the DIA operators of (H, Ls)."""
+def dia_sweep(H: np.ndarray, Ls, batch: int, kind: str = 'rhs', dt: float = 1e-3) -> int:
+    """Fastest sweep form (a ``SWEEP_*`` value) for the DIA operators of (H, Ls):
+    ``kind`` 'rhs' times the Lindblad vector field (DIA Tsit5), 'kraus' times three
+    Rouchon1 steps of length ``dt`` (the split two-pass form is a candidate there)."""
     from .integrators import dia
 
     if not enabled():
@@ -133,22 +136,35 @@ def dia_sweep(H: np.ndarray, Ls, batch: int) -> int:
     n_pad = dia.NB * ((n + dia.NB - 1) // dia.NB)
     Hnh = -1j * H - 0.5 * sum((L.conj().T @ L for L in Ls), np.zeros_like(H))
     ops = [dia._dia_of(Hnh)] + [dia._dia_of(L) for L in Ls]
-    key = dia_key(n_pad, ops)
+    key = f'{kind}|' + dia_key(n_pad, ops)
     hit = lookup(key)
     if hit is not None:
         return int(hit['variant'])
     maxoff = max((abs(int(x)) for o, _ in ops for x in o), default=0)
-    cands = dia_candidates(n_pad, maxoff)
+    cands = dia_candidates(n_pad, maxoff) + ([dia.SWEEP_SPLIT] if kind == 'kraus' else [])
     if len(cands) == 1:
         record(key, {'variant': cands[0], 'us': {}})
         return cands[0]
     rng = np.random.default_rng(0)
     B = max(1, min(int(batch), 4))
-    rho = rng.normal(size=(B, n, n)) + 1j * rng.normal(size=(B, n, n))
     times = {}
-    for v in cands:
-        plan = dia.rhs_plan(H, Ls, rho, variant=v)
-        times[v] = _time_us(plan.run)
+    if kind == 'kraus':
+        x = rng.normal(size=(B, n, 1)) + 1j * rng.normal(size=(B, n, 1))
+        rho = x @ np.conj(np.swapaxes(x, -1, -2))
+        rho /= np.trace(rho, axis1=-2, axis2=-1)[:, None, None]
+        for v in cands:
+            plan = dia.build_plan(H, Ls, rho, np.array([0.0, 3 * dt]), dt, variant=v)
+
+            def run(s, plan=plan):
+                plan.reset()
+                plan.run(s)
+
+            times[v] = _time_us(run, reps=3)
+    else:
+        rho = rng.normal(size=(B, n, n)) + 1j * rng.normal(size=(B, n, n))
+        for v in cands:
+            plan = dia.rhs_plan(H, Ls, rho, variant=v)
+            times[v] = _time_us(plan.run)
     best = min(times, key=times.get)
     record(key, {'variant': int(best), 'us': {str(k): t for k, t in times.items()}, 'batch': B})
     return int(best)

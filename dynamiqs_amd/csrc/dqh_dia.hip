@@ -170,6 +170,89 @@ __global__ __launch_bounds__(256) void k_kraus(int64_t n, int64_t n_true, dqh_di
   }
 }
 
+// Split form of the Kraus sweep, pass 1: dY = dM X with X = rho32 + E (op 0 by
+// diagonals, 3 neighbours along the column for the two-mode operators). Writes
+// dY to scratch; one thread per element.
+__global__ __launch_bounds__(256) void k_kraus_left(int64_t n, dqh_dia_ops ops, const float2 *__restrict__ r32,
+                                                    const float2 *__restrict__ E, float2 *__restrict__ dY) {
+  const size_t nn = (size_t)n * n;
+  const size_t inst = blockIdx.y;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)nn) return;
+  const int i = (int)(e / n), j = (int)(e % n);
+  const float2 *R = r32 + inst * nn, *Ei = E + inst * nn;
+  const float2 *D = diat::cgp<float2>(ops.diags[0]);
+  float ar = 0.f, ai = 0.f;
+  for (int u = 0; u < ops.noff[0]; ++u) {
+    const int64_t a = (int64_t)i + ops.off[0][u];
+    if (a < 0 || a >= n) continue;
+    const size_t o = (size_t)a * n + j;
+    const float2 m = D[(size_t)u * n + a];
+    const float2 r = R[o], x = Ei[o];
+    const float xr = r.x + x.x, xi = r.y + x.y;
+    ar += m.x * xr - m.y * xi;
+    ai += m.x * xi + m.y * xr;
+  }
+  dY[inst * nn + e] = make_float2(ar, ai);
+}
+
+// Split form, pass 2: increment E + dY + (X + dY) dM^dag + sum_{s>=1} A_s X A_s^dag
+// (= E + dM X + X dM^dag + dM X dM^dag + sum A X A^dag), FP64 update and save.
+__global__ __launch_bounds__(256) void k_kraus_right(int64_t n, int64_t n_true, dqh_dia_ops ops,
+                                                     const float2 *__restrict__ r32_in, const float2 *__restrict__ E,
+                                                     const float2 *__restrict__ dY, double2 *__restrict__ r64,
+                                                     float2 *__restrict__ r32_out, float2 *__restrict__ states,
+                                                     int64_t nsave, int64_t save_idx, double theta) {
+  const size_t nn = (size_t)n * n;
+  const size_t inst = blockIdx.y;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)nn) return;
+  const int i = (int)(e / n), j = (int)(e % n);
+  const float2 *R = r32_in + inst * nn, *Ei = E + inst * nn, *Y = dY + inst * nn;
+  const float2 *D = diat::cgp<float2>(ops.diags[0]);
+  float ar = 0.f, ai = 0.f;
+  for (int u = 0; u < ops.noff[0]; ++u) {
+    const int64_t b = (int64_t)j + ops.off[0][u];
+    if (b < 0 || b >= n) continue;
+    const size_t o = (size_t)i * n + b;
+    const float2 m = D[(size_t)u * n + b];  // conj(dM[j][b])
+    const float2 r = R[o], x = Ei[o], y = Y[o];
+    const float wr = r.x + x.x + y.x, wi = r.y + x.y + y.y;
+    ar += wr * m.x + wi * m.y;
+    ai += wi * m.x - wr * m.y;
+  }
+  const diat::GlobalX X{R, Ei, n, i, j};
+  for (int s = 1; s < ops.nops; ++s) {
+    const float2 *A = diat::cgp<float2>(ops.diags[s]);
+    const int no = ops.noff[s];
+    for (int u = 0; u < no; ++u) {
+      const int p = ops.off[s][u];
+      const float2 ma = A[(size_t)u * n + diat::clampi((int64_t)i + p, n)];
+      float zr = 0.f, zi = 0.f;
+      for (int v = 0; v < no; ++v) {
+        const int q = ops.off[s][v];
+        const float2 mb = A[(size_t)v * n + diat::clampi((int64_t)j + q, n)];
+        const float2 x = X(p, q);
+        zr += x.x * mb.x + x.y * mb.y;
+        zi += x.y * mb.x - x.x * mb.y;
+      }
+      ar += ma.x * zr - ma.y * zi;
+      ai += ma.x * zi + ma.y * zr;
+    }
+  }
+  const float2 eij = Ei[e], yij = Y[e];
+  const double dr = (double)eij.x + yij.x + ar, di = (double)eij.y + yij.y + ai;
+  double2 *yv = r64 + inst * nn;
+  const double2 old = yv[e];
+  const double2 nw = make_double2(old.x + dr, old.y + di);
+  yv[e] = nw;
+  r32_out[inst * nn + e] = make_float2((float)nw.x, (float)nw.y);
+  if (save_idx >= 0 && i < n_true && j < n_true) {
+    const double sr = old.x + theta * dr, si = old.y + theta * di;
+    states[((inst * nsave + save_idx) * n_true + i) * n_true + j] = make_float2((float)sr, (float)si);
+  }
+}
+
 // LDS-tiled form of k_kraus (all offsets within the halo, n % 64 == 0): one
 // workgroup per 64 x 64 tile of one instance; X = rho32 + E staged with its halo.
 template <int HALO>
@@ -277,7 +360,8 @@ extern "C" int dqh_mesolve_rouchon1_dia(const dqh_dia_rouchon1 *pb, void *stream
   const dim3 gtile((unsigned)((n / diat::kTI) * (n / diat::kTJ)), (unsigned)p.ninst);
   int maxoff = 0;
   for (int s = 0; s < p.nsets; ++s) maxoff = std::max(maxoff, diat::max_abs_offset(p.ops[s]));
-  const int halo = diat::resolve_halo(p.variant, n, maxoff);
+  const bool split = p.variant == DQH_SWEEP_SPLIT;
+  const int halo = split ? 0 : diat::resolve_halo(p.variant, n, maxoff);
   if (halo < 0)
     return fail(DQH_BAD_ARG, "dqh_mesolve_rouchon1_dia: the tiled sweep needs every |offset| <= its halo");
   int cur = 0;
@@ -304,7 +388,12 @@ extern "C" int dqh_mesolve_rouchon1_dia(const dqh_dia_rouchon1 *pb, void *stream
       th = hs_theta[isave];
       ++isave;
     }
-    if (halo == 8)
+    if (split) {
+      // D1 is consumed by the second solve: its buffer holds dY
+      hipLaunchKernelGGL(dia::k_kraus_left, gel, dim3(256), 0, st, n, p.ops[set], r32[cur], E, D1);
+      hipLaunchKernelGGL(dia::k_kraus_right, gel, dim3(256), 0, st, n, p.n_true, p.ops[set], r32[cur], E, D1, r64,
+                         r32[1 - cur], states, p.nsave, sidx, th);
+    } else if (halo == 8)
       hipLaunchKernelGGL(dia::k_kraus_tiled<8>, gtile, dim3(diat::kThreads), 0, st, n, p.n_true, p.ops[set], r32[cur],
                          E, r64, r32[1 - cur], states, p.nsave, sidx, th);
     else if (halo == 16)

@@ -73,6 +73,8 @@ extern "C" int dqh_lindblad_rhs_dia(const dqh_dia_rhs *pb, void *stream) {
       if (p.ops.off[k][u] <= -p.n || p.ops.off[k][u] >= p.n)
         return fail(DQH_BAD_ARG, "dqh_lindblad_rhs_dia: offset outside (-n, n)");
   }
+  if (p.variant == DQH_SWEEP_SPLIT)
+    return fail(DQH_BAD_ARG, "dqh_lindblad_rhs_dia: the split sweep applies to the Rouchon1 Kraus map only");
   const int halo = diat::resolve_halo(p.variant, p.n, diat::max_abs_offset(p.ops));
   if (halo < 0)
     return fail(DQH_BAD_ARG, "dqh_lindblad_rhs_dia: the tiled sweep needs n % 64 == 0 and every |offset| <= its halo");

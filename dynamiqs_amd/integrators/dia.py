@@ -62,6 +62,7 @@ class DiaRouchon1(ctypes.Structure):
 
 SWEEP_AUTO, SWEEP_GLOBAL, SWEEP_TILED, SWEEP_TILED_H8, SWEEP_TILED_H16, SWEEP_TILED_H32 = 0, 1, 2, 3, 4, 5
 SWEEP_HALO = {SWEEP_TILED_H8: 8, SWEEP_TILED_H16: 16, SWEEP_TILED_H32: 32}
+SWEEP_SPLIT = 6  # Rouchon1 Kraus sweep only
 
 
 class DiaRhs(ctypes.Structure):
@@ -209,7 +210,7 @@ def build_plan(
     if variant == SWEEP_AUTO:
         from .. import autotune
 
-        variant = autotune.dia_sweep(H, Ls, B)
+        variant = autotune.dia_sweep(H, Ls, B, kind='kraus', dt=float(dt))
     prob.nsteps, prob.nsets, prob.variant = nsteps, len(sets), variant
     for si, h in enumerate(sets):
         Vp, Wn, ops = block_solve_operators(H, Ls, h, n_pad)

@@ -106,6 +106,9 @@ int dqh_mesolve_rouchon1_dense(const dqh_me_inst *d_insts, int64_t ninst, void *
 #define DQH_SWEEP_TILED_H8 3
 #define DQH_SWEEP_TILED_H16 4
 #define DQH_SWEEP_TILED_H32 5
+/* Rouchon1 Kraus sweep only: two passes, dY = dM X then the rest with dY read along
+ * rows, so each output reads few neighbours (no dM X dM^dag double sum). */
+#define DQH_SWEEP_SPLIT 6
 
 #define DQH_DIA_MAXOPS 8
 #define DQH_DIA_MAXOFF 16

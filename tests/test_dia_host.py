@@ -173,3 +173,54 @@ def test_tiled_sweep_indexing_reproduces_dense_rhs(na, nb_, halo):
     ref = lindblad.rhs_hermitian(H, Ls, rho)
     np.testing.assert_allclose(out[:n, :n], ref, atol=1e-12)
     assert np.abs(out[n:, :]).max(initial=0.0) == 0 and np.abs(out[:, n:]).max(initial=0.0) == 0
+
+
+def test_split_kraus_passes_match_stencil_form():
+    """k_kraus_left / k_kraus_right index arithmetic: dY[i][j] = sum_u D0[u][i+p] X[i+p][j]
+    then E + dY + sum_u (X + dY)[i][j+p] conj(D0[u][j+p]) + sum_{s>=1} A_s X A_s^dag
+    equals the one-pass increment E + dM X + X dM^dag + dM X dM^dag + sum A X A^dag."""
+    H, Ls = _two_mode(12, 12)
+    n = 144
+    n_pad = 192
+    Vp, Wn, ops = dia.block_solve_operators(H, Ls, 1e-3, n_pad)
+    ops_p = []
+    for offs, d in ops:
+        dp = np.zeros((len(offs), n_pad), dtype=complex)
+        dp[:, :n] = d
+        ops_p.append((offs, dp))
+    rng = np.random.default_rng(5)
+    R = np.zeros((n_pad, n_pad), dtype=complex)
+    E = np.zeros_like(R)
+    R[:n, :n] = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    E[:n, :n] = 1e-3 * (rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)))
+    X = R + E
+    offs0, D0 = ops_p[0]
+    idx = np.arange(n_pad)
+    dY = np.zeros_like(X)
+    for u, p in enumerate(offs0):  # pass 1, row neighbours
+        a = idx + p
+        ok = (a >= 0) & (a < n_pad)
+        dY[ok, :] += D0[u][a[ok]][:, None] * X[a[ok], :]
+    W = X + dY
+    inc = E + dY
+    for u, p in enumerate(offs0):  # pass 2, column neighbours
+        b = idx + p
+        ok = (b >= 0) & (b < n_pad)
+        inc[:, ok] += W[:, b[ok]] * np.conj(D0[u][b[ok]])[None, :]
+
+    def dense(offs, d):
+        out = np.zeros((n_pad, n_pad), dtype=complex)
+        for u, o in enumerate(offs):
+            c = np.arange(max(o, 0), min(n_pad, n_pad + o))
+            out[c - o, c] = d[u, c]
+        return out
+
+    for offs, d in ops_p[1:]:
+        A = dense(offs, d)
+        inc = inc + A @ X @ A.conj().T
+    dM = dense(*ops_p[0])
+    want = E + dM @ X + X @ dM.conj().T + dM @ X @ dM.conj().T
+    for offs, d in ops_p[1:]:
+        A = dense(offs, d)
+        want = want + A @ X @ A.conj().T
+    np.testing.assert_allclose(inc, want, atol=1e-11)

@@ -137,8 +137,8 @@ def test_dia_quadrature_equals_dense_quadrature():
         x = rng.normal(size=(k, n, n)) + 1j * rng.normal(size=(k, n, n))
         return (x + np.conj(np.swapaxes(x, -1, -2))) / 2
 
-    rho = herm(B * Q).reshape(B, Q, n, n).astype(np.complex64)
-    lam = herm(B * Q).reshape(B, Q, n, n).astype(np.complex64)
+    rho = np.ascontiguousarray(herm(B * Q).reshape(B, Q, n, n), dtype=np.complex64)
+    lam = np.ascontiguousarray(herm(B * Q).reshape(B, Q, n, n), dtype=np.complex64)
     w = rng.uniform(0.1, 1.0, size=Q)
     sf = (np.arange(nseg + 1) * nodes).astype(np.int32)
     dev = torch.device('cuda', 0)

@@ -42,6 +42,28 @@ def test_dia_rouchon1_matches_oracle(na, nb_, nsteps):
         assert abs(np.trace(out["final_state"][b]) - 1) < 1e-7
 
 
+@pytest.mark.parametrize('variant', ['global', 'split', 'h16'])
+def test_dia_rouchon1_sweep_forms_agree(variant):
+    """Every Kraus sweep form (one-pass global, two-pass split, LDS-tiled halo 16)
+    reproduces the oracle's Rouchon1 at n = 144 (padded to 192)."""
+    from dynamiqs_amd.integrators import dia
+    from oracle import mesolve as om
+    from oracle import operators as op
+
+    H, Ls, _ = _two_mode(12, 12)
+    dt = 1e-3
+    tsave = np.linspace(0, 0.1, 3)
+    rho0 = np.stack([op.coherent_dm((12, 12), (al, 0.0)) for al in (0.8, 1.6)])
+    v = {'global': dia.SWEEP_GLOBAL, 'split': dia.SWEEP_SPLIT, 'h16': dia.SWEEP_TILED_H16}[variant]
+    plan = dia.build_plan(H, Ls, rho0, tsave, dt, variant=v)
+    plan.run()
+    out = plan.results()
+    for b in range(2):
+        ref = om.mesolve(H, Ls, rho0[b], tsave, method='rouchon1', dt=dt)
+        err = np.linalg.norm(out['states'][b] - ref['states'], axis=(-2, -1)).max()
+        assert err <= TOL, (variant, b, err)
+
+
 def test_dia_rouchon1_config3_shape_short_run():
     """Full 32 x 32 two-mode size (n = 1024, half-bandwidth 62) for 5 steps vs the
     banded FP64 oracle, plus trace preservation."""
@@ -54,9 +76,13 @@ def test_dia_rouchon1_config3_shape_short_run():
     nsteps = 5
     tsave = np.linspace(0, nsteps * dt, 2)
     rho0 = np.stack([op.coherent_dm((32, 32), (al, 0.0)) for al in (0.5, 2.0)])
-    plan = dia.build_plan(H, Ls, rho0, tsave, dt)
-    plan.run()
-    out = plan.results()
+    outs = []
+    for v in (dia.SWEEP_SPLIT, dia.SWEEP_GLOBAL):
+        plan = dia.build_plan(H, Ls, rho0, tsave, dt, variant=v)
+        plan.run()
+        outs.append(plan.results())
+    out = outs[0]
+    assert np.abs(outs[0]['final_state'] - outs[1]['final_state']).max() <= 1e-6
     M0, Ms, S = rouchon.kraus_ops(H, Ls, dt)
     Tab = rouchon.banded_factor(S, 62)
     for b in range(2):
@@ -277,7 +303,7 @@ def test_autotuner_times_and_caches_dia_and_dense_forms(tmp_path, monkeypatch):
     H, Ls, _ = _two_mode(12, 12)
     v = autotune.dia_sweep(H, Ls, 2)
     assert v in (dia.SWEEP_GLOBAL, dia.SWEEP_TILED_H16, dia.SWEEP_TILED_H32)
-    entries = [e for k, e in autotune._memo.items() if k.startswith('dia|')]
+    entries = [e for k, e in autotune._memo.items() if k.startswith('rhs|dia|')]
     assert len(entries) == 1 and len(entries[0]['us']) == 3
     assert autotune.dia_sweep(H, Ls, 2) == v
     g = autotune.dense_global(32, 2, 64)
