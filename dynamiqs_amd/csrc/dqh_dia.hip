@@ -51,14 +51,15 @@ constexpr int kLD = kNB + 2;     // padded LDS row stride (float2)
 constexpr int kWaves = 8;
 constexpr int kThreads = kWaves * 64;
 
-// acc[t] += A[row0:+16, 0:64] (global, row-major lda=64) @ B[0:64, col0+16t:+16] (LDS, ld=kLD)
+// acc[t] += A[row0:+16, kbeg:64] (global, row-major lda=64) @ B[kbeg:64, col0+16t:+16]
+// (LDS, ld=kLD); kbeg (a multiple of 16) skips the columns where A is zero.
 __device__ __forceinline__ void mac_strip2(Acc (&acc)[2], const float2 *__restrict__ A, const float2 *B, int row0,
-                                           int col0, int lane) {
+                                           int col0, int lane, int kbeg = 0) {
   const int r = lane & 15, g = lane >> 4;
   const float2 *Ap = A + (row0 + r) * kNB + 4 * g;
   const float2 *Bp = B + (4 * g) * kLD + col0 + r;
-#pragma unroll
-  for (int k0 = 0; k0 < kNB; k0 += 16) {
+#pragma unroll 4
+  for (int k0 = kbeg; k0 < kNB; k0 += 16) {
     const float4 a01 = *reinterpret_cast<const float4 *>(Ap + k0);
     const float4 a23 = *reinterpret_cast<const float4 *>(Ap + k0 + 2);
     const float2 *bp = Bp + k0 * kLD;
@@ -117,7 +118,8 @@ __global__ __launch_bounds__(kThreads) void k_lsolve(int mode, int64_t n, const 
       acc[t].re = f4{0.f, 0.f, 0.f, 0.f};
       acc[t].im = f4{0.f, 0.f, 0.f, 0.f};
     }
-    mac_strip2(acc, Vp + (size_t)bi * kNB * kNB, s_lds, ti * 16, tj0 * 16, lane);
+    // V'_i is upper triangular: rows ti*16.. of it are zero left of column ti*16
+    mac_strip2(acc, Vp + (size_t)bi * kNB * kNB, s_lds, ti * 16, tj0 * 16, lane, ti * 16);
     if (bi + 1 < NB) mac_strip2(acc, Wn + (size_t)bi * kNB * kNB, prev, ti * 16, tj0 * 16, lane);
     __syncthreads();  // every wave is done reading prev
     const int r = lane & 15, g = lane >> 4;

@@ -120,7 +120,9 @@ def block_solve_operators(H: np.ndarray, Ls, dt: float, n_pad: int):
     for i in range(nblk):
         s = slice(i * NB, (i + 1) * NB)
         V = sla.solve_triangular(T[s, s].conj().T, np.eye(NB), lower=False)
-        Vp[i] = V - np.eye(NB)
+        # inverse of an upper-triangular block: upper triangular (the solve kernel
+        # skips the zero part below the diagonal)
+        Vp[i] = np.triu(V - np.eye(NB))
         if i + 1 < nblk:
             s1 = slice((i + 1) * NB, (i + 2) * NB)
             Wn[i] = -V @ T[s1, s].conj().T
