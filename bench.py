@@ -151,7 +151,9 @@ def bench_dia(args):
         rho0.append(np.outer(v, v.conj()))
     rho0 = np.stack(rho0)
     tsave = np.linspace(0.0, args.dia_T, int(round(args.dia_T / 0.1)) + 1)
-    variant = {'auto': dia.SWEEP_AUTO, 'global': dia.SWEEP_GLOBAL, 'tiled': dia.SWEEP_TILED}[args.dia_variant]
+    variant = {'auto': dia.SWEEP_AUTO, 'global': dia.SWEEP_GLOBAL, 'tiled': dia.SWEEP_TILED, 'split': dia.SWEEP_SPLIT}[
+        args.dia_variant
+    ]
     if args.dia_method == 'tsit5':
         return bench_dia_tsit5(args, H, Ls, rho0, tsave, variant)
     plan = dia.build_plan(H, Ls, rho0, tsave, 1e-3, variant=variant)
@@ -379,7 +381,7 @@ def main(argv=None):
     ap.add_argument('--rates', type=float, nargs='+', default=[100.0, 400.0, 0.0])
     ap.add_argument('--budget-gb', type=float, default=32.0)
     ap.add_argument('--dia-T', type=float, default=5.0)
-    ap.add_argument('--dia-variant', default='auto', choices=['auto', 'global', 'tiled'])
+    ap.add_argument('--dia-variant', default='auto', choices=['auto', 'global', 'tiled', 'split'])
     ap.add_argument('--dia-method', default='rouchon1', choices=['rouchon1', 'tsit5'])
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=3)
