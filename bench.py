@@ -374,9 +374,31 @@ def bench_adjoint(args):
     return 0
 
 
+def bench_copy(args):
+    """A plain device-to-device copy of known size (512 MiB read, 512 MiB written):
+    the calibration run for the FETCH_SIZE / WRITE_SIZE counters, and the HBM
+    bandwidth a streaming kernel reaches on this device."""
+    import torch
+
+    nbytes = 512 << 20
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device='cuda')
+    y = torch.empty_like(x)
+    stream = torch.cuda.current_stream()
+    y.copy_(x)
+    timer = EventTimer(stream)
+    for _ in range(max(args.steps, 1)):
+        e0 = timer.start()
+        y.copy_(x)
+        timer.stop(e0)
+    ms = float(np.median(timer.times_ms()))
+    print(json.dumps({'metric': 'device copy', 'bytes_read': nbytes, 'bytes_written': nbytes, 'ms': ms,
+                      'GBps': 2 * nbytes / (1e-3 * ms) / 1e9, 'launches': max(args.steps, 1) + 1}), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument('--config', default='kerr', choices=['kerr', 'dia', 'serving', 'adjoint'])
+    ap.add_argument('--config', default='kerr', choices=['kerr', 'dia', 'serving', 'adjoint', 'copy'])
     ap.add_argument('--requests', type=int, default=1000)
     ap.add_argument('--rates', type=float, nargs='+', default=[100.0, 400.0, 0.0])
     ap.add_argument('--budget-gb', type=float, default=32.0)
@@ -398,6 +420,8 @@ def main(argv=None):
         )
         return 2
 
+    if args.config == 'copy':
+        return bench_copy(args)
     if args.config == 'dia':
         return bench_dia(args)
     if args.config == 'serving':
