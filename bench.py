@@ -374,6 +374,33 @@ def bench_adjoint(args):
     return 0
 
 
+def bench_grid(args):
+    """One large dense request (n = 256, 3 jump operators, GUE H, T = 10, rtol = 1e-5):
+    latency of the persistent one-workgroup kernel against the grid-wide Tsit5."""
+    import torch
+
+    from dynamiqs_amd.integrators import dense
+
+    reqs = serving_requests(64, seed=7)
+    H, Ls, psi, tsave = next(r for r in reqs if r[0].shape[0] >= 200 and len(r[1]) == 3)
+    n = H.shape[0]
+    inst = dense.Instance(H=H, Ls=Ls, rho0=psi @ psi.conj().T)
+    stream = torch.cuda.current_stream()
+    out = {'metric': 'large-request latency', 'config': {'workload': f'dense Tsit5 n={n}, K=3, T={tsave[-1]:.2f}, rtol=1e-5'}}
+    states = {}
+    for name, g in (('persistent', False), ('grid', True)):
+        plan = dense.build_plan('tsit5', [inst], tsave, rtol=1e-5, atol=1e-6, grid_tsit5=g)
+        run_plan_timed(plan, 1, stream)
+        walls = run_plan_timed(plan, max(args.steps, 1), stream)
+        res = plan.results()
+        states[name] = res['states'][0]
+        out[name] = {'latency_s': float(np.median(walls)), 'nsteps': int(res['nsteps'][0]), 'status': int(res['status'][0])}
+    out['speedup'] = out['persistent']['latency_s'] / out['grid']['latency_s']
+    out['max_diff'] = float(np.abs(states['grid'].astype(complex) - states['persistent']).max())
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def bench_copy(args):
     """A plain device-to-device copy of known size (512 MiB read, 512 MiB written):
     the calibration run for the FETCH_SIZE / WRITE_SIZE counters, and the HBM
@@ -398,7 +425,7 @@ def bench_copy(args):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument('--config', default='kerr', choices=['kerr', 'dia', 'serving', 'adjoint', 'copy'])
+    ap.add_argument('--config', default='kerr', choices=['kerr', 'dia', 'serving', 'adjoint', 'copy', 'grid'])
     ap.add_argument('--requests', type=int, default=1000)
     ap.add_argument('--rates', type=float, nargs='+', default=[100.0, 400.0, 0.0])
     ap.add_argument('--budget-gb', type=float, default=32.0)
@@ -422,6 +449,8 @@ def main(argv=None):
 
     if args.config == 'copy':
         return bench_copy(args)
+    if args.config == 'grid':
+        return bench_grid(args)
     if args.config == 'dia':
         return bench_dia(args)
     if args.config == 'serving':

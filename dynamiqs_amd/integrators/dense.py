@@ -177,6 +177,7 @@ class DensePlan:
         self.y = None
         self.y0 = None
         self.nrhs_flops = 0.0
+        self.grid = None  # (scratch, pinned flags, graph steps) for the grid-wide Tsit5
 
     # ------------------------------------------------------------------ launch
     def reset(self):
@@ -189,6 +190,17 @@ class DensePlan:
         lib = _native.lib()
         s = stream if stream is not None else torch.cuda.current_stream()
         sp = _native.ctypes.c_void_p(s.cuda_stream)
+        if self.grid is not None:
+            c_p = _native.ctypes.c_void_p
+            scratch, flags, steps = self.grid
+            _native.check(
+                lib.dqh_mesolve_tsit5_dense_grid(
+                    c_p(self.d_descs.data_ptr()), self.descs.ctypes.data_as(c_p), self.ninst, c_p(scratch.data_ptr()),
+                    scratch.numel(), steps, c_p(flags.data_ptr()), sp,
+                ),
+                'dqh_mesolve_tsit5_dense_grid',
+            )
+            return
         fn = {
             'tsit5': lib.dqh_mesolve_tsit5_dense,
             'rouchon1': lib.dqh_mesolve_rouchon1_dense,
@@ -268,12 +280,19 @@ def build_plan(
     normalize: bool = True,
     save_states: bool = True,
     form: str = 'auto',
+    grid_tsit5: bool | str = 'auto',
+    graph_steps: int = 16,
 ) -> DensePlan:
     """Pack ``instances`` (same Hilbert dimension) for ``kind`` in {'tsit5', 'rouchon1', 'rhs'}.
 
     ``form`` picks the kernel form of the Tsit5 / RHS kernels: 'resident' (stages
     and rho in registers / LDS, n <= 64 and at most 3 jump operators), 'global'
-    (stages in global memory) or 'auto' (the autotuner's choice for (n, K))."""
+    (stages in global memory) or 'auto' (the autotuner's choice for (n, K)).
+
+    ``grid_tsit5`` runs Tsit5 as grid-wide launches per stage with the controller on the
+    device (``dqh_mesolve_tsit5_dense_grid``) instead of one persistent workgroup
+    per instance: True, False, or 'auto' (when the batch is too small to fill the
+    GPU with one workgroup per instance and n >= GRID_MIN_N)."""
     import torch
 
     dev = _device()
@@ -451,4 +470,27 @@ def build_plan(
     plan.descs = descs
     plan.d_descs = torch.from_numpy(descs.view(np.uint8).copy()).to(dev)
     plan.K = K
+    if kind == 'tsit5' and use_grid(grid_tsit5, n, B, npwc_max, nE, save_states):
+        lib = _native.lib()
+        nbytes = int(lib.dqh_tsit5_grid_scratch_bytes(npad, B))
+        scratch = torch.empty((nbytes,), dtype=torch.uint8, device=dev)
+        flags = torch.zeros((2,), dtype=torch.int64, pin_memory=True)
+        plan.grid = (scratch, flags, int(graph_steps))
     return plan
+
+
+# The grid-wide Tsit5 pays off once one instance has enough tiles to spread and the
+# batch leaves most CUs idle with one workgroup per instance.
+GRID_MIN_N = 128
+GRID_MAX_BATCH = 64
+
+
+def use_grid(grid, n: int, B: int, npwc: int, nE: int, save_states: bool) -> bool:
+    """Whether a Tsit5 plan runs on the grid-wide kernels (constant generators only;
+    expectation values are taken from the saved states)."""
+    ok = npwc == 0 and (nE == 0 or save_states)
+    if grid == 'auto':
+        return ok and n >= GRID_MIN_N and B <= GRID_MAX_BATCH
+    if grid and not ok:
+        raise ValueError('the grid-wide Tsit5 needs constant generators and saved states for expectation values')
+    return bool(grid)

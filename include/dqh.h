@@ -94,6 +94,23 @@ int dqh_mesolve_tsit5_dense(const dqh_me_inst *d_insts, int64_t ninst, void *str
  * tmp + tmp^dag with tmp = sum_k D_k rho C_k to the FP64 state. */
 int dqh_mesolve_rouchon1_dense(const dqh_me_inst *d_insts, int64_t ninst, void *stream);
 
+/* The same Tsit5 solves spread over the whole grid: every stage of every
+ * instance is a set of grid-wide launches (MFMA tiles of the two GEMM phases, an
+ * elementwise stage update) and the PID controller runs on the device, one
+ * control block per instance; graph_steps steps (even, 0: plain launches) are
+ * captured as a hipGraph and replayed until every instance is done. For large n
+ * (one workgroup per instance would leave the GPU idle). Same descriptors and
+ * outputs as dqh_mesolve_tsit5_dense (d_insts on the device, h_insts the same
+ * array on the host); all instances share np; constant generators only (npwc ==
+ * 0); expectation values need saved states. scratch: device memory of
+ * dqh_tsit5_grid_scratch_bytes(np, ninst) bytes; host_flags: pinned int64[2].
+ * Returns when the batch is done; results are ready once ``stream`` is
+ * synchronised. */
+int64_t dqh_tsit5_grid_scratch_bytes(int64_t np, int64_t ninst);
+int dqh_mesolve_tsit5_dense_grid(const dqh_me_inst *d_insts, const dqh_me_inst *h_insts, int64_t ninst,
+                                 void *scratch, int64_t scratch_bytes, int64_t graph_steps, int64_t *host_flags,
+                                 void *stream);
+
 /* ------------------------------------------------------------------ sparse DIA */
 /* Forms of the DIA sweeps. TILED_Hh: 64 x 64 output tiles per workgroup, the input
  * staged in LDS with an h-wide halo (needs n % 64 == 0 and every |offset| <= h).
