@@ -5,7 +5,10 @@ of jump operators, own save times and tolerances) are admitted while earlier
 batches are still on the GPU: a scheduler thread packs whatever is pending into
 one launch of the persistent Tsit5 kernel (every workgroup integrates one request
 with its own adaptive step, so uneven requests share a launch), bounded by a
-device-memory budget, and launches it on the next free HIP stream. Launches on
+device-memory budget, and launches it on the next free HIP stream. Requests of
+padded dimension above GRID_MIN_NP run on the grid-wide Tsit5 instead (each stage
+spread over many CUs, driven by a worker thread), so that one large request does
+not hold a single CU for the whole solve. Launches on
 different streams overlap on the device; when a launch's completion event fires,
 its requests are answered and their memory budget is released, which admits the
 next pending requests. Each request's state, stages and outputs stay resident in
@@ -100,7 +103,7 @@ def footprint_bytes(n: int, K: int, ntsave: int, nE: int = 0) -> int:
 
 
 class _Launch:
-    def __init__(self, reqs, stream, event, tensors, outs, nbytes, t_launch):
+    def __init__(self, reqs, stream, event, tensors, outs, nbytes, t_launch, ready=None):
         self.reqs = reqs
         self.stream = stream
         self.event = event
@@ -108,12 +111,26 @@ class _Launch:
         self.outs = outs
         self.nbytes = nbytes
         self.t_launch = t_launch
+        self.ready = ready  # set by the worker thread of a grid-wide launch
+        self.error = None
+
+    def done(self) -> bool:
+        if self.ready is not None and not self.ready.is_set():
+            return False
+        return self.event is not None and self.event.query()
 
 
-def pack_and_launch(reqs: list, stream):
+# Requests with a padded dimension above this run on the grid-wide Tsit5 (every
+# stage spread over many CUs) instead of one workgroup per request.
+GRID_MIN_NP = 96
+
+
+def pack_and_launch(reqs: list, stream, grid: bool = False):
     """Pack uneven requests into one descriptor array and launch the persistent
-    Tsit5 kernel on ``stream``. Returns (tensors kept alive, per-request output
-    layout, completion event, device bytes held)."""
+    Tsit5 kernel on ``stream`` (``grid``: return a callable that runs the
+    grid-wide Tsit5 on them instead; requests of one grid launch share their padded
+    dimension). Returns (tensors kept alive, per-request output layout, completion
+    event or the callable, device bytes held)."""
     import torch
 
     dev = torch.device('cuda', torch.cuda.current_device())
@@ -171,10 +188,32 @@ def pack_and_launch(reqs: list, stream):
             outs.append((ob + lo['st'], ob + lo['ex'], ob + lo['stats'], n, nt, nE))
         d_descs = torch.from_numpy(descs.view(np.uint8).copy()).to(dev)
         lib = _native.lib()
+        c_p = _native.ctypes.c_void_p
+        if grid:
+            npd = int(descs['np'][0])
+            if np.any(descs['np'] != npd):
+                raise ValueError('a grid-wide launch needs requests of one padded dimension')
+            nb = int(lib.dqh_tsit5_grid_scratch_bytes(npd, B))
+            scratch = torch.empty((nb,), dtype=torch.uint8, device=dev)
+            flags = torch.zeros((2,), dtype=torch.int64, pin_memory=True)
+
+            def run():
+                # blocks this (worker) thread until the batch is done; ctypes releases the GIL
+                with torch.cuda.stream(stream):
+                    _native.check(
+                        lib.dqh_mesolve_tsit5_dense_grid(
+                            c_p(d_descs.data_ptr()), descs.ctypes.data_as(c_p), B, c_p(scratch.data_ptr()), nb, 16,
+                            c_p(flags.data_ptr()), c_p(stream.cuda_stream),
+                        ),
+                        'dqh_mesolve_tsit5_dense_grid',
+                    )
+                    ev = torch.cuda.Event()
+                    ev.record(stream)
+                return ev
+
+            return [oarena, arena, d_descs, scratch, flags, descs], outs, run, int(in_off[-1] + out_off[-1] + nb)
         _native.check(
-            lib.dqh_mesolve_tsit5_dense(
-                _native.ctypes.c_void_p(d_descs.data_ptr()), B, _native.ctypes.c_void_p(stream.cuda_stream)
-            ),
+            lib.dqh_mesolve_tsit5_dense(c_p(d_descs.data_ptr()), B, c_p(stream.cuda_stream)),
             'dqh_mesolve_tsit5_dense',
         )
         ev = torch.cuda.Event()
@@ -195,8 +234,12 @@ def collect(oarena, outs) -> list:
 
 
 def _size_class(r) -> int:
+    """Launch grouping: small requests by size band (a persistent launch lasts as
+    long as its slowest request), grid-wide requests by exact padded dimension."""
     npd = pad16(r.n)
-    return 0 if npd <= 32 else (1 if npd <= 64 else (2 if npd <= 128 else 3))
+    if npd > GRID_MIN_NP:
+        return 1000 + npd
+    return 0 if npd <= 32 else (1 if npd <= 64 else 2)
 
 
 class Server:
@@ -281,23 +324,40 @@ class Server:
         stream = self.streams[self._next_stream % len(self.streams)]
         self._next_stream += 1
         t0 = time.perf_counter()
+        grid = cls >= 1000
         try:
-            tensors, outs, ev, total = pack_and_launch(batch, stream)
+            tensors, outs, ev, total = pack_and_launch(batch, stream, grid=grid)
         except Exception as e:  # answer the batch with the error, keep serving
             for r in batch:
                 r.future.set_exception(e)
             return
         self._used += total
-        self._inflight.append(_Launch(batch, stream, ev, tensors, outs, total, t0))
+        if grid:
+            launch = _Launch(batch, stream, None, tensors, outs, total, t0, ready=threading.Event())
+
+            def work(launch=launch, run=ev):
+                try:
+                    launch.event = run()
+                except Exception as e:  # reported when the launch retires
+                    launch.error = e
+                finally:
+                    launch.ready.set()
+
+            threading.Thread(target=work, name='dqh-grid', daemon=True).start()
+            self._inflight.append(launch)
+        else:
+            self._inflight.append(_Launch(batch, stream, ev, tensors, outs, total, t0))
         self.stats['launches'] += 1
         self.stats['max_inflight_bytes'] = max(self.stats['max_inflight_bytes'], self._used)
         self.stats['max_concurrent_launches'] = max(self.stats['max_concurrent_launches'], len(self._inflight))
 
     def _retire(self) -> bool:
-        done = [L for L in self._inflight if L.event.query()]
+        done = [L for L in self._inflight if (L.ready is not None and L.ready.is_set() and L.error) or L.done()]
         for L in done:
             self._inflight.remove(L)
             try:
+                if L.error is not None:
+                    raise L.error
                 res = collect(L.tensors[0], L.outs)
             except Exception as e:
                 for r in L.reqs:
