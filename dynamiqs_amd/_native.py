@@ -68,6 +68,7 @@ SYMBOLS = {
     'dqh_lindblad_rhs_dense': (ctypes.c_int, [_P, _I64, _P]),
     'dqh_mesolve_tsit5_dense': (ctypes.c_int, [_P, _I64, _P]),
     'dqh_mesolve_rouchon1_dense': (ctypes.c_int, [_P, _I64, _P]),
+    'dqh_mesolve_tsit5_dense_uniform': (ctypes.c_int, [_P, _I64, _I64, _P]),
     'dqh_tsit5_grid_scratch_bytes': (_I64, [_I64, _I64]),
     'dqh_mesolve_tsit5_dense_grid': (ctypes.c_int, [_P, _P, _I64, _P, _I64, _I64, _P, _P]),
     'dqh_sizeof_dia_rouchon1': (_I64, []),

@@ -531,6 +531,22 @@ __global__ __launch_bounds__(kThreads) void k_tsit5(const dqh_me_inst *__restric
   tsit5_global<kWaves>(p, sh);
 }
 
+// A batch whose instances all take the resident body for one NP: the kernel holds
+// only that body, so its registers are allocated for it alone (the dispatcher's
+// allocation covers every body it may call). An instance that does not fit is
+// left untouched with status DQH_UNSUPPORTED.
+template <int NP>
+__global__ __launch_bounds__(kThreads) void k_tsit5_res(const dqh_me_inst *__restrict__ insts) {
+  __shared__ double sh[kWaves];
+  __shared__ float2 lds[kResLdsFloat2];
+  const dqh_me_inst &p = insts[blockIdx.x];
+  if (p.np != NP || p.nL > kResMaxL || (p.flags & DQH_FLAG_GLOBAL)) {
+    if (threadIdx.x == 0) gp<int64_t>(p.stats)[3] = DQH_UNSUPPORTED;
+    return;
+  }
+  tsit5_resident<NP>(p, sh, lds);
+}
+
 __global__ __launch_bounds__(kThreads) void k_rhs(const dqh_me_inst *__restrict__ insts) {
   __shared__ float2 lds[kResLdsFloat2];
   const dqh_me_inst &p = insts[blockIdx.x];
@@ -577,6 +593,21 @@ extern "C" int dqh_mesolve_tsit5_dense(const dqh_me_inst *d_insts, int64_t ninst
   if (!d_insts || ninst <= 0 || ninst > (1 << 30)) return fail(DQH_BAD_ARG, "dqh_mesolve_tsit5_dense: bad arguments");
   hipLaunchKernelGGL(k_tsit5, dim3((unsigned)ninst), dim3(kThreads), 0, (hipStream_t)stream, d_insts);
   return launch_checked("dqh_mesolve_tsit5_dense", hipGetLastError());
+}
+
+extern "C" int dqh_mesolve_tsit5_dense_uniform(const dqh_me_inst *d_insts, int64_t ninst, int64_t np, void *stream) {
+  if (!d_insts || ninst <= 0 || ninst > (1 << 30))
+    return fail(DQH_BAD_ARG, "dqh_mesolve_tsit5_dense_uniform: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  const dim3 grid((unsigned)ninst), block(kThreads);
+  switch (np) {
+    case 16: hipLaunchKernelGGL(k_tsit5_res<16>, grid, block, 0, st, d_insts); break;
+    case 32: hipLaunchKernelGGL(k_tsit5_res<32>, grid, block, 0, st, d_insts); break;
+    case 48: hipLaunchKernelGGL(k_tsit5_res<48>, grid, block, 0, st, d_insts); break;
+    case 64: hipLaunchKernelGGL(k_tsit5_res<64>, grid, block, 0, st, d_insts); break;
+    default: return fail(DQH_BAD_SHAPE, "dqh_mesolve_tsit5_dense_uniform: np must be 16, 32, 48 or 64");
+  }
+  return launch_checked("dqh_mesolve_tsit5_dense_uniform", hipGetLastError());
 }
 
 extern "C" int dqh_mesolve_rouchon1_dense(const dqh_me_inst *d_insts, int64_t ninst, void *stream) {

@@ -178,6 +178,7 @@ class DensePlan:
         self.y0 = None
         self.nrhs_flops = 0.0
         self.grid = None  # (scratch, pinned flags, graph steps) for the grid-wide Tsit5
+        self.uniform_np = 0  # every instance takes the resident Tsit5 body of this np
 
     # ------------------------------------------------------------------ launch
     def reset(self):
@@ -199,6 +200,13 @@ class DensePlan:
                     scratch.numel(), steps, c_p(flags.data_ptr()), sp,
                 ),
                 'dqh_mesolve_tsit5_dense_grid',
+            )
+            return
+        if self.uniform_np:
+            _native.check(
+                lib.dqh_mesolve_tsit5_dense_uniform(_native.ctypes.c_void_p(self.d_descs.data_ptr()), self.ninst,
+                                                    self.uniform_np, sp),
+                'dqh_mesolve_tsit5_dense_uniform',
             )
             return
         fn = {
@@ -470,6 +478,13 @@ def build_plan(
     plan.descs = descs
     plan.d_descs = torch.from_numpy(descs.view(np.uint8).copy()).to(dev)
     plan.K = K
+    if (
+        kind == 'tsit5'
+        and npad in (16, 32, 48, 64)
+        and np.all(descs['nL'] <= 3)
+        and not np.any(descs['flags'] & _native.FLAG_GLOBAL)
+    ):
+        plan.uniform_np = npad
     if kind == 'tsit5' and use_grid(grid_tsit5, n, B, npwc_max, nE, save_states):
         lib = _native.lib()
         nbytes = int(lib.dqh_tsit5_grid_scratch_bytes(npad, B))

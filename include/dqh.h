@@ -94,6 +94,12 @@ int dqh_mesolve_tsit5_dense(const dqh_me_inst *d_insts, int64_t ninst, void *str
  * tmp + tmp^dag with tmp = sum_k D_k rho C_k to the FP64 state. */
 int dqh_mesolve_rouchon1_dense(const dqh_me_inst *d_insts, int64_t ninst, void *stream);
 
+/* dqh_mesolve_tsit5_dense for a batch in which every instance has padded dimension
+ * np (16, 32, 48 or 64), at most 3 jump operators and no DQH_FLAG_GLOBAL: a kernel
+ * that holds only the LDS/register-resident body of that size. An instance that
+ * does not match gets status DQH_UNSUPPORTED in its stats and is not solved. */
+int dqh_mesolve_tsit5_dense_uniform(const dqh_me_inst *d_insts, int64_t ninst, int64_t np, void *stream);
+
 /* The same Tsit5 solves spread over the whole grid: every stage of every
  * instance is a set of grid-wide launches (MFMA tiles of the two GEMM phases, an
  * elementwise stage update) and the PID controller runs on the device, one
