@@ -164,7 +164,7 @@ def test_dia_quadrature_equals_dense_quadrature():
         for j, O in enumerate(ops):
             for q in range(Q):
                 L, R = lam[b, q].astype(complex), rho[b, q].astype(complex)
-                ref[b, j, q // nodes] += w[q] * np.real(np.trace(L @ (-1j) * (O @ R - R @ O)))
+                ref[b, j, q // nodes] += w[q] * np.real(np.trace(L @ (-1j * (O @ R - R @ O))))
     scale = np.abs(ref).max()
     assert np.abs(ga - ref).max() <= 1e-5 * scale, np.abs(ga - ref).max() / scale
     assert np.abs(gd - ref).max() <= 1e-5 * scale, np.abs(gd - ref).max() / scale
