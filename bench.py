@@ -496,7 +496,7 @@ def main(argv=None):
         torch.cuda.synchronize()
         w = run_plan_timed(p1, 1, stream)
         plain_lat.append(w[0])
-    plain_sps = len(plain_lat) / sum(plain_lat)
+    plain_sps = len(plain_lat) / sum(plain_lat) if plain_lat else None
 
     # error of sampled instances vs the complex128 oracle, and the CPU baseline timing
     cpu = None
@@ -551,12 +551,12 @@ def main(argv=None):
             'parallelism': 'one persistent workgroup per instance',
         },
         'plain_solves_per_s': plain_sps,
-        'batched_vs_plain': solves_per_s / plain_sps,
+        'batched_vs_plain': solves_per_s / plain_sps if plain_sps else None,
         'latency_s': {
             'batched_p50': float(np.percentile(walls, 50)),
             'batched_p99': float(np.percentile(walls, 99)),
-            'plain_p50': float(np.percentile(lat_sorted, 50)),
-            'plain_p99': float(np.percentile(lat_sorted, 99)),
+            'plain_p50': float(np.percentile(lat_sorted, 50)) if lat_sorted else None,
+            'plain_p99': float(np.percentile(lat_sorted, 99)) if lat_sorted else None,
         },
         'steps_per_instance': {
             'mean': float(np.mean(res['nsteps'])),

@@ -139,7 +139,7 @@ __global__ __launch_bounds__(256) void k_body(const dqh_me_inst *__restrict__ in
   const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
   double acc = 0.0;
   if (e < (int64_t)nn) {
-    const int i = (int)(e / np), j = (int)(e % np);
+    const int i = (int)e / np, j = (int)e - ((int)e / np) * np;  // e < np^2 < 2^31
     const float2 f = field_at(tmpb(p), np, i, j);
     kslot(p, s, c.rot)[e] = f;
     const double2 yv = gp<const double2>(p.y)[e];
@@ -224,7 +224,7 @@ __global__ __launch_bounds__(256) void k_commit(const dqh_me_inst *__restrict__ 
   const size_t nn = (size_t)np * np;
   const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (e < (int64_t)nn) {
-    const int i = (int)(e / np), j = (int)(e % np);
+    const int i = (int)e / np, j = (int)e - ((int)e / np) * np;  // e < np^2 < 2^31
     double2 *yp = gp<double2>(p.y) + e;
     const double2 yv = *yp;
     double2 y1 = yv;
@@ -286,7 +286,7 @@ __global__ __launch_bounds__(256) void k_init(const dqh_me_inst *__restrict__ in
     const double2 yv = gp<const double2>(p.y)[e];
     const float2 f = make_float2((float)yv.x, (float)yv.y);
     ysb(p)[e] = f;
-    const int i = (int)(e / np), j = (int)(e % np);
+    const int i = (int)e / np, j = (int)e - ((int)e / np) * np;  // e < np^2 < 2^31
     if ((p.flags & DQH_FLAG_SAVE_STATES) && i < n && j < n) {
       float2 *st = gp<float2>(p.states);
       for (int64_t is = 0; is < is0; ++is) st[(size_t)is * n * n + (size_t)i * n + j] = f;
@@ -317,7 +317,7 @@ __global__ __launch_bounds__(256) void k_init_body(const dqh_me_inst *__restrict
   const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
   double a0 = 0.0, a1 = 0.0;
   if (e < (int64_t)nn) {
-    const int i = (int)(e / np), j = (int)(e % np);
+    const int i = (int)e / np, j = (int)e - ((int)e / np) * np;  // e < np^2 < 2^31
     const float2 f = field_at(tmpb(p), np, i, j);
     const double2 yv = gp<const double2>(p.y)[e];
     const double sc = p.atol + p.rtol * sqrt(sq(yv));
@@ -401,7 +401,7 @@ __global__ __launch_bounds__(256) void k_finish(const dqh_me_inst *__restrict__ 
   const size_t nn = (size_t)np * np;
   const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (e < (int64_t)nn) {
-    const int i = (int)(e / np), j = (int)(e % np);
+    const int i = (int)e / np, j = (int)e - ((int)e / np) * np;  // e < np^2 < 2^31
     if (i < n && j < n) {
       const double2 yv = gp<const double2>(p.y)[e];
       const float2 f = make_float2((float)yv.x, (float)yv.y);

@@ -153,9 +153,9 @@ __global__ __launch_bounds__(256) void k_kraus(int64_t n, int64_t n_true, dqh_di
                                                double theta) {
   const size_t nn = (size_t)n * n;
   const size_t inst = blockIdx.y;
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (int64_t)nn) return;
-  const int i = (int)(e / n), j = (int)(e % n);
+  int i, j;
+  if (!diat::row_elem(n, i, j)) return;
+  const int64_t e = (int64_t)i * n + j;
   const float2 *Ei = E + inst * nn;
   const diat::GlobalX X{r32_in + inst * nn, Ei, n, i, j};
   const float2 acc = diat::stencil(X, ops, true, n, i, j);
@@ -179,9 +179,9 @@ __global__ __launch_bounds__(256) void k_kraus_left(int64_t n, dqh_dia_ops ops, 
                                                     const float2 *__restrict__ E, float2 *__restrict__ dY) {
   const size_t nn = (size_t)n * n;
   const size_t inst = blockIdx.y;
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (int64_t)nn) return;
-  const int i = (int)(e / n), j = (int)(e % n);
+  int i, j;
+  if (!diat::row_elem(n, i, j)) return;
+  const int64_t e = (int64_t)i * n + j;
   const float2 *R = r32 + inst * nn, *Ei = E + inst * nn;
   const float2 *D = diat::cgp<float2>(ops.diags[0]);
   float ar = 0.f, ai = 0.f;
@@ -207,9 +207,9 @@ __global__ __launch_bounds__(256) void k_kraus_right(int64_t n, int64_t n_true, 
                                                      int64_t nsave, int64_t save_idx, double theta) {
   const size_t nn = (size_t)n * n;
   const size_t inst = blockIdx.y;
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (int64_t)nn) return;
-  const int i = (int)(e / n), j = (int)(e % n);
+  int i, j;
+  if (!diat::row_elem(n, i, j)) return;
+  const int64_t e = (int64_t)i * n + j;
   const float2 *R = r32_in + inst * nn, *Ei = E + inst * nn, *Y = dY + inst * nn;
   const float2 *D = diat::cgp<float2>(ops.diags[0]);
   float ar = 0.f, ai = 0.f;
@@ -298,9 +298,9 @@ __global__ __launch_bounds__(256) void k_save(int64_t n, int64_t n_true, const d
                                               float2 *__restrict__ states, int64_t nsave, int64_t save_idx) {
   const size_t nn = (size_t)n * n;
   const size_t inst = blockIdx.y;
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (int64_t)nn) return;
-  const int i = (int)(e / n), j = (int)(e % n);
+  int i, j;
+  if (!diat::row_elem(n, i, j)) return;
+  const int64_t e = (int64_t)i * n + j;
   if (i < n_true && j < n_true) {
     const double2 v = r64[inst * nn + e];
     states[((inst * nsave + save_idx) * n_true + i) * n_true + j] = make_float2((float)v.x, (float)v.y);
@@ -314,9 +314,9 @@ __global__ __launch_bounds__(256) void k_save_interp(int64_t n, int64_t n_true, 
                                                      int64_t nsave, int64_t save_idx, double theta) {
   const size_t nn = (size_t)n * n;
   const size_t inst = blockIdx.y;
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (int64_t)nn) return;
-  const int i = (int)(e / n), j = (int)(e % n);
+  int i, j;
+  if (!diat::row_elem(n, i, j)) return;
+  const int64_t e = (int64_t)i * n + j;
   if (i < n_true && j < n_true) {
     const double2 v = r64[inst * nn + e];
     const float2 o = r32_old[inst * nn + e];
@@ -357,7 +357,7 @@ extern "C" int dqh_mesolve_rouchon1_dia(const dqh_dia_rouchon1 *pb, void *stream
   float2 *D1 = reinterpret_cast<float2 *>(p.work);
   float2 *E = D1 + p.ninst * nn;
   float2 *states = reinterpret_cast<float2 *>(p.states);
-  const dim3 gel((unsigned)((nn + 255) / 256), (unsigned)p.ninst);
+  const dim3 gel = diat::row_grid(n, p.ninst);
   const dim3 gsolve((unsigned)(n / dia::kNB), (unsigned)p.ninst);
   const dim3 gtile((unsigned)((n / diat::kTI) * (n / diat::kTJ)), (unsigned)p.ninst);
   int maxoff = 0;

@@ -45,9 +45,9 @@ __global__ __launch_bounds__(256) void k_rhs_global(int64_t n, dqh_dia_ops ops, 
                                                     float2 *__restrict__ drho) {
   const size_t nn = (size_t)n * n;
   const size_t inst = blockIdx.y;
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (int64_t)nn) return;
-  const int i = (int)(e / n), j = (int)(e % n);
+  int i, j;
+  if (!diat::row_elem(n, i, j)) return;
+  const int64_t e = (int64_t)i * n + j;
   const diat::GlobalX X{rho + inst * nn, nullptr, n, i, j};
   drho[inst * nn + e] = diat::stencil(X, ops, false, n, i, j);
 }
@@ -89,8 +89,7 @@ extern "C" int dqh_lindblad_rhs_dia(const dqh_dia_rhs *pb, void *stream) {
   else if (halo == 32)
     hipLaunchKernelGGL(diar::k_rhs_tiled<32>, gtile, dim3(diat::kThreads), 0, st, p.n, p.ops, rho, drho);
   else
-    hipLaunchKernelGGL(diar::k_rhs_global, dim3((unsigned)((p.n * p.n + 255) / 256), (unsigned)p.ninst), dim3(256), 0,
-                       st, p.n, p.ops, rho, drho);
+    hipLaunchKernelGGL(diar::k_rhs_global, diat::row_grid(p.n, p.ninst), dim3(256), 0, st, p.n, p.ops, rho, drho);
   const hipError_t err = hipGetLastError();
   if (err != hipSuccess) return fail(DQH_HIP_ERROR, std::string("dqh_lindblad_rhs_dia: ") + hipGetErrorString(err));
   return DQH_OK;

@@ -109,6 +109,21 @@ __device__ __forceinline__ void load_halo(float2 *sx, const float2 *__restrict__
   }
 }
 
+// Elementwise launches over an n x n matrix use a row-blocked grid: grid.x =
+// n * ceil(n / 256) workgroups of 256 threads, workgroup b covers columns
+// [256 (b mod cpr), +256) of row b / cpr. The row is one scalar division per
+// workgroup instead of a 64-bit division per element.
+inline dim3 row_grid(int64_t n, int64_t ninst) { return dim3((unsigned)(n * ((n + 255) / 256)), (unsigned)ninst); }
+
+__device__ __forceinline__ bool row_elem(int64_t n, int &i, int &j) {
+  const unsigned cpr = (unsigned)((n + 255) >> 8);
+  const unsigned b = blockIdx.x;
+  const unsigned r = b / cpr;
+  i = (int)r;
+  j = (int)((b - r * cpr) * 256u + threadIdx.x);
+  return j < n;
+}
+
 __device__ __forceinline__ int clampi(int64_t a, int64_t n) { return (int)(a < 0 ? 0 : (a >= n ? n - 1 : a)); }
 
 // X read from the LDS halo tile around element (ii, jj) of the tile

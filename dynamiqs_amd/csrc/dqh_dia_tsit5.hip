@@ -116,9 +116,8 @@ __device__ __forceinline__ void sweep(const V &v, const dqh_dia_ops &ops, const 
       fn(i, j, diat::stencil(diat::LdsX<HALO>::at(sx, ii, jj), ops, false, n, i, j));
     }
   } else {
-    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (e < n * n) {
-      const int i = (int)(e / n), j = (int)(e % n);
+    int i, j;
+    if (diat::row_elem(n, i, j)) {
       const diat::GlobalX X{in, nullptr, n, i, j};
       fn(i, j, diat::stencil(X, ops, false, n, i, j));
     }
@@ -230,9 +229,9 @@ __global__ __launch_bounds__(256) void k_commit(V v, int cur) {
   }
   const int64_t n = v.n;
   const size_t nn = (size_t)n * n;
-  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (e < (int64_t)nn) {
-    const int i = (int)(e / n), j = (int)(e % n);
+  int i, j;
+  if (diat::row_elem(n, i, j)) {
+    const int64_t e = (int64_t)i * n + j;
     double2 *yp = v.y + inst * nn + e;
     const double2 yv = *yp;
     double2 y1 = yv;
@@ -287,12 +286,12 @@ __global__ __launch_bounds__(256) void k_init(V v) {
   while (is0 < v.nsave && v.tsave[is0] <= v.t0) ++is0;
   const int64_t n = v.n;
   const size_t nn = (size_t)n * n;
-  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (e < (int64_t)nn) {
+  int i, j;
+  if (diat::row_elem(n, i, j)) {
+    const int64_t e = (int64_t)i * n + j;
     const double2 yv = v.y[inst * nn + e];
     const float2 f = make_float2((float)yv.x, (float)yv.y);
     ysbuf(v, 0, inst)[e] = f;
-    const int i = (int)(e / n), j = (int)(e % n);
     const int64_t ntn = v.n_true * v.n_true;
     if (i < v.n_true && j < v.n_true)
       for (int64_t is = 0; is < is0; ++is) v.states[(inst * v.nsave + is) * ntn + (size_t)i * v.n_true + j] = f;
@@ -364,8 +363,9 @@ __global__ __launch_bounds__(256) void k_init_h0(V v) {
   const double h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
   const int64_t n = v.n;
   const size_t nn = (size_t)n * n;
-  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (e < (int64_t)nn) {
+  int i, j;
+  if (diat::row_elem(n, i, j)) {
+    const int64_t e = (int64_t)i * n + j;
     const double2 yv = v.y[inst * nn + e];
     const float2 kq = kslot(v, 0, 0, inst)[e];
     ysbuf(v, 1, inst)[e] = make_float2((float)(yv.x + h0 * kq.x), (float)(yv.y + h0 * kq.y));
@@ -395,8 +395,9 @@ __global__ __launch_bounds__(256) void k_init_dt(V v) {
   if (!(hn > 0.0) || !(hn > 1e-14 * fmax(1.0, fabs(o.t)))) o.status = DQH_NONFINITE;
   const int64_t n = v.n;
   const size_t nn = (size_t)n * n;
-  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (e < (int64_t)nn && o.status == 0) {
+  int i, j;
+  if (diat::row_elem(n, i, j) && o.status == 0) {
+    const int64_t e = (int64_t)i * n + j;
     const double2 yv = v.y[inst * nn + e];
     const float2 kq = kslot(v, 0, 0, inst)[e];
     ysbuf(v, 0, inst)[e] = make_float2((float)(yv.x + hn * kA[1][0] * kq.x), (float)(yv.y + hn * kA[1][0] * kq.y));
@@ -410,9 +411,9 @@ __global__ __launch_bounds__(256) void k_finish(V v) {
   const Ctrl c = v.ctrl[inst];
   const int64_t n = v.n;
   const size_t nn = (size_t)n * n;
-  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (e < (int64_t)nn) {
-    const int i = (int)(e / n), j = (int)(e % n);
+  int i, j;
+  if (diat::row_elem(n, i, j)) {
+    const int64_t e = (int64_t)i * n + j;
     const int64_t ntn = v.n_true * v.n_true;
     if (i < v.n_true && j < v.n_true) {
       const double2 yv = v.y[inst * nn + e];
@@ -435,7 +436,7 @@ inline size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
 inline Layout layout(int64_t n, int64_t ninst) {
   const size_t nn = (size_t)n * n;
-  const size_t npart = (nn + 255) / 256;
+  const size_t npart = (size_t)n * ((n + 255) / 256);  // workgroups of a row-blocked launch
   Layout L;
   L.k = 0;
   L.ys = L.k + align256(7 * (size_t)ninst * nn * sizeof(float2));
@@ -609,7 +610,7 @@ extern "C" int dqh_mesolve_tsit5_dia(const dqh_dia_tsit5 *pb, void *stream) {
   v.tsave = reinterpret_cast<const double *>(p.tsave);
   v.states = reinterpret_cast<float2 *>(p.states);
   v.stats = reinterpret_cast<int64_t *>(p.stats);
-  const dim3 gflat((unsigned)((nn + 255) / 256), (unsigned)p.ninst);
+  const dim3 gflat = diat::row_grid(p.n, p.ninst);
   hipStream_t st = (hipStream_t)stream;
   const dim3 gtile((unsigned)((p.n / diat::kTI) * (p.n / diat::kTJ)), (unsigned)p.ninst);
   v.npart = halo > 0 ? gtile.x : gflat.x;

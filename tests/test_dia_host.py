@@ -224,3 +224,19 @@ def test_split_kraus_passes_match_stencil_form():
         A = dense(offs, d)
         want = want + A @ X @ A.conj().T
     np.testing.assert_allclose(inc, want, atol=1e-11)
+
+
+@pytest.mark.parametrize('n', [64, 100, 192, 256, 1024])
+def test_row_blocked_grid_covers_each_element_once(n):
+    """diat::row_grid / row_elem: grid.x = n * ceil(n / 256), workgroup b covers
+    columns 256 (b mod cpr) + t of row b / cpr; every (i, j) once, none outside."""
+    cpr = (n + 255) // 256
+    hits = np.zeros((n, n), dtype=int)
+    b = np.arange(n * cpr)[:, None]
+    t = np.arange(256)[None, :]
+    i = b // cpr + 0 * t
+    j = (b - (b // cpr) * cpr) * 256 + t
+    ok = j < n
+    np.add.at(hits, (i[ok], j[ok]), 1)
+    assert np.all(hits == 1)
+    assert np.all(i < n)
