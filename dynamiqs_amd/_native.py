@@ -48,6 +48,8 @@ ME_INST_FIELDS = [
     ('expects', np.uint64),
     ('final_state', np.uint64),
     ('stats', np.uint64),
+    ('step_log', np.uint64),
+    ('step_log_cap', np.int64),
 ]
 ME_INST_DTYPE = np.dtype(ME_INST_FIELDS)
 

@@ -407,6 +407,7 @@ __device__ __forceinline__ void tsit5_global(const dqh_me_inst &p, double *sh) {
     factor = fmin(fmax(factor, keep ? 1.0 : p.factormin), p.factormax);
     ++nsteps;
     if (keep) {
+      if (threadIdx.x == 0 && nacc < p.step_log_cap) gp<double>(p.step_log)[nacc] = t_new;
       ++nacc;
       while (isave < v.ntsave && tsave[isave] <= t_new) {
         const double th = (tsave[isave] - t) / h;

@@ -203,6 +203,7 @@ __global__ __launch_bounds__(256) void k_commit(const dqh_me_inst *__restrict__ 
   o.nsteps += 1;
   int64_t is_end = c.isave;
   if (keep) {
+    if (writer && c.nacc < p.step_log_cap) gp<double>(p.step_log)[c.nacc] = t_new;
     o.nacc += 1;
     o.t = t_new;
     o.rot = (c.rot + 6) % 7;  // FSAL: k_6 of this step is k_0 of the next

@@ -113,7 +113,7 @@ class _Arena:
 
 
 ARENA_FIELDS = ('tsave', 'Hnh', 'Dops', 'Cops', 'EopsT', 'Hpwc', 'pwc_times', 'pwc_vals', 'step_t', 'step_set')
-POINTER_FIELDS = ARENA_FIELDS + ('y', 'work', 'states', 'expects', 'final_state', 'stats')
+POINTER_FIELDS = ARENA_FIELDS + ('y', 'work', 'states', 'expects', 'final_state', 'stats', 'step_log')
 
 
 def check_descriptors(descs: np.ndarray, tensors) -> None:
@@ -136,6 +136,7 @@ def check_descriptors(descs: np.ndarray, tensors) -> None:
         'final_state': n * n * np.uint64(8),
         'expects': np.maximum(nE, one) * nt * np.uint64(8),
         'stats': np.full(len(descs), 64, dtype=np.uint64),
+        'step_log': descs['step_log_cap'].astype(np.uint64) * np.uint64(8),
     }
     for f in POINTER_FIELDS:
         v = descs[f].astype(np.uint64)
@@ -179,6 +180,7 @@ class DensePlan:
         self.nrhs_flops = 0.0
         self.grid = None  # (scratch, pinned flags, graph steps) for the grid-wide Tsit5
         self.uniform_np = 0  # every instance takes the resident Tsit5 body of this np
+        self.step_log = None
 
     # ------------------------------------------------------------------ launch
     def reset(self):
@@ -231,6 +233,9 @@ class DensePlan:
             out['expects'] = c64(self.expects).reshape(self.ninst, self.nE, self.ntsave)
         st = self.stats.cpu().numpy()
         out['nsteps'], out['naccepted'], out['nrejected'] = st[:, 0], st[:, 1], st[:, 2]
+        if getattr(self, 'step_log', None) is not None:
+            log = self.step_log.cpu().numpy()
+            out['step_ends'] = [log[b, : min(int(st[b, 1]), log.shape[1])] for b in range(self.ninst)]
         out['status'], out['nrhs'] = st[:, 3], st[:, 4]
         return out
 
@@ -290,6 +295,7 @@ def build_plan(
     form: str = 'auto',
     grid_tsit5: bool | str = 'auto',
     graph_steps: int = 16,
+    step_log_cap: int = 0,
 ) -> DensePlan:
     """Pack ``instances`` (same Hilbert dimension) for ``kind`` in {'tsit5', 'rouchon1', 'rhs'}.
 
@@ -474,7 +480,14 @@ def build_plan(
         d['final_state'] = final.data_ptr() + b * n * n * 8
         d['expects'] = expects.data_ptr() + b * max(nE, 1) * len(tsave) * 8
         d['stats'] = stats.data_ptr() + b * 64
-    check_descriptors(descs, [*arena.tensors, work, states, final, expects, stats])
+    step_log = None
+    if step_log_cap > 0 and kind == 'tsit5':
+        step_log = torch.zeros((B, int(step_log_cap)), dtype=torch.float64, device=dev)
+        for b in range(B):
+            descs[b]['step_log'] = step_log.data_ptr() + b * int(step_log_cap) * 8
+            descs[b]['step_log_cap'] = int(step_log_cap)
+    plan.step_log = step_log
+    check_descriptors(descs, [*arena.tensors, work, states, final, expects, stats] + ([step_log] if step_log is not None else []))
     plan.descs = descs
     plan.d_descs = torch.from_numpy(descs.view(np.uint8).copy()).to(dev)
     plan.K = K

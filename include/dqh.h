@@ -68,6 +68,10 @@ typedef struct dqh_me_inst {
   uint64_t expects;   /* c64[nE*ntsave]                                             */
   uint64_t final_state; /* c64[n*n]                                                 */
   uint64_t stats;     /* int64[8]: nsteps, naccepted, nrejected, status, nrhs       */
+  /* optional (0: off): end time of every accepted Tsit5 step, the first
+   * step_log_cap of them (replaying the same steps in another precision) */
+  uint64_t step_log;  /* double[step_log_cap]                                        */
+  int64_t step_log_cap;
 } dqh_me_inst;
 
 #define DQH_FLAG_SAVE_STATES 1

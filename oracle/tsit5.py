@@ -102,6 +102,7 @@ class Tsit5Stats:
     nrejected: int = 0
     nrhs: int = 0
     dts: list = field(default_factory=list)
+    ends: list = field(default_factory=list)  # end time of every accepted step
 
 
 def initial_step(f, t0: float, y0: np.ndarray, f0: np.ndarray, rtol: float, atol: float, stats=None) -> float:
@@ -207,6 +208,7 @@ def integrate(
         if keep:
             stats.naccepted += 1
             stats.dts.append(h)
+            stats.ends.append(t_new)
             while isave < len(tsave) and tsave[isave] <= t_new:
                 theta = (tsave[isave] - t) / h
                 w = interp_weights(theta)
@@ -223,3 +225,36 @@ def integrate(
         saved.append(save(y))
         isave += 1
     return saved, y, stats
+
+
+def integrate_steps(f, y0: np.ndarray, tsave: np.ndarray, step_ends, *, t0: float | None = None, save=None):
+    """Tsit5 over a prescribed sequence of accepted steps (``step_ends``: the end
+    time of each step, as an adaptive solve recorded them): the same stages,
+    solution weights and continuous extension as ``integrate`` without the
+    controller, so that a solve in another precision can be replayed in float64
+    on the identical step sequence. ``f(t, y, tseg)`` as in ``integrate``."""
+    tsave = np.asarray(tsave, dtype=np.float64)
+    t = float(tsave[0] if t0 is None else t0)
+    save = save or (lambda x: x.copy())
+    y = np.array(y0, dtype=np.complex128)
+    saved = []
+    isave = 0
+    while isave < len(tsave) and tsave[isave] <= t:
+        saved.append(save(y))
+        isave += 1
+    for t_new in np.asarray(step_ends, dtype=np.float64):
+        h = t_new - t
+        ks = [h * f(t, y, t)]
+        for s in range(1, 7):
+            ys = y + sum(A[s, j] * ks[j] for j in range(s))
+            ks.append(h * f(t + C[s] * h, ys, t))
+        while isave < len(tsave) and tsave[isave] <= t_new:
+            w = interp_weights((tsave[isave] - t) / h)
+            saved.append(save(y + sum(w[j] * ks[j] for j in range(7))))
+            isave += 1
+        y = y + sum(B[j] * ks[j] for j in range(6))
+        t = t_new
+    while isave < len(tsave):
+        saved.append(save(y))
+        isave += 1
+    return saved, y

@@ -429,3 +429,29 @@ def test_tsit5_no_sliver_steps_at_breakpoints():
     _, _, st = tsit5.integrate(f, rho0, tsave, rtol=1e-6, atol=1e-6, jump_ts=times)
     assert min(st.dts) > 1e-6, min(st.dts)
     assert st.nrejected == 0 and st.naccepted <= 82
+
+
+def test_tsit5_replay_of_own_steps_reproduces_adaptive_solve():
+    """integrate_steps on the accepted step ends that integrate recorded gives the
+    adaptive solution (same stages, weights and interpolant; only the FSAL stage
+    is recomputed instead of reused), on a driven lossy cavity with PWC jumps."""
+    from oracle import lindblad, operators as op, tsit5
+
+    n = 10
+    a = op.destroy(n)
+    times = np.linspace(0.0, 2.0, 9)
+    u = np.random.default_rng(2).normal(size=8)
+    Ls = [np.sqrt(0.2) * a]
+
+    def f(t, y, tseg):
+        k = np.searchsorted(times, tseg, side='right') - 1
+        c = u[k] if 0 <= k < 8 else 0.0
+        return lindblad.rhs_hermitian(a.conj().T @ a + c * (a + a.conj().T), Ls, y)
+
+    tsave = np.linspace(0.0, 2.0, 7)
+    rho0 = op.coherent_dm(n, 0.7)
+    saved, ylast, st = tsit5.integrate(f, rho0, tsave, rtol=1e-7, atol=1e-9, jump_ts=times)
+    replay, yrep = tsit5.integrate_steps(f, rho0, tsave, st.ends)
+    assert len(st.ends) == st.naccepted
+    np.testing.assert_allclose(np.stack(replay), np.stack(saved), atol=1e-12)
+    np.testing.assert_allclose(yrep, ylast, atol=1e-12)
