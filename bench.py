@@ -31,6 +31,13 @@ if ROOT not in sys.path:
 FP32_MFMA_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per batched launch of the resident Tsit5 kernel on the default workload
+# (Kerr n=64, B=256), from two rocprofv3 --pmc runs of this script (FETCH_SIZE,
+# then WRITE_SIZE) scaled by the same counters on a device copy of known size
+# (--config copy: FETCH_SIZE x 2, WRITE_SIZE x 1): 1 880 670 853 KiB fetched,
+# 1 089 909 003 KiB written. Mostly register spill traffic of the np = 64 body.
+KERR_TRAFFIC_BYTES = 1_880_670_853 * 1024 * 2 + 1_089_909_003 * 1024
+
 
 def kerr_instances(B: int, n: int = 64, seed: int = 0):
     """Two-photon driven Kerr oscillators (complex128 host operators)."""
@@ -601,8 +608,8 @@ def main(argv=None):
             'peak': FP32_MFMA_PEAK_TFLOPS,
             'unit': 'TFLOP/s',
             'frac': achieved / FP32_MFMA_PEAK_TFLOPS,
-            'traffic': None,
-            'kernel': 'dqh::k_tsit5',
+            'traffic': KERR_TRAFFIC_BYTES if (args.batch == 256 and plan.uniform_np == 64) else None,
+            'kernel': 'dqh::k_tsit5_res<64>' if plan.uniform_np == 64 else 'dqh::k_tsit5',
             'kernel_avg_ms': float(np.mean(kern_ms)),
             'flops_per_launch': flops,
         },
