@@ -52,36 +52,42 @@ constexpr int kWaves = 8;
 constexpr int kThreads = kWaves * 64;
 
 // acc[t] += A[row0:+16, kbeg:64] (global, row-major lda=64) @ B[kbeg:64, col0+16t:+16]
-// (LDS, ld=kLD); kbeg (a multiple of 16) skips the columns where A is zero.
-__device__ __forceinline__ void mac_strip2(Acc (&acc)[2], const float2 *__restrict__ A, const float2 *B, int row0,
-                                           int col0, int lane, int kbeg = 0) {
+// (LDS, row stride ldb); kbeg (a multiple of 16) skips the columns where A is zero.
+template <int TJ>
+__device__ __forceinline__ void mac_strip(Acc (&acc)[TJ], const float2 *__restrict__ A, const float2 *B, int ldb,
+                                          int row0, int col0, int lane, int kbeg = 0) {
   const int r = lane & 15, g = lane >> 4;
   const float2 *Ap = A + (row0 + r) * kNB + 4 * g;
-  const float2 *Bp = B + (4 * g) * kLD + col0 + r;
+  const float2 *Bp = B + (4 * g) * ldb + col0 + r;
 #pragma unroll 4
   for (int k0 = kbeg; k0 < kNB; k0 += 16) {
     const float4 a01 = *reinterpret_cast<const float4 *>(Ap + k0);
     const float4 a23 = *reinterpret_cast<const float4 *>(Ap + k0 + 2);
-    const float2 *bp = Bp + k0 * kLD;
+    const float2 *bp = Bp + k0 * ldb;
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
+    for (int t = 0; t < TJ; ++t) {
       cmfma(acc[t], a01.x, a01.y, bp[16 * t]);
-      cmfma(acc[t], a01.z, a01.w, bp[kLD + 16 * t]);
-      cmfma(acc[t], a23.x, a23.y, bp[2 * kLD + 16 * t]);
-      cmfma(acc[t], a23.z, a23.w, bp[3 * kLD + 16 * t]);
+      cmfma(acc[t], a01.z, a01.w, bp[ldb + 16 * t]);
+      cmfma(acc[t], a23.x, a23.y, bp[2 * ldb + 16 * t]);
+      cmfma(acc[t], a23.z, a23.w, bp[3 * ldb + 16 * t]);
     }
   }
 }
 
-// Banded left solve of one column panel. mode 0: s = rho32, output D1 = x - s.
-// mode 1: s = rho32 + D1^dag, output E = D1^dag + (x - s).
-// grid: (n / kNB panels, ninst); block: kThreads.
+// Banded left solve of one column panel of PW columns (64 or 32). mode 0:
+// s = rho32, output D1 = x - s. mode 1: s = rho32 + D1^dag, output
+// E = D1^dag + (x - s). grid: (n / PW panels, ninst); block: kThreads. The 64 x PW
+// output of a block row is 4 x PW/16 tiles of 16 x 16, TJ = PW / 32 per wave;
+// narrower panels give twice the workgroups (two per CU) for small batches.
+template <int PW>
 __global__ __launch_bounds__(kThreads) void k_lsolve(int mode, int64_t n, const float2 *__restrict__ Vp,
                                                      const float2 *__restrict__ Wn, const float2 *__restrict__ rho32,
                                                      const float2 *__restrict__ D1, float2 *__restrict__ out) {
-  __shared__ float2 s_lds[kNB * kLD];     // s_i panel
-  __shared__ float2 prev[kNB * kLD];      // x_{i+1} panel
-  __shared__ float2 tr[kNB * kLD];        // D1^dag tile (mode 1)
+  constexpr int LDP = PW + 2;  // padded LDS row stride of a panel tile
+  constexpr int TJ = PW / 32;  // output tiles per wave
+  __shared__ float2 s_lds[kNB * LDP];  // s_i panel (64 x PW)
+  __shared__ float2 prev[kNB * LDP];   // x_{i+1} panel
+  __shared__ float2 tr[PW * kLD];      // D1[c0 + i][r0 + j] (mode 1), i < PW, j < 64
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int panel = blockIdx.x;
   const size_t inst = blockIdx.y;
@@ -89,48 +95,52 @@ __global__ __launch_bounds__(kThreads) void k_lsolve(int mode, int64_t n, const 
   const float2 *R = rho32 + inst * nn;
   const float2 *Dt = D1 + inst * nn;
   float2 *O = out + inst * nn;
-  const int c0 = panel * kNB;
+  const int c0 = panel * PW;
   const int NB = (int)(n / kNB);
-  // wave w owns output tiles (row tile w/2, col tiles 2*(w%2) and 2*(w%2)+1)
-  const int ti = wave >> 1, tj0 = (wave & 1) * 2;
-  for (int e = threadIdx.x; e < kNB * kLD; e += kThreads) prev[e] = make_float2(0.f, 0.f);
+  // wave w owns row tile w / 2 and column tiles TJ * (w % 2) .. + TJ - 1
+  const int ti = wave >> 1, tj0 = (wave & 1) * TJ;
+  for (int e = threadIdx.x; e < kNB * LDP; e += kThreads) prev[e] = make_float2(0.f, 0.f);
   for (int bi = NB - 1; bi >= 0; --bi) {
     const int r0 = bi * kNB;
-    // load s_i (64 x 64) and, in mode 1, the transposed conjugate tile of D1
-    for (int e = threadIdx.x; e < kNB * kNB; e += kThreads) {
-      const int i = e / kNB, j = e % kNB;
-      s_lds[i * kLD + j] = R[(size_t)(r0 + i) * n + c0 + j];
-      if (mode == 1) tr[i * kLD + j] = Dt[(size_t)(c0 + i) * n + r0 + j];  // D1[c0+i][r0+j]
+    // load s_i (64 x PW) and, in mode 1, the tile of D1 whose conjugate transpose is added
+    for (int e = threadIdx.x; e < kNB * PW; e += kThreads) {
+      const int i = e / PW, j = e % PW;
+      s_lds[i * LDP + j] = R[(size_t)(r0 + i) * n + c0 + j];
     }
+    if (mode == 1)
+      for (int e = threadIdx.x; e < PW * kNB; e += kThreads) {
+        const int i = e / kNB, j = e % kNB;
+        tr[i * kLD + j] = Dt[(size_t)(c0 + i) * n + r0 + j];  // D1[c0+i][r0+j]
+      }
     __syncthreads();
     if (mode == 1) {
-      for (int e = threadIdx.x; e < kNB * kNB; e += kThreads) {
-        const int i = e / kNB, j = e % kNB;
+      for (int e = threadIdx.x; e < kNB * PW; e += kThreads) {
+        const int i = e / PW, j = e % PW;
         const float2 d = tr[j * kLD + i];  // D1[c0+j][r0+i] -> (D1^dag)[r0+i][c0+j] = conj
-        s_lds[i * kLD + j].x += d.x;
-        s_lds[i * kLD + j].y -= d.y;
+        s_lds[i * LDP + j].x += d.x;
+        s_lds[i * LDP + j].y -= d.y;
       }
       __syncthreads();
     }
-    Acc acc[2];
+    Acc acc[TJ];
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
+    for (int t = 0; t < TJ; ++t) {
       acc[t].re = f4{0.f, 0.f, 0.f, 0.f};
       acc[t].im = f4{0.f, 0.f, 0.f, 0.f};
     }
     // V'_i is upper triangular: rows ti*16.. of it are zero left of column ti*16
-    mac_strip2(acc, Vp + (size_t)bi * kNB * kNB, s_lds, ti * 16, tj0 * 16, lane, ti * 16);
-    if (bi + 1 < NB) mac_strip2(acc, Wn + (size_t)bi * kNB * kNB, prev, ti * 16, tj0 * 16, lane);
+    mac_strip<TJ>(acc, Vp + (size_t)bi * kNB * kNB, s_lds, LDP, ti * 16, tj0 * 16, lane, ti * 16);
+    if (bi + 1 < NB) mac_strip<TJ>(acc, Wn + (size_t)bi * kNB * kNB, prev, LDP, ti * 16, tj0 * 16, lane);
     __syncthreads();  // every wave is done reading prev
     const int r = lane & 15, g = lane >> 4;
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
+    for (int t = 0; t < TJ; ++t) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int i = ti * 16 + 4 * g + v, j = (tj0 + t) * 16 + r;
         const float2 d = make_float2(acc[t].re[v], acc[t].im[v]);
-        const float2 s = s_lds[i * kLD + j];
-        prev[i * kLD + j] = make_float2(s.x + d.x, s.y + d.y);
+        const float2 sv = s_lds[i * LDP + j];
+        prev[i * LDP + j] = make_float2(sv.x + d.x, sv.y + d.y);
         float2 o = d;
         if (mode == 1) {
           const float2 tt = tr[j * kLD + i];
@@ -358,7 +368,12 @@ extern "C" int dqh_mesolve_rouchon1_dia(const dqh_dia_rouchon1 *pb, void *stream
   float2 *E = D1 + p.ninst * nn;
   float2 *states = reinterpret_cast<float2 *>(p.states);
   const dim3 gel = diat::row_grid(n, p.ninst);
-  const dim3 gsolve((unsigned)(n / dia::kNB), (unsigned)p.ninst);
+  // panel width of the block solves: 32 columns when 64-column panels would leave
+  // fewer than two workgroups per CU (256 CUs)
+  if (p.solve_panel != 0 && p.solve_panel != 32 && p.solve_panel != 64)
+    return fail(DQH_BAD_ARG, "dqh_mesolve_rouchon1_dia: solve_panel must be 0, 32 or 64");
+  const int panel = p.solve_panel != 0 ? (int)p.solve_panel : ((n / dia::kNB) * p.ninst < 512 ? 32 : 64);
+  const dim3 gsolve((unsigned)(n / panel), (unsigned)p.ninst);
   const dim3 gtile((unsigned)((n / diat::kTI) * (n / diat::kTJ)), (unsigned)p.ninst);
   int maxoff = 0;
   for (int s = 0; s < p.nsets; ++s) maxoff = std::max(maxoff, diat::max_abs_offset(p.ops[s]));
@@ -380,8 +395,13 @@ extern "C" int dqh_mesolve_rouchon1_dia(const dqh_dia_rouchon1 *pb, void *stream
     const int set = (m == p.nsteps - 1) ? p.nsets - 1 : 0;
     const float2 *Vp = reinterpret_cast<const float2 *>(p.Vp[set]);
     const float2 *Wn = reinterpret_cast<const float2 *>(p.Wn[set]);
-    hipLaunchKernelGGL(dia::k_lsolve, gsolve, dim3(dia::kThreads), 0, st, 0, n, Vp, Wn, r32[cur], D1, D1);
-    hipLaunchKernelGGL(dia::k_lsolve, gsolve, dim3(dia::kThreads), 0, st, 1, n, Vp, Wn, r32[cur], D1, E);
+    if (panel == 32) {
+      hipLaunchKernelGGL(dia::k_lsolve<32>, gsolve, dim3(dia::kThreads), 0, st, 0, n, Vp, Wn, r32[cur], D1, D1);
+      hipLaunchKernelGGL(dia::k_lsolve<32>, gsolve, dim3(dia::kThreads), 0, st, 1, n, Vp, Wn, r32[cur], D1, E);
+    } else {
+      hipLaunchKernelGGL(dia::k_lsolve<64>, gsolve, dim3(dia::kThreads), 0, st, 0, n, Vp, Wn, r32[cur], D1, D1);
+      hipLaunchKernelGGL(dia::k_lsolve<64>, gsolve, dim3(dia::kThreads), 0, st, 1, n, Vp, Wn, r32[cur], D1, E);
+    }
     // saves inside this step: the kernel writes one interpolated state per launch
     int64_t sidx = -1;
     double th = 0.0;

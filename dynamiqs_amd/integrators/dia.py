@@ -57,6 +57,7 @@ class DiaRouchon1(ctypes.Structure):
         ('EopsT', ctypes.c_uint64),
         ('expects', ctypes.c_uint64),
         ('variant', ctypes.c_int64),
+        ('solve_panel', ctypes.c_int64),
     ]
 
 
@@ -185,6 +186,7 @@ def build_plan(
     t0: float | None = None,
     exp_ops=None,
     variant: int = SWEEP_AUTO,
+    solve_panel: int = 0,
 ) -> DiaPlan:
     """Pack a batch of Rouchon1 solves that share H and Ls (dense FP64 host arrays
     whose nonzeros lie on few diagonals) and differ in rho0 (B, n, n). ``exp_ops``
@@ -213,7 +215,7 @@ def build_plan(
         from .. import autotune
 
         variant = autotune.dia_sweep(H, Ls, B, kind='kraus', dt=float(dt))
-    prob.nsteps, prob.nsets, prob.variant = nsteps, len(sets), variant
+    prob.nsteps, prob.nsets, prob.variant, prob.solve_panel = nsteps, len(sets), variant, int(solve_panel)
     for si, h in enumerate(sets):
         Vp, Wn, ops = block_solve_operators(H, Ls, h, n_pad)
         prob.Vp[si] = arena.put_now(Vp.astype(np.complex64))

@@ -178,6 +178,7 @@ typedef struct dqh_dia_rouchon1 {
   uint64_t EopsT;      /* device c64[nE][n_true][n_true]: transposed E_k              */
   uint64_t expects;    /* device c64[ninst][nE][nsave]: Tr(E_k rho(tsave))            */
   int64_t variant;     /* DQH_SWEEP_*: form of the Kraus sweep                        */
+  int64_t solve_panel; /* columns per workgroup of the block solves: 0 (auto), 32, 64 */
 } dqh_dia_rouchon1;
 
 int64_t dqh_sizeof_dia_rouchon1(void);

@@ -310,3 +310,20 @@ def test_autotuner_times_and_caches_dia_and_dense_forms(tmp_path, monkeypatch):
     assert isinstance(g, bool)
     assert any(k.startswith('dense|') for k in autotune._memo)
     importlib.reload(autotune)
+
+
+def test_dia_block_solve_panel_widths_agree():
+    """The banded solves with 32- and 64-column panels per workgroup compute every
+    column with the same operations: bit-identical states."""
+    from dynamiqs_amd.integrators import dia
+    from oracle import operators as op
+
+    H, Ls, _ = _two_mode(16, 16)
+    rho0 = np.stack([op.coherent_dm((16, 16), (al, 0.0)) for al in (0.6, 1.4)])
+    tsave = np.linspace(0, 0.02, 3)
+    outs = []
+    for pw in (32, 64):
+        plan = dia.build_plan(H, Ls, rho0, tsave, 1e-3, variant=dia.SWEEP_SPLIT, solve_panel=pw)
+        plan.run()
+        outs.append(plan.results())
+    assert np.array_equal(outs[0]['states'], outs[1]['states'])
