@@ -350,6 +350,25 @@ def build_plan(
     # ---- operators, shared by key
     cache = {}
 
+    sampling_cache = {}
+
+    def pwc_sampling(pwc):
+        """Union breakpoint grid of the PWC terms and, per term, the value index on
+        each grid interval (resample_pwc's rule), cached for terms that share their
+        times arrays (e.g. a batch of control vectors on one grid)."""
+        key = tuple(id(t) for (t, _, _) in pwc) + tuple(len(v) for (_, v, _) in pwc)
+        if key not in sampling_cache:
+            grid = np.unique(np.concatenate([np.asarray(t, dtype=np.float64) for (t, _, _) in pwc]))
+            left = grid[:-1]
+            samplers = []
+            for t, v, _ in pwc:
+                t = np.asarray(t, dtype=np.float64)
+                idx = np.clip(np.searchsorted(t, left, side='right') - 1, 0, len(v) - 1)
+                samplers.append((idx, (left >= t[0]) & (left < t[-1])))
+            sampling_cache[key] = (grid, samplers, [t for (t, _, _) in pwc])  # keep the arrays alive
+        grid, samplers, _ = sampling_cache[key]
+        return grid, samplers
+
     def shared(key, build):
         # instances share a device copy only when every component of the key is known
         if key is None or any(k is None for k in key[1:]):
@@ -370,14 +389,15 @@ def build_plan(
         nLdev = K + 1
     else:
         nLdev = K
+    # fields common to the batch, as whole columns
+    descs['n'], descs['np'], descs['nE'], descs['ntsave'] = n, npad, nE, len(tsave)
+    descs['rtol'], descs['atol'], descs['safety'] = rtol, atol, safety
+    descs['factormin'], descs['factormax'] = factormin, factormax
+    descs['max_steps'] = max_steps
+    descs['t0'], descs['t1'] = t_start, t_end
+    descs['tsave'] = p_tsave
     for b, inst in enumerate(instances):
         d = descs[b]
-        d['n'], d['np'], d['nE'], d['ntsave'] = n, npad, nE, len(tsave)
-        d['rtol'], d['atol'], d['safety'] = rtol, atol, safety
-        d['factormin'], d['factormax'] = factormin, factormax
-        d['max_steps'] = max_steps
-        d['t0'], d['t1'] = t_start, t_end
-        d['tsave'] = p_tsave
         if per_inst_ts:
             ts_i = np.asarray(inst.tsave, dtype=np.float64)
             d['tsave'] = arena.put(ts_i)
@@ -405,8 +425,9 @@ def build_plan(
                     ('C', inst.key_L), lambda Ls=Ls: _pad(0.5 * np.stack([_dag(L) for L in Ls]), npad).astype(np.complex64)
                 )
             if inst.pwc:
-                grid = np.unique(np.concatenate([np.asarray(t, dtype=np.float64) for (t, _, _) in inst.pwc]))
-                vals = np.stack([resample_pwc(np.asarray(t), np.asarray(v)[None], grid)[0] for (t, v, _) in inst.pwc])
+                grid, samplers = pwc_sampling(inst.pwc)
+                vals = np.stack([np.where(inside, np.asarray(v, dtype=np.complex128)[idx], 0.0)
+                                 for (_, v, _), (idx, inside) in zip(inst.pwc, samplers)])
                 d['npwc'] = len(inst.pwc)
                 d['nseg'] = len(grid)
                 npwc_max = max(npwc_max, len(inst.pwc))

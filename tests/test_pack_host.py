@@ -57,3 +57,36 @@ def test_guard_rejects_unpatched_descriptor(host_device):
     descs['Hnh'] = 0x101  # a raw staging offset, never a device address
     with pytest.raises(RuntimeError, match='outside the plan allocations'):
         dense.check_descriptors(descs, [*plan.arena.tensors, plan.states, plan.final, plan.expects, plan.stats])
+
+
+def test_pwc_values_packed_as_resample_pwc(host_device):
+    """The cached per-grid sampling of build_plan packs the same PWC coefficients as
+    operators.resample_pwc on the union grid (two terms on different time grids,
+    shared by several instances, one with its own grid)."""
+    from dynamiqs_amd.integrators.operators import resample_pwc
+
+    rng = np.random.default_rng(3)
+    n = 8
+    t1 = np.linspace(0.0, 1.0, 6)
+    t2 = np.array([0.1, 0.35, 0.8, 1.2])
+    t3 = np.linspace(0.0, 1.2, 4)
+    H = np.diag(np.arange(n, dtype=float)).astype(complex)
+    X = np.eye(n, k=1) + np.eye(n, k=-1)
+    Y = 1j * np.eye(n, k=1) - 1j * np.eye(n, k=-1)
+    insts = []
+    for b in range(4):
+        tt = t3 if b == 3 else t1
+        pwc = [(tt, rng.normal(size=len(tt) - 1) + 1j * rng.normal(size=len(tt) - 1), X),
+               (t2, rng.normal(size=3), Y)]
+        insts.append(dense.Instance(H=H, Ls=[], rho0=np.eye(n) / n, pwc=pwc))
+    plan = dense.build_plan('tsit5', insts, np.array([0.0, 0.5, 1.0]), form='resident', grid_tsit5=False)
+    host = plan.arena.tensors[0].numpy()
+    base = plan.arena.tensors[0].data_ptr()
+    for b, inst in enumerate(insts):
+        grid = np.unique(np.concatenate([t for (t, _, _) in inst.pwc]))
+        want = np.stack([resample_pwc(t, np.asarray(v)[None], grid)[0] for (t, v, _) in inst.pwc])
+        d = plan.descs[b]
+        assert int(d['nseg']) == len(grid)
+        off = int(d['pwc_vals']) - base
+        got = host[off : off + want.size * 16].view(np.complex128).reshape(want.shape)
+        np.testing.assert_array_equal(got, want)
