@@ -133,184 +133,241 @@ def mesolve_adjoint(
     ``rate_ops``: operators A_r whose rates are differentiated; the corresponding
     jump operators in ``Ls`` are sqrt(kappa_r) A_r (dJ/dkappa_r is summed over the
     whole interval). ``ops`` may be empty (then ``u`` has shape (B, 0, nseg) and
-    ``times`` only sets the quadrature segments).
+    ``times`` only sets the quadrature segments). One-shot form of
+    :class:`AdjointPlan` (which packs once for repeated gradients).
     """
-    import time
-
-    import torch
-
-    H0 = np.asarray(H0, dtype=np.complex128)
-    ops = [np.asarray(o, dtype=np.complex128) for o in ops]
-    times = np.asarray(times, dtype=np.float64)
     u = np.asarray(u)
     if u.ndim == 2:
         u = u[:, None, :]
-    B, nterms, nseg = u.shape
-    if len(ops) != nterms or len(times) != nseg + 1:
-        raise ValueError('u must have shape (B, len(ops), len(times) - 1)')
-    if nterms > 8:
-        raise ValueError('at most 8 control operators')
-    rate_ops = [np.asarray(A, dtype=np.complex128) for A in rate_ops]
-    if len(rate_ops) > 8:
-        raise ValueError('at most 8 rate operators')
-    Ls = [np.asarray(L, dtype=np.complex128) for L in Ls]
-    rho0 = np.asarray(rho0, dtype=np.complex128)
-    if rho0.shape[-1] == 1:
-        rho0 = rho0 @ rho0.conj().T
-    O = np.asarray(O, dtype=np.complex128)
-    n = H0.shape[-1]
-    T = float(times[-1])
-    t_start = float(times[0])
-    timing = {}
+    plan = AdjointPlan(H0, ops, times, Ls, rho0, O, u.shape[0], rate_ops=rate_ops, rtol=rtol, atol=atol,
+                       checkpoint_every=checkpoint_every, nodes=nodes)
+    res = plan.run(u)
+    res.timing.update({k: v for k, v in plan.pack_timing.items()})
+    return res
 
-    # ---------------------------------------------------------------- forward
-    ck_idx = list(range(0, nseg, checkpoint_every)) + [nseg]
-    ck_t = times[ck_idx]
-    fwd_insts = [
-        dense.Instance(
-            H=H0, Ls=Ls, rho0=rho0, pwc=[(times, u[b, j], ops[j]) for j in range(nterms)], key_H=(), key_L=(), key_rho=()
-        )
-        for b in range(B)
-    ]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    fwd = dense.build_plan('tsit5', fwd_insts, ck_t, exp_ops=[O], rtol=rtol, atol=atol)
-    t1 = time.perf_counter()
-    fwd.run()
-    torch.cuda.synchronize()
-    t2 = time.perf_counter()
-    timing['forward_pack_s'] = t1 - t0
-    timing['forward_run_s'] = t2 - t1
-    fres = fwd.results()
-    J = fres['expects'][:, 0, -1].real.astype(np.float64)
-    rho_ck = fres['states']  # (B, nck, n, n) complex64
 
-    # ------------------------------------------ Gauss-Legendre nodes per segment
-    x, w = _gl(nodes)
-    seg_len = np.diff(times)
-    node_t = (times[:-1, None] + 0.5 * seg_len[:, None] * (1.0 + x[None, :])).reshape(-1)  # (nseg*nodes,)
-    node_w = (0.5 * seg_len[:, None] * w[None, :]).reshape(-1)
-    seg_first = (np.arange(nseg + 1) * nodes).astype(np.int32)
+class AdjointPlan:
+    """The three Tsit5 plans and the quadrature of the adjoint gradient, packed once
+    for a batch of B control vectors; :meth:`run` takes new controls and only
+    rewrites their values on the device (the operators, time grids, save times and
+    scratch stay resident), so repeated gradients (optimisation loops) pay the
+    GPU time and a few small copies."""
 
-    # ------------------------------------------- recompute rho from checkpoints
-    t3 = time.perf_counter()
-    rec_insts = []
-    nck = len(ck_idx) - 1
-    for b in range(B):
-        for c in range(nck):
-            k0, k1 = ck_idx[c], ck_idx[c + 1]
-            ts = node_t[k0 * nodes : k1 * nodes]
-            rec_insts.append(
-                dense.Instance(
-                    H=H0,
-                    Ls=Ls,
-                    rho0=rho_ck[b, c].astype(np.complex128),
-                    pwc=[(times, u[b, j], ops[j]) for j in range(nterms)],
-                    tsave=ts,
-                    t0=float(times[k0]),
-                    key_H=(),
-                    key_L=(),
+    def __init__(self, H0, ops, times, Ls, rho0, O, B: int, *, rate_ops=(), rtol: float = 1e-6,
+                 atol: float = 1e-6, checkpoint_every: int = 10, nodes: int = 5):
+        import time
+
+        import torch
+
+        t_begin = time.perf_counter()
+        H0 = np.asarray(H0, dtype=np.complex128)
+        ops = [np.asarray(o, dtype=np.complex128) for o in ops]
+        times = np.asarray(times, dtype=np.float64)
+        nterms, nseg = len(ops), len(times) - 1
+        if nterms > 8:
+            raise ValueError('at most 8 control operators')
+        if nseg < 1 or np.any(np.diff(times) <= 0):
+            raise ValueError('times must be strictly increasing with at least two entries')
+        if nseg % checkpoint_every != 0:
+            raise ValueError('checkpoint_every must divide the number of segments')
+        rate_ops = [np.asarray(A, dtype=np.complex128) for A in rate_ops]
+        if len(rate_ops) > 8:
+            raise ValueError('at most 8 rate operators')
+        Ls = [np.asarray(L, dtype=np.complex128) for L in Ls]
+        rho0 = np.asarray(rho0, dtype=np.complex128)
+        if rho0.shape[-1] == 1:
+            rho0 = rho0 @ rho0.conj().T
+        O = np.asarray(O, dtype=np.complex128)
+        n = H0.shape[-1]
+        T, t_start = float(times[-1]), float(times[0])
+        self.B, self.n, self.nterms, self.nseg, self.nodes = int(B), n, nterms, nseg, nodes
+        self.ops, self.rate_ops = ops, rate_ops
+        zeros = np.zeros(nseg)
+        dev = torch.device('cuda', torch.cuda.current_device())
+        self.dev = dev
+
+        # forward: checkpoints every checkpoint_every segments and J through exp_ops
+        ck_idx = list(range(0, nseg, checkpoint_every)) + [nseg]
+        self.nck = len(ck_idx) - 1
+        fwd_insts = [
+            dense.Instance(H=H0, Ls=Ls, rho0=rho0, pwc=[(times, zeros, ops[j]) for j in range(nterms)],
+                           key_H=(), key_L=(), key_rho=())
+            for _ in range(B)
+        ]
+        t0 = time.perf_counter()
+        self.fwd = dense.build_plan('tsit5', fwd_insts, times[ck_idx], exp_ops=[O], rtol=rtol, atol=atol)
+        t1 = time.perf_counter()
+
+        # Gauss-Legendre nodes per segment
+        x, w = _gl(nodes)
+        seg_len = np.diff(times)
+        node_t = (times[:-1, None] + 0.5 * seg_len[:, None] * (1.0 + x[None, :])).reshape(-1)
+        self.node_w = (0.5 * seg_len[:, None] * w[None, :]).reshape(-1)
+        self.seg_first = (np.arange(nseg + 1) * nodes).astype(np.int32)
+        self.Q = nseg * nodes
+
+        # recompute from the checkpoints: one instance per (control vector, interval);
+        # the initial states are copied from the forward's checkpoints on the device
+        rec_insts = []
+        for _ in range(B):
+            for c in range(self.nck):
+                k0, k1 = ck_idx[c], ck_idx[c + 1]
+                rec_insts.append(
+                    dense.Instance(H=H0, Ls=Ls, rho0=rho0, pwc=[(times, zeros, ops[j]) for j in range(nterms)],
+                                   tsave=node_t[k0 * nodes : k1 * nodes], t0=float(times[k0]), key_H=(), key_L=(),
+                                   key_rho=())
                 )
-            )
-    if len({len(i.tsave) for i in rec_insts}) != 1:
-        raise ValueError('checkpoint_every must divide the number of segments')
-    rec = dense.build_plan('tsit5', rec_insts, rec_insts[0].tsave, rtol=rtol, atol=atol)
-    t4 = time.perf_counter()
-    rec.run()
-    torch.cuda.synchronize()
-    t5 = time.perf_counter()
-    timing['recompute_pack_s'] = t4 - t3
-    timing['recompute_run_s'] = t5 - t4
-    Q = nseg * nodes
-    rho_nodes_dev = rec.states  # (B * nck, Qc * n * n * 2) float32 == (B, Q, n, n) complex64
+        self.rec = dense.build_plan('tsit5', rec_insts, rec_insts[0].tsave, rtol=rtol, atol=atol)
+        t2 = time.perf_counter()
 
-    # -------------------------------------------- backward adjoint (s = T - t)
-    LdL = sum((L.conj().T @ L for L in Ls), np.zeros_like(H0))
-    Hnh_adj = 1j * H0 - 0.5 * LdL
-    rtimes = T - times[::-1]
-    s_nodes = np.sort(T - node_t)
-    bwd_insts = [
-        dense.Instance(
-            H=H0,
-            Ls=[L.conj().T for L in Ls],
-            rho0=O,
-            Hnh=Hnh_adj,
-            pwc=[(rtimes, u[b, j][::-1].copy(), -ops[j]) for j in range(nterms)],
-            key_H=(),
-            key_L=(),
-            key_rho=(),
-        )
-        for b in range(B)
-    ]
-    # saves: s = 0 (lambda = O), the nodes, and s = T - t0 (lambda at the start time)
-    bwd_ts = np.concatenate([[0.0], s_nodes, [T - t_start]])
-    # lambda starts at O: scale the absolute tolerance by |O| so that the backward
-    # solve is held to the same relative accuracy as the forward one (|rho| <= 1)
-    atol_b = atol * max(1.0, float(np.linalg.norm(O, 2)))
-    t6 = time.perf_counter()
-    bwd = dense.build_plan('tsit5', bwd_insts, bwd_ts, rtol=rtol, atol=atol_b)
-    t7 = time.perf_counter()
-    bwd.run()
-    torch.cuda.synchronize()
-    t8 = time.perf_counter()
-    timing['backward_pack_s'] = t7 - t6
-    timing['backward_run_s'] = t8 - t7
-    for name, plan in (('forward', fwd), ('recompute', rec), ('backward', bwd)):
-        _check_status(name, plan)
-    # lambda at the nodes in increasing t, kept on the device
-    bst = bwd.states.view(B, 2 + Q, n * n * 2)
-    lam_dev = torch.flip(bst[:, 1 : 1 + Q], dims=[1]).contiguous()
-    lambda0 = bst[:, 1 + Q].cpu().numpy().view(np.complex64).reshape(B, n, n)
+        # backward adjoint in reversed time s = T - t: Hnh' = iH - 1/2 sum L^dag L,
+        # D = L^dag, C = L / 2, control operators -O_j on the reversed grid
+        LdL = sum((L.conj().T @ L for L in Ls), np.zeros_like(H0))
+        rtimes = T - times[::-1]
+        s_nodes = np.sort(T - node_t)
+        bwd_insts = [
+            dense.Instance(H=H0, Ls=[L.conj().T for L in Ls], rho0=O, Hnh=1j * H0 - 0.5 * LdL,
+                           pwc=[(rtimes, zeros, -ops[j]) for j in range(nterms)], key_H=(), key_L=(), key_rho=())
+            for _ in range(B)
+        ]
+        # saves: s = 0 (lambda = O), the nodes, and s = T - t0 (lambda at the start time)
+        bwd_ts = np.concatenate([[0.0], s_nodes, [T - t_start]])
+        # lambda starts at O: the absolute tolerance scales with |O| so that the
+        # backward solve keeps the forward's relative accuracy (|rho| <= 1)
+        atol_b = atol * max(1.0, float(np.linalg.norm(O, 2)))
+        self.bwd = dense.build_plan('tsit5', bwd_insts, bwd_ts, rtol=rtol, atol=atol_b)
+        t3 = time.perf_counter()
 
-    # ------------------------------------------------------------- quadrature
-    dev = torch.device('cuda', torch.cuda.current_device())
-    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-    w_t = torch.from_numpy(node_w).to(dev)
-    sf_t = torch.from_numpy(seg_first).to(dev)
-    lib = _native.lib()
-    P = ctypes.c_void_p
-    t9 = time.perf_counter()
-    grad = np.zeros((B, nterms, nseg))
-    if nterms:
-        g_t = torch.zeros((B, nterms, nseg), dtype=torch.float64, device=dev)
-        dia_ops = _sparse_controls(ops, n)
-        if dia_ops is not None:
-            # control operators with few diagonals: 2 d n^2 products per node
-            dops, keep = dia_ops
+        # device slots of the PWC coefficients of every instance (complex128 per
+        # segment, the instance's own copy in its plan's staged arena)
+        self._slots = [self._pwc_slots(p) for p in (self.fwd, self.rec, self.bwd)]
+        # quadrature inputs
+        self.w_t = torch.from_numpy(self.node_w).to(dev)
+        self.sf_t = torch.from_numpy(self.seg_first).to(dev)
+        self.dia_ops = _sparse_controls(ops, n) if nterms else None
+        self.o_t = (torch.from_numpy(np.ascontiguousarray(np.stack(ops).astype(np.complex64)).view(np.float32)).to(dev)
+                    if nterms else None)
+        if rate_ops:
+            c64 = lambda xs: torch.from_numpy(np.ascontiguousarray(np.stack(xs).astype(np.complex64)).view(np.float32)).to(dev)  # noqa: E731
+            self.a_t = c64(rate_ops)
+            self.n_t = c64([A.conj().T @ A for A in rate_ops])
+        torch.cuda.synchronize()
+        self.pack_timing = {'forward_pack_s': t1 - t0, 'recompute_pack_s': t2 - t1, 'backward_pack_s': t3 - t2,
+                            'plan_total_s': time.perf_counter() - t_begin}
+
+    def _pwc_slots(self, plan):
+        """(arena as float64, index of every coefficient's real part) of a plan."""
+        import torch
+
+        if not self.nterms:
+            return None
+        arena = plan.arena.tensors[0]
+        base = arena.data_ptr()
+        nv = self.nseg
+        per = self.nterms * nv * 2
+        starts = (plan.descs['pwc_vals'].astype(np.int64) - base) // 8
+        if np.any(plan.descs['nseg'] != nv + 1) or np.any(starts < 0) or np.any(starts + per > arena.numel() // 8):
+            raise RuntimeError('adjoint: unexpected PWC coefficient layout')
+        idx = (starts[:, None] + np.arange(per)[None, :]).reshape(-1)
+        return arena.view(torch.float64), torch.from_numpy(idx).to(self.dev)
+
+    def _write_controls(self, which: int, vals: np.ndarray):
+        """vals (instances, nterms, nseg) real controls -> the plan's complex coefficients."""
+        import torch
+
+        if self._slots[which] is None:
+            return
+        f64, idx = self._slots[which]
+        c = np.zeros(vals.shape + (2,), dtype=np.float64)
+        c[..., 0] = vals
+        f64[idx] = torch.from_numpy(c.reshape(-1)).to(self.dev, non_blocking=False)
+
+    def run(self, u) -> AdjointResult:
+        import time
+
+        import torch
+
+        u = np.asarray(u, dtype=np.float64)
+        if u.ndim == 2:
+            u = u[:, None, :]
+        B, n, nterms, nseg, Q, nck = self.B, self.n, self.nterms, self.nseg, self.Q, self.nck
+        if u.shape != (B, nterms, nseg):
+            raise ValueError(f'u must have shape {(B, nterms, nseg)}')
+        timing = {}
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        self._write_controls(0, u)
+        self._write_controls(1, np.repeat(u, nck, axis=0))
+        self._write_controls(2, u[:, :, ::-1])
+        fwd, rec, bwd = self.fwd, self.rec, self.bwd
+        fwd.reset()
+        bwd.reset()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        fwd.run()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        # checkpoints -> initial states of the recompute instances (FP64, padded)
+        npd = rec.np
+        st = fwd.states.view(B, nck + 1, n, n, 2)[:, :nck].reshape(B * nck, n, n, 2).to(torch.float64)
+        y0 = rec.y[:, 0].view(B * nck, npd, npd, 2)
+        y0.zero_()
+        y0[:, :n, :n, :] = st
+        rec.run()
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        bwd.run()
+        torch.cuda.synchronize()
+        t4 = time.perf_counter()
+        timing.update({'controls_upload_s': t1 - t0, 'forward_run_s': t2 - t1, 'recompute_run_s': t3 - t2,
+                       'backward_run_s': t4 - t3})
+        for name, plan in (('forward', fwd), ('recompute', rec), ('backward', bwd)):
+            _check_status(name, plan)
+        J = fwd.expects.view(B, -1, 2)[:, fwd.ntsave - 1, 0].to(torch.float64).cpu().numpy()
+        rho_nodes_dev = rec.states  # (B * nck, Qc n n 2) float32 == (B, Q, n, n) complex64
+        bst = bwd.states.view(B, 2 + Q, n * n * 2)
+        lam_dev = torch.flip(bst[:, 1 : 1 + Q], dims=[1]).contiguous()
+        lambda0 = bst[:, 1 + Q].cpu().numpy().view(np.complex64).reshape(B, n, n)
+
+        stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        lib = _native.lib()
+        P = ctypes.c_void_p
+        t5 = time.perf_counter()
+        grad = np.zeros((B, nterms, nseg))
+        if nterms:
+            g_t = torch.zeros((B, nterms, nseg), dtype=torch.float64, device=self.dev)
+            if self.dia_ops is not None:
+                dops, _ = self.dia_ops
+                _native.check(
+                    lib.dqh_adjoint_pwc_quadrature_dia(
+                        B, n, Q, nseg, P(rho_nodes_dev.data_ptr()), P(lam_dev.data_ptr()), ctypes.byref(dops),
+                        P(self.w_t.data_ptr()), P(self.sf_t.data_ptr()), P(g_t.data_ptr()), stream,
+                    ),
+                    'dqh_adjoint_pwc_quadrature_dia',
+                )
+            else:
+                _native.check(
+                    lib.dqh_adjoint_pwc_quadrature(
+                        B, n, Q, nterms, nseg, P(rho_nodes_dev.data_ptr()), P(lam_dev.data_ptr()),
+                        P(self.o_t.data_ptr()), P(self.w_t.data_ptr()), P(self.sf_t.data_ptr()), P(g_t.data_ptr()),
+                        stream,
+                    ),
+                    'dqh_adjoint_pwc_quadrature',
+                )
+            grad = g_t.cpu().numpy()
+        grad_rates = None
+        if self.rate_ops:
+            gr_t = torch.zeros((B, len(self.rate_ops), nseg), dtype=torch.float64, device=self.dev)
             _native.check(
-                lib.dqh_adjoint_pwc_quadrature_dia(
-                    B, n, Q, nseg, P(rho_nodes_dev.data_ptr()), P(lam_dev.data_ptr()), ctypes.byref(dops),
-                    P(w_t.data_ptr()), P(sf_t.data_ptr()), P(g_t.data_ptr()), stream,
+                lib.dqh_adjoint_rate_quadrature(
+                    B, n, Q, len(self.rate_ops), nseg, P(rho_nodes_dev.data_ptr()), P(lam_dev.data_ptr()),
+                    P(self.a_t.data_ptr()), P(self.n_t.data_ptr()), P(self.w_t.data_ptr()), P(self.sf_t.data_ptr()),
+                    P(gr_t.data_ptr()), stream,
                 ),
-                'dqh_adjoint_pwc_quadrature_dia',
+                'dqh_adjoint_rate_quadrature',
             )
-            torch.cuda.synchronize()
-            del keep
-        else:
-            o_t = torch.from_numpy(np.ascontiguousarray(np.stack(ops).astype(np.complex64)).view(np.float32)).to(dev)
-            _native.check(
-                lib.dqh_adjoint_pwc_quadrature(
-                    B, n, Q, nterms, nseg, P(rho_nodes_dev.data_ptr()), P(lam_dev.data_ptr()), P(o_t.data_ptr()),
-                    P(w_t.data_ptr()), P(sf_t.data_ptr()), P(g_t.data_ptr()), stream,
-                ),
-                'dqh_adjoint_pwc_quadrature',
-            )
-        grad = g_t.cpu().numpy()
-    grad_rates = None
-    if rate_ops:
-        c64 = lambda xs: torch.from_numpy(np.ascontiguousarray(np.stack(xs).astype(np.complex64)).view(np.float32)).to(dev)  # noqa: E731
-        a_t = c64(rate_ops)
-        n_t = c64([A.conj().T @ A for A in rate_ops])
-        gr_t = torch.zeros((B, len(rate_ops), nseg), dtype=torch.float64, device=dev)
-        _native.check(
-            lib.dqh_adjoint_rate_quadrature(
-                B, n, Q, len(rate_ops), nseg, P(rho_nodes_dev.data_ptr()), P(lam_dev.data_ptr()), P(a_t.data_ptr()),
-                P(n_t.data_ptr()), P(w_t.data_ptr()), P(sf_t.data_ptr()), P(gr_t.data_ptr()), stream,
-            ),
-            'dqh_adjoint_rate_quadrature',
-        )
-        grad_rates = gr_t.cpu().numpy().sum(axis=-1)
-    torch.cuda.synchronize()
-    timing['quadrature_s'] = time.perf_counter() - t9
-    timing['forward_nrhs'] = float(np.sum(fres['nrhs']))
-    return AdjointResult(J=J, grad_u=grad, grad_rates=grad_rates, lambda0=lambda0, timing=timing)
+            grad_rates = gr_t.cpu().numpy().sum(axis=-1)
+        torch.cuda.synchronize()
+        timing['quadrature_s'] = time.perf_counter() - t5
+        timing['forward_nrhs'] = float(fwd.stats[:, 4].sum().item())
+        return AdjointResult(J=J, grad_u=grad, grad_rates=grad_rates, lambda0=lambda0, timing=timing)
