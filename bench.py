@@ -394,6 +394,7 @@ def bench_adjoint(args):
         tg.append(time.perf_counter() - t0)
         tim = res.timing
     res = plan.run(u)
+    # sum of the kernel times (the backward overlaps the forward on its own stream)
     gpu_grad = tim['forward_run_s'] + tim['recompute_run_s'] + tim['backward_run_s'] + tim['quadrature_s']
     out = {
         'metric': 'adjoint gradient cost / forward solve',
