@@ -213,6 +213,8 @@ def mesolve(
 
     n = plan.n
     states = res['states'].reshape(*out_shape, -1, n, n)
+    if not options.save_states:  # only the final state, without a time axis (save_mixin.py:62-66)
+        states = states[..., -1, :, :]
     ysave = QArray(states, dims=rho0.dims)
     Esave = None
     if Es is not None:
@@ -269,10 +271,10 @@ def _mesolve_dia(route, rho0: QArray, tsave, exp_ops, method, gradient, options)
             raise max_steps_error()
         if np.any(res['status'] == -4):
             raise RuntimeError('The solver stopped on a non-finite error estimate or a vanishing step size.')
-    states = res['states']
-    if not options.save_states:
-        states = states[:, -1:]
-    ysave = QArray(states.reshape(*out_shape, -1, n, n), dims=rho0.dims)
+    states = res['states'].reshape(*out_shape, -1, n, n)
+    if not options.save_states:  # only the final state, without a time axis (save_mixin.py:62-66)
+        states = states[..., -1, :, :]
+    ysave = QArray(states, dims=rho0.dims)
     Esave = None
     if Es is not None:
         Esave = res['expects'].reshape(*out_shape, len(Es), len(tsave))

@@ -1,0 +1,78 @@
+"""CPU checks of the drop-in ``mesolve`` error contract: everything the reference
+raises before integrating (``dynamiqs/integrators/apis/mesolve.py:280-291, 350-414``,
+``dynamiqs/_checks.py``, ``dynamiqs/integrators/_utils.py:84-90``) raises here the
+same exception type with the same message, without touching the GPU."""
+
+import re
+
+import numpy as np
+import pytest
+
+import dynamiqs_amd as dq
+
+
+def _args(n=3):
+    a = dq.destroy(n)
+    return a.dag() @ a, [a], dq.fock(n, 0), np.linspace(0.0, 1.0, 3)
+
+
+def test_unsupported_method_type_error():
+    H, Ls, psi, ts = _args()
+    with pytest.raises(TypeError, match=re.escape('Method of type `Dopri5` is not supported (supported method types:')):
+        dq.mesolve(H, Ls, psi, ts, method=dq.method.Dopri5())
+
+
+def test_shape_errors():
+    H, Ls, psi, ts = _args()
+    with pytest.raises(ValueError, match=re.escape('Argument `H` must have shape (...H, n, n), but has shape H.shape=(3, 2)')):
+        dq.mesolve(np.ones((3, 2)), Ls, psi, ts)
+    with pytest.raises(ValueError, match=re.escape('Argument `jump_ops[0]` must have shape (...L0, n, n)')):
+        dq.mesolve(H, [np.ones((3, 2))], psi, ts)
+    with pytest.raises(ValueError, match=re.escape('Argument `rho0` must have shape (...rho0, n, 1) or (...rho0, n, n)')):
+        dq.mesolve(H, Ls, np.ones((3, 2)), ts)
+    with pytest.raises(ValueError, match=re.escape('Argument `exp_ops[0]` must have shape (n, n)')):
+        dq.mesolve(H, Ls, psi, ts, exp_ops=[np.ones((2, 3, 3))])
+
+
+def test_tsave_errors():
+    H, Ls, psi, _ = _args()
+    with pytest.raises(ValueError, match='must be a 1D array, but is a 2D array'):
+        dq.mesolve(H, Ls, psi, np.zeros((2, 2)))
+    with pytest.raises(ValueError, match='must contain at least one element'):
+        dq.mesolve(H, Ls, psi, np.zeros(0))
+    with pytest.raises(ValueError, match='must be sorted in strictly ascending order'):
+        dq.mesolve(H, Ls, psi, np.array([0.0, 1.0, 0.5]))
+
+
+def test_rho0_layout_error():
+    H, Ls, _, ts = _args()
+    rho_dia = dq.asqarray(np.eye(3) / 3, layout=dq.dia)
+    with pytest.raises(ValueError, match=re.escape('Argument rho0 must have layout `dense` but has layout rho0.layout=')):
+        dq.mesolve(H, Ls, rho_dia, ts)
+
+
+def test_option_whitelist():
+    H, Ls, psi, ts = _args()
+    # every mesolve option is accepted by the check; an option of another solver is not
+    opts = dq.Options(save_states=False, cartesian_batching=False, t0=0.0, vectorized=True, assume_hermitian=False)
+    from dynamiqs_amd.options import check_options
+
+    check_options(opts, 'mesolve')
+    with pytest.raises(ValueError, match=re.escape('Option `vectorized` was set to `True` but is not used by the solver `dq.sesolve()`')):
+        check_options(dq.Options(vectorized=True), 'sesolve')
+
+
+def test_empty_jump_ops_with_ket_warns():
+    H, _, psi, ts = _args()
+    with pytest.warns(UserWarning, match='consider using `dq.sesolve\\(\\)`'):
+        try:
+            dq.mesolve(H, [], psi, ts)
+        except RuntimeError:
+            pass  # no GPU here: the warning is raised before the solve
+
+
+def test_modulated_operators_are_reported_unsupported():
+    H, Ls, psi, ts = _args()
+    Hm = dq.modulated(lambda t: np.cos(t), H)
+    with pytest.raises(NotImplementedError, match='ModulatedTimeQArray operators are not supported'):
+        dq.mesolve(Hm, Ls, psi, ts)

@@ -1,0 +1,88 @@
+"""GPU checks of the drop-in ``mesolve``: the output shapes of cartesian and flat
+batching (``tests/mesolve/test_batching.py:20-87`` of the reference, same shape
+cases), every batch element equal to its own single solve, and the result
+object (infos shapes, ``final_state``, ``save_states=False``)."""
+
+import numpy as np
+import pytest
+
+import dynamiqs_amd as dq
+
+pytestmark = pytest.mark.gpu
+
+
+def _rand_op(rng, n, batch, hermitian=True):
+    x = rng.normal(size=(*batch, n, n)) + 1j * rng.normal(size=(*batch, n, n))
+    if hermitian:
+        x = x + np.swapaxes(x.conj(), -1, -2)
+    return x
+
+
+def _rand_ket(rng, n, batch):
+    x = rng.normal(size=(*batch, n, 1)) + 1j * rng.normal(size=(*batch, n, 1))
+    return x / np.linalg.norm(x, axis=(-2, -1), keepdims=True)
+
+
+def _args(n, nH, nLs, npsi0, nEs, seed=42):
+    rng = np.random.default_rng(seed)
+    H = _rand_op(rng, n, nH)
+    Ls = [_rand_op(rng, n, nL, hermitian=False) for nL in nLs]
+    psi0 = _rand_ket(rng, n, npsi0)
+    Es = _rand_op(rng, n, (nEs,), hermitian=False)
+    return H, Ls, psi0, list(Es)
+
+
+@pytest.mark.parametrize('nH,npsi0,nL1,nL2', [((), (), (), ()), ((3,), (5,), (), (9,)), ((3, 4), (), (7, 8), ()),
+                                              ((3,), (5,), (7, 8), (9,))])
+def test_cartesian_batching_shapes(nH, npsi0, nL1, nL2):
+    n, nEs, ntsave = 2, 10, 11
+    H, Ls, psi0, Es = _args(n, nH, [nL1, nL2], npsi0, nEs)
+    tsave = np.linspace(0, 0.01, ntsave)
+    res = dq.mesolve(H, Ls, psi0, tsave, exp_ops=Es)
+    assert res.states.shape == (*nH, *nL1, *nL2, *npsi0, ntsave, n, n)
+    assert res.expects.shape == (*nH, *nL1, *nL2, *npsi0, nEs, ntsave)
+    assert np.shape(res.infos.nsteps) == (*nH, *nL1, *nL2, *npsi0)
+
+
+@pytest.mark.parametrize('nL1', [(), (5, 1, 4)])
+@pytest.mark.parametrize('npsi0', [(), (1,), (4,), (3, 1), (3, 4), (5, 1, 4)])
+def test_flat_batching_shapes(nL1, npsi0):
+    n, nH, nEs, ntsave = 2, (3, 4), 6, 11
+    H, Ls, psi0, Es = _args(n, nH, [nL1, ()], npsi0, nEs)
+    tsave = np.linspace(0, 0.01, ntsave)
+    res = dq.mesolve(H, Ls, psi0, tsave, exp_ops=Es, options=dq.Options(cartesian_batching=False))
+    shape = np.broadcast_shapes(nH, nL1, npsi0)
+    assert res.states.shape == (*shape, ntsave, n, n)
+    assert res.expects.shape == (*shape, nEs, ntsave)
+
+
+def test_batch_elements_equal_single_solves():
+    """Cartesian batch (H: 2, L: 3, rho0: 2) against 12 single mesolve calls."""
+    n = 4
+    H, Ls, psi0, _ = _args(n, (2,), [(3,)], (2,), 1, seed=3)
+    tsave = np.linspace(0, 0.5, 4)
+    res = dq.mesolve(H, Ls, psi0, tsave)
+    got = res.states.to_numpy()
+    for i in range(2):
+        for j in range(3):
+            for k in range(2):
+                one = dq.mesolve(H[i], [Ls[0][j]], psi0[k], tsave)
+                np.testing.assert_allclose(got[i, j, k], one.states.to_numpy(), atol=1e-6)
+
+
+def test_result_fields_and_save_states_false():
+    n = 6
+    a = dq.destroy(n)
+    H = a.dag() @ a
+    tsave = np.linspace(0, 1, 5)
+    res = dq.mesolve(H, [0.3 * a], dq.coherent(n, 0.8), tsave, exp_ops=[H],
+                     options=dq.Options(save_states=False))
+    # the reference keeps only the final state, without a time axis
+    assert res.states.shape == (n, n)
+    full = dq.mesolve(H, [0.3 * a], dq.coherent(n, 0.8), tsave, exp_ops=[H])
+    assert full.states.shape == (5, n, n)
+    np.testing.assert_allclose(np.asarray(full.final_state.to_numpy()), full.states.to_numpy()[-1], atol=0)
+    np.testing.assert_allclose(res.states.to_numpy(), full.states.to_numpy()[-1], atol=1e-6)
+    np.testing.assert_allclose(res.expects, full.expects, atol=1e-6)
+    s = str(full)
+    assert '|' in s or 'MESolveResult' in s
