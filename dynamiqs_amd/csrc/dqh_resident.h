@@ -1,11 +1,14 @@
 // Register/LDS-resident Tsit5 for padded dimensions np <= 64 (gfx950).
 //
-// One 512-thread workgroup integrates one instance. The seven FP32 Tsit5 stages
-// live in registers (thread t owns the row-major elements t, t + 512, ...); the
-// stage input rho_s, the products X_k = D_k rho_s and the pre-symmetrised tmp live
-// in LDS (padded row stride np + 2 keeps the MFMA fragment reads conflict-free).
-// The FP64 state y stays in its global buffer (L2-resident, each thread reads and
-// writes only its own elements, 16 B per element per stage); the operators are
+// One 512-thread workgroup integrates one instance. The state, the stage inputs
+// and every stage k_j are Hermitian (the vector field is tmp + tmp^dag, which is
+// Hermitian in floating point too), so a thread owns upper-triangle elements only:
+// element u = t + 512 q of the row-major upper triangle (diagonal included). The
+// seven FP32 stages of the owned elements live in registers; the stage input
+// rho_s (both triangles), the products X_k = D_k rho_s and the pre-symmetrised tmp
+// live in LDS (padded row stride np + 2 keeps the MFMA fragment reads
+// conflict-free). The FP64 state y is kept packed (upper triangle, in owner order)
+// in two global buffers that alternate on accepted steps; the operators are
 // L2-resident and shared by the batch.
 #pragma once
 
@@ -22,7 +25,34 @@ struct RCfg {
   static constexpr int NJC = NT / TJ;
   static constexpr int ITEMS2 = NT * NJC;
   static constexpr int MAXI2 = (ITEMS2 + kWaves - 1) / kWaves;
+  static constexpr int NU = NP * (NP + 1) / 2;  // upper triangle with the diagonal
+  static constexpr int NEU = (NU + kThreads - 1) / kThreads;
 };
+
+// Owned upper-triangle elements: ij[q] = i | j << 8 (i <= j) for u = t + 512 q < NU,
+// -1 past the end.
+template <int NP>
+__device__ __forceinline__ void upper_map(int (&ij)[RCfg<NP>::NEU]) {
+  using R = RCfg<NP>;
+#pragma unroll
+  for (int q = 0; q < R::NEU; ++q) {
+    const int u = threadIdx.x + kThreads * q;
+    if (u >= R::NU) {
+      ij[q] = -1;
+    } else {
+      int i = 0, rem = u;
+      while (rem >= NP - i) {
+        rem -= NP - i;
+        ++i;
+      }
+      ij[q] = i | ((i + rem) << 8);
+    }
+  }
+}
+__device__ __forceinline__ int ij_row(int x) { return x & 255; }
+__device__ __forceinline__ int ij_col(int x) { return x >> 8; }
+// weight of an owned element in a sum over the full matrix
+__device__ __forceinline__ double ij_weight(int x) { return ij_row(x) == ij_col(x) ? 1.0 : 2.0; }
 
 // (1 + kResMaxL) matrices of 64 x 66 complex64: ys, then X_1..X_3 (X_1 reused for tmp)
 constexpr int kResLdsFloat2 = (1 + kResMaxL) * 64 * 66;
@@ -60,12 +90,12 @@ __device__ __forceinline__ void strip_store(const Acc (&acc)[TJ], float2 *C, int
   for (int t = 0; t < TJ; ++t) tile_store(acc[t], C, ldc, row0, col0 + 16 * t, lane);
 }
 
-// out[q] <- (tmp + tmp^dag) at the owned elements, tmp = [Hnh ys] + sum_k (D_k ys) C_k.
-// Precondition: ys written to lds[0 .. MAT) by every thread (this routine starts
-// with the barrier that publishes it).
+// tmp = [Hnh ys] + sum_k (D_k ys) C_k into lds[MAT .. 2 MAT), published by a final
+// barrier. Precondition: ys written to lds[0 .. MAT) by every thread (this routine
+// starts with the barrier that publishes it).
 template <int NP>
-__device__ __forceinline__ void res_rhs(const float2 *__restrict__ Hnh, const float2 *__restrict__ D, const float2 *__restrict__ C,
-                        int nL, bool hasH, float2 *lds, float2 (&out)[RCfg<NP>::NE]) {
+__device__ __forceinline__ void res_rhs_tmp(const float2 *__restrict__ Hnh, const float2 *__restrict__ D,
+                                            const float2 *__restrict__ C, int nL, bool hasH, float2 *lds) {
   using R = RCfg<NP>;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const float2 *ys = lds;
@@ -108,12 +138,40 @@ __device__ __forceinline__ void res_rhs(const float2 *__restrict__ Hnh, const fl
     }
   }
   __syncthreads();
-  // phase 3: Hermitian symmetrisation into the owned elements
+}
+
+// out[q] <- (tmp + tmp^dag) at the row-major elements e = t + 512 q
+template <int NP>
+__device__ __forceinline__ void res_rhs(const float2 *__restrict__ Hnh, const float2 *__restrict__ D, const float2 *__restrict__ C,
+                        int nL, bool hasH, float2 *lds, float2 (&out)[RCfg<NP>::NE]) {
+  using R = RCfg<NP>;
+  res_rhs_tmp<NP>(Hnh, D, C, nL, hasH, lds);
+  const float2 *X = lds + R::MAT;
 #pragma unroll
   for (int q = 0; q < R::NE; ++q) {
     const int e = threadIdx.x + kThreads * q;
     if (e < R::NN) {
       const int i = e / NP, j = e % NP;
+      const float2 a = X[i * R::LDP + j], b = X[j * R::LDP + i];
+      out[q] = make_float2(a.x + b.x, a.y - b.y);
+    } else {
+      out[q] = make_float2(0.f, 0.f);
+    }
+  }
+}
+
+// out[q] <- (tmp + tmp^dag) at the owned upper-triangle elements
+template <int NP>
+__device__ __forceinline__ void res_rhs_upper(const float2 *__restrict__ Hnh, const float2 *__restrict__ D,
+                                              const float2 *__restrict__ C, int nL, bool hasH, float2 *lds,
+                                              const int (&ij)[RCfg<NP>::NEU], float2 (&out)[RCfg<NP>::NEU]) {
+  using R = RCfg<NP>;
+  res_rhs_tmp<NP>(Hnh, D, C, nL, hasH, lds);
+  const float2 *X = lds + R::MAT;
+#pragma unroll
+  for (int q = 0; q < R::NEU; ++q) {
+    if (ij[q] >= 0) {
+      const int i = ij_row(ij[q]), j = ij_col(ij[q]);
       const float2 a = X[i * R::LDP + j], b = X[j * R::LDP + i];
       out[q] = make_float2(a.x + b.x, a.y - b.y);
     } else {
@@ -129,6 +187,21 @@ __device__ __forceinline__ void res_write_ys(float2 *lds, const float2 (&v)[RCfg
   for (int q = 0; q < R::NE; ++q) {
     const int e = threadIdx.x + kThreads * q;
     if (e < R::NN) lds[(e / NP) * R::LDP + (e % NP)] = v[q];
+  }
+}
+
+// ys <- v on both triangles (v given on the owned upper-triangle elements)
+template <int NP>
+__device__ __forceinline__ void res_write_ys_upper(float2 *lds, const int (&ij)[RCfg<NP>::NEU],
+                                                   const float2 (&v)[RCfg<NP>::NEU]) {
+  using R = RCfg<NP>;
+#pragma unroll
+  for (int q = 0; q < R::NEU; ++q) {
+    if (ij[q] >= 0) {
+      const int i = ij_row(ij[q]), j = ij_col(ij[q]);
+      lds[i * R::LDP + j] = v[q];
+      if (i != j) lds[j * R::LDP + i] = make_float2(v[q].x, -v[q].y);
+    }
   }
 }
 
@@ -162,20 +235,23 @@ __device__ __forceinline__ void res_update_pwc(const dqh_me_inst &p, int seg) {
   __threadfence_block();
 }
 
-// save y + h sum_j w_j k_j at save index isave (states and expectation values)
+// save y + h sum_j w_j k_j at save index isave (states on both triangles, and the
+// expectation values Tr(E rho) = sum_ij E^T_ij rho_ij summed over the owned
+// elements and their mirror images rho_ji = conj(rho_ij))
 template <int NP>
-__device__ __forceinline__ void res_save(const dqh_me_inst &p, const double2 *__restrict__ y, const float2 (&k)[7][RCfg<NP>::NE],
-                         double h, const double (&w)[7], bool interp, int isave, double *sh) {
+__device__ __forceinline__ void res_save(const dqh_me_inst &p, const double2 *__restrict__ y,
+                                         const int (&ij)[RCfg<NP>::NEU], const float2 (&k)[7][RCfg<NP>::NEU],
+                                         double h, const double (&w)[7], bool interp, int isave, double *sh) {
   using R = RCfg<NP>;
   const int n = (int)p.n, nE = (int)p.nE, ntsave = (int)p.ntsave;
   const bool save_states = (p.flags & DQH_FLAG_SAVE_STATES) != 0;
-  float2 *states = gp<float2>(p.states);
+  float2 *states = gp<float2>(p.states) + (size_t)isave * n * n;
   const float2 *ET = gp<const float2>(p.EopsT);
-  double vr[R::NE], vi[R::NE];
+  double vr[R::NEU], vi[R::NEU];
 #pragma unroll
-  for (int q = 0; q < R::NE; ++q) {
-    const int e0 = threadIdx.x + kThreads * q;
-    const double2 yq = e0 < R::NN ? y[e0] : make_double2(0.0, 0.0);
+  for (int q = 0; q < R::NEU; ++q) {
+    const int u = threadIdx.x + kThreads * q;
+    const double2 yq = ij[q] >= 0 ? y[u] : make_double2(0.0, 0.0);
     double re = yq.x, im = yq.y;
     if (interp) {
 #pragma unroll
@@ -186,26 +262,41 @@ __device__ __forceinline__ void res_save(const dqh_me_inst &p, const double2 *__
     }
     vr[q] = re;
     vi[q] = im;
-    const int e = threadIdx.x + kThreads * q;
-    const int i = e / NP, jj = e % NP;
-    if (save_states && e < R::NN && i < n && jj < n)
-      states[((size_t)isave * n + i) * n + jj] = make_float2((float)re, (float)im);
+    const int i = ij_row(ij[q]), jj = ij_col(ij[q]);
+    if (save_states && ij[q] >= 0 && jj < n) {
+      states[(size_t)i * n + jj] = make_float2((float)re, (float)im);
+      if (i != jj) states[(size_t)jj * n + i] = make_float2((float)re, (float)-im);
+    }
   }
   float2 *ex = gp<float2>(p.expects);
   for (int qe = 0; qe < nE; ++qe) {
+    const float2 *E = ET + (size_t)qe * R::NN;
     double ar = 0, ai = 0;
 #pragma unroll
-    for (int q = 0; q < R::NE; ++q) {
-      const int e = threadIdx.x + kThreads * q;
-      if (e < R::NN) {
-        const float2 o = ET[(size_t)qe * R::NN + e];
+    for (int q = 0; q < R::NEU; ++q) {
+      if (ij[q] >= 0) {
+        const int i = ij_row(ij[q]), jj = ij_col(ij[q]);
+        const float2 o = E[i * NP + jj];
         ar += o.x * vr[q] - o.y * vi[q];
         ai += o.x * vi[q] + o.y * vr[q];
+        if (i != jj) {
+          const float2 m = E[jj * NP + i];
+          ar += m.x * vr[q] + m.y * vi[q];
+          ai += m.y * vr[q] - m.x * vi[q];
+        }
       }
     }
     const double sr = block_sum<kWaves>(ar, sh), si = block_sum<kWaves>(ai, sh);
     if (threadIdx.x == 0) ex[(size_t)qe * ntsave + isave] = make_float2((float)sr, (float)si);
   }
+}
+
+// x is equal in every lane of the wave: return it as a wave-uniform value
+__device__ __forceinline__ double wave_uniform(double x) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
 template <int NP>
@@ -238,7 +329,7 @@ __device__ __forceinline__ void rhs_resident(const dqh_me_inst &p, float2 *lds) 
 template <int NP>
 __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh, float2 *lds) {
   using R = RCfg<NP>;
-  constexpr int NE = R::NE;
+  constexpr int NE = R::NEU;
   const int n = (int)p.n, nL = (int)p.nL, ntsave = (int)p.ntsave, nseg = (int)p.nseg;
   const bool pwc = p.npwc > 0;
   const bool hasH = (p.flags & DQH_FLAG_HAS_HNH) != 0;
@@ -252,8 +343,24 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
   const double inv_nn = 1.0 / ((double)n * n);
   const double t_end = p.t1;
 
-  // y: FP64 state in global memory, element e of thread t at yg[e] (e = t + 512 q)
-  double2 *yg = gp<double2>(p.y);
+  int ij[NE];
+  upper_map<NP>(ij);
+  // y: packed FP64 upper triangle, element u = t + 512 q of thread t at yg[u]. rho0
+  // (row-major, np x np) is in the first np^2 elements of the y buffer; the packed
+  // state starts in the second half, and each step's candidate y1 goes to the
+  // other buffer, which becomes y on acceptance.
+  double2 *const ybase = gp<double2>(p.y);
+  double2 *yg = ybase + R::NN;
+  double2 *yn = ybase;
+  auto yload = [&](int q) -> double2 {
+    const int u = threadIdx.x + kThreads * q;
+    return ij[q] >= 0 ? yg[u] : make_double2(0.0, 0.0);
+  };
+#pragma unroll
+  for (int q = 0; q < NE; ++q) {
+    if (ij[q] >= 0) yg[threadIdx.x + kThreads * q] = ybase[ij_row(ij[q]) * NP + ij_col(ij[q])];
+  }
+
   float2 k[7][NE];
   float2 tmp[NE];
 #pragma unroll
@@ -261,10 +368,6 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
 #pragma unroll
     for (int j = 0; j < 7; ++j) k[j][q] = make_float2(0.f, 0.f);
   }
-  auto yload = [&](int q) -> double2 {
-    const int e = threadIdx.x + kThreads * q;
-    return e < R::NN ? yg[e] : make_double2(0.0, 0.0);
-  };
   double w0[7] = {0, 0, 0, 0, 0, 0, 0};
 
   double t = p.t0;
@@ -274,7 +377,7 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
   int seg = -2;
 
   while (isave < ntsave && tsave[isave] <= t) {
-    res_save<NP>(p, yg, k, 0.0, w0, false, isave, sh);
+    res_save<NP>(p, yg, ij, k, 0.0, w0, false, isave, sh);
     ++isave;
   }
   if (pwc) {
@@ -287,11 +390,12 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
     const double2 yq = yload(q);
     tmp[q] = make_float2((float)yq.x, (float)yq.y);
   }
-  res_write_ys<NP>(lds, tmp);
-  res_rhs<NP>(Hnh, D, C, nL, hasH, lds, k[0]);
+  res_write_ys_upper<NP>(lds, ij, tmp);
+  res_rhs_upper<NP>(Hnh, D, C, nL, hasH, lds, ij, k[0]);
   ++nrhs;
 
-  // initial step (Hairer-Norsett-Wanner II.4), scale atol + rtol |y0|
+  // initial step (Hairer-Norsett-Wanner II.4), scale atol + rtol |y0|; sums over
+  // the full matrix count each off-diagonal owned element twice
   double dt;
   {
     double s0 = 0, s1 = 0;
@@ -301,8 +405,9 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
       const double2 yq = yload(q);
       const double ay2 = yq.x * yq.x + yq.y * yq.y;
       const double sc = atol + rtol * sqrt(ay2);
-      s0 += ay2 / (sc * sc);
-      s1 += ((double)k[0][q].x * k[0][q].x + (double)k[0][q].y * k[0][q].y) / (sc * sc);
+      const double wq = ij_weight(ij[q]);
+      s0 += wq * ay2 / (sc * sc);
+      s1 += wq * ((double)k[0][q].x * k[0][q].x + (double)k[0][q].y * k[0][q].y) / (sc * sc);
     }
     const double d0 = sqrt(block_sum<kWaves>(s0, sh) * inv_nn);
     const double d1 = sqrt(block_sum<kWaves>(s1, sh) * inv_nn);
@@ -312,8 +417,8 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
       const double2 yq = yload(q);
       tmp[q] = make_float2((float)(yq.x + h0 * k[0][q].x), (float)(yq.y + h0 * k[0][q].y));
     }
-    res_write_ys<NP>(lds, tmp);
-    res_rhs<NP>(Hnh, D, C, nL, hasH, lds, k[1]);
+    res_write_ys_upper<NP>(lds, ij, tmp);
+    res_rhs_upper<NP>(Hnh, D, C, nL, hasH, lds, ij, k[1]);
     ++nrhs;
     double s2 = 0;
 #pragma unroll
@@ -321,12 +426,12 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
       const double2 yq = yload(q);
       const double sc = atol + rtol * sqrt(yq.x * yq.x + yq.y * yq.y);
       const double dr = (double)k[1][q].x - k[0][q].x, di = (double)k[1][q].y - k[0][q].y;
-      s2 += (dr * dr + di * di) / (sc * sc);
+      s2 += ij_weight(ij[q]) * (dr * dr + di * di) / (sc * sc);
     }
     const double d2 = sqrt(block_sum<kWaves>(s2, sh) * inv_nn) / h0;
     const double md = fmax(d1, d2);
     const double h1 = md <= 1e-15 ? fmax(1e-6, h0 * 1e-3) : ctrl_pow(0.01 / md, 0.2);
-    dt = fmin(100.0 * h0, h1);
+    dt = wave_uniform(fmin(100.0 * h0, h1));
   }
 
   bool need_k0 = false;
@@ -348,8 +453,8 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
         const double2 yq = yload(q);
         tmp[q] = make_float2((float)yq.x, (float)yq.y);
       }
-      res_write_ys<NP>(lds, tmp);
-      res_rhs<NP>(Hnh, D, C, nL, hasH, lds, k[0]);
+      res_write_ys_upper<NP>(lds, ij, tmp);
+      res_rhs_upper<NP>(Hnh, D, C, nL, hasH, lds, ij, k[0]);
       ++nrhs;
       need_k0 = false;
     }
@@ -375,8 +480,10 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
       status = DQH_NONFINITE;
       break;
     }
-    // stages 1..6; the stage loop is a runtime loop (one copy of the GEMM code), the
-    // register-resident stages are selected with compile-time indices
+    // stages 1..6. The stage input y + h sum_j a_sj k_j is accumulated in FP64 (the
+    // coefficients reach |a| ~ 13 and the terms cancel) and rounded once to FP32.
+    // The stage loop is a runtime loop (one copy of the GEMM code); the
+    // register-resident stages are selected with compile-time indices.
 #pragma unroll 1
     for (int s = 1; s < 7; ++s) {
       double c[6];
@@ -393,8 +500,8 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
         }
         tmp[q] = make_float2((float)re, (float)im);
       }
-      res_write_ys<NP>(lds, tmp);
-      res_rhs<NP>(Hnh, D, C, nL, hasH, lds, tmp);
+      res_write_ys_upper<NP>(lds, ij, tmp);
+      res_rhs_upper<NP>(Hnh, D, C, nL, hasH, lds, ij, tmp);
       ++nrhs;
 #pragma unroll
       for (int j = 1; j < 7; ++j) {
@@ -404,7 +511,8 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
         }
       }
     }
-    // error norm in FP64 with the candidate y1 = y + h sum b_j k_j
+    // error norm in FP64 with the candidate y1 = y + h sum b_j k_j, which is stored
+    // to the other state buffer
     double ssum = 0;
 #pragma unroll
     for (int q = 0; q < NE; ++q) {
@@ -420,9 +528,10 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
       er *= h;
       ei *= h;
       const double sc = atol + rtol * fmax(sqrt(yq.x * yq.x + yq.y * yq.y), sqrt(yr * yr + yi * yi));
-      ssum += (er * er + ei * ei) / (sc * sc);
+      ssum += ij_weight(ij[q]) * (er * er + ei * ei) / (sc * sc);
+      if (ij[q] >= 0) yn[threadIdx.x + kThreads * q] = make_double2(yr, yi);
     }
-    const double norm = sqrt(block_sum<kWaves>(ssum, sh) * inv_nn);
+    const double norm = wave_uniform(sqrt(block_sum<kWaves>(ssum, sh) * inv_nn));
     const bool finite = isfinite(norm);
     const bool keep = finite && norm < 1.0;
     double factor;
@@ -442,31 +551,23 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
         double w[7];
 #pragma unroll
         for (int j = 0; j < 7; ++j) w[j] = th * (kR[j][0] + th * (kR[j][1] + th * (kR[j][2] + th * kR[j][3])));
-        res_save<NP>(p, yg, k, h, w, true, isave, sh);
+        res_save<NP>(p, yg, ij, k, h, w, true, isave, sh);
         ++isave;
       }
 #pragma unroll
-      for (int q = 0; q < NE; ++q) {
-        const int e = threadIdx.x + kThreads * q;
-        const double2 yq = yload(q);
-        double yr = yq.x, yi = yq.y;
-#pragma unroll
-        for (int j = 0; j < 6; ++j) {
-          yr += h * kA[6][j] * k[j][q].x;
-          yi += h * kA[6][j] * k[j][q].y;
-        }
-        if (e < R::NN) yg[e] = make_double2(yr, yi);
-        k[0][q] = k[6][q];
-      }
+      for (int q = 0; q < NE; ++q) k[0][q] = k[6][q];
+      double2 *const yo = yg;
+      yg = yn;
+      yn = yo;
       t = t_new;
       need_k0 = made_jump;
     } else {
       ++nrej;
     }
-    dt = factor * h;
+    dt = wave_uniform(factor * h);
   }
   while (isave < ntsave) {
-    res_save<NP>(p, yg, k, 0.0, w0, false, isave, sh);
+    res_save<NP>(p, yg, ij, k, 0.0, w0, false, isave, sh);
     ++isave;
   }
   float2 *fin = gp<float2>(p.final_state);
@@ -474,13 +575,17 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
   const bool save_states = (p.flags & DQH_FLAG_SAVE_STATES) != 0;
 #pragma unroll
   for (int q = 0; q < NE; ++q) {
-    const int e = threadIdx.x + kThreads * q;
-    const int i = e / NP, j = e % NP;
-    if (e < R::NN && i < n && j < n) {
-      const double2 yq = yg[e];
+    const int i = ij_row(ij[q]), j = ij_col(ij[q]);
+    if (ij[q] >= 0 && j < n) {
+      const double2 yq = yg[threadIdx.x + kThreads * q];
       const float2 v = make_float2((float)yq.x, (float)yq.y);
+      const float2 vc = make_float2(v.x, -v.y);
       fin[(size_t)i * n + j] = v;
-      if (!save_states) st[(size_t)i * n + j] = v;
+      if (i != j) fin[(size_t)j * n + i] = vc;
+      if (!save_states) {
+        st[(size_t)i * n + j] = v;
+        if (i != j) st[(size_t)j * n + i] = vc;
+      }
     }
   }
   if (threadIdx.x == 0) {
