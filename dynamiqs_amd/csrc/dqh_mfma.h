@@ -43,6 +43,14 @@ __device__ __forceinline__ void cmfma(Acc &a, float ar, float ai, float2 b) {
   a.im = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, b.x, a.im, 0, 0, 0);
 }
 
+// One complex k-slice with a conjugated B element: D += A * conj(m).
+__device__ __forceinline__ void cmfma_cb(Acc &a, float ar, float ai, float mr, float mi) {
+  a.re = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, mr, a.re, 0, 0, 0);
+  a.re = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, mi, a.re, 0, 0, 0);
+  a.im = __builtin_amdgcn_mfma_f32_16x16x4f32(-ar, mi, a.im, 0, 0, 0);
+  a.im = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, mr, a.im, 0, 0, 0);
+}
+
 // acc += A[row0:row0+16, 0:K] @ B[0:K, col0:col0+16] for row-major complex64
 // matrices (K multiple of 16). Lane (r = lane%16, g = lane/16) loads
 // A[row0+r][k0+4g .. k0+4g+3] (32 contiguous bytes) and B[k0+4g+i][col0+r]; MFMA

@@ -84,6 +84,43 @@ __device__ __forceinline__ void strip_mac(Acc (&acc)[TJ], const float2 *__restri
   }
 }
 
+// acc[t] += A[row0:row0+16, 0:K] @ M[col0+16t : col0+16t+16, 0:K]^dag, t < TJ. Both
+// operands are read along rows: lane (r, g) loads A[row0+r][k0+4g .. k0+4g+3] and
+// M[col0+16t+r][k0+4g .. k0+4g+3] (32 contiguous bytes each), so every fragment is
+// two 16-byte loads. For a Hermitian M this is A @ M.
+template <int TJ, int K>
+__device__ __forceinline__ void strip_mac_h(Acc (&acc)[TJ], const float2 *__restrict__ A, int lda, const float2 *M,
+                                            int ldm, int row0, int col0, int lane) {
+  const int r = lane & 15, g = lane >> 4;
+  const float2 *Ap = A + (size_t)(row0 + r) * lda + 4 * g;
+  const float2 *Mp = M + (size_t)(col0 + r) * ldm + 4 * g;
+  // the whole A strip (global memory) is requested before the first product, so
+  // one memory latency is exposed per strip rather than one per k block
+  float4 af[K / 16][2];
+#pragma unroll
+  for (int b = 0; b < K / 16; ++b) {
+    af[b][0] = *reinterpret_cast<const float4 *>(Ap + 16 * b);
+    af[b][1] = *reinterpret_cast<const float4 *>(Ap + 16 * b + 2);
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int b = 0; b < K / 16; ++b) {
+    const int k0 = 16 * b;
+    const float4 a01 = af[b][0];
+    const float4 a23 = af[b][1];
+#pragma unroll
+    for (int t = 0; t < TJ; ++t) {
+      const float2 *mp = Mp + (size_t)(16 * t) * ldm + k0;
+      const float4 m01 = *reinterpret_cast<const float4 *>(mp);
+      const float4 m23 = *reinterpret_cast<const float4 *>(mp + 2);
+      cmfma_cb(acc[t], a01.x, a01.y, m01.x, m01.y);
+      cmfma_cb(acc[t], a01.z, a01.w, m01.z, m01.w);
+      cmfma_cb(acc[t], a23.x, a23.y, m23.x, m23.y);
+      cmfma_cb(acc[t], a23.z, a23.w, m23.z, m23.w);
+    }
+  }
+}
+
 template <int TJ>
 __device__ __forceinline__ void strip_store(const Acc (&acc)[TJ], float2 *C, int ldc, int row0, int col0, int lane) {
 #pragma unroll
@@ -140,6 +177,66 @@ __device__ __forceinline__ void res_rhs_tmp(const float2 *__restrict__ Hnh, cons
   __syncthreads();
 }
 
+// Hermitian form of res_rhs_tmp for a Hermitian ys and C_k = D_k^dag / 2 (the
+// Tsit5 descriptors): X_k C_k = D_k ys D_k^dag / 2 = D_k (X_k / 2)^dag, so
+//   phase 1: X'_k = (D_k ys) / 2 = D_k ys^dag / 2,
+//   phase 2: tmp = Hnh ys^dag + sum_k D_k X'_k^dag,
+// with every global operand (Hnh, D_k) an A fragment and every LDS operand read
+// along rows. tmp goes to the first free LDS matrix after X'_1..X'_nL (slot
+// nL + 1), so only with nL = 3 jump operators does it wait for the phase-2 reads
+// of X'_1. Returns the tmp matrix; published by a final barrier.
+template <int NP>
+__device__ __forceinline__ const float2 *res_rhs_tmp_h(const float2 *__restrict__ Hnh, const float2 *__restrict__ D,
+                                                       int nL, bool hasH, float2 *lds) {
+  using R = RCfg<NP>;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const float2 *ys = lds;
+  float2 *X = lds + R::MAT;
+  __syncthreads();
+  const int items1 = nL * R::NT * R::NJC;
+  for (int it = wave; it < items1; it += kWaves) {
+    const int kk = it / (R::NT * R::NJC), rem = it % (R::NT * R::NJC);
+    const int ti = rem / R::NJC, jc = rem % R::NJC;
+    Acc acc[R::TJ];
+#pragma unroll
+    for (int t = 0; t < R::TJ; ++t) acc_zero(acc[t]);
+    strip_mac_h<R::TJ, NP>(acc, D + (size_t)kk * R::NN, NP, ys, R::LDP, ti * 16, jc * R::TJ * 16, lane);
+#pragma unroll
+    for (int t = 0; t < R::TJ; ++t) {
+      acc[t].re *= 0.5f;
+      acc[t].im *= 0.5f;
+    }
+    strip_store<R::TJ>(acc, X + kk * R::MAT, R::LDP, ti * 16, jc * R::TJ * 16, lane);
+  }
+  __syncthreads();
+  Acc acc2[R::MAXI2][R::TJ];
+#pragma unroll
+  for (int m = 0; m < R::MAXI2; ++m) {
+    const int it = wave + kWaves * m;
+#pragma unroll
+    for (int t = 0; t < R::TJ; ++t) acc_zero(acc2[m][t]);
+    if (it < R::ITEMS2) {
+      const int ti = it / R::NJC, jc = it % R::NJC;
+      if (hasH) strip_mac_h<R::TJ, NP>(acc2[m], Hnh, NP, ys, R::LDP, ti * 16, jc * R::TJ * 16, lane);
+      for (int kk = 0; kk < nL; ++kk)
+        strip_mac_h<R::TJ, NP>(acc2[m], D + (size_t)kk * R::NN, NP, X + kk * R::MAT, R::LDP, ti * 16,
+                               jc * R::TJ * 16, lane);
+    }
+  }
+  float2 *T = X + (nL < kResMaxL ? nL : 0) * R::MAT;
+  if (nL >= kResMaxL) __syncthreads();
+#pragma unroll
+  for (int m = 0; m < R::MAXI2; ++m) {
+    const int it = wave + kWaves * m;
+    if (it < R::ITEMS2) {
+      const int ti = it / R::NJC, jc = it % R::NJC;
+      strip_store<R::TJ>(acc2[m], T, R::LDP, ti * 16, jc * R::TJ * 16, lane);
+    }
+  }
+  __syncthreads();
+  return T;
+}
+
 // out[q] <- (tmp + tmp^dag) at the row-major elements e = t + 512 q
 template <int NP>
 __device__ __forceinline__ void res_rhs(const float2 *__restrict__ Hnh, const float2 *__restrict__ D, const float2 *__restrict__ C,
@@ -160,14 +257,14 @@ __device__ __forceinline__ void res_rhs(const float2 *__restrict__ Hnh, const fl
   }
 }
 
-// out[q] <- (tmp + tmp^dag) at the owned upper-triangle elements
+// out[q] <- (tmp + tmp^dag) at the owned upper-triangle elements (Hermitian ys,
+// C_k = D_k^dag / 2)
 template <int NP>
 __device__ __forceinline__ void res_rhs_upper(const float2 *__restrict__ Hnh, const float2 *__restrict__ D,
-                                              const float2 *__restrict__ C, int nL, bool hasH, float2 *lds,
+                                              int nL, bool hasH, float2 *lds,
                                               const int (&ij)[RCfg<NP>::NEU], float2 (&out)[RCfg<NP>::NEU]) {
   using R = RCfg<NP>;
-  res_rhs_tmp<NP>(Hnh, D, C, nL, hasH, lds);
-  const float2 *X = lds + R::MAT;
+  const float2 *X = res_rhs_tmp_h<NP>(Hnh, D, nL, hasH, lds);
 #pragma unroll
   for (int q = 0; q < R::NEU; ++q) {
     if (ij[q] >= 0) {
@@ -336,7 +433,6 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
   const double *tsave = gp<const double>(p.tsave);
   const double *ptimes = gp<const double>(p.pwc_times);
   const float2 *D = gp<const float2>(p.Dops);
-  const float2 *C = gp<const float2>(p.Cops);
   const float2 *Hnh = res_hnh(p);
   int64_t *stats = gp<int64_t>(p.stats);
   const double rtol = p.rtol, atol = p.atol;
@@ -391,7 +487,7 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
     tmp[q] = make_float2((float)yq.x, (float)yq.y);
   }
   res_write_ys_upper<NP>(lds, ij, tmp);
-  res_rhs_upper<NP>(Hnh, D, C, nL, hasH, lds, ij, k[0]);
+  res_rhs_upper<NP>(Hnh, D, nL, hasH, lds, ij, k[0]);
   ++nrhs;
 
   // initial step (Hairer-Norsett-Wanner II.4), scale atol + rtol |y0|; sums over
@@ -418,7 +514,7 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
       tmp[q] = make_float2((float)(yq.x + h0 * k[0][q].x), (float)(yq.y + h0 * k[0][q].y));
     }
     res_write_ys_upper<NP>(lds, ij, tmp);
-    res_rhs_upper<NP>(Hnh, D, C, nL, hasH, lds, ij, k[1]);
+    res_rhs_upper<NP>(Hnh, D, nL, hasH, lds, ij, k[1]);
     ++nrhs;
     double s2 = 0;
 #pragma unroll
@@ -454,7 +550,7 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
         tmp[q] = make_float2((float)yq.x, (float)yq.y);
       }
       res_write_ys_upper<NP>(lds, ij, tmp);
-      res_rhs_upper<NP>(Hnh, D, C, nL, hasH, lds, ij, k[0]);
+      res_rhs_upper<NP>(Hnh, D, nL, hasH, lds, ij, k[0]);
       ++nrhs;
       need_k0 = false;
     }
@@ -501,7 +597,7 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
         tmp[q] = make_float2((float)re, (float)im);
       }
       res_write_ys_upper<NP>(lds, ij, tmp);
-      res_rhs_upper<NP>(Hnh, D, C, nL, hasH, lds, ij, tmp);
+      res_rhs_upper<NP>(Hnh, D, nL, hasH, lds, ij, tmp);
       ++nrhs;
 #pragma unroll
       for (int j = 1; j < 7; ++j) {

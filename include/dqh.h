@@ -54,7 +54,10 @@ typedef struct dqh_me_inst {
   uint64_t tsave;     /* const double[ntsave] */
   uint64_t Hnh;       /* const c64[np*np]   -iH0 - 1/2 sum L^dag L                 */
   uint64_t Dops;      /* const c64[nsets*nL*np*np]  left factors  (L_k)            */
-  uint64_t Cops;      /* const c64[nsets*nL*np*np]  right factors (1/2 L_k^dag)    */
+  uint64_t Cops;      /* const c64[nsets*nL*np*np]  right factors (1/2 L_k^dag).
+                         Tsit5 descriptors must hold Cops = Dops^dag / 2 exactly:
+                         the resident body uses X_k C_k = D_k (D_k rho / 2)^dag
+                         for the Hermitian stage inputs and reads Dops only.   */
   uint64_t EopsT;     /* const c64[nE*np*np] transposed exp ops                    */
   uint64_t Hpwc;      /* const c64[npwc*np*np]  -i O_j of each PWC term            */
   uint64_t pwc_times; /* const double[nseg]                                        */
