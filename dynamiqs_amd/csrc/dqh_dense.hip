@@ -524,10 +524,10 @@ __global__ __launch_bounds__(kThreads) void k_tsit5(const dqh_me_inst *__restric
   const dqh_me_inst &p = insts[blockIdx.x];
   const int np = (int)p.np;
   if (p.nL <= kResMaxL && !(p.flags & DQH_FLAG_GLOBAL)) {
-    if (np == 16) return tsit5_resident<16>(p, sh, lds);
-    if (np == 32) return tsit5_resident<32>(p, sh, lds);
-    if (np == 48) return tsit5_resident<48>(p, sh, lds);
-    if (np == 64) return tsit5_resident<64>(p, sh, lds);
+    if (np == 16) return tsit5_resident<16, kWaves>(p, sh, lds);
+    if (np == 32) return tsit5_resident<32, kWaves>(p, sh, lds);
+    if (np == 48) return tsit5_resident<48, kWaves>(p, sh, lds);
+    if (np == 64) return tsit5_resident<64, kWaves>(p, sh, lds);
   }
   tsit5_global<kWaves>(p, sh);
 }
@@ -536,16 +536,24 @@ __global__ __launch_bounds__(kThreads) void k_tsit5(const dqh_me_inst *__restric
 // only that body, so its registers are allocated for it alone (the dispatcher's
 // allocation covers every body it may call). An instance that does not fit is
 // left untouched with status DQH_UNSUPPORTED.
+// Waves per workgroup of the single-size kernel. 16 waves (128 VGPRs each) spill
+// the np = 64 stages around every product, so every size runs 8.
 template <int NP>
-__global__ __launch_bounds__(kThreads) void k_tsit5_res(const dqh_me_inst *__restrict__ insts) {
-  __shared__ double sh[kWaves];
+constexpr int res_waves() {
+  return kWaves;
+}
+
+template <int NP>
+__global__ __launch_bounds__(res_waves<NP>() * 64) void k_tsit5_res(const dqh_me_inst *__restrict__ insts) {
+  constexpr int NW = res_waves<NP>();
+  __shared__ double sh[NW];
   __shared__ float2 lds[kResLdsFloat2];
   const dqh_me_inst &p = insts[blockIdx.x];
   if (p.np != NP || p.nL > kResMaxL || (p.flags & DQH_FLAG_GLOBAL)) {
     if (threadIdx.x == 0) gp<int64_t>(p.stats)[3] = DQH_UNSUPPORTED;
     return;
   }
-  tsit5_resident<NP>(p, sh, lds);
+  tsit5_resident<NP, NW>(p, sh, lds);
 }
 
 __global__ __launch_bounds__(kThreads) void k_rhs(const dqh_me_inst *__restrict__ insts) {
@@ -553,10 +561,10 @@ __global__ __launch_bounds__(kThreads) void k_rhs(const dqh_me_inst *__restrict_
   const dqh_me_inst &p = insts[blockIdx.x];
   const int np = (int)p.np;
   if (p.nL <= kResMaxL && !(p.flags & DQH_FLAG_GLOBAL)) {
-    if (np == 16) return rhs_resident<16>(p, lds);
-    if (np == 32) return rhs_resident<32>(p, lds);
-    if (np == 48) return rhs_resident<48>(p, lds);
-    if (np == 64) return rhs_resident<64>(p, lds);
+    if (np == 16) return rhs_resident<16, kWaves>(p, lds);
+    if (np == 32) return rhs_resident<32, kWaves>(p, lds);
+    if (np == 48) return rhs_resident<48, kWaves>(p, lds);
+    if (np == 64) return rhs_resident<64, kWaves>(p, lds);
   }
   rhs_global<kWaves>(p);
 }
@@ -600,12 +608,12 @@ extern "C" int dqh_mesolve_tsit5_dense_uniform(const dqh_me_inst *d_insts, int64
   if (!d_insts || ninst <= 0 || ninst > (1 << 30))
     return fail(DQH_BAD_ARG, "dqh_mesolve_tsit5_dense_uniform: bad arguments");
   hipStream_t st = (hipStream_t)stream;
-  const dim3 grid((unsigned)ninst), block(kThreads);
+  const dim3 grid((unsigned)ninst);
   switch (np) {
-    case 16: hipLaunchKernelGGL(k_tsit5_res<16>, grid, block, 0, st, d_insts); break;
-    case 32: hipLaunchKernelGGL(k_tsit5_res<32>, grid, block, 0, st, d_insts); break;
-    case 48: hipLaunchKernelGGL(k_tsit5_res<48>, grid, block, 0, st, d_insts); break;
-    case 64: hipLaunchKernelGGL(k_tsit5_res<64>, grid, block, 0, st, d_insts); break;
+    case 16: hipLaunchKernelGGL(k_tsit5_res<16>, grid, dim3(res_waves<16>() * 64), 0, st, d_insts); break;
+    case 32: hipLaunchKernelGGL(k_tsit5_res<32>, grid, dim3(res_waves<32>() * 64), 0, st, d_insts); break;
+    case 48: hipLaunchKernelGGL(k_tsit5_res<48>, grid, dim3(res_waves<48>() * 64), 0, st, d_insts); break;
+    case 64: hipLaunchKernelGGL(k_tsit5_res<64>, grid, dim3(res_waves<64>() * 64), 0, st, d_insts); break;
     default: return fail(DQH_BAD_SHAPE, "dqh_mesolve_tsit5_dense_uniform: np must be 16, 32, 48 or 64");
   }
   return launch_checked("dqh_mesolve_tsit5_dense_uniform", hipGetLastError());

@@ -14,29 +14,32 @@
 
 namespace dqh {
 
-template <int NP>
+template <int NP, int NW>
 struct RCfg {
   static constexpr int NT = NP / 16;
   static constexpr int NN = NP * NP;
   static constexpr int LDP = NP + 2;
   static constexpr int MAT = NP * LDP;  // float2 per LDS matrix
-  static constexpr int NE = (NN + kThreads - 1) / kThreads;
-  static constexpr int TJ = (NT % 2 == 0) ? 2 : 1;  // output tiles sharing one A fragment
+  static constexpr int NTH = NW * 64;  // threads per workgroup
+  static constexpr int NE = (NN + NTH - 1) / NTH;
+  // output tiles sharing one A fragment (one with 16 waves, so that phase 2 still
+  // has a tile strip per wave)
+  static constexpr int TJ = (NT % 2 == 0 && NW <= 8) ? 2 : 1;
   static constexpr int NJC = NT / TJ;
   static constexpr int ITEMS2 = NT * NJC;
-  static constexpr int MAXI2 = (ITEMS2 + kWaves - 1) / kWaves;
+  static constexpr int MAXI2 = (ITEMS2 + NW - 1) / NW;
   static constexpr int NU = NP * (NP + 1) / 2;  // upper triangle with the diagonal
-  static constexpr int NEU = (NU + kThreads - 1) / kThreads;
+  static constexpr int NEU = (NU + NTH - 1) / NTH;
 };
 
 // Owned upper-triangle elements: ij[q] = i | j << 8 (i <= j) for u = t + 512 q < NU,
 // -1 past the end.
-template <int NP>
-__device__ __forceinline__ void upper_map(int (&ij)[RCfg<NP>::NEU]) {
-  using R = RCfg<NP>;
+template <int NP, int NW>
+__device__ __forceinline__ void upper_map(int (&ij)[(RCfg<NP, NW>::NEU)]) {
+  using R = RCfg<NP, NW>;
 #pragma unroll
   for (int q = 0; q < R::NEU; ++q) {
-    const int u = threadIdx.x + kThreads * q;
+    const int u = threadIdx.x + R::NTH * q;
     if (u >= R::NU) {
       ij[q] = -1;
     } else {
@@ -130,17 +133,17 @@ __device__ __forceinline__ void strip_store(const Acc (&acc)[TJ], float2 *C, int
 // tmp = [Hnh ys] + sum_k (D_k ys) C_k into lds[MAT .. 2 MAT), published by a final
 // barrier. Precondition: ys written to lds[0 .. MAT) by every thread (this routine
 // starts with the barrier that publishes it).
-template <int NP>
+template <int NP, int NW>
 __device__ __forceinline__ void res_rhs_tmp(const float2 *__restrict__ Hnh, const float2 *__restrict__ D,
                                             const float2 *__restrict__ C, int nL, bool hasH, float2 *lds) {
-  using R = RCfg<NP>;
+  using R = RCfg<NP, NW>;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const float2 *ys = lds;
   float2 *X = lds + R::MAT;
   __syncthreads();
   // phase 1: X_k = D_k ys (A from L2, B from LDS)
   const int items1 = nL * R::NT * R::NJC;
-  for (int it = wave; it < items1; it += kWaves) {
+  for (int it = wave; it < items1; it += NW) {
     const int kk = it / (R::NT * R::NJC), rem = it % (R::NT * R::NJC);
     const int ti = rem / R::NJC, jc = rem % R::NJC;
     Acc acc[R::TJ];
@@ -154,7 +157,7 @@ __device__ __forceinline__ void res_rhs_tmp(const float2 *__restrict__ Hnh, cons
   Acc acc2[R::MAXI2][R::TJ];
 #pragma unroll
   for (int m = 0; m < R::MAXI2; ++m) {
-    const int it = wave + kWaves * m;
+    const int it = wave + NW * m;
 #pragma unroll
     for (int t = 0; t < R::TJ; ++t) acc_zero(acc2[m][t]);
     if (it < R::ITEMS2) {
@@ -168,7 +171,7 @@ __device__ __forceinline__ void res_rhs_tmp(const float2 *__restrict__ Hnh, cons
   __syncthreads();
 #pragma unroll
   for (int m = 0; m < R::MAXI2; ++m) {
-    const int it = wave + kWaves * m;
+    const int it = wave + NW * m;
     if (it < R::ITEMS2) {
       const int ti = it / R::NJC, jc = it % R::NJC;
       strip_store<R::TJ>(acc2[m], X, R::LDP, ti * 16, jc * R::TJ * 16, lane);
@@ -185,16 +188,16 @@ __device__ __forceinline__ void res_rhs_tmp(const float2 *__restrict__ Hnh, cons
 // along rows. tmp goes to the first free LDS matrix after X'_1..X'_nL (slot
 // nL + 1), so only with nL = 3 jump operators does it wait for the phase-2 reads
 // of X'_1. Returns the tmp matrix; published by a final barrier.
-template <int NP>
+template <int NP, int NW>
 __device__ __forceinline__ const float2 *res_rhs_tmp_h(const float2 *__restrict__ Hnh, const float2 *__restrict__ D,
                                                        int nL, bool hasH, float2 *lds) {
-  using R = RCfg<NP>;
+  using R = RCfg<NP, NW>;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const float2 *ys = lds;
   float2 *X = lds + R::MAT;
   __syncthreads();
   const int items1 = nL * R::NT * R::NJC;
-  for (int it = wave; it < items1; it += kWaves) {
+  for (int it = wave; it < items1; it += NW) {
     const int kk = it / (R::NT * R::NJC), rem = it % (R::NT * R::NJC);
     const int ti = rem / R::NJC, jc = rem % R::NJC;
     Acc acc[R::TJ];
@@ -212,7 +215,7 @@ __device__ __forceinline__ const float2 *res_rhs_tmp_h(const float2 *__restrict_
   Acc acc2[R::MAXI2][R::TJ];
 #pragma unroll
   for (int m = 0; m < R::MAXI2; ++m) {
-    const int it = wave + kWaves * m;
+    const int it = wave + NW * m;
 #pragma unroll
     for (int t = 0; t < R::TJ; ++t) acc_zero(acc2[m][t]);
     if (it < R::ITEMS2) {
@@ -227,7 +230,7 @@ __device__ __forceinline__ const float2 *res_rhs_tmp_h(const float2 *__restrict_
   if (nL >= kResMaxL) __syncthreads();
 #pragma unroll
   for (int m = 0; m < R::MAXI2; ++m) {
-    const int it = wave + kWaves * m;
+    const int it = wave + NW * m;
     if (it < R::ITEMS2) {
       const int ti = it / R::NJC, jc = it % R::NJC;
       strip_store<R::TJ>(acc2[m], T, R::LDP, ti * 16, jc * R::TJ * 16, lane);
@@ -238,15 +241,15 @@ __device__ __forceinline__ const float2 *res_rhs_tmp_h(const float2 *__restrict_
 }
 
 // out[q] <- (tmp + tmp^dag) at the row-major elements e = t + 512 q
-template <int NP>
+template <int NP, int NW>
 __device__ __forceinline__ void res_rhs(const float2 *__restrict__ Hnh, const float2 *__restrict__ D, const float2 *__restrict__ C,
-                        int nL, bool hasH, float2 *lds, float2 (&out)[RCfg<NP>::NE]) {
-  using R = RCfg<NP>;
-  res_rhs_tmp<NP>(Hnh, D, C, nL, hasH, lds);
+                        int nL, bool hasH, float2 *lds, float2 (&out)[(RCfg<NP, NW>::NE)]) {
+  using R = RCfg<NP, NW>;
+  res_rhs_tmp<NP, NW>(Hnh, D, C, nL, hasH, lds);
   const float2 *X = lds + R::MAT;
 #pragma unroll
   for (int q = 0; q < R::NE; ++q) {
-    const int e = threadIdx.x + kThreads * q;
+    const int e = threadIdx.x + R::NTH * q;
     if (e < R::NN) {
       const int i = e / NP, j = e % NP;
       const float2 a = X[i * R::LDP + j], b = X[j * R::LDP + i];
@@ -259,12 +262,12 @@ __device__ __forceinline__ void res_rhs(const float2 *__restrict__ Hnh, const fl
 
 // out[q] <- (tmp + tmp^dag) at the owned upper-triangle elements (Hermitian ys,
 // C_k = D_k^dag / 2)
-template <int NP>
+template <int NP, int NW>
 __device__ __forceinline__ void res_rhs_upper(const float2 *__restrict__ Hnh, const float2 *__restrict__ D,
                                               int nL, bool hasH, float2 *lds,
-                                              const int (&ij)[RCfg<NP>::NEU], float2 (&out)[RCfg<NP>::NEU]) {
-  using R = RCfg<NP>;
-  const float2 *X = res_rhs_tmp_h<NP>(Hnh, D, nL, hasH, lds);
+                                              const int (&ij)[(RCfg<NP, NW>::NEU)], float2 (&out)[(RCfg<NP, NW>::NEU)]) {
+  using R = RCfg<NP, NW>;
+  const float2 *X = res_rhs_tmp_h<NP, NW>(Hnh, D, nL, hasH, lds);
 #pragma unroll
   for (int q = 0; q < R::NEU; ++q) {
     if (ij[q] >= 0) {
@@ -277,21 +280,21 @@ __device__ __forceinline__ void res_rhs_upper(const float2 *__restrict__ Hnh, co
   }
 }
 
-template <int NP>
-__device__ __forceinline__ void res_write_ys(float2 *lds, const float2 (&v)[RCfg<NP>::NE]) {
-  using R = RCfg<NP>;
+template <int NP, int NW>
+__device__ __forceinline__ void res_write_ys(float2 *lds, const float2 (&v)[(RCfg<NP, NW>::NE)]) {
+  using R = RCfg<NP, NW>;
 #pragma unroll
   for (int q = 0; q < R::NE; ++q) {
-    const int e = threadIdx.x + kThreads * q;
+    const int e = threadIdx.x + R::NTH * q;
     if (e < R::NN) lds[(e / NP) * R::LDP + (e % NP)] = v[q];
   }
 }
 
 // ys <- v on both triangles (v given on the owned upper-triangle elements)
-template <int NP>
-__device__ __forceinline__ void res_write_ys_upper(float2 *lds, const int (&ij)[RCfg<NP>::NEU],
-                                                   const float2 (&v)[RCfg<NP>::NEU]) {
-  using R = RCfg<NP>;
+template <int NP, int NW>
+__device__ __forceinline__ void res_write_ys_upper(float2 *lds, const int (&ij)[(RCfg<NP, NW>::NEU)],
+                                                   const float2 (&v)[(RCfg<NP, NW>::NEU)]) {
+  using R = RCfg<NP, NW>;
 #pragma unroll
   for (int q = 0; q < R::NEU; ++q) {
     if (ij[q] >= 0) {
@@ -308,16 +311,16 @@ __device__ __forceinline__ const float2 *res_hnh(const dqh_me_inst &p) {
 
 // Hcur (first np*np of work) <- Hnh0 + sum_j c_j Hpwc_j for segment seg; global memory,
 // published to the workgroup by the barrier at the start of the next res_rhs.
-template <int NP>
+template <int NP, int NW>
 __device__ __forceinline__ void res_update_pwc(const dqh_me_inst &p, int seg) {
-  using R = RCfg<NP>;
+  using R = RCfg<NP, NW>;
   const float2 *H0 = gp<const float2>(p.Hnh);
   const float2 *Hp = gp<const float2>(p.Hpwc);
   const double *vals = gp<const double>(p.pwc_vals);
   float2 *Hc = gp<float2>(p.work);
   const int nv = (int)p.nseg - 1, npwc = (int)p.npwc;
   __syncthreads();  // no wave still reads the previous Hcur
-  for (int e = threadIdx.x; e < R::NN; e += kThreads) {
+  for (int e = threadIdx.x; e < R::NN; e += R::NTH) {
     double re = H0[e].x, im = H0[e].y;
     if (seg >= 0 && seg < nv) {
       for (int j = 0; j < npwc; ++j) {
@@ -335,11 +338,11 @@ __device__ __forceinline__ void res_update_pwc(const dqh_me_inst &p, int seg) {
 // save y + h sum_j w_j k_j at save index isave (states on both triangles, and the
 // expectation values Tr(E rho) = sum_ij E^T_ij rho_ij summed over the owned
 // elements and their mirror images rho_ji = conj(rho_ij))
-template <int NP>
+template <int NP, int NW>
 __device__ __forceinline__ void res_save(const dqh_me_inst &p, const double2 *__restrict__ y,
-                                         const int (&ij)[RCfg<NP>::NEU], const float2 (&k)[7][RCfg<NP>::NEU],
+                                         const int (&ij)[(RCfg<NP, NW>::NEU)], const float2 (&k)[7][(RCfg<NP, NW>::NEU)],
                                          double h, const double (&w)[7], bool interp, int isave, double *sh) {
-  using R = RCfg<NP>;
+  using R = RCfg<NP, NW>;
   const int n = (int)p.n, nE = (int)p.nE, ntsave = (int)p.ntsave;
   const bool save_states = (p.flags & DQH_FLAG_SAVE_STATES) != 0;
   float2 *states = gp<float2>(p.states) + (size_t)isave * n * n;
@@ -347,7 +350,7 @@ __device__ __forceinline__ void res_save(const dqh_me_inst &p, const double2 *__
   double vr[R::NEU], vi[R::NEU];
 #pragma unroll
   for (int q = 0; q < R::NEU; ++q) {
-    const int u = threadIdx.x + kThreads * q;
+    const int u = threadIdx.x + R::NTH * q;
     const double2 yq = ij[q] >= 0 ? y[u] : make_double2(0.0, 0.0);
     double re = yq.x, im = yq.y;
     if (interp) {
@@ -383,7 +386,7 @@ __device__ __forceinline__ void res_save(const dqh_me_inst &p, const double2 *__
         }
       }
     }
-    const double sr = block_sum<kWaves>(ar, sh), si = block_sum<kWaves>(ai, sh);
+    const double sr = block_sum<NW>(ar, sh), si = block_sum<NW>(ai, sh);
     if (threadIdx.x == 0) ex[(size_t)qe * ntsave + isave] = make_float2((float)sr, (float)si);
   }
 }
@@ -396,36 +399,36 @@ __device__ __forceinline__ double wave_uniform(double x) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-template <int NP>
+template <int NP, int NW>
 __device__ __forceinline__ void rhs_resident(const dqh_me_inst &p, float2 *lds) {
-  using R = RCfg<NP>;
+  using R = RCfg<NP, NW>;
   if (p.npwc > 0) {
-    res_update_pwc<NP>(p, pwc_segment(gp<const double>(p.pwc_times), (int)p.nseg, p.t0));
+    res_update_pwc<NP, NW>(p, pwc_segment(gp<const double>(p.pwc_times), (int)p.nseg, p.t0));
   }
   const double2 *yg = gp<const double2>(p.y);
   float2 v[R::NE];
 #pragma unroll
   for (int q = 0; q < R::NE; ++q) {
-    const int e = threadIdx.x + kThreads * q;
+    const int e = threadIdx.x + R::NTH * q;
     v[q] = e < R::NN ? make_float2((float)yg[e].x, (float)yg[e].y) : make_float2(0.f, 0.f);
   }
-  res_write_ys<NP>(lds, v);
+  res_write_ys<NP, NW>(lds, v);
   float2 out[R::NE];
-  res_rhs<NP>(res_hnh(p), gp<const float2>(p.Dops), gp<const float2>(p.Cops),
+  res_rhs<NP, NW>(res_hnh(p), gp<const float2>(p.Dops), gp<const float2>(p.Cops),
               (int)p.nL, (p.flags & DQH_FLAG_HAS_HNH) != 0, lds, out);
   float2 *fin = gp<float2>(p.final_state);
   const int n = (int)p.n;
 #pragma unroll
   for (int q = 0; q < R::NE; ++q) {
-    const int e = threadIdx.x + kThreads * q;
+    const int e = threadIdx.x + R::NTH * q;
     const int i = e / NP, j = e % NP;
     if (e < R::NN && i < n && j < n) fin[(size_t)i * n + j] = out[q];
   }
 }
 
-template <int NP>
+template <int NP, int NW>
 __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh, float2 *lds) {
-  using R = RCfg<NP>;
+  using R = RCfg<NP, NW>;
   constexpr int NE = R::NEU;
   const int n = (int)p.n, nL = (int)p.nL, ntsave = (int)p.ntsave, nseg = (int)p.nseg;
   const bool pwc = p.npwc > 0;
@@ -440,7 +443,7 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
   const double t_end = p.t1;
 
   int ij[NE];
-  upper_map<NP>(ij);
+  upper_map<NP, NW>(ij);
   // y: packed FP64 upper triangle, element u = t + 512 q of thread t at yg[u]. rho0
   // (row-major, np x np) is in the first np^2 elements of the y buffer; the packed
   // state starts in the second half, and each step's candidate y1 goes to the
@@ -449,12 +452,12 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
   double2 *yg = ybase + R::NN;
   double2 *yn = ybase;
   auto yload = [&](int q) -> double2 {
-    const int u = threadIdx.x + kThreads * q;
+    const int u = threadIdx.x + R::NTH * q;
     return ij[q] >= 0 ? yg[u] : make_double2(0.0, 0.0);
   };
 #pragma unroll
   for (int q = 0; q < NE; ++q) {
-    if (ij[q] >= 0) yg[threadIdx.x + kThreads * q] = ybase[ij_row(ij[q]) * NP + ij_col(ij[q])];
+    if (ij[q] >= 0) yg[threadIdx.x + R::NTH * q] = ybase[ij_row(ij[q]) * NP + ij_col(ij[q])];
   }
 
   float2 k[7][NE];
@@ -473,12 +476,12 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
   int seg = -2;
 
   while (isave < ntsave && tsave[isave] <= t) {
-    res_save<NP>(p, yg, ij, k, 0.0, w0, false, isave, sh);
+    res_save<NP, NW>(p, yg, ij, k, 0.0, w0, false, isave, sh);
     ++isave;
   }
   if (pwc) {
     seg = pwc_segment(ptimes, nseg, t);
-    res_update_pwc<NP>(p, seg);
+    res_update_pwc<NP, NW>(p, seg);
   }
   // k0 = f(y0)
 #pragma unroll
@@ -486,8 +489,8 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
     const double2 yq = yload(q);
     tmp[q] = make_float2((float)yq.x, (float)yq.y);
   }
-  res_write_ys_upper<NP>(lds, ij, tmp);
-  res_rhs_upper<NP>(Hnh, D, nL, hasH, lds, ij, k[0]);
+  res_write_ys_upper<NP, NW>(lds, ij, tmp);
+  res_rhs_upper<NP, NW>(Hnh, D, nL, hasH, lds, ij, k[0]);
   ++nrhs;
 
   // initial step (Hairer-Norsett-Wanner II.4), scale atol + rtol |y0|; sums over
@@ -505,16 +508,16 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
       s0 += wq * ay2 / (sc * sc);
       s1 += wq * ((double)k[0][q].x * k[0][q].x + (double)k[0][q].y * k[0][q].y) / (sc * sc);
     }
-    const double d0 = sqrt(block_sum<kWaves>(s0, sh) * inv_nn);
-    const double d1 = sqrt(block_sum<kWaves>(s1, sh) * inv_nn);
+    const double d0 = sqrt(block_sum<NW>(s0, sh) * inv_nn);
+    const double d1 = sqrt(block_sum<NW>(s1, sh) * inv_nn);
     h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
 #pragma unroll
     for (int q = 0; q < NE; ++q) {
       const double2 yq = yload(q);
       tmp[q] = make_float2((float)(yq.x + h0 * k[0][q].x), (float)(yq.y + h0 * k[0][q].y));
     }
-    res_write_ys_upper<NP>(lds, ij, tmp);
-    res_rhs_upper<NP>(Hnh, D, nL, hasH, lds, ij, k[1]);
+    res_write_ys_upper<NP, NW>(lds, ij, tmp);
+    res_rhs_upper<NP, NW>(Hnh, D, nL, hasH, lds, ij, k[1]);
     ++nrhs;
     double s2 = 0;
 #pragma unroll
@@ -524,7 +527,7 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
       const double dr = (double)k[1][q].x - k[0][q].x, di = (double)k[1][q].y - k[0][q].y;
       s2 += ij_weight(ij[q]) * (dr * dr + di * di) / (sc * sc);
     }
-    const double d2 = sqrt(block_sum<kWaves>(s2, sh) * inv_nn) / h0;
+    const double d2 = sqrt(block_sum<NW>(s2, sh) * inv_nn) / h0;
     const double md = fmax(d1, d2);
     const double h1 = md <= 1e-15 ? fmax(1e-6, h0 * 1e-3) : ctrl_pow(0.01 / md, 0.2);
     dt = wave_uniform(fmin(100.0 * h0, h1));
@@ -540,7 +543,7 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
       const int s = pwc_segment(ptimes, nseg, t);
       if (s != seg) {
         seg = s;
-        res_update_pwc<NP>(p, seg);
+        res_update_pwc<NP, NW>(p, seg);
       }
     }
     if (need_k0) {
@@ -549,8 +552,8 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
         const double2 yq = yload(q);
         tmp[q] = make_float2((float)yq.x, (float)yq.y);
       }
-      res_write_ys_upper<NP>(lds, ij, tmp);
-      res_rhs_upper<NP>(Hnh, D, nL, hasH, lds, ij, k[0]);
+      res_write_ys_upper<NP, NW>(lds, ij, tmp);
+      res_rhs_upper<NP, NW>(Hnh, D, nL, hasH, lds, ij, k[0]);
       ++nrhs;
       need_k0 = false;
     }
@@ -596,8 +599,8 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
         }
         tmp[q] = make_float2((float)re, (float)im);
       }
-      res_write_ys_upper<NP>(lds, ij, tmp);
-      res_rhs_upper<NP>(Hnh, D, nL, hasH, lds, ij, tmp);
+      res_write_ys_upper<NP, NW>(lds, ij, tmp);
+      res_rhs_upper<NP, NW>(Hnh, D, nL, hasH, lds, ij, tmp);
       ++nrhs;
 #pragma unroll
       for (int j = 1; j < 7; ++j) {
@@ -625,9 +628,9 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
       ei *= h;
       const double sc = atol + rtol * fmax(sqrt(yq.x * yq.x + yq.y * yq.y), sqrt(yr * yr + yi * yi));
       ssum += ij_weight(ij[q]) * (er * er + ei * ei) / (sc * sc);
-      if (ij[q] >= 0) yn[threadIdx.x + kThreads * q] = make_double2(yr, yi);
+      if (ij[q] >= 0) yn[threadIdx.x + R::NTH * q] = make_double2(yr, yi);
     }
-    const double norm = wave_uniform(sqrt(block_sum<kWaves>(ssum, sh) * inv_nn));
+    const double norm = wave_uniform(sqrt(block_sum<NW>(ssum, sh) * inv_nn));
     const bool finite = isfinite(norm);
     const bool keep = finite && norm < 1.0;
     double factor;
@@ -647,7 +650,7 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
         double w[7];
 #pragma unroll
         for (int j = 0; j < 7; ++j) w[j] = th * (kR[j][0] + th * (kR[j][1] + th * (kR[j][2] + th * kR[j][3])));
-        res_save<NP>(p, yg, ij, k, h, w, true, isave, sh);
+        res_save<NP, NW>(p, yg, ij, k, h, w, true, isave, sh);
         ++isave;
       }
 #pragma unroll
@@ -663,7 +666,7 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
     dt = wave_uniform(factor * h);
   }
   while (isave < ntsave) {
-    res_save<NP>(p, yg, ij, k, 0.0, w0, false, isave, sh);
+    res_save<NP, NW>(p, yg, ij, k, 0.0, w0, false, isave, sh);
     ++isave;
   }
   float2 *fin = gp<float2>(p.final_state);
@@ -673,7 +676,7 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
   for (int q = 0; q < NE; ++q) {
     const int i = ij_row(ij[q]), j = ij_col(ij[q]);
     if (ij[q] >= 0 && j < n) {
-      const double2 yq = yg[threadIdx.x + kThreads * q];
+      const double2 yq = yg[threadIdx.x + R::NTH * q];
       const float2 v = make_float2((float)yq.x, (float)yq.y);
       const float2 vc = make_float2(v.x, -v.y);
       fin[(size_t)i * n + j] = v;

@@ -13,6 +13,8 @@ from __future__ import annotations
 
 import warnings
 
+from dataclasses import replace
+
 import numpy as np
 
 from .._checks import check_hermitian, check_qarray_is_dense, check_shape, check_times
@@ -175,6 +177,16 @@ def mesolve(
     if options.assume_hermitian and isinstance(method, Tsit5):
         for inst in insts:
             check_hermitian(inst.rho0, 'y0')
+    parts, pindex = hermitian_parts([inst.rho0 for inst in insts], options.assume_hermitian and isinstance(method, Tsit5))
+    if len(parts) > len(insts):
+        solve = []
+        for inst, (ia, ib) in zip(insts, pindex):
+            solve.append(replace(inst, rho0=parts[ia], key_rho=None))
+            if ib >= 0:
+                solve.append(replace(inst, rho0=parts[ib], key_rho=None))
+        insts = solve
+    else:
+        insts = [replace(inst, rho0=a) for inst, a in zip(insts, parts)]
     Es = [E.to_numpy() for E in exp_ops] if exp_ops is not None else None
     t0 = None if options.t0 is None else float(options.t0)
 
@@ -212,16 +224,16 @@ def mesolve(
         raise RuntimeError('The solver stopped on a non-finite error estimate or a vanishing step size.')
 
     n = plan.n
-    states = res['states'].reshape(*out_shape, -1, n, n)
+    states = combine_parts(res['states'], pindex).reshape(*out_shape, -1, n, n)
     if not options.save_states:  # only the final state, without a time axis (save_mixin.py:62-66)
         states = states[..., -1, :, :]
     ysave = QArray(states, dims=rho0.dims)
     Esave = None
     if Es is not None:
-        Esave = res['expects'].reshape(*out_shape, len(Es), len(tsave))
+        Esave = combine_parts(res['expects'], pindex).reshape(*out_shape, len(Es), len(tsave))
 
     def shaped(x):
-        x = np.asarray(x).reshape(out_shape)
+        x = np.asarray(x)[pindex[:, 0]].reshape(out_shape)
         return x if out_shape else x[()]
 
     if isinstance(method, Tsit5):
@@ -229,6 +241,45 @@ def mesolve(
     else:
         infos = FixedStepInfos(shaped(res['nsteps']))
     return MESolveResult(tsave, method, gradient, options, ysave, Esave, None, infos)
+
+
+def hermitian_parts(rhos, assume_hermitian: bool):
+    """Hermitian initial states for the kernels, which evolve Hermitian states only
+    (the vector field in the form tmp + tmp^dag, as ``vector_field_unvec_hermitian``
+    at ``dynamiqs/integrators/core/diffrax_integrator.py:319-323``).
+
+    With ``assume_hermitian`` (Tsit5 with ``Options(assume_hermitian=True)``, whose
+    states are checked) every state is Hermitian up to the checked tolerance and is
+    replaced by its Hermitian part. Otherwise (``assume_hermitian=False``, or
+    Rouchon1, which assumes nothing: ``dynamiqs/integrators/apis/mesolve.py:165-171``)
+    a state that is not Hermitian (to 1e-14 relative, above the rounding of a
+    density matrix formed from a ket) is split as rho0 = A + i B with
+    A = (rho0 + rho0^dag) / 2 and B = (rho0 - rho0^dag) / 2i, both Hermitian: the
+    master equation and Tr(E rho) are linear, so rho(t) = A(t) + i B(t) and the two
+    parts are solved as separate instances. Returns the list of states to solve and, per input, the index of A
+    and of B (-1 when there is no B part)."""
+    parts, index = [], []
+    for r in rhos:
+        r = np.asarray(r)
+        a = 0.5 * (r + r.conj().T)
+        ia = len(parts)
+        parts.append(a)
+        ib = -1
+        asym = np.max(np.abs(r - r.conj().T), initial=0.0)
+        if not assume_hermitian and asym > 1e-14 * max(1.0, np.max(np.abs(r), initial=0.0)):
+            ib = len(parts)
+            parts.append(-0.5j * (r - r.conj().T))
+        index.append((ia, ib))
+    return parts, np.asarray(index, dtype=np.int64).reshape(-1, 2)
+
+
+def combine_parts(x: np.ndarray, index: np.ndarray) -> np.ndarray:
+    """x[A] + i x[B] over the first axis (the inverse of ``hermitian_parts``)."""
+    out = x[index[:, 0]].astype(np.result_type(x.dtype, np.complex64))
+    has_b = index[:, 1] >= 0
+    if np.any(has_b):
+        out[has_b] = out[has_b] + 1j * x[index[has_b, 1]]
+    return out
 
 
 def _mesolve_dia(route, rho0: QArray, tsave, exp_ops, method, gradient, options) -> MESolveResult:
@@ -244,10 +295,12 @@ def _mesolve_dia(route, rho0: QArray, tsave, exp_ops, method, gradient, options)
         r = r @ np.conj(np.swapaxes(r, -1, -2))
     Es = [E.to_numpy() for E in exp_ops] if exp_ops is not None else None
     t0 = None if options.t0 is None else float(options.t0)
+    if isinstance(method, Tsit5) and options.assume_hermitian:
+        for x in r:
+            check_hermitian(x, 'y0')
+    parts, pindex = hermitian_parts(r, options.assume_hermitian and isinstance(method, Tsit5))
+    r = np.stack(parts)
     if isinstance(method, Tsit5):
-        if options.assume_hermitian:
-            for x in r:
-                check_hermitian(x, 'y0')
         plan = dia_path.tsit5_plan(
             Hd,
             Ld,
@@ -271,19 +324,20 @@ def _mesolve_dia(route, rho0: QArray, tsave, exp_ops, method, gradient, options)
             raise max_steps_error()
         if np.any(res['status'] == -4):
             raise RuntimeError('The solver stopped on a non-finite error estimate or a vanishing step size.')
-    states = res['states'].reshape(*out_shape, -1, n, n)
+    states = combine_parts(res['states'], pindex).reshape(*out_shape, -1, n, n)
     if not options.save_states:  # only the final state, without a time axis (save_mixin.py:62-66)
         states = states[..., -1, :, :]
     ysave = QArray(states, dims=rho0.dims)
     Esave = None
     if Es is not None:
-        Esave = res['expects'].reshape(*out_shape, len(Es), len(tsave))
+        Esave = combine_parts(res['expects'], pindex).reshape(*out_shape, len(Es), len(tsave))
+
     def shaped(x):
-        x = np.asarray(x).reshape(out_shape)
+        x = np.asarray(x)[pindex[:, 0]].reshape(out_shape)
         return x if out_shape else x[()]
 
     if isinstance(method, Tsit5):
         infos = AdaptiveStepInfos(shaped(res['nsteps']), shaped(res['naccepted']), shaped(res['nrejected']))
     else:
-        infos = FixedStepInfos(shaped(np.full(len(r), plan.nsteps, dtype=np.int64)))
+        infos = FixedStepInfos(shaped(np.full(len(pindex), plan.nsteps, dtype=np.int64)))
     return MESolveResult(tsave, method, gradient, options, ysave, Esave, None, infos)

@@ -76,3 +76,38 @@ def test_modulated_operators_are_reported_unsupported():
     Hm = dq.modulated(lambda t: np.cos(t), H)
     with pytest.raises(NotImplementedError, match='ModulatedTimeQArray operators are not supported'):
         dq.mesolve(Hm, Ls, psi, ts)
+
+
+def test_non_hermitian_rho0_error_with_assume_hermitian():
+    # dynamiqs/integrators/core/diffrax_integrator.py:339-343 + dynamiqs/_checks.py:102-108
+    H, Ls, _, ts = _args()
+    rho = np.zeros((3, 3), dtype=complex)
+    rho[0, 1] = 1.0
+    with pytest.raises(ValueError, match=re.escape('Argument y0 is not hermitian.')):
+        dq.mesolve(H, Ls, rho, ts)
+
+
+def test_hermitian_parts_split_and_recombine():
+    """rho0 = A + iB with A, B Hermitian; a Hermitian state is not split; the
+    recombination inverts the split on a stack of states and expectation values."""
+    from dynamiqs_amd.integrators.mesolve import combine_parts, hermitian_parts
+
+    rng = np.random.default_rng(5)
+    n = 5
+    x = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    h = x + x.conj().T
+    rhos = [x, h, 1j * h, x.T]
+    parts, index = hermitian_parts(rhos, assume_hermitian=False)
+    assert index.tolist() == [[0, 1], [2, -1], [3, 4], [5, 6]]
+    for p in parts:
+        np.testing.assert_array_equal(p, p.conj().T)
+    # the parts of a Hermitian state are the state itself
+    np.testing.assert_array_equal(parts[2], h)
+    # recombination of the parts themselves, and of a (parts, 2, n, n) stack
+    np.testing.assert_allclose(combine_parts(np.stack(parts), index), np.stack(rhos), atol=1e-15)
+    stack = np.stack([np.stack([p, 2 * p]) for p in parts])
+    np.testing.assert_allclose(combine_parts(stack, index), np.stack([np.stack([r, 2 * r]) for r in rhos]), atol=1e-14)
+    # with assume_hermitian every state is replaced by its Hermitian part, never split
+    parts, index = hermitian_parts(rhos, assume_hermitian=True)
+    assert index[:, 1].tolist() == [-1] * 4
+    np.testing.assert_allclose(parts[0], 0.5 * (x + x.conj().T))

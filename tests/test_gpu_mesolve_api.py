@@ -86,3 +86,50 @@ def test_result_fields_and_save_states_false():
     np.testing.assert_allclose(res.expects, full.expects, atol=1e-6)
     s = str(full)
     assert '|' in s or 'MESolveResult' in s
+
+
+def _expm_solution(H, Ls, rho0, tsave):
+    """Exact propagation rho(t) = exp(t L) rho0 with the superoperator in
+    complex128 (linear in rho0, so valid for non-Hermitian states)."""
+    from scipy.linalg import expm
+
+    from oracle import lindblad
+
+    S = lindblad.slindbladian(H, Ls)
+    v0 = lindblad.vectorize(rho0)
+    return np.stack([lindblad.unvectorize(expm(t * S) @ v0) for t in tsave])
+
+
+def test_non_hermitian_rho0_tsit5_and_rouchon1():
+    """assume_hermitian=False (Tsit5) and Rouchon1 (which assumes nothing) on a
+    non-Hermitian initial state, against exact propagation: the host splits rho0
+    into Hermitian parts A + iB, solves both on the GPU and recombines states and
+    expectation values."""
+    rng = np.random.default_rng(8)
+    n = 6
+    a = np.diag(np.sqrt(np.arange(1, n)), k=1).astype(complex)
+    H = a.conj().T @ a + 0.3 * (a + a.conj().T)
+    Ls = [0.4 * a]
+    x = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    rho0 = x / np.linalg.norm(x)
+    E = a.conj().T @ a
+    tsave = np.linspace(0.0, 1.0, 6)
+    ref = _expm_solution(H, Ls, rho0, tsave)
+    ref_e = np.einsum('ij,tji->t', E, ref)
+    res = dq.mesolve(H, Ls, rho0, tsave, exp_ops=[E], options=dq.Options(assume_hermitian=False))
+    np.testing.assert_allclose(res.states.to_numpy(), ref, atol=2e-5)
+    np.testing.assert_allclose(res.expects[0], ref_e, atol=1e-4)
+    # Rouchon1 (first order): against exact propagation at its own O(dt) accuracy
+    # and against the complex128 Rouchon1 oracle at FP32 accuracy
+    from oracle import mesolve as omesolve
+
+    dt = 1e-3
+    r1 = dq.mesolve(H, Ls, rho0, tsave, exp_ops=[E], method=dq.method.Rouchon1(dt=dt))
+    ro = omesolve.mesolve(H, Ls, rho0, tsave, method='rouchon1', dt=dt, exp_ops=[E])
+    np.testing.assert_allclose(r1.states.to_numpy(), ro['states'], atol=1e-5)
+    np.testing.assert_allclose(r1.states.to_numpy(), ref, atol=5e-3)
+    # a batch mixing a Hermitian and a non-Hermitian state keeps each element's own answer
+    herm = 0.5 * (rho0 + rho0.conj().T)
+    rb = dq.mesolve(H, Ls, np.stack([herm, rho0]), tsave, options=dq.Options(assume_hermitian=False))
+    np.testing.assert_allclose(rb.states.to_numpy()[1], res.states.to_numpy(), atol=1e-6)
+    np.testing.assert_allclose(rb.states.to_numpy()[0], _expm_solution(H, Ls, herm, tsave), atol=2e-5)
