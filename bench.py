@@ -34,9 +34,9 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per batched launch of the resident Tsit5 kernel on the default workload
 # (Kerr n=64, B=256), from two rocprofv3 --pmc runs of this script (FETCH_SIZE,
 # then WRITE_SIZE) scaled by the same counters on a device copy of known size
-# (--config copy: FETCH_SIZE x 2, WRITE_SIZE x 1): 1 880 670 853 KiB fetched,
-# 1 089 909 003 KiB written. Mostly register spill traffic of the np = 64 body.
-KERR_TRAFFIC_BYTES = 1_880_670_853 * 1024 * 2 + 1_089_909_003 * 1024
+# (--config copy: FETCH_SIZE x 2, WRITE_SIZE x 1): 133 797 306 KiB fetched,
+# 232 698 948 KiB written (packed FP64 state writes and register spills).
+KERR_TRAFFIC_BYTES = 133_797_306 * 1024 * 2 + 232_698_948 * 1024
 
 
 def kerr_instances(B: int, n: int = 64, seed: int = 0):
