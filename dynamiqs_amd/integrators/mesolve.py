@@ -166,8 +166,6 @@ def mesolve(
     options = options.initialise()
     assert_method_supported(method, SUPPORTED_METHODS)
     method.assert_supports_gradient(gradient)
-    if options.save_extra is not None:
-        raise NotImplementedError('`save_extra` is not supported by the MI355X solver')
 
     route = dia_route(method, H, Ls, H.shape[-1])
     if route is not None:
@@ -189,6 +187,8 @@ def mesolve(
         insts = [replace(inst, rho0=a) for inst, a in zip(insts, parts)]
     Es = [E.to_numpy() for E in exp_ops] if exp_ops is not None else None
     t0 = None if options.t0 is None else float(options.t0)
+    # save_extra sees the state at every save time, whatever save_states is
+    save_all = options.save_states or options.save_extra is not None
 
     if isinstance(method, Tsit5):
         plan = dense.build_plan(
@@ -203,7 +203,7 @@ def mesolve(
             factormin=method.min_factor,
             factormax=method.max_factor,
             max_steps=method.max_steps,
-            save_states=options.save_states,
+            save_states=save_all,
         )
     else:
         plan = dense.build_plan(
@@ -214,7 +214,7 @@ def mesolve(
             t0=t0,
             dt=method.dt,
             normalize=method.normalize,
-            save_states=options.save_states,
+            save_states=save_all,
         )
     plan.run()
     res = plan.results()
@@ -224,7 +224,11 @@ def mesolve(
         raise RuntimeError('The solver stopped on a non-finite error estimate or a vanishing step size.')
 
     n = plan.n
-    states = combine_parts(res['states'], pindex).reshape(*out_shape, -1, n, n)
+    states = combine_parts(res['states'], pindex)
+    extra = None
+    if options.save_extra is not None:
+        extra = save_extra_outputs(options.save_extra, states, rho0.dims, out_shape)
+    states = states.reshape(*out_shape, -1, n, n)
     if not options.save_states:  # only the final state, without a time axis (save_mixin.py:62-66)
         states = states[..., -1, :, :]
     ysave = QArray(states, dims=rho0.dims)
@@ -240,7 +244,53 @@ def mesolve(
         infos = AdaptiveStepInfos(shaped(res['nsteps']), shaped(res['naccepted']), shaped(res['nrejected']))
     else:
         infos = FixedStepInfos(shaped(res['nsteps']))
-    return MESolveResult(tsave, method, gradient, options, ysave, Esave, None, infos)
+    return MESolveResult(tsave, method, gradient, options, ysave, Esave, extra, infos)
+
+
+def _stack_trees(trees: list):
+    """Stack a list of equally structured outputs of ``save_extra`` leaf by leaf
+    (dicts, lists, tuples and named tuples, ``QArray``, array-likes; ``None``
+    stays ``None``) along a new first axis."""
+    t0 = trees[0]
+    if t0 is None:
+        return None
+    if isinstance(t0, dict):
+        return {k: _stack_trees([t[k] for t in trees]) for k in t0}
+    if isinstance(t0, tuple) and hasattr(t0, '_fields'):
+        return type(t0)(*[_stack_trees([t[i] for t in trees]) for i in range(len(t0))])
+    if isinstance(t0, (list, tuple)):
+        return type(t0)(_stack_trees([t[i] for t in trees]) for i in range(len(t0)))
+    if isinstance(t0, QArray):
+        return QArray(np.stack([t.to_numpy() for t in trees]), dims=t0.dims)
+    return np.stack([np.asarray(t) for t in trees])
+
+
+def _reshape_tree(tree, lead: tuple):
+    """Split the first axis of every leaf into ``lead``."""
+    if tree is None:
+        return None
+    if isinstance(tree, dict):
+        return {k: _reshape_tree(v, lead) for k, v in tree.items()}
+    if isinstance(tree, tuple) and hasattr(tree, '_fields'):
+        return type(tree)(*[_reshape_tree(v, lead) for v in tree])
+    if isinstance(tree, (list, tuple)):
+        return type(tree)(_reshape_tree(v, lead) for v in tree)
+    if isinstance(tree, QArray):
+        x = tree.to_numpy()
+        return QArray(x.reshape(*lead, *x.shape[1:]), dims=tree.dims)
+    return tree.reshape(*lead, *tree.shape[1:])
+
+
+def save_extra_outputs(fn, states: np.ndarray, dims, out_shape: tuple):
+    """``Options(save_extra=fn)`` as a host callback: ``fn`` is applied to the state
+    (a ``QArray``) at every save time, as ``SolveSaveMixin.save`` does
+    (``dynamiqs/integrators/core/save_mixin.py:47-54``). The outputs are stacked
+    along time and then over the batch, so each leaf has shape
+    ``(*batch, ntsave, *leaf)`` as in the reference's vmapped scan. ``states`` is
+    ``(nbatch, ntsave, n, n)``."""
+    nb, nt = states.shape[:2]
+    per_inst = [_stack_trees([fn(QArray(states[b, t], dims=dims)) for t in range(nt)]) for b in range(nb)]
+    return _reshape_tree(_stack_trees(per_inst), tuple(out_shape))
 
 
 def hermitian_parts(rhos, assume_hermitian: bool):
@@ -324,7 +374,11 @@ def _mesolve_dia(route, rho0: QArray, tsave, exp_ops, method, gradient, options)
             raise max_steps_error()
         if np.any(res['status'] == -4):
             raise RuntimeError('The solver stopped on a non-finite error estimate or a vanishing step size.')
-    states = combine_parts(res['states'], pindex).reshape(*out_shape, -1, n, n)
+    states = combine_parts(res['states'], pindex)
+    extra = None
+    if options.save_extra is not None:
+        extra = save_extra_outputs(options.save_extra, states, rho0.dims, out_shape)
+    states = states.reshape(*out_shape, -1, n, n)
     if not options.save_states:  # only the final state, without a time axis (save_mixin.py:62-66)
         states = states[..., -1, :, :]
     ysave = QArray(states, dims=rho0.dims)
@@ -340,4 +394,4 @@ def _mesolve_dia(route, rho0: QArray, tsave, exp_ops, method, gradient, options)
         infos = AdaptiveStepInfos(shaped(res['nsteps']), shaped(res['naccepted']), shaped(res['nrejected']))
     else:
         infos = FixedStepInfos(shaped(np.full(len(pindex), plan.nsteps, dtype=np.int64)))
-    return MESolveResult(tsave, method, gradient, options, ysave, Esave, None, infos)
+    return MESolveResult(tsave, method, gradient, options, ysave, Esave, extra, infos)

@@ -85,7 +85,12 @@ class QArray:
             self._data = np.asarray(data)
             self._offsets = None
             self._diags = None
-            n = self._data.shape[-1] if self._data.shape[-1] != 1 else self._data.shape[-2]
+            if dims is not None:
+                n = None
+            elif self._data.ndim >= 2:
+                n = self._data.shape[-1] if self._data.shape[-1] != 1 else self._data.shape[-2]
+            else:  # an element or a row taken out of a qarray with explicit dims
+                n = self._data.shape[-1] if self._data.ndim == 1 else 1
         else:
             self._data = None
             self._offsets = tuple(int(o) for o in offsets)

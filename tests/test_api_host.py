@@ -111,3 +111,29 @@ def test_hermitian_parts_split_and_recombine():
     parts, index = hermitian_parts(rhos, assume_hermitian=True)
     assert index[:, 1].tolist() == [-1] * 4
     np.testing.assert_allclose(parts[0], 0.5 * (x + x.conj().T))
+
+
+def test_save_extra_outputs_stacking():
+    """save_extra outputs are stacked leaf by leaf with shape (*batch, ntsave, *leaf)
+    (dicts, named tuples, lists, QArray leaves), as the reference's vmapped scan
+    stacks them (dynamiqs/integrators/core/save_mixin.py:47-54)."""
+    from collections import namedtuple
+
+    from dynamiqs_amd.integrators.mesolve import save_extra_outputs
+
+    P = namedtuple('P', 'a b')
+    st = np.random.default_rng(0).normal(size=(6, 4, 3, 3)) + 0j
+
+    def fn(r):
+        x = r.to_numpy()
+        return {'tr': r.trace(), 'pair': P(x[0, 0], [x[1, 1]]), 'q': r @ r}
+
+    out = save_extra_outputs(fn, st, (3,), (2, 3))
+    assert out['tr'].shape == (2, 3, 4)
+    assert out['pair'].a.shape == (2, 3, 4) and out['pair'].b[0].shape == (2, 3, 4)
+    assert isinstance(out['q'], dq.QArray) and out['q'].shape == (2, 3, 4, 3, 3)
+    np.testing.assert_allclose(out['tr'][1, 2, 3], np.trace(st[5, 3]))
+    np.testing.assert_allclose(out['q'].to_numpy()[0, 1, 2], st[1, 2] @ st[1, 2])
+    # no batch: the time axis leads
+    one = save_extra_outputs(fn, st[:1], (3,), ())
+    assert one['tr'].shape == (4,) and one['q'].shape == (4, 3, 3)

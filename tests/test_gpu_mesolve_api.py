@@ -133,3 +133,30 @@ def test_non_hermitian_rho0_tsit5_and_rouchon1():
     rb = dq.mesolve(H, Ls, np.stack([herm, rho0]), tsave, options=dq.Options(assume_hermitian=False))
     np.testing.assert_allclose(rb.states.to_numpy()[1], res.states.to_numpy(), atol=1e-6)
     np.testing.assert_allclose(rb.states.to_numpy()[0], _expm_solution(H, Ls, herm, tsave), atol=2e-5)
+
+
+def test_save_extra_every_save_time_with_batching_and_no_states():
+    """Options(save_extra=fn) sees the state at every save time, also with
+    save_states=False (save_mixin.py:47-54, 62-66). Its outputs have shape
+    (*batch, ntsave, ...) and equal fn applied to the states of the same solve with
+    save_states=True."""
+    n = 5
+    a = dq.destroy(n)
+    H = np.stack([(a.dag() @ a).to_numpy() * w for w in (1.0, 1.5, 2.0)])
+    Ls = [0.2 * a]
+    num = (a.dag() @ a).to_numpy()
+    psi = dq.coherent(n, 0.7)
+    tsave = np.linspace(0.0, 1.0, 7)
+
+    def fn(rho):
+        r = rho.to_numpy()
+        return {'pop': np.real(np.trace(num @ r)), 'corner': rho[0, 1]}
+
+    full = dq.mesolve(H, Ls, psi, tsave)
+    res = dq.mesolve(H, Ls, psi, tsave, options=dq.Options(save_extra=fn, save_states=False))
+    assert res.states.shape == (3, n, n)
+    assert res.extra['pop'].shape == (3, len(tsave))
+    st = full.states.to_numpy()
+    np.testing.assert_allclose(res.extra['pop'], np.real(np.einsum('ij,btji->bt', num, st)), atol=1e-6)
+    np.testing.assert_allclose(np.asarray(res.extra['corner'].to_numpy()), st[:, :, 0, 1], atol=1e-6)
+    np.testing.assert_allclose(res.states.to_numpy(), st[:, -1], atol=1e-6)
