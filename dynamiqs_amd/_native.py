@@ -56,6 +56,7 @@ ME_INST_DTYPE = np.dtype(ME_INST_FIELDS)
 FLAG_SAVE_STATES = 1
 FLAG_HAS_HNH = 2
 FLAG_GLOBAL = 4
+FLAG_DOPRI5 = 8
 
 STATUS_OK = 0
 STATUS_MAX_STEPS = -3

@@ -250,6 +250,7 @@ __device__ __forceinline__ void rhs_global(const dqh_me_inst &p) {
 // ----------------------------------------------------------- Tsit5 kernel
 template <int NW>
 __device__ __forceinline__ void tsit5_global(const dqh_me_inst &p, double *sh) {
+  const RKTab &rk = rk_tab(p.flags);
   View v;
   make_view(v, p);
   const double *tsave = gp<const double>(p.tsave);
@@ -366,7 +367,7 @@ __device__ __forceinline__ void tsit5_global(const dqh_me_inst &p, double *sh) {
       for (size_t e = threadIdx.x; e < v.nn; e += NW * 64) {
         double re = y[e].x, im = y[e].y;
         for (int j = 0; j < s; ++j) {
-          const double c = h * kA[s][j];
+          const double c = h * rk.A[s][j];
           const float2 kj = kp(v, j)[e];
           re += c * kj.x;
           im += c * kj.y;
@@ -385,8 +386,8 @@ __device__ __forceinline__ void tsit5_global(const dqh_me_inst &p, double *sh) {
 #pragma unroll
       for (int j = 0; j < 7; ++j) {
         const float2 kj = kp(v, j)[e];
-        er += kE[j] * kj.x;
-        ei += kE[j] * kj.y;
+        er += rk.E[j] * kj.x;
+        ei += rk.E[j] * kj.y;
       }
       er *= h;
       ei *= h;
@@ -412,7 +413,7 @@ __device__ __forceinline__ void tsit5_global(const dqh_me_inst &p, double *sh) {
       while (isave < v.ntsave && tsave[isave] <= t_new) {
         const double th = (tsave[isave] - t) / h;
         double w[7];
-        for (int j = 0; j < 7; ++j) w[j] = th * (kR[j][0] + th * (kR[j][1] + th * (kR[j][2] + th * kR[j][3])));
+        for (int j = 0; j < 7; ++j) w[j] = th * (rk.R[j][0] + th * (rk.R[j][1] + th * (rk.R[j][2] + th * rk.R[j][3])));
         wg_save<NW>(v, p, y, h, w, 7, isave, sh);
         ++isave;
       }

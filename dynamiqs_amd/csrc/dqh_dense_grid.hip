@@ -131,6 +131,7 @@ __device__ __forceinline__ float2 field_at(const float2 *tmp, int np, int i, int
 __global__ __launch_bounds__(256) void k_body(const dqh_me_inst *__restrict__ insts, G g, int s, int cur) {
   __shared__ double sh[4];
   const dqh_me_inst &p = insts[blockIdx.y];
+  const RKTab &rk = rk_tab(p.flags);
   const Ctrl c = ctrl_of(g, cur);
   if (is_done(c, p.t1)) return;
   const double h = step_h(c, p.t1);
@@ -144,21 +145,21 @@ __global__ __launch_bounds__(256) void k_body(const dqh_me_inst *__restrict__ in
     kslot(p, s, c.rot)[e] = f;
     const double2 yv = gp<const double2>(p.y)[e];
     if (s < 6) {
-      double re = yv.x + h * kA[s + 1][s] * f.x, im = yv.y + h * kA[s + 1][s] * f.y;
+      double re = yv.x + h * rk.A[s + 1][s] * f.x, im = yv.y + h * rk.A[s + 1][s] * f.y;
       for (int q = 0; q < s; ++q) {
         const float2 kq = kslot(p, q, c.rot)[e];
-        re += h * kA[s + 1][q] * kq.x;
-        im += h * kA[s + 1][q] * kq.y;
+        re += h * rk.A[s + 1][q] * kq.x;
+        im += h * rk.A[s + 1][q] * kq.y;
       }
       ysb(p)[e] = make_float2((float)re, (float)im);
     } else if (i < n && j < n) {
-      double er = kE[6] * f.x, ei = kE[6] * f.y, yr = yv.x, yi = yv.y;
+      double er = rk.E[6] * f.x, ei = rk.E[6] * f.y, yr = yv.x, yi = yv.y;
       for (int q = 0; q < 6; ++q) {
         const float2 kq = kslot(p, q, c.rot)[e];
-        er += kE[q] * kq.x;
-        ei += kE[q] * kq.y;
-        yr += h * kA[6][q] * kq.x;
-        yi += h * kA[6][q] * kq.y;
+        er += rk.E[q] * kq.x;
+        ei += rk.E[q] * kq.y;
+        yr += h * rk.A[6][q] * kq.x;
+        yi += h * rk.A[6][q] * kq.y;
       }
       er *= h;
       ei *= h;
@@ -177,6 +178,7 @@ __global__ __launch_bounds__(256) void k_commit(const dqh_me_inst *__restrict__ 
   __shared__ double sh[4];
   const size_t inst = blockIdx.y;
   const dqh_me_inst &p = insts[inst];
+  const RKTab &rk = rk_tab(p.flags);
   const Ctrl c = ctrl_of(g, cur);
   Ctrl *nx = g.ctrl + (size_t)(1 - cur) * g.ninst + inst;
   const bool writer = blockIdx.x == 0 && threadIdx.x == 0;
@@ -241,7 +243,7 @@ __global__ __launch_bounds__(256) void k_commit(const dqh_me_inst *__restrict__ 
           double re = yv.x, im = yv.y;
 #pragma unroll
           for (int q = 0; q < 7; ++q) {
-            const double w = th * (kR[q][0] + th * (kR[q][1] + th * (kR[q][2] + th * kR[q][3])));
+            const double w = th * (rk.R[q][0] + th * (rk.R[q][1] + th * (rk.R[q][2] + th * rk.R[q][3])));
             re += h * w * kk[q].x;
             im += h * w * kk[q].y;
           }
@@ -250,15 +252,15 @@ __global__ __launch_bounds__(256) void k_commit(const dqh_me_inst *__restrict__ 
       }
 #pragma unroll
       for (int q = 0; q < 6; ++q) {
-        y1.x += h * kA[6][q] * kk[q].x;
-        y1.y += h * kA[6][q] * kk[q].y;
+        y1.x += h * rk.A[6][q] * kk[q].x;
+        y1.y += h * rk.A[6][q] * kk[q].y;
       }
       *yp = y1;
       k0n = kk[6];
     }
     if (!is_done(o, p.t1)) {
       const double hn = step_h(o, p.t1);
-      ysb(p)[e] = make_float2((float)(y1.x + hn * kA[1][0] * k0n.x), (float)(y1.y + hn * kA[1][0] * k0n.y));
+      ysb(p)[e] = make_float2((float)(y1.x + hn * rk.A[1][0] * k0n.x), (float)(y1.y + hn * rk.A[1][0] * k0n.y));
     }
   }
   if (writer) *nx = o;
@@ -371,6 +373,7 @@ __global__ __launch_bounds__(256) void k_init_dt(const dqh_me_inst *__restrict__
   __shared__ double sh[4];
   const size_t inst = blockIdx.y;
   const dqh_me_inst &p = insts[inst];
+  const RKTab &rk = rk_tab(p.flags);
   const Ctrl c = g.ctrl[g.ninst + inst];
   if (is_done(c, p.t1)) return;
   const double inv_nn = 1.0 / ((double)p.n * (double)p.n);
@@ -388,7 +391,7 @@ __global__ __launch_bounds__(256) void k_init_dt(const dqh_me_inst *__restrict__
   if (e < (int64_t)nn && o.status == 0) {
     const double2 yv = gp<const double2>(p.y)[e];
     const float2 kq = kslot(p, 0, 0)[e];
-    ysb(p)[e] = make_float2((float)(yv.x + hn * kA[1][0] * kq.x), (float)(yv.y + hn * kA[1][0] * kq.y));
+    ysb(p)[e] = make_float2((float)(yv.x + hn * rk.A[1][0] * kq.x), (float)(yv.y + hn * rk.A[1][0] * kq.y));
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) g.ctrl[inst] = o;
 }

@@ -42,7 +42,7 @@ struct Ctrl {
 static_assert(sizeof(Ctrl) == 64, "control block is 64 bytes");
 
 struct V {
-  int64_t n, n_true, ninst, nsave, max_steps, npart;
+  int64_t n, n_true, ninst, nsave, max_steps, npart, flags;
   double rtol, atol, safety, fmin_, fmax_, t0, t1, inv_nn;
   float2 *k;      // [7][ninst][n][n]
   float2 *ys;     // [2][ninst][n][n]
@@ -132,6 +132,7 @@ constexpr int nthreads() {
 // ------------------------------------------------------------------ one step
 template <int HALO>
 __global__ __launch_bounds__(nthreads<HALO>()) void k_stage(V v, dqh_dia_ops ops, int s, int cur) {
+  const RKTab &rk = rk_tab(v.flags);
   constexpr int NT = nthreads<HALO>();
   __shared__ double sh[NT / 64];
   float2 *sx = nullptr;
@@ -154,21 +155,21 @@ __global__ __launch_bounds__(nthreads<HALO>()) void k_stage(V v, dqh_dia_ops ops
     ks[e] = f;
     const double2 yv = y[e];
     if (s < 6) {
-      double re = yv.x + h * kA[s + 1][s] * f.x, im = yv.y + h * kA[s + 1][s] * f.y;
+      double re = yv.x + h * rk.A[s + 1][s] * f.x, im = yv.y + h * rk.A[s + 1][s] * f.y;
       for (int q = 0; q < s; ++q) {
         const float2 kq = kslot(v, q, c.rot, inst)[e];
-        re += h * kA[s + 1][q] * kq.x;
-        im += h * kA[s + 1][q] * kq.y;
+        re += h * rk.A[s + 1][q] * kq.x;
+        im += h * rk.A[s + 1][q] * kq.y;
       }
       nxt[e] = make_float2((float)re, (float)im);
     } else if (i < v.n_true && j < v.n_true) {
-      double er = kE[6] * f.x, ei = kE[6] * f.y, yr = yv.x, yi = yv.y;
+      double er = rk.E[6] * f.x, ei = rk.E[6] * f.y, yr = yv.x, yi = yv.y;
       for (int q = 0; q < 6; ++q) {
         const float2 kq = kslot(v, q, c.rot, inst)[e];
-        er += kE[q] * kq.x;
-        ei += kE[q] * kq.y;
-        yr += h * kA[6][q] * kq.x;
-        yi += h * kA[6][q] * kq.y;
+        er += rk.E[q] * kq.x;
+        ei += rk.E[q] * kq.y;
+        yr += h * rk.A[6][q] * kq.x;
+        yi += h * rk.A[6][q] * kq.y;
       }
       er *= h;
       ei *= h;
@@ -184,6 +185,7 @@ __global__ __launch_bounds__(nthreads<HALO>()) void k_stage(V v, dqh_dia_ops ops
 
 // accept/reject, saves, y update, next stage input; grid (ceil(n^2/256), ninst)
 __global__ __launch_bounds__(256) void k_commit(V v, int cur) {
+  const RKTab &rk = rk_tab(v.flags);
   __shared__ double sh[4];
   const size_t inst = blockIdx.y;
   const Ctrl c = v.ctrl[(size_t)cur * v.ninst + inst];
@@ -246,7 +248,7 @@ __global__ __launch_bounds__(256) void k_commit(V v, int cur) {
         double re = yv.x, im = yv.y;
 #pragma unroll
         for (int q = 0; q < 7; ++q) {
-          const double w = th * (kR[q][0] + th * (kR[q][1] + th * (kR[q][2] + th * kR[q][3])));
+          const double w = th * (rk.R[q][0] + th * (rk.R[q][1] + th * (rk.R[q][2] + th * rk.R[q][3])));
           re += h * w * kk[q].x;
           im += h * w * kk[q].y;
         }
@@ -255,15 +257,15 @@ __global__ __launch_bounds__(256) void k_commit(V v, int cur) {
       }
 #pragma unroll
       for (int q = 0; q < 6; ++q) {
-        y1.x += h * kA[6][q] * kk[q].x;
-        y1.y += h * kA[6][q] * kk[q].y;
+        y1.x += h * rk.A[6][q] * kk[q].x;
+        y1.y += h * rk.A[6][q] * kk[q].y;
       }
       *yp = y1;
       k0n = kk[6];
     }
     if (!is_done(o, v.t1)) {
       const double hn = step_h(o, v.t1);
-      ysbuf(v, 0, inst)[e] = make_float2((float)(y1.x + hn * kA[1][0] * k0n.x), (float)(y1.y + hn * kA[1][0] * k0n.y));
+      ysbuf(v, 0, inst)[e] = make_float2((float)(y1.x + hn * rk.A[1][0] * k0n.x), (float)(y1.y + hn * rk.A[1][0] * k0n.y));
     }
   }
   if (writer) *nx = o;
@@ -380,6 +382,7 @@ __global__ __launch_bounds__(256) void k_init_h0(V v) {
 // dt0 = min(100 h0, (0.01 / max(d1, d2))^(1/5)); control block 0 and ys_1 of the
 // first attempt.
 __global__ __launch_bounds__(256) void k_init_dt(V v) {
+  const RKTab &rk = rk_tab(v.flags);
   __shared__ double sh[4];
   const size_t inst = blockIdx.y;
   const Ctrl c = v.ctrl[v.ninst + inst];
@@ -400,7 +403,7 @@ __global__ __launch_bounds__(256) void k_init_dt(V v) {
     const int64_t e = (int64_t)i * n + j;
     const double2 yv = v.y[inst * nn + e];
     const float2 kq = kslot(v, 0, 0, inst)[e];
-    ysbuf(v, 0, inst)[e] = make_float2((float)(yv.x + hn * kA[1][0] * kq.x), (float)(yv.y + hn * kA[1][0] * kq.y));
+    ysbuf(v, 0, inst)[e] = make_float2((float)(yv.x + hn * rk.A[1][0] * kq.x), (float)(yv.y + hn * rk.A[1][0] * kq.y));
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) v.ctrl[inst] = o;
 }
@@ -593,6 +596,7 @@ extern "C" int dqh_mesolve_tsit5_dia(const dqh_dia_tsit5 *pb, void *stream) {
   v.ninst = p.ninst;
   v.nsave = p.nsave;
   v.max_steps = p.max_steps;
+  v.flags = p.flags;
   v.rtol = p.rtol;
   v.atol = p.atol;
   v.safety = p.safety;

@@ -442,6 +442,7 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
   const double inv_nn = 1.0 / ((double)n * n);
   const double t_end = p.t1;
 
+  const RKTab &rk = rk_tab(p.flags);
   int ij[NE];
   upper_map<NP, NW>(ij);
   // y: packed FP64 upper triangle, element u = t + 512 q of thread t at yg[u]. rho0
@@ -587,7 +588,7 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
     for (int s = 1; s < 7; ++s) {
       double c[6];
 #pragma unroll
-      for (int j = 0; j < 6; ++j) c[j] = (j < s) ? h * kA[s][j] : 0.0;
+      for (int j = 0; j < 6; ++j) c[j] = (j < s) ? h * rk.A[s][j] : 0.0;
 #pragma unroll
       for (int q = 0; q < NE; ++q) {
         const double2 yq = yload(q);
@@ -619,10 +620,10 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
       double er = 0, ei = 0, yr = yq.x, yi = yq.y;
 #pragma unroll
       for (int j = 0; j < 7; ++j) {
-        er += kE[j] * k[j][q].x;
-        ei += kE[j] * k[j][q].y;
-        yr += h * kA[6][j] * k[j][q].x;
-        yi += h * kA[6][j] * k[j][q].y;
+        er += rk.E[j] * k[j][q].x;
+        ei += rk.E[j] * k[j][q].y;
+        yr += h * rk.A[6][j] * k[j][q].x;
+        yi += h * rk.A[6][j] * k[j][q].y;
       }
       er *= h;
       ei *= h;
@@ -649,7 +650,7 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
         const double th = (tsave[isave] - t) / h;
         double w[7];
 #pragma unroll
-        for (int j = 0; j < 7; ++j) w[j] = th * (kR[j][0] + th * (kR[j][1] + th * (kR[j][2] + th * kR[j][3])));
+        for (int j = 0; j < 7; ++j) w[j] = th * (rk.R[j][0] + th * (rk.R[j][1] + th * (rk.R[j][2] + th * rk.R[j][3])));
         res_save<NP, NW>(p, yg, ij, k, h, w, true, isave, sh);
         ++isave;
       }

@@ -296,6 +296,7 @@ def build_plan(
     grid_tsit5: bool | str = 'auto',
     graph_steps: int = 16,
     step_log_cap: int = 0,
+    tableau: str = 'tsit5',
 ) -> DensePlan:
     """Pack ``instances`` (same Hilbert dimension) for ``kind`` in {'tsit5', 'rouchon1', 'rhs'}.
 
@@ -306,8 +307,15 @@ def build_plan(
     ``grid_tsit5`` runs Tsit5 as grid-wide launches per stage with the controller on the
     device (``dqh_mesolve_tsit5_dense_grid``) instead of one persistent workgroup
     per instance: True, False, or 'auto' (when the batch is too small to fill the
-    GPU with one workgroup per instance and n >= GRID_MIN_N)."""
+    GPU with one workgroup per instance and n >= GRID_MIN_N).
+
+    ``tableau`` is the 5(4) pair of the adaptive kinds: 'tsit5' (``Tsit5``) or
+    'dopri5' (``Dopri5``, ``DQH_FLAG_DOPRI5``); both use the same controller,
+    error norm and saving."""
     import torch
+
+    if tableau not in ('tsit5', 'dopri5'):
+        raise ValueError(f'unknown Runge-Kutta pair {tableau!r}')
 
     dev = _device()
     _native.lib()
@@ -409,6 +417,8 @@ def build_plan(
             Ls = inst.Ls
             d['nL'] = K
             d['flags'] |= _native.FLAG_HAS_HNH
+            if tableau == 'dopri5':
+                d['flags'] |= _native.FLAG_DOPRI5
             if use_global:
                 d['flags'] |= _native.FLAG_GLOBAL
 

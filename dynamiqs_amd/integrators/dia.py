@@ -334,7 +334,7 @@ class DiaTsit5(ctypes.Structure):
         ('factormax', ctypes.c_double),
         ('t0', ctypes.c_double),
         ('t1', ctypes.c_double),
-        ('pad_', ctypes.c_double),
+        ('flags', ctypes.c_int64),
         ('ops', DiaOps),
         ('y', ctypes.c_uint64),
         ('work', ctypes.c_uint64),
@@ -380,6 +380,12 @@ class DiaTsit5Plan:
         return out
 
 
+def _tableau_flag(tableau: str) -> int:
+    if tableau not in ('tsit5', 'dopri5'):
+        raise ValueError(f'unknown Runge-Kutta pair {tableau!r}')
+    return _native.FLAG_DOPRI5 if tableau == 'dopri5' else 0
+
+
 def tsit5_plan(
     H: np.ndarray,
     Ls,
@@ -393,12 +399,14 @@ def tsit5_plan(
     factormin: float = 0.2,
     factormax: float = 5.0,
     max_steps: int = 100_000,
+    tableau: str = 'tsit5',
     exp_ops=None,
     variant: int = SWEEP_AUTO,
     graph_steps: int = 16,
 ) -> DiaTsit5Plan:
     """Pack a batch of Tsit5 solves that share H and Ls (dense FP64 host arrays whose
-    nonzeros lie on few diagonals) and differ in rho0 (B, n, n)."""
+    nonzeros lie on few diagonals) and differ in rho0 (B, n, n). ``tableau='dopri5'``
+    integrates with the Dopri5 pair instead."""
     import torch
 
     dev = _device()
@@ -423,6 +431,7 @@ def tsit5_plan(
     prob.factormin, prob.factormax = float(factormin), float(factormax)
     prob.t0 = float(tsave[0] if t0 is None else t0)
     prob.t1 = float(tsave[-1])
+    prob.flags = _tableau_flag(tableau)
     _fill_ops(prob.ops, [_dia_of(Hnh)] + [_dia_of(L) for L in Ls], n_pad, arena)
     r64 = np.zeros((B, n_pad, n_pad), dtype=np.complex128)
     r64[:, :n, :n] = rho0

@@ -19,7 +19,7 @@ import numpy as np
 
 from .._checks import check_hermitian, check_qarray_is_dense, check_shape, check_times
 from ..gradient import Gradient
-from ..method import Method, Rouchon1, Tsit5, _DEAdaptiveStep
+from ..method import Dopri5, Method, Rouchon1, Tsit5, _DEAdaptiveStep
 from ..options import Options, check_options
 from ..qarrays import QArray, asqarray
 from ..result import AdaptiveStepInfos, FixedStepInfos, MESolveResult
@@ -28,7 +28,8 @@ from . import dense
 from . import dia as dia_path
 from .operators import decompose
 
-SUPPORTED_METHODS = (Tsit5, Rouchon1)
+SUPPORTED_METHODS = (Tsit5, Dopri5, Rouchon1)
+ADAPTIVE = (Tsit5, Dopri5)
 
 
 def assert_method_supported(method: Method, supported_methods) -> None:
@@ -83,7 +84,7 @@ def dia_route(method, H: TimeQArray, Ls, n: int):
     Rouchon1 with normalisation, constant un-batched DIA operators, n >= DIA_MIN_N."""
     if n < DIA_MIN_N:
         return None
-    if not (isinstance(method, Tsit5) or (isinstance(method, Rouchon1) and method.normalize)):
+    if not (isinstance(method, ADAPTIVE) or (isinstance(method, Rouchon1) and method.normalize)):
         return None
     Ht = decompose(H)
     Lts = [decompose(L) for L in Ls]
@@ -172,10 +173,10 @@ def mesolve(
         return _mesolve_dia(route, rho0, tsave, exp_ops, method, gradient, options)
 
     out_shape, insts = enumerate_instances(H, Ls, rho0, options.cartesian_batching)
-    if options.assume_hermitian and isinstance(method, Tsit5):
+    if options.assume_hermitian and isinstance(method, ADAPTIVE):
         for inst in insts:
             check_hermitian(inst.rho0, 'y0')
-    parts, pindex = hermitian_parts([inst.rho0 for inst in insts], options.assume_hermitian and isinstance(method, Tsit5))
+    parts, pindex = hermitian_parts([inst.rho0 for inst in insts], options.assume_hermitian and isinstance(method, ADAPTIVE))
     if len(parts) > len(insts):
         solve = []
         for inst, (ia, ib) in zip(insts, pindex):
@@ -190,7 +191,7 @@ def mesolve(
     # save_extra sees the state at every save time, whatever save_states is
     save_all = options.save_states or options.save_extra is not None
 
-    if isinstance(method, Tsit5):
+    if isinstance(method, ADAPTIVE):
         plan = dense.build_plan(
             'tsit5',
             insts,
@@ -204,6 +205,7 @@ def mesolve(
             factormax=method.max_factor,
             max_steps=method.max_steps,
             save_states=save_all,
+            tableau='dopri5' if isinstance(method, Dopri5) else 'tsit5',
         )
     else:
         plan = dense.build_plan(
@@ -240,7 +242,7 @@ def mesolve(
         x = np.asarray(x)[pindex[:, 0]].reshape(out_shape)
         return x if out_shape else x[()]
 
-    if isinstance(method, Tsit5):
+    if isinstance(method, ADAPTIVE):
         infos = AdaptiveStepInfos(shaped(res['nsteps']), shaped(res['naccepted']), shaped(res['nrejected']))
     else:
         infos = FixedStepInfos(shaped(res['nsteps']))
@@ -345,12 +347,12 @@ def _mesolve_dia(route, rho0: QArray, tsave, exp_ops, method, gradient, options)
         r = r @ np.conj(np.swapaxes(r, -1, -2))
     Es = [E.to_numpy() for E in exp_ops] if exp_ops is not None else None
     t0 = None if options.t0 is None else float(options.t0)
-    if isinstance(method, Tsit5) and options.assume_hermitian:
+    if isinstance(method, ADAPTIVE) and options.assume_hermitian:
         for x in r:
             check_hermitian(x, 'y0')
-    parts, pindex = hermitian_parts(r, options.assume_hermitian and isinstance(method, Tsit5))
+    parts, pindex = hermitian_parts(r, options.assume_hermitian and isinstance(method, ADAPTIVE))
     r = np.stack(parts)
-    if isinstance(method, Tsit5):
+    if isinstance(method, ADAPTIVE):
         plan = dia_path.tsit5_plan(
             Hd,
             Ld,
@@ -364,6 +366,7 @@ def _mesolve_dia(route, rho0: QArray, tsave, exp_ops, method, gradient, options)
             factormax=method.max_factor,
             max_steps=method.max_steps,
             exp_ops=Es,
+            tableau='dopri5' if isinstance(method, Dopri5) else 'tsit5',
         )
     else:
         plan = dia_path.build_plan(Hd, Ld, r, tsave, method.dt, t0=t0, exp_ops=Es)
@@ -390,7 +393,7 @@ def _mesolve_dia(route, rho0: QArray, tsave, exp_ops, method, gradient, options)
         x = np.asarray(x)[pindex[:, 0]].reshape(out_shape)
         return x if out_shape else x[()]
 
-    if isinstance(method, Tsit5):
+    if isinstance(method, ADAPTIVE):
         infos = AdaptiveStepInfos(shaped(res['nsteps']), shaped(res['naccepted']), shaped(res['nrejected']))
     else:
         infos = FixedStepInfos(shaped(np.full(len(pindex), plan.nsteps, dtype=np.int64)))

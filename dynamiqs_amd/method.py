@@ -70,6 +70,9 @@ class Tsit5(_DEAdaptiveStep):
 
 
 class Dopri5(_DEAdaptiveStep):
+    """Dormand-Prince 5(4) adaptive ODE method (FSAL, PID controller): the Tsit5
+    kernels with the Dopri5 tableau and dense output (``DQH_FLAG_DOPRI5``)."""
+
     SUPPORTED_GRADIENT = (Direct, BackwardCheckpointed, Forward)
 
 

@@ -80,6 +80,7 @@ typedef struct dqh_me_inst {
 #define DQH_FLAG_SAVE_STATES 1
 #define DQH_FLAG_HAS_HNH 2
 #define DQH_FLAG_GLOBAL 4 /* run the global-memory kernel form even where the LDS-resident one applies */
+#define DQH_FLAG_DOPRI5 8 /* Tsit5 kernels: integrate with the Dopri5 pair (same controller and saving) */
 
 /* Size checks for FFI mirrors. */
 int64_t dqh_sizeof_me_inst(void);
@@ -225,7 +226,7 @@ typedef struct dqh_dia_tsit5 {
   int64_t variant;      /* DQH_SWEEP_* */
   int64_t graph_steps;  /* steps per captured hipGraph (even); 0: plain launches */
   double rtol, atol, safety, factormin, factormax, t0, t1;
-  double pad_;
+  int64_t flags;        /* DQH_FLAG_DOPRI5 selects the Dopri5 pair */
   dqh_dia_ops ops;      /* diags[0] = Hnh = -iH - 1/2 sum L^dag L, diags[k] = L_k */
   uint64_t y;           /* device double2[ninst][n][n]: rho0 on entry, final state on exit */
   uint64_t work;        /* device scratch of dqh_dia_tsit5_work_bytes(n, ninst) bytes */

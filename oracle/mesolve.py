@@ -107,7 +107,7 @@ def mesolve(
     def save(y):
         return (y.copy() if save_states else None, np.array([tracemm(E, y) for E in Es], dtype=np.complex128))
 
-    if method == 'tsit5':
+    if method in ('tsit5', 'dopri5'):
         if form == 'hermitian':
             rhs = lindblad.rhs_hermitian
         elif form == 'standard':
@@ -134,6 +134,7 @@ def mesolve(
             max_steps=max_steps,
             jump_ts=jumps,
             save=save,
+            tab=tsit5.tableau(method),
         )
         infos = dict(nsteps=st.nsteps, naccepted=st.naccepted, nrejected=st.nrejected, nrhs=st.nrhs)
     elif method == 'rouchon1':

@@ -82,9 +82,39 @@ SNAP = 1e-12
 ORDER = 5  # exponent 1/ORDER in the controller and in the initial step
 
 
-def interp_weights(theta: float) -> np.ndarray:
+@dataclass(frozen=True)
+class Tableau:
+    """A 7-stage FSAL pair of order 5(4): stage coefficients ``A`` (the last row
+    holds the solution weights, ``b_7 = 0``), error weights ``E = b - bhat`` and
+    continuous-extension weights ``R`` (``b_i(theta) = sum_p R[i, p] theta^(p+1)``)."""
+
+    C: np.ndarray
+    A: np.ndarray
+    E: np.ndarray
+    R: np.ndarray
+
+
+TSIT5 = Tableau(C, A, E, R)
+
+
+def _dopri5() -> Tableau:
+    from . import dopri5
+
+    return Tableau(dopri5.C, dopri5.A, dopri5.E, dopri5.R)
+
+
+def tableau(name: str) -> Tableau:
+    """``'tsit5'`` or ``'dopri5'`` (the reference's ``Tsit5`` and ``Dopri5``)."""
+    if name == 'tsit5':
+        return TSIT5
+    if name == 'dopri5':
+        return _dopri5()
+    raise ValueError(name)
+
+
+def interp_weights(theta: float, tab: Tableau = TSIT5) -> np.ndarray:
     th = np.array([theta, theta**2, theta**3, theta**4])
-    return R @ th
+    return tab.R @ th
 
 
 class MaxStepsError(RuntimeError):
@@ -140,8 +170,10 @@ def integrate(
     max_steps: int = 100_000,
     jump_ts=None,
     save=None,
+    tab: Tableau = TSIT5,
 ):
-    """Integrate ``dy/dt = f(t, y, tseg)`` from ``t0`` to ``tsave[-1]``.
+    """Integrate ``dy/dt = f(t, y, tseg)`` from ``t0`` to ``tsave[-1]`` with the
+    pair ``tab`` (Tsit5 by default, or ``tableau('dopri5')``).
 
     ``f(t, y, tseg)`` evaluates the vector field at time ``t``; ``tseg`` is the
     start of the current step, used by piecewise-constant generators to pick the
@@ -191,11 +223,11 @@ def integrate(
 
         ks = [h * k1]
         for s in range(1, 7):
-            ys = y + sum(A[s, j] * ks[j] for j in range(s))
-            ks.append(h * f(t + C[s] * h, ys, t))
+            ys = y + sum(tab.A[s, j] * ks[j] for j in range(s))
+            ks.append(h * f(t + tab.C[s] * h, ys, t))
             stats.nrhs += 1
-        y_new = y + sum(B[j] * ks[j] for j in range(6))
-        err = sum(E[j] * ks[j] for j in range(7))
+        y_new = y + sum(tab.A[6, j] * ks[j] for j in range(6))
+        err = sum(tab.E[j] * ks[j] for j in range(7))
         scale = atol + rtol * np.maximum(np.abs(y), np.abs(y_new))
         norm = rms_norm(err / scale)
         keep = norm < 1.0
@@ -211,7 +243,7 @@ def integrate(
             stats.ends.append(t_new)
             while isave < len(tsave) and tsave[isave] <= t_new:
                 theta = (tsave[isave] - t) / h
-                w = interp_weights(theta)
+                w = interp_weights(theta, tab)
                 saved.append(save(y + sum(w[j] * ks[j] for j in range(7))))
                 isave += 1
             t = t_new
@@ -227,7 +259,8 @@ def integrate(
     return saved, y, stats
 
 
-def integrate_steps(f, y0: np.ndarray, tsave: np.ndarray, step_ends, *, t0: float | None = None, save=None):
+def integrate_steps(f, y0: np.ndarray, tsave: np.ndarray, step_ends, *, t0: float | None = None, save=None,
+                    tab: Tableau = TSIT5):
     """Tsit5 over a prescribed sequence of accepted steps (``step_ends``: the end
     time of each step, as an adaptive solve recorded them): the same stages,
     solution weights and continuous extension as ``integrate`` without the
@@ -246,13 +279,13 @@ def integrate_steps(f, y0: np.ndarray, tsave: np.ndarray, step_ends, *, t0: floa
         h = t_new - t
         ks = [h * f(t, y, t)]
         for s in range(1, 7):
-            ys = y + sum(A[s, j] * ks[j] for j in range(s))
-            ks.append(h * f(t + C[s] * h, ys, t))
+            ys = y + sum(tab.A[s, j] * ks[j] for j in range(s))
+            ks.append(h * f(t + tab.C[s] * h, ys, t))
         while isave < len(tsave) and tsave[isave] <= t_new:
-            w = interp_weights((tsave[isave] - t) / h)
+            w = interp_weights((tsave[isave] - t) / h, tab)
             saved.append(save(y + sum(w[j] * ks[j] for j in range(7))))
             isave += 1
-        y = y + sum(B[j] * ks[j] for j in range(6))
+        y = y + sum(tab.A[6, j] * ks[j] for j in range(6))
         t = t_new
     while isave < len(tsave):
         saved.append(save(y))

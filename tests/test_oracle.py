@@ -234,6 +234,90 @@ def test_tsit5_fifth_order_convergence():
     assert 24 < r < 40
 
 
+# --------------------------------------------------------------------- Dopri5
+
+
+def test_dopri5_tableau_order_conditions():
+    from oracle import dopri5
+
+    A, b, c, bhat = dopri5.A, dopri5.B, dopri5.C, dopri5.BHAT
+    np.testing.assert_allclose(A.sum(axis=1), c, atol=1e-14)
+    for key, (val, want) in _order_conditions(b, A, c).items():
+        assert abs(val - want) < 1e-12, key
+    for key, (val, want) in _order_conditions(bhat, A, c).items():
+        if key[0] <= 4:
+            assert abs(val - want) < 1e-12, key
+    assert any(abs(v - w) > 1e-6 for k, (v, w) in _order_conditions(bhat, A, c).items() if k[0] == 5)
+    np.testing.assert_allclose(dopri5.E, b - bhat, atol=1e-16)
+    assert b[6] == 0.0  # FSAL: the last stage is f(y1)
+
+
+def test_dopri5_midpoint_and_interpolant_conditions():
+    """Shampine's midpoint weights are a 4th-order value at theta = 1/2; the quartic
+    dense output is of order 4 at every theta, passes through y1 at theta = 1, has
+    the end derivatives f_1 and f_7, and reproduces the midpoint value."""
+    from oracle import dopri5
+
+    A, c = dopri5.A, dopri5.C
+    Ac = A @ c
+
+    def conds(w, theta):
+        return [(w.sum(), theta), (w @ c, theta**2 / 2), (w @ c**2, theta**3 / 3), (w @ Ac, theta**3 / 6),
+                (w @ c**3, theta**4 / 4), (w @ (c * Ac), theta**4 / 8), (w @ (A @ c**2), theta**4 / 12),
+                (w @ (A @ Ac), theta**4 / 24)]
+
+    for val, want in conds(dopri5.CMID, 0.5):
+        assert abs(val - want) < 1e-12
+    tab = tsit5.tableau('dopri5')
+    for theta in (0.1, 0.37, 0.5, 0.8, 1.0):
+        w = tsit5.interp_weights(theta, tab)
+        for val, want in conds(w, theta):
+            assert abs(val - want) < 1e-12
+    np.testing.assert_allclose(tsit5.interp_weights(1.0, tab), dopri5.B, atol=1e-13)
+    np.testing.assert_allclose(tsit5.interp_weights(0.5, tab), dopri5.CMID, atol=1e-13)
+    # derivative of b_i(theta) at 0 and 1: e_1 and e_7
+    dR = dopri5.R @ np.array([1.0, 2.0, 3.0, 4.0])
+    np.testing.assert_allclose(dopri5.R[:, 0], np.eye(7)[0], atol=1e-15)
+    np.testing.assert_allclose(dR, np.eye(7)[6], atol=1e-12)
+    # the theta^4 column is the dense-output constants of DOPRI5 (Hairer's code)
+    d = [-12715105075 / 11282082432, 0.0, 87487479700 / 32700410799, -10690763975 / 1880347072,
+         701980252875 / 199316789632, -1453857185 / 822651844, 69997945 / 29380423]
+    np.testing.assert_allclose(dopri5.R[:, 3], d, rtol=1e-12)
+
+
+def test_dopri5_fifth_order_convergence_and_closed_form():
+    from oracle import dopri5
+
+    lam = -1.0 + 3.0j
+
+    def run(h):
+        y = 1.0 + 0j
+        for _ in range(int(round(1.0 / h))):
+            ks = [h * lam * y]
+            for s in range(1, 7):
+                ks.append(h * lam * (y + sum(dopri5.A[s, j] * ks[j] for j in range(s))))
+            y = y + sum(dopri5.B[j] * ks[j] for j in range(6))
+        return abs(y - np.exp(lam))
+
+    r = run(0.05) / run(0.025)
+    assert 24 < r < 40
+    # adaptive Dopri5 against the exact superoperator solution
+    rng = np.random.default_rng(5)
+    H, Ls, rho0 = _rand_problem(rng, 6, 2)
+    H /= np.linalg.norm(H, 2)
+    Ls = [0.3 * L / np.linalg.norm(L, 2) for L in Ls]
+    tsave = np.linspace(0, 3, 7)
+    res = om.mesolve(H, Ls, rho0, tsave, rtol=1e-10, atol=1e-12, method='dopri5')
+    Lsup = lindblad.slindbladian(H, Ls)
+    for i, t in enumerate(tsave):
+        ex = lindblad.unvectorize(sla.expm(t * Lsup) @ lindblad.vectorize(rho0))
+        assert np.linalg.norm(res['states'][i] - ex) < 1e-8
+    # it is a different method: the step sequences differ from Tsit5's
+    r5 = om.mesolve(H, Ls, rho0, tsave, rtol=1e-6, atol=1e-6, method='dopri5')
+    rt = om.mesolve(H, Ls, rho0, tsave, rtol=1e-6, atol=1e-6)
+    assert r5['infos']['nsteps'] != rt['infos']['nsteps'] or not np.allclose(r5['states'], rt['states'], atol=1e-12)
+
+
 def _ocavity(n, delta, kappa, alpha0):
     a = op.destroy(n)
     return delta * a.conj().T @ a, [np.sqrt(kappa) * a], op.coherent(n, alpha0)
