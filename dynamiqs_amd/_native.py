@@ -50,7 +50,14 @@ ME_INST_FIELDS = [
     ('stats', np.uint64),
     ('step_log', np.uint64),
     ('step_log_cap', np.int64),
+    ('nmod', np.int64),
+    ('Hmod', np.uint64),
+    ('mod_coef', np.uint64),
+    ('mod_t0', np.float64),
+    ('mod_dt', np.float64),
+    ('mod_nint', np.int64),
 ]
+MAX_MOD = 4
 ME_INST_DTYPE = np.dtype(ME_INST_FIELDS)
 
 FLAG_SAVE_STATES = 1

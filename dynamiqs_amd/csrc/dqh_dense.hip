@@ -66,8 +66,9 @@ __device__ __forceinline__ void make_view(View &v, const dqh_me_inst &p) {
   v.ys = w + 7 * v.nn;
   v.tmp = w + 8 * v.nn;
   v.X = w + 9 * v.nn;
-  v.Hcur = v.npwc > 0 ? w + (9 + v.nL) * v.nn : nullptr;
-  v.Hnh = v.npwc > 0 ? v.Hcur : gp<const float2>(p.Hnh);
+  const bool tdep = v.npwc > 0 || p.nmod > 0;
+  v.Hcur = tdep ? w + (9 + v.nL) * v.nn : nullptr;
+  v.Hnh = tdep ? v.Hcur : gp<const float2>(p.Hnh);
   v.D = gp<const float2>(p.Dops);
   v.C = gp<const float2>(p.Cops);
   double2 *y = gp<double2>(p.y);
@@ -126,13 +127,42 @@ __device__ __forceinline__ void wg_round_state(const View &v, const double2 *y) 
   __syncthreads();
 }
 
-// Hcur <- Hnh0 + sum_j c_j Hpwc_j for the segment containing t (zero outside)
+// f_j(tau) of the smooth terms: the piecewise cubic of the interval that contains
+// tau (clamped to the table), Horner in x = tau - t_i
+__device__ __forceinline__ void mod_coeffs(const dqh_me_inst &p, double tau, double2 (&c)[DQH_MAX_MOD]) {
+  const int nmod = (int)p.nmod;
+  const int64_t nint = p.mod_nint;
+  int64_t i = (int64_t)floor((tau - p.mod_t0) / p.mod_dt);
+  i = i < 0 ? 0 : (i >= nint ? nint - 1 : i);
+  const double x = tau - (p.mod_t0 + (double)i * p.mod_dt);
+  const double2 *tab = gp<const double2>(p.mod_coef);
+#pragma unroll
+  for (int j = 0; j < DQH_MAX_MOD; ++j) {
+    c[j] = make_double2(0.0, 0.0);
+    if (j < nmod) {
+      const double2 *q = tab + ((size_t)j * nint + i) * 4;
+      double re = q[0].x, im = q[0].y;
+#pragma unroll
+      for (int m = 1; m < 4; ++m) {
+        re = re * x + q[m].x;
+        im = im * x + q[m].y;
+      }
+      c[j] = make_double2(re, im);
+    }
+  }
+}
+
+// Hcur <- Hnh0 + sum_j c_j Hpwc_j for PWC segment seg (zero outside) + sum_j f_j(tau) Hmod_j
 template <int NW>
-__device__ __forceinline__ void wg_update_pwc(const View &v, const dqh_me_inst &p, int seg) {
+__device__ __forceinline__ void wg_update_pwc(const View &v, const dqh_me_inst &p, int seg, double tau) {
   const float2 *H0 = gp<const float2>(p.Hnh);
   const float2 *Hp = gp<const float2>(p.Hpwc);
+  const float2 *Hm = gp<const float2>(p.Hmod);
   const double *vals = gp<const double>(p.pwc_vals);
   const int nv = v.nseg - 1;
+  const int nmod = (int)p.nmod;
+  double2 fm[DQH_MAX_MOD];
+  mod_coeffs(p, tau, fm);
   for (size_t e = threadIdx.x; e < v.nn; e += NW * 64) {
     double re = H0[e].x, im = H0[e].y;
     if (seg >= 0 && seg < nv) {
@@ -141,6 +171,14 @@ __device__ __forceinline__ void wg_update_pwc(const View &v, const dqh_me_inst &
         const float2 o = Hp[j * v.nn + e];
         re += cr * o.x - ci * o.y;
         im += cr * o.y + ci * o.x;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < DQH_MAX_MOD; ++j) {
+      if (j < nmod) {
+        const float2 o = Hm[j * v.nn + e];
+        re += fm[j].x * o.x - fm[j].y * o.y;
+        im += fm[j].x * o.y + fm[j].y * o.x;
       }
     }
     v.Hcur[e] = make_float2((float)re, (float)im);
@@ -234,9 +272,9 @@ template <int NW>
 __device__ __forceinline__ void rhs_global(const dqh_me_inst &p) {
   View v;
   make_view(v, p);
-  if (v.npwc > 0) {
-    const int seg = pwc_segment(gp<const double>(p.pwc_times), v.nseg, p.t0);
-    wg_update_pwc<NW>(v, p, seg);
+  if (v.npwc > 0 || p.nmod > 0) {
+    const int seg = v.npwc > 0 ? pwc_segment(gp<const double>(p.pwc_times), v.nseg, p.t0) : -1;
+    wg_update_pwc<NW>(v, p, seg, p.t0);
   }
   wg_round_state<NW>(v, v.y[0]);
   wg_rhs<NW>(v, v.D, v.C, (v.flags & DQH_FLAG_HAS_HNH) != 0, kp(v, 0));
@@ -274,9 +312,10 @@ __device__ __forceinline__ void tsit5_global(const dqh_me_inst &p, double *sh) {
     wg_save<NW>(v, p, v.y[cur], 0.0, w1, 0, isave, sh);
     ++isave;
   }
-  if (v.npwc > 0) {
-    seg = pwc_segment(ptimes, v.nseg, t);
-    wg_update_pwc<NW>(v, p, seg);
+  const bool smooth = p.nmod > 0;  // H changes within a step: updated at every stage time
+  if (v.npwc > 0 || smooth) {
+    seg = v.npwc > 0 ? pwc_segment(ptimes, v.nseg, t) : -1;
+    wg_update_pwc<NW>(v, p, seg, t);
   }
   // k0 = f(y0)
   wg_round_state<NW>(v, v.y[cur]);
@@ -304,6 +343,7 @@ __device__ __forceinline__ void tsit5_global(const dqh_me_inst &p, double *sh) {
       v.ys[e] = make_float2((float)(y0.x + h0 * f0.x), (float)(y0.y + h0 * f0.y));
     }
     __syncthreads();
+    if (smooth) wg_update_pwc<NW>(v, p, seg, t + h0);
     wg_rhs<NW>(v, v.D, v.C, hasH, kp(v, 1));
     ++nrhs;
     double s2 = 0;
@@ -329,11 +369,12 @@ __device__ __forceinline__ void tsit5_global(const dqh_me_inst &p, double *sh) {
       const int s = pwc_segment(ptimes, v.nseg, t);
       if (s != seg) {
         seg = s;
-        wg_update_pwc<NW>(v, p, seg);
+        if (!smooth) wg_update_pwc<NW>(v, p, seg, t);
       }
     }
     if (need_k0) {
       wg_round_state<NW>(v, v.y[cur]);
+      if (smooth) wg_update_pwc<NW>(v, p, seg, t);
       wg_rhs<NW>(v, v.D, v.C, hasH, kp(v, 0));
       ++nrhs;
       need_k0 = false;
@@ -376,6 +417,7 @@ __device__ __forceinline__ void tsit5_global(const dqh_me_inst &p, double *sh) {
         v.ys[e] = make_float2((float)re, (float)im);
       }
       __syncthreads();
+      if (smooth) wg_update_pwc<NW>(v, p, seg, t + rk.C[s] * h);
       wg_rhs<NW>(v, v.D, v.C, hasH, kp(v, s));
       ++nrhs;
     }
@@ -572,7 +614,12 @@ __global__ __launch_bounds__(kThreads) void k_rhs(const dqh_me_inst *__restrict_
 
 __global__ __launch_bounds__(kThreads) void k_rouchon1(const dqh_me_inst *__restrict__ insts) {
   __shared__ double sh[kWaves];
-  rouchon1_global<kWaves>(insts[blockIdx.x], sh);
+  const dqh_me_inst &p = insts[blockIdx.x];
+  if (p.nmod > 0) {  // the Kraus sets are per step length and PWC segment only
+    if (threadIdx.x == 0) gp<int64_t>(p.stats)[3] = DQH_UNSUPPORTED;
+    return;
+  }
+  rouchon1_global<kWaves>(p, sh);
 }
 
 }  // namespace dqh

@@ -509,9 +509,9 @@ extern "C" int dqh_mesolve_tsit5_dense_grid(const dqh_me_inst *d_insts, const dq
     const dqh_me_inst &p = h_insts[b];
     if (p.np != np || p.np <= 0 || p.np % 16 != 0 || p.np > 1024 || p.n <= 0 || p.n > p.np)
       return fail(DQH_BAD_SHAPE, "dqh_mesolve_tsit5_dense_grid: every instance needs the same np (multiple of 16)");
-    if (p.npwc != 0 || p.nL < 0 || p.nL > 8 || p.ntsave < 1 || !p.work || !p.y || !p.tsave || !p.stats ||
+    if (p.npwc != 0 || p.nmod != 0 || p.nL < 0 || p.nL > 8 || p.ntsave < 1 || !p.work || !p.y || !p.tsave || !p.stats ||
         !p.states || !p.final_state || (p.nL > 0 && (!p.Dops || !p.Cops)))
-      return fail(DQH_BAD_ARG, "dqh_mesolve_tsit5_dense_grid: constant generators only (npwc == 0), nL <= 8");
+      return fail(DQH_BAD_ARG, "dqh_mesolve_tsit5_dense_grid: constant generators only (npwc == nmod == 0), nL <= 8");
     if (p.nE > 0 && (!(p.flags & DQH_FLAG_SAVE_STATES) || !p.EopsT || !p.expects))
       return fail(DQH_BAD_ARG, "dqh_mesolve_tsit5_dense_grid: expectation values need saved states");
     if (!(p.t1 >= p.t0) || !(p.rtol + p.atol > 0))

@@ -306,19 +306,23 @@ __device__ __forceinline__ void res_write_ys_upper(float2 *lds, const int (&ij)[
 }
 
 __device__ __forceinline__ const float2 *res_hnh(const dqh_me_inst &p) {
-  return p.npwc > 0 ? gp<const float2>(p.work) : gp<const float2>(p.Hnh);
+  return (p.npwc > 0 || p.nmod > 0) ? gp<const float2>(p.work) : gp<const float2>(p.Hnh);
 }
 
-// Hcur (first np*np of work) <- Hnh0 + sum_j c_j Hpwc_j for segment seg; global memory,
-// published to the workgroup by the barrier at the start of the next res_rhs.
+// Hcur (first np*np of work) <- Hnh0 + sum_j c_j Hpwc_j for PWC segment seg
+// + sum_j f_j(tau) Hmod_j; global memory, published to the workgroup by the
+// barrier at the start of the next res_rhs.
 template <int NP, int NW>
-__device__ __forceinline__ void res_update_pwc(const dqh_me_inst &p, int seg) {
+__device__ __forceinline__ void res_update_pwc(const dqh_me_inst &p, int seg, double tau) {
   using R = RCfg<NP, NW>;
   const float2 *H0 = gp<const float2>(p.Hnh);
   const float2 *Hp = gp<const float2>(p.Hpwc);
+  const float2 *Hm = gp<const float2>(p.Hmod);
   const double *vals = gp<const double>(p.pwc_vals);
   float2 *Hc = gp<float2>(p.work);
-  const int nv = (int)p.nseg - 1, npwc = (int)p.npwc;
+  const int nv = (int)p.nseg - 1, npwc = (int)p.npwc, nmod = (int)p.nmod;
+  double2 fm[DQH_MAX_MOD];
+  mod_coeffs(p, tau, fm);
   __syncthreads();  // no wave still reads the previous Hcur
   for (int e = threadIdx.x; e < R::NN; e += R::NTH) {
     double re = H0[e].x, im = H0[e].y;
@@ -328,6 +332,14 @@ __device__ __forceinline__ void res_update_pwc(const dqh_me_inst &p, int seg) {
         const float2 o = Hp[(size_t)j * R::NN + e];
         re += cr * o.x - ci * o.y;
         im += cr * o.y + ci * o.x;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < DQH_MAX_MOD; ++j) {
+      if (j < nmod) {
+        const float2 o = Hm[(size_t)j * R::NN + e];
+        re += fm[j].x * o.x - fm[j].y * o.y;
+        im += fm[j].x * o.y + fm[j].y * o.x;
       }
     }
     Hc[e] = make_float2((float)re, (float)im);
@@ -402,8 +414,9 @@ __device__ __forceinline__ double wave_uniform(double x) {
 template <int NP, int NW>
 __device__ __forceinline__ void rhs_resident(const dqh_me_inst &p, float2 *lds) {
   using R = RCfg<NP, NW>;
-  if (p.npwc > 0) {
-    res_update_pwc<NP, NW>(p, pwc_segment(gp<const double>(p.pwc_times), (int)p.nseg, p.t0));
+  if (p.npwc > 0 || p.nmod > 0) {
+    const int seg = p.npwc > 0 ? pwc_segment(gp<const double>(p.pwc_times), (int)p.nseg, p.t0) : -1;
+    res_update_pwc<NP, NW>(p, seg, p.t0);
   }
   const double2 *yg = gp<const double2>(p.y);
   float2 v[R::NE];
@@ -480,9 +493,10 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
     res_save<NP, NW>(p, yg, ij, k, 0.0, w0, false, isave, sh);
     ++isave;
   }
-  if (pwc) {
-    seg = pwc_segment(ptimes, nseg, t);
-    res_update_pwc<NP, NW>(p, seg);
+  const bool smooth = p.nmod > 0;  // H changes within a step: updated at every stage time
+  if (pwc || smooth) {
+    seg = pwc ? pwc_segment(ptimes, nseg, t) : -1;
+    res_update_pwc<NP, NW>(p, seg, t);
   }
   // k0 = f(y0)
 #pragma unroll
@@ -517,6 +531,7 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
       const double2 yq = yload(q);
       tmp[q] = make_float2((float)(yq.x + h0 * k[0][q].x), (float)(yq.y + h0 * k[0][q].y));
     }
+    if (smooth) res_update_pwc<NP, NW>(p, seg, t + h0);
     res_write_ys_upper<NP, NW>(lds, ij, tmp);
     res_rhs_upper<NP, NW>(Hnh, D, nL, hasH, lds, ij, k[1]);
     ++nrhs;
@@ -544,7 +559,7 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
       const int s = pwc_segment(ptimes, nseg, t);
       if (s != seg) {
         seg = s;
-        res_update_pwc<NP, NW>(p, seg);
+        if (!smooth) res_update_pwc<NP, NW>(p, seg, t);
       }
     }
     if (need_k0) {
@@ -553,6 +568,7 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
         const double2 yq = yload(q);
         tmp[q] = make_float2((float)yq.x, (float)yq.y);
       }
+      if (smooth) res_update_pwc<NP, NW>(p, seg, t);
       res_write_ys_upper<NP, NW>(lds, ij, tmp);
       res_rhs_upper<NP, NW>(Hnh, D, nL, hasH, lds, ij, k[0]);
       ++nrhs;
@@ -600,6 +616,7 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
         }
         tmp[q] = make_float2((float)re, (float)im);
       }
+      if (smooth) res_update_pwc<NP, NW>(p, seg, t + rk.C[s] * h);
       res_write_ys_upper<NP, NW>(lds, ij, tmp);
       res_rhs_upper<NP, NW>(Hnh, D, nL, hasH, lds, ij, tmp);
       ++nrhs;

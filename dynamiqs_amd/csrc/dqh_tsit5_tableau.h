@@ -9,10 +9,11 @@
 
 namespace dqh {
 
-// 7-stage FSAL pairs of order 5(4): stage coefficients A (the last row holds the
+// 7-stage FSAL pairs of order 5(4): stage times C, stage coefficients A (the last row holds the
 // solution weights b, b_7 = 0), error weights E = b - b_hat and the continuous
 // extension b_i(theta) = sum_p R[i][p] theta^(p+1).
 struct RKTab {
+  double C[7];  // stage times t + C_s h
   double A[7][7];
   double E[7];
   double R[7][4];
@@ -23,7 +24,8 @@ struct RKTab {
 // midpoint value (Math. Comp. 46 (1986) 135-150) written in the same b_i(theta)
 // form; its theta^4 column equals the dense-output constants of the DOPRI5 code.
 static __constant__ RKTab kRK[2] = {
-    {{{0, 0, 0, 0, 0, 0, 0},
+    {{0, 0.161, 0.327, 0.9, 0.9800255409045097, 1, 1},
+     {{0, 0, 0, 0, 0, 0, 0},
       {0.161, 0, 0, 0, 0, 0, 0},
       {-0.008480655492356989, 0.335480655492357, 0, 0, 0, 0, 0},
       {2.897153057105493, -6.359448489975075, 4.3622954328695815, 0, 0, 0, 0},
@@ -39,7 +41,8 @@ static __constant__ RKTab kRK[2] = {
       {0.0, 37.50931341651104, -88.1789048947664, 47.37952196281928},
       {0.0, -27.896526289197286, 65.09189467479366, -34.87065786149661},
       {0.0, 1.5, -4.0, 2.5}}},
-    {{{0, 0, 0, 0, 0, 0, 0},
+    {{0, 0.2, 0.3, 0.8, 0.8888888888888888, 1, 1},
+     {{0, 0, 0, 0, 0, 0, 0},
       {0.2, 0, 0, 0, 0, 0, 0},
       {0.075, 0.225, 0, 0, 0, 0, 0},
       {0.9777777777777777, -3.7333333333333334, 3.5555555555555554, 0, 0, 0, 0},

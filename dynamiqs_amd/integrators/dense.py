@@ -16,7 +16,7 @@ import numpy as np
 import scipy.linalg as sla
 
 from .. import _native
-from .operators import resample_pwc
+from .operators import resample_pwc, spline_table
 
 MAX_NP = 256
 
@@ -52,6 +52,7 @@ class Instance:
     Hnh: np.ndarray | None = None  # explicit constant generator part (overrides -iH - 1/2 sum L^dag L)
     tsave: np.ndarray | None = None  # per-instance save times (all instances of a plan: same length)
     t0: float | None = None  # per-instance start time
+    mod: list = field(default_factory=list)  # [(g, op (n, n), key)]: + g(t) op, g scalar; equal keys share tables
 
 
 class _Arena:
@@ -112,7 +113,8 @@ class _Arena:
         return t
 
 
-ARENA_FIELDS = ('tsave', 'Hnh', 'Dops', 'Cops', 'EopsT', 'Hpwc', 'pwc_times', 'pwc_vals', 'step_t', 'step_set')
+ARENA_FIELDS = ('tsave', 'Hnh', 'Dops', 'Cops', 'EopsT', 'Hpwc', 'pwc_times', 'pwc_vals', 'step_t', 'step_set',
+                'Hmod', 'mod_coef')
 POINTER_FIELDS = ARENA_FIELDS + ('y', 'work', 'states', 'expects', 'final_state', 'stats', 'step_log')
 
 
@@ -137,6 +139,8 @@ def check_descriptors(descs: np.ndarray, tensors) -> None:
         'expects': np.maximum(nE, one) * nt * np.uint64(8),
         'stats': np.full(len(descs), 64, dtype=np.uint64),
         'step_log': descs['step_log_cap'].astype(np.uint64) * np.uint64(8),
+        'Hmod': descs['nmod'].astype(np.uint64) * npd * npd * np.uint64(8),
+        'mod_coef': descs['nmod'].astype(np.uint64) * descs['mod_nint'].astype(np.uint64) * np.uint64(64),
     }
     for f in POINTER_FIELDS:
         v = descs[f].astype(np.uint64)
@@ -316,6 +320,11 @@ def build_plan(
 
     if tableau not in ('tsit5', 'dopri5'):
         raise ValueError(f'unknown Runge-Kutta pair {tableau!r}')
+    if any(inst.mod for inst in instances):
+        if kind == 'rouchon1':
+            raise NotImplementedError('modulated operators are not supported with Rouchon1 on the MI355X solver')
+        if max(len(inst.mod) for inst in instances) > _native.MAX_MOD:
+            raise NotImplementedError(f'at most {_native.MAX_MOD} modulated terms per operator are supported')
 
     dev = _device()
     _native.lib()
@@ -386,6 +395,23 @@ def build_plan(
         return cache[key]
 
     npwc_max = 0
+    nmod_max = max((len(inst.mod) for inst in instances), default=0)
+    t_stop = float(tsave[-1])
+    if instances[0].tsave is not None:
+        t_stop = max(float(inst.tsave[-1]) for inst in instances)
+    t_tab = min([t_start] + [float(inst.t0) for inst in instances if inst.t0 is not None])
+    mod_tables = {}
+
+    def mod_table(g, key):
+        # one table per (function, batch element), shared by the instances that use it
+        k = (key, t_tab, t_stop) if key is not None else None
+        if k is None or k not in mod_tables:
+            tab = spline_table(g, t_tab, t_stop)
+            if k is None:
+                return tab
+            mod_tables[k] = tab
+        return mod_tables[k]
+
     if kind == 'rouchon1':
         if dt is None:
             raise ValueError('Rouchon1 needs a fixed step dt')
@@ -446,6 +472,17 @@ def build_plan(
                 )
                 d['pwc_times'] = shared(('ptimes', inst.key_H), lambda grid=grid: grid)
                 d['pwc_vals'] = arena.put(np.ascontiguousarray(vals).view(np.float64))
+            if inst.mod:
+                tabs = [mod_table(g, key) for (g, _, key) in inst.mod]
+                nint = max(len(c) for (_, _, c) in tabs)
+                tabs = [spline_table(g, t_tab, t_stop, n0=nint, nmax=nint) if len(c) != nint else (a, dtt, c)
+                        for (g, _, _), (a, dtt, c) in zip(inst.mod, tabs)]
+                d['nmod'] = len(inst.mod)
+                d['mod_t0'], d['mod_dt'], d['mod_nint'] = tabs[0][0], tabs[0][1], nint
+                d['mod_coef'] = arena.put(np.ascontiguousarray(np.stack([c for (_, _, c) in tabs])).view(np.float64))
+                d['Hmod'] = shared(
+                    ('Hmod', inst.key_H), lambda inst=inst: _pad(np.stack([-1j * o for (_, o, _) in inst.mod]), npad).astype(np.complex64)
+                )
         else:  # rouchon1: Kraus sets per (PWC segment, step length)
             d['nL'] = nLdev
             grid = None
@@ -483,7 +520,7 @@ def build_plan(
         descs[f] = [arena.patch(int(v)) for v in descs[f]]
 
     # ---- state, scratch, outputs
-    welems = _native.work_elems(npad, nLdev, npwc_max)
+    welems = _native.work_elems(npad, nLdev, npwc_max + nmod_max)
     work = arena.empty(B * welems * 8)
     y = torch.zeros((B, 2, nn, 2), dtype=torch.float64, device=dev)
     y0 = np.zeros((B, npad, npad), dtype=np.complex128)
@@ -529,7 +566,7 @@ def build_plan(
         and not np.any(descs['flags'] & _native.FLAG_GLOBAL)
     ):
         plan.uniform_np = npad
-    if kind == 'tsit5' and use_grid(grid_tsit5, n, B, npwc_max, nE, save_states):
+    if kind == 'tsit5' and use_grid(grid_tsit5, n, B, npwc_max + nmod_max, nE, save_states):
         lib = _native.lib()
         nbytes = int(lib.dqh_tsit5_grid_scratch_bytes(npad, B))
         scratch = torch.empty((nbytes,), dtype=torch.uint8, device=dev)

@@ -98,7 +98,7 @@ def enumerate_instances(H: TimeQArray, Ls, rho0: QArray, cartesian: bool):
     Ht = decompose(H)
     Lts = [decompose(L) for L in Ls]
     for i, Lt in enumerate(Lts):
-        if Lt.pwc:
+        if Lt.pwc or Lt.mod:
             raise NotImplementedError('time-dependent jump operators are not supported by the MI355X solver')
     r = rho0.to_numpy().astype(np.complex128)
     rb = r.shape[:-2]
@@ -126,6 +126,7 @@ def enumerate_instances(H: TimeQArray, Ls, rho0: QArray, cartesian: bool):
                 Ls=[Lt.const_at(i) for Lt, i in zip(Lts, iLs)],
                 rho0=rho,
                 pwc=Ht.pwc_at(iH),
+                mod=[(g, o, (id(f), iH if np.ndim(f(0.0)) else None)) for (g, o, f) in Ht.mod_at(iH)],
                 key_H=iH,
                 key_L=tuple(iLs),
                 key_rho=ir,

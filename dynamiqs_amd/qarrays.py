@@ -261,6 +261,8 @@ class QArray:
         return QArray(fn(self.to_numpy(), _as_numpy(other)), dims=self.dims)
 
     def __mul__(self, y) -> QArray:
+        if getattr(y, '_is_timeqarray', False):
+            return NotImplemented
         if isinstance(y, QArray):
             if self._data is None and y._data is None:
                 return _dia_elmul(self, y)
@@ -280,6 +282,8 @@ class QArray:
         return self * -1
 
     def __add__(self, y) -> QArray:
+        if getattr(y, '_is_timeqarray', False):
+            return NotImplemented
         if isinstance(y, QArray):
             if self._data is None and y._data is None:
                 return _dia_add(self, y)
@@ -292,6 +296,8 @@ class QArray:
     __radd__ = __add__
 
     def __sub__(self, y) -> QArray:
+        if getattr(y, '_is_timeqarray', False):
+            return NotImplemented
         return self + (-y if isinstance(y, QArray) else -np.asarray(y))
 
     def __rsub__(self, y) -> QArray:

@@ -16,6 +16,8 @@ from .qarrays import QArray, asqarray, isqarraylike
 
 
 class TimeQArray:
+    _is_timeqarray = True  # QArray arithmetic defers to the time-dependent operand
+
     def __call__(self, t: float) -> QArray:
         raise NotImplementedError
 
@@ -137,8 +139,16 @@ class ModulatedTimeQArray(TimeQArray):
     def __call__(self, t: float) -> QArray:
         return self.qarray * np.asarray(self.f(t))
 
+    _disc_ts = np.zeros(0)
+
+    @property
+    def discontinuity_ts(self) -> np.ndarray:
+        return self._disc_ts
+
     def __mul__(self, y):
-        return ModulatedTimeQArray(self.f, self.qarray * y)
+        out = ModulatedTimeQArray(self.f, self.qarray * y)
+        out._disc_ts = self._disc_ts
+        return out
 
     __rmul__ = __mul__
 
@@ -222,8 +232,14 @@ def pwc(times, values, qarray) -> PWCTimeQArray:
     return PWCTimeQArray(times, values, asqarray(qarray))
 
 
-def modulated(f, qarray) -> ModulatedTimeQArray:
-    return ModulatedTimeQArray(f, asqarray(qarray))
+def modulated(f, qarray, *, discontinuity_ts=None) -> ModulatedTimeQArray:
+    """``O(t) = f(t) O`` (``dynamiqs/time_qarray.py:124-181``)."""
+    if not callable(f):
+        raise TypeError(f'Argument `f` must be a function, but has type {type(f).__name__}.')
+    out = ModulatedTimeQArray(f, asqarray(qarray))
+    if discontinuity_ts is not None:
+        out._disc_ts = np.sort(np.asarray(discontinuity_ts, dtype=float).reshape(-1))
+    return out
 
 
 def timecallable(f) -> CallableTimeQArray:

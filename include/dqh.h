@@ -75,7 +75,19 @@ typedef struct dqh_me_inst {
    * step_log_cap of them (replaying the same steps in another precision) */
   uint64_t step_log;  /* double[step_log_cap]                                        */
   int64_t step_log_cap;
+  /* optional smooth time dependence (Tsit5 kernels; 0 terms: off): H(t) gains
+   * sum_j f_j(t) O_j with f_j a piecewise cubic on the uniform grid
+   * mod_t0 + i mod_dt (i < mod_nint): on [t_i, t_i + dt] f_j = ((c0 x + c1) x + c2) x + c3
+   * with x = t - t_i. Evaluated at every stage time (modulated operators,
+   * time_qarray.py:669-760). At most DQH_MAX_MOD terms. */
+  int64_t nmod;
+  uint64_t Hmod;      /* const c64[nmod*np*np]  -i O_j of each term                  */
+  uint64_t mod_coef;  /* const double2[nmod*mod_nint*4]: c0..c3 per interval        */
+  double mod_t0, mod_dt;
+  int64_t mod_nint;
 } dqh_me_inst;
+
+#define DQH_MAX_MOD 4
 
 #define DQH_FLAG_SAVE_STATES 1
 #define DQH_FLAG_HAS_HNH 2

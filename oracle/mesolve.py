@@ -14,7 +14,8 @@ Restates the single-instance path of ``dynamiqs/integrators/apis/mesolve.py``
   (``diffrax_integrator.py:149-157``).
 
 Time dependence (``time_qarray.py``): an operator is a constant array or a
-:class:`TimeOp` = constant part + sum of piecewise-constant terms
+:class:`TimeOp` = constant part + sum of modulated terms ``f(t) * op`` evaluated at
+the exact stage time + sum of piecewise-constant terms
 ``values[idx] * op`` with ``idx = searchsorted(times, t, 'right') - 1`` and zero
 outside ``[times[0], times[-1])`` (``PWCTimeQArray._prefactor`` :649-659); the
 discontinuity times feed the controller's ``jump_ts`` (``:627-628``,
@@ -46,23 +47,38 @@ class PWCTerm:
 
 
 @dataclass
+class ModTerm:
+    """``f(t) * op`` (``ModulatedTimeQArray``, ``time_qarray.py:669-760``): a smooth
+    term, evaluated at the exact stage time."""
+
+    f: object
+    op: np.ndarray  # (n, n)
+
+    def coeff(self, t: float) -> complex:
+        return complex(self.f(t))
+
+
+@dataclass
 class TimeOp:
     const: np.ndarray
     terms: list = field(default_factory=list)
 
-    def __call__(self, t: float) -> np.ndarray:
+    def __call__(self, t: float, t_stage: float | None = None) -> np.ndarray:
+        """The operator with its piecewise-constant terms at ``t`` (the segment time)
+        and its modulated terms at ``t_stage`` (default ``t``)."""
         out = np.array(self.const, dtype=np.complex128)
         for term in self.terms:
-            c = term.coeff(t)
+            c = term.coeff(t if isinstance(term, PWCTerm) or t_stage is None else t_stage)
             if c != 0:
                 out = out + c * term.op
         return out
 
     @property
     def jump_ts(self) -> np.ndarray:
-        if not self.terms:
+        pwc = [t for t in self.terms if isinstance(t, PWCTerm)]
+        if not pwc:
             return np.zeros(0)
-        return np.unique(np.concatenate([np.asarray(t.times, dtype=np.float64) for t in self.terms]))
+        return np.unique(np.concatenate([np.asarray(t.times, dtype=np.float64) for t in pwc]))
 
 
 def as_timeop(x) -> TimeOp:
@@ -119,7 +135,7 @@ def mesolve(
 
         def f(t, y, tseg):
             ts = _segment_time(t, tseg, jumps)
-            return rhs(H(ts), [L(ts) for L in Ls], y)
+            return rhs(H(ts, t), [L(ts) for L in Ls], y)
 
         saved, ylast, st = tsit5.integrate(
             f,

@@ -71,11 +71,48 @@ def test_empty_jump_ops_with_ket_warns():
             pass  # no GPU here: the warning is raised before the solve
 
 
-def test_modulated_operators_are_reported_unsupported():
+def test_unsupported_time_dependence_is_reported():
+    """What the kernels cannot run raises before any device work: callable
+    operators, modulated operators with discontinuity times, modulated jump
+    operators, modulated Hamiltonians with Rouchon1; a non-callable f is a
+    TypeError (time_qarray.py:161-165)."""
     H, Ls, psi, ts = _args()
-    Hm = dq.modulated(lambda t: np.cos(t), H)
-    with pytest.raises(NotImplementedError, match='ModulatedTimeQArray operators are not supported'):
-        dq.mesolve(Hm, Ls, psi, ts)
+    with pytest.raises(NotImplementedError, match='CallableTimeQArray operators are not supported'):
+        dq.mesolve(dq.timecallable(lambda t: H), Ls, psi, ts)
+    with pytest.raises(NotImplementedError, match='discontinuity_ts'):
+        dq.mesolve(dq.modulated(np.cos, H, discontinuity_ts=[0.5]), Ls, psi, ts)
+    with pytest.raises(NotImplementedError, match='time-dependent jump operators'):
+        dq.mesolve(H, [dq.modulated(np.cos, Ls[0])], psi, ts)
+    with pytest.raises(NotImplementedError, match='Rouchon1'):
+        dq.mesolve(H + dq.modulated(np.cos, H), Ls, psi, ts, method=dq.method.Rouchon1(dt=1e-2))
+    with pytest.raises(TypeError, match=re.escape('Argument `f` must be a function, but has type float.')):
+        dq.modulated(1.0, H)
+
+
+def test_spline_table_accuracy_and_layout():
+    """The piecewise-cubic table of a modulated coefficient: Horner evaluation in
+    the kernels' layout reproduces f to ~1e-12 at arbitrary times, for a scalar f
+    (vectorised sampling) and a batched f (per-element columns)."""
+    from dynamiqs_amd.integrators.operators import spline_table
+
+    def horner(t0, dt, coef, t):
+        i = np.clip(np.floor((t - t0) / dt).astype(int), 0, len(coef) - 1)
+        x = t - (t0 + i * dt)
+        c = coef[i]
+        return ((c[:, 0] * x + c[:, 1]) * x + c[:, 2]) * x + c[:, 3]
+
+    f = lambda t: np.cos(2 * np.pi * t) + 0.3j * np.sin(5 * t)  # noqa: E731
+    t0, dt, coef = spline_table(f, 0.0, 10.0)
+    assert coef.shape[1] == 4 and np.isclose(t0 + dt * len(coef), 10.0)
+    t = np.random.default_rng(1).uniform(0.0, 10.0, 2000)
+    assert np.max(np.abs(horner(t0, dt, coef, t) - f(t))) < 1e-11
+    g = lambda t: np.array([np.cos(t), np.exp(-t) * 1j])  # noqa: E731
+    t0, dt, coef = spline_table(g, 0.5, 3.0)
+    assert coef.shape[1:] == (4, 2)
+    t = np.linspace(0.5, 3.0, 77)
+    for k in range(2):
+        ref = np.array([g(x)[k] for x in t])
+        assert np.max(np.abs(horner(t0, dt, coef[..., k], t) - ref)) < 1e-11
 
 
 def test_non_hermitian_rho0_error_with_assume_hermitian():

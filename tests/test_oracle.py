@@ -340,17 +340,12 @@ def test_tsit5_otdqubit_closed_form(kats):
     k = kats['otdqubit']
     eps, w, g = k['eps'], k['omega'], k['gamma']
     sx = op.sigmax()
-    H = lambda t: eps * np.cos(w * t) * sx  # noqa: E731
-    # the oracle accepts callables through TimeOp-like objects
-    Hop = om.TimeOp(np.zeros((2, 2)))
-    Hop.__call__  # noqa: B018
+    # H(t) = eps cos(w t) sigma_x as a modulated term (evaluated at every stage time)
+    Hop = om.TimeOp(np.zeros((2, 2), dtype=complex), [om.ModTerm(lambda t: eps * np.cos(w * t), sx)])
+    np.testing.assert_allclose(Hop(0.3), eps * np.cos(w * 0.3) * sx)
+    assert Hop.jump_ts.size == 0
     tsave = np.linspace(0, k['t_end'], k['ntsave'])
-
-    class CallableOp(om.TimeOp):
-        def __call__(self, t):
-            return H(t)
-
-    r = om.mesolve(CallableOp(np.zeros((2, 2))), [np.sqrt(g) * sx], op.fock(2, 0), tsave)
+    r = om.mesolve(Hop, [np.sqrt(g) * sx], op.fock(2, 0), tsave)
     for i, t in enumerate(tsave):
         th = 2 * eps / w * np.sin(w * t)
         eta = np.exp(-2 * g * t)
