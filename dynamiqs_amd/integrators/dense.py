@@ -244,18 +244,42 @@ class DensePlan:
         return out
 
 
-def _kraus_sets(H: np.ndarray, Ls, h: float, normalize: bool):
-    """Rouchon1 operators with the Cholesky normalisation folded in
-    (rouchon_integrator.py:250-263, 552-582): A_0 = M0 T^-dag, A_k = sqrt(h) L_k T^-dag.
-    One step adds tmp + tmp^dag with tmp = dA rho (I + dA^dag / 2) + sum_k A_k rho A_k^dag / 2,
-    dA = A_0 - I, i.e. D = [dA, A_k], C = [I + dA^dag / 2, A_k^dag / 2]."""
+def kraus_operators(H: np.ndarray, Ls, h: float, order: int = 1) -> list:
+    """The Kraus operators M_k of one Rouchon step, the first one the no-jump map.
+
+    order 1 (``KrausEuler``, rouchon_integrator.py:250-263): M0 = I - i h Hnh and
+    sqrt(h) L_k. order 2 (``KrausHeun2`` :272-284 with the midpoint no-jump
+    propagator of :441-443): U = I + h G (I + h/2 G), G = -i H - 1/2 sum L^dag L,
+    and the map U rho U^dag + h/2 U D(rho) U^dag + h/2 D(U rho U^dag)
+    + h^2/2 D(U D(rho) U^dag) of the Heun tableau, whose Kraus operators are U,
+    sqrt(h/2) U L_k, sqrt(h/2) L_k U and h/sqrt(2) L_k U L_j."""
     n = H.shape[-1]
     LdL = sum((_dag(L) @ L for L in Ls), np.zeros_like(H))
-    Hnh = H - 0.5j * LdL
-    M0 = np.eye(n) - 1j * h * Hnh
-    Ms = [np.sqrt(h) * L for L in Ls]
+    if order == 1:
+        M0 = np.eye(n) - 1j * h * (H - 0.5j * LdL)
+        return [M0, *[np.sqrt(h) * L for L in Ls]]
+    if order == 2:
+        G = -1j * H - 0.5 * LdL
+        U = np.eye(n) + h * G @ (np.eye(n) + 0.5 * h * G)
+        out = [U]
+        out += [np.sqrt(0.5 * h) * U @ L for L in Ls]
+        out += [np.sqrt(0.5 * h) * L @ U for L in Ls]
+        out += [h / np.sqrt(2.0) * Lk @ U @ Lj for Lk in Ls for Lj in Ls]
+        return out
+    raise ValueError(f'Rouchon order {order} is not supported')
+
+
+def _kraus_sets(H: np.ndarray, Ls, h: float, normalize: bool, order: int = 1):
+    """Rouchon operators with the Cholesky normalisation folded in
+    (rouchon_integrator.py:394-407, 552-582): S = sum_k M_k^dag M_k = T T^dag and
+    A_k = M_k T^-dag. One step adds tmp + tmp^dag with
+    tmp = dA rho (I + dA^dag / 2) + sum_{k>0} A_k rho A_k^dag / 2, dA = A_0 - I,
+    i.e. D = [dA, A_k], C = [I + dA^dag / 2, A_k^dag / 2]."""
+    n = H.shape[-1]
+    Mall = kraus_operators(H, Ls, h, order)
+    M0, Ms = Mall[0], Mall[1:]
     if normalize:
-        S = _dag(M0) @ M0 + h * LdL
+        S = sum((_dag(M) @ M for M in Mall), np.zeros_like(H))
         T = np.linalg.cholesky(S)
         Tinv_dag = _dag(sla.solve_triangular(T, np.eye(n), lower=True))
         M0 = M0 @ Tinv_dag
@@ -301,6 +325,7 @@ def build_plan(
     graph_steps: int = 16,
     step_log_cap: int = 0,
     tableau: str = 'tsit5',
+    rouchon_order: int = 1,
 ) -> DensePlan:
     """Pack ``instances`` (same Hilbert dimension) for ``kind`` in {'tsit5', 'rouchon1', 'rhs'}.
 
@@ -420,7 +445,7 @@ def build_plan(
         p_step_t = arena.put(step_t)
         hs = np.diff(step_t)
         mids = step_t[:-1] + 0.5 * hs
-        nLdev = K + 1
+        nLdev = K + 1 if rouchon_order == 1 else 1 + 2 * K + K * K
     else:
         nLdev = K
     # fields common to the batch, as whole columns
@@ -504,7 +529,7 @@ def build_plan(
                         if tm >= times[0] and tm < times[-1]:
                             idx = int(np.searchsorted(times, tm, side='right')) - 1
                             Hm = Hm + vals[idx] * op
-                Dk, Ck = _kraus_sets(Hm, inst.Ls, h, normalize)
+                Dk, Ck = _kraus_sets(Hm, inst.Ls, h, normalize, rouchon_order)
                 sets_D.append(np.stack(Dk))
                 sets_C.append(np.stack(Ck))
             key = ('kraus', inst.key_H, inst.key_L)

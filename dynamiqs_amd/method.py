@@ -105,16 +105,40 @@ class Rouchon1(Method):
         self.normalize = normalize
 
 
-class Rouchon2(Method):
+class _RouchonHigherOrder(Method):
+    """Fields of ``Rouchon2`` / ``Rouchon3`` (``dynamiqs/method.py:240-360``)."""
+
     SUPPORTED_GRADIENT = (Direct, BackwardCheckpointed, Forward)
 
-    def __init__(self, dt: float | None = None, normalize: bool = True):
+    def __init__(
+        self,
+        rtol: float = 1e-6,
+        atol: float = 1e-6,
+        safety_factor: float = 0.9,
+        min_factor: float = 0.2,
+        max_factor: float = 5.0,
+        max_steps: int = 100_000,
+        dt: float | None = None,
+        normalize: bool = True,
+    ):
+        self.rtol = rtol
+        self.atol = atol
+        self.safety_factor = safety_factor
+        self.min_factor = min_factor
+        self.max_factor = max_factor
+        self.max_steps = max_steps
         self.dt = dt
         self.normalize = normalize
 
 
-class Rouchon3(Rouchon2):
-    pass
+class Rouchon2(_RouchonHigherOrder):
+    """Second-order Rouchon method. With ``dt`` it is the fixed-step Kraus-Heun map
+    (run on the dense Rouchon kernel with its Kraus operators folded on the host);
+    the adaptive form (``dt=None``) is not supported by the MI355X solver."""
+
+
+class Rouchon3(_RouchonHigherOrder):
+    """Third-order Rouchon method (not supported by the MI355X solver)."""
 
 
 class Expm(Method):

@@ -153,13 +153,14 @@ def mesolve(
             tab=tsit5.tableau(method),
         )
         infos = dict(nsteps=st.nsteps, naccepted=st.naccepted, nrejected=st.nrejected, nrhs=st.nrhs)
-    elif method == 'rouchon1':
+    elif method in ('rouchon1', 'rouchon2'):
         if dt is None:
-            raise ValueError('Rouchon1 needs dt')
+            raise ValueError('fixed-step Rouchon needs dt')
+        kstep = rouchon.kraus_step_dense if method == 'rouchon1' else rouchon.rouchon2_step
 
         def step(t, h, rho):
             tm = t + 0.5 * h
-            return rouchon.kraus_step_dense(H(tm), [L(tm) for L in Ls], rho, h, normalize)
+            return kstep(H(tm), [L(tm) for L in Ls], rho, h, normalize)
 
         saved, ylast, nsteps = rouchon.integrate(step, rho0, tsave, dt, t0=t0, save=save)
         infos = dict(nsteps=nsteps)

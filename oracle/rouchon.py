@@ -1,4 +1,5 @@
-"""Rouchon1 (Kraus-Euler) with Cholesky normalisation, in complex128 (oracle).
+"""Rouchon1 (Kraus-Euler) and Rouchon2 (Kraus-Heun) with Cholesky normalisation,
+in complex128 (oracle).
 
 Restates ``dynamiqs/integrators/core/rouchon_integrator.py``:
 
@@ -67,6 +68,85 @@ def normalized_kraus(H: np.ndarray, Ls, dt: float, normalize: bool = True):
     T = np.linalg.cholesky(S)
     Tinv_dag = dag(sla.solve_triangular(T, np.eye(T.shape[0]), lower=True))
     return [M @ Tinv_dag for M in (M0, *Ms)]
+
+
+# --------------------------------------------------------------------- Rouchon2
+# KrausMapRK (rouchon_integrator.py:85-224) with the Heun tableau of KrausHeun2
+# (:272-284: c = (0, 1), A = ((0, 0), (1, 0)), b = (1/2, 1/2)) and the no-jump
+# propagator of MESolveFixedRouchon2Integrator (:435-443): one step of the explicit
+# midpoint method on dU/dt = G U, U(t) = I, G = -i H - 1/2 sum L^dag L, i.e.
+# U(1) = I + dt G (I + dt/2 G) for an operator constant over the step.
+HEUN2_C = (0.0, 1.0)
+HEUN2_A = ((0.0, 0.0), (1.0, 0.0))
+HEUN2_B = (0.5, 0.5)
+
+
+def rouchon2_propagator(H: np.ndarray, Ls, dt: float) -> np.ndarray:
+    n = H.shape[-1]
+    G = -1j * H - 0.5 * sum((dag(L) @ L for L in Ls), np.zeros_like(H))
+    return np.eye(n) + dt * G @ (np.eye(n) + 0.5 * dt * G)
+
+
+def rouchon2_step(H: np.ndarray, Ls, rho: np.ndarray, dt: float, normalize: bool = True) -> np.ndarray:
+    """One Rouchon2 step in the reference's stage form: the stages
+    rho^(i) = U(c_i) rho U(c_i)^dag + dt sum_j a_ij P_ij D(rho^(j)) P_ij^dag
+    (``compute_stages`` :154-171), the update
+    rho' = U(1) rho U(1)^dag + dt sum_i b_i Q_i D(rho^(i)) Q_i^dag (``__call__``
+    :173-184) with P_ij = U(c_i) U(c_j)^-1, Q_i = U(1) U(c_i)^-1, and, with
+    ``normalize``, rho first mapped to T^-dag rho T^-1 where T T^dag = S from the
+    adjoint pass (``S_stage`` / ``S`` :193-224)."""
+    n = H.shape[-1]
+    I = np.eye(n)
+    U1 = rouchon2_propagator(H, Ls, dt)
+    U = {0.0: I, 1.0: U1}
+
+    def ratio(ci, cj):
+        if ci == cj:
+            return I
+        if cj == 0.0:
+            return U[ci]
+        return np.linalg.solve(U[cj].T, U[ci].T).T
+
+    def D(r):
+        return sum((L @ r @ dag(L) for L in Ls), np.zeros_like(r))
+
+    def Dadj(o):
+        return sum((dag(L) @ o @ L for L in Ls), np.zeros_like(o))
+
+    def S_stage(i, O):
+        ci = HEUN2_C[i]
+        res = dag(U[ci]) @ O @ U[ci]
+        for j in range(i):
+            if HEUN2_A[i][j] == 0.0:
+                continue
+            P = ratio(ci, HEUN2_C[j])
+            res = res + S_stage(j, dt * HEUN2_A[i][j] * Dadj(dag(P) @ O @ P))
+        return res
+
+    if normalize:
+        S = dag(U1) @ U1
+        for i, bi in enumerate(HEUN2_B):
+            if bi == 0.0:
+                continue
+            Q = ratio(1.0, HEUN2_C[i])
+            S = S + dt * bi * S_stage(i, Dadj(dag(Q) @ Q))
+        rho = cholesky_normalize(S, rho)
+    stages = []
+    for i, ci in enumerate(HEUN2_C):
+        r = U[ci] @ rho @ dag(U[ci])
+        for j in range(i):
+            if HEUN2_A[i][j] == 0.0:
+                continue
+            P = ratio(ci, HEUN2_C[j])
+            r = r + dt * HEUN2_A[i][j] * P @ D(stages[j]) @ dag(P)
+        stages.append(r)
+    out = U1 @ rho @ dag(U1)
+    for i, bi in enumerate(HEUN2_B):
+        if bi == 0.0:
+            continue
+        Q = ratio(1.0, HEUN2_C[i])
+        out = out + dt * bi * Q @ D(stages[i]) @ dag(Q)
+    return out
 
 
 def _to_lower_band(S: np.ndarray, w: int) -> np.ndarray:

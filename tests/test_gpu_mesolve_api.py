@@ -160,3 +160,30 @@ def test_save_extra_every_save_time_with_batching_and_no_states():
     np.testing.assert_allclose(res.extra['pop'], np.real(np.einsum('ij,btji->bt', num, st)), atol=1e-6)
     np.testing.assert_allclose(np.asarray(res.extra['corner'].to_numpy()), st[:, :, 0, 1], atol=1e-6)
     np.testing.assert_allclose(res.states.to_numpy(), st[:, -1], atol=1e-6)
+
+
+def test_rouchon2_fixed_step_matches_oracle():
+    """dq.method.Rouchon2(dt) on the dense Kraus kernel (1 + 2K + K^2 normalised
+    Kraus operators folded on the host) against the complex128 oracle's
+    restatement of the reference's Kraus-Heun map, with and without normalisation,
+    on a batch of initial states; trace preserved with normalisation."""
+    from oracle import mesolve as omesolve
+
+    n = 8
+    a = dq.destroy(n)
+    H = a.dag() @ a + 0.3 * (a + a.dag())
+    Ls = [0.5 * a, 0.2 * a.dag() @ a]
+    rho0 = np.stack([dq.coherent_dm(n, al).to_numpy() for al in (0.3, 0.9)])
+    tsave = np.linspace(0.0, 1.0, 6)
+    for normalize in (True, False):
+        res = dq.mesolve(H, Ls, rho0, tsave, exp_ops=[a.dag() @ a],
+                         method=dq.method.Rouchon2(dt=1e-2, normalize=normalize))
+        st = res.states.to_numpy()
+        for b in range(2):
+            ref = omesolve.mesolve(H.to_numpy(), [L.to_numpy() for L in Ls], rho0[b], tsave, method='rouchon2',
+                                   dt=1e-2, normalize=normalize, exp_ops=[(a.dag() @ a).to_numpy()])
+            assert np.linalg.norm(st[b] - ref['states'], axis=(-2, -1)).max() <= 1e-5
+            np.testing.assert_allclose(res.expects[b, 0], ref['expects'][0], atol=1e-5)
+        if normalize:
+            assert np.abs(np.trace(st[:, -1], axis1=-2, axis2=-1) - 1).max() < 1e-5
+    assert str(res).split('\n')[1] == 'Method  : Rouchon2'

@@ -534,3 +534,46 @@ def test_tsit5_replay_of_own_steps_reproduces_adaptive_solve():
     assert len(st.ends) == st.naccepted
     np.testing.assert_allclose(np.stack(replay), np.stack(saved), atol=1e-12)
     np.testing.assert_allclose(yrep, ylast, atol=1e-12)
+
+
+def test_rouchon2_kraus_form_equals_reference_stage_form():
+    """The host's Kraus operators of the Rouchon2 step (U, sqrt(dt/2) U L_k,
+    sqrt(dt/2) L_k U, dt/sqrt(2) L_k U L_j, normalised by S = sum M^dag M) applied as
+    sum A rho A^dag equal the oracle's restatement of the reference's stage form
+    (KrausMapRK with the Heun tableau, including its adjoint-pass S), with and
+    without normalisation; the normalised operators satisfy sum A^dag A = I."""
+    from dynamiqs_amd.integrators.dense import _dag, _kraus_sets
+
+    rng = np.random.default_rng(21)
+    H, Ls, rho = _rand_problem(rng, 6, 2)
+    for normalize in (True, False):
+        D, C = _kraus_sets(H, Ls, 0.05, normalize, order=2)
+        A = [np.eye(6) + D[0], *D[1:]]
+        got = sum(a @ rho @ _dag(a) for a in A)
+        want = rouchon.rouchon2_step(H, Ls, rho, 0.05, normalize)
+        np.testing.assert_allclose(got, want, atol=1e-13)
+        # the kernel's form: rho + tmp + tmp^dag with tmp = sum_k D_k rho C_k
+        tmp = sum(d @ rho @ c for d, c in zip(D, C))
+        np.testing.assert_allclose(rho + tmp + _dag(tmp), want, atol=1e-13)
+        if normalize:
+            np.testing.assert_allclose(sum(_dag(a) @ a for a in A), np.eye(6), atol=1e-13)
+    assert len(D) == 1 + 2 * 2 + 2 * 2
+
+
+def test_rouchon2_second_order_and_trace():
+    n = 5
+    a = op.destroy(n)
+    H = a.conj().T @ a + 0.2 * (a + a.conj().T)
+    Ls = [0.5 * a, 0.3 * a.conj().T @ a]
+    rho0 = op.coherent_dm(n, 0.7)
+    ex = lindblad.unvectorize(sla.expm(lindblad.slindbladian(H, Ls)) @ lindblad.vectorize(rho0))
+
+    def run(dt):
+        r = rho0.copy()
+        for _ in range(int(round(1.0 / dt))):
+            r = rouchon.rouchon2_step(H, Ls, r, dt)
+        return r
+
+    e1, e2 = np.linalg.norm(run(0.02) - ex), np.linalg.norm(run(0.01) - ex)
+    assert 3.6 < e1 / e2 < 4.4
+    assert abs(np.trace(run(0.05)) - 1) < 1e-13
