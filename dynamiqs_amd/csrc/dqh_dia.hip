@@ -85,6 +85,7 @@ __global__ __launch_bounds__(kThreads) void k_lsolve(int mode, int64_t n, const 
                                                      const float2 *__restrict__ D1, float2 *__restrict__ out) {
   constexpr int LDP = PW + 2;  // padded LDS row stride of a panel tile
   constexpr int TJ = PW / 32;  // output tiles per wave
+  constexpr int NQ = kNB * PW / kThreads;  // panel (and transposed tile) elements per thread
   __shared__ float2 s_lds[kNB * LDP];  // s_i panel (64 x PW)
   __shared__ float2 prev[kNB * LDP];   // x_{i+1} panel
   __shared__ float2 tr[PW * kLD];      // D1[c0 + i][r0 + j] (mode 1), i < PW, j < 64
@@ -99,19 +100,30 @@ __global__ __launch_bounds__(kThreads) void k_lsolve(int mode, int64_t n, const 
   const int NB = (int)(n / kNB);
   // wave w owns row tile w / 2 and column tiles TJ * (w % 2) .. + TJ - 1
   const int ti = wave >> 1, tj0 = (wave & 1) * TJ;
+  // s_i (64 x PW) and, in mode 1, the tile of D1 whose conjugate transpose is added,
+  // loaded into registers one block row ahead: the next panel is in flight while
+  // the current block's products run
+  float2 sreg[NQ], treg[NQ];
+  auto load_panel = [&](int bi) {
+    const int r0 = bi * kNB;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int e = threadIdx.x + kThreads * q;
+      sreg[q] = R[(size_t)(r0 + e / PW) * n + c0 + e % PW];
+      if (mode == 1) treg[q] = Dt[(size_t)(c0 + e / kNB) * n + r0 + e % kNB];  // D1[c0+i][r0+j]
+    }
+  };
   for (int e = threadIdx.x; e < kNB * LDP; e += kThreads) prev[e] = make_float2(0.f, 0.f);
+  load_panel(NB - 1);
   for (int bi = NB - 1; bi >= 0; --bi) {
     const int r0 = bi * kNB;
-    // load s_i (64 x PW) and, in mode 1, the tile of D1 whose conjugate transpose is added
-    for (int e = threadIdx.x; e < kNB * PW; e += kThreads) {
-      const int i = e / PW, j = e % PW;
-      s_lds[i * LDP + j] = R[(size_t)(r0 + i) * n + c0 + j];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int e = threadIdx.x + kThreads * q;
+      s_lds[(e / PW) * LDP + e % PW] = sreg[q];
+      if (mode == 1) tr[(e / kNB) * kLD + e % kNB] = treg[q];
     }
-    if (mode == 1)
-      for (int e = threadIdx.x; e < PW * kNB; e += kThreads) {
-        const int i = e / kNB, j = e % kNB;
-        tr[i * kLD + j] = Dt[(size_t)(c0 + i) * n + r0 + j];  // D1[c0+i][r0+j]
-      }
+    if (bi > 0) load_panel(bi - 1);
     __syncthreads();
     if (mode == 1) {
       for (int e = threadIdx.x; e < kNB * PW; e += kThreads) {
