@@ -6,8 +6,9 @@
 // with S = M0^dag M0 + dt sum L^dag L = T T^dag banded (half-bandwidth w <= nb).
 // The state rho is kept in FP64 and every step adds an FP32 increment:
 //   K1  D1 = T^{-dag} rho - rho                          (banded left solve, MFMA)
-//   K2  E  = D1^dag + (T^{-dag} Z - Z),  Z = rho + D1^dag (same solve on Z)
-//       so that X = rho + E is the normalised state,
+//   K2  E  = D1 + (T^{-dag} Z - Z)^dag,  Z = (rho + D1)^dag (same solve on Z)
+//       so that X = rho + E = (rho + D1) T^{-1} is the normalised state for any
+//       rho, Hermitian or not,
 //   K3  rho += E + dM X + X dM^dag + dM X dM^dag + sum_k A_k X A_k^dag
 //       (dM = M0 - I = -i dt Hnh, A_k = sqrt(dt) L_k; a DIA stencil sweep).
 // The banded solve T^dag x = s runs over blocks of nb rows from the bottom:
@@ -75,10 +76,13 @@ __device__ __forceinline__ void mac_strip(Acc (&acc)[TJ], const float2 *__restri
 }
 
 // Banded left solve of one column panel of PW columns (64 or 32). mode 0:
-// s = rho32, output D1 = x - s. mode 1: s = rho32 + D1^dag, output
-// E = D1^dag + (x - s). grid: (n / PW panels, ninst); block: kThreads. The 64 x PW
-// output of a block row is 4 x PW/16 tiles of 16 x 16, TJ = PW / 32 per wave;
-// narrower panels give twice the workgroups (two per CU) for small batches.
+// s = rho32, output D1 = x - s. mode 1: s = Z = (rho32 + D1)^dag, output
+// E = D1 + (x - s)^dag, written transposed (through LDS, coalesced), so that
+// rho + E = (rho + D1) T^-1 = T^-dag rho T^-1 holds for any rho, Hermitian or not:
+// rounding that leaves an anti-Hermitian part in the state is then contracted by
+// the map instead of amplified. grid: (n / PW panels, ninst); block: kThreads. The
+// 64 x PW output of a block row is 4 x PW/16 tiles of 16 x 16, TJ = PW / 32 per
+// wave; narrower panels give twice the workgroups (two per CU) for small batches.
 template <int PW>
 __global__ __launch_bounds__(kThreads) void k_lsolve(int mode, int64_t n, const float2 *__restrict__ Vp,
                                                      const float2 *__restrict__ Wn, const float2 *__restrict__ rho32,
@@ -88,7 +92,10 @@ __global__ __launch_bounds__(kThreads) void k_lsolve(int mode, int64_t n, const 
   constexpr int NQ = kNB * PW / kThreads;  // panel (and transposed tile) elements per thread
   __shared__ float2 s_lds[kNB * LDP];  // s_i panel (64 x PW)
   __shared__ float2 prev[kNB * LDP];   // x_{i+1} panel
-  __shared__ float2 tr[PW * kLD];      // D1[c0 + i][r0 + j] (mode 1), i < PW, j < 64
+  // mode 1: tr holds (rho + D1)[c0 + j][r0 + i] (j < PW, i < 64), trd holds
+  // D1[c0 + j][r0 + i] and then the transposed output block E[c0 + j][r0 + i]
+  __shared__ float2 tr[PW * kLD];
+  __shared__ float2 trd[PW * kLD];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int panel = blockIdx.x;
   const size_t inst = blockIdx.y;
@@ -100,17 +107,23 @@ __global__ __launch_bounds__(kThreads) void k_lsolve(int mode, int64_t n, const 
   const int NB = (int)(n / kNB);
   // wave w owns row tile w / 2 and column tiles TJ * (w % 2) .. + TJ - 1
   const int ti = wave >> 1, tj0 = (wave & 1) * TJ;
-  // s_i (64 x PW) and, in mode 1, the tile of D1 whose conjugate transpose is added,
-  // loaded into registers one block row ahead: the next panel is in flight while
-  // the current block's products run
+  // the inputs of block row i, loaded into registers one block row ahead so that
+  // the next ones are in flight while the current block's products run: mode 0 the
+  // s_i panel of rho, mode 1 the transposed tiles rho[c0+j][r0+i], D1[c0+j][r0+i]
+  // (coalesced rows of 64)
   float2 sreg[NQ], treg[NQ];
   auto load_panel = [&](int bi) {
     const int r0 = bi * kNB;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const int e = threadIdx.x + kThreads * q;
-      sreg[q] = R[(size_t)(r0 + e / PW) * n + c0 + e % PW];
-      if (mode == 1) treg[q] = Dt[(size_t)(c0 + e / kNB) * n + r0 + e % kNB];  // D1[c0+i][r0+j]
+      if (mode == 0) {
+        sreg[q] = R[(size_t)(r0 + e / PW) * n + c0 + e % PW];
+      } else {
+        const size_t at = (size_t)(c0 + e / kNB) * n + r0 + e % kNB;
+        sreg[q] = R[at];
+        treg[q] = Dt[at];
+      }
     }
   };
   for (int e = threadIdx.x; e < kNB * LDP; e += kThreads) prev[e] = make_float2(0.f, 0.f);
@@ -120,17 +133,22 @@ __global__ __launch_bounds__(kThreads) void k_lsolve(int mode, int64_t n, const 
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const int e = threadIdx.x + kThreads * q;
-      s_lds[(e / PW) * LDP + e % PW] = sreg[q];
-      if (mode == 1) tr[(e / kNB) * kLD + e % kNB] = treg[q];
+      if (mode == 0) {
+        s_lds[(e / PW) * LDP + e % PW] = sreg[q];
+      } else {
+        const int at = (e / kNB) * kLD + e % kNB;
+        tr[at] = make_float2(sreg[q].x + treg[q].x, sreg[q].y + treg[q].y);
+        trd[at] = treg[q];
+      }
     }
     if (bi > 0) load_panel(bi - 1);
     __syncthreads();
     if (mode == 1) {
+      // s[i][j] = Z[r0+i][c0+j] = conj((rho + D1)[c0+j][r0+i])
       for (int e = threadIdx.x; e < kNB * PW; e += kThreads) {
         const int i = e / PW, j = e % PW;
-        const float2 d = tr[j * kLD + i];  // D1[c0+j][r0+i] -> (D1^dag)[r0+i][c0+j] = conj
-        s_lds[i * LDP + j].x += d.x;
-        s_lds[i * LDP + j].y -= d.y;
+        const float2 z = tr[j * kLD + i];
+        s_lds[i * LDP + j] = make_float2(z.x, -z.y);
       }
       __syncthreads();
     }
@@ -153,16 +171,23 @@ __global__ __launch_bounds__(kThreads) void k_lsolve(int mode, int64_t n, const 
         const float2 d = make_float2(acc[t].re[v], acc[t].im[v]);
         const float2 sv = s_lds[i * LDP + j];
         prev[i * LDP + j] = make_float2(sv.x + d.x, sv.y + d.y);
-        float2 o = d;
-        if (mode == 1) {
-          const float2 tt = tr[j * kLD + i];
-          o.x += tt.x;
-          o.y -= tt.y;
+        if (mode == 0) {
+          O[(size_t)(r0 + i) * n + c0 + j] = d;
+        } else {
+          // E[c0+j][r0+i] = D1[c0+j][r0+i] + conj(d[i][j]), staged in place
+          const float2 dd = trd[j * kLD + i];
+          trd[j * kLD + i] = make_float2(dd.x + d.x, dd.y - d.y);
         }
-        O[(size_t)(r0 + i) * n + c0 + j] = o;
       }
     }
     __syncthreads();
+    if (mode == 1) {
+      for (int e = threadIdx.x; e < PW * kNB; e += kThreads) {
+        const int j = e / kNB, i = e % kNB;
+        O[(size_t)(c0 + j) * n + r0 + i] = trd[j * kLD + i];
+      }
+      __syncthreads();  // trd is reloaded by the next block row
+    }
   }
 }
 

@@ -20,7 +20,9 @@ def _two_mode(na, nb_):
 
 
 def _emulate_step(rho, Vp, Wn, ops, n_pad):
-    """Increment-form step exactly as k_lsolve (x2) and k_kraus compute it."""
+    """Increment-form step exactly as k_lsolve (x2) and k_kraus compute it: mode 0
+    gives D1 = T^-dag rho - rho; mode 1 solves on (rho + D1)^dag and writes
+    E = D1 + d^dag (its output transposed)."""
     NB = dia.NB
     nblk = n_pad // NB
 
@@ -37,8 +39,10 @@ def _emulate_step(rho, Vp, Wn, ops, n_pad):
         return d
 
     D1 = lsolve(rho)
-    Z = rho + D1.conj().T
-    E = D1.conj().T + lsolve(Z)
+    # second solve on Z = (rho + D1)^dag: X = rho + E = (T^-dag Z)^dag = T^-dag rho T^-1
+    # for any rho (no Hermitian assumption)
+    Z = (rho + D1).conj().T
+    E = D1 + lsolve(Z).conj().T
     X = rho + E
     n = n_pad
 
@@ -240,3 +244,36 @@ def test_row_blocked_grid_covers_each_element_once(n):
     np.add.at(hits, (i[ok], j[ok]), 1)
     assert np.all(hits == 1)
     assert np.all(i < n)
+
+
+@pytest.mark.parametrize('na', [12, 16])
+def test_block_solve_step_is_exact_off_the_hermitian_matrices(na):
+    """The kernels' step equals the exact Rouchon1 map on non-Hermitian matrices
+    too, so rounding that leaves an anti-Hermitian part in the state is contracted
+    by the map, not amplified (a Hermitian-only shortcut, T^-dag rho T^-1 =
+    T^-dag (T^-dag rho)^dag, amplifies it by ~1 + 20 dt per step at n = 256)."""
+    H, Ls = _two_mode(na, na)
+    n = na * na
+    n_pad = dia.NB * ((n + dia.NB - 1) // dia.NB)
+    dt = 1e-3
+    Vp, Wn, ops = dia.block_solve_operators(H, Ls, dt, n_pad)
+    ops_p = []
+    for offs, d in ops:
+        dp = np.zeros((len(offs), n_pad), dtype=complex)
+        dp[:, :n] = d
+        ops_p.append((offs, dp))
+    rng = np.random.default_rng(1)
+    x = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    for rho in (x, x - x.conj().T):
+        rp = np.zeros((n_pad, n_pad), dtype=complex)
+        rp[:n, :n] = rho
+        got = _emulate_step(rp, Vp, Wn, ops_p, n_pad)
+        np.testing.assert_allclose(got[:n, :n], rouchon.kraus_step_dense(H, Ls, rho, dt), atol=1e-11)
+    # power iteration on anti-Hermitian inputs: no growth
+    A = np.zeros((n_pad, n_pad), dtype=complex)
+    A[:n, :n] = x - x.conj().T
+    A /= np.linalg.norm(A)
+    for _ in range(200):
+        A = _emulate_step(A, Vp, Wn, ops_p, n_pad)
+        assert np.linalg.norm(A) <= 1.0 + 1e-9
+        A /= np.linalg.norm(A)
