@@ -463,9 +463,49 @@ def bench_copy(args):
     return 0
 
 
+def bench_c1(args):
+    """The plain single-solve configuration: lossy oscillator n=16, H = a^dag a,
+    L = sqrt(0.1) a, rho0 = coherent(1.0), 101 save times on [0, 2 pi], one instance,
+    through ``dq.mesolve`` end to end (host packing, launch, read-back). Measured at
+    the configuration's tolerances (rtol 1e-6, atol 1e-8) and at the dynamiqs README
+    example's (atol 1e-6; README.md:48-77 prints 6.30 ms on unstated hardware)."""
+    import dynamiqs_amd as dq
+    from oracle import mesolve as om
+    from oracle import operators as op
+
+    n = 16
+    a = dq.destroy(n)
+    H, Ls, psi = a.dag() @ a, [np.sqrt(0.1) * a], dq.coherent(n, 1.0)
+    tsave = np.linspace(0.0, 2 * np.pi, 101)
+    rows = {}
+    for name, atol in (('rtol 1e-6, atol 1e-8', 1e-8), ('rtol 1e-6, atol 1e-6 (README example)', 1e-6)):
+        method = dq.method.Tsit5(rtol=1e-6, atol=atol)
+        for _ in range(max(args.warmup, 1)):
+            dq.mesolve(H, Ls, psi, tsave, method=method)
+        lat = []
+        for _ in range(max(args.steps, 5)):
+            t0 = time.perf_counter()
+            res = dq.mesolve(H, Ls, psi, tsave, method=method)
+            lat.append(time.perf_counter() - t0)
+        ao = op.destroy(n)
+        ref = om.mesolve(ao.conj().T @ ao, [np.sqrt(0.1) * ao], op.coherent(n, 1.0), tsave, rtol=1e-6, atol=atol)
+        st = res.states.to_numpy()
+        rows[name] = {
+            'p50_latency_ms': 1e3 * float(np.median(lat)),
+            'min_latency_ms': 1e3 * float(np.min(lat)),
+            'nsteps': int(res.infos.nsteps),
+            'oracle_nsteps': int(ref['infos']['nsteps']),
+            'max_frobenius_vs_fp64': float(np.linalg.norm(st - ref['states'], axis=(-2, -1)).max()),
+        }
+    print(json.dumps({'metric': 'plain mesolve latency (one solve per call)', 'unit': 'ms',
+                      'config': {'workload': 'lossy oscillator n=16, 101 tsave on [0, 2 pi], Tsit5, one instance'},
+                      'runs': rows}), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument('--config', default='kerr', choices=['kerr', 'dia', 'serving', 'adjoint', 'copy', 'grid'])
+    ap.add_argument('--config', default='kerr', choices=['kerr', 'c1', 'dia', 'serving', 'adjoint', 'copy', 'grid'])
     ap.add_argument('--requests', type=int, default=1000)
     ap.add_argument('--rates', type=float, nargs='+', default=[100.0, 400.0, 0.0])
     ap.add_argument('--budget-gb', type=float, default=32.0)
@@ -489,6 +529,8 @@ def main(argv=None):
 
     if args.config == 'copy':
         return bench_copy(args)
+    if args.config == 'c1':
+        return bench_c1(args)
     if args.config == 'grid':
         return bench_grid(args)
     if args.config == 'dia':
