@@ -270,8 +270,10 @@ __device__ __forceinline__ void res_rhs_upper(const float2 *__restrict__ Hnh, co
   const float2 *X = res_rhs_tmp_h<NP, NW>(Hnh, D, nL, hasH, lds);
 #pragma unroll
   for (int q = 0; q < R::NEU; ++q) {
-    if (ij[q] >= 0) {
-      const int i = ij_row(ij[q]), j = ij_col(ij[q]);
+    int x = ij[q];
+    asm volatile("" : "+v"(x));
+    if (x >= 0) {
+      const int i = ij_row(x), j = ij_col(x);
       const float2 a = X[i * R::LDP + j], b = X[j * R::LDP + i];
       out[q] = make_float2(a.x + b.x, a.y - b.y);
     } else {
@@ -297,8 +299,10 @@ __device__ __forceinline__ void res_write_ys_upper(float2 *lds, const int (&ij)[
   using R = RCfg<NP, NW>;
 #pragma unroll
   for (int q = 0; q < R::NEU; ++q) {
-    if (ij[q] >= 0) {
-      const int i = ij_row(ij[q]), j = ij_col(ij[q]);
+    int x = ij[q];
+    asm volatile("" : "+v"(x));  // recompute the addresses here instead of keeping them live
+    if (x >= 0) {
+      const int i = ij_row(x), j = ij_col(x);
       lds[i * R::LDP + j] = v[q];
       if (i != j) lds[j * R::LDP + i] = make_float2(v[q].x, -v[q].y);
     }
