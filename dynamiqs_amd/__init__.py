@@ -1,18 +1,19 @@
 """dynamiqs_amd -- MI355X-native (gfx950) drop-in for ``dynamiqs.mesolve``.
 
 ``import dynamiqs_amd as dq`` exposes the subset of the dynamiqs API that the
-``mesolve`` hot path uses: ``dq.mesolve``, quantum arrays (dense and sparse DIA),
-time-dependent operators (``constant``, ``pwc``), operator/state builders,
-``Options``, ``method.Tsit5`` / ``method.Rouchon1`` and ``MESolveResult``.
+``mesolve`` hot path uses: ``dq.mesolve`` and ``dq.sesolve``, quantum arrays
+(dense and sparse DIA), time-dependent operators (``constant``, ``pwc``,
+``modulated``), operator/state builders, ``Options``, the methods
+``Tsit5`` / ``Dopri5`` / ``Rouchon1`` / ``Rouchon2`` and the result classes.
 The solve runs in hand-written HIP kernels (``libdqh.so``, C-ABI in
 ``include/dqh.h``) loaded with ctypes; there is no CPU fallback.
 """
 
 from . import gradient, method
-from .integrators import mesolve
+from .integrators import mesolve, sesolve
 from .options import Options
 from .qarrays import QArray, asqarray, dense, dia, set_layout, set_precision
-from .result import MESolveResult
+from .result import MESolveResult, SESolveResult
 from .time_qarray import TimeQArray, constant, modulated, pwc, timecallable
 from .utils import (
     coherent,

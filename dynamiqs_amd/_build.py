@@ -11,7 +11,8 @@ CSRC = os.path.join(HERE, 'csrc')
 INCLUDE = os.path.join(os.path.dirname(HERE), 'include')
 LIB = os.path.join(HERE, 'libdqh.so')
 
-HIP_SOURCES = ['dqh_dense.hip', 'dqh_dense_grid.hip', 'dqh_dia.hip', 'dqh_dia_rhs.hip', 'dqh_dia_tsit5.hip', 'dqh_adjoint.hip']
+HIP_SOURCES = ['dqh_dense.hip', 'dqh_dense_grid.hip', 'dqh_dia.hip', 'dqh_dia_rhs.hip', 'dqh_dia_tsit5.hip', 'dqh_adjoint.hip',
+               'dqh_sesolve.hip']
 CXX_SOURCES = ['dqh_abi.cpp']
 
 # Packed FP32 VALU forms are disabled for every HIP translation unit.

@@ -1,3 +1,4 @@
 from .mesolve import mesolve
+from .sesolve import sesolve
 
-__all__ = ['mesolve']
+__all__ = ['mesolve', 'sesolve']

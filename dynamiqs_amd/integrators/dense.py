@@ -118,10 +118,12 @@ ARENA_FIELDS = ('tsave', 'Hnh', 'Dops', 'Cops', 'EopsT', 'Hpwc', 'pwc_times', 'p
 POINTER_FIELDS = ARENA_FIELDS + ('y', 'work', 'states', 'expects', 'final_state', 'stats', 'step_log')
 
 
-def check_descriptors(descs: np.ndarray, tensors) -> None:
+def check_descriptors(descs: np.ndarray, tensors, extents: dict | None = None) -> None:
     """Host-side guard before a launch: every non-null pointer of every descriptor
     must lie inside one of the device allocations of the plan, and the buffers
-    whose extent the descriptor fixes must end inside the same allocation."""
+    whose extent the descriptor fixes must end inside the same allocation.
+    ``extents`` replaces the byte extents of the named fields (per descriptor) for
+    kernels that read the descriptor with another layout."""
     spans = sorted((t.data_ptr(), t.data_ptr() + t.numel() * t.element_size()) for t in tensors)
     lo = np.array([s[0] for s in spans], dtype=np.uint64)
     hi = np.array([s[1] for s in spans], dtype=np.uint64)
@@ -142,6 +144,7 @@ def check_descriptors(descs: np.ndarray, tensors) -> None:
         'Hmod': descs['nmod'].astype(np.uint64) * npd * npd * np.uint64(8),
         'mod_coef': descs['nmod'].astype(np.uint64) * descs['mod_nint'].astype(np.uint64) * np.uint64(64),
     }
+    extent.update(extents or {})
     for f in POINTER_FIELDS:
         v = descs[f].astype(np.uint64)
         nz = v != 0

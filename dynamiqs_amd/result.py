@@ -116,3 +116,12 @@ class MESolveResult(SolveResult):
     ``errors`` (optional per-instance check against an FP64 reference when
     requested by the caller).
     """
+
+
+class SESolveResult(SolveResult):
+    """Result of :func:`dynamiqs_amd.sesolve`.
+
+    ``states`` (..., nsave, n, 1), ``final_state``, ``expects`` (..., nE, ntsave)
+    or None, ``infos``, ``tsave``, ``method``, ``gradient``, ``options``
+    (``dynamiqs/result.py:189-190``).
+    """

@@ -186,3 +186,45 @@ def test_rouchon_method_support():
         dq.mesolve(H, Ls, psi, ts, method=dq.method.Rouchon3(dt=0.1))
     m = dq.method.Rouchon2(dt=0.01, normalize=False)
     assert (m.dt, m.normalize, m.rtol, m.max_steps) == (0.01, False, 1e-6, 100_000)
+
+
+def test_sesolve_argument_errors():
+    """``dq.sesolve`` raises what the reference raises before integrating
+    (dynamiqs/integrators/apis/sesolve.py:222-225, 282, 307-318): shapes, save
+    times, options of another solver, unsupported method types; callable
+    Hamiltonians raise NotImplementedError before any device work."""
+    a = dq.destroy(3)
+    H, psi, ts = a.dag() @ a, dq.fock(3, 0), np.linspace(0.0, 1.0, 3)
+    with pytest.raises(ValueError, match=re.escape('Argument `H` must have shape (...H, n, n), but has shape H.shape=(3, 2)')):
+        dq.sesolve(np.ones((3, 2)), psi, ts)
+    with pytest.raises(ValueError, match=re.escape('Argument `psi0` must have shape (...psi0, n, 1)')):
+        dq.sesolve(H, dq.fock_dm(3, 0), ts)
+    with pytest.raises(ValueError, match=re.escape('Argument `exp_ops[0]` must have shape (n, n)')):
+        dq.sesolve(H, psi, ts, exp_ops=[np.ones((2, 3, 3))])
+    with pytest.raises(ValueError, match='must be sorted in strictly ascending order'):
+        dq.sesolve(H, psi, np.array([0.0, 1.0, 0.5]))
+    with pytest.raises(ValueError, match=re.escape('Option `vectorized` was set to `True` but is not used by the solver `dq.sesolve()`')):
+        dq.sesolve(H, psi, ts, options=dq.Options(vectorized=True))
+    with pytest.raises(TypeError, match=re.escape('Method of type `Rouchon1` is not supported (supported method types: `Tsit5`, `Dopri5`)')):
+        dq.sesolve(H, psi, ts, method=dq.method.Rouchon1(dt=0.1))
+    with pytest.raises(NotImplementedError, match='CallableTimeQArray operators are not supported'):
+        dq.sesolve(dq.timecallable(lambda t: H), psi, ts)
+
+
+def test_sesolve_batching_shapes():
+    """Cartesian batching concatenates (...H, ...psi0); flat batching broadcasts
+    them (sesolve.py:234-260); each instance gets its H element and its ket."""
+    from dynamiqs_amd.integrators.sesolve import enumerate_instances
+
+    n = 3
+    Hs = dq.asqarray(np.stack([k * np.eye(n) for k in range(4)]).reshape(2, 2, n, n))
+    psis = dq.asqarray(np.stack([dq.fock(n, k).to_numpy() for k in range(2)]))
+    shape, insts = enumerate_instances(dq.constant(Hs), psis, True)
+    assert shape == (2, 2, 2) and len(insts) == 8
+    np.testing.assert_array_equal(insts[5].H, 2 * np.eye(n))  # index (1, 0, 1)
+    np.testing.assert_array_equal(insts[5].psi0, [0, 1, 0])
+    shape, insts = enumerate_instances(dq.constant(Hs), psis, False)
+    assert shape == (2, 2) and len(insts) == 4
+    np.testing.assert_array_equal(insts[3].H, 3 * np.eye(n))
+    np.testing.assert_array_equal(insts[3].psi0, [0, 1, 0])
+    np.testing.assert_array_equal(insts[2].psi0, [1, 0, 0])

@@ -577,3 +577,95 @@ def test_rouchon2_second_order_and_trace():
     e1, e2 = np.linalg.norm(run(0.02) - ex), np.linalg.norm(run(0.01) - ex)
     assert 3.6 < e1 / e2 < 4.4
     assert abs(np.trace(run(0.05)) - 1) < 1e-13
+
+
+# ----------------------------------------------------------------- sesolve
+
+
+def test_sesolve_cavity_closed_form():
+    """The reference's Cavity system (tests/systems/closed_system.py:46-104,
+    n=8, delta=2 pi, alpha0=0.5): H = delta a^dag a is diagonal, so every Fock
+    amplitude rotates as exp(-1j delta k t); the truncated <x>, <p> follow
+    alpha(t) = alpha0 exp(-1j delta t) within the reference's own tolerances."""
+    from oracle import sesolve as ose
+
+    n, delta, a0 = 8, 2 * np.pi, 0.5
+    ts = np.linspace(0.0, 0.3, 11)
+    psi0 = op.coherent(n, a0)
+    out = ose.sesolve(delta * op.number(n), psi0, ts, exp_ops=[op.position(n), op.momentum(n)], rtol=1e-10, atol=1e-10)
+    k = np.arange(n)
+    exact = np.stack([np.exp(-1j * delta * k * t) * psi0[:, 0] for t in ts])
+    np.testing.assert_allclose(out['states'][..., 0], exact, atol=1e-8)
+    alpha = a0 * np.exp(-1j * delta * ts)
+    np.testing.assert_allclose(out['expects'][0], alpha.real, rtol=1e-3, atol=1e-4)
+    np.testing.assert_allclose(out['expects'][1], alpha.imag, rtol=1e-3, atol=1e-4)
+    assert out['infos']['naccepted'] > 0
+
+
+def test_sesolve_tdqubit_closed_form():
+    """The reference's TDQubit (closed_system.py:107-176, eps=3, omega=10): H(t) =
+    eps cos(omega t) sigma_x from |0> gives cos(theta/2)|0> - 1j sin(theta/2)|1>,
+    theta = 2 eps/omega sin(omega t); <sigma_y> = -sin theta, <sigma_z> = cos theta."""
+    from oracle import sesolve as ose
+
+    eps, w = 3.0, 10.0
+    ts = np.linspace(0.0, 1.0, 11)
+    H = om.TimeOp(np.zeros((2, 2), dtype=complex), [om.ModTerm(lambda t: eps * np.cos(w * t), op.sigmax())])
+    for method in ('tsit5', 'dopri5'):
+        out = ose.sesolve(H, op.fock(2, 0), ts, exp_ops=[op.sigmax(), op.sigmay(), op.sigmaz()], method=method,
+                          rtol=1e-10, atol=1e-10)
+        th = 2 * eps / w * np.sin(w * ts)
+        exact = np.stack([np.cos(th / 2), -1j * np.sin(th / 2)], axis=-1)
+        np.testing.assert_allclose(out['states'][..., 0], exact, atol=1e-8)
+        np.testing.assert_allclose(out['expects'], np.stack([0 * th, -np.sin(th), np.cos(th)]), atol=1e-8)
+
+
+def test_sesolve_pwc_matches_product_of_exponentials():
+    """H(t) = H0 + v(t) O with v piecewise constant on [0, 0.4), [0.4, 1.1) and zero
+    after 1.1 (PWCTimeQArray, time_qarray.py:572-666): psi(T) is the product of
+    the segment propagators expm(-1j H_k dt_k) (scipy.linalg.expm); the step
+    sequence lands on both breakpoints."""
+    from oracle import sesolve as ose
+
+    rng = np.random.default_rng(11)
+    n = 5
+    x = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    H0 = x + x.conj().T
+    y = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    O = y + y.conj().T
+    v = np.array([0.7, -1.3 + 0.0j])
+    H = om.TimeOp(H0, [om.PWCTerm(np.array([0.0, 0.4, 1.1]), v, O)])
+    psi0 = rng.normal(size=(n, 1)) + 1j * rng.normal(size=(n, 1))
+    psi0 /= np.linalg.norm(psi0)
+    ts = np.array([0.0, 0.2, 0.4, 0.9, 1.1, 1.5])
+    out = ose.sesolve(H, psi0, ts, rtol=1e-11, atol=1e-11)
+    expected = [
+        psi0,
+        sla.expm(-1j * (H0 + v[0] * O) * 0.2) @ psi0,
+        sla.expm(-1j * (H0 + v[0] * O) * 0.4) @ psi0,
+        sla.expm(-1j * (H0 + v[1] * O) * 0.5) @ sla.expm(-1j * (H0 + v[0] * O) * 0.4) @ psi0,
+    ]
+    p11 = sla.expm(-1j * (H0 + v[1] * O) * 0.7) @ sla.expm(-1j * (H0 + v[0] * O) * 0.4) @ psi0
+    expected += [p11, sla.expm(-1j * H0 * 0.4) @ p11]
+    np.testing.assert_allclose(out['states'], np.stack(expected), atol=2e-8)
+    ends = np.asarray(out['infos']['ends'])
+    assert np.any(ends == 0.4) and np.any(ends == 1.1)
+
+
+def test_sesolve_matches_mesolve_without_jump_operators():
+    """With no jump operators the master equation on |psi><psi| is the
+    Schrodinger equation: the two oracle drivers agree on states and <E>."""
+    from oracle import sesolve as ose
+
+    rng = np.random.default_rng(3)
+    n = 6
+    x = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    H = x + x.conj().T
+    E = np.diag(np.arange(n, dtype=complex))
+    psi0 = op.coherent(n, 0.8 + 0.3j)
+    ts = np.linspace(0.0, 0.7, 8)
+    se = ose.sesolve(H, psi0, ts, exp_ops=[E], rtol=1e-11, atol=1e-11)
+    me = om.mesolve(H, [], psi0, ts, exp_ops=[E], rtol=1e-11, atol=1e-11)
+    rho = np.einsum('tia,tja->tij', se['states'], se['states'].conj())
+    np.testing.assert_allclose(rho, me['states'], atol=1e-8)
+    np.testing.assert_allclose(se['expects'], me['expects'], atol=1e-8)
