@@ -1,12 +1,16 @@
 // Schrodinger equation d psi / dt = -i H(t) psi (sesolve) with adaptive Tsit5 or
 // Dopri5, for gfx950.
 //
-// One 256-thread workgroup integrates one instance (its own adaptive step). The FP64
-// state, the FP32 stage input and the seven FP32 stages live in LDS. The vector field is
-// a GEMV with G = -i H: thread i owns rows i, i + 256, ... and reads G^T (stored
-// transposed, so that consecutive threads read consecutive elements) from global
-// memory. Piecewise-constant and modulated terms are combined element by element.
-// The controller, error norm (RMS over the n entries), FSAL, step snapping, jump
+// One workgroup integrates one instance (its own adaptive step): one wave for
+// np <= 64, four waves up to np = 256. The FP64 state, the FP32 stage input and the
+// seven FP32 stages live in LDS. The vector field is a GEMV with G = -i H read as
+// G^T (stored transposed, so that consecutive rows are consecutive addresses) from
+// global memory, where a batch that shares H finds it in L2. In the one-wave
+// kernel each row is split over 64 / np lane groups (strided columns) and reduced
+// with cross-lane shuffles, so that every lane works for np = 16 and 32; the
+// four-wave kernel gives each thread whole rows. Piecewise-constant and modulated
+// terms are separate products combined with their coefficients in FP64. The
+// controller, error norm (RMS over the n entries), FSAL, step snapping, jump
 // handling and dense-output saves are those of the mesolve kernels.
 #include <hip/hip_runtime.h>
 
@@ -21,11 +25,16 @@
 namespace dqh {
 namespace se {
 
-constexpr int kT = 256;
-constexpr int kW = kT / 64;
-constexpr int kMaxNp = 256;
+// workgroup size T: the largest padded dimension it serves
+template <int T>
+struct SeCfg {
+  static constexpr int kW = T / 64;
+  static constexpr int kMaxNp = T == 64 ? 64 : 256;
+};
 
+template <int T>
 __device__ __forceinline__ double bsum(double v, double *red) {
+  constexpr int kW = SeCfg<T>::kW;
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   __syncthreads();
@@ -90,11 +99,11 @@ __device__ __forceinline__ void coefficients(const dqh_me_inst &p, int seg, doub
   }
 }
 
-// sum_j M[i][j] ys[j] with M^T stored, so that consecutive rows (threads) read
-// consecutive elements (FP32 products and sums)
-__device__ __forceinline__ float2 row_dot(const float2 *MT, int np, int i, const float2 *ys) {
+// sum over j = j0, j0 + js, ... of M[i][j] ys[j] with M^T stored, so that
+// consecutive rows (lanes) read consecutive elements (FP32 products and sums)
+__device__ __forceinline__ float2 row_dot(const float2 *MT, int np, int i, const float2 *ys, int j0, int js) {
   float ar = 0.f, ai = 0.f;
-  for (int j = 0; j < np; ++j) {
+  for (int j = j0; j < np; j += js) {
     const float2 g = MT[(size_t)j * np + i];
     const float2 v = ys[j];
     ar = fmaf(g.x, v.x, fmaf(-g.y, v.y, ar));
@@ -105,36 +114,53 @@ __device__ __forceinline__ float2 row_dot(const float2 *MT, int np, int i, const
 
 // out = G(t) ys, G = -i H(t) = G0 + sum_k c_k G_k: one product per term, combined
 // with the coefficients in FP64. G0^T is p.Hnh, the PWC terms' p.Hpwc and the
-// modulated terms' p.Hmod.
+// modulated terms' p.Hmod. Lane group q of the one-wave kernel sums the columns
+// j = q (mod 64 / np) of row i; the groups are reduced with shuffles.
+template <int T>
 __device__ __forceinline__ void field(const dqh_me_inst &p, const Coef &cf, const float2 *ys, float2 *out) {
   const int np = (int)p.np;
   const size_t nn = (size_t)np * np;
   const float2 *G0 = gp<const float2>(p.Hnh);
   const float2 *Gp = gp<const float2>(p.Hpwc);
   const float2 *Gm = gp<const float2>(p.Hmod);
-  for (int i = threadIdx.x; i < np; i += kT) {
-    const float2 a = row_dot(G0, np, i, ys);
-    double re = a.x, im = a.y;
+  const int nq = T == 64 ? 64 / np : 1;  // np <= 64 in the one-wave kernel
+  const int q = T == 64 ? (int)threadIdx.x / np : 0;
+  for (int i0 = 0; i0 < np; i0 += T) {
+    const int i = T == 64 ? (int)threadIdx.x - q * np : i0 + (int)threadIdx.x;
+    const bool row = q < nq && i < np;
+    double re = 0.0, im = 0.0;
+    if (row) {
+      const float2 a = row_dot(G0, np, i, ys, q, nq);
+      re = a.x;
+      im = a.y;
 #pragma unroll
-    for (int k = 0; k < 8 + DQH_MAX_MOD; ++k) {
-      const bool on = k < 8 ? k < cf.npwc : k - 8 < cf.nmod;
-      const double2 c = cf.c[k];
-      if (on && (c.x != 0.0 || c.y != 0.0)) {
-        const float2 r = row_dot(k < 8 ? Gp + (size_t)k * nn : Gm + (size_t)(k - 8) * nn, np, i, ys);
-        re += c.x * r.x - c.y * r.y;
-        im += c.x * r.y + c.y * r.x;
+      for (int k = 0; k < 8 + DQH_MAX_MOD; ++k) {
+        const bool on = k < 8 ? k < cf.npwc : k - 8 < cf.nmod;
+        const double2 c = cf.c[k];
+        if (on && (c.x != 0.0 || c.y != 0.0)) {
+          const float2 r = row_dot(k < 8 ? Gp + (size_t)k * nn : Gm + (size_t)(k - 8) * nn, np, i, ys, q, nq);
+          re += c.x * r.x - c.y * r.y;
+          im += c.x * r.y + c.y * r.x;
+        }
       }
     }
-    out[i] = make_float2((float)re, (float)im);
+    if (T == 64) {
+      for (int o = np; o < np * nq; o <<= 1) {
+        re += __shfl_xor(re, o, 64);
+        im += __shfl_xor(im, o, 64);
+      }
+    }
+    if (row && q == 0) out[i] = make_float2((float)re, (float)im);
   }
   __syncthreads();
 }
 
 // save y + h sum_j w_j k_j (state and <psi|E|psi>) at save index isave
-__device__ void save_at(const dqh_me_inst &p, const double2 *y, float2 (*k)[kMaxNp], double h, const double *w,
+template <int T>
+__device__ void save_at(const dqh_me_inst &p, const double2 *y, float2 (*k)[SeCfg<T>::kMaxNp], double h, const double *w,
                         bool interp, int isave, double2 *v, double *red) {
   const int n = (int)p.n, np = (int)p.np, nE = (int)p.nE, ntsave = (int)p.ntsave;
-  for (int i = threadIdx.x; i < np; i += kT) {
+  for (int i = threadIdx.x; i < np; i += T) {
     double re = y[i].x, im = y[i].y;
     if (interp)
       for (int j = 0; j < 7; ++j) {
@@ -150,7 +176,7 @@ __device__ void save_at(const dqh_me_inst &p, const double2 *y, float2 (*k)[kMax
   for (int qe = 0; qe < nE; ++qe) {
     // <v|E|v> = sum_i conj(v_i) sum_j E[i][j] v_j with E^T stored: E[i][j] = ET[j][i]
     double ar = 0.0, ai = 0.0;
-    for (int i = threadIdx.x; i < np; i += kT) {
+    for (int i = threadIdx.x; i < np; i += T) {
       double sr = 0.0, si = 0.0;
       for (int j = 0; j < np; ++j) {
         const float2 e = ET[(size_t)qe * np * np + (size_t)j * np + i];
@@ -160,14 +186,16 @@ __device__ void save_at(const dqh_me_inst &p, const double2 *y, float2 (*k)[kMax
       ar += v[i].x * sr + v[i].y * si;
       ai += v[i].x * si - v[i].y * sr;
     }
-    const double sr = bsum(ar, red), si = bsum(ai, red);
+    const double sr = bsum<T>(ar, red), si = bsum<T>(ai, red);
     if (threadIdx.x == 0) gp<float2>(p.expects)[(size_t)qe * ntsave + isave] = make_float2((float)sr, (float)si);
   }
   __syncthreads();
 }
 
-__global__ __launch_bounds__(kT) void k_sesolve(const dqh_me_inst *__restrict__ insts) {
-  __shared__ double red[kW];
+template <int T>
+__global__ __launch_bounds__(T) void k_sesolve(const dqh_me_inst *__restrict__ insts) {
+  constexpr int kMaxNp = SeCfg<T>::kMaxNp;
+  __shared__ double red[SeCfg<T>::kW];
   __shared__ double2 y[kMaxNp], y1[kMaxNp], vsave[kMaxNp];
   __shared__ float2 ys[kMaxNp];
   __shared__ float2 k[7][kMaxNp];
@@ -185,7 +213,7 @@ __global__ __launch_bounds__(kT) void k_sesolve(const dqh_me_inst *__restrict__ 
   const double rtol = p.rtol, atol = p.atol, inv_n = 1.0 / (double)n, t_end = p.t1;
   const double w0[7] = {0, 0, 0, 0, 0, 0, 0};
   const double2 *y0 = gp<const double2>(p.y);
-  for (int i = threadIdx.x; i < np; i += kT) {
+  for (int i = threadIdx.x; i < np; i += T) {
     y[i] = i < n ? y0[i] : make_double2(0.0, 0.0);
     for (int j = 0; j < 7; ++j) k[j][i] = make_float2(0.f, 0.f);
   }
@@ -194,41 +222,41 @@ __global__ __launch_bounds__(kT) void k_sesolve(const dqh_me_inst *__restrict__ 
   int isave = 0, status = 0;
   int64_t nsteps = 0, nacc = 0, nrej = 0, nrhs = 0;
   while (isave < ntsave && tsave[isave] <= t) {
-    save_at(p, y, k, 0.0, w0, false, isave, vsave, red);
+    save_at<T>(p, y, k, 0.0, w0, false, isave, vsave, red);
     ++isave;
   }
   int seg = pwc ? segment_of(ptimes, nseg, t) : -1;
   Coef cf;
   coefficients(p, seg, t, cf);
-  for (int i = threadIdx.x; i < np; i += kT) ys[i] = make_float2((float)y[i].x, (float)y[i].y);
+  for (int i = threadIdx.x; i < np; i += T) ys[i] = make_float2((float)y[i].x, (float)y[i].y);
   __syncthreads();
-  field(p, cf, ys, k[0]);
+  field<T>(p, cf, ys, k[0]);
   ++nrhs;
   // initial step (Hairer-Norsett-Wanner II.4), scale atol + rtol |y0|
   double dt;
   {
     double s0 = 0.0, s1 = 0.0;
-    for (int i = threadIdx.x; i < n; i += kT) {
+    for (int i = threadIdx.x; i < n; i += T) {
       const double ay2 = y[i].x * y[i].x + y[i].y * y[i].y;
       const double sc = atol + rtol * sqrt(ay2);
       s0 += ay2 / (sc * sc);
       s1 += ((double)k[0][i].x * k[0][i].x + (double)k[0][i].y * k[0][i].y) / (sc * sc);
     }
-    const double d0 = sqrt(bsum(s0, red) * inv_n), d1 = sqrt(bsum(s1, red) * inv_n);
+    const double d0 = sqrt(bsum<T>(s0, red) * inv_n), d1 = sqrt(bsum<T>(s1, red) * inv_n);
     const double h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
-    for (int i = threadIdx.x; i < np; i += kT)
+    for (int i = threadIdx.x; i < np; i += T)
       ys[i] = make_float2((float)(y[i].x + h0 * k[0][i].x), (float)(y[i].y + h0 * k[0][i].y));
     __syncthreads();
     if (smooth) coefficients(p, seg, t + h0, cf);
-    field(p, cf, ys, k[1]);
+    field<T>(p, cf, ys, k[1]);
     ++nrhs;
     double s2 = 0.0;
-    for (int i = threadIdx.x; i < n; i += kT) {
+    for (int i = threadIdx.x; i < n; i += T) {
       const double sc = atol + rtol * sqrt(y[i].x * y[i].x + y[i].y * y[i].y);
       const double dr = (double)k[1][i].x - k[0][i].x, di = (double)k[1][i].y - k[0][i].y;
       s2 += (dr * dr + di * di) / (sc * sc);
     }
-    const double d2 = sqrt(bsum(s2, red) * inv_n) / h0;
+    const double d2 = sqrt(bsum<T>(s2, red) * inv_n) / h0;
     const double md = fmax(d1, d2);
     const double h1 = md <= 1e-15 ? fmax(1e-6, h0 * 1e-3) : ctrl_pow(0.01 / md, 0.2);
     dt = fmin(100.0 * h0, h1);
@@ -245,9 +273,9 @@ __global__ __launch_bounds__(kT) void k_sesolve(const dqh_me_inst *__restrict__ 
     }
     if (need_k0) {
       coefficients(p, seg, t, cf);
-      for (int i = threadIdx.x; i < np; i += kT) ys[i] = make_float2((float)y[i].x, (float)y[i].y);
+      for (int i = threadIdx.x; i < np; i += T) ys[i] = make_float2((float)y[i].x, (float)y[i].y);
       __syncthreads();
-      field(p, cf, ys, k[0]);
+      field<T>(p, cf, ys, k[0]);
       ++nrhs;
       need_k0 = false;
     }
@@ -273,7 +301,7 @@ __global__ __launch_bounds__(kT) void k_sesolve(const dqh_me_inst *__restrict__ 
     }
     if (!smooth) coefficients(p, seg, t, cf);
     for (int s = 1; s < 7; ++s) {
-      for (int i = threadIdx.x; i < np; i += kT) {
+      for (int i = threadIdx.x; i < np; i += T) {
         double re = y[i].x, im = y[i].y;
         for (int j = 0; j < s; ++j) {
           re += h * rk.A[s][j] * k[j][i].x;
@@ -284,11 +312,11 @@ __global__ __launch_bounds__(kT) void k_sesolve(const dqh_me_inst *__restrict__ 
       }
       __syncthreads();
       if (smooth) coefficients(p, seg, t + rk.C[s] * h, cf);
-      field(p, cf, ys, k[s]);
+      field<T>(p, cf, ys, k[s]);
       ++nrhs;
     }
     double ssum = 0.0;
-    for (int i = threadIdx.x; i < n; i += kT) {
+    for (int i = threadIdx.x; i < n; i += T) {
       double er = 0.0, ei = 0.0;
       for (int j = 0; j < 7; ++j) {
         er += rk.E[j] * k[j][i].x;
@@ -300,7 +328,7 @@ __global__ __launch_bounds__(kT) void k_sesolve(const dqh_me_inst *__restrict__ 
                                            sqrt(y1[i].x * y1[i].x + y1[i].y * y1[i].y));
       ssum += (er * er + ei * ei) / (sc * sc);
     }
-    const double norm = sqrt(bsum(ssum, red) * inv_n);
+    const double norm = sqrt(bsum<T>(ssum, red) * inv_n);
     const bool finite = isfinite(norm), keep = finite && norm < 1.0;
     double factor = !finite ? p.factormin : (norm == 0.0 ? p.factormax : p.safety * ctrl_pow(norm, -0.2));
     factor = fmin(fmax(factor, keep ? 1.0 : p.factormin), p.factormax);
@@ -312,10 +340,10 @@ __global__ __launch_bounds__(kT) void k_sesolve(const dqh_me_inst *__restrict__ 
         const double th = (tsave[isave] - t) / h;
         double w[7];
         for (int j = 0; j < 7; ++j) w[j] = th * (rk.R[j][0] + th * (rk.R[j][1] + th * (rk.R[j][2] + th * rk.R[j][3])));
-        save_at(p, y, k, h, w, true, isave, vsave, red);
+        save_at<T>(p, y, k, h, w, true, isave, vsave, red);
         ++isave;
       }
-      for (int i = threadIdx.x; i < np; i += kT) {
+      for (int i = threadIdx.x; i < np; i += T) {
         y[i] = y1[i];
         k[0][i] = k[6][i];
       }
@@ -328,10 +356,10 @@ __global__ __launch_bounds__(kT) void k_sesolve(const dqh_me_inst *__restrict__ 
     dt = factor * h;
   }
   while (isave < ntsave) {
-    save_at(p, y, k, 0.0, w0, false, isave, vsave, red);
+    save_at<T>(p, y, k, 0.0, w0, false, isave, vsave, red);
     ++isave;
   }
-  for (int i = threadIdx.x; i < n; i += kT) {
+  for (int i = threadIdx.x; i < n; i += T) {
     const float2 v = make_float2((float)y[i].x, (float)y[i].y);
     gp<float2>(p.final_state)[i] = v;
     if (!(p.flags & DQH_FLAG_SAVE_STATES)) gp<float2>(p.states)[i] = v;
@@ -348,9 +376,13 @@ __global__ __launch_bounds__(kT) void k_sesolve(const dqh_me_inst *__restrict__ 
 }  // namespace se
 }  // namespace dqh
 
-extern "C" int dqh_sesolve_tsit5(const dqh_me_inst *d_insts, int64_t ninst, void *stream) {
-  if (!d_insts || ninst <= 0 || ninst > (1 << 30)) return dqh::fail(DQH_BAD_ARG, "dqh_sesolve_tsit5: bad arguments");
-  hipLaunchKernelGGL(dqh::se::k_sesolve, dim3((unsigned)ninst), dim3(dqh::se::kT), 0, (hipStream_t)stream, d_insts);
+extern "C" int dqh_sesolve_tsit5(const dqh_me_inst *d_insts, int64_t ninst, int64_t np_max, void *stream) {
+  if (!d_insts || ninst <= 0 || ninst > (1 << 30) || np_max <= 0 || np_max > 256)
+    return dqh::fail(DQH_BAD_ARG, "dqh_sesolve_tsit5: bad arguments");
+  if (np_max <= 64)
+    hipLaunchKernelGGL(dqh::se::k_sesolve<64>, dim3((unsigned)ninst), dim3(64), 0, (hipStream_t)stream, d_insts);
+  else
+    hipLaunchKernelGGL(dqh::se::k_sesolve<256>, dim3((unsigned)ninst), dim3(256), 0, (hipStream_t)stream, d_insts);
   const hipError_t err = hipGetLastError();
   if (err != hipSuccess) return dqh::fail(DQH_HIP_ERROR, std::string("dqh_sesolve_tsit5: ") + hipGetErrorString(err));
   return DQH_OK;

@@ -62,7 +62,7 @@ class SEPlan:
         lib = _native.lib()
         s = stream if stream is not None else torch.cuda.current_stream()
         _native.check(
-            lib.dqh_sesolve_tsit5(_native.ctypes.c_void_p(self.d_descs.data_ptr()), self.ninst,
+            lib.dqh_sesolve_tsit5(_native.ctypes.c_void_p(self.d_descs.data_ptr()), self.ninst, self.np,
                                   _native.ctypes.c_void_p(s.cuda_stream)),
             'dqh_sesolve_tsit5',
         )
@@ -88,6 +88,80 @@ class SEPlan:
         return out
 
 
+def _header(n: int, tsave, Es, t0, rtol, atol, safety, factormin, factormax, max_steps, save_states, tableau,
+            B: int, arena):
+    """Descriptor columns common to a batch (arena offsets for the shared inputs)."""
+    if tableau not in ('tsit5', 'dopri5'):
+        raise ValueError(f'unknown Runge-Kutta pair {tableau!r}')
+    npad = pad16(n)
+    if npad > MAX_NP:
+        raise ValueError(f'the MI355X sesolve supports n <= {MAX_NP} (got n={n})')
+    descs = np.zeros(B, dtype=_native.ME_INST_DTYPE)
+    descs['n'], descs['np'], descs['nE'], descs['ntsave'] = n, npad, len(Es), len(tsave)
+    descs['rtol'], descs['atol'], descs['safety'] = rtol, atol, safety
+    descs['factormin'], descs['factormax'], descs['max_steps'] = factormin, factormax, max_steps
+    descs['t0'], descs['t1'] = float(tsave[0] if t0 is None else t0), float(tsave[-1])
+    descs['tsave'] = arena.put(tsave)
+    descs['EopsT'] = arena.put(_pad(np.stack([E.T for E in Es]), npad).astype(np.complex64)) if Es else 0
+    flags = _native.FLAG_SAVE_STATES if save_states else 0
+    if tableau == 'dopri5':
+        flags |= _native.FLAG_DOPRI5
+    descs['flags'] = flags
+    return descs
+
+
+def _finish(plan: SEPlan, descs, arena, y0: np.ndarray, dev, step_log_cap: int) -> SEPlan:
+    """Upload the staged inputs, allocate the state and the outputs, fill the
+    pointer columns, check every pointer against the allocations."""
+    import torch
+
+    B, npad, n, nt, nE = plan.ninst, plan.np, plan.n, plan.ntsave, plan.nE
+    arena.upload()
+    u = np.uint64
+    for f in SE_FIELDS:
+        v = descs[f]
+        descs[f] = np.where(v != 0, v + u(arena.base) - u(1), u(0))
+    plan.y0 = torch.from_numpy(np.ascontiguousarray(y0).view(np.float64).reshape(B, npad, 2)).to(dev)
+    plan.y = plan.y0.clone()
+    ns = nt if plan.save_states else 1
+    plan.states = torch.empty((B, ns * n * 2), dtype=torch.float32, device=dev)
+    plan.final = torch.empty((B, n * 2), dtype=torch.float32, device=dev)
+    plan.expects = torch.zeros((B, max(nE, 1) * nt * 2), dtype=torch.float32, device=dev)
+    plan.stats = torch.zeros((B, 8), dtype=torch.int64, device=dev)
+    idx = np.arange(B, dtype=u)
+    descs['y'] = u(plan.y.data_ptr()) + idx * u(npad * 16)
+    descs['states'] = u(plan.states.data_ptr()) + idx * u(ns * n * 8)
+    descs['final_state'] = u(plan.final.data_ptr()) + idx * u(n * 8)
+    descs['expects'] = u(plan.expects.data_ptr()) + idx * u(max(nE, 1) * nt * 8)
+    descs['stats'] = u(plan.stats.data_ptr()) + idx * u(64)
+    if step_log_cap > 0:
+        plan.step_log = torch.zeros((B, int(step_log_cap)), dtype=torch.float64, device=dev)
+        descs['step_log'] = u(plan.step_log.data_ptr()) + idx * u(int(step_log_cap) * 8)
+        descs['step_log_cap'] = int(step_log_cap)
+    npd = descs['np'].astype(u)
+    extents = {
+        'y': npd * u(16),
+        'states': np.full(B, ns * n * 8, dtype=u),
+        'final_state': np.full(B, n * 8, dtype=u),
+        'Hpwc': descs['npwc'].astype(u) * npd * npd * u(8),
+        'pwc_times': descs['nseg'].astype(u) * u(8),
+        'pwc_vals': descs['npwc'].astype(u) * np.maximum(descs['nseg'].astype(np.int64) - 1, 0).astype(u) * u(16),
+    }
+    tensors = [*arena.tensors, plan.y, plan.states, plan.final, plan.expects, plan.stats]
+    if plan.step_log is not None:
+        tensors.append(plan.step_log)
+    check_descriptors(descs, tensors, extents)
+    plan.tensors = tensors
+    plan.descs = descs
+    plan.d_descs = torch.from_numpy(descs.view(np.uint8).copy()).to(dev)
+    return plan
+
+
+def _G_T(ops, npad: int) -> np.ndarray:
+    """(-i O)^T of each operator, zero-padded to npad, complex64."""
+    return _pad(np.swapaxes(-1j * np.asarray(ops), -1, -2), npad).astype(np.complex64)
+
+
 def build_plan(
     instances: list,
     tsave: np.ndarray,
@@ -111,16 +185,9 @@ def build_plan(
     as ``E^T``. PWC terms are resampled on the union of their breakpoints;
     modulated terms become piecewise-cubic tables over [t0, tsave[-1]] with one
     interval count per instance."""
-    import torch
-
-    if tableau not in ('tsit5', 'dopri5'):
-        raise ValueError(f'unknown Runge-Kutta pair {tableau!r}')
     n = instances[0].H.shape[-1]
     if any(inst.H.shape[-1] != n for inst in instances):
         raise ValueError('all instances of a sesolve plan must share the Hilbert dimension')
-    npad = pad16(n)
-    if npad > MAX_NP:
-        raise ValueError(f'the MI355X sesolve supports n <= {MAX_NP} (got n={n})')
     if any(len(inst.pwc) > MAX_PWC for inst in instances):
         raise NotImplementedError(f'at most {MAX_PWC} piecewise-constant terms per Hamiltonian are supported')
     if any(len(inst.mod) > _native.MAX_MOD for inst in instances):
@@ -128,17 +195,14 @@ def build_plan(
     dev = _device()
     _native.lib()
     tsave = np.asarray(tsave, dtype=np.float64)
-    B, nt = len(instances), len(tsave)
+    B = len(instances)
     Es = [np.asarray(E, dtype=np.complex128) for E in (exp_ops or [])]
-    nE = len(Es)
-    t_start = float(tsave[0] if t0 is None else t0)
-    t_end = float(tsave[-1])
-    plan = SEPlan(n, nt, nE, B, save_states)
     arena = _Arena(dev)
-
-    def G_T(ops):
-        return _pad(np.swapaxes(-1j * np.asarray(ops), -1, -2), npad).astype(np.complex64)
-
+    descs = _header(n, tsave, Es, t0, rtol, atol, safety, factormin, factormax, max_steps, save_states, tableau, B,
+                    arena)
+    npad = int(descs['np'][0])
+    t_start, t_end = float(descs['t0'][0]), float(descs['t1'][0])
+    plan = SEPlan(n, len(tsave), len(Es), B, save_states)
     cache = {}
 
     def shared(key, build):
@@ -148,26 +212,16 @@ def build_plan(
             cache[key] = arena.put(build())
         return cache[key]
 
-    descs = np.zeros(B, dtype=_native.ME_INST_DTYPE)
-    descs['n'], descs['np'], descs['nE'], descs['ntsave'] = n, npad, nE, nt
-    descs['rtol'], descs['atol'], descs['safety'] = rtol, atol, safety
-    descs['factormin'], descs['factormax'], descs['max_steps'] = factormin, factormax, max_steps
-    descs['t0'], descs['t1'] = t_start, t_end
-    descs['tsave'] = arena.put(tsave)
-    descs['EopsT'] = arena.put(_pad(np.stack([E.T for E in Es]), npad).astype(np.complex64)) if nE else 0
-    flags = _native.FLAG_SAVE_STATES if save_states else 0
-    if tableau == 'dopri5':
-        flags |= _native.FLAG_DOPRI5
-    descs['flags'] = flags
     tables = {}
     for b, inst in enumerate(instances):
         d = descs[b]
-        d['Hnh'] = shared(('H', inst.key_H), lambda inst=inst: G_T(inst.H))
+        d['Hnh'] = shared(('H', inst.key_H), lambda inst=inst: _G_T(inst.H, npad))
         if inst.pwc:
             grid = union_breakpoints([inst.pwc])
             vals = np.stack([resample_pwc(t, np.asarray(v, dtype=np.complex128), grid) for (t, v, _) in inst.pwc])
             d['npwc'], d['nseg'] = len(inst.pwc), len(grid)
-            d['Hpwc'] = shared(('Hpwc', inst.key_H), lambda inst=inst: G_T(np.stack([o for (_, _, o) in inst.pwc])))
+            d['Hpwc'] = shared(('Hpwc', inst.key_H),
+                               lambda inst=inst: _G_T(np.stack([o for (_, _, o) in inst.pwc]), npad))
             d['pwc_times'] = shared(('ptimes', inst.key_H), lambda grid=grid: grid)
             d['pwc_vals'] = arena.put(np.ascontiguousarray(vals).view(np.float64))
         if inst.mod:
@@ -187,51 +241,63 @@ def build_plan(
             d['nmod'] = len(inst.mod)
             d['mod_t0'], d['mod_dt'], d['mod_nint'] = tabs[0][0], tabs[0][1], nint
             d['mod_coef'] = arena.put(np.ascontiguousarray(np.stack([c for (_, _, c) in tabs])).view(np.float64))
-            d['Hmod'] = shared(('Hmod', inst.key_H), lambda inst=inst: G_T(np.stack([o for (_, o, _) in inst.mod])))
-    arena.upload()
-    for f in SE_FIELDS:
-        descs[f] = [arena.patch(int(v)) for v in descs[f]]
-
+            d['Hmod'] = shared(('Hmod', inst.key_H),
+                               lambda inst=inst: _G_T(np.stack([o for (_, o, _) in inst.mod]), npad))
     y0 = np.zeros((B, npad), dtype=np.complex128)
     for b, inst in enumerate(instances):
         y0[b, :n] = inst.psi0
-    plan.y0 = torch.from_numpy(y0.view(np.float64).reshape(B, npad, 2)).to(dev)
-    plan.y = plan.y0.clone()
-    ns = nt if save_states else 1
-    plan.states = torch.empty((B, ns * n * 2), dtype=torch.float32, device=dev)
-    plan.final = torch.empty((B, n * 2), dtype=torch.float32, device=dev)
-    plan.expects = torch.zeros((B, max(nE, 1) * nt * 2), dtype=torch.float32, device=dev)
-    plan.stats = torch.zeros((B, 8), dtype=torch.int64, device=dev)
-    for b in range(B):
-        d = descs[b]
-        d['y'] = plan.y.data_ptr() + b * npad * 16
-        d['states'] = plan.states.data_ptr() + b * ns * n * 8
-        d['final_state'] = plan.final.data_ptr() + b * n * 8
-        d['expects'] = plan.expects.data_ptr() + b * max(nE, 1) * nt * 8
-        d['stats'] = plan.stats.data_ptr() + b * 64
-    if step_log_cap > 0:
-        plan.step_log = torch.zeros((B, int(step_log_cap)), dtype=torch.float64, device=dev)
-        for b in range(B):
-            descs[b]['step_log'] = plan.step_log.data_ptr() + b * int(step_log_cap) * 8
-            descs[b]['step_log_cap'] = int(step_log_cap)
-    u = np.uint64
-    npd = descs['np'].astype(u)
-    extents = {
-        'y': npd * u(16),
-        'states': np.full(B, ns * n * 8, dtype=u),
-        'final_state': np.full(B, n * 8, dtype=u),
-        'Hpwc': descs['npwc'].astype(u) * npd * npd * u(8),
-        'pwc_times': descs['nseg'].astype(u) * u(8),
-        'pwc_vals': descs['npwc'].astype(u) * np.maximum(descs['nseg'].astype(np.int64) - 1, 0).astype(u) * u(16),
-    }
-    tensors = [*arena.tensors, plan.y, plan.states, plan.final, plan.expects, plan.stats]
-    if plan.step_log is not None:
-        tensors.append(plan.step_log)
-    check_descriptors(descs, tensors, extents)
-    plan.tensors = tensors
-    plan.descs = descs
-    plan.d_descs = torch.from_numpy(descs.view(np.uint8).copy()).to(dev)
-    return plan
+    return _finish(plan, descs, arena, y0, dev, step_log_cap)
+
+
+def build_plan_constant(
+    Hs: np.ndarray,
+    h_index: np.ndarray,
+    psis: np.ndarray,
+    p_index: np.ndarray,
+    tsave: np.ndarray,
+    *,
+    exp_ops=None,
+    t0: float | None = None,
+    rtol: float = 1e-6,
+    atol: float = 1e-6,
+    safety: float = 0.9,
+    factormin: float = 0.2,
+    factormax: float = 5.0,
+    max_steps: int = 100_000,
+    save_states: bool = True,
+    tableau: str = 'tsit5',
+    step_log_cap: int = 0,
+) -> SEPlan:
+    """The same plan for constant Hamiltonians, packed as arrays: instance b
+    integrates ``Hs[h_index[b]]`` (``Hs``: (nH, n, n)) from ``psis[p_index[b]]``
+    (``psis``: (nP, n)). One staged block holds every (-i H)^T; the descriptor
+    columns are filled without a per-instance loop, so that a batch of thousands
+    of small systems packs in milliseconds."""
+    Hs = np.asarray(Hs, dtype=np.complex128)
+    psis = np.asarray(psis, dtype=np.complex128)
+    h_index = np.asarray(h_index, dtype=np.int64).reshape(-1)
+    p_index = np.asarray(p_index, dtype=np.int64).reshape(-1)
+    if h_index.shape != p_index.shape or h_index.size == 0:
+        raise ValueError('h_index and p_index must be non-empty and of equal length')
+    if np.any((h_index < 0) | (h_index >= len(Hs))) or np.any((p_index < 0) | (p_index >= len(psis))):
+        raise ValueError('instance index out of range')
+    n = Hs.shape[-1]
+    if Hs.shape[-2:] != (n, n) or psis.shape[-1] != n:
+        raise ValueError('Hs must be (nH, n, n) and psis (nP, n)')
+    dev = _device()
+    _native.lib()
+    tsave = np.asarray(tsave, dtype=np.float64)
+    B = len(h_index)
+    Es = [np.asarray(E, dtype=np.complex128) for E in (exp_ops or [])]
+    arena = _Arena(dev)
+    descs = _header(n, tsave, Es, t0, rtol, atol, safety, factormin, factormax, max_steps, save_states, tableau, B,
+                    arena)
+    npad = int(descs['np'][0])
+    base = arena.put(_G_T(Hs, npad))
+    descs['Hnh'] = np.uint64(base) + h_index.astype(np.uint64) * np.uint64(npad * npad * 8)
+    y0 = np.zeros((B, npad), dtype=np.complex128)
+    y0[:, :n] = psis[p_index]
+    return _finish(SEPlan(n, len(tsave), len(Es), B, save_states), descs, arena, y0, dev, step_log_cap)
 
 
 def _check_sesolve_args(H: TimeQArray, psi0: QArray, exp_ops) -> None:
@@ -270,6 +336,29 @@ def enumerate_instances(H: TimeQArray, psi0: QArray, cartesian: bool):
     return out_shape, insts
 
 
+def constant_batch(Ht, psi0: QArray, cartesian: bool):
+    """Batch of a constant Hamiltonian as arrays: (out_shape, Hs (nH, n, n),
+    h_index, psis (nP, n), p_index) with instance b = (Hs[h_index[b]],
+    psis[p_index[b]]), in the order of ``enumerate_instances``."""
+    n = Ht.n
+    Hb = tuple(Ht.batch)
+    const = Ht.const if Ht.const is not None else np.zeros((*Hb, n, n), dtype=np.complex128)
+    Hs = np.ascontiguousarray(const).reshape(-1, n, n)
+    p = psi0.to_numpy().astype(np.complex128)
+    Pb = p.shape[:-2]
+    psis = p.reshape(-1, n)
+    nH, nP = len(Hs), len(psis)
+    if cartesian:
+        out_shape = Hb + tuple(Pb)
+        h_index = np.repeat(np.arange(nH), nP)
+        p_index = np.tile(np.arange(nP), nH)
+    else:
+        out_shape = tuple(np.broadcast_shapes(Hb, Pb))
+        h_index = np.broadcast_to(np.arange(nH).reshape(Hb), out_shape).reshape(-1)
+        p_index = np.broadcast_to(np.arange(nP).reshape(Pb), out_shape).reshape(-1)
+    return out_shape, Hs, h_index, psis, p_index
+
+
 def sesolve(
     H,
     psi0,
@@ -303,12 +392,8 @@ def sesolve(
     assert_method_supported(method, SUPPORTED_METHODS)
     method.assert_supports_gradient(gradient)
 
-    out_shape, insts = enumerate_instances(H, psi0, options.cartesian_batching)
     Es = [E.to_numpy() for E in exp_ops] if exp_ops is not None else None
-    save_all = options.save_states or options.save_extra is not None
-    plan = build_plan(
-        insts,
-        tsave,
+    kw = dict(
         exp_ops=Es,
         t0=None if options.t0 is None else float(options.t0),
         rtol=method.rtol,
@@ -317,9 +402,16 @@ def sesolve(
         factormin=method.min_factor,
         factormax=method.max_factor,
         max_steps=method.max_steps,
-        save_states=save_all,
+        save_states=options.save_states or options.save_extra is not None,
         tableau='dopri5' if isinstance(method, Dopri5) else 'tsit5',
     )
+    Ht = decompose(H)
+    if Ht.is_constant:
+        out_shape, Hs, h_index, psis, p_index = constant_batch(Ht, psi0, options.cartesian_batching)
+        plan = build_plan_constant(Hs, h_index, psis, p_index, tsave, **kw)
+    else:
+        out_shape, insts = enumerate_instances(H, psi0, options.cartesian_batching)
+        plan = build_plan(insts, tsave, **kw)
     plan.run()
     res = plan.results()
     if np.any(res['status'] == -3):

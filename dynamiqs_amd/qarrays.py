@@ -268,12 +268,25 @@ class QArray:
                 return _dia_elmul(self, y)
             return self._binary_dense(y, np.multiply)
         y = np.asarray(y)
+        # a scalar or a batch of scalars of shape (..., 1, 1), broadcast against the
+        # qarray (is_batched_scalar, dynamiqs/_utils.py:48-57; qarray.py:496-504)
+        if not (y.ndim == 0 or y.shape == (1,) or (y.ndim > 1 and y.shape[-2:] == (1, 1))):
+            raise NotImplementedError(
+                'Element-wise multiplication of two qarrays with the `*` operator is '
+                'not supported. For matrix multiplication, use `x @ y`. For '
+                'element-wise multiplication, use `x.elmul(y)`.'
+            )
         if self._data is not None:
-            return QArray(self._data * (y[..., None, None] if y.ndim else y), dims=self.dims)
-        d = self._diags * (y[..., None, None] if y.ndim else y)
-        return QArray(offsets=self._offsets, diags=d, dims=self.dims)
+            return QArray(self._data * y, dims=self.dims)
+        return QArray(offsets=self._offsets, diags=self._diags * y, dims=self.dims)
 
     __rmul__ = __mul__
+
+    def elmul(self, y) -> QArray:
+        """Element-wise product with a qarray-like (qarray.py ``elmul``)."""
+        if isinstance(y, QArray) and self._data is None and y._data is None:
+            return _dia_elmul(self, y)
+        return self._binary_dense(y, np.multiply)
 
     def __truediv__(self, y) -> QArray:
         return self * (1 / np.asarray(y))

@@ -111,7 +111,7 @@ class PWCTimeQArray(TimeQArray):
         return self.values[..., idx]
 
     def __call__(self, t: float) -> QArray:
-        return self.qarray * self.prefactor(t)
+        return self.qarray * np.asarray(self.prefactor(t))[..., None, None]
 
     def __mul__(self, y):
         return PWCTimeQArray(self.times, self.values, self.qarray * y)
@@ -137,7 +137,7 @@ class ModulatedTimeQArray(TimeQArray):
         return self.qarray.dims
 
     def __call__(self, t: float) -> QArray:
-        return self.qarray * np.asarray(self.f(t))
+        return self.qarray * np.asarray(self.f(t))[..., None, None]
 
     _disc_ts = np.zeros(0)
 

@@ -228,3 +228,40 @@ def test_sesolve_batching_shapes():
     np.testing.assert_array_equal(insts[3].H, 3 * np.eye(n))
     np.testing.assert_array_equal(insts[3].psi0, [0, 1, 0])
     np.testing.assert_array_equal(insts[2].psi0, [1, 0, 0])
+
+
+def test_sesolve_constant_batch_matches_instance_order():
+    """The array packing of a constant batch (one staged block of Hamiltonians,
+    index columns) lists the same (H, psi0) pairs in the same order as the
+    per-instance enumeration, for cartesian and broadcast batching."""
+    from dynamiqs_amd.integrators.operators import decompose
+    from dynamiqs_amd.integrators.sesolve import constant_batch, enumerate_instances
+
+    rng = np.random.default_rng(2)
+    n = 3
+    Hs = dq.asqarray(rng.normal(size=(2, 1, n, n)) + 0j)
+    psis = dq.asqarray(rng.normal(size=(3, n, 1)) + 1j * rng.normal(size=(3, n, 1)))
+    for cart in (True, False):
+        H = dq.constant(Hs)
+        shape, insts = enumerate_instances(H, psis, cart)
+        shape2, Hflat, hi, pflat, pi = constant_batch(decompose(H), psis, cart)
+        assert shape == shape2 and len(insts) == len(hi) == len(pi)
+        for b, inst in enumerate(insts):
+            np.testing.assert_array_equal(Hflat[hi[b]], inst.H)
+            np.testing.assert_array_equal(pflat[pi[b]], inst.psi0)
+
+
+def test_batched_scalar_multiplication():
+    """A qarray times a scalar or a batch of scalars of shape (..., 1, 1) broadcasts
+    (dynamiqs/_utils.py:48-57, qarray.py:496-504), as in the batching tutorial's
+    omega[:, None, None] * sigma_z + epsilon[:, None, None, None] * sigma_x; any
+    other array raises NotImplementedError."""
+    omega = np.linspace(0.0, 10.0, 100)[:, None, None]
+    epsilon = np.linspace(0.0, 1.0, 30)[:, None, None, None]
+    H = omega * dq.sigmaz() + epsilon * dq.sigmax()
+    assert H.shape == (30, 100, 2, 2)
+    np.testing.assert_allclose(H.to_numpy()[3, 7], omega[7, 0, 0] * np.diag([1, -1]) + epsilon[3, 0, 0, 0] * np.array([[0, 1], [1, 0]]))
+    assert (np.ones((4, 1, 1)) * dq.destroy(3, layout=dq.dia)).shape == (4, 3, 3)
+    with pytest.raises(NotImplementedError, match='Element-wise multiplication of two qarrays'):
+        dq.sigmaz() * np.ones(3)
+    np.testing.assert_array_equal(dq.sigmaz().elmul(dq.sigmaz()).to_numpy(), np.eye(2))
