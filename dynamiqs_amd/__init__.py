@@ -1,7 +1,8 @@
 """dynamiqs_amd -- MI355X-native (gfx950) drop-in for ``dynamiqs.mesolve``.
 
 ``import dynamiqs_amd as dq`` exposes the subset of the dynamiqs API that the
-``mesolve`` hot path uses: ``dq.mesolve`` and ``dq.sesolve``, quantum arrays
+``mesolve`` hot path uses: ``dq.mesolve``, ``dq.sesolve`` and the propagators
+``dq.sepropagator`` / ``dq.mepropagator``, quantum arrays
 (dense and sparse DIA), time-dependent operators (``constant``, ``pwc``,
 ``modulated``), operator/state builders, ``Options``, the methods
 ``Tsit5`` / ``Dopri5`` / ``Rouchon1`` / ``Rouchon2`` and the result classes.
@@ -10,10 +11,10 @@ The solve runs in hand-written HIP kernels (``libdqh.so``, C-ABI in
 """
 
 from . import gradient, method
-from .integrators import mesolve, sesolve
+from .integrators import mepropagator, mesolve, sepropagator, sesolve
 from .options import Options
 from .qarrays import QArray, asqarray, dense, dia, set_layout, set_precision
-from .result import MESolveResult, SESolveResult
+from .result import MEPropagatorResult, MESolveResult, SEPropagatorResult, SESolveResult
 from .time_qarray import TimeQArray, constant, modulated, pwc, timecallable
 from .utils import (
     coherent,

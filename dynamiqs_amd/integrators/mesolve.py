@@ -235,8 +235,8 @@ def mesolve(
     if options.save_extra is not None:
         extra = save_extra_outputs(options.save_extra, states, rho0.dims, out_shape)
     states = states.reshape(*out_shape, -1, n, n)
-    if not options.save_states:  # only the final state, without a time axis (save_mixin.py:62-66)
-        states = states[..., -1, :, :]
+    if not options.save_states:  # the final state only, nsave = 1 (save_mixin.py:62-66, mesolve.py:184-187)
+        states = states[..., -1:, :, :]
     ysave = QArray(states, dims=rho0.dims)
     Esave = None
     if Es is not None:
@@ -386,8 +386,8 @@ def _mesolve_dia(route, rho0: QArray, tsave, exp_ops, method, gradient, options)
     if options.save_extra is not None:
         extra = save_extra_outputs(options.save_extra, states, rho0.dims, out_shape)
     states = states.reshape(*out_shape, -1, n, n)
-    if not options.save_states:  # only the final state, without a time axis (save_mixin.py:62-66)
-        states = states[..., -1, :, :]
+    if not options.save_states:  # the final state only, nsave = 1 (save_mixin.py:62-66, mesolve.py:184-187)
+        states = states[..., -1:, :, :]
     ysave = QArray(states, dims=rho0.dims)
     Esave = None
     if Es is not None:

@@ -427,8 +427,8 @@ def sesolve(
     if options.save_extra is not None:
         extra = save_extra_outputs(options.save_extra, states, psi0.dims, out_shape)
     states = states.reshape(*out_shape, -1, n, 1)
-    if not options.save_states:  # only the final state, without a time axis (save_mixin.py:62-66)
-        states = states[..., -1, :, :]
+    if not options.save_states:  # the final state only, nsave = 1 (save_mixin.py:62-66, mesolve.py:184-187)
+        states = states[..., -1:, :, :]
     ysave = QArray(states, dims=psi0.dims)
     Esave = res['expects'].reshape(*out_shape, len(Es), len(tsave)) if Es is not None else None
 

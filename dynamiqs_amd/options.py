@@ -63,6 +63,8 @@ _SUPPORTED = {
         'vectorized',
         'assume_hermitian',
     ),
+    'sepropagator': ('save_propagators', 'progress_meter', 't0', 'save_extra'),
+    'mepropagator': ('save_propagators', 'cartesian_batching', 't0', 'save_extra'),
 }
 
 

@@ -125,3 +125,30 @@ class SESolveResult(SolveResult):
     or None, ``infos``, ``tsave``, ``method``, ``gradient``, ``options``
     (``dynamiqs/result.py:189-190``).
     """
+
+
+class PropagatorResult(Result):
+    """``propagators`` (..., nsave, N, N) with nsave = ntsave, or 1 without
+    ``save_propagators``; ``final_propagator`` (``dynamiqs/result.py:149-162``)."""
+
+    @property
+    def propagators(self) -> QArray:
+        return self._ysave
+
+    @property
+    def final_propagator(self) -> QArray:
+        return self.propagators[..., -1, :, :]
+
+    def _str_parts(self) -> dict:
+        d = super()._str_parts()
+        d['Propagators'] = _array_str(self.propagators)
+        return d
+
+
+class SEPropagatorResult(PropagatorResult):
+    """Result of :func:`dynamiqs_amd.sepropagator`: N = n (``dynamiqs/result.py:213-214``)."""
+
+
+class MEPropagatorResult(PropagatorResult):
+    """Result of :func:`dynamiqs_amd.mepropagator`: N = n^2, columns in the
+    column-stacking order of ``dq.vectorize`` (``dynamiqs/result.py:217-218``)."""

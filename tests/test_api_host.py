@@ -265,3 +265,46 @@ def test_batched_scalar_multiplication():
     with pytest.raises(NotImplementedError, match='Element-wise multiplication of two qarrays'):
         dq.sigmaz() * np.ones(3)
     np.testing.assert_array_equal(dq.sigmaz().elmul(dq.sigmaz()).to_numpy(), np.eye(2))
+
+
+def test_superoperator_assembly_from_hermitian_basis():
+    """mepropagator's host assembly: the images of the n^2 Hermitian basis matrices
+    under a random linear map S (given as an n^2 x n^2 matrix in the column-stacking
+    convention) give back S exactly, for two save times and two batch elements."""
+    from dynamiqs_amd.integrators.propagators import hermitian_basis, superoperator_from_basis
+
+    rng = np.random.default_rng(9)
+    n = 4
+    basis = hermitian_basis(n)
+    assert basis.shape == (n * n, n, n)
+    for b in basis:
+        np.testing.assert_array_equal(b, b.conj().T)
+    # the basis spans the matrices: rank n^2 as vectors
+    assert np.linalg.matrix_rank(basis.reshape(n * n, -1)) == n * n
+    S = rng.normal(size=(2, 2, n * n, n * n)) + 1j * rng.normal(size=(2, 2, n * n, n * n))
+    vec = lambda x: np.swapaxes(x, -1, -2).reshape(*x.shape[:-2], -1)  # noqa: E731
+    unvec = lambda v: np.swapaxes(v.reshape(*v.shape[:-1], n, n), -1, -2)  # noqa: E731
+    phi = np.stack([unvec(np.einsum('btrc,c->btr', S, vec(bm))) for bm in basis], axis=1)  # (B, n^2, ns, n, n)
+    np.testing.assert_allclose(superoperator_from_basis(phi), S, atol=1e-12)
+
+
+def test_propagator_argument_errors():
+    """sepropagator / mepropagator checks (sepropagator.py:261-263,
+    mepropagator.py:309-321, options.py:80-123) and the method registry."""
+    a = dq.destroy(3)
+    H, ts = a.dag() @ a, np.linspace(0.0, 1.0, 3)
+    with pytest.raises(ValueError, match=re.escape('Argument `H` must have shape (...H, n, n)')):
+        dq.sepropagator(np.ones((3, 2)), ts)
+    with pytest.raises(ValueError, match=re.escape('Option `cartesian_batching` was set to `False` but is not used by the solver `dq.sepropagator()`')):
+        dq.sepropagator(H, ts, options=dq.Options(cartesian_batching=False))
+    with pytest.raises(ValueError, match=re.escape('Option `save_states` was set to `False` but is not used by the solver `dq.mepropagator()`')):
+        dq.mepropagator(H, [a], ts, options=dq.Options(save_states=False))
+    with pytest.raises(TypeError, match=re.escape('Method of type `Expm` is not supported')):
+        dq.mepropagator(H, [a], ts, method=dq.method.Expm())
+    with pytest.raises(ValueError, match=re.escape('Argument `jump_ops[0]` must have shape (...L0, n, n)')):
+        dq.mepropagator(H, [np.ones((3, 2))], ts)
+    with pytest.warns(UserWarning, match='consider using `dq.sepropagator\\(\\)`'):
+        try:
+            dq.mepropagator(H, [], ts)
+        except RuntimeError:
+            pass  # no GPU here: the warning comes first

@@ -77,12 +77,13 @@ def test_result_fields_and_save_states_false():
     tsave = np.linspace(0, 1, 5)
     res = dq.mesolve(H, [0.3 * a], dq.coherent(n, 0.8), tsave, exp_ops=[H],
                      options=dq.Options(save_states=False))
-    # the reference keeps only the final state, without a time axis
-    assert res.states.shape == (n, n)
+    # the reference keeps only the final state, nsave = 1 (mesolve.py:184-187)
+    assert res.states.shape == (1, n, n)
     full = dq.mesolve(H, [0.3 * a], dq.coherent(n, 0.8), tsave, exp_ops=[H])
     assert full.states.shape == (5, n, n)
     np.testing.assert_allclose(np.asarray(full.final_state.to_numpy()), full.states.to_numpy()[-1], atol=0)
-    np.testing.assert_allclose(res.states.to_numpy(), full.states.to_numpy()[-1], atol=1e-6)
+    np.testing.assert_allclose(res.states.to_numpy()[0], full.states.to_numpy()[-1], atol=1e-6)
+    np.testing.assert_allclose(res.final_state.to_numpy(), full.states.to_numpy()[-1], atol=1e-6)
     np.testing.assert_allclose(res.expects, full.expects, atol=1e-6)
     s = str(full)
     assert '|' in s or 'MESolveResult' in s
@@ -154,12 +155,12 @@ def test_save_extra_every_save_time_with_batching_and_no_states():
 
     full = dq.mesolve(H, Ls, psi, tsave)
     res = dq.mesolve(H, Ls, psi, tsave, options=dq.Options(save_extra=fn, save_states=False))
-    assert res.states.shape == (3, n, n)
+    assert res.states.shape == (3, 1, n, n)
     assert res.extra['pop'].shape == (3, len(tsave))
     st = full.states.to_numpy()
     np.testing.assert_allclose(res.extra['pop'], np.real(np.einsum('ij,btji->bt', num, st)), atol=1e-6)
     np.testing.assert_allclose(np.asarray(res.extra['corner'].to_numpy()), st[:, :, 0, 1], atol=1e-6)
-    np.testing.assert_allclose(res.states.to_numpy(), st[:, -1], atol=1e-6)
+    np.testing.assert_allclose(res.states.to_numpy()[:, 0], st[:, -1], atol=1e-6)
 
 
 def test_rouchon2_fixed_step_matches_oracle():

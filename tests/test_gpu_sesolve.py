@@ -177,8 +177,9 @@ def test_batching_shapes_and_instance_independence():
     assert flat.states.shape == (2, 3, 5, n, 1)
     np.testing.assert_array_equal(flat.states.to_numpy(), res.states.to_numpy())
     fin = dq.sesolve(Hs, psis, ts, options=dq.Options(save_states=False))
-    assert fin.states.shape == (2, 3, n, 1)
-    np.testing.assert_array_equal(fin.states.to_numpy(), res.states.to_numpy()[:, :, -1])
+    assert fin.states.shape == (2, 3, 1, n, 1)  # nsave = 1 (sesolve.py:127-130)
+    np.testing.assert_array_equal(fin.states.to_numpy()[:, :, 0], res.states.to_numpy()[:, :, -1])
+    np.testing.assert_array_equal(fin.final_state.to_numpy(), res.states.to_numpy()[:, :, -1])
 
 
 def test_large_dimension_replay():

@@ -669,3 +669,36 @@ def test_sesolve_matches_mesolve_without_jump_operators():
     rho = np.einsum('tia,tja->tij', se['states'], se['states'].conj())
     np.testing.assert_allclose(rho, me['states'], atol=1e-8)
     np.testing.assert_allclose(se['expects'], me['expects'], atol=1e-8)
+
+
+# ----------------------------------------------------------------- propagators
+
+
+def test_propagators_match_matrix_exponentials():
+    """For constant generators the oracle propagators (one solve per basis state)
+    equal expm(-i H t) and expm(t slindbladian(H, Ls)) in the column-stacking
+    convention; a PWC Hamiltonian gives the product of segment exponentials."""
+    from oracle import propagators as op_
+
+    rng = np.random.default_rng(4)
+    n = 3
+    x = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    H = x + x.conj().T
+    L = 0.4 * (rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)))
+    ts = np.array([0.0, 0.3, 0.8])
+    U = op_.sepropagator(H, ts, rtol=1e-11, atol=1e-11)
+    for k, t in enumerate(ts):
+        np.testing.assert_allclose(U[k], sla.expm(-1j * H * t), atol=1e-8)
+    S = lindblad.slindbladian(H, [L])
+    P = op_.mepropagator(H, [L], ts, rtol=1e-11, atol=1e-11)
+    for k, t in enumerate(ts):
+        np.testing.assert_allclose(P[k], sla.expm(S * t), atol=1e-8)
+    # vec(P rho) = P vec(rho) on a random state
+    rho = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    np.testing.assert_allclose(lindblad.unvectorize(P[2] @ lindblad.vectorize(rho)),
+                               om.mesolve(H, [L], rho, ts, form='standard', rtol=1e-11, atol=1e-11)['states'][2],
+                               atol=1e-8)
+    O = np.diag([1.0, -1.0, 0.5]).astype(complex)
+    Hp = om.TimeOp(H, [om.PWCTerm(np.array([0.0, 0.5, 2.0]), np.array([1.0, -2.0]), O)])
+    Up = op_.sepropagator(Hp, ts, rtol=1e-11, atol=1e-11)
+    np.testing.assert_allclose(Up[2], sla.expm(-1j * (H - 2 * O) * 0.3) @ sla.expm(-1j * (H + O) * 0.5), atol=1e-8)
