@@ -1,0 +1,132 @@
+// Matrix exponentials of piecewise-constant generators and their time-ordered
+// products (the Expm method of the propagators), for gfx950.
+//
+// Every interval k of every chain b has a generator already scaled on the host:
+// X[b][k] = dt_k A_k / 2^s. The library forms E = T_m(X)^(2^s) with the degree-m
+// Taylor polynomial in Horner form, T <- I + (X / j) T for j = m .. 1, then s
+// squarings, and chains U[b][k] = E[b][k] U[b][k-1]. Everything is a batched
+// complex64 GEMM on FP32 MFMA (four real v_mfma_f32_16x16x4f32 per complex
+// k-slice, FP32 accumulation): one 16x16 output tile per wave, read straight from
+// L2 (the operands of a batch of N <= 1024 matrices stay resident there), with the
+// Horner epilogue (scale, + I) fused into the store.
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "dqh.h"
+#include "dqh_common.h"
+#include "dqh_mfma.h"
+
+namespace dqh {
+namespace ex {
+
+constexpr int kWaves = 4;
+constexpr int64_t kMaxGridY = 65535;
+
+// C[b] = scale * (A[b] @ B[b]) (+ I when add_id), N x N row-major, N % 16 == 0.
+__global__ __launch_bounds__(64 * kWaves) void k_cgemm(const float2 *__restrict__ A, int64_t sA,
+                                                      const float2 *__restrict__ B, int64_t sB,
+                                                      float2 *__restrict__ C, int64_t sC, int N, float scale,
+                                                      int add_id, int64_t b0) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nt = N / 16;
+  const int tile = blockIdx.x * kWaves + wave;
+  if (tile >= nt * nt) return;  // whole waves; the kernel has no barrier
+  const int64_t b = b0 + blockIdx.y;
+  const int tr = tile / nt, tc = tile - tr * nt;
+  Acc acc;
+  acc_zero(acc);
+  tile_mac(acc, A + b * sA, N, B + b * sB, N, 16 * tr, 16 * tc, N, lane);
+  const int r = lane & 15, g = lane >> 4;
+  float2 *Cb = C + b * sC;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int row = 16 * tr + 4 * g + v, col = 16 * tc + r;
+    const float id = (add_id && row == col) ? 1.f : 0.f;
+    Cb[(size_t)row * N + col] = make_float2(scale * acc.re[v] + id, scale * acc.im[v]);
+  }
+}
+
+// T[b] = I for every matrix of the batch (vector stores; a fill kernel keeps the
+// sequence free of memset nodes)
+__global__ __launch_bounds__(256) void k_eye(float2 *__restrict__ T, int N, int64_t nmat) {
+  const int64_t nn = (int64_t)N * N;
+  const int64_t total = nmat * nn;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t w = e % nn;
+    const int row = (int)(w / N), col = (int)(w - (int64_t)row * N);
+    T[e] = make_float2(row == col ? 1.f : 0.f, 0.f);
+  }
+}
+
+// U[b][0] = E[b][0] for every chain (strided copy)
+__global__ __launch_bounds__(256) void k_first(const float2 *__restrict__ E, float2 *__restrict__ U, int64_t nn,
+                                              int64_t stride, int64_t nchain) {
+  const int64_t total = nchain * nn;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = e / nn, w = e - b * nn;
+    U[b * stride + w] = E[b * stride + w];
+  }
+}
+
+int gemm(const float2 *A, int64_t sA, const float2 *B, int64_t sB, float2 *C, int64_t sC, int N, float scale,
+         int add_id, int64_t nbatch, hipStream_t st) {
+  const int nt = N / 16;
+  const unsigned gx = (unsigned)((nt * nt + kWaves - 1) / kWaves);
+  for (int64_t b0 = 0; b0 < nbatch; b0 += kMaxGridY) {
+    const unsigned gy = (unsigned)(nbatch - b0 < kMaxGridY ? nbatch - b0 : kMaxGridY);
+    hipLaunchKernelGGL(k_cgemm, dim3(gx, gy), dim3(64 * kWaves), 0, st, A, sA, B, sB, C, sC, N, scale, add_id, b0);
+  }
+  const hipError_t err = hipGetLastError();
+  return err == hipSuccess ? DQH_OK : fail(DQH_HIP_ERROR, std::string("dqh_expm_propagators: ") + hipGetErrorString(err));
+}
+
+}  // namespace ex
+}  // namespace dqh
+
+extern "C" int64_t dqh_expm_work_bytes(int64_t nchain, int64_t nint, int64_t N) {
+  if (nchain <= 0 || nint <= 0 || N <= 0) return 0;
+  return 2 * nchain * nint * N * N * 8;
+}
+
+extern "C" int dqh_expm_propagators(const void *X, void *work, void *U, int64_t nchain, int64_t nint, int64_t N,
+                                    int64_t m, int64_t s, void *stream) {
+  using namespace dqh;
+  if (!X || !work || !U || nchain <= 0 || nint <= 0 || N <= 0 || N % 16 != 0 || N > 4096 || m < 1 || m > 30 ||
+      s < 0 || s > 60 || nchain * nint > (int64_t)1 << 31)
+    return fail(DQH_BAD_ARG, "dqh_expm_propagators: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t nn = N * N, nmat = nchain * nint;
+  const float2 *x = (const float2 *)X;
+  float2 *w0 = (float2 *)work, *w1 = w0 + nmat * nn;
+  float2 *u = (float2 *)U;
+  const unsigned fill_blocks = (unsigned)(nmat * nn / 256 + 1 < 8192 ? nmat * nn / 256 + 1 : 8192);
+  hipLaunchKernelGGL(ex::k_eye, dim3(fill_blocks), dim3(256), 0, st, w0, (int)N, nmat);
+  // Horner: T <- I + (X / j) T, j = m .. 1
+  for (int64_t j = m; j >= 1; --j) {
+    const int rc = ex::gemm(x, nn, w0, nn, w1, nn, (int)N, (float)(1.0 / (double)j), 1, nmat, st);
+    if (rc != DQH_OK) return rc;
+    float2 *t = w0;
+    w0 = w1;
+    w1 = t;
+  }
+  // squarings: T <- T T
+  for (int64_t i = 0; i < s; ++i) {
+    const int rc = ex::gemm(w0, nn, w0, nn, w1, nn, (int)N, 1.f, 0, nmat, st);
+    if (rc != DQH_OK) return rc;
+    float2 *t = w0;
+    w0 = w1;
+    w1 = t;
+  }
+  // chains: U[b][0] = E[b][0]; U[b][k] = E[b][k] U[b][k-1]
+  hipLaunchKernelGGL(ex::k_first, dim3((unsigned)(nchain * nn / 256 + 1 < 8192 ? nchain * nn / 256 + 1 : 8192)), dim3(256),
+                     0, st, w0, u, nn, nint * nn, nchain);
+  for (int64_t k = 1; k < nint; ++k) {
+    const int rc = ex::gemm(w0 + k * nn, nint * nn, u + (k - 1) * nn, nint * nn, u + k * nn, nint * nn, (int)N, 1.f, 0,
+                            nchain, st);
+    if (rc != DQH_OK) return rc;
+  }
+  const hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return fail(DQH_HIP_ERROR, std::string("dqh_expm_propagators: ") + hipGetErrorString(err));
+  return DQH_OK;
+}

@@ -20,9 +20,16 @@ independent solves on the existing kernels:
 
 Every basis solve takes its own adaptive step (the reference steps the whole N x N
 matrix with one controller); both are within the method's tolerance of the exact
-propagator. ``Tsit5`` and ``Dopri5`` are supported. The reference's default for
-constant and piecewise-constant generators, ``Expm`` (sepropagator.py:224-225,
-mepropagator.py:269-270), is not built: ``method=None`` runs ``Tsit5``.
+propagator. ``Tsit5`` and ``Dopri5`` are supported, and ``Expm``, the reference's default for
+constant and piecewise-constant generators (sepropagator.py:224-225,
+mepropagator.py:269-270): as ``ExpmIntegrator.run`` (expm_integrator.py:60-90) the
+generator A (-iH, or the Liouvillian ``slindbladian`` in the column-stacking
+convention) is taken at the left end of every interval between t0, the save times
+and the discontinuities, exponentiated and chained, U_k = expm(dt_k A_k) U_{k-1}.
+The exponentials and products run in ``dqh_expm_propagators`` (Taylor degree 8 on
+dt A / 2^s with ||dt A / 2^s||_1 <= 1/2, then s squarings, all as batched complex64
+GEMMs on MFMA). With ``method=None`` the generator type picks ``Expm`` or ``Tsit5``
+as in the reference.
 """
 
 from __future__ import annotations
@@ -34,19 +41,117 @@ import numpy as np
 
 from .._checks import check_shape, check_times
 from ..gradient import Gradient
-from ..method import Dopri5, Method, Tsit5
+from .. import _native
+from ..method import Dopri5, Expm, Method, Tsit5
 from ..options import Options, check_options
 from ..qarrays import QArray
-from ..result import AdaptiveStepInfos, MEPropagatorResult, SEPropagatorResult
-from ..time_qarray import astimeqarray
+from ..result import AdaptiveStepInfos, FixedStepInfos, MEPropagatorResult, SEPropagatorResult
+from ..time_qarray import ConstantTimeQArray, PWCTimeQArray, SummedTimeQArray, astimeqarray
 from . import dense
 from .mesolve import assert_method_supported, max_steps_error, save_extra_outputs
 from .mesolve import enumerate_instances as me_instances
 from .operators import decompose
-from .sesolve import build_plan, build_plan_constant, constant_batch
+from .sesolve import build_plan, build_plan_constant, constant_batch, pad16
 from .sesolve import enumerate_instances as se_instances
 
-SUPPORTED_METHODS = (Tsit5, Dopri5)
+SUPPORTED_METHODS = (Expm, Tsit5, Dopri5)
+TAYLOR_DEGREE = 8  # remainder 0.5^9 / 9! e^0.5 < 1e-8 at ||X||_1 <= THETA
+THETA = 0.5
+
+
+def ispwc(x) -> bool:
+    """Constant or piecewise constant, or a sum of such (``integrators/_utils.py:41-48``)."""
+    if isinstance(x, (ConstantTimeQArray, PWCTimeQArray)):
+        return True
+    if isinstance(x, SummedTimeQArray):
+        return all(ispwc(t) for t in x.timeqarrays)
+    return False
+
+
+def expm_times(t0: float, tsave: np.ndarray, disc: np.ndarray):
+    """Interval boundaries of the Expm method: t0, the save times and the
+    discontinuities inside [t0, tsave[-1]] (expm_integrator.py:63-69), without the
+    zero-length intervals; and for each save time the number of intervals before it."""
+    disc = np.clip(np.asarray(disc, dtype=np.float64), t0, tsave[-1])
+    times = np.unique(np.concatenate([[t0], tsave, disc]))
+    return times, np.searchsorted(times, tsave)
+
+
+def expm_chain(X: np.ndarray) -> np.ndarray:
+    """Chained exponentials on the GPU: ``X`` (nchain, nint, N, N) complex holds
+    dt_k A_k; returns U (nchain, nint, N, N) complex64 with
+    U[b, k] = expm(X[b, k]) ... expm(X[b, 0]) (``dqh_expm_propagators``)."""
+    import torch
+
+    from .dense import _device
+
+    nchain, nint, N = X.shape[0], X.shape[1], X.shape[-1]
+    Np = pad16(N)
+    norm = float(np.max(np.sum(np.abs(X), axis=-2), initial=0.0))  # largest 1-norm
+    s = int(max(0, np.ceil(np.log2(norm / THETA)))) if norm > THETA else 0
+    Xs = np.zeros((nchain, nint, Np, Np), dtype=np.complex64)
+    Xs[..., :N, :N] = X / 2.0**s
+    dev = _device()
+    lib = _native.lib()
+    x = torch.from_numpy(Xs.view(np.float32)).to(dev)
+    work = torch.empty(int(lib.dqh_expm_work_bytes(nchain, nint, Np)), dtype=torch.uint8, device=dev)
+    U = torch.empty_like(x)
+    stream = torch.cuda.current_stream()
+    c = _native.ctypes.c_void_p
+    _native.check(
+        lib.dqh_expm_propagators(c(x.data_ptr()), c(work.data_ptr()), c(U.data_ptr()), nchain, nint, Np,
+                                 TAYLOR_DEGREE, s, c(stream.cuda_stream)),
+        'dqh_expm_propagators',
+    )
+    torch.cuda.synchronize()
+    out = U.cpu().numpy().view(np.complex64).reshape(nchain, nint, Np, Np)
+    return out[..., :N, :N]
+
+
+def _op_at(terms, idx, t: float) -> np.ndarray:
+    """Constant part plus the PWC terms of batch element ``idx`` at time t
+    (``PWCTimeQArray``: zero outside [times[0], times[-1]))."""
+    out = terms.const_at(idx).copy()
+    for times, vals, op in terms.pwc_at(idx):
+        if times[0] <= t < times[-1]:
+            out = out + vals[int(np.searchsorted(times, t, side='right')) - 1] * op
+    return out
+
+
+def _expm_result(gens_at, chains, N, dims, out_shape, tsave, t0, disc, method, gradient, options, cls):
+    """Propagators by ``expm_chain`` for generators ``gens_at(chain, t)`` (N x N)."""
+    times, nbefore = expm_times(t0, tsave, disc)
+    nint = len(times) - 1
+    ns = len(tsave)
+    U = np.empty((len(chains), ns, N, N), dtype=np.complex64)
+    if nint > 0:
+        dts = np.diff(times)
+        X = np.stack([np.stack([dts[k] * gens_at(c, times[k]) for k in range(nint)]) for c in chains])
+        Uc = expm_chain(X)
+    eye = np.eye(N, dtype=np.complex64)
+    for i, nb in enumerate(nbefore):
+        U[:, i] = eye if nb == 0 else Uc[:, nb - 1]
+    infos = FixedStepInfos(np.full(out_shape, nint, dtype=np.int64) if out_shape else np.int64(nint))
+    extra = None
+    if options.save_extra is not None:
+        extra = save_extra_outputs(options.save_extra, U, dims, out_shape)
+    U = U.reshape(*out_shape, ns, N, N)
+    if not options.save_propagators:
+        U = U[..., -1:, :, :]
+    return cls(tsave, method, gradient, options, QArray(U, dims=dims), None, extra, infos)
+
+
+def slindbladian(H: np.ndarray, Ls) -> np.ndarray:
+    """The Liouvillian as an n^2 x n^2 matrix in the column-stacking convention
+    (``dynamiqs/utils/vectorization.py:100-259``: spre(x) = I (x) x,
+    spost(x) = x^T (x) I, sprepost(x, y) = y^T (x) x)."""
+    n = H.shape[-1]
+    eye = np.eye(n)
+    out = -1j * np.kron(eye, H) + 1j * np.kron(H.T, eye)
+    for L in Ls:
+        LdL = L.conj().T @ L
+        out = out + np.kron(L.conj(), L) - 0.5 * np.kron(eye, LdL) - 0.5 * np.kron(LdL.T, eye)
+    return out
 
 
 def _method_kwargs(method, options, save_all: bool) -> dict:
@@ -111,11 +216,20 @@ def sepropagator(
     tsave = check_times(tsave, 'tsave')
     check_options(options, 'sepropagator')
     options = options.initialise()
-    method = Tsit5() if method is None else method
+    if method is None:  # default method (sepropagator.py:224-225)
+        method = Expm() if ispwc(H) else Tsit5()
     assert_method_supported(method, SUPPORTED_METHODS)
     method.assert_supports_gradient(gradient)
 
     n = H.shape[-1]
+    if isinstance(method, Expm):
+        if not ispwc(H):
+            raise TypeError('Method `Expm` requires a constant or piecewise constant Hamiltonian.')
+        Ht = decompose(H)
+        chains = list(np.ndindex(*Ht.batch)) if Ht.batch else [()]
+        t0 = float(tsave[0] if options.t0 is None else options.t0)
+        return _expm_result(lambda c, t: -1j * _op_at(Ht, c, t), chains, n, H.dims, tuple(Ht.batch), tsave, t0,
+                            H.discontinuity_ts, method, gradient, options, SEPropagatorResult)
     basis = QArray(np.eye(n, dtype=np.complex128)[:, :, None], dims=H.dims)  # (n, n, 1): the kets |j>
     kw = _method_kwargs(method, options, options.save_propagators or options.save_extra is not None)
     Ht = decompose(H)
@@ -197,11 +311,43 @@ def mepropagator(
     tsave = check_times(tsave, 'tsave')
     check_options(options, 'mepropagator')
     options = options.initialise()
-    method = Tsit5() if method is None else method
+    if method is None:  # default method (mepropagator.py:269-270)
+        method = Expm() if all(ispwc(x) for x in [H, *Ls]) else Tsit5()
     assert_method_supported(method, SUPPORTED_METHODS)
     method.assert_supports_gradient(gradient)
 
     n = H.shape[-1]
+    if isinstance(method, Expm):
+        if not ispwc(H):
+            raise TypeError('Method `Expm` requires a constant or piecewise constant Hamiltonian.')
+        if not all(ispwc(L) for L in Ls):
+            raise TypeError('Method `Expm` requires constant or piecewise constant jump operators.')
+        Ht = decompose(H)
+        Lts = [decompose(L) for L in Ls]
+        shapes = [Ht.batch, *[Lt.batch for Lt in Lts]]
+        if options.cartesian_batching:
+            out_shape = tuple(d for sh in shapes for d in sh)
+        else:
+            out_shape = tuple(np.broadcast_shapes(*shapes))
+        chains = []
+        for idx in np.ndindex(*out_shape) if out_shape else [()]:
+            if options.cartesian_batching:
+                parts, o = [], 0
+                for sh in shapes:
+                    parts.append(tuple(idx[o : o + len(sh)]))
+                    o += len(sh)
+            else:
+                parts = [tuple(i if d > 1 else 0 for i, d in zip(idx[len(idx) - len(sh) :] if sh else (), sh))
+                         for sh in shapes]
+            chains.append(parts)
+        disc = np.unique(np.concatenate([H.discontinuity_ts, *[L.discontinuity_ts for L in Ls]]))
+        t0 = float(tsave[0] if options.t0 is None else options.t0)
+
+        def gen(parts, t):
+            return slindbladian(_op_at(Ht, parts[0], t), [_op_at(Lt, p, t) for Lt, p in zip(Lts, parts[1:])])
+
+        return _expm_result(gen, chains, n * n, H.dims, out_shape, tsave, t0, disc, method, gradient, options,
+                            MEPropagatorResult)
     out_shape, insts = me_instances(H, Ls, QArray(np.zeros((n, n), dtype=np.complex128), dims=H.dims),
                                     options.cartesian_batching)
     basis = hermitian_basis(n)

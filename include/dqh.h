@@ -274,6 +274,19 @@ int dqh_mesolve_tsit5_dia(const dqh_dia_tsit5 *problem, void *stream);
  * expects[e][i] = <psi(tsave[i])| E_e |psi(tsave[i])>. */
 int dqh_sesolve_tsit5(const dqh_me_inst *d_insts, int64_t ninst, int64_t np_max, void *stream);
 
+/* ------------------------------------------------------------------ Expm */
+/* Propagators of piecewise-constant generators by explicit exponentiation
+ * (replaces ExpmIntegrator.run, expm_integrator.py:60-90: expm of dt_k A_k on every
+ * interval, then U_k = E_k U_{k-1}). X: device complex64 [nchain][nint][N][N], the
+ * generators already scaled, X = dt_k A_k / 2^s; E = T_m(X)^(2^s) with T_m the
+ * degree-m Taylor polynomial (Horner form) and s squarings. U: device complex64
+ * [nchain][nint][N][N] receives U[b][k] = E[b][k] ... E[b][0]. work: device memory
+ * of dqh_expm_work_bytes(nchain, nint, N) bytes. N % 16 == 0, N <= 4096,
+ * 1 <= m <= 30, 0 <= s <= 60. */
+int64_t dqh_expm_work_bytes(int64_t nchain, int64_t nint, int64_t N);
+int dqh_expm_propagators(const void *X, void *work, void *U, int64_t nchain, int64_t nint, int64_t N, int64_t m,
+                         int64_t s, void *stream);
+
 /* ------------------------------------------------------------------ gradients */
 /* Gauss-Legendre quadrature of the adjoint gradient of piecewise-constant controls
  * (replaces the reverse-mode path jax.grad -> dx.RecursiveCheckpointAdjoint,

@@ -299,8 +299,10 @@ def test_propagator_argument_errors():
         dq.sepropagator(H, ts, options=dq.Options(cartesian_batching=False))
     with pytest.raises(ValueError, match=re.escape('Option `save_states` was set to `False` but is not used by the solver `dq.mepropagator()`')):
         dq.mepropagator(H, [a], ts, options=dq.Options(save_states=False))
-    with pytest.raises(TypeError, match=re.escape('Method of type `Expm` is not supported')):
-        dq.mepropagator(H, [a], ts, method=dq.method.Expm())
+    with pytest.raises(TypeError, match=re.escape('Method of type `Dopri8` is not supported (supported method types: `Expm`, `Tsit5`, `Dopri5`)')):
+        dq.mepropagator(H, [a], ts, method=dq.method.Dopri8())
+    with pytest.raises(TypeError, match='Method `Expm` requires a constant or piecewise constant Hamiltonian.'):
+        dq.sepropagator(dq.modulated(np.cos, H), ts, method=dq.method.Expm())
     with pytest.raises(ValueError, match=re.escape('Argument `jump_ops[0]` must have shape (...L0, n, n)')):
         dq.mepropagator(H, [np.ones((3, 2))], ts)
     with pytest.warns(UserWarning, match='consider using `dq.sepropagator\\(\\)`'):
@@ -308,3 +310,35 @@ def test_propagator_argument_errors():
             dq.mepropagator(H, [], ts)
         except RuntimeError:
             pass  # no GPU here: the warning comes first
+
+
+def test_expm_interval_bookkeeping_and_liouvillian():
+    """Expm method host side: the interval boundaries are t0, the save times and the
+    discontinuities clipped to [t0, t1] without zero-length intervals
+    (expm_integrator.py:63-69); each save time maps to the number of intervals
+    before it; the column-stacking Liouvillian equals the oracle's; the PWC operator
+    at an interval's left end follows PWCTimeQArray (zero outside [t0, t_end))."""
+    from oracle import lindblad
+
+    from dynamiqs_amd.integrators.operators import decompose
+    from dynamiqs_amd.integrators.propagators import _op_at, expm_times, ispwc, slindbladian
+
+    times, nb = expm_times(0.0, np.array([0.0, 0.5, 1.0]), np.array([-1.0, 0.25, 0.5, 3.0]))
+    np.testing.assert_array_equal(times, [0.0, 0.25, 0.5, 1.0])
+    np.testing.assert_array_equal(nb, [0, 2, 3])
+    times, nb = expm_times(-0.5, np.array([0.0, 1.0]), np.zeros(0))
+    np.testing.assert_array_equal(times, [-0.5, 0.0, 1.0])
+    np.testing.assert_array_equal(nb, [1, 2])
+    rng = np.random.default_rng(6)
+    n = 3
+    H = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    H = H + H.conj().T
+    L = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    np.testing.assert_allclose(slindbladian(H, [L]), lindblad.slindbladian(H, [L]), atol=1e-13)
+    O = np.diag([1.0, 2.0, 3.0]).astype(complex)
+    Ht = dq.asqarray(H) + dq.pwc(np.array([0.0, 1.0, 2.0]), np.array([5.0, -1.0]), dq.asqarray(O))
+    terms = decompose(Ht)
+    np.testing.assert_allclose(_op_at(terms, (), 0.0), H + 5 * O)
+    np.testing.assert_allclose(_op_at(terms, (), 1.0), H - O)
+    np.testing.assert_allclose(_op_at(terms, (), 2.0), H)
+    assert ispwc(Ht) and not ispwc(dq.modulated(np.cos, dq.asqarray(O)))

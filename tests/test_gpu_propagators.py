@@ -106,3 +106,87 @@ def test_mepropagator_batched_random_and_consistent_with_mesolve():
     np.testing.assert_allclose(via_p, me, atol=1e-5)
     fin = dq.mepropagator(Hs, [Ls, L2], ts, options=dq.Options(save_propagators=False))
     assert fin.propagators.shape == (2, 3, 1, n * n, n * n)
+
+
+def test_expm_default_method_mepropagator_reference_example():
+    """With method=None a constant Liouvillian takes Expm, as in the reference
+    (mepropagator.py:269-270): its docstring example (n = 16, H = a^dag a, [a],
+    11 save times on [0, 1]) prints 'Method : Expm', 'Infos : 10 steps' and
+    propagators (11, 256, 256); here equal to expm(t slindbladian) within 2e-5
+    (complex64 GEMMs, Taylor degree 8 plus squarings)."""
+    import time
+
+    import dynamiqs_amd as dq
+    from oracle import lindblad
+    from oracle import operators as op
+
+    n = 16
+    a = dq.destroy(n)
+    ts = np.linspace(0.0, 1.0, 11)
+    dq.mepropagator(a.dag() @ a, [a], ts)
+    t0 = time.perf_counter()
+    res = dq.mepropagator(a.dag() @ a, [a], ts)
+    print('Expm mepropagator n=16 wall ms', 1e3 * (time.perf_counter() - t0))
+    assert type(res.method).__name__ == 'Expm'
+    assert int(res.infos.nsteps) == 10
+    assert res.propagators.shape == (11, n * n, n * n)
+    ao = op.destroy(n)
+    S = lindblad.slindbladian(ao.conj().T @ ao, [ao])
+    P = res.propagators.to_numpy()
+    err = max(np.max(np.abs(P[k] - sla.expm(S * t))) for k, t in enumerate(ts))
+    print('Expm mepropagator n=16 max |P - expm| =', err)
+    assert err <= 2e-5
+    s = str(res)
+    assert 'Expm' in s and '10 steps' in s and 'Propagators' in s
+
+
+def test_expm_sepropagator_batched_and_pwc():
+    """Expm on a batch of constant Hamiltonians and on a PWC Hamiltonian whose
+    breakpoints fall between the save times (the product of segment exponentials)."""
+    import dynamiqs_amd as dq
+
+    rng = np.random.default_rng(7)
+    n = 6
+    Hs = np.stack([3.0 * _herm(rng, n) for _ in range(3)])
+    ts = np.linspace(0.0, 2.0, 5)
+    res = dq.sepropagator(Hs, ts)
+    assert type(res.method).__name__ == 'Expm' and res.propagators.shape == (3, 5, n, n)
+    U = res.propagators.to_numpy()
+    err = max(np.max(np.abs(U[b, k] - sla.expm(-1j * Hs[b] * t))) for b in range(3) for k, t in enumerate(ts))
+    print('Expm sepropagator batched max err', err)
+    assert err <= 2e-5
+    H0, O = _herm(rng, n), _herm(rng, n)
+    H = dq.asqarray(H0) + dq.pwc(np.array([0.1, 0.7, 1.3]), np.array([2.0, -1.0]), dq.asqarray(O))
+    ts = np.array([0.0, 1.0, 2.0])
+    res = dq.sepropagator(H, ts)
+    assert int(res.infos.nsteps) == 5  # intervals between 0, 0.1, 0.7, 1.0, 1.3, 2.0
+    E = lambda M, h: sla.expm(-1j * M * h)  # noqa: E731
+    U1 = E(H0 - O, 0.3) @ E(H0 + 2 * O, 0.6) @ E(H0, 0.1)
+    U2 = E(H0, 0.7) @ E(H0 - O, 0.3) @ U1
+    got = res.propagators.to_numpy()
+    np.testing.assert_allclose(got[1], U1, atol=2e-5)
+    np.testing.assert_allclose(got[2], U2, atol=2e-5)
+
+
+def test_expm_mepropagator_batched_jump_operators():
+    """Cartesian batching over H (2,) and a jump operator (3,) with Expm, against
+    expm(t S) per element; flat batching broadcasts."""
+    import dynamiqs_amd as dq
+    from oracle import lindblad
+
+    rng = np.random.default_rng(8)
+    n = 3
+    Hs = np.stack([_herm(rng, n) for _ in range(2)])
+    Ls = np.stack([0.4 * (rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))) for _ in range(3)])
+    ts = np.linspace(0.0, 1.0, 3)
+    res = dq.mepropagator(Hs, [Ls], ts, method=dq.method.Expm())
+    assert res.propagators.shape == (2, 3, 3, n * n, n * n)
+    P = res.propagators.to_numpy()
+    for i in range(2):
+        for j in range(3):
+            S = lindblad.slindbladian(Hs[i], [Ls[j]])
+            for k, t in enumerate(ts):
+                np.testing.assert_allclose(P[i, j, k], sla.expm(S * t), atol=2e-5)
+    flat = dq.mepropagator(Hs[:, None].repeat(3, axis=1), [Ls], ts, options=dq.Options(cartesian_batching=False))
+    assert flat.propagators.shape == (2, 3, 3, n * n, n * n)
+    np.testing.assert_allclose(flat.propagators.to_numpy(), P, atol=1e-6)
