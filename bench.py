@@ -598,9 +598,32 @@ def bench_sesolve(args):
         'oracle_fp64_steps': [ref['infos']['nsteps'], ref['infos']['naccepted'], ref['infos']['nrejected']],
         'max_abs_error_sigma_z': float(np.max(np.abs(rw.expects[0].real - sz_exact))),
     }
-    print(json.dumps({'metric': 'sesolve wall time (dq.sesolve end to end)', 'unit': 'ms',
-                      'config': {'workload': 'dynamiqs documentation sesolve examples'},
-                      'batching_3000_two_level': batching, 'workflow_dopri5': workflow}), flush=True)
+    # mepropagator docstring example (mepropagator.py:143-165): Expm by default
+    from oracle import lindblad
+    import scipy.linalg as sla
+
+    n = 16
+    a = dq.destroy(n)
+    tp = np.linspace(0.0, 1.0, 11)
+    dq.mepropagator(a.dag() @ a, [a], tp)
+    lat = []
+    for _ in range(max(args.steps, 5)):
+        t0 = time.perf_counter()
+        rp = dq.mepropagator(a.dag() @ a, [a], tp)
+        lat.append(time.perf_counter() - t0)
+    ao = op.destroy(n)
+    S = lindblad.slindbladian(ao.conj().T @ ao, [ao])
+    P = rp.propagators.to_numpy()
+    propagator = {
+        'p50_ms': 1e3 * float(np.median(lat)),
+        'method': type(rp.method).__name__,
+        'steps': int(rp.infos.nsteps),
+        'max_abs_error_vs_scipy_expm': float(max(np.max(np.abs(P[k] - sla.expm(S * t))) for k, t in enumerate(tp))),
+    }
+    print(json.dumps({'metric': 'sesolve / propagator wall time (end to end)', 'unit': 'ms',
+                      'config': {'workload': 'dynamiqs documentation sesolve and mepropagator examples'},
+                      'batching_3000_two_level': batching, 'workflow_dopri5': workflow,
+                      'mepropagator_expm_n16': propagator}), flush=True)
     return 0
 
 
