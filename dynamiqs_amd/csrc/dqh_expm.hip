@@ -6,9 +6,10 @@
 // Taylor polynomial in Horner form, T <- I + (X / j) T for j = m .. 1, then s
 // squarings, and chains U[b][k] = E[b][k] U[b][k-1]. Everything is a batched
 // complex64 GEMM on FP32 MFMA (four real v_mfma_f32_16x16x4f32 per complex
-// k-slice, FP32 accumulation): one 16x16 output tile per wave, read straight from
-// L2 (the operands of a batch of N <= 1024 matrices stay resident there), with the
-// Horner epilogue (scale, + I) fused into the store.
+// k-slice, FP32 accumulation): one 16x16 output tile per workgroup, split over its
+// four waves along k, operands read straight from L2 (a batch of N <= 1024
+// matrices stays resident there), with the Horner epilogue (scale, + I) fused
+// into the store.
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -24,19 +25,43 @@ constexpr int kWaves = 4;
 constexpr int64_t kMaxGridY = 65535;
 
 // C[b] = scale * (A[b] @ B[b]) (+ I when add_id), N x N row-major, N % 16 == 0.
+// One workgroup per 16x16 output tile; its four waves take consecutive quarters
+// of the k range (split-K, so that a small batch still spreads over the CUs and
+// each wave's chain of dependent loads is four times shorter) and wave 0 adds the
+// partial tiles from LDS and stores with the epilogue.
 __global__ __launch_bounds__(64 * kWaves) void k_cgemm(const float2 *__restrict__ A, int64_t sA,
                                                       const float2 *__restrict__ B, int64_t sB,
                                                       float2 *__restrict__ C, int64_t sC, int N, float scale,
                                                       int add_id, int64_t b0) {
+  __shared__ float part[kWaves - 1][8][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nt = N / 16;
-  const int tile = blockIdx.x * kWaves + wave;
-  if (tile >= nt * nt) return;  // whole waves; the kernel has no barrier
+  const int tile = blockIdx.x;
   const int64_t b = b0 + blockIdx.y;
   const int tr = tile / nt, tc = tile - tr * nt;
+  const int kc = ((nt + kWaves - 1) / kWaves) * 16;  // k columns per wave, a multiple of 16
+  const int kbeg = wave * kc < N ? wave * kc : N;
+  const int kend = kbeg + kc < N ? kbeg + kc : N;
   Acc acc;
   acc_zero(acc);
-  tile_mac(acc, A + b * sA, N, B + b * sB, N, 16 * tr, 16 * tc, N, lane);
+  if (kend > kbeg)
+    tile_mac(acc, A + b * sA + kbeg, N, B + b * sB + (int64_t)kbeg * N, N, 16 * tr, 16 * tc, kend - kbeg, lane);
+  if (wave > 0) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      part[wave - 1][v][lane] = acc.re[v];
+      part[wave - 1][4 + v][lane] = acc.im[v];
+    }
+  }
+  __syncthreads();
+  if (wave != 0) return;
+#pragma unroll
+  for (int w = 0; w < kWaves - 1; ++w)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      acc.re[v] += part[w][v][lane];
+      acc.im[v] += part[w][4 + v][lane];
+    }
   const int r = lane & 15, g = lane >> 4;
   float2 *Cb = C + b * sC;
 #pragma unroll
@@ -72,7 +97,7 @@ __global__ __launch_bounds__(256) void k_first(const float2 *__restrict__ E, flo
 int gemm(const float2 *A, int64_t sA, const float2 *B, int64_t sB, float2 *C, int64_t sC, int N, float scale,
          int add_id, int64_t nbatch, hipStream_t st) {
   const int nt = N / 16;
-  const unsigned gx = (unsigned)((nt * nt + kWaves - 1) / kWaves);
+  const unsigned gx = (unsigned)(nt * nt);
   for (int64_t b0 = 0; b0 < nbatch; b0 += kMaxGridY) {
     const unsigned gy = (unsigned)(nbatch - b0 < kMaxGridY ? nbatch - b0 : kMaxGridY);
     hipLaunchKernelGGL(k_cgemm, dim3(gx, gy), dim3(64 * kWaves), 0, st, A, sA, B, sB, C, sC, N, scale, add_id, b0);

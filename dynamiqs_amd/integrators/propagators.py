@@ -118,15 +118,24 @@ def _op_at(terms, idx, t: float) -> np.ndarray:
     return out
 
 
-def _expm_result(gens_at, chains, N, dims, out_shape, tsave, t0, disc, method, gradient, options, cls):
-    """Propagators by ``expm_chain`` for generators ``gens_at(chain, t)`` (N x N)."""
+def _expm_result(ops_at, build, chains, N, dims, out_shape, tsave, t0, disc, method, gradient, options, cls):
+    """Propagators by ``expm_chain``: ``ops_at(chain, t)`` gives the operators at t
+    (a tuple of n x n arrays) and ``build(ops)`` the N x N generator."""
     times, nbefore = expm_times(t0, tsave, disc)
     nint = len(times) - 1
     ns = len(tsave)
     U = np.empty((len(chains), ns, N, N), dtype=np.complex64)
     if nint > 0:
         dts = np.diff(times)
-        X = np.stack([np.stack([dts[k] * gens_at(c, times[k]) for k in range(nint)]) for c in chains])
+        X = np.empty((len(chains), nint, N, N), dtype=np.complex128)
+        for b, c in enumerate(chains):
+            cache = {}  # one generator per distinct set of operators (a constant one: once)
+            for k in range(nint):
+                ops = ops_at(c, times[k])
+                key = b''.join(o.tobytes() for o in ops)
+                if key not in cache:
+                    cache[key] = build(ops)
+                X[b, k] = dts[k] * cache[key]
         Uc = expm_chain(X)
     eye = np.eye(N, dtype=np.complex64)
     for i, nb in enumerate(nbefore):
@@ -228,8 +237,9 @@ def sepropagator(
         Ht = decompose(H)
         chains = list(np.ndindex(*Ht.batch)) if Ht.batch else [()]
         t0 = float(tsave[0] if options.t0 is None else options.t0)
-        return _expm_result(lambda c, t: -1j * _op_at(Ht, c, t), chains, n, H.dims, tuple(Ht.batch), tsave, t0,
-                            H.discontinuity_ts, method, gradient, options, SEPropagatorResult)
+        return _expm_result(lambda c, t: (_op_at(Ht, c, t),), lambda ops: -1j * ops[0], chains, n, H.dims,
+                            tuple(Ht.batch), tsave, t0, H.discontinuity_ts, method, gradient, options,
+                            SEPropagatorResult)
     basis = QArray(np.eye(n, dtype=np.complex128)[:, :, None], dims=H.dims)  # (n, n, 1): the kets |j>
     kw = _method_kwargs(method, options, options.save_propagators or options.save_extra is not None)
     Ht = decompose(H)
@@ -343,11 +353,11 @@ def mepropagator(
         disc = np.unique(np.concatenate([H.discontinuity_ts, *[L.discontinuity_ts for L in Ls]]))
         t0 = float(tsave[0] if options.t0 is None else options.t0)
 
-        def gen(parts, t):
-            return slindbladian(_op_at(Ht, parts[0], t), [_op_at(Lt, p, t) for Lt, p in zip(Lts, parts[1:])])
+        def ops_at(parts, t):
+            return (_op_at(Ht, parts[0], t), *[_op_at(Lt, p, t) for Lt, p in zip(Lts, parts[1:])])
 
-        return _expm_result(gen, chains, n * n, H.dims, out_shape, tsave, t0, disc, method, gradient, options,
-                            MEPropagatorResult)
+        return _expm_result(ops_at, lambda ops: slindbladian(ops[0], ops[1:]), chains, n * n, H.dims, out_shape,
+                            tsave, t0, disc, method, gradient, options, MEPropagatorResult)
     out_shape, insts = me_instances(H, Ls, QArray(np.zeros((n, n), dtype=np.complex128), dims=H.dims),
                                     options.cartesian_batching)
     basis = hermitian_basis(n)
