@@ -269,6 +269,34 @@ def kraus_operators(H: np.ndarray, Ls, h: float, order: int = 1) -> list:
         out += [np.sqrt(0.5 * h) * L @ U for L in Ls]
         out += [h / np.sqrt(2.0) * Lk @ U @ Lj for Lk in Ls for Lj in Ls]
         return out
+    if order == 3:
+        # KrausHeun3 (:288-302, c = (0, 1/3, 2/3), a10 = 1/3, a21 = 2/3,
+        # b = (1/4, 0, 3/4)) with the Bosh3 no-jump propagator (:445-453): U1 after
+        # one Bogacki-Shampine step, U(1/3), U(2/3) on its cubic Hermite dense output.
+        # The stage recursion expands to the Kraus operators U1, sqrt(h/4) U1 L_k,
+        # sqrt(3h/4) Q2 L_k Ub, h/sqrt(2) Q2 L_k P L_j Ua and
+        # sqrt(h^3/6) Q2 L_k P L_j Ua L_i, with P = Ub Ua^-1 and Q2 = U1 Ub^-1.
+        eye = np.eye(n)
+        G = -1j * H - 0.5 * LdL
+        k1 = G
+        k2 = G @ (eye + 0.5 * h * k1)
+        k3 = G @ (eye + 0.75 * h * k2)
+        U1 = eye + h * (2.0 / 9.0 * k1 + 1.0 / 3.0 * k2 + 4.0 / 9.0 * k3)
+        d0, d1 = h * G, h * G @ U1
+
+        def herm(th):
+            t2, t3 = th * th, th * th * th
+            return (2 * t3 - 3 * t2 + 1) * eye + (t3 - 2 * t2 + th) * d0 + (-2 * t3 + 3 * t2) * U1 + (t3 - t2) * d1
+
+        Ua, Ub = herm(1.0 / 3.0), herm(2.0 / 3.0)
+        P = np.linalg.solve(Ua.T, Ub.T).T
+        Q2 = np.linalg.solve(Ub.T, U1.T).T
+        out = [U1]
+        out += [np.sqrt(0.25 * h) * U1 @ L for L in Ls]
+        out += [np.sqrt(0.75 * h) * Q2 @ L @ Ub for L in Ls]
+        out += [h / np.sqrt(2.0) * Q2 @ Lk @ P @ Lj @ Ua for Lk in Ls for Lj in Ls]
+        out += [np.sqrt(h**3 / 6.0) * Q2 @ Lk @ P @ Lj @ Ua @ Li for Lk in Ls for Lj in Ls for Li in Ls]
+        return out
     raise ValueError(f'Rouchon order {order} is not supported')
 
 
@@ -448,7 +476,7 @@ def build_plan(
         p_step_t = arena.put(step_t)
         hs = np.diff(step_t)
         mids = step_t[:-1] + 0.5 * hs
-        nLdev = K + 1 if rouchon_order == 1 else 1 + 2 * K + K * K
+        nLdev = {1: K + 1, 2: 1 + 2 * K + K * K, 3: 1 + 2 * K + K * K + K**3}[rouchon_order]
     else:
         nLdev = K
     # fields common to the batch, as whole columns

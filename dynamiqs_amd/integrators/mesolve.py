@@ -19,7 +19,7 @@ import numpy as np
 
 from .._checks import check_hermitian, check_qarray_is_dense, check_shape, check_times
 from ..gradient import Gradient
-from ..method import Dopri5, Method, Rouchon1, Rouchon2, Tsit5, _DEAdaptiveStep
+from ..method import Dopri5, Method, Rouchon1, Rouchon2, Rouchon3, Tsit5, _DEAdaptiveStep
 from ..options import Options, check_options
 from ..qarrays import QArray, asqarray
 from ..result import AdaptiveStepInfos, FixedStepInfos, MESolveResult
@@ -28,7 +28,7 @@ from . import dense
 from . import dia as dia_path
 from .operators import decompose
 
-SUPPORTED_METHODS = (Tsit5, Dopri5, Rouchon1, Rouchon2)
+SUPPORTED_METHODS = (Tsit5, Dopri5, Rouchon1, Rouchon2, Rouchon3)
 ADAPTIVE = (Tsit5, Dopri5)
 
 
@@ -168,8 +168,10 @@ def mesolve(
     options = options.initialise()
     assert_method_supported(method, SUPPORTED_METHODS)
     method.assert_supports_gradient(gradient)
-    if isinstance(method, Rouchon2) and method.dt is None:
-        raise NotImplementedError('the adaptive Rouchon2 (dt=None) is not supported by the MI355X solver: set `dt`')
+    if isinstance(method, (Rouchon2, Rouchon3)) and method.dt is None:
+        raise NotImplementedError(
+            f'the adaptive {type(method).__name__} (dt=None) is not supported by the MI355X solver: set `dt`'
+        )
 
     route = dia_route(method, H, Ls, H.shape[-1])
     if route is not None:
@@ -220,7 +222,7 @@ def mesolve(
             dt=method.dt,
             normalize=method.normalize,
             save_states=save_all,
-            rouchon_order=2 if isinstance(method, Rouchon2) else 1,
+            rouchon_order=3 if isinstance(method, Rouchon3) else (2 if isinstance(method, Rouchon2) else 1),
         )
     plan.run()
     res = plan.results()

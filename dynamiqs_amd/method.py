@@ -138,7 +138,9 @@ class Rouchon2(_RouchonHigherOrder):
 
 
 class Rouchon3(_RouchonHigherOrder):
-    """Third-order Rouchon method (not supported by the MI355X solver)."""
+    """Third-order Rouchon method. With ``dt`` it is the fixed-step Kraus-Heun3 map
+    (1 + 2K + K^2 + K^3 Kraus operators folded on the host, on the dense Rouchon
+    kernel); the adaptive form (``dt=None``) is not supported by the MI355X solver."""
 
 
 class Expm(Method):

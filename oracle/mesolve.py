@@ -153,10 +153,11 @@ def mesolve(
             tab=tsit5.tableau(method),
         )
         infos = dict(nsteps=st.nsteps, naccepted=st.naccepted, nrejected=st.nrejected, nrhs=st.nrhs)
-    elif method in ('rouchon1', 'rouchon2'):
+    elif method in ('rouchon1', 'rouchon2', 'rouchon3'):
         if dt is None:
             raise ValueError('fixed-step Rouchon needs dt')
-        kstep = rouchon.kraus_step_dense if method == 'rouchon1' else rouchon.rouchon2_step
+        kstep = {'rouchon1': rouchon.kraus_step_dense, 'rouchon2': rouchon.rouchon2_step,
+                 'rouchon3': rouchon.rouchon3_step}[method]
 
         def step(t, h, rho):
             tm = t + 0.5 * h

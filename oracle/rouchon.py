@@ -1,5 +1,5 @@
-"""Rouchon1 (Kraus-Euler) and Rouchon2 (Kraus-Heun) with Cholesky normalisation,
-in complex128 (oracle).
+"""Rouchon1 (Kraus-Euler), Rouchon2 (Kraus-Heun) and Rouchon3 (Kraus-Heun3) with
+Cholesky normalisation, in complex128 (oracle).
 
 Restates ``dynamiqs/integrators/core/rouchon_integrator.py``:
 
@@ -87,18 +87,18 @@ def rouchon2_propagator(H: np.ndarray, Ls, dt: float) -> np.ndarray:
     return np.eye(n) + dt * G @ (np.eye(n) + 0.5 * dt * G)
 
 
-def rouchon2_step(H: np.ndarray, Ls, rho: np.ndarray, dt: float, normalize: bool = True) -> np.ndarray:
-    """One Rouchon2 step in the reference's stage form: the stages
+def kraus_rk_step(U: dict, tab_c, tab_a, tab_b, Ls, rho: np.ndarray, dt: float, normalize: bool = True) -> np.ndarray:
+    """One step of ``KrausMapRK`` (rouchon_integrator.py:83-224) for the tableau
+    (c, A, b) and the no-jump propagators ``U[c]`` at the nodes and at 1: the stages
     rho^(i) = U(c_i) rho U(c_i)^dag + dt sum_j a_ij P_ij D(rho^(j)) P_ij^dag
-    (``compute_stages`` :154-171), the update
+    (``compute_stages`` :160-175), the update
     rho' = U(1) rho U(1)^dag + dt sum_i b_i Q_i D(rho^(i)) Q_i^dag (``__call__``
-    :173-184) with P_ij = U(c_i) U(c_j)^-1, Q_i = U(1) U(c_i)^-1, and, with
-    ``normalize``, rho first mapped to T^-dag rho T^-1 where T T^dag = S from the
-    adjoint pass (``S_stage`` / ``S`` :193-224)."""
-    n = H.shape[-1]
+    :177-190) with P_ij = U(c_i) U(c_j)^-1, Q_i = U(1) U(c_i)^-1 (linear solves, as
+    ``propagator_ratio`` :140-154), and, with ``normalize``, rho first mapped to
+    T^-dag rho T^-1 where T T^dag = S from the adjoint pass (``S_stage`` / ``S``
+    :196-224)."""
+    n = rho.shape[-1]
     I = np.eye(n)
-    U1 = rouchon2_propagator(H, Ls, dt)
-    U = {0.0: I, 1.0: U1}
 
     def ratio(ci, cj):
         if ci == cj:
@@ -114,39 +114,86 @@ def rouchon2_step(H: np.ndarray, Ls, rho: np.ndarray, dt: float, normalize: bool
         return sum((dag(L) @ o @ L for L in Ls), np.zeros_like(o))
 
     def S_stage(i, O):
-        ci = HEUN2_C[i]
+        ci = tab_c[i]
         res = dag(U[ci]) @ O @ U[ci]
         for j in range(i):
-            if HEUN2_A[i][j] == 0.0:
+            if tab_a[i][j] == 0.0:
                 continue
-            P = ratio(ci, HEUN2_C[j])
-            res = res + S_stage(j, dt * HEUN2_A[i][j] * Dadj(dag(P) @ O @ P))
+            P = ratio(ci, tab_c[j])
+            res = res + S_stage(j, dt * tab_a[i][j] * Dadj(dag(P) @ O @ P))
         return res
 
     if normalize:
-        S = dag(U1) @ U1
-        for i, bi in enumerate(HEUN2_B):
+        S = dag(U[1.0]) @ U[1.0]
+        for i, bi in enumerate(tab_b):
             if bi == 0.0:
                 continue
-            Q = ratio(1.0, HEUN2_C[i])
+            Q = ratio(1.0, tab_c[i])
             S = S + dt * bi * S_stage(i, Dadj(dag(Q) @ Q))
         rho = cholesky_normalize(S, rho)
     stages = []
-    for i, ci in enumerate(HEUN2_C):
+    for i, ci in enumerate(tab_c):
         r = U[ci] @ rho @ dag(U[ci])
         for j in range(i):
-            if HEUN2_A[i][j] == 0.0:
+            if tab_a[i][j] == 0.0:
                 continue
-            P = ratio(ci, HEUN2_C[j])
-            r = r + dt * HEUN2_A[i][j] * P @ D(stages[j]) @ dag(P)
+            P = ratio(ci, tab_c[j])
+            r = r + dt * tab_a[i][j] * P @ D(stages[j]) @ dag(P)
         stages.append(r)
-    out = U1 @ rho @ dag(U1)
-    for i, bi in enumerate(HEUN2_B):
+    out = U[1.0] @ rho @ dag(U[1.0])
+    for i, bi in enumerate(tab_b):
         if bi == 0.0:
             continue
-        Q = ratio(1.0, HEUN2_C[i])
+        Q = ratio(1.0, tab_c[i])
         out = out + dt * bi * Q @ D(stages[i]) @ dag(Q)
     return out
+
+
+def rouchon2_step(H: np.ndarray, Ls, rho: np.ndarray, dt: float, normalize: bool = True) -> np.ndarray:
+    """One Rouchon2 step: ``kraus_rk_step`` with the Heun tableau and U(1) of the
+    midpoint no-jump propagator."""
+    n = H.shape[-1]
+    U = {0.0: np.eye(n), 1.0: rouchon2_propagator(H, Ls, dt)}
+    return kraus_rk_step(U, HEUN2_C, HEUN2_A, HEUN2_B, Ls, rho, dt, normalize)
+
+
+# --------------------------------------------------------------------- Rouchon3
+# KrausHeun3 (rouchon_integrator.py:288-302: c = (0, 1/3, 2/3),
+# A = ((0, 0, 0), (1/3, 0, 0), (0, 2/3, 0)), b = (1/4, 0, 3/4)) with the no-jump
+# propagator of MESolveFixedRouchon3Integrator (:445-453): one Bogacki-Shampine
+# (Bosh3) step on dU/dt = G U from U = I, whose dense output (diffrax's
+# ThirdOrderHermitePolynomialInterpolation) is the cubic Hermite polynomial through
+# (I, dt G) at theta = 0 and (U(1), dt G U(1)) at theta = 1 (make_nojump_propagator
+# :326-354). Bosh3 tableau (Bogacki & Shampine 1989): c = (0, 1/2, 3/4, 1),
+# a21 = 1/2, a32 = 3/4, b = (2/9, 1/3, 4/9), the fourth stage at y1 (FSAL).
+HEUN3_C = (0.0, 1.0 / 3.0, 2.0 / 3.0)
+HEUN3_A = ((0.0, 0.0, 0.0), (1.0 / 3.0, 0.0, 0.0), (0.0, 2.0 / 3.0, 0.0))
+HEUN3_B = (0.25, 0.0, 0.75)
+
+
+def hermite3(y0, y1, k0, k1, theta: float):
+    """Cubic through y0, y1 with theta-derivatives k0, k1 (dt-scaled slopes)."""
+    t2, t3 = theta * theta, theta * theta * theta
+    return (2 * t3 - 3 * t2 + 1) * y0 + (t3 - 2 * t2 + theta) * k0 + (-2 * t3 + 3 * t2) * y1 + (t3 - t2) * k1
+
+
+def rouchon3_propagators(H: np.ndarray, Ls, dt: float) -> dict:
+    """No-jump propagators U(c) at c = 0, 1/3, 2/3, 1 for Rouchon3."""
+    n = H.shape[-1]
+    I = np.eye(n)
+    G = -1j * H - 0.5 * sum((dag(L) @ L for L in Ls), np.zeros_like(H))
+    k1 = G
+    k2 = G @ (I + 0.5 * dt * k1)
+    k3 = G @ (I + 0.75 * dt * k2)
+    U1 = I + dt * (2.0 / 9.0 * k1 + 1.0 / 3.0 * k2 + 4.0 / 9.0 * k3)
+    d0, d1 = dt * G, dt * G @ U1
+    return {0.0: I, HEUN3_C[1]: hermite3(I, U1, d0, d1, HEUN3_C[1]), HEUN3_C[2]: hermite3(I, U1, d0, d1, HEUN3_C[2]),
+            1.0: U1}
+
+
+def rouchon3_step(H: np.ndarray, Ls, rho: np.ndarray, dt: float, normalize: bool = True) -> np.ndarray:
+    """One Rouchon3 step: ``kraus_rk_step`` with the Heun3 tableau."""
+    return kraus_rk_step(rouchon3_propagators(H, Ls, dt), HEUN3_C, HEUN3_A, HEUN3_B, Ls, rho, dt, normalize)
 
 
 def _to_lower_band(S: np.ndarray, w: int) -> np.ndarray:

@@ -177,13 +177,13 @@ def test_save_extra_outputs_stacking():
 
 
 def test_rouchon_method_support():
-    """Fixed-step Rouchon2 is supported; the adaptive Rouchon2 (dt=None) raises
-    NotImplementedError before any device work; Rouchon3 is not a supported type."""
+    """Fixed-step Rouchon2 and Rouchon3 are supported; their adaptive forms
+    (dt=None) raise NotImplementedError before any device work."""
     H, Ls, psi, ts = _args()
     with pytest.raises(NotImplementedError, match='adaptive Rouchon2'):
         dq.mesolve(H, Ls, psi, ts, method=dq.method.Rouchon2())
-    with pytest.raises(TypeError, match=re.escape('Method of type `Rouchon3` is not supported')):
-        dq.mesolve(H, Ls, psi, ts, method=dq.method.Rouchon3(dt=0.1))
+    with pytest.raises(NotImplementedError, match='adaptive Rouchon3'):
+        dq.mesolve(H, Ls, psi, ts, method=dq.method.Rouchon3())
     m = dq.method.Rouchon2(dt=0.01, normalize=False)
     assert (m.dt, m.normalize, m.rtol, m.max_steps) == (0.01, False, 1e-6, 100_000)
 

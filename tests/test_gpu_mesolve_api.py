@@ -188,3 +188,41 @@ def test_rouchon2_fixed_step_matches_oracle():
         if normalize:
             assert np.abs(np.trace(st[:, -1], axis1=-2, axis2=-1) - 1).max() < 1e-5
     assert str(res).split('\n')[1] == 'Method  : Rouchon2'
+
+
+def test_rouchon3_fixed_step_matches_oracle():
+    """dq.method.Rouchon3(dt) on the dense Kraus kernel (1 + 2K + K^2 + K^3
+    normalised Kraus operators folded on the host) against the oracle's stage-form
+    restatement of KrausHeun3 with the Bosh3 no-jump propagator, with and without
+    normalisation, on a batch of initial states; closer to expm than Rouchon2 at
+    the same dt."""
+    import scipy.linalg as sla
+
+    from oracle import lindblad
+    from oracle import mesolve as omesolve
+
+    n = 8
+    a = dq.destroy(n)
+    H = a.dag() @ a + 0.3 * (a + a.dag())
+    Ls = [0.5 * a, 0.2 * a.dag() @ a]
+    rho0 = np.stack([dq.coherent_dm(n, al).to_numpy() for al in (0.3, 0.9)])
+    tsave = np.linspace(0.0, 1.0, 6)
+    for normalize in (True, False):
+        res = dq.mesolve(H, Ls, rho0, tsave, exp_ops=[a.dag() @ a],
+                         method=dq.method.Rouchon3(dt=2e-2, normalize=normalize))
+        st = res.states.to_numpy()
+        for b in range(2):
+            ref = omesolve.mesolve(H.to_numpy(), [L.to_numpy() for L in Ls], rho0[b], tsave, method='rouchon3',
+                                   dt=2e-2, normalize=normalize, exp_ops=[(a.dag() @ a).to_numpy()])
+            err = np.linalg.norm(st[b] - ref['states'], axis=(-2, -1)).max()
+            print('rouchon3 normalize', normalize, 'instance', b, 'err vs oracle', err)
+            assert err <= 1e-5
+            np.testing.assert_allclose(res.expects[b, 0], ref['expects'][0], atol=1e-5)
+        if normalize:
+            assert np.abs(np.trace(st[:, -1], axis1=-2, axis2=-1) - 1).max() < 1e-5
+    S = lindblad.slindbladian(H.to_numpy(), [L.to_numpy() for L in Ls])
+    exact = lindblad.unvectorize(sla.expm(S) @ lindblad.vectorize(rho0[1]))
+    r3 = dq.mesolve(H, Ls, rho0[1], tsave, method=dq.method.Rouchon3(dt=2e-2)).states.to_numpy()[-1]
+    r2 = dq.mesolve(H, Ls, rho0[1], tsave, method=dq.method.Rouchon2(dt=2e-2)).states.to_numpy()[-1]
+    assert np.linalg.norm(r3 - exact) < np.linalg.norm(r2 - exact)
+    assert str(res).split('\n')[1] == 'Method  : Rouchon3'

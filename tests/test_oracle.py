@@ -702,3 +702,59 @@ def test_propagators_match_matrix_exponentials():
     Hp = om.TimeOp(H, [om.PWCTerm(np.array([0.0, 0.5, 2.0]), np.array([1.0, -2.0]), O)])
     Up = op_.sepropagator(Hp, ts, rtol=1e-11, atol=1e-11)
     np.testing.assert_allclose(Up[2], sla.expm(-1j * (H - 2 * O) * 0.3) @ sla.expm(-1j * (H + O) * 0.5), atol=1e-8)
+
+
+# ----------------------------------------------------------------- Rouchon3
+
+
+def test_rouchon3_kraus_form_equals_reference_stage_form():
+    """The host's 1 + 2K + K^2 + K^3 Kraus operators of the Rouchon3 step, applied
+    as sum A rho A^dag (and in the kernel's form rho + tmp + tmp^dag), equal the
+    oracle's stage form (KrausMapRK with the Heun3 tableau, Bosh3 no-jump
+    propagator and its cubic Hermite dense output, adjoint-pass S), with and without
+    normalisation."""
+    from dynamiqs_amd.integrators.dense import _dag, _kraus_sets
+
+    rng = np.random.default_rng(23)
+    H, Ls, rho = _rand_problem(rng, 5, 2)
+    for normalize in (True, False):
+        D, C = _kraus_sets(H, Ls, 0.05, normalize, order=3)
+        A = [np.eye(5) + D[0], *D[1:]]
+        want = rouchon.rouchon3_step(H, Ls, rho, 0.05, normalize)
+        np.testing.assert_allclose(sum(a @ rho @ _dag(a) for a in A), want, atol=1e-13)
+        tmp = sum(d @ rho @ c for d, c in zip(D, C))
+        np.testing.assert_allclose(rho + tmp + _dag(tmp), want, atol=1e-13)
+        if normalize:
+            np.testing.assert_allclose(sum(_dag(a) @ a for a in A), np.eye(5), atol=1e-13)
+    assert len(D) == 1 + 2 * 2 + 2 * 2 + 2**3
+
+
+def test_rouchon3_third_order_trace_and_propagators():
+    """Third-order convergence to expm of the superoperator (error ratio ~8 when dt
+    halves, with and without normalisation); trace preserved with normalisation; the
+    Bosh3 no-jump propagator matches expm(dt G) to third order and its Hermite
+    dense output interpolates U(c) = expm(c dt G) to third order."""
+    n = 5
+    a = op.destroy(n)
+    H = a.conj().T @ a + 0.2 * (a + a.conj().T)
+    Ls = [0.5 * a, 0.3 * a.conj().T @ a]
+    rho0 = op.coherent_dm(n, 0.7)
+    ex = lindblad.unvectorize(sla.expm(lindblad.slindbladian(H, Ls)) @ lindblad.vectorize(rho0))
+
+    def run(dt, normalize=True):
+        r = rho0.copy()
+        for _ in range(int(round(1.0 / dt))):
+            r = rouchon.rouchon3_step(H, Ls, r, dt, normalize)
+        return r
+
+    for normalize in (True, False):
+        e1, e2 = np.linalg.norm(run(0.04, normalize) - ex), np.linalg.norm(run(0.02, normalize) - ex)
+        assert 6.5 < e1 / e2 < 9.5, (normalize, e1, e2)
+    assert abs(np.trace(run(0.05)) - 1) < 1e-13
+    G = -1j * H - 0.5 * sum(L.conj().T @ L for L in Ls)
+    errs = []
+    for dt in (0.02, 0.01):
+        U = rouchon.rouchon3_propagators(H, Ls, dt)
+        errs.append([np.linalg.norm(U[c] - sla.expm(c * dt * G)) for c in (1.0 / 3.0, 2.0 / 3.0, 1.0)])
+    ratios = np.array(errs[0]) / np.array(errs[1])
+    assert np.all((ratios > 12) & (ratios < 20)), ratios  # local error O(dt^4)
