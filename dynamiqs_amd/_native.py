@@ -64,6 +64,7 @@ FLAG_SAVE_STATES = 1
 FLAG_HAS_HNH = 2
 FLAG_GLOBAL = 4
 FLAG_DOPRI5 = 8
+FLAG_EULER = 16
 
 STATUS_OK = 0
 STATUS_MAX_STEPS = -3

@@ -206,7 +206,8 @@ __global__ __launch_bounds__(T) void k_sesolve(const dqh_me_inst *__restrict__ i
   const double *tsave = gp<const double>(p.tsave);
   const double *ptimes = gp<const double>(p.pwc_times);
   int64_t *stats = gp<int64_t>(p.stats);
-  if (np > kMaxNp || n <= 0 || n > np || p.npwc > 8 || p.nmod > DQH_MAX_MOD || (p.npwc > 0 && nseg < 2)) {
+  if (np > kMaxNp || n <= 0 || n > np || p.npwc > 8 || p.nmod > DQH_MAX_MOD || (p.npwc > 0 && nseg < 2) ||
+      ((p.flags & DQH_FLAG_EULER) && (!p.step_t || p.max_steps < 0))) {
     if (threadIdx.x == 0) stats[3] = DQH_UNSUPPORTED;
     return;
   }
@@ -225,135 +226,161 @@ __global__ __launch_bounds__(T) void k_sesolve(const dqh_me_inst *__restrict__ i
     save_at<T>(p, y, k, 0.0, w0, false, isave, vsave, red);
     ++isave;
   }
-  int seg = pwc ? segment_of(ptimes, nseg, t) : -1;
-  Coef cf;
-  coefficients(p, seg, t, cf);
-  for (int i = threadIdx.x; i < np; i += T) ys[i] = make_float2((float)y[i].x, (float)y[i].y);
-  __syncthreads();
-  field<T>(p, cf, ys, k[0]);
-  ++nrhs;
-  // initial step (Hairer-Norsett-Wanner II.4), scale atol + rtol |y0|
-  double dt;
-  {
-    double s0 = 0.0, s1 = 0.0;
-    for (int i = threadIdx.x; i < n; i += T) {
-      const double ay2 = y[i].x * y[i].x + y[i].y * y[i].y;
-      const double sc = atol + rtol * sqrt(ay2);
-      s0 += ay2 / (sc * sc);
-      s1 += ((double)k[0][i].x * k[0][i].x + (double)k[0][i].y * k[0][i].y) / (sc * sc);
-    }
-    const double d0 = sqrt(bsum<T>(s0, red) * inv_n), d1 = sqrt(bsum<T>(s1, red) * inv_n);
-    const double h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
-    for (int i = threadIdx.x; i < np; i += T)
-      ys[i] = make_float2((float)(y[i].x + h0 * k[0][i].x), (float)(y[i].y + h0 * k[0][i].y));
-    __syncthreads();
-    if (smooth) coefficients(p, seg, t + h0, cf);
-    field<T>(p, cf, ys, k[1]);
-    ++nrhs;
-    double s2 = 0.0;
-    for (int i = threadIdx.x; i < n; i += T) {
-      const double sc = atol + rtol * sqrt(y[i].x * y[i].x + y[i].y * y[i].y);
-      const double dr = (double)k[1][i].x - k[0][i].x, di = (double)k[1][i].y - k[0][i].y;
-      s2 += (dr * dr + di * di) / (sc * sc);
-    }
-    const double d2 = sqrt(bsum<T>(s2, red) * inv_n) / h0;
-    const double md = fmax(d1, d2);
-    const double h1 = md <= 1e-15 ? fmax(1e-6, h0 * 1e-3) : ctrl_pow(0.01 / md, 0.2);
-    dt = fmin(100.0 * h0, h1);
-  }
-  bool need_k0 = false;
-  while (t < t_end) {
-    if (nsteps >= p.max_steps) {
-      status = DQH_MAX_STEPS;
-      break;
-    }
-    if (pwc) {
-      const int s = segment_of(ptimes, nseg, t);
-      if (s != seg) seg = s;
-    }
-    if (need_k0) {
-      coefficients(p, seg, t, cf);
+  if (p.flags & DQH_FLAG_EULER) {
+    // fixed steps between the boundaries step_t[0..max_steps]: y <- y + h G(t) y with
+    // the operators at the step start, linear interpolation to the save times
+    const double *st = gp<const double>(p.step_t);
+    Coef cf;
+    double wl[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int64_t m = 0; m < p.max_steps; ++m) {
+      const double ta = st[m], tb = st[m + 1], h = tb - ta;
+      coefficients(p, pwc ? segment_of(ptimes, nseg, ta) : -1, ta, cf);
       for (int i = threadIdx.x; i < np; i += T) ys[i] = make_float2((float)y[i].x, (float)y[i].y);
       __syncthreads();
       field<T>(p, cf, ys, k[0]);
       ++nrhs;
-      need_k0 = false;
-    }
-    double h = dt, target = t_end;
-    bool target_jump = false;
-    if (pwc) {
-      const int m = segment_of(ptimes, nseg, t) + 1;
-      if (m < nseg && ptimes[m] < t_end) {
-        target = ptimes[m];
-        target_jump = true;
-      }
-    }
-    bool made_jump = false;
-    double t_new = t + h;
-    if (snaps_to(t, h, target)) {
-      h = target - t;
-      t_new = target;
-      made_jump = target_jump;
-    }
-    if (!(h > 0.0) || !(h > 1e-14 * fmax(1.0, fabs(t)))) {
-      status = DQH_NONFINITE;
-      break;
-    }
-    if (!smooth) coefficients(p, seg, t, cf);
-    for (int s = 1; s < 7; ++s) {
-      for (int i = threadIdx.x; i < np; i += T) {
-        double re = y[i].x, im = y[i].y;
-        for (int j = 0; j < s; ++j) {
-          re += h * rk.A[s][j] * k[j][i].x;
-          im += h * rk.A[s][j] * k[j][i].y;
-        }
-        if (s == 6) y1[i] = make_double2(re, im);
-        ys[i] = make_float2((float)re, (float)im);
-      }
-      __syncthreads();
-      if (smooth) coefficients(p, seg, t + rk.C[s] * h, cf);
-      field<T>(p, cf, ys, k[s]);
-      ++nrhs;
-    }
-    double ssum = 0.0;
-    for (int i = threadIdx.x; i < n; i += T) {
-      double er = 0.0, ei = 0.0;
-      for (int j = 0; j < 7; ++j) {
-        er += rk.E[j] * k[j][i].x;
-        ei += rk.E[j] * k[j][i].y;
-      }
-      er *= h;
-      ei *= h;
-      const double sc = atol + rtol * fmax(sqrt(y[i].x * y[i].x + y[i].y * y[i].y),
-                                           sqrt(y1[i].x * y1[i].x + y1[i].y * y1[i].y));
-      ssum += (er * er + ei * ei) / (sc * sc);
-    }
-    const double norm = sqrt(bsum<T>(ssum, red) * inv_n);
-    const bool finite = isfinite(norm), keep = finite && norm < 1.0;
-    double factor = !finite ? p.factormin : (norm == 0.0 ? p.factormax : p.safety * ctrl_pow(norm, -0.2));
-    factor = fmin(fmax(factor, keep ? 1.0 : p.factormin), p.factormax);
-    ++nsteps;
-    if (keep) {
-      if (threadIdx.x == 0 && nacc < p.step_log_cap) gp<double>(p.step_log)[nacc] = t_new;
-      ++nacc;
-      while (isave < ntsave && tsave[isave] <= t_new) {
-        const double th = (tsave[isave] - t) / h;
-        double w[7];
-        for (int j = 0; j < 7; ++j) w[j] = th * (rk.R[j][0] + th * (rk.R[j][1] + th * (rk.R[j][2] + th * rk.R[j][3])));
-        save_at<T>(p, y, k, h, w, true, isave, vsave, red);
+      while (isave < ntsave && tsave[isave] <= tb + 1e-12 * fmax(1.0, fabs(tb))) {
+        wl[0] = fmin(fmax((tsave[isave] - ta) / h, 0.0), 1.0);
+        save_at<T>(p, y, k, h, wl, true, isave, vsave, red);
         ++isave;
       }
-      for (int i = threadIdx.x; i < np; i += T) {
-        y[i] = y1[i];
-        k[0][i] = k[6][i];
-      }
+      for (int i = threadIdx.x; i < np; i += T)
+        y[i] = make_double2(y[i].x + h * (double)k[0][i].x, y[i].y + h * (double)k[0][i].y);
       __syncthreads();
-      t = t_new;
-      need_k0 = made_jump;
-    } else {
-      ++nrej;
+      ++nsteps;
+      ++nacc;
     }
-    dt = factor * h;
+  } else {
+    int seg = pwc ? segment_of(ptimes, nseg, t) : -1;
+    Coef cf;
+    coefficients(p, seg, t, cf);
+    for (int i = threadIdx.x; i < np; i += T) ys[i] = make_float2((float)y[i].x, (float)y[i].y);
+    __syncthreads();
+    field<T>(p, cf, ys, k[0]);
+    ++nrhs;
+    // initial step (Hairer-Norsett-Wanner II.4), scale atol + rtol |y0|
+    double dt;
+    {
+      double s0 = 0.0, s1 = 0.0;
+      for (int i = threadIdx.x; i < n; i += T) {
+        const double ay2 = y[i].x * y[i].x + y[i].y * y[i].y;
+        const double sc = atol + rtol * sqrt(ay2);
+        s0 += ay2 / (sc * sc);
+        s1 += ((double)k[0][i].x * k[0][i].x + (double)k[0][i].y * k[0][i].y) / (sc * sc);
+      }
+      const double d0 = sqrt(bsum<T>(s0, red) * inv_n), d1 = sqrt(bsum<T>(s1, red) * inv_n);
+      const double h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
+      for (int i = threadIdx.x; i < np; i += T)
+        ys[i] = make_float2((float)(y[i].x + h0 * k[0][i].x), (float)(y[i].y + h0 * k[0][i].y));
+      __syncthreads();
+      if (smooth) coefficients(p, seg, t + h0, cf);
+      field<T>(p, cf, ys, k[1]);
+      ++nrhs;
+      double s2 = 0.0;
+      for (int i = threadIdx.x; i < n; i += T) {
+        const double sc = atol + rtol * sqrt(y[i].x * y[i].x + y[i].y * y[i].y);
+        const double dr = (double)k[1][i].x - k[0][i].x, di = (double)k[1][i].y - k[0][i].y;
+        s2 += (dr * dr + di * di) / (sc * sc);
+      }
+      const double d2 = sqrt(bsum<T>(s2, red) * inv_n) / h0;
+      const double md = fmax(d1, d2);
+      const double h1 = md <= 1e-15 ? fmax(1e-6, h0 * 1e-3) : ctrl_pow(0.01 / md, 0.2);
+      dt = fmin(100.0 * h0, h1);
+    }
+    bool need_k0 = false;
+    while (t < t_end) {
+      if (nsteps >= p.max_steps) {
+        status = DQH_MAX_STEPS;
+        break;
+      }
+      if (pwc) {
+        const int s = segment_of(ptimes, nseg, t);
+        if (s != seg) seg = s;
+      }
+      if (need_k0) {
+        coefficients(p, seg, t, cf);
+        for (int i = threadIdx.x; i < np; i += T) ys[i] = make_float2((float)y[i].x, (float)y[i].y);
+        __syncthreads();
+        field<T>(p, cf, ys, k[0]);
+        ++nrhs;
+        need_k0 = false;
+      }
+      double h = dt, target = t_end;
+      bool target_jump = false;
+      if (pwc) {
+        const int m = segment_of(ptimes, nseg, t) + 1;
+        if (m < nseg && ptimes[m] < t_end) {
+          target = ptimes[m];
+          target_jump = true;
+        }
+      }
+      bool made_jump = false;
+      double t_new = t + h;
+      if (snaps_to(t, h, target)) {
+        h = target - t;
+        t_new = target;
+        made_jump = target_jump;
+      }
+      if (!(h > 0.0) || !(h > 1e-14 * fmax(1.0, fabs(t)))) {
+        status = DQH_NONFINITE;
+        break;
+      }
+      if (!smooth) coefficients(p, seg, t, cf);
+      for (int s = 1; s < 7; ++s) {
+        for (int i = threadIdx.x; i < np; i += T) {
+          double re = y[i].x, im = y[i].y;
+          for (int j = 0; j < s; ++j) {
+            re += h * rk.A[s][j] * k[j][i].x;
+            im += h * rk.A[s][j] * k[j][i].y;
+          }
+          if (s == 6) y1[i] = make_double2(re, im);
+          ys[i] = make_float2((float)re, (float)im);
+        }
+        __syncthreads();
+        if (smooth) coefficients(p, seg, t + rk.C[s] * h, cf);
+        field<T>(p, cf, ys, k[s]);
+        ++nrhs;
+      }
+      double ssum = 0.0;
+      for (int i = threadIdx.x; i < n; i += T) {
+        double er = 0.0, ei = 0.0;
+        for (int j = 0; j < 7; ++j) {
+          er += rk.E[j] * k[j][i].x;
+          ei += rk.E[j] * k[j][i].y;
+        }
+        er *= h;
+        ei *= h;
+        const double sc = atol + rtol * fmax(sqrt(y[i].x * y[i].x + y[i].y * y[i].y),
+                                             sqrt(y1[i].x * y1[i].x + y1[i].y * y1[i].y));
+        ssum += (er * er + ei * ei) / (sc * sc);
+      }
+      const double norm = sqrt(bsum<T>(ssum, red) * inv_n);
+      const bool finite = isfinite(norm), keep = finite && norm < 1.0;
+      double factor = !finite ? p.factormin : (norm == 0.0 ? p.factormax : p.safety * ctrl_pow(norm, -0.2));
+      factor = fmin(fmax(factor, keep ? 1.0 : p.factormin), p.factormax);
+      ++nsteps;
+      if (keep) {
+        if (threadIdx.x == 0 && nacc < p.step_log_cap) gp<double>(p.step_log)[nacc] = t_new;
+        ++nacc;
+        while (isave < ntsave && tsave[isave] <= t_new) {
+          const double th = (tsave[isave] - t) / h;
+          double w[7];
+          for (int j = 0; j < 7; ++j) w[j] = th * (rk.R[j][0] + th * (rk.R[j][1] + th * (rk.R[j][2] + th * rk.R[j][3])));
+          save_at<T>(p, y, k, h, w, true, isave, vsave, red);
+          ++isave;
+        }
+        for (int i = threadIdx.x; i < np; i += T) {
+          y[i] = y1[i];
+          k[0][i] = k[6][i];
+        }
+        __syncthreads();
+        t = t_new;
+        need_k0 = made_jump;
+      } else {
+        ++nrej;
+      }
+      dt = factor * h;
+    }
   }
   while (isave < ntsave) {
     save_at<T>(p, y, k, 0.0, w0, false, isave, vsave, red);

@@ -307,6 +307,13 @@ def _kraus_sets(H: np.ndarray, Ls, h: float, normalize: bool, order: int = 1):
     tmp = dA rho (I + dA^dag / 2) + sum_{k>0} A_k rho A_k^dag / 2, dA = A_0 - I,
     i.e. D = [dA, A_k], C = [I + dA^dag / 2, A_k^dag / 2]."""
     n = H.shape[-1]
+    if order == 0:
+        # Euler on the Lindblad equation (diffrax Euler in the reference's Euler
+        # integrator): rho' = rho + h (G rho + rho G^dag + sum L rho L^dag), G = -i H -
+        # 1/2 sum L^dag L, i.e. tmp = h G rho + sum (sqrt(h) L) rho (sqrt(h) L)^dag / 2
+        LdL = sum((_dag(L) @ L for L in Ls), np.zeros_like(H))
+        G = -1j * H - 0.5 * LdL
+        return [h * G, *[np.sqrt(h) * L for L in Ls]], [np.eye(n), *[0.5 * np.sqrt(h) * _dag(L) for L in Ls]]
     Mall = kraus_operators(H, Ls, h, order)
     M0, Ms = Mall[0], Mall[1:]
     if normalize:
@@ -371,7 +378,11 @@ def build_plan(
 
     ``tableau`` is the 5(4) pair of the adaptive kinds: 'tsit5' (``Tsit5``) or
     'dopri5' (``Dopri5``, ``DQH_FLAG_DOPRI5``); both use the same controller,
-    error norm and saving."""
+    error norm and saving.
+
+    ``rouchon_order`` selects the fixed-step map of kind 'rouchon1': 1, 2, 3 for
+    Rouchon1/2/3 (Kraus operators at the step midpoint, optionally normalised) and
+    0 for Euler (the Lindblad vector field at the step start, no normalisation)."""
     import torch
 
     if tableau not in ('tsit5', 'dopri5'):
@@ -476,7 +487,7 @@ def build_plan(
         p_step_t = arena.put(step_t)
         hs = np.diff(step_t)
         mids = step_t[:-1] + 0.5 * hs
-        nLdev = {1: K + 1, 2: 1 + 2 * K + K * K, 3: 1 + 2 * K + K * K + K**3}[rouchon_order]
+        nLdev = {0: K + 1, 1: K + 1, 2: 1 + 2 * K + K * K, 3: 1 + 2 * K + K * K + K**3}[rouchon_order]
     else:
         nLdev = K
     # fields common to the batch, as whole columns
@@ -546,7 +557,9 @@ def build_plan(
                 grid = np.unique(np.concatenate([np.asarray(t, dtype=np.float64) for (t, _, _) in inst.pwc]))
             keys = []
             for m in range(nsteps):
-                seg = -1 if grid is None else int(np.searchsorted(grid, mids[m], side='right')) - 1
+                # Rouchon takes the operators at the step midpoint, Euler (order 0) at its start
+                tsel = step_t[m] if rouchon_order == 0 else mids[m]
+                seg = -1 if grid is None else int(np.searchsorted(grid, tsel, side='right')) - 1
                 keys.append((seg, float(hs[m])))
             uniq = sorted(set(keys))
             index = {k: i for i, k in enumerate(uniq)}

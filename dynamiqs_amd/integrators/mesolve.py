@@ -19,7 +19,7 @@ import numpy as np
 
 from .._checks import check_hermitian, check_qarray_is_dense, check_shape, check_times
 from ..gradient import Gradient
-from ..method import Dopri5, Method, Rouchon1, Rouchon2, Rouchon3, Tsit5, _DEAdaptiveStep
+from ..method import Dopri5, Euler, Method, Rouchon1, Rouchon2, Rouchon3, Tsit5, _DEAdaptiveStep
 from ..options import Options, check_options
 from ..qarrays import QArray, asqarray
 from ..result import AdaptiveStepInfos, FixedStepInfos, MESolveResult
@@ -28,7 +28,7 @@ from . import dense
 from . import dia as dia_path
 from .operators import decompose
 
-SUPPORTED_METHODS = (Tsit5, Dopri5, Rouchon1, Rouchon2, Rouchon3)
+SUPPORTED_METHODS = (Tsit5, Dopri5, Euler, Rouchon1, Rouchon2, Rouchon3)
 ADAPTIVE = (Tsit5, Dopri5)
 
 
@@ -50,6 +50,13 @@ def max_steps_error() -> RuntimeError:
         ' `max_steps` with the `method` argument, e.g.'
         ' `method=dq.method.Tsit5(max_steps=1_000_000)`.'
     )
+
+
+def _fixed_step_order(method) -> int:
+    """The dense Kraus kernel's map for a fixed-step method: 0 Euler, 1-3 Rouchon."""
+    if isinstance(method, Euler):
+        return 0
+    return 3 if isinstance(method, Rouchon3) else (2 if isinstance(method, Rouchon2) else 1)
 
 
 def _check_mesolve_args(H: TimeQArray, Ls, rho0: QArray, exp_ops) -> None:
@@ -220,9 +227,9 @@ def mesolve(
             exp_ops=Es,
             t0=t0,
             dt=method.dt,
-            normalize=method.normalize,
+            normalize=getattr(method, 'normalize', False),
             save_states=save_all,
-            rouchon_order=3 if isinstance(method, Rouchon3) else (2 if isinstance(method, Rouchon2) else 1),
+            rouchon_order=_fixed_step_order(method),
         )
     plan.run()
     res = plan.results()

@@ -17,18 +17,18 @@ import numpy as np
 from .. import _native
 from .._checks import check_shape, check_times
 from ..gradient import Gradient
-from ..method import Dopri5, Method, Tsit5
+from ..method import Dopri5, Euler, Method, Tsit5
 from ..options import Options, check_options
 from ..qarrays import QArray, asqarray
-from ..result import AdaptiveStepInfos, SESolveResult
+from ..result import AdaptiveStepInfos, FixedStepInfos, SESolveResult
 from ..time_qarray import TimeQArray, astimeqarray
-from .dense import MAX_NP, _Arena, _device, _pad, check_descriptors, pad16
+from .dense import MAX_NP, _Arena, _device, _pad, check_descriptors, pad16, rouchon_step_times
 from .mesolve import assert_method_supported, max_steps_error, save_extra_outputs
 from .operators import decompose, resample_pwc, spline_table, union_breakpoints
 
-SUPPORTED_METHODS = (Tsit5, Dopri5)
+SUPPORTED_METHODS = (Tsit5, Dopri5, Euler)
 MAX_PWC = 8
-SE_FIELDS = ('tsave', 'Hnh', 'EopsT', 'Hpwc', 'pwc_times', 'pwc_vals', 'Hmod', 'mod_coef')
+SE_FIELDS = ('tsave', 'Hnh', 'EopsT', 'Hpwc', 'pwc_times', 'pwc_vals', 'Hmod', 'mod_coef', 'step_t')
 
 
 @dataclass
@@ -89,8 +89,10 @@ class SEPlan:
 
 
 def _header(n: int, tsave, Es, t0, rtol, atol, safety, factormin, factormax, max_steps, save_states, tableau,
-            B: int, arena):
-    """Descriptor columns common to a batch (arena offsets for the shared inputs)."""
+            B: int, arena, euler_dt=None):
+    """Descriptor columns common to a batch (arena offsets for the shared inputs).
+    With ``euler_dt`` the kernel takes fixed Euler steps of that size (the last one
+    clipped to tsave[-1]) instead of the adaptive pair."""
     if tableau not in ('tsit5', 'dopri5'):
         raise ValueError(f'unknown Runge-Kutta pair {tableau!r}')
     npad = pad16(n)
@@ -106,6 +108,11 @@ def _header(n: int, tsave, Es, t0, rtol, atol, safety, factormin, factormax, max
     flags = _native.FLAG_SAVE_STATES if save_states else 0
     if tableau == 'dopri5':
         flags |= _native.FLAG_DOPRI5
+    if euler_dt is not None:
+        step_t = rouchon_step_times(float(descs['t0'][0]), float(descs['t1'][0]), float(euler_dt))
+        flags |= _native.FLAG_EULER
+        descs['step_t'] = arena.put(step_t)
+        descs['max_steps'] = len(step_t) - 1
     descs['flags'] = flags
     return descs
 
@@ -146,6 +153,7 @@ def _finish(plan: SEPlan, descs, arena, y0: np.ndarray, dev, step_log_cap: int) 
         'Hpwc': descs['npwc'].astype(u) * npd * npd * u(8),
         'pwc_times': descs['nseg'].astype(u) * u(8),
         'pwc_vals': descs['npwc'].astype(u) * np.maximum(descs['nseg'].astype(np.int64) - 1, 0).astype(u) * u(16),
+        'step_t': (np.maximum(descs['max_steps'], 0).astype(u) + u(1)) * u(8),
     }
     tensors = [*arena.tensors, plan.y, plan.states, plan.final, plan.expects, plan.stats]
     if plan.step_log is not None:
@@ -177,6 +185,7 @@ def build_plan(
     save_states: bool = True,
     tableau: str = 'tsit5',
     step_log_cap: int = 0,
+    euler_dt: float | None = None,
 ) -> SEPlan:
     """Pack ``instances`` (same Hilbert dimension) for ``dqh_sesolve_tsit5``.
 
@@ -199,7 +208,7 @@ def build_plan(
     Es = [np.asarray(E, dtype=np.complex128) for E in (exp_ops or [])]
     arena = _Arena(dev)
     descs = _header(n, tsave, Es, t0, rtol, atol, safety, factormin, factormax, max_steps, save_states, tableau, B,
-                    arena)
+                    arena, euler_dt)
     npad = int(descs['np'][0])
     t_start, t_end = float(descs['t0'][0]), float(descs['t1'][0])
     plan = SEPlan(n, len(tsave), len(Es), B, save_states)
@@ -267,6 +276,7 @@ def build_plan_constant(
     save_states: bool = True,
     tableau: str = 'tsit5',
     step_log_cap: int = 0,
+    euler_dt: float | None = None,
 ) -> SEPlan:
     """The same plan for constant Hamiltonians, packed as arrays: instance b
     integrates ``Hs[h_index[b]]`` (``Hs``: (nH, n, n)) from ``psis[p_index[b]]``
@@ -291,7 +301,7 @@ def build_plan_constant(
     Es = [np.asarray(E, dtype=np.complex128) for E in (exp_ops or [])]
     arena = _Arena(dev)
     descs = _header(n, tsave, Es, t0, rtol, atol, safety, factormin, factormax, max_steps, save_states, tableau, B,
-                    arena)
+                    arena, euler_dt)
     npad = int(descs['np'][0])
     base = arena.put(_G_T(Hs, npad))
     descs['Hnh'] = np.uint64(base) + h_index.astype(np.uint64) * np.uint64(npad * npad * 8)
@@ -373,7 +383,7 @@ def sesolve(
 
     Same signature, batching rules, errors and result as ``dq.sesolve``
     (``dynamiqs/integrators/apis/sesolve.py:36-230``). Supported methods:
-    ``Tsit5`` and ``Dopri5``; supported Hamiltonians: constant (dense or DIA,
+    ``Tsit5``, ``Dopri5`` and ``Euler`` (fixed step); supported Hamiltonians: constant (dense or DIA,
     n <= 256), piecewise-constant (``dq.pwc``) and modulated (``dq.modulated``)
     terms and their sums.
     """
@@ -396,15 +406,20 @@ def sesolve(
     kw = dict(
         exp_ops=Es,
         t0=None if options.t0 is None else float(options.t0),
-        rtol=method.rtol,
-        atol=method.atol,
-        safety=method.safety_factor,
-        factormin=method.min_factor,
-        factormax=method.max_factor,
-        max_steps=method.max_steps,
         save_states=options.save_states or options.save_extra is not None,
-        tableau='dopri5' if isinstance(method, Dopri5) else 'tsit5',
     )
+    if isinstance(method, Euler):
+        kw['euler_dt'] = float(method.dt)
+    else:
+        kw.update(
+            rtol=method.rtol,
+            atol=method.atol,
+            safety=method.safety_factor,
+            factormin=method.min_factor,
+            factormax=method.max_factor,
+            max_steps=method.max_steps,
+            tableau='dopri5' if isinstance(method, Dopri5) else 'tsit5',
+        )
     Ht = decompose(H)
     if Ht.is_constant:
         out_shape, Hs, h_index, psis, p_index = constant_batch(Ht, psi0, options.cartesian_batching)
@@ -436,5 +451,8 @@ def sesolve(
         x = np.asarray(x).reshape(out_shape)
         return x if out_shape else x[()]
 
-    infos = AdaptiveStepInfos(shaped(res['nsteps']), shaped(res['naccepted']), shaped(res['nrejected']))
+    if isinstance(method, Euler):
+        infos = FixedStepInfos(shaped(res['nsteps']))
+    else:
+        infos = AdaptiveStepInfos(shaped(res['nsteps']), shaped(res['naccepted']), shaped(res['nrejected']))
     return SESolveResult(tsave, method, gradient, options, ysave, Esave, extra, infos)

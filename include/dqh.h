@@ -93,6 +93,7 @@ typedef struct dqh_me_inst {
 #define DQH_FLAG_HAS_HNH 2
 #define DQH_FLAG_GLOBAL 4 /* run the global-memory kernel form even where the LDS-resident one applies */
 #define DQH_FLAG_DOPRI5 8 /* Tsit5 kernels: integrate with the Dopri5 pair (same controller and saving) */
+#define DQH_FLAG_EULER 16 /* dqh_sesolve_tsit5: fixed Euler steps between step_t[0..max_steps] */
 
 /* Size checks for FFI mirrors. */
 int64_t dqh_sizeof_me_inst(void);

@@ -153,15 +153,21 @@ def mesolve(
             tab=tsit5.tableau(method),
         )
         infos = dict(nsteps=st.nsteps, naccepted=st.naccepted, nrejected=st.nrejected, nrhs=st.nrhs)
-    elif method in ('rouchon1', 'rouchon2', 'rouchon3'):
+    elif method in ('rouchon1', 'rouchon2', 'rouchon3', 'euler'):
         if dt is None:
-            raise ValueError('fixed-step Rouchon needs dt')
-        kstep = {'rouchon1': rouchon.kraus_step_dense, 'rouchon2': rouchon.rouchon2_step,
-                 'rouchon3': rouchon.rouchon3_step}[method]
+            raise ValueError('fixed-step methods need dt')
+        if method == 'euler':
+            # diffrax Euler: y + h f(t, y), the vector field at the step start
 
-        def step(t, h, rho):
-            tm = t + 0.5 * h
-            return kstep(H(tm), [L(tm) for L in Ls], rho, h, normalize)
+            def step(t, h, rho):
+                return rho + h * lindblad.rhs_standard(H(t), [L(t) for L in Ls], rho)
+        else:
+            kstep = {'rouchon1': rouchon.kraus_step_dense, 'rouchon2': rouchon.rouchon2_step,
+                     'rouchon3': rouchon.rouchon3_step}[method]
+
+            def step(t, h, rho):
+                tm = t + 0.5 * h
+                return kstep(H(tm), [L(tm) for L in Ls], rho, h, normalize)
 
         saved, ylast, nsteps = rouchon.integrate(step, rho0, tsave, dt, t0=t0, save=save)
         infos = dict(nsteps=nsteps)

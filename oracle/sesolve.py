@@ -1,7 +1,8 @@
 """``sesolve`` driver in complex128 (oracle).
 
 Restates the single-instance path of ``dynamiqs/integrators/apis/sesolve.py``
-(``_sesolve`` :263-304) for ``Tsit5`` and ``Dopri5``:
+(``_sesolve`` :263-304) for ``Tsit5``, ``Dopri5`` and ``Euler`` (fixed steps, the
+vector field at the step start, linear interpolation to ``tsave``):
 
 * the vector field is ``-1j * H(t) @ psi`` (``SEDiffraxIntegrator.terms``,
   ``dynamiqs/integrators/core/diffrax_integrator.py:225-228``), with the time
@@ -49,6 +50,7 @@ def sesolve(
     max_steps: int = 100_000,
     t0: float | None = None,
     save_states: bool = True,
+    dt: float | None = None,
 ) -> dict:
     H = as_timeop(H)
     psi0 = np.asarray(psi0, dtype=np.complex128).reshape(-1, 1)
@@ -64,6 +66,18 @@ def sesolve(
         ts = _segment_time(t, tseg, jumps)
         return -1j * (H(ts, t) @ y)
 
+    if method == 'euler':
+        # diffrax Euler: psi + dt f(t, psi) at the step start, fixed steps, linear
+        # interpolation to tsave (the fixed-step loop of oracle.rouchon)
+        from . import rouchon
+
+        if dt is None:
+            raise ValueError('Euler needs dt')
+        saved, ylast, nsteps = rouchon.integrate(lambda t, h, y: y + h * f(t, y, t), psi0, tsave, dt, t0=t0,
+                                                 save=save)
+        states = np.stack([s[0] for s in saved]) if save_states else ylast[None]
+        expects = np.stack([s[1] for s in saved], axis=-1) if Es else None
+        return dict(states=states, final_state=ylast, expects=expects, infos=dict(nsteps=nsteps))
     if method not in ('tsit5', 'dopri5'):
         raise ValueError(method)
     saved, ylast, st = tsit5.integrate(

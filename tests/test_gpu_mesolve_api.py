@@ -226,3 +226,28 @@ def test_rouchon3_fixed_step_matches_oracle():
     r2 = dq.mesolve(H, Ls, rho0[1], tsave, method=dq.method.Rouchon2(dt=2e-2)).states.to_numpy()[-1]
     assert np.linalg.norm(r3 - exact) < np.linalg.norm(r2 - exact)
     assert str(res).split('\n')[1] == 'Method  : Rouchon3'
+
+
+def test_euler_fixed_step_matches_oracle():
+    """dq.method.Euler(dt) for mesolve on the dense Kraus kernel (sets D = [dt G,
+    sqrt(dt) L_k], C = [I, sqrt(dt) L_k^dag / 2], the operators at the step start)
+    against the oracle's Euler, with a PWC drive; trace preserved."""
+    from oracle import mesolve as omesolve
+
+    n = 6
+    a = dq.destroy(n)
+    H0 = (a.dag() @ a).to_numpy()
+    X = (a + a.dag()).to_numpy()
+    times, vals = np.array([0.0, 0.33, 0.8]), np.array([0.4, -0.7])
+    H = dq.asqarray(H0) + dq.pwc(times, vals, dq.asqarray(X))
+    Ls = [0.4 * a]
+    rho0 = dq.coherent_dm(n, 0.6).to_numpy()
+    tsave = np.linspace(0.0, 1.0, 5)
+    res = dq.mesolve(H, Ls, rho0, tsave, method=dq.method.Euler(dt=1e-3))
+    Hop = omesolve.TimeOp(H0, [omesolve.PWCTerm(times, vals, X)])
+    ref = omesolve.mesolve(Hop, [L.to_numpy() for L in Ls], rho0, tsave, method='euler', dt=1e-3)
+    err = np.linalg.norm(res.states.to_numpy() - ref['states'], axis=(-2, -1)).max()
+    print('euler mesolve err vs oracle', err)
+    assert err <= 1e-5
+    assert abs(np.trace(res.states.to_numpy()[-1]) - 1) < 1e-5
+    assert int(res.infos.nsteps) == 1000

@@ -215,3 +215,29 @@ def test_save_extra_and_t0():
     k = np.arange(6)
     exact = np.stack([np.exp(-1j * 2 * np.pi * k * t) * psi0.to_numpy()[:, 0] for t in ts])
     assert np.max(np.abs(res.states.to_numpy()[..., 0] - exact)) <= 1e-5
+
+
+def test_euler_fixed_step_against_oracle():
+    """dq.method.Euler(dt) on the sesolve kernel (fixed steps, the operators at the
+    step start, linear interpolation to the save times) against the oracle's Euler,
+    with a PWC and a modulated term; FixedStepInfos counts the steps."""
+    import dynamiqs_amd as dq
+    from oracle import mesolve as om
+    from oracle import sesolve as ose
+
+    rng = np.random.default_rng(41)
+    n = 12
+    H0, O, M = _herm(rng, n), _herm(rng, n), _herm(rng, n)
+    times, vals = np.array([0.0, 0.37, 1.0]), np.array([0.5, -0.8])
+    f = lambda t: 0.6 * np.cos(2.0 * t)  # noqa: E731
+    psi0 = _ket(rng, n)
+    ts = np.linspace(0.0, 1.2, 7)
+    H = dq.asqarray(H0) + dq.pwc(times, vals, dq.asqarray(O)) + dq.modulated(f, dq.asqarray(M))
+    res = dq.sesolve(H, dq.asqarray(psi0[:, None]), ts, exp_ops=[dq.asqarray(O)], method=dq.method.Euler(dt=1e-3))
+    Hop = om.TimeOp(H0, [om.PWCTerm(times, vals, O), om.ModTerm(f, M)])
+    ref = ose.sesolve(Hop, psi0, ts, exp_ops=[O], method='euler', dt=1e-3)
+    err = np.max(np.abs(res.states.to_numpy()[..., 0] - ref['states'][..., 0]))
+    print('euler sesolve max err vs oracle', err, 'steps', res.infos.nsteps)
+    assert err <= 1e-5
+    np.testing.assert_allclose(res.expects[0], ref['expects'][0], atol=1e-5)
+    assert int(res.infos.nsteps) == 1200 and 'steps' in str(res.infos)

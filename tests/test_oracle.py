@@ -758,3 +758,58 @@ def test_rouchon3_third_order_trace_and_propagators():
         errs.append([np.linalg.norm(U[c] - sla.expm(c * dt * G)) for c in (1.0 / 3.0, 2.0 / 3.0, 1.0)])
     ratios = np.array(errs[0]) / np.array(errs[1])
     assert np.all((ratios > 12) & (ratios < 20)), ratios  # local error O(dt^4)
+
+
+# ----------------------------------------------------------------- Euler
+
+
+def test_euler_step_identity_order_and_host_sets():
+    """Euler (diffrax Euler on the Lindblad equation): rho + h L(rho) equals the
+    unnormalised Rouchon1 map minus h^2 G rho G^dag (M0 = I + h G); first-order
+    convergence to expm; exact trace preservation; the host's kernel sets (order 0:
+    D = [h G, sqrt(h) L_k], C = [I, sqrt(h) L_k^dag / 2]) give the same step."""
+    from dynamiqs_amd.integrators.dense import _dag, _kraus_sets
+
+    rng = np.random.default_rng(31)
+    H, Ls, rho = _rand_problem(rng, 5, 2)
+    h = 0.03
+    G = -1j * H - 0.5 * sum(L.conj().T @ L for L in Ls)
+    ts = np.array([0.0, h])
+    eul = om.mesolve(H, Ls, rho, ts, method='euler', dt=h)['states'][-1]
+    r1 = rouchon.kraus_step_dense(H, Ls, rho, h, False)
+    np.testing.assert_allclose(eul, r1 - h * h * G @ rho @ G.conj().T, atol=1e-13)
+    D, C = _kraus_sets(H, Ls, h, False, order=0)
+    tmp = sum(d @ rho @ c for d, c in zip(D, C))
+    np.testing.assert_allclose(rho + tmp + _dag(tmp), eul, atol=1e-13)
+    assert abs(np.trace(eul) - np.trace(rho)) < 1e-13
+    n = 5
+    a = op.destroy(n)
+    H2 = a.conj().T @ a + 0.2 * (a + a.conj().T)
+    L2 = [0.5 * a]
+    rho0 = op.coherent_dm(n, 0.7)
+    ex = lindblad.unvectorize(sla.expm(lindblad.slindbladian(H2, L2)) @ lindblad.vectorize(rho0))
+    e = [np.linalg.norm(om.mesolve(H2, L2, rho0, np.array([0.0, 1.0]), method='euler', dt=dt)['states'][-1] - ex)
+         for dt in (0.004, 0.002)]
+    assert 1.8 < e[0] / e[1] < 2.2
+
+
+def test_sesolve_euler_first_order():
+    """Euler for sesolve converges at first order to exp(-i H t) psi0 and saves by
+    linear interpolation between steps (a save time inside a step)."""
+    from oracle import sesolve as ose
+
+    rng = np.random.default_rng(12)
+    n = 4
+    x = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    H = (x + x.conj().T) / 4
+    psi0 = np.eye(n)[:, :1].astype(complex)
+    ex = sla.expm(-1j * H) @ psi0
+    e = [np.linalg.norm(ose.sesolve(H, psi0, np.array([0.0, 1.0]), method='euler', dt=dt)['states'][-1] - ex)
+         for dt in (0.002, 0.001)]
+    assert 1.8 < e[0] / e[1] < 2.2
+    out = ose.sesolve(H, psi0, np.array([0.0, 0.25, 0.3]), method='euler', dt=0.1)
+    y = [psi0]
+    for _ in range(3):
+        y.append(y[-1] - 0.1j * H @ y[-1])
+    np.testing.assert_allclose(out['states'][1], y[2] + 0.5 * (y[3] - y[2]), atol=1e-14)
+    assert out['infos']['nsteps'] == 3
