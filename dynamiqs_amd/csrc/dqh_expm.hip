@@ -30,7 +30,7 @@ constexpr int64_t kMaxGridY = 65535;
 // each wave's chain of dependent loads is four times shorter) and wave 0 adds the
 // partial tiles from LDS and stores with the epilogue.
 __global__ __launch_bounds__(64 * kWaves) void k_cgemm(const float2 *__restrict__ A, int64_t sA,
-                                                      const float2 *__restrict__ B, int64_t sB,
+                                                      const float2 *__restrict__ B, int64_t sB, int64_t bdiv,
                                                       float2 *__restrict__ C, int64_t sC, int N, float scale,
                                                       int add_id, int64_t b0) {
   __shared__ float part[kWaves - 1][8][64];
@@ -45,7 +45,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_cgemm(const float2 *__restrict_
   Acc acc;
   acc_zero(acc);
   if (kend > kbeg)
-    tile_mac(acc, A + b * sA + kbeg, N, B + b * sB + (int64_t)kbeg * N, N, 16 * tr, 16 * tc, kend - kbeg, lane);
+    tile_mac(acc, A + b * sA + kbeg, N, B + (b / bdiv) * sB + (int64_t)kbeg * N, N, 16 * tr, 16 * tc, kend - kbeg,
+             lane);
   if (wave > 0) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
@@ -94,16 +95,18 @@ __global__ __launch_bounds__(256) void k_first(const float2 *__restrict__ E, flo
   }
 }
 
+// C[b] = scale A[b] B[b / bdiv] (+ I) for b < nbatch
 int gemm(const float2 *A, int64_t sA, const float2 *B, int64_t sB, float2 *C, int64_t sC, int N, float scale,
-         int add_id, int64_t nbatch, hipStream_t st) {
+         int add_id, int64_t nbatch, hipStream_t st, int64_t bdiv = 1) {
   const int nt = N / 16;
   const unsigned gx = (unsigned)(nt * nt);
   for (int64_t b0 = 0; b0 < nbatch; b0 += kMaxGridY) {
     const unsigned gy = (unsigned)(nbatch - b0 < kMaxGridY ? nbatch - b0 : kMaxGridY);
-    hipLaunchKernelGGL(k_cgemm, dim3(gx, gy), dim3(64 * kWaves), 0, st, A, sA, B, sB, C, sC, N, scale, add_id, b0);
+    hipLaunchKernelGGL(k_cgemm, dim3(gx, gy), dim3(64 * kWaves), 0, st, A, sA, B, sB, bdiv, C, sC, N, scale, add_id,
+                       b0);
   }
   const hipError_t err = hipGetLastError();
-  return err == hipSuccess ? DQH_OK : fail(DQH_HIP_ERROR, std::string("dqh_expm_propagators: ") + hipGetErrorString(err));
+  return err == hipSuccess ? DQH_OK : fail(DQH_HIP_ERROR, std::string("dqh batched GEMM: ") + hipGetErrorString(err));
 }
 
 }  // namespace ex
@@ -154,4 +157,13 @@ extern "C" int dqh_expm_propagators(const void *X, void *work, void *U, int64_t 
   const hipError_t err = hipGetLastError();
   if (err != hipSuccess) return fail(DQH_HIP_ERROR, std::string("dqh_expm_propagators: ") + hipGetErrorString(err));
   return DQH_OK;
+}
+
+extern "C" int dqh_cgemm_batched(const void *A, int64_t sA, const void *B, int64_t sB, int64_t bdiv, void *C,
+                                 int64_t sC, int64_t N, int64_t nbatch, void *stream) {
+  if (!A || !B || !C || N <= 0 || N % 16 != 0 || N > 4096 || nbatch <= 0 || bdiv <= 0 || sA < 0 || sB < 0 ||
+      sC < N * N)
+    return dqh::fail(DQH_BAD_ARG, "dqh_cgemm_batched: bad arguments");
+  return dqh::ex::gemm((const float2 *)A, sA, (const float2 *)B, sB, (float2 *)C, sC, (int)N, 1.f, 0, nbatch,
+                       (hipStream_t)stream, bdiv);
 }

@@ -19,7 +19,7 @@ import numpy as np
 
 from .._checks import check_hermitian, check_qarray_is_dense, check_shape, check_times
 from ..gradient import Gradient
-from ..method import Dopri5, Euler, Method, Rouchon1, Rouchon2, Rouchon3, Tsit5, _DEAdaptiveStep
+from ..method import Dopri5, Euler, Expm, Method, Rouchon1, Rouchon2, Rouchon3, Tsit5, _DEAdaptiveStep
 from ..options import Options, check_options
 from ..qarrays import QArray, asqarray
 from ..result import AdaptiveStepInfos, FixedStepInfos, MESolveResult
@@ -28,7 +28,7 @@ from . import dense
 from . import dia as dia_path
 from .operators import decompose
 
-SUPPORTED_METHODS = (Tsit5, Dopri5, Euler, Rouchon1, Rouchon2, Rouchon3)
+SUPPORTED_METHODS = (Tsit5, Dopri5, Euler, Rouchon1, Rouchon2, Rouchon3, Expm)
 ADAPTIVE = (Tsit5, Dopri5)
 
 
@@ -179,6 +179,11 @@ def mesolve(
         raise NotImplementedError(
             f'the adaptive {type(method).__name__} (dt=None) is not supported by the MI355X solver: set `dt`'
         )
+    if isinstance(method, Expm):  # propagators on MFMA applied to the initial states
+        from .propagators import expm_solve
+
+        Es_np = [E.to_numpy() for E in exp_ops] if exp_ops is not None else None
+        return expm_solve(H, Ls, rho0, tsave, Es_np, method, gradient, options, me=True)
 
     route = dia_route(method, H, Ls, H.shape[-1])
     if route is not None:

@@ -77,10 +77,12 @@ def expm_times(t0: float, tsave: np.ndarray, disc: np.ndarray):
     return times, np.searchsorted(times, tsave)
 
 
-def expm_chain(X: np.ndarray) -> np.ndarray:
+def expm_chain(X: np.ndarray, Y0: np.ndarray | None = None) -> np.ndarray:
     """Chained exponentials on the GPU: ``X`` (nchain, nint, N, N) complex holds
     dt_k A_k; returns U (nchain, nint, N, N) complex64 with
-    U[b, k] = expm(X[b, k]) ... expm(X[b, 0]) (``dqh_expm_propagators``)."""
+    U[b, k] = expm(X[b, k]) ... expm(X[b, 0]) (``dqh_expm_propagators``), or, with
+    ``Y0`` (nchain, N, M) a block of initial states per chain as columns, the states
+    U[b, k] Y0[b] (nchain, nint, N, M) (``dqh_cgemm_batched`` on the resident U)."""
     import torch
 
     from .dense import _device
@@ -103,6 +105,23 @@ def expm_chain(X: np.ndarray) -> np.ndarray:
                                  TAYLOR_DEGREE, s, c(stream.cuda_stream)),
         'dqh_expm_propagators',
     )
+    if Y0 is not None:
+        M = Y0.shape[-1]
+        blocks = []
+        Y = torch.empty_like(x)
+        for c0 in range(0, M, Np):  # at most Np states per product
+            c1 = min(M, c0 + Np)
+            Yp = np.zeros((nchain, Np, Np), dtype=np.complex64)
+            Yp[:, :N, : c1 - c0] = Y0[..., c0:c1]
+            y0 = torch.from_numpy(Yp.view(np.float32)).to(dev)
+            _native.check(
+                lib.dqh_cgemm_batched(c(U.data_ptr()), Np * Np, c(y0.data_ptr()), Np * Np, nint, c(Y.data_ptr()), Np * Np,
+                                      Np, nchain * nint, c(stream.cuda_stream)),
+                'dqh_cgemm_batched',
+            )
+            torch.cuda.synchronize()
+            blocks.append(Y.cpu().numpy().view(np.complex64).reshape(nchain, nint, Np, Np)[..., :N, : c1 - c0])
+        return np.concatenate(blocks, axis=-1)
     torch.cuda.synchronize()
     out = U.cpu().numpy().view(np.complex64).reshape(nchain, nint, Np, Np)
     return out[..., :N, :N]
@@ -371,3 +390,104 @@ def mepropagator(
     phi = st.reshape(len(insts), n * n, *st.shape[1:])
     U = superoperator_from_basis(phi)
     return _finish(U, H.dims, tuple(out_shape), tsave, method, gradient, options, res, n * n, MEPropagatorResult)
+
+
+def _batch_parts(shapes, cartesian: bool):
+    """(out_shape, [per-instance tuple of per-argument batch indices]) with the
+    reference's cartesian / broadcast batching rules."""
+    if cartesian:
+        out_shape = tuple(d for sh in shapes for d in sh)
+    else:
+        out_shape = tuple(np.broadcast_shapes(*shapes))
+    parts_all = []
+    for idx in np.ndindex(*out_shape) if out_shape else [()]:
+        if cartesian:
+            parts, o = [], 0
+            for sh in shapes:
+                parts.append(tuple(idx[o : o + len(sh)]))
+                o += len(sh)
+        else:
+            parts = [tuple(i if d > 1 else 0 for i, d in zip(idx[len(idx) - len(sh) :] if sh else (), sh)) for sh in shapes]
+        parts_all.append(tuple(parts))
+    return out_shape, parts_all
+
+
+def expm_solve(H, Ls, y0: QArray, tsave, Es, method, gradient, options, me: bool):
+    """``sesolve`` / ``mesolve`` with ``Expm`` (expm_integrator.py:60-90 with a state
+    as carry): the instances that share their operators form one chain whose
+    propagators are applied on the GPU to a block of initial states (psi0, or
+    vec(rho0) in the column-stacking convention, as columns). The expectation
+    values <psi|E|psi> / Tr(E rho) are formed on the host from the saved states."""
+    from ..result import MESolveResult, SESolveResult
+
+    if not ispwc(H):
+        raise TypeError('Method `Expm` requires a constant or piecewise constant Hamiltonian.')
+    if me and not all(ispwc(L) for L in Ls):
+        raise TypeError('Method `Expm` requires constant or piecewise constant jump operators.')
+    Ht = decompose(H)
+    Lts = [decompose(L) for L in Ls]
+    n = H.shape[-1]
+    y = y0.to_numpy().astype(np.complex128)
+    yb = y.shape[:-2]
+    y = y.reshape(-1, *y.shape[-2:])
+    if me:
+        if y.shape[-1] == 1:
+            y = y @ np.conj(np.swapaxes(y, -1, -2))
+        cols = np.swapaxes(y, -1, -2).reshape(len(y), n * n)  # vec(rho0), column stacking
+        N = n * n
+    else:
+        cols = y[..., 0]
+        N = n
+    shapes = [Ht.batch, *[Lt.batch for Lt in Lts], yb]
+    out_shape, parts_all = _batch_parts(shapes, options.cartesian_batching)
+    chain_keys = sorted({p[:-1] for p in parts_all})
+    chain_of = {k: i for i, k in enumerate(chain_keys)}
+    members = [[] for _ in chain_keys]  # per chain: (instance, y index)
+    for b, p in enumerate(parts_all):
+        yi = int(np.ravel_multi_index(p[-1], yb)) if yb else 0
+        members[chain_of[p[:-1]]].append((b, yi))
+    M = max(len(m) for m in members)
+    Y0 = np.zeros((len(chain_keys), N, M), dtype=np.complex128)
+    for ci, mem in enumerate(members):
+        for j, (_, yi) in enumerate(mem):
+            Y0[ci, :, j] = cols[yi]
+    disc = np.unique(np.concatenate([H.discontinuity_ts, *[L.discontinuity_ts for L in Ls]]))
+    t0 = float(tsave[0] if options.t0 is None else options.t0)
+    times, nbefore = expm_times(t0, tsave, disc)
+    nint = len(times) - 1
+    B, ns = len(parts_all), len(tsave)
+    out = np.empty((B, ns, N), dtype=np.complex64)
+    if nint > 0:
+        dts = np.diff(times)
+        X = np.empty((len(chain_keys), nint, N, N), dtype=np.complex128)
+        for ci, key in enumerate(chain_keys):
+            cache = {}
+            for k in range(nint):
+                ops = (_op_at(Ht, key[0], times[k]), *[_op_at(Lt, p, times[k]) for Lt, p in zip(Lts, key[1:])])
+                kb = b''.join(o.tobytes() for o in ops)
+                if kb not in cache:
+                    cache[kb] = slindbladian(ops[0], ops[1:]) if me else -1j * ops[0]
+                X[ci, k] = dts[k] * cache[kb]
+        Y = expm_chain(X, Y0)
+    for ci, mem in enumerate(members):
+        for j, (b, _) in enumerate(mem):
+            for i, nb in enumerate(nbefore):
+                out[b, i] = Y0[ci, :, j] if nb == 0 else Y[ci, nb - 1, :, j]
+    if me:
+        states = np.swapaxes(out.reshape(B, ns, n, n), -1, -2)  # unvec
+        Esave = None if Es is None else np.stack([np.einsum('ij,btji->bt', E, states) for E in Es], axis=1)
+    else:
+        states = out[..., None]
+        Esave = None if Es is None else np.stack(
+            [np.einsum('bti,ij,btj->bt', states[..., 0].conj(), E, states[..., 0]) for E in Es], axis=1)
+    extra = None
+    if options.save_extra is not None:
+        extra = save_extra_outputs(options.save_extra, states, y0.dims, out_shape)
+    states = states.reshape(*out_shape, ns, *states.shape[-2:])
+    if not options.save_states:
+        states = states[..., -1:, :, :]
+    if Esave is not None:
+        Esave = Esave.astype(np.complex64).reshape(*out_shape, len(Es), ns)
+    infos = FixedStepInfos(np.full(out_shape, nint, dtype=np.int64) if out_shape else np.int64(nint))
+    cls = MESolveResult if me else SESolveResult
+    return cls(tsave, method, gradient, options, QArray(states, dims=y0.dims), Esave, extra, infos)

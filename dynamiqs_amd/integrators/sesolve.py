@@ -17,7 +17,7 @@ import numpy as np
 from .. import _native
 from .._checks import check_shape, check_times
 from ..gradient import Gradient
-from ..method import Dopri5, Euler, Method, Tsit5
+from ..method import Dopri5, Euler, Expm, Method, Tsit5
 from ..options import Options, check_options
 from ..qarrays import QArray, asqarray
 from ..result import AdaptiveStepInfos, FixedStepInfos, SESolveResult
@@ -26,7 +26,7 @@ from .dense import MAX_NP, _Arena, _device, _pad, check_descriptors, pad16, rouc
 from .mesolve import assert_method_supported, max_steps_error, save_extra_outputs
 from .operators import decompose, resample_pwc, spline_table, union_breakpoints
 
-SUPPORTED_METHODS = (Tsit5, Dopri5, Euler)
+SUPPORTED_METHODS = (Tsit5, Dopri5, Euler, Expm)
 MAX_PWC = 8
 SE_FIELDS = ('tsave', 'Hnh', 'EopsT', 'Hpwc', 'pwc_times', 'pwc_vals', 'Hmod', 'mod_coef', 'step_t')
 
@@ -383,7 +383,8 @@ def sesolve(
 
     Same signature, batching rules, errors and result as ``dq.sesolve``
     (``dynamiqs/integrators/apis/sesolve.py:36-230``). Supported methods:
-    ``Tsit5``, ``Dopri5`` and ``Euler`` (fixed step); supported Hamiltonians: constant (dense or DIA,
+    ``Tsit5``, ``Dopri5``, ``Euler`` (fixed step) and ``Expm`` (constant and PWC
+    Hamiltonians); supported Hamiltonians: constant (dense or DIA,
     n <= 256), piecewise-constant (``dq.pwc``) and modulated (``dq.modulated``)
     terms and their sums.
     """
@@ -403,6 +404,10 @@ def sesolve(
     method.assert_supports_gradient(gradient)
 
     Es = [E.to_numpy() for E in exp_ops] if exp_ops is not None else None
+    if isinstance(method, Expm):  # propagators on MFMA applied to the initial kets
+        from .propagators import expm_solve
+
+        return expm_solve(H, [], psi0, tsave, Es, method, gradient, options, me=False)
     kw = dict(
         exp_ops=Es,
         t0=None if options.t0 is None else float(options.t0),

@@ -288,6 +288,14 @@ int64_t dqh_expm_work_bytes(int64_t nchain, int64_t nint, int64_t N);
 int dqh_expm_propagators(const void *X, void *work, void *U, int64_t nchain, int64_t nint, int64_t N, int64_t m,
                          int64_t s, void *stream);
 
+/* Batched complex64 product C[b] = A[b] B[b / bdiv] for b < nbatch (N x N row-major,
+ * N % 16 == 0, strides in elements; C must not alias A or B), the GEMM of
+ * dqh_expm_propagators. The Expm solves apply the propagators to a block of initial
+ * states (vec(rho0) or psi0 as columns): ExpmIntegrator.run's x @ carry with a
+ * state as carry (expm_integrator.py:73-78). */
+int dqh_cgemm_batched(const void *A, int64_t sA, const void *B, int64_t sB, int64_t bdiv, void *C, int64_t sC,
+                      int64_t N, int64_t nbatch, void *stream);
+
 /* ------------------------------------------------------------------ gradients */
 /* Gauss-Legendre quadrature of the adjoint gradient of piecewise-constant controls
  * (replaces the reverse-mode path jax.grad -> dx.RecursiveCheckpointAdjoint,

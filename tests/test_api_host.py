@@ -205,7 +205,7 @@ def test_sesolve_argument_errors():
         dq.sesolve(H, psi, np.array([0.0, 1.0, 0.5]))
     with pytest.raises(ValueError, match=re.escape('Option `vectorized` was set to `True` but is not used by the solver `dq.sesolve()`')):
         dq.sesolve(H, psi, ts, options=dq.Options(vectorized=True))
-    with pytest.raises(TypeError, match=re.escape('Method of type `Rouchon1` is not supported (supported method types: `Tsit5`, `Dopri5`, `Euler`)')):
+    with pytest.raises(TypeError, match=re.escape('Method of type `Rouchon1` is not supported (supported method types: `Tsit5`, `Dopri5`, `Euler`, `Expm`)')):
         dq.sesolve(H, psi, ts, method=dq.method.Rouchon1(dt=0.1))
     with pytest.raises(NotImplementedError, match='CallableTimeQArray operators are not supported'):
         dq.sesolve(dq.timecallable(lambda t: H), psi, ts)

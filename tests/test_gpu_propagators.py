@@ -190,3 +190,49 @@ def test_expm_mepropagator_batched_jump_operators():
     flat = dq.mepropagator(Hs[:, None].repeat(3, axis=1), [Ls], ts, options=dq.Options(cartesian_batching=False))
     assert flat.propagators.shape == (2, 3, 3, n * n, n * n)
     np.testing.assert_allclose(flat.propagators.to_numpy(), P, atol=1e-6)
+
+
+def test_expm_sesolve_and_mesolve_against_matrix_exponentials():
+    """method=Expm() for sesolve and mesolve (the propagator chain applied to the
+    initial states on the GPU): cartesian batches over H and the initial state, a
+    PWC drive, expectation values, FixedStepInfos with the interval count."""
+    import dynamiqs_amd as dq
+    from oracle import lindblad
+
+    rng = np.random.default_rng(12)
+    n = 5
+    Hs = np.stack([_herm(rng, n) for _ in range(2)])
+    psis = np.stack([rng.normal(size=(n, 1)) + 1j * rng.normal(size=(n, 1)) for _ in range(3)])
+    psis /= np.linalg.norm(psis, axis=1, keepdims=True)
+    E = np.diag(np.arange(n, dtype=float)).astype(complex)
+    ts = np.linspace(0.0, 1.5, 4)
+    res = dq.sesolve(Hs, psis, ts, exp_ops=[E], method=dq.method.Expm())
+    assert res.states.shape == (2, 3, 4, n, 1) and res.expects.shape == (2, 3, 1, 4)
+    st = res.states.to_numpy()[..., 0]
+    for i in range(2):
+        for j in range(3):
+            for k, t in enumerate(ts):
+                ex = sla.expm(-1j * Hs[i] * t) @ psis[j, :, 0]
+                np.testing.assert_allclose(st[i, j, k], ex, atol=2e-5)
+                np.testing.assert_allclose(res.expects[i, j, 0, k], np.vdot(ex, E @ ex), atol=1e-4)
+    assert int(np.asarray(res.infos.nsteps).max()) == 3
+    # mesolve: PWC H, one jump operator, two initial states
+    a = dq.destroy(n)
+    X = (a + a.dag()).to_numpy()
+    H0 = (a.dag() @ a).to_numpy()
+    H = dq.asqarray(H0) + dq.pwc(np.array([0.0, 0.6, 2.0]), np.array([0.5, -0.5]), dq.asqarray(X))
+    L = 0.3 * a.to_numpy()
+    rho = np.stack([dq.coherent_dm(n, al).to_numpy() for al in (0.2, 0.8)])
+    ts = np.array([0.0, 0.6, 1.0])
+    rm = dq.mesolve(H, [L], rho, ts, exp_ops=[dq.asqarray(H0)], method=dq.method.Expm())
+    S1 = lindblad.slindbladian(H0 + 0.5 * X, [L])
+    S2 = lindblad.slindbladian(H0 - 0.5 * X, [L])
+    P06, P10 = sla.expm(0.6 * S1), sla.expm(0.4 * S2) @ sla.expm(0.6 * S1)
+    got = rm.states.to_numpy()
+    for b in range(2):
+        v = lindblad.vectorize(rho[b])
+        np.testing.assert_allclose(got[b, 1], lindblad.unvectorize(P06 @ v), atol=2e-5)
+        ref = lindblad.unvectorize(P10 @ v)
+        np.testing.assert_allclose(got[b, 2], ref, atol=2e-5)
+        np.testing.assert_allclose(rm.expects[b, 0, 2], np.trace(H0 @ ref), atol=1e-4)
+    assert str(rm).split('\n')[1] == 'Method  : Expm'
