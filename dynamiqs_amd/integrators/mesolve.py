@@ -26,7 +26,7 @@ from ..result import AdaptiveStepInfos, FixedStepInfos, MESolveResult
 from ..time_qarray import TimeQArray, astimeqarray
 from . import dense
 from . import dia as dia_path
-from .operators import decompose
+from .operators import decompose, resolve_callables
 
 SUPPORTED_METHODS = (Tsit5, Dopri5, Euler, Rouchon1, Rouchon2, Rouchon3, Expm)
 ADAPTIVE = (Tsit5, Dopri5)
@@ -184,6 +184,10 @@ def mesolve(
 
         Es_np = [E.to_numpy() for E in exp_ops] if exp_ops is not None else None
         return expm_solve(H, Ls, rho0, tsave, Es_np, method, gradient, options, me=True)
+    # timecallable operators become constant + modulated terms over the solve interval
+    t_first = float(tsave[0] if options.t0 is None else options.t0)
+    H = resolve_callables(H, t_first, float(tsave[-1]))
+    Ls = [resolve_callables(L, t_first, float(tsave[-1])) for L in Ls]
 
     route = dia_route(method, H, Ls, H.shape[-1])
     if route is not None:

@@ -137,6 +137,7 @@ def spline_table(f, ta: float, tb: float, *, tol: float = 1e-12, n0: int = 256, 
     ta, tb = float(ta), float(tb)
     if not tb > ta:
         tb = ta + 1.0
+    tol = max(tol, float(getattr(f, 'table_tol', 0.0)))  # f known to limited precision
     n = n0
     while True:
         t = np.linspace(ta, tb, n + 1)
@@ -167,3 +168,101 @@ def resample_pwc(times: np.ndarray, values: np.ndarray, grid: np.ndarray) -> np.
     idx = np.clip(idx, 0, values.shape[-1] - 1)
     out = np.asarray(values, dtype=np.complex128)[..., idx]
     return np.where(inside, out, 0.0)
+
+
+def resolve_callables(x, ta: float, tb: float, max_terms: int = 4, seed: int = 0):
+    """A ``timecallable`` operator H(t) = f(t) (``CallableTimeQArray``,
+    ``dynamiqs/time_qarray.py:763-833``), alone or inside a sum, rewritten as a
+    constant part plus at most ``max_terms`` modulated terms g_r(t) O_r, which the
+    kernels run through their piecewise-cubic tables.
+
+    The operator span of f over [ta, tb] is found from the samples f(t_i) at 65
+    uniform and 16 random times: with s_i = [Re f(t_i), Im f(t_i)] and mean m, the
+    right singular vectors of (s_i - m) above 1e-6 of the largest scale give
+    orthonormal directions B_r (smaller ones are the rounding of f's own arithmetic,
+    below what the complex64 kernels resolve); O_r = B_r as a matrix (Hermitian when
+    f is), the constant part is m and g_r(t) = (s(t) - m) . B_r, tabulated to 10x
+    the largest dropped direction. The span is checked at 32 other random times
+    (residual <= 1e-5 relative). A callable whose samples need more
+    than ``max_terms`` directions, or leave the span, raises NotImplementedError;
+    so does a batched one. Other operators pass through unchanged."""
+    from ..qarrays import QArray, asqarray
+    from ..time_qarray import CallableTimeQArray, ConstantTimeQArray, ModulatedTimeQArray, SummedTimeQArray
+
+    if isinstance(x, SummedTimeQArray):
+        # the callables of a sum are resolved together (their joint span), within
+        # the term budget the sum's modulated terms leave
+        calls = [t for t in x.timeqarrays if isinstance(t, CallableTimeQArray)]
+        if not calls:
+            return x
+        others = [t for t in x.timeqarrays if not isinstance(t, CallableTimeQArray)]
+        budget = max_terms - sum(isinstance(t, ModulatedTimeQArray) for t in others)
+
+        def fsum(t, fs=tuple(c.f for c in calls)):
+            out = asqarray(fs[0](t))
+            for f in fs[1:]:
+                out = out + asqarray(f(t))
+            return out
+
+        r = resolve_callables(CallableTimeQArray(fsum), ta, tb, max(budget, 0), seed)
+        return SummedTimeQArray(others + (list(r.timeqarrays) if isinstance(r, SummedTimeQArray) else [r]))
+    if not isinstance(x, CallableTimeQArray):
+        return x
+    ta, tb = float(ta), float(tb)
+    if not tb > ta:
+        tb = ta + 1.0
+    probe = asqarray(x.f(ta))
+    dims = probe.dims
+    if probe.ndim != 2:
+        raise NotImplementedError('batched CallableTimeQArray operators are not supported by the MI355X solver')
+    n = probe.shape[-1]
+    cache = {}
+    # directions below 1e-6 of the largest scale are rounding of f's own arithmetic
+    # (single-precision qarrays by default): below what the complex64 kernels resolve
+    rank_tol, span_tol = 1e-6, 1e-5
+
+    def s_of(t: float) -> np.ndarray:
+        t = float(t)
+        if t not in cache:
+            m = asqarray(x.f(t)).to_numpy().astype(np.complex128)
+            cache[t] = np.concatenate([m.real.ravel(), m.imag.ravel()])
+        return cache[t]
+
+    rng = np.random.default_rng(seed)
+    ts = np.concatenate([np.linspace(ta, tb, 65), rng.uniform(ta, tb, 16)])
+    S = np.stack([s_of(t) for t in ts])
+    mean = S.mean(axis=0)
+    _, sv, Vh = np.linalg.svd(S - mean, full_matrices=False)
+    scale = max(float(sv[0]) if sv.size else 0.0, float(np.linalg.norm(mean)), 1e-300)
+    r = int(np.sum(sv > rank_tol * scale))
+    noise = float(sv[r]) / scale if r < sv.size else 0.0  # the largest dropped direction
+    if r > max_terms:
+        raise NotImplementedError(
+            f'this timecallable operator spans {r} independent operators over [{ta}, {tb}]; the MI355X solver '
+            f'runs at most {max_terms} time-dependent terms (use dq.modulated / dq.pwc terms instead)'
+        )
+    B = Vh[:r]
+    check = rng.uniform(ta, tb, 32)
+    for t in check:
+        v = s_of(t) - mean
+        res = np.linalg.norm(v - B.T @ (B @ v))
+        if res > span_tol * max(np.linalg.norm(s_of(t)), 1e-300):
+            raise NotImplementedError(
+                'this timecallable operator is not spanned by a few fixed operators over the solve interval; '
+                'the MI355X solver runs it only in that form (use dq.modulated / dq.pwc terms instead)'
+            )
+
+    def as_op(v: np.ndarray) -> np.ndarray:
+        return (v[: n * n] + 1j * v[n * n :]).reshape(n, n)
+
+    terms = [ConstantTimeQArray(QArray(as_op(mean), dims=dims))]
+    for k in range(r):
+        def g(t, b=B[k]):
+            t = np.asarray(t, dtype=np.float64)
+            if t.ndim == 0:
+                return float((s_of(t) - mean) @ b)
+            return np.array([(s_of(u) - mean) @ b for u in t])
+
+        g.table_tol = max(1e-12, 10 * noise)
+        terms.append(ModulatedTimeQArray(g, QArray(as_op(B[k]), dims=dims)))
+    return terms[0] if r == 0 else SummedTimeQArray(terms)

@@ -50,7 +50,7 @@ from ..time_qarray import ConstantTimeQArray, PWCTimeQArray, SummedTimeQArray, a
 from . import dense
 from .mesolve import assert_method_supported, max_steps_error, save_extra_outputs
 from .mesolve import enumerate_instances as me_instances
-from .operators import decompose
+from .operators import decompose, resolve_callables
 from .sesolve import build_plan, build_plan_constant, constant_batch, pad16
 from .sesolve import enumerate_instances as se_instances
 
@@ -259,6 +259,7 @@ def sepropagator(
         return _expm_result(lambda c, t: (_op_at(Ht, c, t),), lambda ops: -1j * ops[0], chains, n, H.dims,
                             tuple(Ht.batch), tsave, t0, H.discontinuity_ts, method, gradient, options,
                             SEPropagatorResult)
+    H = resolve_callables(H, float(tsave[0] if options.t0 is None else options.t0), float(tsave[-1]))
     basis = QArray(np.eye(n, dtype=np.complex128)[:, :, None], dims=H.dims)  # (n, n, 1): the kets |j>
     kw = _method_kwargs(method, options, options.save_propagators or options.save_extra is not None)
     Ht = decompose(H)
@@ -377,6 +378,9 @@ def mepropagator(
 
         return _expm_result(ops_at, lambda ops: slindbladian(ops[0], ops[1:]), chains, n * n, H.dims, out_shape,
                             tsave, t0, disc, method, gradient, options, MEPropagatorResult)
+    t_first = float(tsave[0] if options.t0 is None else options.t0)
+    H = resolve_callables(H, t_first, float(tsave[-1]))
+    Ls = [resolve_callables(L, t_first, float(tsave[-1])) for L in Ls]
     out_shape, insts = me_instances(H, Ls, QArray(np.zeros((n, n), dtype=np.complex128), dims=H.dims),
                                     options.cartesian_batching)
     basis = hermitian_basis(n)

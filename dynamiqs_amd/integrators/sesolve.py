@@ -24,7 +24,7 @@ from ..result import AdaptiveStepInfos, FixedStepInfos, SESolveResult
 from ..time_qarray import TimeQArray, astimeqarray
 from .dense import MAX_NP, _Arena, _device, _pad, check_descriptors, pad16, rouchon_step_times
 from .mesolve import assert_method_supported, max_steps_error, save_extra_outputs
-from .operators import decompose, resample_pwc, spline_table, union_breakpoints
+from .operators import decompose, resample_pwc, resolve_callables, spline_table, union_breakpoints
 
 SUPPORTED_METHODS = (Tsit5, Dopri5, Euler, Expm)
 MAX_PWC = 8
@@ -408,6 +408,8 @@ def sesolve(
         from .propagators import expm_solve
 
         return expm_solve(H, [], psi0, tsave, Es, method, gradient, options, me=False)
+    # timecallable operators become constant + modulated terms over the solve interval
+    H = resolve_callables(H, float(tsave[0] if options.t0 is None else options.t0), float(tsave[-1]))
     kw = dict(
         exp_ops=Es,
         t0=None if options.t0 is None else float(options.t0),

@@ -73,12 +73,16 @@ def test_empty_jump_ops_with_ket_warns():
 
 def test_unsupported_time_dependence_is_reported():
     """What the kernels cannot run raises before any device work: callable
-    operators, modulated operators with discontinuity times, modulated jump
-    operators, modulated Hamiltonians with Rouchon1; a non-callable f is a
-    TypeError (time_qarray.py:161-165)."""
+    operators that span more than four time-dependent directions, modulated
+    operators with discontinuity times, modulated jump operators, modulated
+    Hamiltonians with Rouchon1; a non-callable f is a TypeError
+    (time_qarray.py:161-165)."""
     H, Ls, psi, ts = _args()
-    with pytest.raises(NotImplementedError, match='CallableTimeQArray operators are not supported'):
-        dq.mesolve(dq.timecallable(lambda t: H), Ls, psi, ts)
+    rng = np.random.default_rng(0)
+    ops = [(lambda x: x + x.conj().T)(rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))) for _ in range(5)]
+    five = dq.timecallable(lambda t: dq.asqarray(sum(np.cos((6 * k + 7) * t) * o for k, o in enumerate(ops[:3]))))
+    with pytest.raises(NotImplementedError, match='timecallable operator'):
+        dq.mesolve(five + dq.timecallable(lambda t: dq.asqarray(np.sin(29 * t) * ops[3] + t * ops[4])), Ls, psi, ts)
     with pytest.raises(NotImplementedError, match='discontinuity_ts'):
         dq.mesolve(dq.modulated(np.cos, H, discontinuity_ts=[0.5]), Ls, psi, ts)
     with pytest.raises(NotImplementedError, match='time-dependent jump operators'):
@@ -192,7 +196,7 @@ def test_sesolve_argument_errors():
     """``dq.sesolve`` raises what the reference raises before integrating
     (dynamiqs/integrators/apis/sesolve.py:222-225, 282, 307-318): shapes, save
     times, options of another solver, unsupported method types; callable
-    Hamiltonians raise NotImplementedError before any device work."""
+    batched callable Hamiltonians raise NotImplementedError before any device work."""
     a = dq.destroy(3)
     H, psi, ts = a.dag() @ a, dq.fock(3, 0), np.linspace(0.0, 1.0, 3)
     with pytest.raises(ValueError, match=re.escape('Argument `H` must have shape (...H, n, n), but has shape H.shape=(3, 2)')):
@@ -207,8 +211,8 @@ def test_sesolve_argument_errors():
         dq.sesolve(H, psi, ts, options=dq.Options(vectorized=True))
     with pytest.raises(TypeError, match=re.escape('Method of type `Rouchon1` is not supported (supported method types: `Tsit5`, `Dopri5`, `Euler`, `Expm`)')):
         dq.sesolve(H, psi, ts, method=dq.method.Rouchon1(dt=0.1))
-    with pytest.raises(NotImplementedError, match='CallableTimeQArray operators are not supported'):
-        dq.sesolve(dq.timecallable(lambda t: H), psi, ts)
+    with pytest.raises(NotImplementedError, match='batched CallableTimeQArray'):
+        dq.sesolve(dq.timecallable(lambda t: dq.asqarray(np.stack([H.to_numpy(), t * H.to_numpy()]))), psi, ts)
 
 
 def test_sesolve_batching_shapes():
@@ -342,3 +346,23 @@ def test_expm_interval_bookkeeping_and_liouvillian():
     np.testing.assert_allclose(_op_at(terms, (), 1.0), H - O)
     np.testing.assert_allclose(_op_at(terms, (), 2.0), H)
     assert ispwc(Ht) and not ispwc(dq.modulated(np.cos, dq.asqarray(O)))
+
+
+def test_timecallable_low_rank_rewrite():
+    """A timecallable H(t) is rewritten as a constant part plus at most four
+    modulated terms spanning its samples: the rewrite reproduces f at times off the
+    sampling grid to rounding, a constant callable becomes a constant operator, and
+    the reference's TDQubit callable (closed_system.py:121-123) needs one term."""
+    from dynamiqs_amd.integrators.operators import resolve_callables
+    from dynamiqs_amd.time_qarray import ConstantTimeQArray, ModulatedTimeQArray, SummedTimeQArray
+
+    H = dq.timecallable(lambda t: 3.0 * np.cos(10.0 * t) * dq.sigmax())
+    R = resolve_callables(H, 0.0, 1.0)
+    assert isinstance(R, SummedTimeQArray)
+    assert sum(isinstance(t, ModulatedTimeQArray) for t in R.timeqarrays) == 1
+    a = dq.destroy(4)
+    f = lambda t: dq.asqarray(a.dag() @ a) + np.sin(2 * t) * (a + a.dag()) + t**2 * (a @ a + a.dag() @ a.dag())  # noqa: E731
+    R = resolve_callables(dq.timecallable(f) + dq.asqarray(0.5 * np.eye(4)), -1.0, 2.0)
+    for t in np.random.default_rng(3).uniform(-1.0, 2.0, 20):
+        np.testing.assert_allclose(R(t).to_numpy(), f(t).to_numpy() + 0.5 * np.eye(4), atol=1e-12)
+    assert isinstance(resolve_callables(dq.timecallable(lambda t: dq.sigmaz()), 0.0, 1.0), ConstantTimeQArray)

@@ -118,3 +118,45 @@ def test_mesolve_batched_modulation():
         one = dq.mesolve(a.dag() @ a + dq.modulated(lambda t, w=w: 0.4 * np.cos(w * t), a + a.dag()), [0.2 * a],
                          dq.coherent(n, 0.5), tsave)
         np.testing.assert_allclose(res.states.to_numpy()[b], one.states.to_numpy(), atol=1e-6)
+
+
+def test_timecallable_hamiltonians(kats):
+    """timecallable Hamiltonians, rewritten on the host as a constant part plus
+    modulated terms spanning their samples: the reference's OTDQubit and TDQubit
+    written as dq.timecallable (open_system.py:134-229, closed_system.py:107-176)
+    against their closed forms, and a two-direction callable against the oracle
+    with the exact f at every stage time."""
+    import dynamiqs_amd as dq
+    from oracle import mesolve as om
+
+    k = kats['otdqubit']
+    eps, w, g = k['eps'], k['omega'], k['gamma']
+    tsave = np.linspace(0, k['t_end'], k['ntsave'])
+    H = dq.timecallable(lambda t: eps * np.cos(w * t) * dq.sigmax())
+    res = dq.mesolve(H, [np.sqrt(g) * dq.sigmax()], dq.fock(2, 0), tsave, method=dq.method.Tsit5(rtol=1e-8, atol=1e-8))
+    st = res.states.to_numpy()
+    for i, t in enumerate(tsave):
+        th = 2 * eps / w * np.sin(w * t)
+        eta = np.exp(-2 * g * t)
+        want = np.array([[0.5 * (1 + eta * np.cos(th)), 0.5j * eta * np.sin(th)],
+                         [-0.5j * eta * np.sin(th), 0.5 * (1 - eta * np.cos(th))]])
+        assert np.linalg.norm(st[i] - want) < 2e-6, (i, np.linalg.norm(st[i] - want))
+    ts = np.linspace(0.0, 1.0, 11)
+    rs = dq.sesolve(dq.timecallable(lambda t: 3.0 * np.cos(10.0 * t) * dq.sigmax()), dq.fock(2, 0), ts,
+                    method=dq.method.Tsit5(rtol=1e-8, atol=1e-8))
+    th = 0.6 * np.sin(10.0 * ts)
+    exact = np.stack([np.cos(th / 2), -1j * np.sin(th / 2)], axis=-1)
+    assert np.max(np.abs(rs.states.to_numpy()[..., 0] - exact)) < 2e-6
+    n = 6
+    a = dq.destroy(n)
+    A, X, Y = (a.dag() @ a).to_numpy(), (a + a.dag()).to_numpy(), (a @ a + a.dag() @ a.dag()).to_numpy()
+    f = lambda t: dq.asqarray(A + np.sin(2 * t) * X + t**2 * Y)  # noqa: E731
+    L = 0.3 * a.to_numpy()
+    rho0 = dq.coherent_dm(n, 0.5).to_numpy()
+    ts = np.linspace(0.0, 1.0, 6)
+    got = dq.mesolve(dq.timecallable(f), [L], rho0, ts, method=dq.method.Tsit5(rtol=1e-8, atol=1e-8)).states.to_numpy()
+    Hop = om.TimeOp(A, [om.ModTerm(lambda t: np.sin(2 * t), X), om.ModTerm(lambda t: t**2, Y)])
+    ref = om.mesolve(Hop, [L], rho0, ts, rtol=1e-10, atol=1e-10)['states']
+    err = np.linalg.norm(got - ref, axis=(-2, -1)).max()
+    print('two-direction timecallable vs oracle', err)
+    assert err < 1e-5
