@@ -2,7 +2,7 @@
 // Dopri5, for gfx950.
 //
 // One workgroup integrates one instance (its own adaptive step): one wave for
-// np <= 64, four waves up to np = 256. The FP64 state, the FP32 stage input and the
+// np <= 64, four waves up to np = 1024. The FP64 state, the FP32 stage input and the
 // seven FP32 stages live in LDS. The vector field is a GEMV with G = -i H read as
 // G^T (stored transposed, so that consecutive rows are consecutive addresses) from
 // global memory, where a batch that shares H finds it in L2. In the one-wave
@@ -25,11 +25,10 @@
 namespace dqh {
 namespace se {
 
-// workgroup size T: the largest padded dimension it serves
+// workgroup size T (one wave or four)
 template <int T>
 struct SeCfg {
   static constexpr int kW = T / 64;
-  static constexpr int kMaxNp = T == 64 ? 64 : 256;
 };
 
 template <int T>
@@ -156,8 +155,8 @@ __device__ __forceinline__ void field(const dqh_me_inst &p, const Coef &cf, cons
 }
 
 // save y + h sum_j w_j k_j (state and <psi|E|psi>) at save index isave
-template <int T>
-__device__ void save_at(const dqh_me_inst &p, const double2 *y, float2 (*k)[SeCfg<T>::kMaxNp], double h, const double *w,
+template <int T, int MAXNP>
+__device__ void save_at(const dqh_me_inst &p, const double2 *y, float2 (*k)[MAXNP], double h, const double *w,
                         bool interp, int isave, double2 *v, double *red) {
   const int n = (int)p.n, np = (int)p.np, nE = (int)p.nE, ntsave = (int)p.ntsave;
   for (int i = threadIdx.x; i < np; i += T) {
@@ -192,9 +191,11 @@ __device__ void save_at(const dqh_me_inst &p, const double2 *y, float2 (*k)[SeCf
   __syncthreads();
 }
 
-template <int T>
+// MAXNP: the largest padded dimension the LDS arrays hold (64, 256 or 1024; the
+// 1024 variant takes 112 KB of LDS, one workgroup per CU)
+template <int T, int MAXNP>
 __global__ __launch_bounds__(T) void k_sesolve(const dqh_me_inst *__restrict__ insts) {
-  constexpr int kMaxNp = SeCfg<T>::kMaxNp;
+  constexpr int kMaxNp = MAXNP;
   __shared__ double red[SeCfg<T>::kW];
   __shared__ double2 y[kMaxNp], y1[kMaxNp], vsave[kMaxNp];
   __shared__ float2 ys[kMaxNp];
@@ -223,7 +224,7 @@ __global__ __launch_bounds__(T) void k_sesolve(const dqh_me_inst *__restrict__ i
   int isave = 0, status = 0;
   int64_t nsteps = 0, nacc = 0, nrej = 0, nrhs = 0;
   while (isave < ntsave && tsave[isave] <= t) {
-    save_at<T>(p, y, k, 0.0, w0, false, isave, vsave, red);
+    save_at<T, MAXNP>(p, y, k, 0.0, w0, false, isave, vsave, red);
     ++isave;
   }
   if (p.flags & DQH_FLAG_EULER) {
@@ -241,7 +242,7 @@ __global__ __launch_bounds__(T) void k_sesolve(const dqh_me_inst *__restrict__ i
       ++nrhs;
       while (isave < ntsave && tsave[isave] <= tb + 1e-12 * fmax(1.0, fabs(tb))) {
         wl[0] = fmin(fmax((tsave[isave] - ta) / h, 0.0), 1.0);
-        save_at<T>(p, y, k, h, wl, true, isave, vsave, red);
+        save_at<T, MAXNP>(p, y, k, h, wl, true, isave, vsave, red);
         ++isave;
       }
       for (int i = threadIdx.x; i < np; i += T)
@@ -366,7 +367,7 @@ __global__ __launch_bounds__(T) void k_sesolve(const dqh_me_inst *__restrict__ i
           const double th = (tsave[isave] - t) / h;
           double w[7];
           for (int j = 0; j < 7; ++j) w[j] = th * (rk.R[j][0] + th * (rk.R[j][1] + th * (rk.R[j][2] + th * rk.R[j][3])));
-          save_at<T>(p, y, k, h, w, true, isave, vsave, red);
+          save_at<T, MAXNP>(p, y, k, h, w, true, isave, vsave, red);
           ++isave;
         }
         for (int i = threadIdx.x; i < np; i += T) {
@@ -383,7 +384,7 @@ __global__ __launch_bounds__(T) void k_sesolve(const dqh_me_inst *__restrict__ i
     }
   }
   while (isave < ntsave) {
-    save_at<T>(p, y, k, 0.0, w0, false, isave, vsave, red);
+    save_at<T, MAXNP>(p, y, k, 0.0, w0, false, isave, vsave, red);
     ++isave;
   }
   for (int i = threadIdx.x; i < n; i += T) {
@@ -404,12 +405,16 @@ __global__ __launch_bounds__(T) void k_sesolve(const dqh_me_inst *__restrict__ i
 }  // namespace dqh
 
 extern "C" int dqh_sesolve_tsit5(const dqh_me_inst *d_insts, int64_t ninst, int64_t np_max, void *stream) {
-  if (!d_insts || ninst <= 0 || ninst > (1 << 30) || np_max <= 0 || np_max > 256)
+  if (!d_insts || ninst <= 0 || ninst > (1 << 30) || np_max <= 0 || np_max > 1024)
     return dqh::fail(DQH_BAD_ARG, "dqh_sesolve_tsit5: bad arguments");
   if (np_max <= 64)
-    hipLaunchKernelGGL(dqh::se::k_sesolve<64>, dim3((unsigned)ninst), dim3(64), 0, (hipStream_t)stream, d_insts);
+    hipLaunchKernelGGL((dqh::se::k_sesolve<64, 64>), dim3((unsigned)ninst), dim3(64), 0, (hipStream_t)stream, d_insts);
+  else if (np_max <= 256)
+    hipLaunchKernelGGL((dqh::se::k_sesolve<256, 256>), dim3((unsigned)ninst), dim3(256), 0, (hipStream_t)stream,
+                       d_insts);
   else
-    hipLaunchKernelGGL(dqh::se::k_sesolve<256>, dim3((unsigned)ninst), dim3(256), 0, (hipStream_t)stream, d_insts);
+    hipLaunchKernelGGL((dqh::se::k_sesolve<256, 1024>), dim3((unsigned)ninst), dim3(256), 0, (hipStream_t)stream,
+                       d_insts);
   const hipError_t err = hipGetLastError();
   if (err != hipSuccess) return dqh::fail(DQH_HIP_ERROR, std::string("dqh_sesolve_tsit5: ") + hipGetErrorString(err));
   return DQH_OK;

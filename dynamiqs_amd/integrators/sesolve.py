@@ -22,12 +22,13 @@ from ..options import Options, check_options
 from ..qarrays import QArray, asqarray
 from ..result import AdaptiveStepInfos, FixedStepInfos, SESolveResult
 from ..time_qarray import TimeQArray, astimeqarray
-from .dense import MAX_NP, _Arena, _device, _pad, check_descriptors, pad16, rouchon_step_times
+from .dense import _Arena, _device, _pad, check_descriptors, pad16, rouchon_step_times
 from .mesolve import assert_method_supported, max_steps_error, save_extra_outputs
 from .operators import decompose, resample_pwc, resolve_callables, spline_table, union_breakpoints
 
 SUPPORTED_METHODS = (Tsit5, Dopri5, Euler, Expm)
 MAX_PWC = 8
+MAX_NP = 1024  # padded dimension of the largest sesolve kernel variant
 SE_FIELDS = ('tsave', 'Hnh', 'EopsT', 'Hpwc', 'pwc_times', 'pwc_vals', 'Hmod', 'mod_coef', 'step_t')
 
 
@@ -385,7 +386,7 @@ def sesolve(
     (``dynamiqs/integrators/apis/sesolve.py:36-230``). Supported methods:
     ``Tsit5``, ``Dopri5``, ``Euler`` (fixed step) and ``Expm`` (constant and PWC
     Hamiltonians); supported Hamiltonians: constant (dense or DIA,
-    n <= 256), piecewise-constant (``dq.pwc``) and modulated (``dq.modulated``)
+    n <= 1024), piecewise-constant (``dq.pwc``) and modulated (``dq.modulated``)
     terms and their sums.
     """
     # === convert arguments

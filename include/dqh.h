@@ -266,8 +266,8 @@ int dqh_mesolve_tsit5_dia(const dqh_dia_tsit5 *problem, void *stream);
 /* Adaptive Tsit5 (or Dopri5 with DQH_FLAG_DOPRI5) solve of the Schrodinger
  * equation d psi/dt = -i H(t) psi (replaces SESolveDiffraxIntegrator.run,
  * diffrax_integrator.py:225-228, 257-280, with the controller of :101-147), one
- * workgroup per instance; np_max = the largest np of the batch (<= 256) picks the
- * one-wave kernel (np_max <= 64) or the four-wave one. Same descriptor as the mesolve kernels, read
+ * workgroup per instance; np_max = the largest np of the batch (<= 1024) picks the
+ * one-wave kernel (np_max <= 64) or a four-wave one (LDS sized for 256 or 1024). Same descriptor as the mesolve kernels, read
  * for a ket: Hnh = (-i H0)^T, Hpwc / Hmod = the stacked (-i O_k)^T of the
  * piecewise-constant / modulated terms (complex64 [np][np] each), EopsT = E^T,
  * y = double2 [np] holding psi0 (n used), nL = 0; outputs states = complex64

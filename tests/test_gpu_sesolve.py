@@ -241,3 +241,26 @@ def test_euler_fixed_step_against_oracle():
     assert err <= 1e-5
     np.testing.assert_allclose(res.expects[0], ref['expects'][0], atol=1e-5)
     assert int(res.infos.nsteps) == 1200 and 'steps' in str(res.infos)
+
+
+def test_large_dimension_variant_replay():
+    """n = 600 (padded to 608) runs on the four-wave kernel whose LDS holds up to
+    np = 1024 (112 KB): replay of its own steps within 1e-5 and the norm conserved;
+    DIA operators of that size are accepted through their dense form."""
+    import dynamiqs_amd as dq
+    from oracle import mesolve as om
+
+    from dynamiqs_amd.integrators.sesolve import SEInstance
+
+    rng = np.random.default_rng(17)
+    n = 600
+    a = np.diag(np.sqrt(np.arange(1, n)), 1).astype(complex)
+    H0 = a.conj().T @ a * 0.01 + 0.3 * (a + a.conj().T) / np.sqrt(n)
+    psi = _ket(rng, n)
+    tsave = np.linspace(0.0, 1.0, 4)
+    res, errs = _run_plan([SEInstance(H=H0, psi0=psi, key_H=())], tsave, [om.TimeOp(H0)], [psi])
+    print('n=600 replay error', errs, 'steps', res['naccepted'])
+    assert np.all(errs <= 1e-5)
+    assert np.max(np.abs(np.linalg.norm(res['states'][0], axis=-1) - 1.0)) <= 1e-5
+    r = dq.sesolve(dq.asqarray(H0, layout=dq.dia), dq.asqarray(psi[:, None]), tsave)
+    np.testing.assert_allclose(r.states.to_numpy()[-1, :, 0], res['states'][0, -1], atol=1e-5)
