@@ -65,6 +65,7 @@ FLAG_HAS_HNH = 2
 FLAG_GLOBAL = 4
 FLAG_DOPRI5 = 8
 FLAG_EULER = 16
+FLAG_LMOD = 32
 
 STATUS_OK = 0
 STATUS_MAX_STEPS = -3

@@ -37,6 +37,13 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
   return s;
 }
 
+// time-dependent jump operators (DQH_FLAG_LMOD): the per-stage copies of D and C
+__device__ __forceinline__ const float2 *lmod_D(const dqh_me_inst &p) {
+  return gp<const float2>(p.work) + (size_t)(10 + p.nL) * (size_t)(p.np * p.np);
+}
+__device__ __forceinline__ const float2 *lmod_C(const dqh_me_inst &p) {
+  return lmod_D(p) + (size_t)p.nL * (size_t)(p.np * p.np);
+}
 // ----------------------------------------------------------- instance view
 struct View {
   int n, np, nL, nE, ntsave, npwc, nseg, flags;
@@ -69,8 +76,9 @@ __device__ __forceinline__ void make_view(View &v, const dqh_me_inst &p) {
   const bool tdep = v.npwc > 0 || p.nmod > 0;
   v.Hcur = tdep ? w + (9 + v.nL) * v.nn : nullptr;
   v.Hnh = tdep ? v.Hcur : gp<const float2>(p.Hnh);
-  v.D = gp<const float2>(p.Dops);
-  v.C = gp<const float2>(p.Cops);
+  const bool lmod = (p.flags & DQH_FLAG_LMOD) != 0;
+  v.D = lmod ? lmod_D(p) : gp<const float2>(p.Dops);
+  v.C = lmod ? lmod_C(p) : gp<const float2>(p.Cops);
   double2 *y = gp<double2>(p.y);
   v.y[0] = y;
   v.y[1] = y + v.nn;
@@ -152,7 +160,42 @@ __device__ __forceinline__ void mod_coeffs(const dqh_me_inst &p, double tau, dou
   }
 }
 
+// f_j(tau) of table j of mod_coef (Horner on the interval that contains tau)
+__device__ __forceinline__ double2 mod_eval(const dqh_me_inst &p, int j, double tau) {
+  const int64_t nint = p.mod_nint;
+  int64_t i = (int64_t)floor((tau - p.mod_t0) / p.mod_dt);
+  i = i < 0 ? 0 : (i >= nint ? nint - 1 : i);
+  const double x = tau - (p.mod_t0 + (double)i * p.mod_dt);
+  const double2 *q = gp<const double2>(p.mod_coef) + ((size_t)j * nint + i) * 4;
+  double re = q[0].x, im = q[0].y;
+#pragma unroll
+  for (int m = 1; m < 4; ++m) {
+    re = re * x + q[m].x;
+    im = im * x + q[m].y;
+  }
+  return make_double2(re, im);
+}
+
+// time-dependent jump operators (DQH_FLAG_LMOD): D_k <- f_k(tau) L_k and
+// C_k <- conj(f_k(tau)) L_k^dag / 2 in work at (10 + nL) np^2, f_k = table nmod + k
+__device__ __forceinline__ void lmod_update(const dqh_me_inst &p, double tau, int nthreads) {
+  const int nL = (int)p.nL, nmod = (int)p.nmod;
+  const size_t nn = (size_t)(p.np * p.np);
+  float2 *Dc = gp<float2>(p.work) + (size_t)(10 + nL) * nn;
+  float2 *Cc = Dc + (size_t)nL * nn;
+  const float2 *D0 = gp<const float2>(p.Dops), *C0 = gp<const float2>(p.Cops);
+  for (int k = 0; k < nL; ++k) {
+    const double2 f = mod_eval(p, nmod + k, tau);
+    for (size_t e = threadIdx.x; e < nn; e += nthreads) {
+      const float2 d = D0[k * nn + e], c = C0[k * nn + e];
+      Dc[k * nn + e] = make_float2((float)(f.x * d.x - f.y * d.y), (float)(f.x * d.y + f.y * d.x));
+      Cc[k * nn + e] = make_float2((float)(f.x * c.x + f.y * c.y), (float)(f.x * c.y - f.y * c.x));
+    }
+  }
+}
+
 // Hcur <- Hnh0 + sum_j c_j Hpwc_j for PWC segment seg (zero outside) + sum_j f_j(tau) Hmod_j
+// (and the time-dependent jump operators at tau)
 template <int NW>
 __device__ __forceinline__ void wg_update_pwc(const View &v, const dqh_me_inst &p, int seg, double tau) {
   const float2 *H0 = gp<const float2>(p.Hnh);
@@ -183,6 +226,7 @@ __device__ __forceinline__ void wg_update_pwc(const View &v, const dqh_me_inst &
     }
     v.Hcur[e] = make_float2((float)re, (float)im);
   }
+  if (p.flags & DQH_FLAG_LMOD) lmod_update(p, tau, NW * 64);
   __syncthreads();
 }
 
@@ -566,7 +610,7 @@ __global__ __launch_bounds__(kThreads) void k_tsit5(const dqh_me_inst *__restric
   __shared__ float2 lds[kResLdsFloat2];
   const dqh_me_inst &p = insts[blockIdx.x];
   const int np = (int)p.np;
-  if (p.nL <= kResMaxL && !(p.flags & DQH_FLAG_GLOBAL)) {
+  if (p.nL <= kResMaxL && !(p.flags & (DQH_FLAG_GLOBAL | DQH_FLAG_LMOD))) {
     if (np == 16) return tsit5_resident<16, kWaves>(p, sh, lds);
     if (np == 32) return tsit5_resident<32, kWaves>(p, sh, lds);
     if (np == 48) return tsit5_resident<48, kWaves>(p, sh, lds);
@@ -592,7 +636,7 @@ __global__ __launch_bounds__(res_waves<NP>() * 64) void k_tsit5_res(const dqh_me
   __shared__ double sh[NW];
   __shared__ float2 lds[kResLdsFloat2];
   const dqh_me_inst &p = insts[blockIdx.x];
-  if (p.np != NP || p.nL > kResMaxL || (p.flags & DQH_FLAG_GLOBAL)) {
+  if (p.np != NP || p.nL > kResMaxL || (p.flags & (DQH_FLAG_GLOBAL | DQH_FLAG_LMOD))) {
     if (threadIdx.x == 0) gp<int64_t>(p.stats)[3] = DQH_UNSUPPORTED;
     return;
   }
@@ -603,7 +647,7 @@ __global__ __launch_bounds__(kThreads) void k_rhs(const dqh_me_inst *__restrict_
   __shared__ float2 lds[kResLdsFloat2];
   const dqh_me_inst &p = insts[blockIdx.x];
   const int np = (int)p.np;
-  if (p.nL <= kResMaxL && !(p.flags & DQH_FLAG_GLOBAL)) {
+  if (p.nL <= kResMaxL && !(p.flags & (DQH_FLAG_GLOBAL | DQH_FLAG_LMOD))) {
     if (np == 16) return rhs_resident<16, kWaves>(p, lds);
     if (np == 32) return rhs_resident<32, kWaves>(p, lds);
     if (np == 48) return rhs_resident<48, kWaves>(p, lds);

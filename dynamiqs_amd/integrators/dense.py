@@ -53,6 +53,7 @@ class Instance:
     tsave: np.ndarray | None = None  # per-instance save times (all instances of a plan: same length)
     t0: float | None = None  # per-instance start time
     mod: list = field(default_factory=list)  # [(g, op (n, n), key)]: + g(t) op, g scalar; equal keys share tables
+    Lmod: list = field(default_factory=list)  # per jump operator: None (constant) or (g, key): L_k(t) = g(t) Ls[k]
 
 
 class _Arena:
@@ -342,6 +343,33 @@ def rouchon_step_times(t0: float, t1: float, dt: float) -> np.ndarray:
     return ts
 
 
+def _abs2(g):
+    """|g(t)|^2 as a function (the rate of a modulated jump operator)."""
+
+    def h(t, g=g):
+        return np.abs(np.asarray(g(t))) ** 2
+
+    h.table_tol = getattr(g, 'table_tol', 0.0)
+    return h
+
+
+def _one(t):
+    return np.ones(np.shape(t)) + 0j
+
+
+def _hmods(inst: Instance) -> list:
+    """The modulated Hamiltonian-side terms of an instance: its own, plus for every
+    modulated jump operator g_k(t) L_k the term -1/2 |g_k(t)|^2 L_k^dag L_k of
+    -1/2 sum L^dag L, written as |g_k|^2 times O = -i/2 L_k^dag L_k (the kernels add
+    -i O times the coefficient)."""
+    out = list(inst.mod)
+    for L, lm in zip(inst.Ls, inst.Lmod or []):
+        if lm is not None:
+            g, key = lm
+            out.append((_abs2(g), -0.5j * (_dag(L) @ L), None if key is None else ('abs2', key)))
+    return out
+
+
 def build_plan(
     kind: str,
     instances: list,
@@ -387,11 +415,15 @@ def build_plan(
 
     if tableau not in ('tsit5', 'dopri5'):
         raise ValueError(f'unknown Runge-Kutta pair {tableau!r}')
-    if any(inst.mod for inst in instances):
+    lmod_any = any(lm is not None for inst in instances for lm in (inst.Lmod or []))
+    if any(inst.mod for inst in instances) or lmod_any:
         if kind == 'rouchon1':
             raise NotImplementedError('modulated operators are not supported with Rouchon1 on the MI355X solver')
-        if max(len(inst.mod) for inst in instances) > _native.MAX_MOD:
-            raise NotImplementedError(f'at most {_native.MAX_MOD} modulated terms per operator are supported')
+        if max(len(_hmods(inst)) for inst in instances) > _native.MAX_MOD:
+            raise NotImplementedError(
+                f'at most {_native.MAX_MOD} modulated terms per operator are supported (a modulated jump operator'
+                ' counts one for its rate in the effective Hamiltonian)'
+            )
 
     dev = _device()
     _native.lib()
@@ -462,7 +494,7 @@ def build_plan(
         return cache[key]
 
     npwc_max = 0
-    nmod_max = max((len(inst.mod) for inst in instances), default=0)
+    nmod_max = max((len(_hmods(inst)) for inst in instances), default=0)
     t_stop = float(tsave[-1])
     if instances[0].tsave is not None:
         t_stop = max(float(inst.tsave[-1]) for inst in instances)
@@ -512,16 +544,21 @@ def build_plan(
             d['flags'] |= _native.FLAG_HAS_HNH
             if tableau == 'dopri5':
                 d['flags'] |= _native.FLAG_DOPRI5
-            if use_global:
+            lmod = [lm is not None for lm in (inst.Lmod or [])]
+            if use_global or any(lmod):
                 d['flags'] |= _native.FLAG_GLOBAL
+            if any(lmod):
+                d['flags'] |= _native.FLAG_LMOD
 
-            def bH(inst=inst):
+            def bH(inst=inst, lmod=lmod):
                 if inst.Hnh is not None:
                     return _pad(np.asarray(inst.Hnh), npad).astype(np.complex64)
-                LdL = sum((_dag(L) @ L for L in inst.Ls), np.zeros_like(inst.H))
+                # the modulated jump operators' L^dag L terms are time dependent (in Hmod)
+                LdL = sum((_dag(L) @ L for k, L in enumerate(inst.Ls) if not (k < len(lmod) and lmod[k])),
+                          np.zeros_like(inst.H))
                 return _pad(-1j * inst.H - 0.5 * LdL, npad).astype(np.complex64)
 
-            d['Hnh'] = shared(('Hnh', inst.key_H, inst.key_L), bH)
+            d['Hnh'] = shared(('Hnh', inst.key_H, inst.key_L, tuple(lmod)), bH)
             if K:
                 d['Dops'] = shared(('D', inst.key_L), lambda Ls=Ls: _pad(np.stack(Ls), npad).astype(np.complex64))
                 d['Cops'] = shared(
@@ -539,16 +576,23 @@ def build_plan(
                 )
                 d['pwc_times'] = shared(('ptimes', inst.key_H), lambda grid=grid: grid)
                 d['pwc_vals'] = arena.put(np.ascontiguousarray(vals).view(np.float64))
-            if inst.mod:
-                tabs = [mod_table(g, key) for (g, _, key) in inst.mod]
+            hm = _hmods(inst)
+            if hm:
+                # tables: the nmod Hamiltonian-side terms, then (DQH_FLAG_LMOD) one per jump
+                # operator, g_k or a constant 1, all on one grid
+                fns = [(g, key) for (g, _, key) in hm]
+                if any(lmod):
+                    fns += [inst.Lmod[k] if lmod[k] else (_one, ('one',)) for k in range(K)]
+                tabs = [mod_table(g, key) for (g, key) in fns]
                 nint = max(len(c) for (_, _, c) in tabs)
                 tabs = [spline_table(g, t_tab, t_stop, n0=nint, nmax=nint) if len(c) != nint else (a, dtt, c)
-                        for (g, _, _), (a, dtt, c) in zip(inst.mod, tabs)]
-                d['nmod'] = len(inst.mod)
+                        for (g, _), (a, dtt, c) in zip(fns, tabs)]
+                d['nmod'] = len(hm)
                 d['mod_t0'], d['mod_dt'], d['mod_nint'] = tabs[0][0], tabs[0][1], nint
                 d['mod_coef'] = arena.put(np.ascontiguousarray(np.stack([c for (_, _, c) in tabs])).view(np.float64))
                 d['Hmod'] = shared(
-                    ('Hmod', inst.key_H), lambda inst=inst: _pad(np.stack([-1j * o for (_, o, _) in inst.mod]), npad).astype(np.complex64)
+                    ('Hmod', inst.key_H, inst.key_L if any(lmod) else None),
+                    lambda hm=hm: _pad(np.stack([-1j * o for (_, o, _) in hm]), npad).astype(np.complex64),
                 )
         else:  # rouchon1: Kraus sets per (PWC segment, step length)
             d['nL'] = nLdev
@@ -590,6 +634,8 @@ def build_plan(
 
     # ---- state, scratch, outputs
     welems = _native.work_elems(npad, nLdev, npwc_max + nmod_max)
+    if lmod_any:  # per-stage copies of D and C (DQH_FLAG_LMOD)
+        welems += 2 * nLdev * nn
     work = arena.empty(B * welems * 8)
     y = torch.zeros((B, 2, nn, 2), dtype=torch.float64, device=dev)
     y0 = np.zeros((B, npad, npad), dtype=np.complex128)

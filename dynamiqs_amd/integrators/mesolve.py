@@ -105,8 +105,13 @@ def enumerate_instances(H: TimeQArray, Ls, rho0: QArray, cartesian: bool):
     Ht = decompose(H)
     Lts = [decompose(L) for L in Ls]
     for i, Lt in enumerate(Lts):
-        if Lt.pwc or Lt.mod:
-            raise NotImplementedError('time-dependent jump operators are not supported by the MI355X solver')
+        # a time-dependent jump operator must be one modulated term f(t) L (its rate
+        # |f|^2 and phase are applied at every stage time)
+        if Lt.pwc or len(Lt.mod) > 1 or (Lt.mod and Lt.const is not None and np.any(Lt.const)):
+            raise NotImplementedError(
+                'time-dependent jump operators are supported by the MI355X solver only as one modulated term'
+                ' dq.modulated(f, L)'
+            )
     r = rho0.to_numpy().astype(np.complex128)
     rb = r.shape[:-2]
     shapes = [Ht.batch, *[Lt.batch for Lt in Lts], rb]
@@ -127,10 +132,18 @@ def enumerate_instances(H: TimeQArray, Ls, rho0: QArray, cartesian: bool):
         rho = r[ir]
         if rho.shape[-1] == 1:
             rho = rho @ rho.conj().T
+        Lmods = []
+        for Lt, i in zip(Lts, iLs):
+            if Lt.mod:
+                g, _, f = Lt.mod_at(i)[0]
+                Lmods.append((g, (id(f), i if np.ndim(f(0.0)) else None)))
+            else:
+                Lmods.append(None)
         insts.append(
             dense.Instance(
                 H=Ht.const_at(iH),
-                Ls=[Lt.const_at(i) for Lt, i in zip(Lts, iLs)],
+                Ls=[Lt.mod[0][1] if Lt.mod else Lt.const_at(i) for Lt, i in zip(Lts, iLs)],
+                Lmod=Lmods if any(m is not None for m in Lmods) else [],
                 rho0=rho,
                 pwc=Ht.pwc_at(iH),
                 mod=[(g, o, (id(f), iH if np.ndim(f(0.0)) else None)) for (g, o, f) in Ht.mod_at(iH)],

@@ -94,6 +94,13 @@ typedef struct dqh_me_inst {
 #define DQH_FLAG_GLOBAL 4 /* run the global-memory kernel form even where the LDS-resident one applies */
 #define DQH_FLAG_DOPRI5 8 /* Tsit5 kernels: integrate with the Dopri5 pair (same controller and saving) */
 #define DQH_FLAG_EULER 16 /* dqh_sesolve_tsit5: fixed Euler steps between step_t[0..max_steps] */
+/* Tsit5 kernels: jump operator k is f_k(t) L_k with f_k the table nmod + k of mod_coef
+ * (a constant 1 for the constant ones); Dops / Cops hold L_k and L_k^dag / 2, and the
+ * kernels rebuild f_k L_k and conj(f_k) L_k^dag / 2 at every stage time in work at
+ * (10 + nL) np^2 (work needs (10 + 3 nL) np^2 elements). The time-dependent part of
+ * -1/2 sum L^dag L is among the nmod Hamiltonian terms. Such instances always run on
+ * the global-memory kernel form (the uniform entry point reports DQH_UNSUPPORTED). */
+#define DQH_FLAG_LMOD 32
 
 /* Size checks for FFI mirrors. */
 int64_t dqh_sizeof_me_inst(void);

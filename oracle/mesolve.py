@@ -135,7 +135,7 @@ def mesolve(
 
         def f(t, y, tseg):
             ts = _segment_time(t, tseg, jumps)
-            return rhs(H(ts, t), [L(ts) for L in Ls], y)
+            return rhs(H(ts, t), [L(ts, t) for L in Ls], y)
 
         saved, ylast, st = tsit5.integrate(
             f,

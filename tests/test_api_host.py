@@ -74,8 +74,8 @@ def test_empty_jump_ops_with_ket_warns():
 def test_unsupported_time_dependence_is_reported():
     """What the kernels cannot run raises before any device work: callable
     operators that span more than four time-dependent directions, modulated
-    operators with discontinuity times, modulated jump operators, modulated
-    Hamiltonians with Rouchon1; a non-callable f is a TypeError
+    operators with discontinuity times, jump operators other than one modulated
+    term, modulated operators with Rouchon1; a non-callable f is a TypeError
     (time_qarray.py:161-165)."""
     H, Ls, psi, ts = _args()
     rng = np.random.default_rng(0)
@@ -86,7 +86,11 @@ def test_unsupported_time_dependence_is_reported():
     with pytest.raises(NotImplementedError, match='discontinuity_ts'):
         dq.mesolve(dq.modulated(np.cos, H, discontinuity_ts=[0.5]), Ls, psi, ts)
     with pytest.raises(NotImplementedError, match='time-dependent jump operators'):
-        dq.mesolve(H, [dq.modulated(np.cos, Ls[0])], psi, ts)
+        dq.mesolve(H, [Ls[0] + dq.modulated(np.cos, Ls[0])], psi, ts)
+    with pytest.raises(NotImplementedError, match='time-dependent jump operators'):
+        dq.mesolve(H, [dq.pwc(np.array([0.0, 1.0]), np.array([1.0]), Ls[0])], psi, ts)
+    with pytest.raises(NotImplementedError, match='Rouchon1'):
+        dq.mesolve(H, [dq.modulated(np.cos, Ls[0])], psi, ts, method=dq.method.Rouchon1(dt=1e-2))
     with pytest.raises(NotImplementedError, match='Rouchon1'):
         dq.mesolve(H + dq.modulated(np.cos, H), Ls, psi, ts, method=dq.method.Rouchon1(dt=1e-2))
     with pytest.raises(TypeError, match=re.escape('Argument `f` must be a function, but has type float.')):
