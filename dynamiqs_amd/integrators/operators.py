@@ -255,14 +255,21 @@ def resolve_callables(x, ta: float, tb: float, max_terms: int = 4, seed: int = 0
     def as_op(v: np.ndarray) -> np.ndarray:
         return (v[: n * n] + 1j * v[n * n :]).reshape(n, n)
 
-    terms = [ConstantTimeQArray(QArray(as_op(mean), dims=dims))]
+    # a mean inside the span is absorbed by the terms (f(t) L becomes one modulated
+    # term without a constant part, the form a time-dependent jump operator takes)
+    center = mean
+    if r > 0 and np.linalg.norm(mean - B.T @ (B @ mean)) <= span_tol * scale:
+        center = np.zeros_like(mean)
+    terms = [] if not center.any() else [ConstantTimeQArray(QArray(as_op(center), dims=dims))]
     for k in range(r):
         def g(t, b=B[k]):
             t = np.asarray(t, dtype=np.float64)
             if t.ndim == 0:
-                return float((s_of(t) - mean) @ b)
-            return np.array([(s_of(u) - mean) @ b for u in t])
+                return float((s_of(t) - center) @ b)
+            return np.array([(s_of(u) - center) @ b for u in t])
 
         g.table_tol = max(1e-12, 10 * noise)
         terms.append(ModulatedTimeQArray(g, QArray(as_op(B[k]), dims=dims)))
-    return terms[0] if r == 0 else SummedTimeQArray(terms)
+    if r == 0:
+        return ConstantTimeQArray(QArray(as_op(mean), dims=dims))
+    return terms[0] if len(terms) == 1 else SummedTimeQArray(terms)

@@ -211,3 +211,26 @@ def test_modulated_jump_operators_match_oracle(n):
     err = np.linalg.norm(r['states'][0] - ref, axis=(-2, -1)).max()
     print('replay err', err)
     assert err < 2e-6
+
+
+def test_timecallable_jump_operator():
+    """A timecallable jump operator sqrt(kappa(t)) a becomes one modulated term
+    f(t) a (its mean absorbed) and runs as a modulated jump operator; against the
+    oracle with the exact operator at every stage time."""
+    import dynamiqs_amd as dq
+    from oracle import mesolve as om
+
+    n = 8
+    a = dq.destroy(n)
+    an = a.to_numpy().astype(complex)
+    kappa = lambda t: 0.4 * np.exp(-0.7 * t) + 0.05  # noqa: E731
+    L = dq.timecallable(lambda t: np.sqrt(kappa(t)) * a)
+    H = a.dag() @ a
+    rho0 = dq.coherent_dm(n, 1.0).to_numpy()
+    ts = np.linspace(0.0, 3.0, 7)
+    got = dq.mesolve(H, [L], rho0, ts, method=dq.method.Tsit5(rtol=1e-8, atol=1e-8)).states.to_numpy()
+    Lop = om.TimeOp(np.zeros((n, n), complex), [om.ModTerm(lambda t: np.sqrt(kappa(t)), an)])
+    ref = om.mesolve(H.to_numpy(), [Lop], rho0, ts, rtol=1e-10, atol=1e-10)['states']
+    err = np.linalg.norm(got - ref, axis=(-2, -1)).max()
+    print('timecallable jump operator err vs oracle', err)
+    assert err < 1e-5
