@@ -146,15 +146,17 @@ def check_descriptors(descs: np.ndarray, tensors, extents: dict | None = None) -
         'mod_coef': descs['nmod'].astype(np.uint64) * descs['mod_nint'].astype(np.uint64) * np.uint64(64),
     }
     extent.update(extents or {})
-    for f in POINTER_FIELDS:
-        v = descs[f].astype(np.uint64)
-        nz = v != 0
-        idx = np.clip(np.searchsorted(lo, v, side='right') - 1, 0, len(hi) - 1)
-        end = v + extent.get(f, np.ones(len(descs), dtype=np.uint64))
-        ok = (v >= lo[idx]) & (end <= hi[idx])
-        if np.any(nz & ~ok):
-            bad = int(v[nz & ~ok][0])
-            raise RuntimeError(f'descriptor field {f} = {bad:#x} is outside the plan allocations')
+    # every (field, descriptor) pair in one vectorised pass
+    B = len(descs)
+    ones = np.ones(B, dtype=np.uint64)
+    v = np.concatenate([descs[f].astype(np.uint64) for f in POINTER_FIELDS])
+    end = v + np.concatenate([np.broadcast_to(np.asarray(extent.get(f, ones), dtype=np.uint64), (B,))
+                              for f in POINTER_FIELDS])
+    idx = np.clip(np.searchsorted(lo, v, side='right') - 1, 0, len(hi) - 1)
+    bad = (v != 0) & ((v < lo[idx]) | (end > hi[idx]))
+    if bad.any():
+        j = int(np.flatnonzero(bad)[0])
+        raise RuntimeError(f'descriptor field {POINTER_FIELDS[j // B]} = {int(v[j]):#x} is outside the plan allocations')
 
 
 def _device():
