@@ -42,7 +42,7 @@ import numpy as np
 from .._checks import check_shape, check_times
 from ..gradient import Gradient
 from .. import _native
-from ..method import Dopri5, Expm, Method, Tsit5
+from ..method import Dopri5, Euler, Expm, Method, Tsit5
 from ..options import Options, check_options
 from ..qarrays import QArray
 from ..result import AdaptiveStepInfos, FixedStepInfos, MEPropagatorResult, SEPropagatorResult
@@ -54,7 +54,7 @@ from .operators import decompose, resolve_callables
 from .sesolve import build_plan, build_plan_constant, constant_batch, pad16
 from .sesolve import enumerate_instances as se_instances
 
-SUPPORTED_METHODS = (Expm, Tsit5, Dopri5)
+SUPPORTED_METHODS = (Expm, Tsit5, Dopri5, Euler)
 TAYLOR_DEGREE = 8  # remainder 0.5^9 / 9! e^0.5 < 1e-8 at ||X||_1 <= THETA
 THETA = 0.5
 
@@ -183,6 +183,9 @@ def slindbladian(H: np.ndarray, Ls) -> np.ndarray:
 
 
 def _method_kwargs(method, options, save_all: bool) -> dict:
+    if isinstance(method, Euler):
+        return dict(t0=None if options.t0 is None else float(options.t0), save_states=save_all,
+                    euler_dt=float(method.dt))
     return dict(
         t0=None if options.t0 is None else float(options.t0),
         rtol=method.rtol,
@@ -205,13 +208,15 @@ def _check_status(status) -> None:
         raise RuntimeError('an instance descriptor was rejected by the kernel (unsupported size or term count).')
 
 
-def _infos(res: dict, nbasis: int, out_shape: tuple) -> AdaptiveStepInfos:
+def _infos(res: dict, nbasis: int, out_shape: tuple, fixed: bool = False):
     """Step counts per propagator: those of its slowest basis solve."""
 
     def per(x):
         x = np.asarray(x).reshape(-1, nbasis).max(axis=1).reshape(out_shape)
         return x if out_shape else x[()]
 
+    if fixed:
+        return FixedStepInfos(per(res['nsteps']))
     return AdaptiveStepInfos(per(res['nsteps']), per(res['naccepted']), per(res['nrejected']))
 
 
@@ -224,7 +229,8 @@ def _finish(U: np.ndarray, dims, out_shape, tsave, method, gradient, options, re
     U = U.reshape(*out_shape, -1, N, N)
     if not options.save_propagators:  # the final propagator only, nsave = 1 (save_mixin.py:26-41)
         U = U[..., -1:, :, :]
-    return cls(tsave, method, gradient, options, QArray(U, dims=dims), None, extra, _infos(res, nbasis, out_shape))
+    infos = _infos(res, nbasis, out_shape, fixed=isinstance(method, Euler))
+    return cls(tsave, method, gradient, options, QArray(U, dims=dims), None, extra, infos)
 
 
 def sepropagator(
@@ -386,7 +392,11 @@ def mepropagator(
     basis = hermitian_basis(n)
     solve = [replace(inst, rho0=b, key_rho=None) for inst in insts for b in basis]
     kw = _method_kwargs(method, options, options.save_propagators or options.save_extra is not None)
-    plan = dense.build_plan('tsit5', solve, tsave, **kw)
+    if isinstance(method, Euler):  # the dense Kraus kernel with the Euler sets (no normalisation)
+        plan = dense.build_plan('rouchon1', solve, tsave, t0=kw['t0'], dt=kw['euler_dt'], normalize=False,
+                                save_states=kw['save_states'], rouchon_order=0)
+    else:
+        plan = dense.build_plan('tsit5', solve, tsave, **kw)
     plan.run()
     res = plan.results()
     _check_status(res['status'])

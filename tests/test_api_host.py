@@ -307,7 +307,7 @@ def test_propagator_argument_errors():
         dq.sepropagator(H, ts, options=dq.Options(cartesian_batching=False))
     with pytest.raises(ValueError, match=re.escape('Option `save_states` was set to `False` but is not used by the solver `dq.mepropagator()`')):
         dq.mepropagator(H, [a], ts, options=dq.Options(save_states=False))
-    with pytest.raises(TypeError, match=re.escape('Method of type `Dopri8` is not supported (supported method types: `Expm`, `Tsit5`, `Dopri5`)')):
+    with pytest.raises(TypeError, match=re.escape('Method of type `Dopri8` is not supported (supported method types: `Expm`, `Tsit5`, `Dopri5`, `Euler`)')):
         dq.mepropagator(H, [a], ts, method=dq.method.Dopri8())
     with pytest.raises(TypeError, match='Method `Expm` requires a constant or piecewise constant Hamiltonian.'):
         dq.sepropagator(dq.modulated(np.cos, H), ts, method=dq.method.Expm())

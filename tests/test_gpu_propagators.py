@@ -236,3 +236,27 @@ def test_expm_sesolve_and_mesolve_against_matrix_exponentials():
         np.testing.assert_allclose(got[b, 2], ref, atol=2e-5)
         np.testing.assert_allclose(rm.expects[b, 0, 2], np.trace(H0 @ ref), atol=1e-4)
     assert str(rm).split('\n')[1] == 'Method  : Expm'
+
+
+def test_euler_propagators_against_oracle():
+    """Euler(dt) propagators: sepropagator on the sesolve kernel's fixed-step mode,
+    mepropagator on the dense Kraus kernel with the Euler sets, against the oracle's
+    column-by-column Euler solves (linear interpolation between steps included)."""
+    import dynamiqs_amd as dq
+    from oracle import propagators as op_
+
+    rng = np.random.default_rng(14)
+    n = 4
+    H = _herm(rng, n)
+    L = 0.4 * (rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))) / np.sqrt(n)
+    ts = np.array([0.0, 0.25, 0.5])
+    m = dq.method.Euler(dt=0.01)
+    U = dq.sepropagator(H, ts, method=m)
+    ref = op_.sepropagator(H, ts, method='euler', dt=0.01)
+    np.testing.assert_allclose(U.propagators.to_numpy(), ref, atol=1e-5)
+    assert int(U.infos.nsteps) == 50
+    P = dq.mepropagator(H, [L], ts, method=m)
+    refP = op_.mepropagator(H, [L], ts, method='euler', dt=0.01)
+    err = np.max(np.abs(P.propagators.to_numpy() - refP))
+    print('Euler mepropagator max err vs oracle', err)
+    assert err <= 1e-5
