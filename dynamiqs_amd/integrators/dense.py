@@ -418,9 +418,7 @@ def build_plan(
     if tableau not in ('tsit5', 'dopri5'):
         raise ValueError(f'unknown Runge-Kutta pair {tableau!r}')
     lmod_any = any(lm is not None for inst in instances for lm in (inst.Lmod or []))
-    if any(inst.mod for inst in instances) or lmod_any:
-        if kind == 'rouchon1':
-            raise NotImplementedError('modulated operators are not supported with Rouchon1 on the MI355X solver')
+    if (any(inst.mod for inst in instances) or lmod_any) and kind != 'rouchon1':
         if max(len(_hmods(inst)) for inst in instances) > _native.MAX_MOD:
             raise NotImplementedError(
                 f'at most {_native.MAX_MOD} modulated terms per operator are supported (a modulated jump operator'
@@ -601,16 +599,19 @@ def build_plan(
             grid = None
             if inst.pwc:
                 grid = np.unique(np.concatenate([np.asarray(t, dtype=np.float64) for (t, _, _) in inst.pwc]))
+            # smooth (modulated) terms make every step's operators distinct: one set per step
+            lmods = list(inst.Lmod or [None] * K)
+            smooth = bool(inst.mod) or any(lm is not None for lm in lmods)
             keys = []
             for m in range(nsteps):
                 # Rouchon takes the operators at the step midpoint, Euler (order 0) at its start
                 tsel = step_t[m] if rouchon_order == 0 else mids[m]
                 seg = -1 if grid is None else int(np.searchsorted(grid, tsel, side='right')) - 1
-                keys.append((seg, float(hs[m])))
+                keys.append((seg, float(hs[m]), float(tsel) if smooth else 0.0))
             uniq = sorted(set(keys))
             index = {k: i for i, k in enumerate(uniq)}
             sets_D, sets_C = [], []
-            for seg, h in uniq:
+            for seg, h, tk in uniq:
                 tm = None
                 Hm = inst.H.copy()
                 if inst.pwc:
@@ -619,7 +620,12 @@ def build_plan(
                         if tm >= times[0] and tm < times[-1]:
                             idx = int(np.searchsorted(times, tm, side='right')) - 1
                             Hm = Hm + vals[idx] * op
-                Dk, Ck = _kraus_sets(Hm, inst.Ls, h, normalize, rouchon_order)
+                Lm = inst.Ls
+                if smooth:
+                    for g, op, _ in inst.mod:
+                        Hm = Hm + complex(g(tk)) * op
+                    Lm = [L if lm is None else complex(lm[0](tk)) * L for L, lm in zip(inst.Ls, lmods)]
+                Dk, Ck = _kraus_sets(Hm, Lm, h, normalize, rouchon_order)
                 sets_D.append(np.stack(Dk))
                 sets_C.append(np.stack(Ck))
             key = ('kraus', inst.key_H, inst.key_L)

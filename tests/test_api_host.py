@@ -75,8 +75,7 @@ def test_unsupported_time_dependence_is_reported():
     """What the kernels cannot run raises before any device work: callable
     operators that span more than four time-dependent directions, modulated
     operators with discontinuity times, jump operators other than one modulated
-    term, modulated operators with Rouchon1; a non-callable f is a TypeError
-    (time_qarray.py:161-165)."""
+    term; a non-callable f is a TypeError (time_qarray.py:161-165)."""
     H, Ls, psi, ts = _args()
     rng = np.random.default_rng(0)
     ops = [(lambda x: x + x.conj().T)(rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))) for _ in range(5)]
@@ -89,10 +88,7 @@ def test_unsupported_time_dependence_is_reported():
         dq.mesolve(H, [Ls[0] + dq.modulated(np.cos, Ls[0])], psi, ts)
     with pytest.raises(NotImplementedError, match='time-dependent jump operators'):
         dq.mesolve(H, [dq.pwc(np.array([0.0, 1.0]), np.array([1.0]), Ls[0])], psi, ts)
-    with pytest.raises(NotImplementedError, match='Rouchon1'):
-        dq.mesolve(H, [dq.modulated(np.cos, Ls[0])], psi, ts, method=dq.method.Rouchon1(dt=1e-2))
-    with pytest.raises(NotImplementedError, match='Rouchon1'):
-        dq.mesolve(H + dq.modulated(np.cos, H), Ls, psi, ts, method=dq.method.Rouchon1(dt=1e-2))
+
     with pytest.raises(TypeError, match=re.escape('Argument `f` must be a function, but has type float.')):
         dq.modulated(1.0, H)
 

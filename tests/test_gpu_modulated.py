@@ -234,3 +234,32 @@ def test_timecallable_jump_operator():
     err = np.linalg.norm(got - ref, axis=(-2, -1)).max()
     print('timecallable jump operator err vs oracle', err)
     assert err < 1e-5
+
+
+@pytest.mark.parametrize('method', ['rouchon1', 'rouchon2', 'euler'])
+def test_fixed_step_methods_with_modulated_operators(method):
+    """Rouchon1 / Rouchon2 / Euler with a modulated Hamiltonian term and a modulated
+    jump operator: one Kraus set per step, the operators at the step midpoint
+    (Rouchon) or start (Euler), against the oracle's fixed-step solve."""
+    import dynamiqs_amd as dq
+    from oracle import mesolve as om
+
+    n = 6
+    a = np.diag(np.sqrt(np.arange(1, n)), 1).astype(complex)
+    H0 = a.conj().T @ a
+    X = a + a.conj().T
+    fh = lambda t: 0.5 * np.cos(4.0 * t)  # noqa: E731
+    fl = lambda t: 0.5 * np.exp(-0.3 * t) * np.exp(1j * t)  # noqa: E731
+    rho0 = np.zeros((n, n), complex)
+    rho0[1, 1] = 1.0
+    ts = np.linspace(0.0, 1.0, 5)
+    m = {'rouchon1': dq.method.Rouchon1(dt=2e-3), 'rouchon2': dq.method.Rouchon2(dt=2e-3),
+         'euler': dq.method.Euler(dt=2e-3)}[method]
+    res = dq.mesolve(dq.asqarray(H0) + dq.modulated(fh, dq.asqarray(X)), [dq.modulated(fl, dq.asqarray(a))], rho0,
+                     ts, method=m)
+    Hop = om.TimeOp(H0, [om.ModTerm(fh, X)])
+    Lop = om.TimeOp(np.zeros((n, n), complex), [om.ModTerm(fl, a)])
+    ref = om.mesolve(Hop, [Lop], rho0, ts, method=method, dt=2e-3)['states']
+    err = np.linalg.norm(res.states.to_numpy() - ref, axis=(-2, -1)).max()
+    print(method, 'err vs oracle', err)
+    assert err < 1e-5
