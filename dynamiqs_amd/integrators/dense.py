@@ -602,6 +602,11 @@ def build_plan(
             # smooth (modulated) terms make every step's operators distinct: one set per step
             lmods = list(inst.Lmod or [None] * K)
             smooth = bool(inst.mod) or any(lm is not None for lm in lmods)
+            if smooth and nsteps * nLdev * nn * 8 > (2 << 30):
+                raise NotImplementedError(
+                    f'{nsteps} steps of modulated operators need one Kraus set per step '
+                    f'({nsteps * nLdev * nn * 8 / 2**30:.1f} GiB); use a larger dt or an adaptive method'
+                )
             keys = []
             for m in range(nsteps):
                 # Rouchon takes the operators at the step midpoint, Euler (order 0) at its start
