@@ -13,7 +13,10 @@
  *    dynamiqs/integrators/core/abstract_integrator.py:16-54). The library runs
  *   what ``MESolveDiffraxIntegrator.run`` (diffrax_integrator.py:135-147) and
  *   ``MESolveFixedRouchon1Integrator`` (rouchon_integrator.py:385-432) run, for a
- *   batch of independent instances per call.
+ *   batch of independent instances per call. The same seams of ``_sesolve``
+ *   (apis/sesolve.py:263-304) and of the propagators' Expm integrator
+ *   (expm_integrator.py:60-90) are served by ``dqh_sesolve_tsit5`` and
+ *   ``dqh_expm_propagators`` / ``dqh_cgemm_batched``.
  */
 #ifndef DQH_H
 #define DQH_H

@@ -20,11 +20,15 @@ It restates, in NumPy/SciPy complex128, the algorithm of the reference's
   from Tsitouras 2011 and Hairer-Norsett-Wanner II.4).
 * ``rouchon``    -- Rouchon1 Kraus step with Cholesky normalisation
   (``dynamiqs/integrators/core/rouchon_integrator.py:226-269, 552-582``).
-* ``mesolve``    -- the solver driver with PWC time dependence, saving and
-  expectation values (``dynamiqs/integrators/apis/mesolve.py``,
-  ``dynamiqs/time_qarray.py``, ``dynamiqs/integrators/core/save_mixin.py``).
+* ``mesolve``    -- the solver driver with PWC and modulated time dependence,
+  saving and expectation values (``dynamiqs/integrators/apis/mesolve.py``,
+  ``dynamiqs/time_qarray.py``, ``dynamiqs/integrators/core/save_mixin.py``), also
+  for Dopri5 (``dopri5``), Rouchon2 / Rouchon3 and Euler.
+* ``sesolve``    -- the Schrodinger-equation driver
+  (``dynamiqs/integrators/apis/sesolve.py``).
+* ``propagators`` -- ``sepropagator`` / ``mepropagator`` column by column.
 
 How each function is pinned (closed forms, library routines, reference KATs,
 order conditions) is stated in its module header and tested in
-``tests/test_oracle_*.py``.
+``tests/test_oracle.py``.
 """

@@ -12,7 +12,7 @@ Restates ``dynamiqs/qarrays/sparsedia_primitives.py``:
 * ``matmul_sparsedia_sparsedia`` :199-229 -> :func:`dia_matmul_dia`
 * ``transpose_sparsedia`` :18-29 (+ conj) -> :func:`dia_dag`
 
-Pinned in ``tests/test_oracle_dia.py`` against ``scipy.sparse.dia_matrix`` and
+Pinned in ``tests/test_oracle.py`` against ``scipy.sparse.dia_matrix`` and
 against dense products on the fixture patterns of the reference's DIA tests
 (offsets (-2, 0, 2) and (-2, -1, 1, 3) at N=10, ``tests/qarrays/
 test_sparse_dia_qarray.py:32-46``).

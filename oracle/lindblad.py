@@ -11,7 +11,7 @@ Restates ``dynamiqs/integrators/core/diffrax_integrator.py``:
   -> :func:`slindbladian`
 * ``dissipator`` / ``lindbladian`` (``dynamiqs/utils/general.py:560-654``).
 
-Pinned in ``tests/test_oracle_lindblad.py`` by the reference doctest KATs
+Pinned in ``tests/test_oracle.py`` by the reference doctest KATs
 (``dissipator(destroy(4), fock_dm(4, 2)) = diag(0, 2, -2, 0)`` and
 ``lindbladian(a^dag a, [a, a^dag a], fock_dm(4, 1)) = diag(1, -1, 0, 0)``),
 trace preservation, Hermiticity, agreement of the three forms, and a brute-force

@@ -7,7 +7,7 @@ matrices :624-740) and ``dynamiqs/utils/states.py`` (``fock`` :33,
 the Kronecker ordering of ``dynamiqs/utils/general.py:tensor`` (first mode is
 the most significant index).
 
-Pinned in ``tests/test_oracle_operators.py`` against the reference docstring
+Pinned in ``tests/test_oracle.py`` against the reference docstring
 printouts (``coherent(4, 0.5)``, ``coherent_dm(4, 0.5)``, ``displace(4, 0.5)``,
 ``position(3)``, ``momentum(3)``) and the Poisson photon statistics of a
 coherent state.

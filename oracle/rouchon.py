@@ -18,7 +18,7 @@ Restates ``dynamiqs/integrators/core/rouchon_integrator.py``:
 Two implementations of the same step: :func:`kraus_step_dense` (dense LAPACK
 Cholesky / triangular solves, exactly the reference's calls) and
 :func:`kraus_step_banded` (banded Cholesky and banded solves for DIA operators).
-Pinned in ``tests/test_oracle_rouchon.py``: the two agree to rounding; trace
+Pinned in ``tests/test_oracle.py``: the two agree to rounding; trace
 preservation to machine precision; complete positivity (normalised Kraus
 operators satisfy ``sum A_k^dag A_k = I``); first-order convergence to the exact
 Lindblad solution (``expm`` of the superoperator) on the lossy oscillator.

@@ -28,7 +28,7 @@ reference, so this module restates its published algorithm:
 * Saving: states at ``tsave`` by the continuous extension of the step that
   contains them; ``tsave[0] == t0`` saves the initial state.
 
-Pinned in ``tests/test_oracle_tsit5.py``: row sums ``c_i = sum_j a_ij``, the 17
+Pinned in ``tests/test_oracle.py``: row sums ``c_i = sum_j a_ij``, the 17
 order conditions up to order 5 for ``b`` and the order-4 conditions for
 ``b - e``, the interpolant's order-4 conditions at several theta and
 ``b_i(1) = b_i``; empirical 5th-order convergence; the closed-form lossy
