@@ -260,3 +260,23 @@ def test_euler_propagators_against_oracle():
     err = np.max(np.abs(P.propagators.to_numpy() - refP))
     print('Euler mepropagator max err vs oracle', err)
     assert err <= 1e-5
+
+
+def test_single_save_time_edge_cases():
+    """tsave with one time (nothing to integrate): sesolve returns psi0, the Expm
+    propagators the identity with zero steps, mesolve with Expm rho0."""
+    import dynamiqs_amd as dq
+
+    H = dq.sigmax()
+    ts = np.array([0.0])
+    r = dq.sesolve(H, dq.fock(2, 0), ts)
+    assert r.states.shape == (1, 2, 1)
+    np.testing.assert_allclose(r.states.to_numpy()[0, :, 0], [1, 0], atol=1e-7)
+    U = dq.sepropagator(H, ts)
+    assert type(U.method).__name__ == 'Expm' and int(U.infos.nsteps) == 0
+    np.testing.assert_allclose(U.propagators.to_numpy()[0], np.eye(2), atol=0)
+    rm = dq.mesolve(H, [0.1 * dq.sigmam()], dq.fock(2, 0), ts, method=dq.method.Expm())
+    np.testing.assert_allclose(rm.states.to_numpy()[0], np.diag([1.0, 0.0]), atol=1e-7)
+    # t0 before the first save time
+    U2 = dq.sepropagator(H, np.array([0.5]), options=dq.Options(t0=0.0))
+    np.testing.assert_allclose(U2.propagators.to_numpy()[0], sla.expm(-1j * 0.5 * H.to_numpy()), atol=2e-6)
