@@ -16,7 +16,7 @@ import numpy as np
 import scipy.linalg as sla
 
 from .. import _native
-from .operators import resample_pwc, spline_table
+from .operators import spline_table
 
 MAX_NP = 256
 
