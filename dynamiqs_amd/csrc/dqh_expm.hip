@@ -36,8 +36,14 @@ __global__ __launch_bounds__(64 * kWaves) void k_cgemm(const float2 *__restrict_
   __shared__ float part[kWaves - 1][8][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nt = N / 16;
-  const int tile = blockIdx.x;
-  const int64_t b = b0 + blockIdx.y;
+  // XCD-aware order: workgroup L runs on XCD L % 8, so work item
+  // (L % 8) * (total / 8) + L / 8 gives each XCD a contiguous run of tiles, and the
+  // tiles of one product share that XCD's L2 (identity order when total % 8 != 0)
+  const int64_t total = (int64_t)gridDim.x * gridDim.y;
+  const int64_t L = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+  const int64_t w = (total % 8 == 0) ? (L % 8) * (total / 8) + L / 8 : L;
+  const int tile = (int)(w % gridDim.x);
+  const int64_t b = b0 + w / gridDim.x;
   const int tr = tile / nt, tc = tile - tr * nt;
   const int kc = ((nt + kWaves - 1) / kWaves) * 16;  // k columns per wave, a multiple of 16
   const int kbeg = wave * kc < N ? wave * kc : N;
