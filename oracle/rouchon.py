@@ -270,3 +270,60 @@ def integrate(step, rho0: np.ndarray, tsave: np.ndarray, dt: float, *, t0=None, 
             isave += 1
         rho = new
     return saved, rho, len(ts) - 1
+
+
+# ------------------------------------------------- block-diagonal normalisation
+# When S splits into independent blocks (index sets that S never couples: for the
+# two-mode problem with an excitation-number-conserving H and jump operators whose
+# L^dag L is diagonal, the sets of fixed a^dag a + b^dag b), the Cholesky factor of
+# S is block-diagonal on the same sets, provided each set keeps the original
+# relative order of its indices: the factor assembled from the blocks' factors is
+# lower triangular in the original order and satisfies T T^dag = S, so by
+# uniqueness of the Cholesky factorisation it IS chol(S). The normalisation
+# T^-dag rho T^-1 is then a set of small products on row and column slices. Same
+# step as kraus_step_dense, at a cost that allows 5000 FP64 steps at n = 1024.
+
+
+def coupled_blocks(S: np.ndarray, tol: float = 0.0):
+    """Index sets (each sorted ascending) of the connected components of the graph
+    with an edge i-j wherever |S[i, j]| > tol."""
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import connected_components
+
+    ncomp, labels = connected_components(csr_matrix(np.abs(S) > tol), directed=False)
+    return [np.flatnonzero(labels == c) for c in range(ncomp)]
+
+
+def block_factor(S: np.ndarray, blocks):
+    """``[(idx, T_b^-1)]``: the inverse Cholesky factor of every diagonal block."""
+    out = []
+    for idx in blocks:
+        Tb = np.linalg.cholesky(S[np.ix_(idx, idx)])
+        out.append((idx, sla.solve_triangular(Tb, np.eye(len(idx)), lower=True)))
+    return out
+
+
+def block_normalize(fac, rho: np.ndarray) -> np.ndarray:
+    """``T^-dag rho T^-1`` with ``T`` block-diagonal on the index sets of ``fac``."""
+    x = np.empty_like(rho)
+    for idx, Ti in fac:
+        x[idx, :] = dag(Ti) @ rho[idx, :]
+    y = np.empty_like(rho)
+    for idx, Ti in fac:
+        y[:, idx] = x[:, idx] @ Ti
+    return y
+
+
+def kraus_step_dia_blocks(dM, Ls_dia, rho: np.ndarray, fac) -> np.ndarray:
+    """One Rouchon1 step with DIA operators: ``X = T^-dag rho T^-1``,
+    ``rho' = M0 X M0^dag + sum_k M_k X M_k^dag`` with ``M0 = I + dM`` and ``dM``,
+    ``M_k = sqrt(dt) L_k`` given as ``(offsets, diags)`` (products as in
+    ``oracle.dia``)."""
+    from .dia import dense_matmul_dia, dia_dag, dia_matmul_dense
+
+    X = block_normalize(fac, rho)
+    Y = X + dia_matmul_dense(*dM, X)  # M0 X
+    out = Y + dense_matmul_dia(Y, *dia_dag(*dM))  # M0 X M0^dag
+    for o, d in Ls_dia:
+        out = out + dense_matmul_dia(dia_matmul_dense(o, d, X), *dia_dag(o, d))
+    return out

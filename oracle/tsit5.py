@@ -171,9 +171,14 @@ def integrate(
     jump_ts=None,
     save=None,
     tab: Tableau = TSIT5,
+    dtype=np.complex128,
 ):
     """Integrate ``dy/dt = f(t, y, tseg)`` from ``t0`` to ``tsave[-1]`` with the
     pair ``tab`` (Tsit5 by default, or ``tableau('dopri5')``).
+
+    ``dtype=np.complex64`` runs the same algorithm in single precision (state,
+    stages and tableau in complex64 / float32, the error norm in float64): the
+    complex64 CPU restatement that is timed beside the GPU, not a parity reference.
 
     ``f(t, y, tseg)`` evaluates the vector field at time ``t``; ``tseg`` is the
     start of the current step, used by piecewise-constant generators to pick the
@@ -190,7 +195,9 @@ def integrate(
     jumps = jumps[(jumps > t) & (jumps < t_end)]
 
     stats = Tsit5Stats()
-    y = np.array(y0, dtype=np.complex128)
+    y = np.array(y0, dtype=dtype)
+    real = np.float32 if np.dtype(dtype) == np.complex64 else np.float64
+    tA, tE = tab.A.astype(real), tab.E.astype(real)
     saved = []
     isave = 0
     while isave < len(tsave) and tsave[isave] <= t:
@@ -221,15 +228,16 @@ def integrate(
             t_new = target
             made_jump = target_jump
 
-        ks = [h * k1]
+        hs = real(h)
+        ks = [hs * k1]
         for s in range(1, 7):
-            ys = y + sum(tab.A[s, j] * ks[j] for j in range(s))
-            ks.append(h * f(t + tab.C[s] * h, ys, t))
+            ys = y + sum(tA[s, j] * ks[j] for j in range(s))
+            ks.append(hs * f(t + tab.C[s] * h, ys, t))
             stats.nrhs += 1
-        y_new = y + sum(tab.A[6, j] * ks[j] for j in range(6))
-        err = sum(tab.E[j] * ks[j] for j in range(7))
-        scale = atol + rtol * np.maximum(np.abs(y), np.abs(y_new))
-        norm = rms_norm(err / scale)
+        y_new = y + sum(tA[6, j] * ks[j] for j in range(6))
+        err = sum(tE[j] * ks[j] for j in range(7))
+        scale = atol + rtol * np.maximum(np.abs(y), np.abs(y_new)).astype(np.float64)
+        norm = rms_norm(err.astype(np.complex128) / scale)
         keep = norm < 1.0
         if norm == 0.0:
             factor = factormax
@@ -243,12 +251,12 @@ def integrate(
             stats.ends.append(t_new)
             while isave < len(tsave) and tsave[isave] <= t_new:
                 theta = (tsave[isave] - t) / h
-                w = interp_weights(theta, tab)
+                w = interp_weights(theta, tab).astype(real)
                 saved.append(save(y + sum(w[j] * ks[j] for j in range(7))))
                 isave += 1
             t = t_new
             y = y_new
-            k1 = ks[6] / h
+            k1 = ks[6] / hs
             need_k1 = made_jump
         else:
             stats.nrejected += 1

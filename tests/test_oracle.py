@@ -450,6 +450,77 @@ def test_rouchon_banded_matches_dense():
     assert abs(np.trace(x) - 1) < 1e-13
 
 
+def test_rouchon_block_normalisation_equals_dense_step():
+    """kraus_step_dia_blocks (Cholesky of S block by block on the index sets S never
+    couples, DIA products) equals the dense LAPACK step on non-Hermitian inputs; the
+    blocks of the two-mode problem are the sets of fixed excitation number; and a
+    partition that splits a coupled set (a plausible mistake) is caught."""
+    rng = np.random.default_rng(8)
+    a, b = op.destroy(6, 5)
+    ad, bd = a.conj().T, b.conj().T
+    H = 5 * ad @ a + 6 * bd @ b + 0.1 * (ad @ b + a @ bd) - 0.01 * ad @ a @ bd @ b
+    Ls = [np.sqrt(0.02) * a, np.sqrt(0.05) * b]
+    n = H.shape[0]
+    dt = 0.02
+    M0, Ms, S = rouchon.kraus_ops(H, Ls, dt)
+    blocks = rouchon.coupled_blocks(S)
+    Ntot = np.real(np.diag(ad @ a + bd @ b)).round().astype(int)
+    assert len(blocks) == len(np.unique(Ntot))
+    for idx in blocks:
+        assert len(np.unique(Ntot[idx])) == 1 and np.all(np.diff(idx) > 0)
+    fac = rouchon.block_factor(S, blocks)
+    dM = dia.dia_from_dense(M0 - np.eye(n))
+    Ld = [dia.dia_from_dense(M) for M in Ms]
+    rho = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    x, y = rho.copy(), rho.copy()
+    for _ in range(5):
+        x = rouchon.kraus_step_dense(H, Ls, x, dt)
+        y = rouchon.kraus_step_dia_blocks(dM, Ld, y, fac)
+    np.testing.assert_allclose(y, x, atol=1e-13 * np.abs(x).max())
+    big = max(blocks, key=len)
+    split = [blk for blk in blocks if blk is not big] + [big[:1], big[1:]]
+    z = rouchon.kraus_step_dia_blocks(dM, Ld, rho, rouchon.block_factor(S, split))
+    assert np.abs(z - rouchon.kraus_step_dense(H, Ls, rho, dt)).max() > 1e-6
+
+
+def test_c3_fixture_is_a_trace_one_state_consistent_with_its_expectations():
+    """The committed full-length DIA Rouchon1 fixture (alpha = 2, 5000 steps): the
+    normalised scheme keeps Tr(rho) = 1 at every save to rounding, <a^dag a>(0) =
+    |alpha|^2, purity in (0, 1], and the stored final block is Hermitian with
+    <a^dag a> equal to the last saved expectation value."""
+    import os
+
+    fx = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden', 'c3_dia_rouchon1_alpha2.npz'))
+    assert int(fx['nsteps']) == 5000 and len(fx['tsave']) == 51
+    assert np.abs(fx['trace'] - 1).max() < 1e-10
+    assert abs(fx['expect_a'][0] - 4.0) < 1e-10 and abs(fx['expect_b'][0]) < 1e-12
+    assert np.all(fx['purity'] <= 1 + 1e-12) and np.all(fx['purity'] > 0.9)
+    blk = fx['final_block'].astype(complex)
+    idx = fx['final_index']
+    assert np.abs(blk - blk.conj().T).max() < 1e-7
+    na = idx // 32  # a^dag a of the basis state (i_a, i_b) with index 32 i_a + i_b
+    assert abs(np.real(np.diag(blk)) @ na - fx['expect_a'][-1]) < 1e-6
+    assert abs(np.real(np.trace(blk)) - 1) < 1e-6 and float(fx['final_tail_frobenius']) < 1e-6
+
+
+def test_c5_fixture_fd_matches_exact_derivative():
+    """The committed full-size adjoint fixture: its central differences agree with
+    the exact (Frechet) derivative to O(h^2), and the loss of the first control
+    vector equals the oracle's tight-tolerance Tsit5 solve."""
+    import os
+
+    fx = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden', 'c5_adjoint_n32_fd.npz'))
+    fd, ex = fx['grad_fd'], fx['grad_exact']
+    assert fd.shape == (2, 1, 100)
+    assert np.abs(fd - ex).max() <= 1e-8 * np.abs(ex).max()
+    a = op.destroy(32)
+    X = a + a.conj().T
+    Hop = om.TimeOp(a.conj().T @ a, [om.PWCTerm(fx['times'], fx['u'][0, 0], X)])
+    r = om.mesolve(Hop, [np.sqrt(0.05) * a], op.fock_dm(32, 0), np.array([0.0, 10.0]), exp_ops=[a.conj().T @ a],
+                   rtol=1e-10, atol=1e-12)
+    assert abs(r['expects'][0, -1].real - fx['J'][0]) < 1e-8
+
+
 def test_rouchon_cholesky_matches_reference_formula():
     # cholesky_normalize(S, rho) == T^{-dag} rho T^{-1} with S = T T^dag
     rng = np.random.default_rng(7)
