@@ -127,7 +127,7 @@ def dia_candidates(n_pad: int, maxoff: int) -> list[int]:
 def dia_sweep(H: np.ndarray, Ls, batch: int, kind: str = 'rhs', dt: float = 1e-3) -> int:
     """Fastest sweep form (a ``SWEEP_*`` value) for the DIA operators of (H, Ls):
     ``kind`` 'rhs' times the Lindblad vector field (DIA Tsit5), 'kraus' times three
-    Rouchon1 steps of length ``dt`` (the split two-pass form is a candidate there)."""
+    Rouchon1 steps of length ``dt``."""
     from .integrators import dia
 
     if not enabled():
@@ -141,7 +141,7 @@ def dia_sweep(H: np.ndarray, Ls, batch: int, kind: str = 'rhs', dt: float = 1e-3
     if hit is not None:
         return int(hit['variant'])
     maxoff = max((abs(int(x)) for o, _ in ops for x in o), default=0)
-    cands = dia_candidates(n_pad, maxoff) + ([dia.SWEEP_SPLIT] if kind == 'kraus' else [])
+    cands = dia_candidates(n_pad, maxoff)
     if len(cands) == 1:
         record(key, {'variant': cands[0], 'us': {}})
         return cands[0]

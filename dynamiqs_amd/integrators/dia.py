@@ -58,12 +58,13 @@ class DiaRouchon1(ctypes.Structure):
         ('expects', ctypes.c_uint64),
         ('variant', ctypes.c_int64),
         ('solve_panel', ctypes.c_int64),
+        ('timing', ctypes.c_uint64),
     ]
 
 
 SWEEP_AUTO, SWEEP_GLOBAL, SWEEP_TILED, SWEEP_TILED_H8, SWEEP_TILED_H16, SWEEP_TILED_H32 = 0, 1, 2, 3, 4, 5
 SWEEP_HALO = {SWEEP_TILED_H8: 8, SWEEP_TILED_H16: 16, SWEEP_TILED_H32: 32}
-SWEEP_SPLIT = 6  # Rouchon1 Kraus sweep only
+SWEEP_SPLIT = 6  # accepted by the Rouchon1 Kraus sweep, runs its global form
 
 
 class DiaRhs(ctypes.Structure):
@@ -164,6 +165,21 @@ class DiaPlan:
 
     def reset(self):
         self.rho64.copy_(self.rho64_0)
+
+    def phase_us(self, stream=None) -> dict:
+        """Reset and run once with HIP events around the block solves and the Kraus
+        sweep of steps 1..8: mean microseconds per step of each phase."""
+        import torch
+
+        t = np.zeros(2, dtype=np.float64)
+        self.reset()
+        self.problem.timing = t.ctypes.data
+        try:
+            self.run(stream)
+        finally:
+            self.problem.timing = 0
+        torch.cuda.synchronize()
+        return {'solve': 1e3 * float(t[0]), 'sweep': 1e3 * float(t[1])}
 
     def results(self) -> dict:
         import torch

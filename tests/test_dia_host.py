@@ -179,38 +179,20 @@ def test_tiled_sweep_indexing_reproduces_dense_rhs(na, nb_, halo):
     assert np.abs(out[n:, :]).max(initial=0.0) == 0 and np.abs(out[:, n:]).max(initial=0.0) == 0
 
 
-def test_split_kraus_passes_match_stencil_form():
-    """k_kraus_left / k_kraus_right index arithmetic: dY[i][j] = sum_u D0[u][i+p] X[i+p][j]
-    then E + dY + sum_u (X + dY)[i][j+p] conj(D0[u][j+p]) + sum_{s>=1} A_s X A_s^dag
-    equals the one-pass increment E + dM X + X dM^dag + dM X dM^dag + sum A X A^dag."""
+def test_hi_lo_state_and_fused_sweep_emulation_matches_oracle():
+    """The kernels' arithmetic of a Rouchon1 step, emulated in the precisions they
+    use: the state as FP32 planes hi + lo, the block solves on hi (complex64
+    outputs), the second solve writing E = D1 + d^dag and X = (hi + D1) + d^dag in
+    complex64, the fused sweep's increment E + dM X + X dM^dag + dM X dM^dag +
+    sum A X A^dag added to hi + lo in FP64 and split back. Twenty steps stay within
+    2e-6 of the oracle's dense FP64 Rouchon1; hi + lo carries the state to ~1e-14."""
     H, Ls = _two_mode(12, 12)
     n = 144
     n_pad = 192
-    Vp, Wn, ops = dia.block_solve_operators(H, Ls, 1e-3, n_pad)
-    ops_p = []
-    for offs, d in ops:
-        dp = np.zeros((len(offs), n_pad), dtype=complex)
-        dp[:, :n] = d
-        ops_p.append((offs, dp))
-    rng = np.random.default_rng(5)
-    R = np.zeros((n_pad, n_pad), dtype=complex)
-    E = np.zeros_like(R)
-    R[:n, :n] = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
-    E[:n, :n] = 1e-3 * (rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)))
-    X = R + E
-    offs0, D0 = ops_p[0]
-    idx = np.arange(n_pad)
-    dY = np.zeros_like(X)
-    for u, p in enumerate(offs0):  # pass 1, row neighbours
-        a = idx + p
-        ok = (a >= 0) & (a < n_pad)
-        dY[ok, :] += D0[u][a[ok]][:, None] * X[a[ok], :]
-    W = X + dY
-    inc = E + dY
-    for u, p in enumerate(offs0):  # pass 2, column neighbours
-        b = idx + p
-        ok = (b >= 0) & (b < n_pad)
-        inc[:, ok] += W[:, b[ok]] * np.conj(D0[u][b[ok]])[None, :]
+    dt = 1e-3
+    NB = dia.NB
+    nblk = n_pad // NB
+    Vp, Wn, ops = dia.block_solve_operators(H, Ls, dt, n_pad)
 
     def dense(offs, d):
         out = np.zeros((n_pad, n_pad), dtype=complex)
@@ -219,15 +201,56 @@ def test_split_kraus_passes_match_stencil_form():
             out[c - o, c] = d[u, c]
         return out
 
-    for offs, d in ops_p[1:]:
-        A = dense(offs, d)
-        inc = inc + A @ X @ A.conj().T
-    dM = dense(*ops_p[0])
-    want = E + dM @ X + X @ dM.conj().T + dM @ X @ dM.conj().T
-    for offs, d in ops_p[1:]:
-        A = dense(offs, d)
-        want = want + A @ X @ A.conj().T
-    np.testing.assert_allclose(inc, want, atol=1e-11)
+    mats = []
+    for offs, d in ops:
+        dp = np.zeros((len(offs), n_pad), dtype=complex)
+        dp[:, :n] = d
+        mats.append(dense(offs, dp))
+
+    def lsolve(s):
+        d = np.zeros_like(s)
+        prev = np.zeros((NB, n_pad), dtype=complex)
+        for i in reversed(range(nblk)):
+            r = slice(i * NB, (i + 1) * NB)
+            di = Vp[i].astype(np.complex64) @ s[r] + (Wn[i].astype(np.complex64) @ prev if i + 1 < nblk else 0)
+            d[r] = di
+            prev = s[r] + di
+        return d.astype(np.complex64)
+
+    rng = np.random.default_rng(2)
+    psi = rng.normal(size=(n, 1)) + 1j * rng.normal(size=(n, 1))
+    rho = np.zeros((n_pad, n_pad), dtype=complex)
+    rho[:n, :n] = psi @ psi.conj().T / np.vdot(psi, psi).real
+    hi = rho.astype(np.complex64)
+    lo = (rho - hi).astype(np.complex64)
+    assert np.abs(hi.astype(complex) + lo - rho).max() < 1e-14
+    want = rho[:n, :n]
+    for _ in range(20):
+        D1 = lsolve(hi)
+        T1 = (hi + D1).astype(np.complex64)
+        d = lsolve(T1.conj().T)
+        E = (D1 + d.conj().T).astype(np.complex64)
+        X = (T1 + d.conj().T).astype(np.complex64)
+        Xc = X.astype(complex)
+        dM = mats[0]
+        acc = dM @ Xc + Xc @ dM.conj().T + dM @ Xc @ dM.conj().T
+        for A in mats[1:]:
+            acc = acc + A @ Xc @ A.conj().T
+        new = hi.astype(complex) + lo + (E.astype(complex) + acc.astype(np.complex64))
+        hi = new.astype(np.complex64)
+        lo = (new - hi).astype(np.complex64)
+        want = rouchon.kraus_step_dense(H, Ls, want, dt)
+    got = hi.astype(complex) + lo
+    err = np.linalg.norm(got[:n, :n] - want)
+    assert err < 2e-6, err
+    assert abs(np.trace(got) - 1) < 1e-6
+    assert np.abs(got[n:, :]).max() < 1e-12 and np.abs(got[:, n:]).max() < 1e-12
+    # the normalisation is what X carries: forming X without the second solve's
+    # correction (X = hi + D1) is off by far more than rounding
+    D1 = lsolve(hi)
+    bad = (hi + D1).astype(complex)
+    good = (hi + D1).astype(np.complex64) + lsolve((hi + D1).astype(np.complex64).conj().T).conj().T
+    assert np.abs(bad - good).max() > 1e-4
 
 
 @pytest.mark.parametrize('n', [64, 100, 192, 256, 1024])
