@@ -66,6 +66,7 @@ FLAG_GLOBAL = 4
 FLAG_DOPRI5 = 8
 FLAG_EULER = 16
 FLAG_LMOD = 32
+FLAG_SIGNAL = 64
 
 STATUS_OK = 0
 STATUS_MAX_STEPS = -3
@@ -97,6 +98,7 @@ SYMBOLS = {
     'dqh_adjoint_pwc_quadrature': (ctypes.c_int, [_I64, _I64, _I64, _I64, _I64, _P, _P, _P, _P, _P, _P, _P]),
     'dqh_adjoint_pwc_quadrature_dia': (ctypes.c_int, [_I64, _I64, _I64, _I64, _P, _P, _P, _P, _P, _P, _P]),
     'dqh_adjoint_rate_quadrature': (ctypes.c_int, [_I64, _I64, _I64, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P]),
+    'dqh_host_device_pointer': (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
     'dqh_last_error': (ctypes.c_char_p, []),
     'dqh_device_info': (ctypes.c_int, [ctypes.c_char_p, _I64, ctypes.POINTER(_I64)]),
 }
@@ -140,6 +142,13 @@ def check(code: int, what: str) -> None:
 
 def work_elems(np_: int, nL: int, npwc: int) -> int:
     return int(lib().dqh_me_work_elems(np_, nL, npwc))
+
+
+def host_device_pointer(host_ptr: int) -> int:
+    """Device address of pinned host memory (raises if it is not pinned and mapped)."""
+    out = ctypes.c_void_p()
+    check(lib().dqh_host_device_pointer(ctypes.c_void_p(host_ptr), ctypes.byref(out)), 'dqh_host_device_pointer')
+    return int(out.value)
 
 
 def current_stream_ptr():

@@ -32,3 +32,13 @@ extern "C" int dqh_device_info(char *name, int64_t len, int64_t *cus) {
   if (cus) *cus = prop.multiProcessorCount;
   return DQH_OK;
 }
+
+extern "C" int dqh_host_device_pointer(void *host, void **device) {
+  if (!host || !device) return dqh::fail(DQH_BAD_ARG, "dqh_host_device_pointer: null argument");
+  void *d = nullptr;
+  const hipError_t e = hipHostGetDevicePointer(&d, host, 0);
+  if (e != hipSuccess || !d)
+    return dqh::fail(DQH_HIP_ERROR, std::string("dqh_host_device_pointer: ") + hipGetErrorString(e));
+  *device = d;
+  return DQH_OK;
+}

@@ -605,18 +605,36 @@ namespace dqh {
 // One launch serves a heterogeneous batch: each workgroup picks the body that
 // fits its instance (register/LDS-resident for np <= 64 and at most 3 jump
 // operators, global-memory state otherwise).
+// With DQH_FLAG_SIGNAL, publish the instance's completion: every thread makes its
+// output stores visible at system scope, then thread 0 sets stats[7] = 1 with a
+// system-scope release, so that a host polling stats[7] (outputs in pinned host
+// memory) reads the finished outputs while the rest of the launch still runs.
+// flags is the same for the whole workgroup, so every thread reaches the barrier.
+__device__ __forceinline__ void signal_done(const dqh_me_inst &p) {
+  if (!(p.flags & DQH_FLAG_SIGNAL)) return;
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0)
+    __hip_atomic_store(gp<int64_t>(p.stats) + 7, (int64_t)1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 __global__ __launch_bounds__(kThreads) void k_tsit5(const dqh_me_inst *__restrict__ insts) {
   __shared__ double sh[kWaves];
   __shared__ float2 lds[kResLdsFloat2];
   const dqh_me_inst &p = insts[blockIdx.x];
   const int np = (int)p.np;
-  if (p.nL <= kResMaxL && !(p.flags & (DQH_FLAG_GLOBAL | DQH_FLAG_LMOD))) {
-    if (np == 16) return tsit5_resident<16, kWaves>(p, sh, lds);
-    if (np == 32) return tsit5_resident<32, kWaves>(p, sh, lds);
-    if (np == 48) return tsit5_resident<48, kWaves>(p, sh, lds);
-    if (np == 64) return tsit5_resident<64, kWaves>(p, sh, lds);
-  }
-  tsit5_global<kWaves>(p, sh);
+  const bool res = p.nL <= kResMaxL && !(p.flags & (DQH_FLAG_GLOBAL | DQH_FLAG_LMOD));
+  if (res && np == 16)
+    tsit5_resident<16, kWaves>(p, sh, lds);
+  else if (res && np == 32)
+    tsit5_resident<32, kWaves>(p, sh, lds);
+  else if (res && np == 48)
+    tsit5_resident<48, kWaves>(p, sh, lds);
+  else if (res && np == 64)
+    tsit5_resident<64, kWaves>(p, sh, lds);
+  else
+    tsit5_global<kWaves>(p, sh);
+  signal_done(p);
 }
 
 // A batch whose instances all take the resident body for one NP: the kernel holds
@@ -638,9 +656,10 @@ __global__ __launch_bounds__(res_waves<NP>() * 64) void k_tsit5_res(const dqh_me
   const dqh_me_inst &p = insts[blockIdx.x];
   if (p.np != NP || p.nL > kResMaxL || (p.flags & (DQH_FLAG_GLOBAL | DQH_FLAG_LMOD))) {
     if (threadIdx.x == 0) gp<int64_t>(p.stats)[3] = DQH_UNSUPPORTED;
-    return;
+  } else {
+    tsit5_resident<NP, NW>(p, sh, lds);
   }
-  tsit5_resident<NP, NW>(p, sh, lds);
+  signal_done(p);
 }
 
 __global__ __launch_bounds__(kThreads) void k_rhs(const dqh_me_inst *__restrict__ insts) {

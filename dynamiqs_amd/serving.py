@@ -8,11 +8,21 @@ with its own adaptive step, so uneven requests share a launch), bounded by a
 device-memory budget, and launches it on the next free HIP stream. Requests of
 padded dimension above GRID_MIN_NP run on the grid-wide Tsit5 instead (each stage
 spread over many CUs, driven by a worker thread), so that one large request does
-not hold a single CU for the whole solve. Launches on
-different streams overlap on the device; when a launch's completion event fires,
-its requests are answered and their memory budget is released, which admits the
-next pending requests. Each request's state, stages and outputs stay resident in
-HBM for the whole solve.
+not hold a single CU for the whole solve. Launches on different streams overlap on
+the device.
+
+Packing: the inputs of a launch (generators, jump operators, initial states,
+save times) and its descriptor array are written on the host into one pinned
+staging buffer and cross to the device in one copy; the FP64 state, the stage
+scratch and every input stay resident in HBM for the whole solve.
+
+Retirement is per request: the outputs of a persistent launch (saved states,
+expectation values, final state, statistics) live in pinned host memory that the
+kernel writes directly, and each workgroup publishes its request's completion in
+stats[7] with a system-scope release (``DQH_FLAG_SIGNAL``). The scheduler polls
+those words and answers a request as soon as its own solve is done, whatever the
+other requests of its launch are still doing; the launch's device memory returns
+to the budget when its last request is done.
 
 The result of a request does not depend on what it was batched with: the kernel
 keeps no cross-instance state and each workgroup's arithmetic is fixed by its own
@@ -47,6 +57,7 @@ class Request:
     atol: float = 1e-6
     max_steps: int = 100_000
     exp_ops: list = field(default_factory=list)
+    step_log_cap: int = 0  # > 0: record the end time of each accepted step (the first step_log_cap)
     t_submit: float = 0.0
     future: Future | None = None
 
@@ -65,6 +76,7 @@ class Response:
     status: int
     latency_s: float
     queue_s: float
+    step_ends: np.ndarray | None = None  # end times of the accepted steps (when requested)
 
 
 def _needs_work(npd: int, K: int) -> bool:
@@ -72,46 +84,60 @@ def _needs_work(npd: int, K: int) -> bool:
     return npd > 64 or K > 3
 
 
-def _layout(n: int, K: int, nt: int, nE: int):
-    """Byte offsets of one request's input and output buffers (256-byte aligned)."""
+def _align(nbytes: int) -> int:
+    return (nbytes + 255) // 256 * 256
+
+
+def _layout(n: int, K: int, nt: int, nE: int, nlog: int = 0):
+    """Byte offsets of one request's buffers, 256-byte aligned: the staged inputs
+    (copied from the host, the FP64 state buffers last), the device scratch of the
+    stages (never copied) and the outputs. Returns (input offsets, input bytes,
+    scratch bytes, output offsets, output bytes)."""
     npd = pad16(n)
     nn = npd * npd
     lay_in, off = {}, 0
-    for name, nbytes in (
-        ('H', nn * 8),
-        ('D', max(K, 1) * nn * 8),
-        ('C', max(K, 1) * nn * 8),
-        ('E', max(nE, 1) * nn * 8),
-        ('y', 2 * nn * 16),
-        ('w', _native.work_elems(npd, K, 0) * 8 if _needs_work(npd, K) else 16),
-        ('ts', nt * 8),
-    ):
+    for name, nbytes in (('H', nn * 8), ('D', max(K, 1) * nn * 8), ('C', max(K, 1) * nn * 8),
+                         ('E', max(nE, 1) * nn * 8), ('ts', nt * 8), ('y', 2 * nn * 16)):
         lay_in[name] = off
-        off += (nbytes + 255) // 256 * 256
+        off += _align(nbytes)
     size_in = off
+    work = _align(_native.work_elems(npd, K, 0) * 8) if _needs_work(npd, K) else 256
     lay_out, off = {}, 0
-    for name, nbytes in (('st', nt * n * n * 8), ('fin', n * n * 8), ('ex', max(nE, 1) * nt * 8), ('stats', 64)):
+    for name, nbytes in (('st', nt * n * n * 8), ('fin', n * n * 8), ('ex', max(nE, 1) * nt * 8), ('stats', 64),
+                         ('log', nlog * 8)):
         lay_out[name] = off
-        off += (nbytes + 255) // 256 * 256
-    return lay_in, size_in, lay_out, off
+        off += _align(nbytes)
+    return lay_in, size_in, work, lay_out, off
 
 
-def footprint_bytes(n: int, K: int, ntsave: int, nE: int = 0) -> int:
-    """Device bytes one request holds while in flight."""
-    _, a, _, b = _layout(n, K, ntsave, nE)
-    return a + b
+def footprint_bytes(n: int, K: int, ntsave: int, nE: int = 0, nlog: int = 0) -> int:
+    """Device bytes one request holds while in flight (inputs, state and scratch,
+    and its outputs when they are on the device)."""
+    _, size_in, work, _, size_out = _layout(n, K, ntsave, nE, nlog)
+    return size_in + work + size_out
+
+
+# Requests with a padded dimension above this run on the grid-wide Tsit5 (every
+# stage spread over many CUs) instead of one workgroup per request.
+GRID_MIN_NP = 96
 
 
 class _Launch:
-    def __init__(self, reqs, stream, event, tensors, outs, nbytes, t_launch, ready=None):
+    """One launch of packed requests: its device and pinned buffers, its requests
+    and which of them have been answered."""
+
+    def __init__(self, reqs, stream, tensors, outs, nbytes, t_launch, host_out):
         self.reqs = reqs
         self.stream = stream
-        self.event = event
         self.tensors = tensors
         self.outs = outs
         self.nbytes = nbytes
         self.t_launch = t_launch
-        self.ready = ready  # set by the worker thread of a grid-wide launch
+        self.host_out = host_out  # numpy view of the pinned outputs (persistent launches)
+        self.answered = [False] * len(reqs)
+        self.event = None  # completion event (recorded after the kernel)
+        self.ev0 = None  # timing event recorded before the launch
+        self.ready = None  # set by the worker thread of a grid-wide launch
         self.error = None
 
     def done(self) -> bool:
@@ -120,81 +146,102 @@ class _Launch:
         return self.event is not None and self.event.query()
 
 
-# Requests with a padded dimension above this run on the grid-wide Tsit5 (every
-# stage spread over many CUs) instead of one workgroup per request.
-GRID_MIN_NP = 96
+def _pack_host(reqs: list, base: int, staging: np.ndarray, descs: np.ndarray, lays, in_off, out_base: int, out_off,
+               scratch_base: int, scratch_off, signal: bool) -> None:
+    """Fill the staging buffer (inputs) and the descriptors of ``reqs``; ``base`` /
+    ``out_base`` / ``scratch_base`` are the device addresses the staged inputs, the
+    outputs and the scratch will have."""
+    for b, r in enumerate(reqs):
+        n, K, nt, nE = r.n, len(r.Ls), len(r.tsave), len(r.exp_ops)
+        npd = pad16(n)
+        nn = npd * npd
+        li, _, _, lo, _ = lays[b]
+        ib, ob = int(in_off[b]), int(out_off[b])
+
+        def cview(a, count=1):
+            return staging[ib + a : ib + a + count * nn * 8].view(np.complex64).reshape(count, npd, npd)
+
+        H = np.asarray(r.H, dtype=np.complex128)
+        Ls = [np.asarray(L, dtype=np.complex128) for L in r.Ls]
+        Hnh = -1j * H
+        for L in Ls:
+            Hnh = Hnh - 0.5 * (L.conj().T @ L)
+        cview(li['H'])[0, :n, :n] = Hnh
+        if K:
+            cview(li['D'], K)[:, :n, :n] = np.stack(Ls)
+            cview(li['C'], K)[:, :n, :n] = 0.5 * np.conj(np.swapaxes(np.stack(Ls), -1, -2))
+        if nE:
+            cview(li['E'], nE)[:, :n, :n] = np.swapaxes(np.stack([np.asarray(E) for E in r.exp_ops]), -1, -2)
+        rho = np.asarray(r.rho0, dtype=np.complex128)
+        if rho.shape[-1] == 1:
+            rho = rho @ rho.conj().T
+        a = ib + li['y']
+        staging[a : a + nn * 16].view(np.complex128).reshape(npd, npd)[:n, :n] = rho
+        a = ib + li['ts']
+        staging[a : a + nt * 8].view(np.float64)[:] = np.asarray(r.tsave, dtype=np.float64)
+        d = descs[b]
+        d['n'], d['np'], d['nL'], d['nE'], d['ntsave'] = n, npd, K, nE, nt
+        d['flags'] = _native.FLAG_SAVE_STATES | _native.FLAG_HAS_HNH | (_native.FLAG_SIGNAL if signal else 0)
+        d['rtol'], d['atol'], d['safety'], d['factormin'], d['factormax'] = r.rtol, r.atol, 0.9, 0.2, 5.0
+        d['max_steps'] = r.max_steps
+        d['t0'], d['t1'] = float(r.tsave[0]), float(r.tsave[-1])
+        d['tsave'] = base + ib + li['ts']
+        d['Hnh'], d['Dops'] = base + ib + li['H'], base + ib + li['D']
+        d['Cops'], d['EopsT'] = base + ib + li['C'], base + ib + li['E']
+        d['y'], d['work'] = base + ib + li['y'], scratch_base + int(scratch_off[b])
+        d['states'], d['final_state'] = out_base + ob + lo['st'], out_base + ob + lo['fin']
+        d['expects'], d['stats'] = out_base + ob + lo['ex'], out_base + ob + lo['stats']
+        if r.step_log_cap > 0:
+            d['step_log'], d['step_log_cap'] = out_base + ob + lo['log'], r.step_log_cap
 
 
 def pack_and_launch(reqs: list, stream, grid: bool = False):
-    """Pack uneven requests into one descriptor array and launch the persistent
-    Tsit5 kernel on ``stream`` (``grid``: return a callable that runs the
-    grid-wide Tsit5 on them instead; requests of one grid launch share their padded
-    dimension). Returns (tensors kept alive, per-request output layout, completion
-    event or the callable, device bytes held)."""
+    """Pack uneven requests and launch them on ``stream``: the persistent Tsit5 kernel
+    with outputs in pinned host memory and per-request completion words, or (``grid``)
+    a callable that runs the grid-wide Tsit5 on them with device outputs (requests of
+    one grid launch share their padded dimension). Returns (tensors kept alive,
+    per-request output layout, pinned output view or None, completion event or the
+    callable, device bytes held)."""
     import torch
 
     dev = torch.device('cuda', torch.cuda.current_device())
     B = len(reqs)
-    descs = np.zeros(B, dtype=_native.ME_INST_DTYPE)
-    lays = [_layout(r.n, len(r.Ls), len(r.tsave), len(r.exp_ops)) for r in reqs]
+    lays = [_layout(r.n, len(r.Ls), len(r.tsave), len(r.exp_ops), r.step_log_cap) for r in reqs]
     in_off = np.cumsum([0] + [lay[1] for lay in lays])
-    out_off = np.cumsum([0] + [lay[3] for lay in lays])
-    with torch.cuda.stream(stream):
-        arena = torch.zeros(int(in_off[-1]), dtype=torch.uint8, device=dev)
+    w_off = np.cumsum([0] + [lay[2] for lay in lays])
+    out_off = np.cumsum([0] + [lay[4] for lay in lays])
+    dsz = B * _native.ME_INST_DTYPE.itemsize
+    n_in = int(in_off[-1])
+    # staging: every request's inputs, then the descriptor array: one copy
+    staging_t = torch.zeros(n_in + _align(dsz), dtype=torch.uint8, pin_memory=True)
+    staging = staging_t.numpy()
+    arena = torch.empty(n_in + _align(dsz), dtype=torch.uint8, device=dev)
+    scratch = torch.empty(max(int(w_off[-1]), 256), dtype=torch.uint8, device=dev)
+    descs = staging[n_in : n_in + dsz].view(_native.ME_INST_DTYPE)
+    lib = _native.lib()
+    c_p = _native.ctypes.c_void_p
+    if grid:
         oarena = torch.zeros(int(out_off[-1]), dtype=torch.uint8, device=dev)
-        base, obase = arena.data_ptr(), oarena.data_ptr()
-
-        def cview(a, npd):
-            return arena[a : a + npd * npd * 8].view(torch.complex64).view(npd, npd)
-
-        outs = []
-        for b, r in enumerate(reqs):
-            n, K, nt, nE = r.n, len(r.Ls), len(r.tsave), len(r.exp_ops)
-            npd = pad16(n)
-            nn = npd * npd
-            li, _, lo, _ = lays[b]
-            ib, ob = int(in_off[b]), int(out_off[b])
-            H = torch.from_numpy(np.ascontiguousarray(r.H, dtype=np.complex128)).to(dev)
-            Ls = [torch.from_numpy(np.ascontiguousarray(L, dtype=np.complex128)).to(dev) for L in r.Ls]
-            Hnh = -1j * H
-            for L in Ls:
-                Hnh = Hnh - 0.5 * (L.conj().T @ L)
-            cview(ib + li['H'], npd)[:n, :n] = Hnh.to(torch.complex64)
-            for k, L in enumerate(Ls):
-                cview(ib + li['D'] + k * nn * 8, npd)[:n, :n] = L.to(torch.complex64)
-                cview(ib + li['C'] + k * nn * 8, npd)[:n, :n] = (0.5 * L.conj().T).to(torch.complex64)
-            for q, E in enumerate(r.exp_ops):
-                ET = torch.from_numpy(np.ascontiguousarray(np.asarray(E).T, dtype=np.complex64)).to(dev)
-                cview(ib + li['E'] + q * nn * 8, npd)[:n, :n] = ET
-            rho = np.asarray(r.rho0, dtype=np.complex128)
-            if rho.shape[-1] == 1:
-                rho = rho @ rho.conj().T
-            a = ib + li['y']
-            arena[a : a + nn * 16].view(torch.complex128).view(npd, npd)[:n, :n] = torch.from_numpy(rho).to(dev)
-            a = ib + li['ts']
-            arena[a : a + nt * 8].view(torch.float64).copy_(torch.from_numpy(np.asarray(r.tsave, dtype=np.float64)))
-            d = descs[b]
-            d['n'], d['np'], d['nL'], d['nE'], d['ntsave'] = n, npd, K, nE, nt
-            d['flags'] = _native.FLAG_SAVE_STATES | _native.FLAG_HAS_HNH
-            d['rtol'], d['atol'], d['safety'], d['factormin'], d['factormax'] = r.rtol, r.atol, 0.9, 0.2, 5.0
-            d['max_steps'] = r.max_steps
-            d['t0'], d['t1'] = float(r.tsave[0]), float(r.tsave[-1])
-            d['tsave'] = base + ib + li['ts']
-            d['Hnh'], d['Dops'] = base + ib + li['H'], base + ib + li['D']
-            d['Cops'], d['EopsT'] = base + ib + li['C'], base + ib + li['E']
-            d['y'], d['work'] = base + ib + li['y'], base + ib + li['w']
-            d['states'], d['final_state'] = obase + ob + lo['st'], obase + ob + lo['fin']
-            d['expects'], d['stats'] = obase + ob + lo['ex'], obase + ob + lo['stats']
-            outs.append((ob + lo['st'], ob + lo['ex'], ob + lo['stats'], n, nt, nE))
-        d_descs = torch.from_numpy(descs.view(np.uint8).copy()).to(dev)
-        lib = _native.lib()
-        c_p = _native.ctypes.c_void_p
+        out_base, host_out = oarena.data_ptr(), None
+    else:
+        oarena = torch.zeros(int(out_off[-1]), dtype=torch.uint8, pin_memory=True)
+        out_base, host_out = _native.host_device_pointer(oarena.data_ptr()), oarena.numpy()
+    _pack_host(reqs, arena.data_ptr(), staging, descs, lays, in_off, out_base, out_off, scratch.data_ptr(), w_off,
+               signal=not grid)
+    npd = int(descs['np'][0])
+    if grid and np.any(descs['np'] != npd):
+        raise ValueError('a grid-wide launch needs requests of one padded dimension')
+    h_descs = descs.copy()
+    d_descs_ptr = arena.data_ptr() + n_in
+    outs = [(int(out_off[b]) + lays[b][3]['st'], int(out_off[b]) + lays[b][3]['ex'],
+             int(out_off[b]) + lays[b][3]['stats'], r.n, len(r.tsave), len(r.exp_ops),
+             int(out_off[b]) + lays[b][3]['log'], r.step_log_cap) for b, r in enumerate(reqs)]
+    nbytes = int(arena.numel() + scratch.numel() + (oarena.numel() if grid else 0))
+    with torch.cuda.stream(stream):
+        arena.copy_(staging_t, non_blocking=True)
         if grid:
-            npd = int(descs['np'][0])
-            if np.any(descs['np'] != npd):
-                raise ValueError('a grid-wide launch needs requests of one padded dimension')
             nb = int(lib.dqh_tsit5_grid_scratch_bytes(npd, B))
-            scratch = torch.empty((nb,), dtype=torch.uint8, device=dev)
+            gscratch = torch.empty((nb,), dtype=torch.uint8, device=dev)
             flags = torch.zeros((2,), dtype=torch.int64, pin_memory=True)
 
             def run():
@@ -202,40 +249,66 @@ def pack_and_launch(reqs: list, stream, grid: bool = False):
                 with torch.cuda.stream(stream):
                     _native.check(
                         lib.dqh_mesolve_tsit5_dense_grid(
-                            c_p(d_descs.data_ptr()), descs.ctypes.data_as(c_p), B, c_p(scratch.data_ptr()), nb, 16,
+                            c_p(d_descs_ptr), h_descs.ctypes.data_as(c_p), B, c_p(gscratch.data_ptr()), nb, 16,
                             c_p(flags.data_ptr()), c_p(stream.cuda_stream),
                         ),
                         'dqh_mesolve_tsit5_dense_grid',
                     )
-                    ev = torch.cuda.Event()
+                    ev = torch.cuda.Event(enable_timing=True)
                     ev.record(stream)
                 return ev
 
-            return [oarena, arena, d_descs, scratch, flags, descs], outs, run, int(in_off[-1] + out_off[-1] + nb)
-        _native.check(
-            lib.dqh_mesolve_tsit5_dense(c_p(d_descs.data_ptr()), B, c_p(stream.cuda_stream)),
-            'dqh_mesolve_tsit5_dense',
-        )
-        ev = torch.cuda.Event()
+            keep = [oarena, arena, scratch, staging_t, gscratch, flags, h_descs]
+            return keep, outs, None, run, nbytes + nb
+        _native.check(lib.dqh_mesolve_tsit5_dense(c_p(d_descs_ptr), B, c_p(stream.cuda_stream)),
+                      'dqh_mesolve_tsit5_dense')
+        ev = torch.cuda.Event(enable_timing=True)
         ev.record(stream)
-    return [oarena, arena, d_descs], outs, ev, int(in_off[-1] + out_off[-1])
+    return [oarena, arena, scratch, staging_t], outs, host_out, ev, nbytes
+
+
+def _read_outputs(host: np.ndarray, out) -> tuple:
+    """(states, expects, nsteps, naccepted, nrejected, status, step_ends) of one
+    request from its output region (a numpy byte array on the host)."""
+    a_st, a_ex, a_stats, n, nt, nE, a_log, nlog = out
+    st = host[a_st : a_st + nt * n * n * 8].view(np.complex64).reshape(nt, n, n).copy()
+    ex = host[a_ex : a_ex + nE * nt * 8].view(np.complex64).reshape(nE, nt).copy() if nE else None
+    stats = host[a_stats : a_stats + 64].view(np.int64)
+    log = host[a_log : a_log + nlog * 8].view(np.float64)[: min(int(stats[1]), nlog)].copy() if nlog else None
+    return st, ex, int(stats[0]), int(stats[1]), int(stats[2]), int(stats[3]), log
 
 
 def collect(oarena, outs) -> list:
-    """Copy the outputs of a finished launch to host numpy arrays."""
-    host = oarena.cpu().numpy()
-    res = []
-    for a_st, a_ex, a_stats, n, nt, nE in outs:
-        st = host[a_st : a_st + nt * n * n * 8].view(np.complex64).reshape(nt, n, n).copy()
-        ex = host[a_ex : a_ex + nE * nt * 8].view(np.complex64).reshape(nE, nt).copy() if nE else None
-        stats = host[a_stats : a_stats + 64].view(np.int64)
-        res.append((st, ex, int(stats[0]), int(stats[1]), int(stats[2]), int(stats[3])))
-    return res
+    """Outputs of a finished launch as host numpy arrays."""
+    host = oarena.numpy() if oarena.device.type == 'cpu' else oarena.cpu().numpy()
+    return [_read_outputs(host, o) for o in outs]
+
+
+def status_error(status: int, nsteps: int, naccepted: int, nrejected: int) -> Exception:
+    """The exception a request answers with when its solve stopped early (the same
+    messages as ``dq.mesolve``: integrators/_utils.py:66-72 for max_steps)."""
+    if status == _native.STATUS_MAX_STEPS:
+        return RuntimeError('The maximum number of method steps has been reached.')
+    if status == _native.STATUS_NONFINITE:
+        return RuntimeError(f'solve stopped: non-finite error estimate or vanishing step size (after {nsteps} steps, '
+                            f'{naccepted} accepted, {nrejected} rejected)')
+    return RuntimeError(f'solve rejected by the kernel with status {status}')
+
+
+def _request_bytes(r) -> int:
+    """Device bytes a request holds while in flight, with its share of the grid-wide
+    Tsit5 scratch when it runs grid-wide."""
+    nb = footprint_bytes(r.n, len(r.Ls), len(r.tsave), len(r.exp_ops), r.step_log_cap)
+    npd = pad16(r.n)
+    if npd > GRID_MIN_NP:
+        nb += int(_native.lib().dqh_tsit5_grid_scratch_bytes(npd, 1))
+    return nb
 
 
 def _size_class(r) -> int:
-    """Launch grouping: small requests by size band (a persistent launch lasts as
-    long as its slowest request), grid-wide requests by exact padded dimension."""
+    """Launch grouping: small requests by size band (a persistent launch holds its
+    memory until its slowest request is done), grid-wide requests by exact padded
+    dimension."""
     npd = pad16(r.n)
     if npd > GRID_MIN_NP:
         return 1000 + npd
@@ -268,12 +341,19 @@ class Server:
         self._used = 0
         self._stop = threading.Event()
         self._next_stream = 0
-        self.stats = {'launches': 0, 'requests': 0, 'max_inflight_bytes': 0, 'max_concurrent_launches': 0}
+        self.stats = {'launches': 0, 'requests': 0, 'max_inflight_bytes': 0, 'max_concurrent_launches': 0,
+                      'answered_before_launch_end': 0}
+        # device-busy accounting: a timing event before and after every launch,
+        # measured from a reference event recorded now
+        self._t_ref = torch.cuda.Event(enable_timing=True)
+        self._t_ref.record(torch.cuda.current_stream())
+        self._busy = []
         self._thread = threading.Thread(target=self._loop, name='dqh-server', daemon=True)
         self._thread.start()
 
     # ---------------------------------------------------------------- client
-    def submit(self, H, Ls, rho0, tsave, *, rtol=1e-6, atol=1e-6, max_steps=100_000, exp_ops=None) -> Future:
+    def submit(self, H, Ls, rho0, tsave, *, rtol=1e-6, atol=1e-6, max_steps=100_000, exp_ops=None,
+               step_log_cap: int = 0) -> Future:
         H = np.asarray(H)
         n = H.shape[-1]
         if H.shape != (n, n) or pad16(n) > MAX_NP:
@@ -282,8 +362,9 @@ class Server:
         if tsave.ndim != 1 or len(tsave) == 0 or np.any(np.diff(tsave) < 0):
             raise ValueError('Argument tsave must be sorted in strictly ascending order.')
         req = Request(H=H, Ls=[np.asarray(L) for L in Ls], rho0=np.asarray(rho0), tsave=tsave, rtol=rtol, atol=atol,
-                      max_steps=max_steps, exp_ops=[np.asarray(E) for E in (exp_ops or [])])
-        if footprint_bytes(n, len(req.Ls), len(tsave), len(req.exp_ops)) > self.memory_budget:
+                      max_steps=max_steps, exp_ops=[np.asarray(E) for E in (exp_ops or [])],
+                      step_log_cap=int(step_log_cap))
+        if _request_bytes(req) > self.memory_budget:
             raise ValueError('request does not fit in the server memory budget')
         req.future = Future()
         req.t_submit = time.perf_counter()
@@ -293,6 +374,24 @@ class Server:
     def close(self) -> None:
         self._stop.set()
         self._thread.join()
+        self.stats['busy_s'] = self.busy_seconds()
+
+    def busy_seconds(self) -> float:
+        """Time during which at least one launch of this server was on the device
+        (union of the [start, end] intervals of the retired launches, HIP event
+        timestamps)."""
+        iv = sorted((self._t_ref.elapsed_time(a), self._t_ref.elapsed_time(b)) for a, b in self._busy)
+        total, cur0, cur1 = 0.0, None, None
+        for a, b in iv:
+            if cur1 is None or a > cur1:
+                if cur1 is not None:
+                    total += cur1 - cur0
+                cur0, cur1 = a, b
+            else:
+                cur1 = max(cur1, b)
+        if cur1 is not None:
+            total += cur1 - cur0
+        return 1e-3 * total
 
     def __enter__(self):
         return self
@@ -304,15 +403,15 @@ class Server:
     def _admit(self) -> None:
         if not self._pending or len(self._inflight) >= self.max_inflight:
             return
-        # a launch lasts as long as its slowest request: fill it with requests of
-        # the same size class as the oldest pending one (FIFO inside the class)
+        # fill a launch with requests of the same size class as the oldest pending one
+        # (FIFO inside the class)
         cls = _size_class(self._pending[0])
         batch, nbytes, keep = [], 0, []
         for r in self._pending:
             if len(batch) >= self.max_batch or _size_class(r) != cls:
                 keep.append(r)
                 continue
-            fb = footprint_bytes(r.n, len(r.Ls), len(r.tsave), len(r.exp_ops))
+            fb = _request_bytes(r)
             if self._used + nbytes + fb > self.memory_budget:
                 keep.append(r)
                 continue
@@ -321,19 +420,25 @@ class Server:
         self._pending = keep
         if not batch:
             return
+        import torch
+
         stream = self.streams[self._next_stream % len(self.streams)]
         self._next_stream += 1
         t0 = time.perf_counter()
         grid = cls >= 1000
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
         try:
-            tensors, outs, ev, total = pack_and_launch(batch, stream, grid=grid)
+            tensors, outs, host_out, ev, total = pack_and_launch(batch, stream, grid=grid)
         except Exception as e:  # answer the batch with the error, keep serving
             for r in batch:
                 r.future.set_exception(e)
             return
         self._used += total
+        launch = _Launch(batch, stream, tensors, outs, total, t0, host_out)
+        launch.ev0 = ev0
         if grid:
-            launch = _Launch(batch, stream, None, tensors, outs, total, t0, ready=threading.Event())
+            launch.ready = threading.Event()
 
             def work(launch=launch, run=ev):
                 try:
@@ -344,35 +449,59 @@ class Server:
                     launch.ready.set()
 
             threading.Thread(target=work, name='dqh-grid', daemon=True).start()
-            self._inflight.append(launch)
         else:
-            self._inflight.append(_Launch(batch, stream, ev, tensors, outs, total, t0))
+            launch.event = ev
+        self._inflight.append(launch)
         self.stats['launches'] += 1
         self.stats['max_inflight_bytes'] = max(self.stats['max_inflight_bytes'], self._used)
         self.stats['max_concurrent_launches'] = max(self.stats['max_concurrent_launches'], len(self._inflight))
 
+    def _answer(self, L: _Launch, b: int, res, t: float) -> None:
+        r = L.reqs[b]
+        st, ex, ns, na, nr, status, log = res
+        L.answered[b] = True
+        self.stats['requests'] += 1
+        if status != 0:
+            r.future.set_exception(status_error(status, ns, na, nr))
+            return
+        r.future.set_result(Response(st, ex, ns, na, nr, status, t - r.t_submit, L.t_launch - r.t_submit,
+                                     step_ends=log))
+
     def _retire(self) -> bool:
-        done = [L for L in self._inflight if (L.ready is not None and L.ready.is_set() and L.error) or L.done()]
-        for L in done:
-            self._inflight.remove(L)
-            try:
-                if L.error is not None:
-                    raise L.error
-                res = collect(L.tensors[0], L.outs)
-            except Exception as e:
-                for r in L.reqs:
-                    r.future.set_exception(e)
-                self._used -= L.nbytes
+        progress = False
+        for L in list(self._inflight):
+            if L.host_out is not None:
+                # per request: stats[7] is set by the kernel once that request is done
+                for b, out in enumerate(L.outs):
+                    if not L.answered[b] and L.host_out[out[2] + 56 : out[2] + 64].view(np.int64)[0] == 1:
+                        self._answer(L, b, _read_outputs(L.host_out, out), time.perf_counter())
+                        progress = True
+                        if not L.done():
+                            self.stats['answered_before_launch_end'] += 1
+            finished = (L.ready is not None and L.ready.is_set() and L.error is not None) or L.done()
+            if not finished:
                 continue
-            t = time.perf_counter()
-            for r, (st, ex, ns, na, nr, status) in zip(L.reqs, res):
-                self.stats['requests'] += 1
-                if status == -3:
-                    r.future.set_exception(RuntimeError('The maximum number of method steps has been reached.'))
-                    continue
-                r.future.set_result(Response(st, ex, ns, na, nr, status, t - r.t_submit, L.t_launch - r.t_submit))
+            self._inflight.remove(L)
+            progress = True
+            if L.event is not None:
+                self._busy.append((L.ev0, L.event))
+            rest = [b for b in range(len(L.reqs)) if not L.answered[b]]
+            if rest:
+                try:
+                    if L.error is not None:
+                        raise L.error
+                    res = collect(L.tensors[0], [L.outs[b] for b in rest])
+                except Exception as e:
+                    for b in rest:
+                        L.answered[b] = True
+                        L.reqs[b].future.set_exception(e)
+                    res = None
+                if res is not None:
+                    t = time.perf_counter()
+                    for b, rb in zip(rest, res):
+                        self._answer(L, b, rb, t)
             self._used -= L.nbytes
-        return bool(done)
+        return progress
 
     def _loop(self) -> None:
         while not (self._stop.is_set() and self._q.empty() and not self._pending and not self._inflight):
@@ -386,4 +515,4 @@ class Server:
             retired = self._retire()
             self._admit()
             if not (got or retired):
-                time.sleep(0.0005)
+                time.sleep(0.0002)

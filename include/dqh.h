@@ -104,6 +104,12 @@ typedef struct dqh_me_inst {
  * -1/2 sum L^dag L is among the nmod Hamiltonian terms. Such instances always run on
  * the global-memory kernel form (the uniform entry point reports DQH_UNSUPPORTED). */
 #define DQH_FLAG_LMOD 32
+/* Tsit5 kernels (dqh_mesolve_tsit5_dense, _uniform): when an instance is done, its
+ * workgroup makes its outputs visible at system scope and then stores 1 to stats[7]
+ * (release, system scope). A host that places states / expects / final_state /
+ * stats in pinned host memory (device address from dqh_host_device_pointer) can
+ * poll stats[7] and take each instance's results while the launch still runs. */
+#define DQH_FLAG_SIGNAL 64
 
 /* Size checks for FFI mirrors. */
 int64_t dqh_sizeof_me_inst(void);
@@ -160,8 +166,8 @@ int dqh_mesolve_tsit5_dense_grid(const dqh_me_inst *d_insts, const dqh_me_inst *
 #define DQH_SWEEP_TILED_H8 3
 #define DQH_SWEEP_TILED_H16 4
 #define DQH_SWEEP_TILED_H32 5
-/* Rouchon1 Kraus sweep only: two passes, dY = dM X then the rest with dY read along
- * rows, so each output reads few neighbours (no dM X dM^dag double sum). */
+/* Accepted for the Rouchon1 Kraus sweep and run as its GLOBAL form (the fused
+ * one-pass sweep reads the normalised state X once per neighbour). */
 #define DQH_SWEEP_SPLIT 6
 
 #define DQH_DIA_MAXOPS 8
@@ -196,8 +202,8 @@ typedef struct dqh_dia_rouchon1 {
   uint64_t Wn[2];  /* device c64[n/nb][nb][nb]: -V_i T_{i+1,i}^dag */
   dqh_dia_ops ops[2];
   uint64_t rho64;      /* device double2[ninst][n][n]: rho0 on entry, final state on exit */
-  uint64_t rho32;      /* device c64[2][ninst][n][n] scratch */
-  uint64_t work;       /* device c64[2][ninst][n][n] scratch */
+  uint64_t rho32;      /* device c64[2][ninst][n][n] scratch: the state as hi + lo planes */
+  uint64_t work;       /* device c64[2][ninst][n][n] scratch: the stages X and E */
   uint64_t states;     /* device c64[ninst][nsave][n_true][n_true] */
   uint64_t save_step;  /* HOST int64[nsave]: step index whose end state is saved (-1: initial) */
   uint64_t save_theta; /* HOST double[nsave]: interpolation weight inside that step */
@@ -206,6 +212,9 @@ typedef struct dqh_dia_rouchon1 {
   uint64_t expects;    /* device c64[ninst][nE][nsave]: Tr(E_k rho(tsave))            */
   int64_t variant;     /* DQH_SWEEP_*: form of the Kraus sweep                        */
   int64_t solve_panel; /* columns per workgroup of the block solves: 0 (auto), 32, 64 */
+  uint64_t timing;     /* optional HOST double[2] (0: off): mean ms per step of the block
+                          solves and of the Kraus sweep, from HIP events around steps
+                          1..8; the call then synchronises the stream before returning */
 } dqh_dia_rouchon1;
 
 int64_t dqh_sizeof_dia_rouchon1(void);
@@ -213,8 +222,9 @@ int64_t dqh_sizeof_dia_rouchon1(void);
 /* Fixed-step Rouchon1 with Cholesky normalisation for DIA operators
  * (replaces MESolveFixedRouchon1Integrator, rouchon_integrator.py:385-432, for
  * large sparse problems): per step two banded block solves on MFMA and one
- * bandwidth-bound stencil sweep, FP64 state with FP32 increments. Launches on
- * ``stream`` and returns without synchronising. */
+ * bandwidth-bound stencil sweep; the state is held as two FP32 planes
+ * (hi + lo) updated with FP32 increments in FP64 arithmetic. Launches on
+ * ``stream`` and returns without synchronising (unless ``timing`` is set). */
 int dqh_mesolve_rouchon1_dia(const dqh_dia_rouchon1 *problem, void *stream);
 
 /* A batch of Lindblad vector fields for DIA operators shared by the batch:
@@ -334,6 +344,11 @@ int dqh_adjoint_pwc_quadrature_dia(int64_t ninst, int64_t n, int64_t nnodes, int
 int dqh_adjoint_rate_quadrature(int64_t ninst, int64_t n, int64_t nnodes, int64_t nrates, int64_t nseg,
                                 const void *rho_nodes, const void *lam_nodes, const void *Aops, const void *Nops,
                                 const double *weights, const int32_t *seg_first, double *grad, void *stream);
+
+/* Device address of pinned host memory (hipHostGetDevicePointer): descriptors that
+ * point outputs at host memory use it. Returns DQH_HIP_ERROR when ``host`` is not
+ * pinned, mapped memory. No counterpart in the reference (serving plumbing). */
+int dqh_host_device_pointer(void *host, void **device);
 
 /* Message of the last failing call on this thread. */
 const char *dqh_last_error(void);

@@ -61,3 +61,74 @@ def test_continuous_admission_under_memory_budget():
     assert stats['launches'] >= 6
     assert stats['max_inflight_bytes'] <= budget
     assert all(r.status == 0 for r in res)
+
+
+def _serving_request(rng, n, K, T, nt=50):
+    """A request of the serving mix: GUE H of spectral norm 1, K jump operators
+    sqrt(gamma) G with G complex Gaussian of entry variance 1/n, random pure state."""
+    A = (rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))) / np.sqrt(2)
+    H = (A + A.conj().T) / 2
+    H /= np.max(np.abs(np.linalg.eigvalsh(H)))
+    Ls = [np.sqrt(rng.uniform(0.01, 0.5)) * (rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))) / np.sqrt(2 * n)
+          for _ in range(K)]
+    psi = rng.normal(size=(n, 1)) + 1j * rng.normal(size=(n, 1))
+    psi /= np.linalg.norm(psi)
+    return H, Ls, psi, np.linspace(0.0, T, nt)
+
+
+def test_serving_mix_full_range_vs_oracle_replay():
+    """The serving workload's range: n from 8 to 256 (the largest on the grid-wide
+    Tsit5), T up to 20, 50 save times, rtol 1e-5. Every request matches the complex128
+    oracle replaying its own accepted steps to 1e-5, and its solve alone."""
+    from dynamiqs_amd.serving import Server
+    from oracle import tsit5
+
+    rng = np.random.default_rng(21)
+    shapes = [(8, 1, 20.0), (24, 3, 15.0), (44, 2, 20.0), (64, 1, 12.0), (90, 3, 8.0), (128, 2, 5.0), (200, 1, 3.0),
+              (256, 3, 2.0)]
+    reqs = [_serving_request(rng, n, K, T) for n, K, T in shapes]
+    with Server(memory_budget=4 << 30) as srv:
+        futs = [srv.submit(H, Ls, psi, ts, rtol=1e-5, step_log_cap=20_000) for (H, Ls, psi, ts) in reqs]
+        mix = [f.result(timeout=300) for f in futs]
+    for i, ((H, Ls, psi, ts), m) in enumerate(zip(reqs, mix)):
+        print(f'serving request n={H.shape[0]} K={len(Ls)} T={ts[-1]}: {m.nsteps} steps', flush=True)
+        assert m.status == 0 and m.step_ends is not None and len(m.step_ends) == m.naccepted
+        LdL = sum(L.conj().T @ L for L in Ls)
+        Hnh = -1j * H - 0.5 * LdL
+        Cs = [0.5 * L.conj().T for L in Ls]
+
+        def f(t, y, tseg, Hnh=Hnh, Ls=Ls, Cs=Cs):
+            tmp = Hnh @ y
+            for L, C in zip(Ls, Cs):
+                tmp = tmp + (L @ y) @ C
+            return tmp + tmp.conj().T
+
+        saved, _ = tsit5.integrate_steps(f, psi @ psi.conj().T, ts, m.step_ends)
+        err = np.linalg.norm(m.states - np.stack(saved), axis=(-2, -1)).max()
+        assert err <= TOL, (i, err)
+    with Server(memory_budget=4 << 30) as srv:
+        for i in (0, 4, 7):
+            H, Ls, psi, ts = reqs[i]
+            a = srv.submit(H, Ls, psi, ts, rtol=1e-5).result(timeout=300)
+            np.testing.assert_array_equal(a.states, mix[i].states)
+
+
+def test_requests_retire_before_their_launch_ends():
+    """A short request packed into one launch with a long one is answered when its
+    own solve is done (its completion word), long before the launch ends."""
+    from dynamiqs_amd.serving import Server
+
+    rng = np.random.default_rng(8)
+    short = _serving_request(rng, 40, 1, 0.5, nt=5)
+    long_ = _serving_request(rng, 64, 3, 20.0)
+    with Server(memory_budget=1 << 30) as srv:
+        f_long = srv.submit(*long_, rtol=1e-7, atol=1e-9)
+        f_short = srv.submit(*short, rtol=1e-5)
+        r_short = f_short.result(timeout=120)
+        r_long = f_long.result(timeout=120)
+        stats = dict(srv.stats)
+    print(f'short {r_short.latency_s * 1e3:.2f} ms ({r_short.nsteps} steps), long {r_long.latency_s * 1e3:.2f} ms '
+          f'({r_long.nsteps} steps), launches {stats["launches"]}', flush=True)
+    if stats['launches'] == 1:  # both in one launch (the usual case): the short one left early
+        assert stats['answered_before_launch_end'] >= 1
+    assert r_short.latency_s < 0.5 * r_long.latency_s
