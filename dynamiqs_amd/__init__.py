@@ -4,8 +4,9 @@
 ``mesolve`` hot path uses: ``dq.mesolve``, ``dq.sesolve`` and the propagators
 ``dq.sepropagator`` / ``dq.mepropagator``, quantum arrays
 (dense and sparse DIA), time-dependent operators (``constant``, ``pwc``,
-``modulated``), operator/state builders, ``Options``, the methods
-``Tsit5`` / ``Dopri5`` / ``Rouchon1`` / ``Rouchon2`` and the result classes.
+``modulated``, ``timecallable``), operator/state builders, ``Options``, the
+methods ``Tsit5`` / ``Dopri5`` / ``Euler`` / ``Rouchon1`` / ``Rouchon2`` /
+``Rouchon3`` / ``Expm`` and the result classes.
 The solve runs in hand-written HIP kernels (``libdqh.so``, C-ABI in
 ``include/dqh.h``) loaded with ctypes; there is no CPU fallback.
 """

@@ -1,10 +1,25 @@
 """Gradient selectors, mirroring ``dynamiqs/gradient.py:10-110``.
 
-On the GPU path a gradient request is served by the hand-written adjoint
-(``dynamiqs_amd.adjoint``) instead of diffrax's checkpointed reverse mode.
+They are accepted and validated against the method as in the reference
+(``method.assert_supports_gradient``), but the solvers of this package are not
+differentiated through an autodiff framework (there is no JAX): passing one to
+``mesolve`` / ``sesolve`` / the propagators raises ``NotImplementedError`` after
+that validation. Gradients come from the hand-written continuous adjoint,
+``dynamiqs_amd.adjoint.mesolve_adjoint`` and ``AdjointPlan``.
 """
 
 from __future__ import annotations
+
+
+def reject_gradient(gradient, solver: str) -> None:
+    """Raise for a gradient request the solvers cannot serve (after the reference's
+    method / gradient compatibility check)."""
+    if gradient is not None:
+        raise NotImplementedError(
+            f'`gradient={gradient}` asks {solver} for autodiff gradients, which are not available here (no JAX): '
+            'use dynamiqs_amd.adjoint.mesolve_adjoint or dynamiqs_amd.adjoint.AdjointPlan for adjoint gradients '
+            'of mesolve losses.'
+        )
 
 
 class Gradient:

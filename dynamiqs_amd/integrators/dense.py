@@ -331,6 +331,10 @@ def _kraus_sets(H: np.ndarray, Ls, h: float, normalize: bool, order: int = 1):
     return D, C
 
 
+# device bytes the per-step Kraus sets (D and C) of one batch may take
+KRAUS_SET_BUDGET = 2 << 30
+
+
 def rouchon_step_times(t0: float, t1: float, dt: float) -> np.ndarray:
     """Boundaries ``t0 + k dt`` with the last step clipped to ``t1`` (diffrax
     ``ConstantStepSize`` with ``dt0 = method.dt``)."""
@@ -494,6 +498,8 @@ def build_plan(
         return cache[key]
 
     npwc_max = 0
+    kraus_index = {}  # ('kraus', key_H, key_L) -> step key -> set index, for sets already built
+    kraus_bytes = [0]  # device bytes of the Kraus sets built for this batch
     nmod_max = max((len(_hmods(inst)) for inst in instances), default=0)
     t_stop = float(tsave[-1])
     if instances[0].tsave is not None:
@@ -602,19 +608,36 @@ def build_plan(
             # smooth (modulated) terms make every step's operators distinct: one set per step
             lmods = list(inst.Lmod or [None] * K)
             smooth = bool(inst.mod) or any(lm is not None for lm in lmods)
-            if smooth and nsteps * nLdev * nn * 8 > (2 << 30):
-                raise NotImplementedError(
-                    f'{nsteps} steps of modulated operators need one Kraus set per step '
-                    f'({nsteps * nLdev * nn * 8 / 2**30:.1f} GiB); use a larger dt or an adaptive method'
-                )
+            key = ('kraus', inst.key_H, inst.key_L)
+            cacheable = not any(k is None for k in key[1:])
             keys = []
             for m in range(nsteps):
                 # Rouchon takes the operators at the step midpoint, Euler (order 0) at its start
                 tsel = step_t[m] if rouchon_order == 0 else mids[m]
                 seg = -1 if grid is None else int(np.searchsorted(grid, tsel, side='right')) - 1
                 keys.append((seg, float(hs[m]), float(tsel) if smooth else 0.0))
+            if cacheable and key in kraus_index:
+                # instances sharing key_H / key_L (a batched rho0, the Hermitian split,
+                # propagator basis solves) share the sets already on the device
+                index = kraus_index[key]
+                d['Dops'], d['Cops'] = cache[key + ('D',)], cache[key + ('C',)]
+                d['step_t'] = p_step_t
+                d['step_set'] = arena.put(np.asarray([index[k] for k in keys], dtype=np.int32))
+                d['max_steps'] = nsteps
+                continue
             uniq = sorted(set(keys))
             index = {k: i for i, k in enumerate(uniq)}
+            # device bytes of D and C sets together, over every distinct set built so far
+            set_bytes = 2 * len(uniq) * nLdev * nn * 8
+            if kraus_bytes[0] + set_bytes > KRAUS_SET_BUDGET:
+                raise NotImplementedError(
+                    f'{len(uniq)} Kraus sets of {nLdev} operators ({(kraus_bytes[0] + set_bytes) / 2**30:.1f} GiB with '
+                    f'the sets of the batch built so far) exceed {KRAUS_SET_BUDGET / 2**30:.0f} GiB: modulated '
+                    'operators need one set per step; use a larger dt, fewer instances per call or an adaptive method'
+                )
+            kraus_bytes[0] += set_bytes
+            if cacheable:
+                kraus_index[key] = index
             sets_D, sets_C = [], []
             for seg, h, tk in uniq:
                 tm = None
@@ -633,7 +656,6 @@ def build_plan(
                 Dk, Ck = _kraus_sets(Hm, Lm, h, normalize, rouchon_order)
                 sets_D.append(np.stack(Dk))
                 sets_C.append(np.stack(Ck))
-            key = ('kraus', inst.key_H, inst.key_L)
             d['Dops'] = shared(key + ('D',), lambda: _pad(np.stack(sets_D), npad).astype(np.complex64))
             d['Cops'] = shared(key + ('C',), lambda: _pad(np.stack(sets_C), npad).astype(np.complex64))
             d['step_t'] = p_step_t

@@ -18,7 +18,7 @@ from dataclasses import replace
 import numpy as np
 
 from .._checks import check_hermitian, check_qarray_is_dense, check_shape, check_times
-from ..gradient import Gradient
+from ..gradient import Gradient, reject_gradient
 from ..method import Dopri5, Euler, Expm, Method, Rouchon1, Rouchon2, Rouchon3, Tsit5, _DEAdaptiveStep
 from ..options import Options, check_options
 from ..qarrays import QArray, asqarray
@@ -169,9 +169,14 @@ def mesolve(
     r"""Solve the Lindblad master equation on the MI355X.
 
     Same signature, batching rules, errors and result as ``dq.mesolve``
-    (``dynamiqs/integrators/apis/mesolve.py:62-304``). Supported methods:
-    ``Tsit5`` and ``Rouchon1``; supported operators: constant (dense or DIA) and
-    piecewise-constant Hamiltonians (``dq.pwc``), constant jump operators.
+    (``dynamiqs/integrators/apis/mesolve.py:62-304``). Methods: ``Tsit5`` and
+    ``Dopri5`` (adaptive; with ``Options(vectorized=True)`` the Liouvillian as a
+    superoperator, n <= 32), ``Euler``, ``Rouchon1`` and the fixed-step ``Rouchon2``
+    / ``Rouchon3``, ``Expm``. Operators: constant (dense or DIA; constant DIA
+    operators from n = 128 on run on the DIA kernels), piecewise-constant
+    (``dq.pwc``), modulated and ``timecallable`` Hamiltonians, constant jump
+    operators or one modulated term ``f(t) L``. ``gradient`` is validated as in the
+    reference and then rejected (no autodiff here; see ``dynamiqs_amd.adjoint``).
     """
     # === convert arguments
     H = astimeqarray(H)
@@ -188,6 +193,7 @@ def mesolve(
     options = options.initialise()
     assert_method_supported(method, SUPPORTED_METHODS)
     method.assert_supports_gradient(gradient)
+    reject_gradient(gradient, '`dq.mesolve()`')
     if isinstance(method, (Rouchon2, Rouchon3)) and method.dt is None:
         raise NotImplementedError(
             f'the adaptive {type(method).__name__} (dt=None) is not supported by the MI355X solver: set `dt`'
@@ -201,6 +207,9 @@ def mesolve(
     t_first = float(tsave[0] if options.t0 is None else options.t0)
     H = resolve_callables(H, t_first, float(tsave[-1]))
     Ls = [resolve_callables(L, t_first, float(tsave[-1])) for L in Ls]
+
+    if options.vectorized and isinstance(method, ADAPTIVE):
+        return _mesolve_vectorized(H, Ls, rho0, tsave, exp_ops, method, gradient, options)
 
     route = dia_route(method, H, Ls, H.shape[-1])
     if route is not None:
@@ -282,6 +291,77 @@ def mesolve(
     else:
         infos = FixedStepInfos(shaped(res['nsteps']))
     return MESolveResult(tsave, method, gradient, options, ysave, Esave, extra, infos)
+
+
+# n^2 <= this: the vectorized form runs on the sesolve kernel variant of that size
+VECTORIZED_MAX_N2 = 1024
+
+
+def _mesolve_vectorized(H, Ls, rho0, tsave, exp_ops, method, gradient, options) -> MESolveResult:
+    """``Options(vectorized=True)``: the reference's ``vector_field_vec``
+    (diffrax_integrator.py:325-327), d vec(rho)/dt = slindbladian(H, Ls) vec(rho) in
+    the column-stacking convention (utils/vectorization.py:213-259), integrated as a
+    GEMV on the ``dqh_sesolve_tsit5`` kernel with the n^2 x n^2 superoperator as
+    generator (its PWC and modulated Hamiltonian terms as superoperator terms). The
+    adaptive controller sees the same n^2 entries as the matrix form. Expectation
+    values are taken from the saved states on the host."""
+    from .propagators import slindbladian
+    from .sesolve import SEInstance
+    from .sesolve import build_plan as se_build_plan
+
+    out_shape, insts = enumerate_instances(H, Ls, rho0, options.cartesian_batching)
+    n = H.shape[-1]
+    if n * n > VECTORIZED_MAX_N2:
+        raise NotImplementedError(
+            f'Option `vectorized=True` integrates the {n * n} x {n * n} Liouvillian as a matrix-vector product on the '
+            f'MI355X, which supports n^2 <= {VECTORIZED_MAX_N2} (n <= 32); use the default unvectorized form.'
+        )
+    if any(inst.Lmod for inst in insts):
+        raise NotImplementedError('Option `vectorized=True` is supported with constant jump operators only.')
+    if options.assume_hermitian:
+        for inst in insts:
+            check_hermitian(inst.rho0, 'y0')
+    sup = lambda O: 1j * slindbladian(O, [])  # noqa: E731  (G = -i (1j S) = S for a term O: -i[O, .])
+    se_insts = []
+    for inst in insts:
+        se_insts.append(SEInstance(
+            H=1j * slindbladian(inst.H, inst.Ls),
+            psi0=np.swapaxes(inst.rho0, -1, -2).reshape(-1),  # vec(rho), column stacking
+            pwc=[(t, v, sup(O)) for (t, v, O) in inst.pwc],
+            mod=[(g, sup(O), key) for (g, O, key) in inst.mod],
+            key_H=(inst.key_H, inst.key_L),
+        ))
+    plan = se_build_plan(
+        se_insts, tsave, t0=None if options.t0 is None else float(options.t0), rtol=method.rtol, atol=method.atol,
+        safety=method.safety_factor, factormin=method.min_factor, factormax=method.max_factor,
+        max_steps=method.max_steps, save_states=True, tableau='dopri5' if isinstance(method, Dopri5) else 'tsit5',
+    )
+    plan.run()
+    res = plan.results()
+    if np.any(res['status'] == -3):
+        raise max_steps_error()
+    if np.any(res['status'] == -4):
+        raise RuntimeError('The solver stopped on a non-finite error estimate or a vanishing step size.')
+    B = len(insts)
+    states = np.swapaxes(res['states'].reshape(B, len(tsave), n, n), -1, -2)  # unvectorize
+    extra = None
+    if options.save_extra is not None:
+        extra = save_extra_outputs(options.save_extra, states, rho0.dims, out_shape)
+    Esave = None
+    if exp_ops is not None:
+        Es = np.stack([E.to_numpy() for E in exp_ops]).astype(np.complex128)
+        Esave = np.einsum('eij,btji->bet', Es, states.astype(np.complex128)).astype(np.complex64)
+        Esave = Esave.reshape(*out_shape, len(Es), len(tsave))
+    states = states.reshape(*out_shape, len(tsave), n, n)
+    if not options.save_states:
+        states = states[..., -1:, :, :]
+
+    def shaped(x):
+        x = np.asarray(x).reshape(out_shape)
+        return x if out_shape else x[()]
+
+    infos = AdaptiveStepInfos(shaped(res['nsteps']), shaped(res['naccepted']), shaped(res['nrejected']))
+    return MESolveResult(tsave, method, gradient, options, QArray(states, dims=rho0.dims), Esave, extra, infos)
 
 
 def _stack_trees(trees: list):

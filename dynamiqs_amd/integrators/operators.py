@@ -146,7 +146,14 @@ def spline_table(f, ta: float, tb: float, *, tol: float = 1e-12, n0: int = 256, 
         ci = CubicSpline(t, y.imag, axis=0)
         tm = 0.5 * (t[:-1] + t[1:])
         err = np.max(np.abs(cr(tm) + 1j * ci(tm) - _sample(f, tm)))
-        if err <= tol * max(1.0, float(np.max(np.abs(y)))) or n >= nmax:
+        ok = err <= tol * max(1.0, float(np.max(np.abs(y))))
+        if ok or n >= nmax:
+            if not ok and nmax > n0:
+                raise ValueError(
+                    f'the time-dependent coefficient could not be tabulated to {tol:.1e} with {n} cubic intervals on '
+                    f'[{ta}, {tb}] (error {err:.1e}): it is not smooth there. Pass its jumps as `dq.pwc` terms, or '
+                    'split the time interval at them.'
+                )
             coef = np.moveaxis(cr.c + 1j * ci.c, 0, 1)  # (n, 4, *b)
             return ta, (tb - ta) / n, np.ascontiguousarray(coef)
         n *= 2
@@ -168,6 +175,14 @@ def resample_pwc(times: np.ndarray, values: np.ndarray, grid: np.ndarray) -> np.
     idx = np.clip(idx, 0, values.shape[-1] - 1)
     out = np.asarray(values, dtype=np.complex128)[..., idx]
     return np.where(inside, out, 0.0)
+
+
+def _reject_discontinuities(c) -> None:
+    if np.size(c.discontinuity_ts):
+        raise NotImplementedError(
+            'timecallable operators with `discontinuity_ts` are not supported by the MI355X solver (the kernels '
+            'tabulate their coefficients as smooth piecewise cubics): use `dq.pwc` for the discontinuous part.'
+        )
 
 
 def resolve_callables(x, ta: float, tb: float, max_terms: int = 4, seed: int = 0):
@@ -195,6 +210,8 @@ def resolve_callables(x, ta: float, tb: float, max_terms: int = 4, seed: int = 0
         calls = [t for t in x.timeqarrays if isinstance(t, CallableTimeQArray)]
         if not calls:
             return x
+        for c in calls:
+            _reject_discontinuities(c)
         others = [t for t in x.timeqarrays if not isinstance(t, CallableTimeQArray)]
         budget = max_terms - sum(isinstance(t, ModulatedTimeQArray) for t in others)
 
@@ -208,6 +225,7 @@ def resolve_callables(x, ta: float, tb: float, max_terms: int = 4, seed: int = 0
         return SummedTimeQArray(others + (list(r.timeqarrays) if isinstance(r, SummedTimeQArray) else [r]))
     if not isinstance(x, CallableTimeQArray):
         return x
+    _reject_discontinuities(x)
     ta, tb = float(ta), float(tb)
     if not tb > ta:
         tb = ta + 1.0

@@ -40,7 +40,7 @@ from dataclasses import replace
 import numpy as np
 
 from .._checks import check_shape, check_times
-from ..gradient import Gradient
+from ..gradient import Gradient, reject_gradient
 from .. import _native
 from ..method import Dopri5, Euler, Expm, Method, Tsit5
 from ..options import Options, check_options
@@ -75,6 +75,30 @@ def expm_times(t0: float, tsave: np.ndarray, disc: np.ndarray):
     disc = np.clip(np.asarray(disc, dtype=np.float64), t0, tsave[-1])
     times = np.unique(np.concatenate([[t0], tsave, disc]))
     return times, np.searchsorted(times, tsave)
+
+
+# Largest padded generator dimension dqh_expm_propagators takes (N = n^2 for the
+# Liouvillian: n <= 64), and the memory one Expm call may use for its generators,
+# propagators and work matrices (host complex128 copy included).
+EXPM_MAX_N = 4096
+EXPM_MAX_BYTES = 16 << 30
+
+
+def check_expm_size(nchain: int, nint: int, N: int) -> None:
+    """Raise before any generator is built when an Expm call would exceed what
+    ``dqh_expm_propagators`` accepts or a sane memory footprint."""
+    Np = pad16(N)
+    if Np > EXPM_MAX_N:
+        raise NotImplementedError(
+            f'Method `Expm` exponentiates {N} x {N} generators here (at most {EXPM_MAX_N}: n <= 64 for the '
+            'Liouvillian of mesolve / mepropagator); use `Tsit5` for larger systems.'
+        )
+    nbytes = nchain * nint * (N * N * 16 + 4 * Np * Np * 8)
+    if nbytes > EXPM_MAX_BYTES:
+        raise NotImplementedError(
+            f'Method `Expm` would hold {nbytes / 2**30:.1f} GiB of generators and propagators for {nchain} chains of '
+            f'{nint} intervals (limit {EXPM_MAX_BYTES / 2**30:.0f} GiB); use fewer save times or `Tsit5`.'
+        )
 
 
 def expm_chain(X: np.ndarray, Y0: np.ndarray | None = None) -> np.ndarray:
@@ -145,6 +169,7 @@ def _expm_result(ops_at, build, chains, N, dims, out_shape, tsave, t0, disc, met
     ns = len(tsave)
     U = np.empty((len(chains), ns, N, N), dtype=np.complex64)
     if nint > 0:
+        check_expm_size(len(chains), nint, N)
         dts = np.diff(times)
         X = np.empty((len(chains), nint, N, N), dtype=np.complex128)
         for b, c in enumerate(chains):
@@ -254,6 +279,7 @@ def sepropagator(
         method = Expm() if ispwc(H) else Tsit5()
     assert_method_supported(method, SUPPORTED_METHODS)
     method.assert_supports_gradient(gradient)
+    reject_gradient(gradient, '`dq.sepropagator()`')
 
     n = H.shape[-1]
     if isinstance(method, Expm):
@@ -351,6 +377,7 @@ def mepropagator(
         method = Expm() if all(ispwc(x) for x in [H, *Ls]) else Tsit5()
     assert_method_supported(method, SUPPORTED_METHODS)
     method.assert_supports_gradient(gradient)
+    reject_gradient(gradient, '`dq.mepropagator()`')
 
     n = H.shape[-1]
     if isinstance(method, Expm):
@@ -472,6 +499,7 @@ def expm_solve(H, Ls, y0: QArray, tsave, Es, method, gradient, options, me: bool
     B, ns = len(parts_all), len(tsave)
     out = np.empty((B, ns, N), dtype=np.complex64)
     if nint > 0:
+        check_expm_size(len(chain_keys), nint, N)
         dts = np.diff(times)
         X = np.empty((len(chain_keys), nint, N, N), dtype=np.complex128)
         for ci, key in enumerate(chain_keys):

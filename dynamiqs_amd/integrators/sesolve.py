@@ -16,7 +16,7 @@ import numpy as np
 
 from .. import _native
 from .._checks import check_shape, check_times
-from ..gradient import Gradient
+from ..gradient import Gradient, reject_gradient
 from ..method import Dopri5, Euler, Expm, Method, Tsit5
 from ..options import Options, check_options
 from ..qarrays import QArray, asqarray
@@ -403,6 +403,7 @@ def sesolve(
     options = options.initialise()
     assert_method_supported(method, SUPPORTED_METHODS)
     method.assert_supports_gradient(gradient)
+    reject_gradient(gradient, '`dq.sesolve()`')
 
     Es = [E.to_numpy() for E in exp_ops] if exp_ops is not None else None
     if isinstance(method, Expm):  # propagators on MFMA applied to the initial kets

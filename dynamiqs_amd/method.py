@@ -1,8 +1,10 @@
 """Integration methods, mirroring ``dynamiqs/method.py``.
 
-``Tsit5`` (:438-476, defaults of ``_DEAdaptiveStep`` :121-127) and ``Rouchon1``
-(:207-237) run on the GPU. The other method names of the reference exist so that
-``mesolve`` can raise the reference's ``TypeError`` for unsupported methods.
+``Tsit5`` (:438-476, defaults of ``_DEAdaptiveStep`` :121-127), ``Dopri5``,
+``Euler``, ``Rouchon1`` (:207-237), the fixed-step ``Rouchon2`` / ``Rouchon3`` and
+``Expm`` run on the GPU. The other method names of the reference exist so that the
+solvers can raise the reference's ``TypeError`` for the methods they do not
+support.
 """
 
 from __future__ import annotations

@@ -112,9 +112,8 @@ class MESolveResult(SolveResult):
     """Result of :func:`dynamiqs_amd.mesolve`.
 
     ``states`` (..., nsave, n, n), ``final_state``, ``expects`` (..., nE, ntsave)
-    or None, ``infos``, ``tsave``, ``method``, ``gradient``, ``options``, plus
-    ``errors`` (optional per-instance check against an FP64 reference when
-    requested by the caller).
+    or None, ``extra``, ``infos``, ``tsave``, ``method``, ``gradient``,
+    ``options`` (``dynamiqs/result.py:193-194``).
     """
 
 

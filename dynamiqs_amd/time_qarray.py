@@ -4,8 +4,11 @@ Mirrors ``dynamiqs/time_qarray.py``: ``TimeQArray.__call__ = prefactor * operato
 (:450-451), :class:`ConstantTimeQArray` (:497-569), :class:`PWCTimeQArray`
 (:572-666, prefactor ``values[..., searchsorted(times, t, 'right') - 1]`` and zero
 outside ``[times[0], times[-1])``, :649-659), :class:`SummedTimeQArray`
-(:836-956). ``modulated``/``timecallable`` operators are represented too but the
-GPU solver accepts only constant and piecewise-constant operators.
+(:836-956), :class:`ModulatedTimeQArray` (:669-760) and
+:class:`CallableTimeQArray` (:763-834). The solvers take modulated terms through
+piecewise-cubic coefficient tables evaluated on the device, and rewrite
+``timecallable`` operators as constant + modulated terms over the solve interval
+(``dynamiqs_amd.integrators.operators``).
 """
 
 from __future__ import annotations
@@ -154,8 +157,14 @@ class ModulatedTimeQArray(TimeQArray):
 
 
 class CallableTimeQArray(TimeQArray):
-    def __init__(self, f):
+    def __init__(self, f, discontinuity_ts=None):
         self.f = f
+        self._disc_ts = (np.zeros(0) if discontinuity_ts is None
+                         else np.sort(np.asarray(discontinuity_ts, dtype=float).reshape(-1)))
+
+    @property
+    def discontinuity_ts(self) -> np.ndarray:
+        return self._disc_ts
 
     @property
     def shape(self):
@@ -174,7 +183,7 @@ class CallableTimeQArray(TimeQArray):
 
     def __mul__(self, y):
         f = self.f
-        return CallableTimeQArray(lambda t: asqarray(f(t)) * y)
+        return CallableTimeQArray(lambda t: asqarray(f(t)) * y, self._disc_ts)
 
     __rmul__ = __mul__
 
@@ -242,8 +251,11 @@ def modulated(f, qarray, *, discontinuity_ts=None) -> ModulatedTimeQArray:
     return out
 
 
-def timecallable(f) -> CallableTimeQArray:
-    return CallableTimeQArray(f)
+def timecallable(f, *, discontinuity_ts=None) -> CallableTimeQArray:
+    """``O(t) = f(t)`` (``dynamiqs/time_qarray.py:184-237``)."""
+    if not callable(f):
+        raise TypeError(f'Argument `f` must be a function, but has type {type(f).__name__}.')
+    return CallableTimeQArray(f, discontinuity_ts)
 
 
 def astimeqarray(x) -> TimeQArray:

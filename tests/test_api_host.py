@@ -91,6 +91,18 @@ def test_unsupported_time_dependence_is_reported():
 
     with pytest.raises(TypeError, match=re.escape('Argument `f` must be a function, but has type float.')):
         dq.modulated(1.0, H)
+    # timecallable takes the reference's keyword-only discontinuity_ts; the kernels
+    # cannot tabulate a jump, so it is reported
+    step = dq.timecallable(lambda t: dq.asqarray((1.0 if t < 0.5 else 2.0) * H), discontinuity_ts=[0.5])
+    assert list(step.discontinuity_ts) == [0.5]
+    with pytest.raises(NotImplementedError, match='discontinuity_ts'):
+        dq.mesolve(step, Ls, psi, ts)
+    # a jump passed without discontinuity_ts is caught by the coefficient table
+    # (it cannot meet its tolerance) instead of being silently smoothed
+    from dynamiqs_amd.integrators.operators import spline_table
+
+    with pytest.raises(ValueError, match='not smooth'):
+        spline_table(lambda t: np.where(np.asarray(t) < 0.37, 1.0, 2.0) + 0j, 0.0, 1.0, nmax=1 << 12)
 
 
 def test_spline_table_accuracy_and_layout():
@@ -314,6 +326,21 @@ def test_propagator_argument_errors():
             dq.mepropagator(H, [], ts)
         except RuntimeError:
             pass  # no GPU here: the warning comes first
+
+
+def test_expm_rejects_large_liouvillians_before_building_them():
+    """n = 65 gives a 4225 x 4225 Liouvillian, above what the Expm kernels take: the
+    call raises before any generator is built or any device memory is touched."""
+    import time
+
+    a = dq.destroy(65)
+    ts = np.linspace(0.0, 1.0, 11)
+    t0 = time.perf_counter()
+    with pytest.raises(NotImplementedError, match='Expm'):
+        dq.mepropagator(a.dag() @ a, [a], ts, method=dq.method.Expm())
+    with pytest.raises(NotImplementedError, match='Expm'):
+        dq.mesolve(a.dag() @ a, [a], dq.fock(65, 1), ts, method=dq.method.Expm())
+    assert time.perf_counter() - t0 < 10.0
 
 
 def test_expm_interval_bookkeeping_and_liouvillian():

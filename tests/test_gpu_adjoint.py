@@ -184,3 +184,72 @@ def test_adjoint_plan_reuse_matches_one_shot():
         J, g = mesolve_pwc_grad(H0, ops, times, uk, Ls, rho0, O, checkpoint_every=5)
         np.testing.assert_allclose(r.J, J, rtol=0, atol=1e-12)
         np.testing.assert_allclose(r.grad_u, g, rtol=0, atol=1e-12)
+
+
+def test_readme_gradient_kat_with_parameter_gradient():
+    """The README gradient example (README.md:85-121) with dJ/domega as a parameter
+    gradient of the constant term omega a^dag a (no control at all): d/domega 0.0000,
+    d/dkappa -3.3520, d/dalpha0 1.0670."""
+    from dynamiqs_amd.adjoint import mesolve_adjoint
+    from oracle import operators as op
+
+    n, omega, kappa, alpha0 = 16, 1.0, 0.1, 1.0
+    T = 2 * np.pi
+    a = op.destroy(n)
+    N = a.conj().T @ a
+    times = np.linspace(0.0, T, 41)
+    res = mesolve_adjoint(omega * N, [], times, np.zeros((1, 0, 40)), [np.sqrt(kappa) * a], op.coherent_dm(n, alpha0),
+                          N, rate_ops=[a], param_ops=[N], checkpoint_every=4)
+    h = 1e-5
+    drho = (op.coherent_dm(n, alpha0 + h) - op.coherent_dm(n, alpha0 - h)) / (2 * h)
+    g_omega = float(res.grad_params[0, 0])
+    g_kappa = float(res.grad_rates[0, 0])
+    g_alpha = float(np.real(np.trace(res.lambda0[0].astype(complex) @ drho)))
+    assert abs(g_omega) <= 1e-4, g_omega
+    assert abs(g_kappa - (-3.3520)) <= 1.5e-4 and abs(g_alpha - 1.0670) <= 1.5e-4, (g_kappa, g_alpha)
+
+
+def test_loss_over_several_times_and_modulated_parameter_vs_fd():
+    """J = Tr(N rho(t_a)) + Tr(X rho(T)) with a PWC drive, a modulated term
+    theta g(t) X and a constant term phi Y: dJ/du, dJ/dtheta and dJ/dphi against
+    central differences (h = 1e-4) of the complex128 oracle at tight tolerances."""
+    from dynamiqs_amd.adjoint import mesolve_adjoint
+    from oracle import mesolve as om
+    from oracle import operators as op
+
+    rng = np.random.default_rng(3)
+    n, nseg, T = 8, 10, 2.0
+    a = op.destroy(n)
+    N, X = a.conj().T @ a, a + a.conj().T
+    Y = 1j * (a.conj().T - a)
+    times = np.linspace(0.0, T, nseg + 1)
+    u = rng.normal(0.0, 0.5, size=(2, 1, nseg))
+    g = lambda t: 0.3 * np.cos(2.0 * t)  # noqa: E731
+    phi = 0.2
+    H0 = N + phi * Y
+    Ls = [np.sqrt(0.1) * a]
+    rho0 = op.fock_dm(n, 0)
+    losses = [(times[4], N), (T, X)]
+
+    def loss(ub, theta=1.0, ph=phi):
+        Hop = om.TimeOp(N + ph * Y, [om.PWCTerm(times, ub[0], X), om.ModTerm(lambda t: theta * g(t), X)])
+        r = om.mesolve(Hop, Ls, rho0, times, exp_ops=[N, X], rtol=1e-10, atol=1e-12)
+        return float(r['expects'][0, 4].real + r['expects'][1, -1].real)
+
+    res = mesolve_adjoint(H0, [X], times, u, Ls, rho0, losses, mod_terms=[(g, X)], param_ops=[(g, X), Y],
+                          rtol=1e-8, atol=1e-10, checkpoint_every=5)
+    hh = 1e-4
+    for b in range(2):
+        assert abs(res.J[b] - loss(u[b])) < 1e-5
+        fd = np.zeros(nseg)
+        for k in range(nseg):
+            up, um = u[b].copy(), u[b].copy()
+            up[0, k] += hh
+            um[0, k] -= hh
+            fd[k] = (loss(up) - loss(um)) / (2 * hh)
+        err = np.abs(res.grad_u[b, 0] - fd).max()
+        assert err <= 1e-4 * np.abs(fd).max() + 1e-6, (b, err)
+        fd_theta = (loss(u[b], theta=1 + hh) - loss(u[b], theta=1 - hh)) / (2 * hh)
+        fd_phi = (loss(u[b], ph=phi + hh) - loss(u[b], ph=phi - hh)) / (2 * hh)
+        assert abs(res.grad_params[b, 0] - fd_theta) <= 1e-4 * max(1.0, abs(fd_theta)), (res.grad_params[b], fd_theta)
+        assert abs(res.grad_params[b, 1] - fd_phi) <= 1e-4 * max(1.0, abs(fd_phi)), (res.grad_params[b], fd_phi)

@@ -90,3 +90,40 @@ def test_pwc_values_packed_as_resample_pwc(host_device):
         off = int(d['pwc_vals']) - base
         got = host[off : off + want.size * 16].view(np.complex128).reshape(want.shape)
         np.testing.assert_array_equal(got, want)
+
+
+def test_shared_kraus_sets_built_once_and_guarded(host_device, monkeypatch):
+    """Rouchon1 with a modulated Hamiltonian term (one Kraus set per step): instances
+    that share key_H / key_L (a batched rho0) build their sets once and point at the
+    same device copy; the size guard counts D and C over the distinct sets of the
+    batch."""
+    rng = np.random.default_rng(3)
+    n = 6
+    H = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    H = H + H.conj().T
+    L = 0.2 * np.diag(np.sqrt(np.arange(1, n)), k=1)
+    g = lambda t: np.cos(3.0 * t)  # noqa: E731
+    calls = []
+    orig = dense._kraus_sets
+
+    def counting(*a, **k):
+        calls.append(1)
+        return orig(*a, **k)
+
+    monkeypatch.setattr(dense, '_kraus_sets', counting)
+    insts = [dense.Instance(H=H, Ls=[L], rho0=np.eye(n) / n * (1 + 0.1 * b), mod=[(g, H, ('g',))], key_H=('h',),
+                            key_L=('l',), key_rho=(b,)) for b in range(3)]
+    tsave = np.linspace(0.0, 0.2, 3)
+    plan = dense.build_plan('rouchon1', insts, tsave, dt=0.01)
+    nsteps = 20
+    assert len(calls) == nsteps  # one set per step, built for the first instance only
+    assert len(set(plan.descs['Dops'])) == 1 and len(set(plan.descs['Cops'])) == 1
+    assert len(set(plan.descs['step_set'])) == 3  # each instance keeps its own index array
+    assert np.all(plan.descs['max_steps'] == nsteps)
+    dense.check_descriptors(plan.descs, [*plan.arena.tensors, plan.states, plan.final, plan.expects, plan.stats])
+    # distinct keys: the guard sums the D and C sets of every instance
+    monkeypatch.setattr(dense, 'KRAUS_SET_BUDGET', 2 * 2 * nsteps * 2 * 16 * 16 * 8 + 1)
+    distinct = [dense.Instance(H=H, Ls=[L], rho0=np.eye(n) / n, mod=[(g, H, ('g',))], key_H=(b,), key_L=('l',),
+                               key_rho=()) for b in range(3)]
+    with pytest.raises(NotImplementedError, match='exceed'):
+        dense.build_plan('rouchon1', distinct, tsave, dt=0.01)
