@@ -8,10 +8,15 @@ rho0 = vacuum, 201 save times on [0, 10], Tsit5 rtol = atol = 1e-6, complex64
 operators and states, B = 256 instances.
 
 Printed JSON line: solves/s of the batched path (``value``), the plain
-one-solve-per-call GPU path and their ratio, p50/p99 latencies, the FP32-MFMA
-roofline of the persistent Tsit5 kernel (HIP-event timing on the launch stream),
-the error of sampled instances against the complex128 oracle, and the oracle
-timed on the host CPU as the baseline.
+one-solve-per-call GPU path (the faster of its two single-solve forms) and their
+ratio, p50/p99 latencies, the FP32-MFMA roofline of the persistent Tsit5 kernel
+(HIP-event timing on the launch stream), the error of 8 instances (the largest
+eps2 among them) replayed by the complex128 oracle on the GPU's accepted steps,
+and the CPU restatement in complex64 and complex128 timed on the host cores as
+the baseline. A ``secondary`` block measures the other workloads in the same run:
+the plain n = 16 solve (C1), sparse-DIA Rouchon1 at n = 1024 (C3), the serving mix
+at one offered rate and at saturation with sampled errors (C4), and the adjoint
+gradient against a forward solve (C5).
 """
 
 from __future__ import annotations
@@ -37,6 +42,7 @@ HBM_PEAK_GBS = 8000.0
 # (--config copy: FETCH_SIZE x 2, WRITE_SIZE x 1): 133 797 306 KiB fetched,
 # 232 698 948 KiB written (packed FP64 state writes and register spills).
 KERR_TRAFFIC_BYTES = 133_797_306 * 1024 * 2 + 232_698_948 * 1024
+KERR_TRAFFIC_SOURCE = 'rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this script, build of 0eaabea'
 
 
 def kerr_instances(B: int, n: int = 64, seed: int = 0):
@@ -61,44 +67,6 @@ def kerr_instances(B: int, n: int = 64, seed: int = 0):
 def trace_distance(a, b):
     d = a - b
     return 0.5 * np.abs(np.linalg.eigvalsh(0.5 * (d + d.conj().T))).sum()
-
-
-def cpu_oracle_solve(inst, tsave, rtol, atol):
-    """The complex128 oracle (CPU restatement) on one instance, 1 BLAS thread."""
-    from oracle import tsit5
-
-    H, Ls = inst.H, inst.Ls
-    LdL = sum(L.conj().T @ L for L in Ls)
-    Hnh = -1j * H - 0.5 * LdL
-    Lds = [0.5 * L.conj().T for L in Ls]
-
-    def f(t, y, tseg):
-        tmp = Hnh @ y
-        for L, Ld in zip(Ls, Lds):
-            tmp = tmp + (L @ y) @ Ld
-        return tmp + tmp.conj().T
-
-    saved, ylast, st = tsit5.integrate(f, inst.rho0, tsave, rtol=rtol, atol=atol)
-    return np.stack(saved), st
-
-
-def cpu_oracle_replay(inst, tsave, step_ends):
-    """The complex128 oracle's Tsit5 on a prescribed sequence of accepted steps."""
-    from oracle import tsit5
-
-    H, Ls = inst.H, inst.Ls
-    LdL = sum(L.conj().T @ L for L in Ls)
-    Hnh = -1j * H - 0.5 * LdL
-    Lds = [0.5 * L.conj().T for L in Ls]
-
-    def f(t, y, tseg):
-        tmp = Hnh @ y
-        for L, Ld in zip(Ls, Lds):
-            tmp = tmp + (L @ y) @ Ld
-        return tmp + tmp.conj().T
-
-    saved, _ = tsit5.integrate_steps(f, inst.rho0, tsave, step_ends)
-    return np.stack(saved)
 
 
 class EventTimer:
@@ -627,6 +595,393 @@ def bench_sesolve(args):
     return 0
 
 
+# ------------------------------------------------------------------ CPU pool
+# The CPU restatement runs in worker processes forked before the GPU is touched
+# (one BLAS thread each), so replays and baseline timings of many instances run
+# side by side on the host cores after the GPU measurements.
+
+
+def _worker_init():
+    os.environ['OMP_NUM_THREADS'] = '1'
+    try:
+        from threadpoolctl import threadpool_limits
+
+        threadpool_limits(limits=1)
+    except ImportError:  # pragma: no cover
+        pass
+
+
+def make_cpu_pool(nworkers: int):
+    import multiprocessing as mp
+
+    return mp.get_context('fork').Pool(nworkers, initializer=_worker_init)
+
+
+def cpu_workers(requested: int) -> int:
+    cores = len(os.sched_getaffinity(0))
+    return max(1, min(requested if requested > 0 else 16, cores, 16))
+
+
+def _lindblad_f(H, Ls, dtype):
+    """The Hermitian-form vector field (diffrax_integrator.py:319-323) in ``dtype``."""
+    LdL = sum(L.conj().T @ L for L in Ls)
+    Hnh = (-1j * H - 0.5 * LdL).astype(dtype)
+    Ds = [L.astype(dtype) for L in Ls]
+    Cs = [(0.5 * L.conj().T).astype(dtype) for L in Ls]
+
+    def f(t, y, tseg):
+        tmp = Hnh @ y
+        for D, C in zip(Ds, Cs):
+            tmp = tmp + (D @ y) @ C
+        return tmp + tmp.conj().T
+
+    return f
+
+
+def _task_replay(job):
+    """FP64 oracle replay of the GPU's accepted steps; error of the GPU states."""
+    from oracle import tsit5
+
+    H, Ls, rho0, tsave, step_ends, got = job
+    saved, _ = tsit5.integrate_steps(_lindblad_f(H, Ls, np.complex128), rho0, tsave, step_ends)
+    ref = np.stack(saved)
+    got = got.astype(np.complex128)
+    fro = float(np.linalg.norm(got - ref, axis=(-2, -1)).max())
+    td = float(max(trace_distance(got[j], ref[j]) for j in range(len(tsave))))
+    return fro, td
+
+
+def _task_adaptive(job):
+    """The oracle's own adaptive Tsit5 in ``dtype`` on one instance, timed (one
+    core); with ``got``, the largest Frobenius distance of the GPU states to it."""
+    from oracle import tsit5
+
+    H, Ls, rho0, tsave, rtol, atol, dtype, got = job
+    f = _lindblad_f(H, Ls, dtype)
+    t0 = time.perf_counter()
+    saved, _, st = tsit5.integrate(f, rho0.astype(dtype), tsave, rtol=rtol, atol=atol, dtype=dtype)
+    sec = time.perf_counter() - t0
+    fro = None
+    if got is not None:
+        fro = float(np.linalg.norm(got.astype(np.complex128) - np.stack(saved), axis=(-2, -1)).max())
+    return sec, st.nsteps, fro
+
+
+# ------------------------------------------------------------------ headline
+
+
+def bench_kerr(args, pool):
+    """The headline: batched dense Tsit5 on the Kerr batch (see module docstring).
+    Returns the JSON dict and the CPU jobs to run once every GPU measurement is done."""
+    import torch
+
+    from dynamiqs_amd.integrators import dense
+
+    n, K = 64, 2
+    rtol = atol = 1e-6
+    tsave = np.linspace(0.0, 10.0, 201)
+    insts, eps2 = kerr_instances(args.batch, n)
+    # the accepted step ends of each instance are logged (one store per accepted
+    # step) so that the FP64 oracle can replay the identical step sequence
+    plan = dense.build_plan('tsit5', insts, tsave, rtol=rtol, atol=atol, step_log_cap=40_000)
+    stream = torch.cuda.current_stream()
+
+    # warmup + timed batched solves (one step = one batched solve of B instances)
+    run_plan_timed(plan, args.warmup, stream)
+    torch.cuda.synchronize()
+    timer = EventTimer(stream)
+    walls = run_plan_timed(plan, args.steps, stream, timer)
+    torch.cuda.synchronize()
+    kern_ms = timer.times_ms()
+    res = plan.results()
+    if np.any(res['status'] != 0):
+        raise RuntimeError(f'solver status {np.unique(res["status"])}')
+    solves_per_s = args.batch * args.steps / sum(walls)
+    ms_per_step = 1e3 * sum(walls) / args.steps
+    f_rhs = (1 + 2 * K) * 8.0 * n**3
+    flops = float(np.sum(res['nrhs'])) * f_rhs
+    kern_avg_s = 1e-3 * float(np.mean(kern_ms))
+    achieved = flops / kern_avg_s / 1e12
+
+    # plain path: one solve per call on the same device-resident inputs, in the
+    # faster of the two single-solve forms (one persistent workgroup, or every
+    # stage spread grid-wide with the step captured as a hipGraph)
+    plain = {}
+    for name, g in (('persistent', False), ('grid', True)) if args.plain > 0 else ():
+        lat = []
+        for i in range(args.plain):
+            p1 = dense.build_plan('tsit5', [insts[i]], tsave, rtol=rtol, atol=atol, grid_tsit5=g)
+            run_plan_timed(p1, 1, stream)
+            lat.append(run_plan_timed(p1, 1, stream)[0])
+        plain[name] = lat
+    best = min(plain, key=lambda k: float(np.median(plain[k]))) if plain else None
+    plain_lat = sorted(plain[best]) if plain else []
+    plain_sps = len(plain_lat) / sum(plain_lat) if plain_lat else None
+
+    out = {
+        'metric': 'solves/s + p50/p99 latency, batched/served vs plain GPU mesolve at <=1e-5 err',
+        'value': solves_per_s,
+        'unit': 'solves/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': ms_per_step,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'c64 (fp32 MFMA), fp64 state/norms',
+        'data': 'synthetic',
+        'config': {
+            'workload': 'batched dense Tsit5, two-photon driven Kerr oscillator n=64, 2 jump ops, B=256, 201 tsave on [0,10], rtol=atol=1e-6',
+            'global_batch': args.batch,
+            'n': n,
+            'jump_ops': K,
+            'parallelism': 'one persistent workgroup per instance',
+            'timed': 'device-resident inputs and outputs (packing and the read-back of the saved states excluded, '
+                     'for the batched and the plain path alike)',
+        },
+        'plain_solves_per_s': plain_sps,
+        'plain_form': best,
+        'plain_latency_s_by_form': {k: float(np.median(v)) for k, v in plain.items()},
+        'batched_vs_plain': solves_per_s / plain_sps if plain_sps else None,
+        'latency_s': {
+            'batched_p50': float(np.percentile(walls, 50)),
+            'batched_p99': float(np.percentile(walls, 99)),
+            'plain_p50': float(np.percentile(plain_lat, 50)) if plain_lat else None,
+            'plain_p99': float(np.percentile(plain_lat, 99)) if plain_lat else None,
+        },
+        'steps_per_instance': {
+            'mean': float(np.mean(res['nsteps'])),
+            'max': int(np.max(res['nsteps'])),
+            'rejected_mean': float(np.mean(res['nrejected'])),
+        },
+        'roofline': {
+            'bound': 'mfma',
+            'achieved': achieved,
+            'peak': FP32_MFMA_PEAK_TFLOPS,
+            'unit': 'TFLOP/s',
+            'frac': achieved / FP32_MFMA_PEAK_TFLOPS,
+            'traffic': KERR_TRAFFIC_BYTES if (args.batch == 256 and plan.uniform_np == 64) else None,
+            'traffic_source': KERR_TRAFFIC_SOURCE,
+            'kernel': 'dqh::k_tsit5_res<64>' if plan.uniform_np == 64 else 'dqh::k_tsit5',
+            'kernel_avg_ms': float(np.mean(kern_ms)),
+            'flops_per_launch': flops,
+        },
+        'error_vs_fp64': None,
+        'cpu_baseline': None,
+    }
+
+    def finish(pool):
+        if pool is None or args.cpu_sample <= 0:
+            return
+        # replay >= 8 instances, the largest eps2 among them, on the GPU's steps
+        order = list(range(min(args.replay, args.batch)))
+        big = int(np.argmax(eps2))
+        if big not in order:
+            order[-1] = big
+        jobs_r = [(insts[i].H, insts[i].Ls, insts[i].rho0, tsave, res['step_ends'][i], res['states'][i])
+                  for i in order]
+        ns = min(args.cpu_sample, args.batch)
+        jobs_128 = [(insts[i].H, insts[i].Ls, insts[i].rho0, tsave, rtol, atol, np.complex128, res['states'][i])
+                    for i in range(ns)]
+        jobs_64 = [(insts[i].H, insts[i].Ls, insts[i].rho0, tsave, rtol, atol, np.complex64, None)
+                   for i in range(ns)]
+        a_r = pool.map_async(_task_replay, jobs_r, chunksize=1)
+        a_128 = pool.map_async(_task_adaptive, jobs_128, chunksize=1)
+        a_64 = pool.map_async(_task_adaptive, jobs_64, chunksize=1)
+        rr, r128, r64 = a_r.get(), a_128.get(), a_64.get()
+        med128 = float(np.median([x[0] for x in r128]))
+        med64 = float(np.median([x[0] for x in r64]))
+        cores = cpu_workers(args.cpu_workers)
+        out['error_vs_fp64'] = {
+            'max_frobenius': max(x[0] for x in rr),
+            'max_trace_distance': max(x[1] for x in rr),
+            'instances_checked': len(rr),
+            'instances': order,
+            'largest_eps2_instance': big,
+            'reference': 'complex128 Tsit5 oracle on the identical accepted step sequence',
+            'vs_adaptive_fp64_max_frobenius': max(x[2] for x in r128),
+            'vs_adaptive_fp64_instances': ns,
+            'note': 'the FP64 oracle with its own adaptive steps differs from a tight-tolerance solve by the '
+            'same order at rtol=atol=1e-6 (stiff, stability-limited steps)',
+        }
+        out['cpu_baseline'] = {
+            'value': cores / med64,
+            'unit': 'solves/s',
+            'cores': cores,
+            'kind': 'port',
+            'sample': f'{ns} of {args.batch} instances per precision, one instance per worker process on '
+                      f'{cores} host cores (1 BLAS thread each, numpy {np.__version__}); value = cores / median '
+                      f'seconds per instance of the complex64 restatement',
+            'median_s_per_instance': {'c64': med64, 'c128': med128},
+            'c128_solves_per_s': cores / med128,
+            'runs': len(r64) + len(r128),
+        }
+
+    return out, finish
+
+
+# ------------------------------------------------------------------ secondary
+
+
+def secondary_c1():
+    """Plain single solve of the lossy oscillator through dq.mesolve end to end."""
+    import dynamiqs_amd as dq
+
+    n = 16
+    a = dq.destroy(n)
+    H, Ls, psi = a.dag() @ a, [np.sqrt(0.1) * a], dq.coherent(n, 1.0)
+    tsave = np.linspace(0.0, 2 * np.pi, 101)
+    method = dq.method.Tsit5(rtol=1e-6, atol=1e-8)
+    for _ in range(2):
+        dq.mesolve(H, Ls, psi, tsave, method=method)
+    lat = []
+    for _ in range(9):
+        t0 = time.perf_counter()
+        res = dq.mesolve(H, Ls, psi, tsave, method=method)
+        lat.append(time.perf_counter() - t0)
+    return {'workload': 'lossy oscillator n=16, 101 tsave on [0, 2 pi], Tsit5 rtol 1e-6 atol 1e-8, one solve per '
+                        'call through dq.mesolve (packing, launch, read-back)',
+            'p50_ms': 1e3 * float(np.median(lat)), 'nsteps': int(res.infos.nsteps)}
+
+
+def secondary_c3(T: float = 0.5, reps: int = 3):
+    """DIA Rouchon1 on the 1024-dimensional two-mode batch of 16, over [0, T]."""
+    import torch
+
+    from dynamiqs_amd.integrators import dia
+
+    H, Ls, a, b = two_mode_dia()
+    rng = np.random.default_rng(1)
+    B = 16
+    rho0 = []
+    for al in rng.uniform(0.5, 2.0, size=B):
+        v = np.kron(coherent_ket(32, al), np.eye(32)[:, 0])
+        rho0.append(np.outer(v, v.conj()))
+    rho0 = np.stack(rho0)
+    tsave = np.linspace(0.0, T, int(round(T / 0.1)) + 1)
+    plan = dia.build_plan(H, Ls, rho0, tsave, 1e-3)
+    stream = torch.cuda.current_stream()
+    plan.reset()
+    plan.run(stream)
+    timer = EventTimer(stream)
+    walls = run_plan_timed(plan, reps, stream, timer)
+    kern_s = 1e-3 * float(np.median(timer.times_ms()))
+    res = plan.results()
+    tr = float(np.abs(np.trace(res['final_state'], axis1=-2, axis2=-1) - 1).max())
+    step_s = kern_s / plan.nsteps
+    out = {
+        'workload': f'sparse-DIA Rouchon1, 32x32 two-mode n=1024, B={B}, dt=1e-3, T={T} ({plan.nsteps} steps)',
+        'solves_per_s': B / float(np.median(walls)),
+        'us_per_step': 1e6 * step_s,
+        'sweep_algorithmic_GBps_over_step': plan.sweep_bytes_per_step / step_s / 1e9,
+        'solve_TFLOPs_over_step': plan.solve_flops_per_step / step_s / 1e12,
+        'max_trace_error': tr,
+    }
+    phases = getattr(plan, 'phase_us', None)
+    if callable(phases):
+        ph = phases(stream)
+        out['phase_us_per_step'] = ph
+        out['sweep_algorithmic_GBps'] = plan.sweep_bytes_per_step / (1e-6 * ph['sweep']) / 1e9
+        out['solve_TFLOPs'] = plan.solve_flops_per_step / (1e-6 * ph['solve']) / 1e12
+    return out
+
+
+def secondary_c4(pool_jobs, nreq: int = 1000, rate: float = 700.0, nsample: int = 32):
+    """Serving: Poisson arrivals at ``rate`` and all at once (saturation); sampled
+    requests log their accepted steps for the FP64 replay."""
+    import threading
+
+    from dynamiqs_amd.serving import Server
+
+    reqs = serving_requests(nreq)
+    ns = np.array([r[0].shape[0] for r in reqs])
+    sample = set(np.argsort(ns, kind='stable')[np.linspace(0, nreq - 1, nsample).astype(int)].tolist())
+    rows = []
+    sampled = {}
+    for r in (rate, 0.0):
+        rng = np.random.default_rng(3)
+        gaps = rng.exponential(1.0 / r, size=nreq) if r > 0 else np.zeros(nreq)
+        with Server(memory_budget=32 << 30) as srv:
+            futs = [None] * nreq
+
+            def client():
+                t_next = time.perf_counter()
+                for i, (H, Ls, psi, ts) in enumerate(reqs):
+                    t_next += gaps[i]
+                    dt = t_next - time.perf_counter()
+                    if dt > 0:
+                        time.sleep(dt)
+                    futs[i] = srv.submit(H, Ls, psi, ts, rtol=1e-5, step_log_cap=20_000 if i in sample else 0)
+
+            t0 = time.perf_counter()
+            th = threading.Thread(target=client)
+            th.start()
+            th.join()
+            got = [f.result() for f in futs]
+            wall = time.perf_counter() - t0
+        st = dict(srv.stats)
+        lat = np.array([g.latency_s for g in got])
+        rows.append({
+            'offered_rate': r if r > 0 else 'all-at-once',
+            'solves_per_s': nreq / wall,
+            'p50_latency_s': float(np.percentile(lat, 50)),
+            'p99_latency_s': float(np.percentile(lat, 99)),
+            'gpu_busy_fraction': st.get('busy_s', 0.0) / wall,
+            'launches': st['launches'],
+        })
+        if r > 0:
+            sampled = {i: got[i] for i in sample}
+    for i, g in sampled.items():
+        H, Ls, psi, ts = reqs[i]
+        pool_jobs.append((H, Ls, psi @ psi.conj().T, ts, g.step_ends, g.states))
+    return {'workload': f'serving mix, {nreq} requests, n log-uniform 8..256 (median {int(np.median(ns))}), K 1-3, '
+                        f'T U[1,20], 50 tsave, rtol 1e-5', 'rates': rows,
+            'sampled_requests': sorted(int(i) for i in sample),
+            'sampled_n': sorted(int(ns[i]) for i in sample)}
+
+
+def secondary_c5(reps: int = 5):
+    """Adjoint gradient of 64 PWC control vectors (n = 32, 100 segments) vs a
+    forward solve of the same batch."""
+    import torch
+
+    from dynamiqs_amd.adjoint import AdjointPlan
+    from dynamiqs_amd.integrators import dense
+
+    n, nseg, B = 32, 100, 64
+    a = np.diag(np.sqrt(np.arange(1, n, dtype=np.float64)), k=1).astype(np.complex128)
+    H0, X = a.conj().T @ a, a + a.conj().T
+    times = np.linspace(0.0, 10.0, nseg + 1)
+    u = np.random.default_rng(4).normal(0.0, 0.5, size=(B, 1, nseg))
+    Ls = [np.sqrt(0.05) * a]
+    rho0 = np.zeros((n, n), dtype=np.complex128)
+    rho0[0, 0] = 1
+    insts = [dense.Instance(H=H0, Ls=Ls, rho0=rho0, pwc=[(times, u[b, 0], X)], key_L=(), key_rho=()) for b in range(B)]
+    fwd = dense.build_plan('tsit5', insts, np.array([0.0, 10.0]), exp_ops=[H0])
+    fwd.run()
+    torch.cuda.synchronize()
+    tf = []
+    for _ in range(reps):
+        fwd.reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fwd.run()
+        torch.cuda.synchronize()
+        tf.append(time.perf_counter() - t0)
+    plan = AdjointPlan(H0, [X], times, Ls, rho0, H0, B, checkpoint_every=10)
+    plan.run(u)
+    tg = []
+    for k in range(reps):
+        t0 = time.perf_counter()
+        plan.run(u * (1.0 + 0.01 * k))
+        tg.append(time.perf_counter() - t0)
+    return {'workload': 'PWC-driven oscillator n=32, 100 segments on [0,10], B=64, loss Tr(a^dag a rho(T)), '
+                        'checkpoints every 10 segments, packed AdjointPlan',
+            'forward_s': float(np.median(tf)), 'gradient_s': float(np.median(tg)),
+            'gradient_vs_forward': float(np.median(tg)) / float(np.median(tf))}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument('--config', default='kerr', choices=['kerr', 'c1', 'dia', 'serving', 'adjoint', 'copy', 'grid', 'sesolve'])
@@ -640,8 +995,11 @@ def main(argv=None):
     ap.add_argument('--steps', type=int, default=3)
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--batch', type=int, default=256)
-    ap.add_argument('--plain', type=int, default=2, help='number of one-solve-per-call solves timed')
-    ap.add_argument('--cpu-sample', type=int, default=1, help='instances solved by the CPU oracle (0: skip)')
+    ap.add_argument('--plain', type=int, default=2, help='one-solve-per-call solves timed per single-solve form')
+    ap.add_argument('--cpu-sample', type=int, default=5, help='instances the CPU restatement times per precision (0: skip)')
+    ap.add_argument('--replay', type=int, default=8, help='instances replayed by the FP64 oracle on the GPU steps')
+    ap.add_argument('--cpu-workers', type=int, default=0, help='host worker processes (0: min(16, cores))')
+    ap.add_argument('--no-secondary', action='store_true', help='only the headline (for profiling runs)')
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print(
@@ -666,138 +1024,33 @@ def main(argv=None):
     if args.config == 'adjoint':
         return bench_adjoint(args)
 
-    import torch
-
-    from dynamiqs_amd.integrators import dense
-
-    n, K = 64, 2
-    rtol = atol = 1e-6
-    tsave = np.linspace(0.0, 10.0, 201)
-    insts, eps2 = kerr_instances(args.batch, n)
-    # the accepted step ends of each instance are logged (one store per accepted
-    # step) so that the FP64 oracle can replay the identical step sequence
-    plan = dense.build_plan('tsit5', insts, tsave, rtol=rtol, atol=atol, step_log_cap=40_000)
-    stream = torch.cuda.current_stream()
-
-    # warmup + timed batched solves (one step = one batched solve of B instances)
-    run_plan_timed(plan, args.warmup, stream)
-    torch.cuda.synchronize()
-    timer = EventTimer(stream)
-    t0 = time.perf_counter()
-    walls = run_plan_timed(plan, args.steps, stream, timer)
-    torch.cuda.synchronize()
-    total = time.perf_counter() - t0
-    kern_ms = timer.times_ms()
-    res = plan.results()
-    if np.any(res['status'] != 0):
-        raise RuntimeError(f'solver status {np.unique(res["status"])}')
-    solves_per_s = args.batch * args.steps / sum(walls)
-    ms_per_step = 1e3 * sum(walls) / args.steps
-    f_rhs = (1 + 2 * K) * 8.0 * n**3
-    flops = float(np.sum(res['nrhs'])) * f_rhs
-    kern_avg_s = 1e-3 * float(np.mean(kern_ms))
-    achieved = flops / kern_avg_s / 1e12
-
-    # plain path: one solve per call, same kernel, B = 1
-    plain_lat = []
-    for i in range(args.plain):
-        p1 = dense.build_plan('tsit5', [insts[i]], tsave, rtol=rtol, atol=atol)
-        p1.run(stream)
-        torch.cuda.synchronize()
-        w = run_plan_timed(p1, 1, stream)
-        plain_lat.append(w[0])
-    plain_sps = len(plain_lat) / sum(plain_lat) if plain_lat else None
-
-    # error of sampled instances vs the complex128 oracle, and the CPU baseline timing
-    cpu = None
-    err = None
-    if args.cpu_sample > 0:
-        try:
-            from threadpoolctl import threadpool_limits
-        except ImportError:  # pragma: no cover
-            threadpool_limits = None
-        errs, tds, errs_ad, tcpu = [], [], [], 0.0
-        for i in range(args.cpu_sample):
-            limit = threadpool_limits(limits=1) if threadpool_limits is not None else None
-            if limit is not None:
-                limit.__enter__()
-            try:
-                c0 = time.perf_counter()
-                ref_ad, st = cpu_oracle_solve(insts[i], tsave, rtol, atol)
-                tcpu += time.perf_counter() - c0
-                ref = cpu_oracle_replay(insts[i], tsave, res['step_ends'][i])
-            finally:
-                if limit is not None:
-                    limit.__exit__(None, None, None)
-            got = res['states'][i].astype(np.complex128)
-            errs.append(float(np.linalg.norm(got - ref, axis=(-2, -1)).max()))
-            tds.append(max(trace_distance(got[j], ref[j]) for j in range(len(tsave))))
-            errs_ad.append(float(np.linalg.norm(got - ref_ad, axis=(-2, -1)).max()))
-        cpu = {
-            'value': args.cpu_sample / tcpu,
-            'unit': 'solves/s',
-            'cores': 1,
-            'kind': 'port',
-            'sample': f'{args.cpu_sample} of {args.batch} instances (complex128 Tsit5 oracle, 1 BLAS thread)',
-        }
-        err = {
-            'max_frobenius': max(errs),
-            'max_trace_distance': max(tds),
-            'instances_checked': args.cpu_sample,
-            'reference': 'complex128 Tsit5 oracle on the identical accepted step sequence',
-            'vs_adaptive_fp64_max_frobenius': max(errs_ad),
-            'note': 'the FP64 oracle with its own adaptive steps differs from a tight-tolerance solve by the '
-            'same order at rtol=atol=1e-6 (stiff, stability-limited steps)',
-        }
-
-    lat_sorted = sorted(plain_lat)
-    out = {
-        'metric': 'solves/s + p50/p99 latency, batched/served vs plain GPU mesolve at <=1e-5 err',
-        'value': solves_per_s,
-        'unit': 'solves/s',
-        'n_gpus': 1,
-        'steps': args.steps,
-        'warmup': args.warmup,
-        'ms_per_step': ms_per_step,
-        'higher_is_better': True,
-        'scaling': 'weak',
-        'vs_baseline': None,
-        'dtype': 'c64 (fp32 MFMA), fp64 state/norms',
-        'data': 'synthetic',
-        'config': {
-            'workload': 'batched dense Tsit5, two-photon driven Kerr oscillator n=64, 2 jump ops, B=256, 201 tsave on [0,10], rtol=atol=1e-6',
-            'global_batch': args.batch,
-            'n': n,
-            'jump_ops': K,
-            'parallelism': 'one persistent workgroup per instance',
-        },
-        'plain_solves_per_s': plain_sps,
-        'batched_vs_plain': solves_per_s / plain_sps if plain_sps else None,
-        'latency_s': {
-            'batched_p50': float(np.percentile(walls, 50)),
-            'batched_p99': float(np.percentile(walls, 99)),
-            'plain_p50': float(np.percentile(lat_sorted, 50)) if lat_sorted else None,
-            'plain_p99': float(np.percentile(lat_sorted, 99)) if lat_sorted else None,
-        },
-        'steps_per_instance': {
-            'mean': float(np.mean(res['nsteps'])),
-            'max': int(np.max(res['nsteps'])),
-            'rejected_mean': float(np.mean(res['nrejected'])),
-        },
-        'roofline': {
-            'bound': 'mfma',
-            'achieved': achieved,
-            'peak': FP32_MFMA_PEAK_TFLOPS,
-            'unit': 'TFLOP/s',
-            'frac': achieved / FP32_MFMA_PEAK_TFLOPS,
-            'traffic': KERR_TRAFFIC_BYTES if (args.batch == 256 and plan.uniform_np == 64) else None,
-            'kernel': 'dqh::k_tsit5_res<64>' if plan.uniform_np == 64 else 'dqh::k_tsit5',
-            'kernel_avg_ms': float(np.mean(kern_ms)),
-            'flops_per_launch': flops,
-        },
-        'error_vs_fp64': err,
-        'cpu_baseline': cpu,
-    }
+    # worker processes for the CPU restatement, forked before the GPU is touched
+    pool = make_cpu_pool(cpu_workers(args.cpu_workers)) if args.cpu_sample > 0 else None
+    try:
+        out, finish = bench_kerr(args, pool)
+        serving_jobs = []
+        if not args.no_secondary:
+            sec = {}
+            for name, fn in (('c1_plain_latency', secondary_c1), ('c3_dia_rouchon1', secondary_c3),
+                             ('c4_serving', lambda: secondary_c4(serving_jobs)), ('c5_adjoint', secondary_c5)):
+                t0 = time.perf_counter()
+                sec[name] = fn()
+                sec[name]['bench_wall_s'] = time.perf_counter() - t0
+                print(f'# {name}: {json.dumps(sec[name])}', file=sys.stderr, flush=True)
+            out['secondary'] = sec
+        # CPU work after every GPU measurement: replays, baseline timings, serving errors
+        finish(pool)
+        if serving_jobs and pool is not None:
+            errs = pool.map(_task_replay, serving_jobs, chunksize=1)
+            out['secondary']['c4_serving']['error_vs_fp64'] = {
+                'max_frobenius': max(e[0] for e in errs),
+                'max_trace_distance': max(e[1] for e in errs),
+                'requests_checked': len(errs),
+                'reference': 'complex128 Tsit5 oracle on each sampled request\'s accepted steps',
+            }
+    finally:
+        if pool is not None:
+            pool.terminate()
     print(json.dumps(out), flush=True)
     return 0
 
