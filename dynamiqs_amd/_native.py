@@ -94,6 +94,7 @@ SYMBOLS = {
     'dqh_sesolve_tsit5': (ctypes.c_int, [_P, _I64, _I64, _P]),
     'dqh_expm_work_bytes': (_I64, [_I64, _I64, _I64]),
     'dqh_expm_propagators': (ctypes.c_int, [_P, _P, _P, _I64, _I64, _I64, _I64, _I64, _P]),
+    'dqh_expm_propagators_noalloc': (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _I64, _I64, _P]),
     'dqh_cgemm_batched': (ctypes.c_int, [_P, _I64, _P, _I64, _I64, _P, _I64, _I64, _I64, _P]),
     'dqh_adjoint_pwc_quadrature': (ctypes.c_int, [_I64, _I64, _I64, _I64, _I64, _P, _P, _P, _P, _P, _P, _P]),
     'dqh_adjoint_pwc_quadrature_dia': (ctypes.c_int, [_I64, _I64, _I64, _I64, _P, _P, _P, _P, _P, _P, _P]),

@@ -12,6 +12,7 @@
 // into the store.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 
 #include "dqh.h"
@@ -123,22 +124,21 @@ extern "C" int64_t dqh_expm_work_bytes(int64_t nchain, int64_t nint, int64_t N) 
   return 2 * nchain * nint * N * N * 8;
 }
 
-extern "C" int dqh_expm_propagators(const void *X, void *work, void *U, int64_t nchain, int64_t nint, int64_t N,
-                                    int64_t m, int64_t s, void *stream) {
-  using namespace dqh;
-  if (!X || !work || !U || nchain <= 0 || nint <= 0 || N <= 0 || N % 16 != 0 || N > 4096 || m < 1 || m > 30 ||
-      s < 0 || s > 60 || nchain * nint > (int64_t)1 << 31)
-    return fail(DQH_BAD_ARG, "dqh_expm_propagators: bad arguments");
-  hipStream_t st = (hipStream_t)stream;
+namespace dqh {
+namespace ex {
+
+// Taylor + scaling-and-squaring exponentials and the chained products, with the
+// two work matrices of every interval at w0 and w1 (nchain * nint * N * N
+// complex64 elements each). Every element of w0 / w1 is written before it is
+// read: k_eye fills w0, each GEMM fills its whole output.
+int expm_run(const float2 *x, float2 *w0, float2 *w1, float2 *u, int64_t nchain, int64_t nint, int64_t N, int64_t m,
+             int64_t s, hipStream_t st) {
   const int64_t nn = N * N, nmat = nchain * nint;
-  const float2 *x = (const float2 *)X;
-  float2 *w0 = (float2 *)work, *w1 = w0 + nmat * nn;
-  float2 *u = (float2 *)U;
   const unsigned fill_blocks = (unsigned)(nmat * nn / 256 + 1 < 8192 ? nmat * nn / 256 + 1 : 8192);
-  hipLaunchKernelGGL(ex::k_eye, dim3(fill_blocks), dim3(256), 0, st, w0, (int)N, nmat);
+  hipLaunchKernelGGL(k_eye, dim3(fill_blocks), dim3(256), 0, st, w0, (int)N, nmat);
   // Horner: T <- I + (X / j) T, j = m .. 1
   for (int64_t j = m; j >= 1; --j) {
-    const int rc = ex::gemm(x, nn, w0, nn, w1, nn, (int)N, (float)(1.0 / (double)j), 1, nmat, st);
+    const int rc = gemm(x, nn, w0, nn, w1, nn, (int)N, (float)(1.0 / (double)j), 1, nmat, st);
     if (rc != DQH_OK) return rc;
     float2 *t = w0;
     w0 = w1;
@@ -146,23 +146,89 @@ extern "C" int dqh_expm_propagators(const void *X, void *work, void *U, int64_t 
   }
   // squarings: T <- T T
   for (int64_t i = 0; i < s; ++i) {
-    const int rc = ex::gemm(w0, nn, w0, nn, w1, nn, (int)N, 1.f, 0, nmat, st);
+    const int rc = gemm(w0, nn, w0, nn, w1, nn, (int)N, 1.f, 0, nmat, st);
     if (rc != DQH_OK) return rc;
     float2 *t = w0;
     w0 = w1;
     w1 = t;
   }
   // chains: U[b][0] = E[b][0]; U[b][k] = E[b][k] U[b][k-1]
-  hipLaunchKernelGGL(ex::k_first, dim3((unsigned)(nchain * nn / 256 + 1 < 8192 ? nchain * nn / 256 + 1 : 8192)), dim3(256),
+  hipLaunchKernelGGL(k_first, dim3((unsigned)(nchain * nn / 256 + 1 < 8192 ? nchain * nn / 256 + 1 : 8192)), dim3(256),
                      0, st, w0, u, nn, nint * nn, nchain);
   for (int64_t k = 1; k < nint; ++k) {
-    const int rc = ex::gemm(w0 + k * nn, nint * nn, u + (k - 1) * nn, nint * nn, u + k * nn, nint * nn, (int)N, 1.f, 0,
-                            nchain, st);
+    const int rc = gemm(w0 + k * nn, nint * nn, u + (k - 1) * nn, nint * nn, u + k * nn, nint * nn, (int)N, 1.f, 0,
+                        nchain, st);
     if (rc != DQH_OK) return rc;
   }
   const hipError_t err = hipGetLastError();
-  if (err != hipSuccess) return fail(DQH_HIP_ERROR, std::string("dqh_expm_propagators: ") + hipGetErrorString(err));
+  if (err != hipSuccess) return fail(DQH_HIP_ERROR, std::string("dqh Expm: ") + hipGetErrorString(err));
   return DQH_OK;
+}
+
+bool expm_args_ok(const void *X, const void *U, int64_t nchain, int64_t nint, int64_t N, int64_t m, int64_t s) {
+  return X && U && nchain > 0 && nint > 0 && N > 0 && N % 16 == 0 && N <= 4096 && m >= 1 && m <= 30 && s >= 0 &&
+         s <= 60 && nchain * nint <= ((int64_t)1 << 31);
+}
+
+// Work memory of dqh_expm_propagators_noalloc: a file-scope device array, one copy
+// per GPU, held until the process ends. Host code never passes the array's name
+// to a kernel (on the host it is a stand-in, not a device address): each device's
+// address comes from hipGetSymbolAddress on the first call made with that device
+// current, and only that address is used.
+constexpr int64_t kNoAllocBytes = DQH_EXPM_NOALLOC_MAX_BYTES;
+__device__ __align__(256) float2 scratch[kNoAllocBytes / (int64_t)sizeof(float2)];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of ``scratch`` on the current device (looked up once per device).
+int scratch_ptr(float2 **out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(DQH_HIP_ERROR, std::string("dqh_expm_propagators_noalloc: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return fail(DQH_UNSUPPORTED, "dqh_expm_propagators_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(DQH_HIP_ERROR, std::string("dqh_expm_propagators_noalloc: hipGetSymbolAddress: ") +
+                                     hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = (float2 *)g_scratch_ptr[dev];
+  return DQH_OK;
+}
+
+}  // namespace ex
+}  // namespace dqh
+
+extern "C" int dqh_expm_propagators(const void *X, void *work, void *U, int64_t nchain, int64_t nint, int64_t N,
+                                    int64_t m, int64_t s, void *stream) {
+  using namespace dqh;
+  if (!work || !ex::expm_args_ok(X, U, nchain, nint, N, m, s))
+    return fail(DQH_BAD_ARG, "dqh_expm_propagators: bad arguments");
+  const int64_t nmat = nchain * nint;
+  float2 *w0 = (float2 *)work, *w1 = w0 + nmat * N * N;
+  return ex::expm_run((const float2 *)X, w0, w1, (float2 *)U, nchain, nint, N, m, s, (hipStream_t)stream);
+}
+
+extern "C" int dqh_expm_propagators_noalloc(const void *X, void *U, int64_t nchain, int64_t nint, int64_t N,
+                                            int64_t m, int64_t s, void *stream) {
+  using namespace dqh;
+  if (!ex::expm_args_ok(X, U, nchain, nint, N, m, s))
+    return fail(DQH_BAD_ARG, "dqh_expm_propagators_noalloc: bad arguments");
+  const int64_t need = dqh_expm_work_bytes(nchain, nint, N);
+  if (need > ex::kNoAllocBytes)
+    return fail(DQH_BAD_SHAPE, "dqh_expm_propagators_noalloc: needs " + std::to_string(need) +
+                                   " bytes of work memory, above its fixed " + std::to_string(ex::kNoAllocBytes) +
+                                   " (2 * nchain * nint * N * N * 8 must not exceed it)");
+  float2 *w0 = nullptr;
+  const int rc = ex::scratch_ptr(&w0);
+  if (rc != DQH_OK) return rc;
+  float2 *w1 = w0 + nchain * nint * N * N;
+  return ex::expm_run((const float2 *)X, w0, w1, (float2 *)U, nchain, nint, N, m, s, (hipStream_t)stream);
 }
 
 extern "C" int dqh_cgemm_batched(const void *A, int64_t sA, const void *B, int64_t sB, int64_t bdiv, void *C,

@@ -308,6 +308,20 @@ int64_t dqh_expm_work_bytes(int64_t nchain, int64_t nint, int64_t N);
 int dqh_expm_propagators(const void *X, void *work, void *U, int64_t nchain, int64_t nint, int64_t N, int64_t m,
                          int64_t s, void *stream);
 
+/* The same exponentials and products with no allocation and no work argument:
+ * the work memory is a fixed device array of DQH_EXPM_NOALLOC_MAX_BYTES bytes
+ * inside the library (one per GPU, held until the process ends), so
+ * dqh_expm_work_bytes(nchain, nint, N) must not exceed it (DQH_BAD_SHAPE
+ * otherwise). No counterpart in the reference: this entry point is for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc). It calls no hipMalloc / hipFree. Every call on
+ * a GPU uses that GPU's one array: calls that overlap on one GPU (different streams
+ * or threads) race and must be serialised by the caller. Results equal
+ * dqh_expm_propagators' for the same inputs. */
+#define DQH_EXPM_NOALLOC_MAX_BYTES (64ll << 20)
+int dqh_expm_propagators_noalloc(const void *X, void *U, int64_t nchain, int64_t nint, int64_t N, int64_t m,
+                                 int64_t s, void *stream);
+
 /* Batched complex64 product C[b] = A[b] B[b / bdiv] for b < nbatch (N x N row-major,
  * N % 16 == 0, strides in elements; C must not alias A or B), the GEMM of
  * dqh_expm_propagators. The Expm solves apply the propagators to a block of initial
