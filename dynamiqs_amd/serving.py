@@ -215,13 +215,16 @@ def pack_and_launch(reqs: list, stream, grid: bool = False):
     # staging: every request's inputs, then the descriptor array: one copy
     staging_t = torch.zeros(n_in + _align(dsz), dtype=torch.uint8, pin_memory=True)
     staging = staging_t.numpy()
-    arena = torch.empty(n_in + _align(dsz), dtype=torch.uint8, device=dev)
-    scratch = torch.empty(max(int(w_off[-1]), 256), dtype=torch.uint8, device=dev)
+    # device buffers belong to the launch stream: the allocator's reuse and the
+    # zero fill of device outputs are then ordered with the launch itself
+    with torch.cuda.stream(stream):
+        arena = torch.empty(n_in + _align(dsz), dtype=torch.uint8, device=dev)
+        scratch = torch.empty(max(int(w_off[-1]), 256), dtype=torch.uint8, device=dev)
+        oarena = torch.zeros(int(out_off[-1]), dtype=torch.uint8, device=dev) if grid else None
     descs = staging[n_in : n_in + dsz].view(_native.ME_INST_DTYPE)
     lib = _native.lib()
     c_p = _native.ctypes.c_void_p
     if grid:
-        oarena = torch.zeros(int(out_off[-1]), dtype=torch.uint8, device=dev)
         out_base, host_out = oarena.data_ptr(), None
     else:
         oarena = torch.zeros(int(out_off[-1]), dtype=torch.uint8, pin_memory=True)

@@ -120,9 +120,9 @@ def test_requests_retire_before_their_launch_ends():
 
     rng = np.random.default_rng(8)
     short = _serving_request(rng, 40, 1, 0.5, nt=5)
-    long_ = _serving_request(rng, 64, 3, 20.0)
+    long_ = _serving_request(rng, 64, 3, 200.0)
     with Server(memory_budget=1 << 30) as srv:
-        f_long = srv.submit(*long_, rtol=1e-7, atol=1e-9)
+        f_long = srv.submit(*long_, rtol=1e-8, atol=1e-10)
         f_short = srv.submit(*short, rtol=1e-5)
         r_short = f_short.result(timeout=120)
         r_long = f_long.result(timeout=120)
