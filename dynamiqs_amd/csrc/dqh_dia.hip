@@ -14,7 +14,11 @@
 //   K3  rho += E + dM X + X dM^dag + dM X dM^dag + sum_k A_k X A_k^dag
 //       (dM = M0 - I = -i dt Hnh, A_k = sqrt(dt) L_k): one fused stencil sweep
 //       that reads X at the few neighbours the diagonals name and hi, lo, E at the
-//       element, and writes hi, lo in place (40 bytes of HBM per element).
+//       element, and writes hi, lo in place (48 bytes of HBM per element).
+// When S decouples into small index sets, K1 and K2 are instead the two memory-bound
+// passes of the block form (k_blk_left / k_blk_right, below), with the same D1, E
+// and X. K3 gives each thread one element of up to four instances, which share
+// the operators and so the coefficient loads.
 // The banded solve T^dag x = s runs over blocks of nb rows from the bottom:
 //   x_i = V_i s_i - V_i U_{i,i+1} x_{i+1},  V_i = U_ii^{-1},  U = T^dag,
 // i.e. d_i = x_i - s_i = V'_i s_i + Wn_i x_{i+1} with V' = V - I, Wn = -V U_{i,i+1}
@@ -211,6 +215,167 @@ __global__ __launch_bounds__(kThreads) void k_lsolve(int mode, int64_t n, const 
   }
 }
 
+// ---------------------------------------------------------------- block form
+// When S couples indices only inside small sets, T (lower Cholesky factor of S) is
+// block-diagonal on them and the normalisation is two passes over the matrix:
+//   K1  D1[r][c] = sum_{k in set(r)} (T_b^-dag - I)[loc r][loc k] hi[k][c]
+//       (rows mixed within a set; one workgroup per set and 256 columns, each
+//       thread one column: the d values of its column in the set's rows are loaded
+//       into registers, coalesced across the workgroup, then the d outputs written),
+//   K2  Y = hi + D1, E[r][c] = D1[r][c] + sum_{k in set(c)} Y[r][k] (T_b^-1 - I)[loc k][loc c],
+//       X = Y + (that sum) written over D1 (columns mixed within a set; one
+//       workgroup per strip of W rows, Y of the strip in LDS, each thread one
+//       column and the W rows),
+// so that hi + E = (hi + D1) T^-1 = T^-dag rho T^-1, as in the banded form. Both
+// passes read and write each element once (memory bound); a set of d indices costs
+// d^2 complex multiply-adds per row or column instead of the 64 x 128 of a block
+// row of the banded recurrence.
+struct Blocks {
+  const int32_t *of, *loc, *start, *idx;
+  const int64_t *mat;
+};
+
+// K1 for sets of at most D indices. grid: (ceil(n / 256), nblk, ninst).
+template <int D>
+__global__ __launch_bounds__(256) void k_blk_left(int64_t n, Blocks bk, const float2 *__restrict__ Lm,
+                                                  const float2 *__restrict__ hi, float2 *__restrict__ D1) {
+  const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (c >= n) return;
+  const size_t nn = (size_t)n * n;
+  const size_t inst = blockIdx.z;
+  const int b = blockIdx.y;
+  const int s0 = bk.start[b], d = bk.start[b + 1] - s0;
+  const int32_t *rows = bk.idx + s0;
+  const float2 *H = hi + inst * nn + c;
+  float2 *O = D1 + inst * nn + c;
+  const float2 *L = Lm + bk.mat[b];  // d x d, row l zero left of l
+  float2 x[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) x[k] = k < d ? H[(size_t)rows[k] * n] : make_float2(0.f, 0.f);
+  for (int l = 0; l < d; ++l) {
+    const float2 *Lr = L + (size_t)l * d;
+    float ar = 0.f, ai = 0.f;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      if (k >= l && k < d) {
+        const float2 m = Lr[k];
+        ar += m.x * x[k].x - m.y * x[k].y;
+        ai += m.x * x[k].y + m.y * x[k].x;
+      }
+    }
+    O[(size_t)rows[l] * n] = make_float2(ar, ai);
+  }
+}
+
+template <int W>
+__global__ __launch_bounds__(256) void k_blk_right(int64_t n, Blocks bk, const float2 *__restrict__ Rm,
+                                                   const float2 *__restrict__ hi, float2 *D1X,
+                                                   float2 *__restrict__ E) {
+  extern __shared__ float2 ystrip[];  // W x n: Y[r0 + r][c] at ystrip[r * n + c]
+  const size_t nn = (size_t)n * n;
+  const size_t inst = blockIdx.y;
+  const int r0 = blockIdx.x * W;
+  const float2 *H = hi + inst * nn;
+  float2 *DX = D1X + inst * nn;
+  float2 *Eo = E + inst * nn;
+  for (int e = threadIdx.x; e < W * n; e += blockDim.x) {
+    const size_t g = (size_t)r0 * n + e;
+    const float2 h = H[g], d1 = DX[g];
+    ystrip[e] = make_float2(h.x + d1.x, h.y + d1.y);
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < n; c += blockDim.x) {
+    const int b = bk.of[c], l = bk.loc[c];
+    const int s0 = bk.start[b], d = bk.start[b + 1] - s0;
+    const float2 *Rc = Rm + bk.mat[b] + (int64_t)l * d;  // column l of T_b^-1 - I: zero above l
+    float ar[W], ai[W];
+#pragma unroll
+    for (int r = 0; r < W; ++r) ar[r] = ai[r] = 0.f;
+    for (int k = l; k < d; ++k) {
+      const float2 m = Rc[k];
+      const int src = bk.idx[s0 + k];
+#pragma unroll
+      for (int r = 0; r < W; ++r) {
+        const float2 y = ystrip[(int64_t)r * n + src];
+        ar[r] += y.x * m.x - y.y * m.y;
+        ai[r] += y.x * m.y + y.y * m.x;
+      }
+    }
+    // each (row, column) of the strip is this thread's alone: D1 is read here once
+    // more (for E) and then overwritten with X
+#pragma unroll
+    for (int r = 0; r < W; ++r) {
+      const size_t g = (size_t)(r0 + r) * n + c;
+      const float2 d1 = DX[g];
+      const float2 y = ystrip[(int64_t)r * n + c];
+      Eo[g] = make_float2(d1.x + ar[r], d1.y + ai[r]);
+      DX[g] = make_float2(y.x + ar[r], y.y + ai[r]);
+    }
+  }
+}
+
+// K1 and K2 fused for sets of at most D indices and n = blockDim.x columns: one
+// workgroup per (set, instance), each thread one column. The d values of its
+// column in the set's rows stay in registers; for each row l of the set the thread
+// forms D1 and Y = hi + D1 at its column, the workgroup publishes the row of Y in
+// LDS (double-buffered, one barrier per row), and each thread mixes the columns of
+// its own column's set from it and writes E and X. HBM: hi read once, E and X
+// written once (24 bytes per element against 48 for the two passes).
+template <int D>
+__global__ __launch_bounds__(1024) void k_blk_norm(int64_t n, Blocks bk, const float2 *__restrict__ Lm,
+                                                   const float2 *__restrict__ Rm, const float2 *__restrict__ hi,
+                                                   float2 *__restrict__ Xo, float2 *__restrict__ Eo) {
+  extern __shared__ float2 yrow[];  // 2 x n
+  const int c = threadIdx.x;        // blockDim.x == n
+  const size_t nn = (size_t)n * n;
+  const size_t inst = blockIdx.y;
+  const int b = blockIdx.x;
+  const int s0 = bk.start[b], d = bk.start[b + 1] - s0;
+  const int32_t *rows = bk.idx + s0;
+  const float2 *L = Lm + bk.mat[b];  // d x d, row l zero left of l
+  const float2 *H = hi + inst * nn + c;
+  float2 *X = Xo + inst * nn + c;
+  float2 *E = Eo + inst * nn + c;
+  // the set of this thread's column: its column of T_cb^-1 - I and where the set's
+  // indices are
+  const int cb = bk.of[c], cl = bk.loc[c];
+  const int cs0 = bk.start[cb], cd = bk.start[cb + 1] - cs0;
+  const float2 *Rc = Rm + bk.mat[cb] + (int64_t)cl * cd;  // zero above cl
+  const int32_t *cidx = bk.idx + cs0;
+  float2 x[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) x[k] = k < d ? H[(size_t)rows[k] * n] : make_float2(0.f, 0.f);
+#pragma unroll
+  for (int l = 0; l < D; ++l) {
+    if (l < d) {
+      const float2 *Lr = L + (size_t)l * d;
+      float dr = 0.f, di = 0.f;
+#pragma unroll
+      for (int k = l; k < D; ++k) {
+        if (k < d) {
+          const float2 m = Lr[k];
+          dr += m.x * x[k].x - m.y * x[k].y;
+          di += m.x * x[k].y + m.y * x[k].x;
+        }
+      }
+      const float yr = x[l].x + dr, yi = x[l].y + di;
+      float2 *yb = yrow + (l & 1) * n;
+      yb[c] = make_float2(yr, yi);
+      __syncthreads();
+      float ar = 0.f, ai = 0.f;
+      for (int k = cl; k < cd; ++k) {
+        const float2 m = Rc[k];
+        const float2 y = yb[cidx[k]];
+        ar += y.x * m.x - y.y * m.y;
+        ai += y.x * m.y + y.y * m.x;
+      }
+      const size_t g = (size_t)rows[l] * n;
+      E[g] = make_float2(dr + ar, di + ai);
+      X[g] = make_float2(yr + ar, yi + ai);
+    }
+  }
+}
+
 // Save times inside one step: up to kMaxSaves interpolated states
 // old + theta (new - old) written by one sweep; update = 0 leaves the state as it
 // was (a step holding more save times runs the sweep once per group of saves).
@@ -243,24 +408,184 @@ __device__ __forceinline__ void update_elem(size_t e, int i, int j, int64_t n_tr
   }
 }
 
+// The Kraus increment dM X + X dM^dag + dM X dM^dag + sum_{s>=1} A_s X A_s^dag at
+// (i, j) (op 0 = dM, ops 1.. = A_s), X(p, q) = X[i + p][j + q]. The column-side
+// coefficients conj(A[j][j + q]) of each operator are loaded once into registers
+// (at most DQH_DIA_MAXOFF per operator) and reused by every row offset; the
+// row-side ones are the same for the whole row.
+template <class XA>
+__device__ __forceinline__ float2 kraus_stencil(const XA &X, const dqh_dia_ops &ops, int64_t n, int i, int j) {
+  float ar = 0.f, ai = 0.f;
+  for (int s = 0; s < ops.nops; ++s) {
+    const float2 *A = diat::cgp<float2>(ops.diags[s]);
+    const int no = ops.noff[s];
+    float2 mb[DQH_DIA_MAXOFF];
+#pragma unroll
+    for (int v = 0; v < DQH_DIA_MAXOFF; ++v)
+      mb[v] = v < no ? A[(size_t)v * n + diat::clampi((int64_t)j + ops.off[s][v], n)] : make_float2(0.f, 0.f);
+    for (int u = 0; u < no; ++u) {
+      const int p = ops.off[s][u];
+      const float2 ma = A[(size_t)u * n + diat::clampi((int64_t)i + p, n)];
+      // z = (X A^dag)[i + p][j]; for op 0 also (X)[i + p][j] for the one-sided dM X
+      float zr = 0.f, zi = 0.f;
+#pragma unroll
+      for (int v = 0; v < DQH_DIA_MAXOFF; ++v) {
+        if (v < no) {
+          const float2 x = X(p, ops.off[s][v]);
+          zr += x.x * mb[v].x + x.y * mb[v].y;
+          zi += x.y * mb[v].x - x.x * mb[v].y;
+        }
+      }
+      if (s == 0) {
+        const float2 x0 = X(p, 0);
+        zr += x0.x;
+        zi += x0.y;
+      }
+      ar += ma.x * zr - ma.y * zi;
+      ai += ma.x * zi + ma.y * zr;
+    }
+    if (s == 0) {  // X dM^dag
+#pragma unroll
+      for (int v = 0; v < DQH_DIA_MAXOFF; ++v) {
+        if (v < no) {
+          const float2 x = X(0, ops.off[s][v]);
+          ar += x.x * mb[v].x + x.y * mb[v].y;
+          ai += x.y * mb[v].x - x.x * mb[v].y;
+        }
+      }
+    }
+  }
+  return make_float2(ar, ai);
+}
+
+// row_elem (diat) with an XCD-aware workgroup order: consecutive workgroup ids go
+// round-robin over the 8 XCDs, so id b takes item (b % 8) * (nb / 8) + b / 8 and
+// each XCD sweeps a contiguous band of rows; the neighbour rows the stencil reads
+// (a few dozen rows away) then stay in that XCD's L2.
+__device__ __forceinline__ bool row_elem_xcd(int64_t n, int &i, int &j) {
+  const unsigned cpr = (unsigned)((n + 255) >> 8);
+  const unsigned nb = gridDim.x;
+  unsigned b = blockIdx.x;
+  if ((nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+  const unsigned r = b / cpr;
+  i = (int)r;
+  j = (int)((b - r * cpr) * 256u + threadIdx.x);
+  return j < n;
+}
+
+// X at one element (i, j) of IB instances (instance stride nn; nib of them valid,
+// the rest read the last valid instance and are discarded by the caller). A
+// neighbour outside the matrix reads as zero through a clamped address and a zero
+// factor, so the loads of all IB instances issue back to back without branches.
+template <int IB>
+struct XBatch {
+  const float2 *base;
+  size_t nn;
+  int64_t n;
+  int i, j, nib;
+  __device__ __forceinline__ void load(int p, int q, float2 (&v)[IB]) const {
+    const int64_t a = (int64_t)i + p, b = (int64_t)j + q;
+    const bool ok = a >= 0 && a < n && b >= 0 && b < n;
+    const size_t o = ok ? (size_t)a * n + b : 0;
+    const float m = ok ? 1.f : 0.f;
+#pragma unroll
+    for (int k = 0; k < IB; ++k) {
+      const float2 x = base[(size_t)(k < nib ? k : nib - 1) * nn + o];
+      v[k] = make_float2(x.x * m, x.y * m);
+    }
+  }
+};
+
+// kraus_stencil for IB instances that share the operators: the coefficient loads
+// are made once for all of them and every X load is IB independent loads, so a
+// thread keeps IB times more memory requests in flight per round trip.
+template <int IB>
+__device__ __forceinline__ void kraus_stencil_batch(const XBatch<IB> &X, const dqh_dia_ops &ops, int64_t n, int i,
+                                                    int j, float (&ar)[IB], float (&ai)[IB]) {
+#pragma unroll
+  for (int k = 0; k < IB; ++k) ar[k] = ai[k] = 0.f;
+  for (int s = 0; s < ops.nops; ++s) {
+    const float2 *A = diat::cgp<float2>(ops.diags[s]);
+    const int no = ops.noff[s];
+    float2 mb[DQH_DIA_MAXOFF];
+#pragma unroll
+    for (int v = 0; v < DQH_DIA_MAXOFF; ++v)
+      mb[v] = v < no ? A[(size_t)v * n + diat::clampi((int64_t)j + ops.off[s][v], n)] : make_float2(0.f, 0.f);
+    for (int u = 0; u < no; ++u) {
+      const int p = ops.off[s][u];
+      const float2 ma = A[(size_t)u * n + diat::clampi((int64_t)i + p, n)];
+      float zr[IB], zi[IB];
+      float2 x[IB];
+      if (s == 0) {  // the one-sided dM X term
+        X.load(p, 0, x);
+#pragma unroll
+        for (int k = 0; k < IB; ++k) zr[k] = x[k].x, zi[k] = x[k].y;
+      } else {
+#pragma unroll
+        for (int k = 0; k < IB; ++k) zr[k] = zi[k] = 0.f;
+      }
+#pragma unroll
+      for (int v = 0; v < DQH_DIA_MAXOFF; ++v) {
+        if (v < no) {
+          X.load(p, ops.off[s][v], x);
+#pragma unroll
+          for (int k = 0; k < IB; ++k) {
+            zr[k] += x[k].x * mb[v].x + x[k].y * mb[v].y;
+            zi[k] += x[k].y * mb[v].x - x[k].x * mb[v].y;
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < IB; ++k) {
+        ar[k] += ma.x * zr[k] - ma.y * zi[k];
+        ai[k] += ma.x * zi[k] + ma.y * zr[k];
+      }
+    }
+    if (s == 0) {  // X dM^dag
+#pragma unroll
+      for (int v = 0; v < DQH_DIA_MAXOFF; ++v) {
+        if (v < no) {
+          float2 x[IB];
+          X.load(0, ops.off[s][v], x);
+#pragma unroll
+          for (int k = 0; k < IB; ++k) {
+            ar[k] += x[k].x * mb[v].x + x[k].y * mb[v].y;
+            ai[k] += x[k].y * mb[v].x - x[k].x * mb[v].y;
+          }
+        }
+      }
+    }
+  }
+}
+
 // Fused Kraus sweep: increment E + dM X + X dM^dag + dM X dM^dag + sum A X A^dag at
 // every element, X (the FP32 normalised state) read at the neighbours the
 // diagonals name through the caches (any offsets, any n), then the FP64 update of
-// the hi / lo planes and the saves. One thread per element, rows coalesced.
-__global__ __launch_bounds__(256) void k_kraus(int64_t n, int64_t n_true, dqh_dia_ops ops,
+// the hi / lo planes and the saves. One thread per element of IB instances
+// (blockIdx.y * IB onwards), rows coalesced.
+template <int IB>
+__global__ __launch_bounds__(256) void k_kraus(int64_t n, int64_t n_true, int64_t ninst, dqh_dia_ops ops,
                                                const float2 *__restrict__ X, const float2 *__restrict__ E,
                                                float2 *__restrict__ hi, float2 *__restrict__ lo,
                                                float2 *__restrict__ states, int64_t nsave, Saves sv) {
   const size_t nn = (size_t)n * n;
-  const size_t inst = blockIdx.y;
+  const int64_t inst0 = (int64_t)blockIdx.y * IB;
+  const int nib = (int)(ninst - inst0 < IB ? ninst - inst0 : IB);
   int i, j;
-  if (!diat::row_elem(n, i, j)) return;
-  const size_t e = inst * nn + (size_t)i * n + j;
-  const diat::GlobalX Xg{X + inst * nn, nullptr, n, i, j};
-  const float2 acc = diat::stencil(Xg, ops, true, n, i, j);
-  const float2 eij = E[e];
-  update_elem(e, i, j, n_true, (double)eij.x + acc.x, (double)eij.y + acc.y, hi, lo,
-              states + inst * nsave * n_true * n_true, nsave, sv);
+  if (!row_elem_xcd(n, i, j)) return;
+  const XBatch<IB> Xb{X + inst0 * nn, nn, n, i, j, nib};
+  float ar[IB], ai[IB];
+  kraus_stencil_batch<IB>(Xb, ops, n, i, j, ar, ai);
+#pragma unroll
+  for (int k = 0; k < IB; ++k) {
+    if (k < nib) {
+      const int64_t inst = inst0 + k;
+      const size_t e = inst * nn + (size_t)i * n + j;
+      const float2 eij = E[e];
+      update_elem(e, i, j, n_true, (double)eij.x + ar[k], (double)eij.y + ai[k], hi, lo,
+                  states + inst * nsave * n_true * n_true, nsave, sv);
+    }
+  }
 }
 
 // LDS-tiled form of k_kraus (all offsets within the halo, n % 64 == 0): one
@@ -284,7 +609,7 @@ __global__ __launch_bounds__(diat::kThreads) void k_kraus_tiled(int64_t n, int64
   for (int m = 0; m < diat::kRowsPerThread; ++m) {
     const int ii = (threadIdx.x >> 6) + (diat::kThreads / diat::kTJ) * m;
     const int i = i0 + ii, j = j0 + jj;
-    const float2 acc = diat::stencil(diat::LdsX<HALO>::at(sx, ii, jj), ops, true, n, i, j);
+    const float2 acc = kraus_stencil(diat::LdsX<HALO>::at(sx, ii, jj), ops, n, i, j);
     const size_t e = inst * nn + (size_t)i * n + j;
     const float2 eij = E[e];
     update_elem(e, i, j, n_true, (double)eij.x + acc.x, (double)eij.y + acc.y, hi, lo, st_inst, nsave, sv);
@@ -360,6 +685,24 @@ extern "C" int dqh_mesolve_rouchon1_dia(const dqh_dia_rouchon1 *pb, void *stream
   if (p.solve_panel != 0 && p.solve_panel != 32 && p.solve_panel != 64)
     return fail(DQH_BAD_ARG, "dqh_mesolve_rouchon1_dia: solve_panel must be 0, 32 or 64");
   const int panel = p.solve_panel != 0 ? (int)p.solve_panel : ((n / dia::kNB) * p.ninst < 512 ? 32 : 64);
+  // block form of the normalisation: strip width so that a strip of n elements fits
+  // 64 KiB of LDS
+  const bool blocks = p.nblk > 0;
+  const int bw = n <= 1024 ? 8 : (n <= 2048 ? 4 : 2);
+  if (blocks && (n > 4096 || p.nblk > n || p.blk_maxd < 1 || p.blk_maxd > DQH_DIA_MAXBLK || p.ninst > 65535 || !p.blk_of || !p.blk_loc || !p.blk_start || !p.blk_idx || !p.blk_mat || !p.Lm[0] ||
+                 !p.Rm[0] || (p.nsets > 1 && (!p.Lm[1] || !p.Rm[1]))))
+    return fail(DQH_BAD_ARG, "dqh_mesolve_rouchon1_dia: block normalisation needs n <= 4096, 1 <= blk_maxd <= DQH_DIA_MAXBLK, ninst <= 65535 and every block array");
+  dia::Blocks bk{reinterpret_cast<const int32_t *>(p.blk_of), reinterpret_cast<const int32_t *>(p.blk_loc),
+                 reinterpret_cast<const int32_t *>(p.blk_start), reinterpret_cast<const int32_t *>(p.blk_idx),
+                 reinterpret_cast<const int64_t *>(p.blk_mat)};
+  const dim3 gblk((unsigned)(n / bw), (unsigned)p.ninst);
+  const dim3 gleft((unsigned)((n + 255) / 256), (unsigned)std::max<int64_t>(p.nblk, 1), (unsigned)p.ninst);
+  // both passes in one kernel when a workgroup can hold every column (n <= 1024)
+  // and the sets are small enough for their column values to stay in registers
+  const bool fused = blocks && n <= 1024 && p.blk_maxd <= 32;
+  const dim3 gnorm((unsigned)std::max<int64_t>(p.nblk, 1), (unsigned)p.ninst);
+  const size_t norm_lds = 2 * (size_t)n * sizeof(float2);
+  const size_t blk_lds = (size_t)n * bw * sizeof(float2);
   const dim3 gsolve((unsigned)(n / panel), (unsigned)p.ninst);
   const dim3 gtile((unsigned)((n / diat::kTI) * (n / diat::kTJ)), (unsigned)p.ninst);
   int maxoff = 0;
@@ -385,6 +728,9 @@ extern "C" int dqh_mesolve_rouchon1_dia(const dqh_dia_rouchon1 *pb, void *stream
   hipEvent_t ev[3 * 8] = {};
   for (int64_t k = 0; k < 3 * t_count; ++k)
     if (hipEventCreate(&ev[k]) != hipSuccess) return fail(DQH_HIP_ERROR, "dqh_mesolve_rouchon1_dia: hipEventCreate");
+  // global sweep: IB instances per thread (4 from a batch of 4 onwards)
+  const int kib = p.ninst >= 4 ? 4 : (p.ninst >= 2 ? 2 : 1);
+  const dim3 gkr(gel.x, (unsigned)((p.ninst + kib - 1) / kib));
   auto sweep = [&](const dqh_dia_ops &ops, const dia::Saves &sv) {
     if (halo == 8)
       hipLaunchKernelGGL(dia::k_kraus_tiled<8>, gtile, dim3(diat::kThreads), 0, st, n, p.n_true, ops, X, E, hi, lo,
@@ -395,8 +741,15 @@ extern "C" int dqh_mesolve_rouchon1_dia(const dqh_dia_rouchon1 *pb, void *stream
     else if (halo == 32)
       hipLaunchKernelGGL(dia::k_kraus_tiled<32>, gtile, dim3(diat::kThreads), 0, st, n, p.n_true, ops, X, E, hi, lo,
                          states, p.nsave, sv);
+    else if (kib == 4)
+      hipLaunchKernelGGL(dia::k_kraus<4>, gkr, dim3(256), 0, st, n, p.n_true, p.ninst, ops, X, E, hi, lo, states,
+                         p.nsave, sv);
+    else if (kib == 2)
+      hipLaunchKernelGGL(dia::k_kraus<2>, gkr, dim3(256), 0, st, n, p.n_true, p.ninst, ops, X, E, hi, lo, states,
+                         p.nsave, sv);
     else
-      hipLaunchKernelGGL(dia::k_kraus, gel, dim3(256), 0, st, n, p.n_true, ops, X, E, hi, lo, states, p.nsave, sv);
+      hipLaunchKernelGGL(dia::k_kraus<1>, gkr, dim3(256), 0, st, n, p.n_true, p.ninst, ops, X, E, hi, lo, states,
+                         p.nsave, sv);
   };
   for (int64_t m = 0; m < p.nsteps; ++m) {
     const int set = (m == p.nsteps - 1) ? p.nsets - 1 : 0;
@@ -406,7 +759,33 @@ extern "C" int dqh_mesolve_rouchon1_dia(const dqh_dia_rouchon1 *pb, void *stream
     const bool timed = tk >= 0 && tk < t_count;
     if (timed) hipEventRecord(ev[3 * tk], st);
     // D1 = T^-dag hi - hi into the X buffer, then E and X = hi + E (over D1)
-    if (panel == 32) {
+    if (blocks) {
+      const float2 *Lm = reinterpret_cast<const float2 *>(p.Lm[set]);
+      const float2 *Rm = reinterpret_cast<const float2 *>(p.Rm[set]);
+      if (fused) {
+        if (p.blk_maxd <= 8)
+          hipLaunchKernelGGL(dia::k_blk_norm<8>, gnorm, dim3((unsigned)n), norm_lds, st, n, bk, Lm, Rm, hi, X, E);
+        else if (p.blk_maxd <= 16)
+          hipLaunchKernelGGL(dia::k_blk_norm<16>, gnorm, dim3((unsigned)n), norm_lds, st, n, bk, Lm, Rm, hi, X, E);
+        else
+          hipLaunchKernelGGL(dia::k_blk_norm<32>, gnorm, dim3((unsigned)n), norm_lds, st, n, bk, Lm, Rm, hi, X, E);
+      } else if (p.blk_maxd <= 8)
+        hipLaunchKernelGGL(dia::k_blk_left<8>, gleft, dim3(256), 0, st, n, bk, Lm, hi, X);
+      else if (p.blk_maxd <= 16)
+        hipLaunchKernelGGL(dia::k_blk_left<16>, gleft, dim3(256), 0, st, n, bk, Lm, hi, X);
+      else if (p.blk_maxd <= 32)
+        hipLaunchKernelGGL(dia::k_blk_left<32>, gleft, dim3(256), 0, st, n, bk, Lm, hi, X);
+      else
+        hipLaunchKernelGGL(dia::k_blk_left<64>, gleft, dim3(256), 0, st, n, bk, Lm, hi, X);
+      if (fused)
+        ;
+      else if (bw == 8)
+        hipLaunchKernelGGL(dia::k_blk_right<8>, gblk, dim3(256), blk_lds, st, n, bk, Rm, hi, X, E);
+      else if (bw == 4)
+        hipLaunchKernelGGL(dia::k_blk_right<4>, gblk, dim3(256), blk_lds, st, n, bk, Rm, hi, X, E);
+      else
+        hipLaunchKernelGGL(dia::k_blk_right<2>, gblk, dim3(256), blk_lds, st, n, bk, Rm, hi, X, E);
+    } else if (panel == 32) {
       hipLaunchKernelGGL(dia::k_lsolve<32>, gsolve, dim3(dia::kThreads), 0, st, 0, n, Vp, Wn, hi, X, X);
       hipLaunchKernelGGL(dia::k_lsolve<32>, gsolve, dim3(dia::kThreads), 0, st, 1, n, Vp, Wn, hi, X, E);
     } else {

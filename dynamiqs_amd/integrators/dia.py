@@ -59,12 +59,23 @@ class DiaRouchon1(ctypes.Structure):
         ('variant', ctypes.c_int64),
         ('solve_panel', ctypes.c_int64),
         ('timing', ctypes.c_uint64),
+        ('nblk', ctypes.c_int64),
+        ('blk_maxd', ctypes.c_int64),
+        ('blk_of', ctypes.c_uint64),
+        ('blk_loc', ctypes.c_uint64),
+        ('blk_start', ctypes.c_uint64),
+        ('blk_idx', ctypes.c_uint64),
+        ('blk_mat', ctypes.c_uint64),
+        ('Lm', ctypes.c_uint64 * 2),
+        ('Rm', ctypes.c_uint64 * 2),
     ]
 
 
 SWEEP_AUTO, SWEEP_GLOBAL, SWEEP_TILED, SWEEP_TILED_H8, SWEEP_TILED_H16, SWEEP_TILED_H32 = 0, 1, 2, 3, 4, 5
 SWEEP_HALO = {SWEEP_TILED_H8: 8, SWEEP_TILED_H16: 16, SWEEP_TILED_H32: 32}
 SWEEP_SPLIT = 6  # accepted by the Rouchon1 Kraus sweep, runs its global form
+MAXBLK = 64  # largest index set of the block form of the normalisation (DQH_DIA_MAXBLK)
+MAXN_BLOCKS = 4096  # the block form's strips of n elements fit 64 KiB of LDS
 
 
 class DiaRhs(ctypes.Structure):
@@ -96,11 +107,68 @@ def half_bandwidth(S: np.ndarray) -> int:
     return max(abs(o) for o in offs) if offs else 0
 
 
-def block_solve_operators(H: np.ndarray, Ls, dt: float, n_pad: int):
-    """FP64 host precomputation of one step length: (Vp, Wn, dM diagonals, A_k diagonals)."""
+def normalisation_S(H: np.ndarray, Ls, dt: float, n_pad: int) -> np.ndarray:
+    """S = M0^dag M0 + dt sum L^dag L of one Rouchon1 step (rouchon_integrator.py:259-263),
+    padded with the identity to n_pad."""
+    n = H.shape[-1]
+    LdL = sum((L.conj().T @ L for L in Ls), np.zeros_like(H))
+    M0 = np.eye(n) - 1j * dt * (H - 0.5j * LdL)
+    Sp = np.eye(n_pad, dtype=np.complex128)
+    Sp[:n, :n] = M0.conj().T @ M0 + dt * LdL
+    return Sp
+
+
+def coupled_sets(S: np.ndarray):
+    """Index sets of the connected components of the graph with an edge i-j where
+    S[i, j] != 0 (each set ascending), or None when they do not make the block form
+    of the normalisation worthwhile: one set only, or a set above MAXBLK indices."""
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import connected_components
+
+    ncomp, labels = connected_components(csr_matrix(S != 0), directed=False)
+    if ncomp < 2:
+        return None
+    sets = [np.flatnonzero(labels == c) for c in range(ncomp)]
+    if max(len(x) for x in sets) > MAXBLK:
+        return None
+    return sets
+
+
+def block_normalisation(S: np.ndarray, sets):
+    """Device arrays of the block form: (of, loc, start, idx, mat, L, R) with, per set b
+    of d indices, L_b = T_b^-dag - I (row-major) and R_b = (T_b^-1 - I)^T (row-major),
+    T_b the lower Cholesky factor of S on the set (FP64, rounded to complex64). The
+    factor assembled from the sets' factors is lower triangular in the original order
+    and reproduces S, so it is the Cholesky factor of S."""
+    n = S.shape[0]
+    of = np.zeros(n, dtype=np.int32)
+    loc = np.zeros(n, dtype=np.int32)
+    start = np.zeros(len(sets) + 1, dtype=np.int32)
+    mat = np.zeros(len(sets) + 1, dtype=np.int64)
+    Ls, Rs = [], []
+    for b, idx in enumerate(sets):
+        d = len(idx)
+        of[idx] = b
+        loc[idx] = np.arange(d, dtype=np.int32)
+        start[b + 1] = start[b] + d
+        mat[b + 1] = mat[b] + d * d
+        T = np.linalg.cholesky(S[np.ix_(idx, idx)])
+        Ti = sla.solve_triangular(T, np.eye(d), lower=True)
+        Ls.append((Ti.conj().T - np.eye(d)).reshape(-1))
+        Rs.append((Ti - np.eye(d)).T.reshape(-1))
+    idx_all = np.concatenate(sets).astype(np.int32)
+    return of, loc, start, idx_all, mat, np.concatenate(Ls).astype(np.complex64), np.concatenate(Rs).astype(np.complex64)
+
+
+def block_solve_operators(H: np.ndarray, Ls, dt: float, n_pad: int, banded: bool = True):
+    """FP64 host precomputation of one step length: (Vp, Wn, dM diagonals, A_k
+    diagonals); Vp and Wn are None without ``banded``."""
     n = H.shape[-1]
     LdL = sum((L.conj().T @ L for L in Ls), np.zeros_like(H))
     Hnh = H - 0.5j * LdL
+    if not banded:
+        ops = [_dia_of(-1j * dt * Hnh)] + [_dia_of(np.sqrt(dt) * L) for L in Ls]
+        return None, None, ops
     M0 = np.eye(n) - 1j * dt * Hnh
     S = M0.conj().T @ M0 + dt * LdL
     w = half_bandwidth(S)
@@ -203,6 +271,7 @@ def build_plan(
     exp_ops=None,
     variant: int = SWEEP_AUTO,
     solve_panel: int = 0,
+    normalisation: str = 'auto',
 ) -> DiaPlan:
     """Pack a batch of Rouchon1 solves that share H and Ls (dense FP64 host arrays
     whose nonzeros lie on few diagonals) and differ in rho0 (B, n, n). ``exp_ops``
@@ -232,10 +301,29 @@ def build_plan(
 
         variant = autotune.dia_sweep(H, Ls, B, kind='kraus', dt=float(dt))
     prob.nsteps, prob.nsets, prob.variant, prob.solve_panel = nsteps, len(sets), variant, int(solve_panel)
+    # normalisation: block form when S decouples into small index sets (the same
+    # sets for every step length: they follow the sparsity of Hnh and L^dag L)
+    if normalisation not in ('auto', 'banded', 'blocks'):
+        raise ValueError("normalisation must be 'auto', 'banded' or 'blocks'")
+    isets = None
+    if normalisation != 'banded' and n_pad <= MAXN_BLOCKS:
+        isets = coupled_sets(normalisation_S(H, Ls, sets[0], n_pad))
+    if normalisation == 'blocks' and isets is None:
+        raise ValueError('the normalisation matrix S does not decouple into sets of at most '
+                         f'{MAXBLK} indices (or n > {MAXN_BLOCKS})')
     for si, h in enumerate(sets):
-        Vp, Wn, ops = block_solve_operators(H, Ls, h, n_pad)
-        prob.Vp[si] = arena.put_now(Vp.astype(np.complex64))
-        prob.Wn[si] = arena.put_now(Wn.astype(np.complex64))
+        Vp, Wn, ops = block_solve_operators(H, Ls, h, n_pad, banded=isets is None)
+        if isets is None:
+            prob.Vp[si] = arena.put_now(Vp.astype(np.complex64))
+            prob.Wn[si] = arena.put_now(Wn.astype(np.complex64))
+        else:
+            of, loc, start, idx, mat, Lm, Rm = block_normalisation(normalisation_S(H, Ls, h, n_pad), isets)
+            if si == 0:
+                prob.nblk, prob.blk_maxd = len(isets), max(len(x) for x in isets)
+                prob.blk_of, prob.blk_loc = arena.put_now(of), arena.put_now(loc)
+                prob.blk_start, prob.blk_idx = arena.put_now(start), arena.put_now(idx)
+                prob.blk_mat = arena.put_now(mat)
+            prob.Lm[si], prob.Rm[si] = arena.put_now(Lm), arena.put_now(Rm)
         _fill_ops(prob.ops[si], ops, n_pad, arena)
     r64 = np.zeros((B, n_pad, n_pad), dtype=np.complex128)
     r64[:, :n, :n] = rho0
@@ -264,6 +352,10 @@ def build_plan(
     # roofline units: the Kraus sweep reads rho and writes rho' (2 n^2 complex64) per step and instance
     plan.sweep_bytes_per_step = 2.0 * n * n * 8 * B
     plan.solve_flops_per_step = 2 * 2 * 8.0 * NB * NB * n_pad * (n_pad // NB) * B
+    plan.normalisation = 'banded' if isets is None else 'blocks'
+    if isets is not None:  # two one-sided products of d x d triangular blocks per row / column
+        d2 = float(sum(len(x) * (len(x) + 1) / 2 for x in isets))
+        plan.solve_flops_per_step = 2 * 8.0 * d2 * n_pad * B
     return plan
 
 

@@ -215,7 +215,26 @@ typedef struct dqh_dia_rouchon1 {
   uint64_t timing;     /* optional HOST double[2] (0: off): mean ms per step of the block
                           solves and of the Kraus sweep, from HIP events around steps
                           1..8; the call then synchronises the stream before returning */
+  /* Block form of the normalisation (nblk = 0: the banded block solves above). When S
+   * never couples indices of different sets (each set of at most DQH_DIA_MAXBLK indices,
+   * in ascending order), the Cholesky factor of S is block-diagonal on those sets, and
+   * T^-dag rho T^-1 is two memory-bound passes: a pass applying T_b^-dag - I to the
+   * rows of each set, and a row-strip pass applying T_b^-1 - I to its columns (one
+   * fused pass, a workgroup per set, when n <= 1024 and every set has at most 32
+   * indices). Needs n <= 4096. */
+  int64_t nblk;        /* number of index sets (0: banded form) */
+  int64_t blk_maxd;    /* largest set size (<= DQH_DIA_MAXBLK) */
+  uint64_t blk_of;     /* device int32[n]: set of each index */
+  uint64_t blk_loc;    /* device int32[n]: position of each index inside its set */
+  uint64_t blk_start;  /* device int32[nblk + 1]: first position of each set in blk_idx */
+  uint64_t blk_idx;    /* device int32[n]: the indices of every set, ascending within a set */
+  uint64_t blk_mat;    /* device int64[nblk + 1]: first element of each set's d x d matrices */
+  uint64_t Lm[2];      /* device c64[sum d^2] per step set: T_b^-dag - I, row-major (upper) */
+  uint64_t Rm[2];      /* device c64[sum d^2] per step set: (T_b^-1 - I)^T, row-major, so that
+                          column l of T_b^-1 - I (nonzero from row l on) is contiguous */
 } dqh_dia_rouchon1;
+
+#define DQH_DIA_MAXBLK 64
 
 int64_t dqh_sizeof_dia_rouchon1(void);
 

@@ -300,3 +300,47 @@ def test_block_solve_step_is_exact_off_the_hermitian_matrices(na):
         A = _emulate_step(A, Vp, Wn, ops_p, n_pad)
         assert np.linalg.norm(A) <= 1.0 + 1e-9
         A /= np.linalg.norm(A)
+
+
+def test_block_normalisation_emulation_matches_cholesky():
+    """The block form of the normalisation as the strip kernels compute it
+    (k_blk_left: D1[r] = sum_k (T_b^-dag - I)[loc r][k] hi[idx_k]; k_blk_right:
+    E = D1 + Y (T_b^-1 - I) on the columns of each set, X = Y + the same sum, Y = hi + D1)
+    gives hi + E = T^-dag hi T^-1 with T the dense Cholesky factor of S, on the
+    two-mode problem padded to a multiple of 64; its sets are the excitation-number
+    sectors (at most 32 indices at 32 x 32)."""
+    H, Ls = _two_mode(12, 12)
+    n_pad = 192
+    S = dia.normalisation_S(H, Ls, 1e-3, n_pad)
+    sets = dia.coupled_sets(S)
+    assert sets is not None and max(len(x) for x in sets) == 12
+    of, loc, start, idx, mat, Lm, Rm = dia.block_normalisation(S, sets)
+    rng = np.random.default_rng(7)
+    hi = (rng.normal(size=(n_pad, n_pad)) + 1j * rng.normal(size=(n_pad, n_pad))).astype(np.complex64)
+    D1 = np.zeros_like(hi)
+    for r in range(n_pad):
+        b, l = of[r], loc[r]
+        s0, d = start[b], start[b + 1] - start[b]
+        Lrow = Lm[mat[b] + l * d : mat[b] + (l + 1) * d]
+        D1[r] = Lrow[l:] @ hi[idx[s0 + l : s0 + d]]
+    Y = hi + D1
+    E = np.zeros_like(hi)
+    X = np.zeros_like(hi)
+    for c in range(n_pad):
+        b, l = of[c], loc[c]
+        s0, d = start[b], start[b + 1] - start[b]
+        Rcol = Rm[mat[b] + l * d : mat[b] + (l + 1) * d]
+        acc = Y[:, idx[s0 + l : s0 + d]] @ Rcol[l:]
+        E[:, c] = D1[:, c] + acc
+        X[:, c] = Y[:, c] + acc
+    T = np.linalg.cholesky(S)
+    Ti = np.linalg.inv(T)
+    want = Ti.conj().T @ hi.astype(complex) @ Ti
+    assert np.abs(X - want).max() < 2e-5 * np.abs(want).max()
+    assert np.abs((hi.astype(complex) + E) - want).max() < 2e-5 * np.abs(want).max()
+    # the C3 operators: 63 sectors of at most 32 indices (plus nothing to pad at 1024)
+    a, bb = op.destroy(32, 32)
+    ad, bd = a.conj().T, bb.conj().T
+    H3 = 5 * ad @ a + 6 * bd @ bb + 0.1 * (ad @ bb + a @ bd) - 0.01 * ad @ a @ bd @ bb
+    sets3 = dia.coupled_sets(dia.normalisation_S(H3, [np.sqrt(0.02) * a, np.sqrt(0.05) * bb], 1e-3, 1024))
+    assert len(sets3) == 63 and max(len(x) for x in sets3) == 32

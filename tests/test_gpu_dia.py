@@ -19,8 +19,9 @@ def _two_mode(na, nb_):
     return H, [np.sqrt(0.02) * a, np.sqrt(0.05) * b], (a, b)
 
 
+@pytest.mark.parametrize('normalisation', ['banded', 'blocks'])
 @pytest.mark.parametrize('na,nb_,nsteps', [(8, 9, 300), (12, 12, 200)])
-def test_dia_rouchon1_matches_oracle(na, nb_, nsteps):
+def test_dia_rouchon1_matches_oracle(na, nb_, nsteps, normalisation):
     from dynamiqs_amd.integrators import dia
     from oracle import mesolve as om
     from oracle import operators as op
@@ -31,7 +32,8 @@ def test_dia_rouchon1_matches_oracle(na, nb_, nsteps):
     tsave = np.linspace(0, T, 6)
     alphas = [0.5, 1.2, 2.0]
     rho0 = np.stack([op.coherent_dm((na, nb_), (al, 0.0)) for al in alphas])
-    plan = dia.build_plan(H, Ls, rho0, tsave, dt)
+    plan = dia.build_plan(H, Ls, rho0, tsave, dt, normalisation=normalisation)
+    assert plan.normalisation == normalisation
     plan.run()
     out = plan.results()
     for b in range(len(alphas)):
@@ -77,12 +79,13 @@ def test_dia_rouchon1_config3_shape_short_run():
     tsave = np.linspace(0, nsteps * dt, 2)
     rho0 = np.stack([op.coherent_dm((32, 32), (al, 0.0)) for al in (0.5, 2.0)])
     outs = []
-    for v in (dia.SWEEP_SPLIT, dia.SWEEP_GLOBAL):
-        plan = dia.build_plan(H, Ls, rho0, tsave, dt, variant=v)
+    for v, norm in ((dia.SWEEP_SPLIT, 'banded'), (dia.SWEEP_GLOBAL, 'banded'), (dia.SWEEP_GLOBAL, 'blocks')):
+        plan = dia.build_plan(H, Ls, rho0, tsave, dt, variant=v, normalisation=norm)
         plan.run()
         outs.append(plan.results())
     out = outs[0]
     assert np.abs(outs[0]['final_state'] - outs[1]['final_state']).max() <= 1e-6
+    assert np.abs(outs[0]['final_state'] - outs[2]['final_state']).max() <= 1e-6
     M0, Ms, S = rouchon.kraus_ops(H, Ls, dt)
     Tab = rouchon.banded_factor(S, 62)
     for b in range(2):
@@ -323,14 +326,14 @@ def test_dia_block_solve_panel_widths_agree():
     tsave = np.linspace(0, 0.02, 3)
     outs = []
     for pw in (32, 64):
-        plan = dia.build_plan(H, Ls, rho0, tsave, 1e-3, variant=dia.SWEEP_SPLIT, solve_panel=pw)
+        plan = dia.build_plan(H, Ls, rho0, tsave, 1e-3, variant=dia.SWEEP_SPLIT, solve_panel=pw, normalisation='banded')
         plan.run()
         outs.append(plan.results())
     assert np.array_equal(outs[0]['states'], outs[1]['states'])
 
 
-@pytest.mark.parametrize('variant', ['split', 'global'])
-def test_dia_rouchon1_long_run_is_stable_and_matches_dense(variant):
+@pytest.mark.parametrize('normalisation', ['banded', 'blocks'])
+def test_dia_rouchon1_long_run_is_stable_and_matches_dense(normalisation):
     """5000 steps (T = 5, dt = 1e-3) of the 16 x 16 two-mode problem (n = 256): the
     state stays trace-preserving and Hermitian and agrees with the dense Rouchon1
     kernel (a different kernel for the same scheme) at every save. Rounding seeds
@@ -344,8 +347,7 @@ def test_dia_rouchon1_long_run_is_stable_and_matches_dense(variant):
     v = np.kron(op.coherent(16, 1.5).ravel(), np.eye(16)[:, 0])
     rho0 = np.outer(v, v.conj())
     tsave = np.linspace(0.0, 5.0, 11)
-    v_ = {'split': dia.SWEEP_SPLIT, 'global': dia.SWEEP_GLOBAL}[variant]
-    plan = dia.build_plan(H, Ls, rho0[None], tsave, 1e-3, variant=v_)
+    plan = dia.build_plan(H, Ls, rho0[None], tsave, 1e-3, normalisation=normalisation)
     plan.run()
     st = plan.results()['states'][0]
     tr = np.abs(np.trace(st, axis1=-2, axis2=-1) - 1)
