@@ -796,6 +796,8 @@ def bench_kerr(args, pool):
         out['error_vs_fp64'] = {
             'max_frobenius': max(x[0] for x in rr),
             'max_trace_distance': max(x[1] for x in rr),
+            'frobenius_by_instance': [x[0] for x in rr],
+            'eps2_by_instance': [float(eps2[i]) for i in order],
             'instances_checked': len(rr),
             'instances': order,
             'largest_eps2_instance': big,

@@ -79,6 +79,7 @@ SYMBOLS = {
     'dqh_sizeof_me_inst': (_I64, []),
     'dqh_me_work_elems': (_I64, [_I64, _I64, _I64]),
     'dqh_lindblad_rhs_dense': (ctypes.c_int, [_P, _I64, _P]),
+    'dqh_dense_prepare_ops': (ctypes.c_int, [_P, _I64, _P]),
     'dqh_mesolve_tsit5_dense': (ctypes.c_int, [_P, _I64, _P]),
     'dqh_mesolve_rouchon1_dense': (ctypes.c_int, [_P, _I64, _P]),
     'dqh_mesolve_tsit5_dense_uniform': (ctypes.c_int, [_P, _I64, _I64, _P]),

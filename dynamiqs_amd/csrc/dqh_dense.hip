@@ -357,6 +357,7 @@ __device__ __forceinline__ void tsit5_global(const dqh_me_inst &p, double *sh) {
     ++isave;
   }
   const bool smooth = p.nmod > 0;  // H changes within a step: updated at every stage time
+  const bool lin = !smooth && !(p.flags & DQH_FLAG_LMOD);  // constant generator over each step
   if (v.npwc > 0 || smooth) {
     seg = v.npwc > 0 ? pwc_segment(ptimes, v.nseg, t) : -1;
     wg_update_pwc<NW>(v, p, seg, t);
@@ -447,15 +448,25 @@ __device__ __forceinline__ void tsit5_global(const dqh_me_inst &p, double *sh) {
     }
     const double2 *y = v.y[cur];
     double2 *ynew = v.y[1 - cur];
-    // stages 1..6 (stage 6 input is the candidate solution)
+    // stages 1..6 (stage 6 input is the candidate solution). With a generator that
+    // is constant over the step (lin), stages 1..5 evaluate the vector field on the
+    // increment Delta_s = h sum_j a_sj k_j only and hold g_s = f(Delta_s), with
+    // k_s = k_0 + g_s folded into the weight of k_0 wherever the stages combine.
     for (int s = 1; s < 7; ++s) {
+      double c[6];
+      for (int j = 0; j < 6; ++j) c[j] = (j < s) ? h * rk.A[s][j] : 0.0;
+      if (lin) {
+        double a0 = 0.0;
+        for (int j = 0; j < s; ++j) a0 += rk.A[s][j];
+        c[0] = h * a0;
+      }
+      const bool with_y = !lin || s == 6;
       for (size_t e = threadIdx.x; e < v.nn; e += NW * 64) {
-        double re = y[e].x, im = y[e].y;
+        double re = with_y ? y[e].x : 0.0, im = with_y ? y[e].y : 0.0;
         for (int j = 0; j < s; ++j) {
-          const double c = h * rk.A[s][j];
           const float2 kj = kp(v, j)[e];
-          re += c * kj.x;
-          im += c * kj.y;
+          re += c[j] * kj.x;
+          im += c[j] * kj.y;
         }
         if (s == 6) ynew[e] = make_double2(re, im);
         v.ys[e] = make_float2((float)re, (float)im);
@@ -466,14 +477,17 @@ __device__ __forceinline__ void tsit5_global(const dqh_me_inst &p, double *sh) {
       ++nrhs;
     }
     // error norm (FP64 accumulation)
+    double ew[7];
+    for (int j = 0; j < 7; ++j) ew[j] = rk.E[j];
+    if (lin) fold_k0(ew);
     double s = 0;
     for (size_t e = threadIdx.x; e < v.nn; e += NW * 64) {
       double er = 0, ei = 0;
 #pragma unroll
       for (int j = 0; j < 7; ++j) {
         const float2 kj = kp(v, j)[e];
-        er += rk.E[j] * kj.x;
-        ei += rk.E[j] * kj.y;
+        er += ew[j] * kj.x;
+        ei += ew[j] * kj.y;
       }
       er *= h;
       ei *= h;
@@ -500,6 +514,7 @@ __device__ __forceinline__ void tsit5_global(const dqh_me_inst &p, double *sh) {
         const double th = (tsave[isave] - t) / h;
         double w[7];
         for (int j = 0; j < 7; ++j) w[j] = th * (rk.R[j][0] + th * (rk.R[j][1] + th * (rk.R[j][2] + th * rk.R[j][3])));
+        if (lin) fold_k0(w);
         wg_save<NW>(v, p, y, h, w, 7, isave, sh);
         ++isave;
       }
@@ -685,6 +700,41 @@ __global__ __launch_bounds__(kThreads) void k_rouchon1(const dqh_me_inst *__rest
   rouchon1_global<kWaves>(p, sh);
 }
 
+// Operator preparation on the device (diffrax_integrator.py:319-323 builds
+// Hnh = -iH - 1/2 sum L^dag L on every call): on entry Hnh holds -iH and Dops the
+// jump operators L_k (zero-padded to np); on exit Hnh = -iH - 1/2 sum_k D_k^dag D_k,
+// the products accumulated in FP64 from the complex64 D_k (the same D_k the jump
+// term uses), and Cops = D_k^dag / 2. grid: (kPrepTiles, ninst); each workgroup
+// strides over the np^2 elements of its instance (D_k[m][i] is the same for the
+// threads of one row, D_k[m][j] coalesced).
+constexpr int kPrepTiles = 16;
+
+__global__ __launch_bounds__(256) void k_prepare_ops(const dqh_me_inst *__restrict__ insts) {
+  const dqh_me_inst &p = insts[blockIdx.y];
+  const int np = (int)p.np, nL = (int)p.nL;
+  const int nn = np * np;
+  float2 *Hnh = gp<float2>(p.Hnh);
+  const float2 *D = gp<const float2>(p.Dops);
+  float2 *C = gp<float2>(p.Cops);
+  for (int e = blockIdx.x * 256 + threadIdx.x; e < nn; e += kPrepTiles * 256) {
+    const int i = e / np, j = e - i * np;
+    double ar = 0.0, ai = 0.0;
+    for (int k = 0; k < nL; ++k) {
+      const float2 *Dk = D + (size_t)k * nn;
+      for (int m = 0; m < np; ++m) {
+        const float2 a = Dk[(size_t)m * np + i], b = Dk[(size_t)m * np + j];
+        // conj(a) b
+        ar += (double)a.x * b.x + (double)a.y * b.y;
+        ai += (double)a.x * b.y - (double)a.y * b.x;
+      }
+      const float2 t = Dk[(size_t)j * np + i];
+      C[(size_t)k * nn + e] = make_float2(0.5f * t.x, -0.5f * t.y);
+    }
+    const float2 h = Hnh[e];
+    Hnh[e] = make_float2((float)((double)h.x - 0.5 * ar), (float)((double)h.y - 0.5 * ai));
+  }
+}
+
 }  // namespace dqh
 
 using namespace dqh;
@@ -701,6 +751,12 @@ static int launch_checked(const char *what, hipError_t err) {
     return DQH_HIP_ERROR;
   }
   return DQH_OK;
+}
+
+extern "C" int dqh_dense_prepare_ops(const dqh_me_inst *d_insts, int64_t ninst, void *stream) {
+  if (!d_insts || ninst <= 0 || ninst > 65535) return fail(DQH_BAD_ARG, "dqh_dense_prepare_ops: bad arguments");
+  hipLaunchKernelGGL(k_prepare_ops, dim3(kPrepTiles, (unsigned)ninst), dim3(256), 0, (hipStream_t)stream, d_insts);
+  return launch_checked("dqh_dense_prepare_ops", hipGetLastError());
 }
 
 extern "C" int dqh_lindblad_rhs_dense(const dqh_me_inst *d_insts, int64_t ninst, void *stream) {

@@ -6,8 +6,10 @@
 // whole grid, in three launches:
 //   k_xprod  X_k = L_k ys                          16 x 16 MFMA tiles, one per wave
 //   k_tmp    tmp = Hnh ys + sum_k X_k (L_k^dag / 2) 16 x 16 MFMA tiles, one per wave
-//   k_body   f = tmp + tmp^dag; k_s = f; ys <- y + h sum a_{s+1,j} k_j (stage
-//            input of the next stage, in place: the GEMMs that read ys are done),
+//   k_body   f = tmp + tmp^dag; k_s = f; ys <- the next stage input, in place (the
+//            GEMMs that read ys are done): the increment h sum a_{s+1,j} k_j for
+//            stages 1..5 (the generator is constant, so f(y + Delta) = k_0 + f(Delta)
+//            and k_1..k_5 hold f(Delta_s) only), y + that increment for stage 6;
 //            or on the last stage the FP64 partial sums of the scaled error
 // and one commit launch per step (accept/reject, dense-output saves, y <- y1 in
 // FP64, the next stage input), as in the DIA grid integrator: every workgroup
@@ -143,23 +145,38 @@ __global__ __launch_bounds__(256) void k_body(const dqh_me_inst *__restrict__ in
     const int i = (int)e / np, j = (int)e - ((int)e / np) * np;  // e < np^2 < 2^31
     const float2 f = field_at(tmpb(p), np, i, j);
     kslot(p, s, c.rot)[e] = f;
-    const double2 yv = gp<const double2>(p.y)[e];
+    // increment form (constant generators): stages 1..5 hold g_s = f(Delta_s),
+    // k_s = k_0 + g_s, folded into the weight of k_0 (stage 6 is f(y1))
+    const float2 k0 = kslot(p, 0, c.rot)[e];
     if (s < 6) {
-      double re = yv.x + h * rk.A[s + 1][s] * f.x, im = yv.y + h * rk.A[s + 1][s] * f.y;
-      for (int q = 0; q < s; ++q) {
+      double a0 = 0.0;
+      for (int q = 0; q <= s; ++q) a0 += rk.A[s + 1][q];
+      double re = h * a0 * k0.x + h * rk.A[s + 1][s] * f.x, im = h * a0 * k0.y + h * rk.A[s + 1][s] * f.y;
+      if (s + 1 == 6) {
+        const double2 yv = gp<const double2>(p.y)[e];
+        re += yv.x;
+        im += yv.y;
+      }
+      for (int q = 1; q < s; ++q) {
         const float2 kq = kslot(p, q, c.rot)[e];
         re += h * rk.A[s + 1][q] * kq.x;
         im += h * rk.A[s + 1][q] * kq.y;
       }
       ysb(p)[e] = make_float2((float)re, (float)im);
     } else if (i < n && j < n) {
-      double er = rk.E[6] * f.x, ei = rk.E[6] * f.y, yr = yv.x, yi = yv.y;
-      for (int q = 0; q < 6; ++q) {
+      const double2 yv = gp<const double2>(p.y)[e];
+      double ew[7], bw[7];
+      for (int q = 0; q < 7; ++q) ew[q] = rk.E[q], bw[q] = rk.A[6][q];
+      fold_k0(ew);
+      fold_k0(bw);
+      double er = ew[6] * f.x + ew[0] * k0.x, ei = ew[6] * f.y + ew[0] * k0.y;
+      double yr = yv.x + h * bw[0] * k0.x, yi = yv.y + h * bw[0] * k0.y;
+      for (int q = 1; q < 6; ++q) {
         const float2 kq = kslot(p, q, c.rot)[e];
-        er += rk.E[q] * kq.x;
-        ei += rk.E[q] * kq.y;
-        yr += h * rk.A[6][q] * kq.x;
-        yi += h * rk.A[6][q] * kq.y;
+        er += ew[q] * kq.x;
+        ei += ew[q] * kq.y;
+        yr += h * bw[q] * kq.x;
+        yi += h * bw[q] * kq.y;
       }
       er *= h;
       ei *= h;
@@ -240,27 +257,35 @@ __global__ __launch_bounds__(256) void k_commit(const dqh_me_inst *__restrict__ 
         float2 *st = gp<float2>(p.states);
         for (int64_t is = c.isave; is < is_end; ++is) {
           const double th = (tsave[is] - c.t) / h;
+          double w[7];
+#pragma unroll
+          for (int q = 0; q < 7; ++q) w[q] = th * (rk.R[q][0] + th * (rk.R[q][1] + th * (rk.R[q][2] + th * rk.R[q][3])));
+          fold_k0(w);
           double re = yv.x, im = yv.y;
 #pragma unroll
           for (int q = 0; q < 7; ++q) {
-            const double w = th * (rk.R[q][0] + th * (rk.R[q][1] + th * (rk.R[q][2] + th * rk.R[q][3])));
-            re += h * w * kk[q].x;
-            im += h * w * kk[q].y;
+            re += h * w[q] * kk[q].x;
+            im += h * w[q] * kk[q].y;
           }
           if (i < n && j < n) st[(size_t)is * n * n + (size_t)i * n + j] = make_float2((float)re, (float)im);
         }
       }
+      double bw[7];
+#pragma unroll
+      for (int q = 0; q < 7; ++q) bw[q] = rk.A[6][q];
+      fold_k0(bw);
 #pragma unroll
       for (int q = 0; q < 6; ++q) {
-        y1.x += h * rk.A[6][q] * kk[q].x;
-        y1.y += h * rk.A[6][q] * kk[q].y;
+        y1.x += h * bw[q] * kk[q].x;
+        y1.y += h * bw[q] * kk[q].y;
       }
       *yp = y1;
       k0n = kk[6];
     }
+    // stage 1 input of the next attempt: the increment hn a_10 k_0
     if (!is_done(o, p.t1)) {
       const double hn = step_h(o, p.t1);
-      ysb(p)[e] = make_float2((float)(y1.x + hn * rk.A[1][0] * k0n.x), (float)(y1.y + hn * rk.A[1][0] * k0n.y));
+      ysb(p)[e] = make_float2((float)(hn * rk.A[1][0] * k0n.x), (float)(hn * rk.A[1][0] * k0n.y));
     }
   }
   if (writer) *nx = o;
@@ -388,10 +413,9 @@ __global__ __launch_bounds__(256) void k_init_dt(const dqh_me_inst *__restrict__
   if (!(hn > 0.0) || !(hn > 1e-14 * fmax(1.0, fabs(o.t)))) o.status = DQH_NONFINITE;
   const size_t nn = (size_t)p.np * p.np;
   const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (e < (int64_t)nn && o.status == 0) {
-    const double2 yv = gp<const double2>(p.y)[e];
+  if (e < (int64_t)nn && o.status == 0) {  // stage 1 input (increment form)
     const float2 kq = kslot(p, 0, 0)[e];
-    ysb(p)[e] = make_float2((float)(yv.x + hn * rk.A[1][0] * kq.x), (float)(yv.y + hn * rk.A[1][0] * kq.y));
+    ysb(p)[e] = make_float2((float)(hn * rk.A[1][0] * kq.x), (float)(hn * rk.A[1][0] * kq.y));
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) g.ctrl[inst] = o;
 }

@@ -498,6 +498,7 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
     ++isave;
   }
   const bool smooth = p.nmod > 0;  // H changes within a step: updated at every stage time
+  const bool lin = !smooth;        // the generator is constant over each step
   if (pwc || smooth) {
     seg = pwc ? pwc_segment(ptimes, nseg, t) : -1;
     res_update_pwc<NP, NW>(p, seg, t);
@@ -602,6 +603,12 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
     }
     // stages 1..6. The stage input y + h sum_j a_sj k_j is accumulated in FP64 (the
     // coefficients reach |a| ~ 13 and the terms cancel) and rounded once to FP32.
+    // With a generator that is constant over the step (lin), stages 1..5 evaluate
+    // the vector field on the increment only: k_s = k_0 + g_s, g_s = f(Delta_s),
+    // Delta_s = h sum_j a_sj k_j = h (sum_j a_sj) k_0 + h sum_{j>=1} a_sj g_j, so
+    // the FP32 rounding of a stage scales with |Delta_s| instead of |y|; k[1..5]
+    // then hold g_s, every combination below folds k_0 into the weight of k[0],
+    // and stage 6 (the FSAL stage, the next k_0) is a full evaluation at y1.
     // The stage loop is a runtime loop (one copy of the GEMM code); the
     // register-resident stages are selected with compile-time indices.
 #pragma unroll 1
@@ -609,9 +616,16 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
       double c[6];
 #pragma unroll
       for (int j = 0; j < 6; ++j) c[j] = (j < s) ? h * rk.A[s][j] : 0.0;
+      const bool with_y = !lin || s == 6;
+      if (lin) {
+        double a0 = 0.0;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) a0 += (j < s) ? rk.A[s][j] : 0.0;
+        c[0] = h * a0;
+      }
 #pragma unroll
       for (int q = 0; q < NE; ++q) {
-        const double2 yq = yload(q);
+        const double2 yq = with_y ? yload(q) : make_double2(0.0, 0.0);
         double re = yq.x, im = yq.y;
 #pragma unroll
         for (int j = 0; j < 6; ++j) {
@@ -633,7 +647,14 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
       }
     }
     // error norm in FP64 with the candidate y1 = y + h sum b_j k_j, which is stored
-    // to the other state buffer
+    // to the other state buffer (error and solution weights folded for lin)
+    double ew[7], bw[7];
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+      ew[j] = rk.E[j];
+      bw[j] = rk.A[6][j];
+    }
+    if (lin) fold_k0(ew), fold_k0(bw);
     double ssum = 0;
 #pragma unroll
     for (int q = 0; q < NE; ++q) {
@@ -641,10 +662,10 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
       double er = 0, ei = 0, yr = yq.x, yi = yq.y;
 #pragma unroll
       for (int j = 0; j < 7; ++j) {
-        er += rk.E[j] * k[j][q].x;
-        ei += rk.E[j] * k[j][q].y;
-        yr += h * rk.A[6][j] * k[j][q].x;
-        yi += h * rk.A[6][j] * k[j][q].y;
+        er += ew[j] * k[j][q].x;
+        ei += ew[j] * k[j][q].y;
+        yr += h * bw[j] * k[j][q].x;
+        yi += h * bw[j] * k[j][q].y;
       }
       er *= h;
       ei *= h;
@@ -672,6 +693,7 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
         double w[7];
 #pragma unroll
         for (int j = 0; j < 7; ++j) w[j] = th * (rk.R[j][0] + th * (rk.R[j][1] + th * (rk.R[j][2] + th * rk.R[j][3])));
+        if (lin) fold_k0(w);
         res_save<NP, NW>(p, yg, ij, k, h, w, true, isave, sh);
         ++isave;
       }

@@ -72,4 +72,15 @@ __device__ __forceinline__ bool snaps_to(double t, double h, double target) {
   return t + h >= target - kSnap * fmax(1.0, fabs(target));
 }
 
+// Increment form of the stages (generator constant over the step): stages 1..5
+// hold g_s = f(Delta_s), Delta_s = h sum_j a_sj k_j, and k_s = k_0 + g_s, so the
+// weights w_0..w_6 of the true stages k_0..k_6 become weights of
+// (k_0, g_1..g_5, k_6) with w_0 <- w_0 + ... + w_5.
+__device__ __forceinline__ void fold_k0(double (&w)[7]) {
+  double a = 0.0;
+#pragma unroll
+  for (int j = 0; j < 6; ++j) a += w[j];
+  w[0] = a;
+}
+
 }  // namespace dqh

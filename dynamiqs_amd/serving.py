@@ -11,10 +11,12 @@ spread over many CUs, driven by a worker thread), so that one large request does
 not hold a single CU for the whole solve. Launches on different streams overlap on
 the device.
 
-Packing: the inputs of a launch (generators, jump operators, initial states,
-save times) and its descriptor array are written on the host into one pinned
-staging buffer and cross to the device in one copy; the FP64 state, the stage
-scratch and every input stay resident in HBM for the whole solve.
+Packing: the inputs of a launch (-iH, jump operators, initial states, save
+times) and its descriptor array are written on the host into one pinned staging
+buffer and cross to the device in one copy; a small kernel then completes the
+non-Hermitian Hamiltonian -iH - 1/2 sum L^dag L and the right factors L^dag / 2
+on the device (FP64 products), so no O(n^3) work is left on the host. The FP64
+state, the stage scratch and every input stay resident in HBM for the whole solve.
 
 Retirement is per request: the outputs of a persistent launch (saved states,
 expectation values, final state, statistics) live in pinned host memory that the
@@ -35,10 +37,11 @@ is the same Tsit5 path as :func:`dynamiqs_amd.mesolve`.
 
 from __future__ import annotations
 
+import os
 import queue
 import threading
 import time
-from concurrent.futures import Future
+from concurrent.futures import Future, ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -146,6 +149,21 @@ class _Launch:
         return self.event is not None and self.event.query()
 
 
+# Host threads that copy finished requests out of pinned memory and answer them
+# (numpy releases the GIL in the copies), off the scheduler thread.
+HOST_THREADS = max(1, min(8, os.cpu_count() or 1))
+_pool_lock = threading.Lock()
+_pool_obj = None
+
+
+def _pool() -> ThreadPoolExecutor:
+    global _pool_obj
+    with _pool_lock:
+        if _pool_obj is None:
+            _pool_obj = ThreadPoolExecutor(max_workers=HOST_THREADS, thread_name_prefix='dqh-host')
+        return _pool_obj
+
+
 def _pack_host(reqs: list, base: int, staging: np.ndarray, descs: np.ndarray, lays, in_off, out_base: int, out_off,
                scratch_base: int, scratch_off, signal: bool) -> None:
     """Fill the staging buffer (inputs) and the descriptors of ``reqs``; ``base`` /
@@ -161,15 +179,11 @@ def _pack_host(reqs: list, base: int, staging: np.ndarray, descs: np.ndarray, la
         def cview(a, count=1):
             return staging[ib + a : ib + a + count * nn * 8].view(np.complex64).reshape(count, npd, npd)
 
-        H = np.asarray(r.H, dtype=np.complex128)
-        Ls = [np.asarray(L, dtype=np.complex128) for L in r.Ls]
-        Hnh = -1j * H
-        for L in Ls:
-            Hnh = Hnh - 0.5 * (L.conj().T @ L)
-        cview(li['H'])[0, :n, :n] = Hnh
-        if K:
-            cview(li['D'], K)[:, :n, :n] = np.stack(Ls)
-            cview(li['C'], K)[:, :n, :n] = 0.5 * np.conj(np.swapaxes(np.stack(Ls), -1, -2))
+        # -iH and the jump operators; dqh_dense_prepare_ops completes Hnh and the
+        # right factors C_k = L_k^dag / 2 on the device
+        cview(li['H'])[0, :n, :n] = np.asarray(r.H) * -1j
+        for k, L in enumerate(r.Ls):
+            cview(li['D'] + k * nn * 8)[0, :n, :n] = L
         if nE:
             cview(li['E'], nE)[:, :n, :n] = np.swapaxes(np.stack([np.asarray(E) for E in r.exp_ops]), -1, -2)
         rho = np.asarray(r.rho0, dtype=np.complex128)
@@ -227,8 +241,12 @@ def pack_and_launch(reqs: list, stream, grid: bool = False):
     if grid:
         out_base, host_out = oarena.data_ptr(), None
     else:
-        oarena = torch.zeros(int(out_off[-1]), dtype=torch.uint8, pin_memory=True)
+        # the kernel writes every output; only the completion words must start at zero
+        oarena = torch.empty(int(out_off[-1]), dtype=torch.uint8, pin_memory=True)
         out_base, host_out = _native.host_device_pointer(oarena.data_ptr()), oarena.numpy()
+        for b in range(B):
+            a = int(out_off[b]) + lays[b][3]['stats']
+            host_out[a : a + 64] = 0
     _pack_host(reqs, arena.data_ptr(), staging, descs, lays, in_off, out_base, out_off, scratch.data_ptr(), w_off,
                signal=not grid)
     npd = int(descs['np'][0])
@@ -242,6 +260,7 @@ def pack_and_launch(reqs: list, stream, grid: bool = False):
     nbytes = int(arena.numel() + scratch.numel() + (oarena.numel() if grid else 0))
     with torch.cuda.stream(stream):
         arena.copy_(staging_t, non_blocking=True)
+        _native.check(lib.dqh_dense_prepare_ops(c_p(d_descs_ptr), B, c_p(stream.cuda_stream)), 'dqh_dense_prepare_ops')
         if grid:
             nb = int(lib.dqh_tsit5_grid_scratch_bytes(npd, B))
             gscratch = torch.empty((nb,), dtype=torch.uint8, device=dev)
@@ -460,15 +479,23 @@ class Server:
         self.stats['max_concurrent_launches'] = max(self.stats['max_concurrent_launches'], len(self._inflight))
 
     def _answer(self, L: _Launch, b: int, res, t: float) -> None:
+        """Complete request b of launch L with its outputs (any thread; the scheduler
+        has already marked it answered and counted it)."""
         r = L.reqs[b]
         st, ex, ns, na, nr, status, log = res
-        L.answered[b] = True
-        self.stats['requests'] += 1
         if status != 0:
             r.future.set_exception(status_error(status, ns, na, nr))
             return
         r.future.set_result(Response(st, ex, ns, na, nr, status, t - r.t_submit, L.t_launch - r.t_submit,
                                      step_ends=log))
+
+    def _answer_pinned(self, L: _Launch, b: int, out) -> None:
+        try:
+            res = _read_outputs(L.host_out, out)
+        except Exception as e:  # pragma: no cover - answered with the error
+            L.reqs[b].future.set_exception(e)
+            return
+        self._answer(L, b, res, time.perf_counter())
 
     def _retire(self) -> bool:
         progress = False
@@ -477,7 +504,11 @@ class Server:
                 # per request: stats[7] is set by the kernel once that request is done
                 for b, out in enumerate(L.outs):
                     if not L.answered[b] and L.host_out[out[2] + 56 : out[2] + 64].view(np.int64)[0] == 1:
-                        self._answer(L, b, _read_outputs(L.host_out, out), time.perf_counter())
+                        L.answered[b] = True
+                        self.stats['requests'] += 1
+                        # the copy out of pinned memory runs on a host thread (the
+                        # launch object keeps the pinned buffer alive until then)
+                        _pool().submit(self._answer_pinned, L, b, out)
                         progress = True
                         if not L.done():
                             self.stats['answered_before_launch_end'] += 1
@@ -489,6 +520,9 @@ class Server:
             if L.event is not None:
                 self._busy.append((L.ev0, L.event))
             rest = [b for b in range(len(L.reqs)) if not L.answered[b]]
+            for b in rest:
+                L.answered[b] = True
+            self.stats['requests'] += len(rest)
             if rest:
                 try:
                     if L.error is not None:
@@ -496,7 +530,6 @@ class Server:
                     res = collect(L.tensors[0], [L.outs[b] for b in rest])
                 except Exception as e:
                     for b in rest:
-                        L.answered[b] = True
                         L.reqs[b].future.set_exception(e)
                     res = None
                 if res is not None:

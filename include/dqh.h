@@ -121,6 +121,14 @@ int64_t dqh_me_work_elems(int64_t np, int64_t nL, int64_t npwc);
  * rounded to complex64. One workgroup per instance. */
 int dqh_lindblad_rhs_dense(const dqh_me_inst *d_insts, int64_t ninst, void *stream);
 
+/* Device-side operator preparation for a batch packed by a server: each
+ * descriptor's Hnh holds -iH and Dops the jump operators on entry; on return
+ * Hnh = -iH - 1/2 sum_k D_k^dag D_k (products in FP64) and Cops = D_k^dag / 2.
+ * Replaces the host-side construction of the non-Hermitian Hamiltonian of
+ * dynamiqs/integrators/core/diffrax_integrator.py:319-323 for served requests.
+ * ninst <= 65535; stream-ordered, allocates nothing. */
+int dqh_dense_prepare_ops(const dqh_me_inst *d_insts, int64_t ninst, void *stream);
+
 /* Adaptive Tsit5 solve of every instance, one persistent workgroup per instance
  * (replaces dx.diffeqsolve(..., dx.Tsit5(), dx.PIDController(...)) at
  * diffrax_integrator.py:101-147 with the vector field of :319-323). */

@@ -132,3 +132,41 @@ def test_requests_retire_before_their_launch_ends():
     if stats['launches'] == 1:  # both in one launch (the usual case): the short one left early
         assert stats['answered_before_launch_end'] >= 1
     assert r_short.latency_s < 0.5 * r_long.latency_s
+
+
+def test_device_operator_preparation():
+    """dqh_dense_prepare_ops: Hnh = -iH - 1/2 sum L^dag L and C_k = L_k^dag / 2 on
+    the device, from -iH and the complex64 jump operators, for requests of several
+    padded sizes and jump counts (zero padding preserved)."""
+    import torch
+
+    from dynamiqs_amd import _native, serving
+    from dynamiqs_amd.integrators.dense import pad16
+
+    rng = np.random.default_rng(3)
+    reqs = []
+    for n, K in [(8, 1), (37, 3), (100, 2), (256, 1), (20, 0)]:
+        H, Ls, psi, ts = _random_request(rng, n, K)
+        reqs.append(serving.Request(H=H, Ls=Ls, rho0=psi, tsave=ts))
+    stream = torch.cuda.Stream()
+    keep, outs, host_out, ev, _ = serving.pack_and_launch(reqs, stream)
+    ev.synchronize()
+    arena = keep[1].cpu().numpy()
+    lays = [serving._layout(r.n, len(r.Ls), len(r.tsave), len(r.exp_ops), r.step_log_cap) for r in reqs]
+    in_off = np.cumsum([0] + [lay[1] for lay in lays])
+    for b, r in enumerate(reqs):
+        n, K, npd = r.n, len(r.Ls), pad16(r.n)
+        nn = npd * npd
+        li = lays[b][0]
+        ib = int(in_off[b])
+        Hnh = arena[ib + li['H'] : ib + li['H'] + nn * 8].view(np.complex64).reshape(npd, npd)
+        L64 = [L.astype(np.complex64).astype(np.complex128) for L in r.Ls]
+        want = -1j * r.H.astype(np.complex64).astype(np.complex128) - 0.5 * sum((L.conj().T @ L for L in L64), np.zeros((n, n)))
+        scale = max(1.0, np.abs(want).max())
+        assert np.abs(Hnh[:n, :n] - want).max() <= 2e-7 * scale, (n, K)
+        assert not np.any(Hnh[n:, :]) and not np.any(Hnh[:, n:])
+        for k in range(K):
+            C = arena[ib + li['C'] + k * nn * 8 : ib + li['C'] + (k + 1) * nn * 8].view(np.complex64).reshape(npd, npd)
+            np.testing.assert_array_equal(C[:n, :n], (0.5 * L64[k].conj().T).astype(np.complex64))
+            assert not np.any(C[n:, :]) and not np.any(C[:, n:])
+    del keep, outs, host_out
