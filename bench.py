@@ -39,10 +39,10 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per batched launch of the resident Tsit5 kernel on the default workload
 # (Kerr n=64, B=256), from two rocprofv3 --pmc runs of this script (FETCH_SIZE,
 # then WRITE_SIZE) scaled by the same counters on a device copy of known size
-# (--config copy: FETCH_SIZE x 2, WRITE_SIZE x 1): 133 797 306 KiB fetched,
-# 232 698 948 KiB written (packed FP64 state writes and register spills).
-KERR_TRAFFIC_BYTES = 133_797_306 * 1024 * 2 + 232_698_948 * 1024
-KERR_TRAFFIC_SOURCE = 'rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this script, build of 0eaabea'
+# (--config copy: FETCH_SIZE x 2, WRITE_SIZE x 1): 124 576 769 KiB fetched,
+# 232 674 728 KiB written (packed FP64 state writes and register spills).
+KERR_TRAFFIC_BYTES = 124_576_769 * 1024 * 2 + 232_674_728 * 1024
+KERR_TRAFFIC_SOURCE = 'rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this script, build of c347621'
 
 
 def kerr_instances(B: int, n: int = 64, seed: int = 0):
@@ -931,6 +931,7 @@ def secondary_c4(pool_jobs, nreq: int = 1000, rate: float = 700.0, nsample: int 
             'p99_latency_s': float(np.percentile(lat, 99)),
             'gpu_busy_fraction': st.get('busy_s', 0.0) / wall,
             'launches': st['launches'],
+            'scheduler_host_s': {k: round(v, 4) for k, v in srv.host_s.items()},
         })
         if r > 0:
             sampled = {i: got[i] for i in sample}

@@ -56,6 +56,7 @@ ME_INST_FIELDS = [
     ('mod_t0', np.float64),
     ('mod_dt', np.float64),
     ('mod_nint', np.int64),
+    ('hdiag', np.uint64),
 ]
 MAX_MOD = 4
 ME_INST_DTYPE = np.dtype(ME_INST_FIELDS)
@@ -83,6 +84,7 @@ SYMBOLS = {
     'dqh_mesolve_tsit5_dense': (ctypes.c_int, [_P, _I64, _P]),
     'dqh_mesolve_rouchon1_dense': (ctypes.c_int, [_P, _I64, _P]),
     'dqh_mesolve_tsit5_dense_uniform': (ctypes.c_int, [_P, _I64, _I64, _P]),
+    'dqh_mesolve_tsit5_dense_uniform_fp64diag': (ctypes.c_int, [_P, _I64, _I64, _P]),
     'dqh_tsit5_grid_scratch_bytes': (_I64, [_I64, _I64]),
     'dqh_mesolve_tsit5_dense_grid': (ctypes.c_int, [_P, _P, _I64, _P, _I64, _I64, _P, _P]),
     'dqh_sizeof_dia_rouchon1': (_I64, []),

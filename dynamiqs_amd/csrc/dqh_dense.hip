@@ -636,17 +636,20 @@ __device__ __forceinline__ void signal_done(const dqh_me_inst &p) {
 __global__ __launch_bounds__(kThreads) void k_tsit5(const dqh_me_inst *__restrict__ insts) {
   __shared__ double sh[kWaves];
   __shared__ float2 lds[kResLdsFloat2];
+  __shared__ double2 dg[64];
   const dqh_me_inst &p = insts[blockIdx.x];
   const int np = (int)p.np;
   const bool res = p.nL <= kResMaxL && !(p.flags & (DQH_FLAG_GLOBAL | DQH_FLAG_LMOD));
-  if (res && np == 16)
-    tsit5_resident<16, kWaves>(p, sh, lds);
+  if (p.hdiag != 0) {  // the FP64 diagonal runs on k_tsit5_res<NP, true> only
+    if (threadIdx.x == 0) gp<int64_t>(p.stats)[3] = DQH_UNSUPPORTED;
+  } else if (res && np == 16)
+    tsit5_resident<16, kWaves, false>(p, sh, lds, dg);
   else if (res && np == 32)
-    tsit5_resident<32, kWaves>(p, sh, lds);
+    tsit5_resident<32, kWaves, false>(p, sh, lds, dg);
   else if (res && np == 48)
-    tsit5_resident<48, kWaves>(p, sh, lds);
+    tsit5_resident<48, kWaves, false>(p, sh, lds, dg);
   else if (res && np == 64)
-    tsit5_resident<64, kWaves>(p, sh, lds);
+    tsit5_resident<64, kWaves, false>(p, sh, lds, dg);
   else
     tsit5_global<kWaves>(p, sh);
   signal_done(p);
@@ -663,16 +666,18 @@ constexpr int res_waves() {
   return kWaves;
 }
 
-template <int NP>
+// HD: every instance carries its generator's diagonal in FP64 (hdiag).
+template <int NP, bool HD>
 __global__ __launch_bounds__(res_waves<NP>() * 64) void k_tsit5_res(const dqh_me_inst *__restrict__ insts) {
   constexpr int NW = res_waves<NP>();
   __shared__ double sh[NW];
   __shared__ float2 lds[kResLdsFloat2];
+  __shared__ double2 dg[HD ? NP : 1];
   const dqh_me_inst &p = insts[blockIdx.x];
-  if (p.np != NP || p.nL > kResMaxL || (p.flags & (DQH_FLAG_GLOBAL | DQH_FLAG_LMOD))) {
+  if (p.np != NP || p.nL > kResMaxL || (p.flags & (DQH_FLAG_GLOBAL | DQH_FLAG_LMOD)) || (p.hdiag != 0) != HD) {
     if (threadIdx.x == 0) gp<int64_t>(p.stats)[3] = DQH_UNSUPPORTED;
   } else {
-    tsit5_resident<NP, NW>(p, sh, lds);
+    tsit5_resident<NP, NW, HD>(p, sh, lds, dg);
   }
   signal_done(p);
 }
@@ -771,19 +776,28 @@ extern "C" int dqh_mesolve_tsit5_dense(const dqh_me_inst *d_insts, int64_t ninst
   return launch_checked("dqh_mesolve_tsit5_dense", hipGetLastError());
 }
 
-extern "C" int dqh_mesolve_tsit5_dense_uniform(const dqh_me_inst *d_insts, int64_t ninst, int64_t np, void *stream) {
-  if (!d_insts || ninst <= 0 || ninst > (1 << 30))
-    return fail(DQH_BAD_ARG, "dqh_mesolve_tsit5_dense_uniform: bad arguments");
+template <bool HD>
+static int launch_uniform(const char *what, const dqh_me_inst *d_insts, int64_t ninst, int64_t np, void *stream) {
+  if (!d_insts || ninst <= 0 || ninst > (1 << 30)) return fail(DQH_BAD_ARG, std::string(what) + ": bad arguments");
   hipStream_t st = (hipStream_t)stream;
   const dim3 grid((unsigned)ninst);
   switch (np) {
-    case 16: hipLaunchKernelGGL(k_tsit5_res<16>, grid, dim3(res_waves<16>() * 64), 0, st, d_insts); break;
-    case 32: hipLaunchKernelGGL(k_tsit5_res<32>, grid, dim3(res_waves<32>() * 64), 0, st, d_insts); break;
-    case 48: hipLaunchKernelGGL(k_tsit5_res<48>, grid, dim3(res_waves<48>() * 64), 0, st, d_insts); break;
-    case 64: hipLaunchKernelGGL(k_tsit5_res<64>, grid, dim3(res_waves<64>() * 64), 0, st, d_insts); break;
-    default: return fail(DQH_BAD_SHAPE, "dqh_mesolve_tsit5_dense_uniform: np must be 16, 32, 48 or 64");
+    case 16: hipLaunchKernelGGL((k_tsit5_res<16, HD>), grid, dim3(res_waves<16>() * 64), 0, st, d_insts); break;
+    case 32: hipLaunchKernelGGL((k_tsit5_res<32, HD>), grid, dim3(res_waves<32>() * 64), 0, st, d_insts); break;
+    case 48: hipLaunchKernelGGL((k_tsit5_res<48, HD>), grid, dim3(res_waves<48>() * 64), 0, st, d_insts); break;
+    case 64: hipLaunchKernelGGL((k_tsit5_res<64, HD>), grid, dim3(res_waves<64>() * 64), 0, st, d_insts); break;
+    default: return fail(DQH_BAD_SHAPE, std::string(what) + ": np must be 16, 32, 48 or 64");
   }
-  return launch_checked("dqh_mesolve_tsit5_dense_uniform", hipGetLastError());
+  return launch_checked(what, hipGetLastError());
+}
+
+extern "C" int dqh_mesolve_tsit5_dense_uniform(const dqh_me_inst *d_insts, int64_t ninst, int64_t np, void *stream) {
+  return launch_uniform<false>("dqh_mesolve_tsit5_dense_uniform", d_insts, ninst, np, stream);
+}
+
+extern "C" int dqh_mesolve_tsit5_dense_uniform_fp64diag(const dqh_me_inst *d_insts, int64_t ninst, int64_t np,
+                                                        void *stream) {
+  return launch_uniform<true>("dqh_mesolve_tsit5_dense_uniform_fp64diag", d_insts, ninst, np, stream);
 }
 
 extern "C" int dqh_mesolve_rouchon1_dense(const dqh_me_inst *d_insts, int64_t ninst, void *stream) {

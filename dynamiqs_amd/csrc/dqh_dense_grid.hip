@@ -536,6 +536,9 @@ extern "C" int dqh_mesolve_tsit5_dense_grid(const dqh_me_inst *d_insts, const dq
     if (p.npwc != 0 || p.nmod != 0 || p.nL < 0 || p.nL > 8 || p.ntsave < 1 || !p.work || !p.y || !p.tsave || !p.stats ||
         !p.states || !p.final_state || (p.nL > 0 && (!p.Dops || !p.Cops)))
       return fail(DQH_BAD_ARG, "dqh_mesolve_tsit5_dense_grid: constant generators only (npwc == nmod == 0), nL <= 8");
+    if (p.hdiag != 0)
+      return fail(DQH_BAD_ARG, "dqh_mesolve_tsit5_dense_grid: descriptors with hdiag (FP64 diagonal) run on "
+                               "dqh_mesolve_tsit5_dense_uniform_fp64diag only");
     if (p.nE > 0 && (!(p.flags & DQH_FLAG_SAVE_STATES) || !p.EopsT || !p.expects))
       return fail(DQH_BAD_ARG, "dqh_mesolve_tsit5_dense_grid: expectation values need saved states");
     if (!(p.t1 >= p.t0) || !(p.rtol + p.atol > 0))

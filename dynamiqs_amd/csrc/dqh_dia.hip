@@ -267,8 +267,33 @@ __global__ __launch_bounds__(256) void k_blk_left(int64_t n, Blocks bk, const fl
   }
 }
 
+// K2 launch shape: threads per workgroup and the widest strip (rows of Y in LDS,
+// W * n * 8 bytes), which together set how many workgroups share a CU.
+#ifndef DQH_BLK_THREADS
+#define DQH_BLK_THREADS 512
+#endif
+#ifndef DQH_BLK_WMAX
+#define DQH_BLK_WMAX 8
+#endif
+constexpr int kBlkThreads = DQH_BLK_THREADS;
+// one-pass form: on or off, and the rows of Y it publishes per barrier (at the
+// two-mode n = 1024 batch of 16: 8 rows 296 us per step, 4 rows 391 us, the two
+// passes 320 us with 512-thread strips and 361 us with 256)
+#ifndef DQH_NORM_FUSE
+#define DQH_NORM_FUSE 1
+#endif
+#ifndef DQH_NORM_ROWS
+#define DQH_NORM_ROWS 8
+#endif
+constexpr bool kNormFuse = DQH_NORM_FUSE != 0;
+constexpr int kNormRows = DQH_NORM_ROWS;
+static_assert(kNormRows == 4 || kNormRows == 8, "DQH_NORM_ROWS must divide the set bounds 8, 16 and 32");
+constexpr int kBlkWMax = DQH_BLK_WMAX;
+static_assert(kBlkThreads == 256 || kBlkThreads == 512 || kBlkThreads == 1024, "DQH_BLK_THREADS");
+static_assert(kBlkWMax == 2 || kBlkWMax == 4 || kBlkWMax == 8, "DQH_BLK_WMAX");
+
 template <int W>
-__global__ __launch_bounds__(256) void k_blk_right(int64_t n, Blocks bk, const float2 *__restrict__ Rm,
+__global__ __launch_bounds__(kBlkThreads) void k_blk_right(int64_t n, Blocks bk, const float2 *__restrict__ Rm,
                                                    const float2 *__restrict__ hi, float2 *D1X,
                                                    float2 *__restrict__ E) {
   extern __shared__ float2 ystrip[];  // W x n: Y[r0 + r][c] at ystrip[r * n + c]
@@ -314,19 +339,20 @@ __global__ __launch_bounds__(256) void k_blk_right(int64_t n, Blocks bk, const f
   }
 }
 
-// K1 and K2 fused for sets of at most D indices and n = blockDim.x columns: one
-// workgroup per (set, instance), each thread one column. The d values of its
-// column in the set's rows stay in registers; for each row l of the set the thread
-// forms D1 and Y = hi + D1 at its column, the workgroup publishes the row of Y in
-// LDS (double-buffered, one barrier per row), and each thread mixes the columns of
-// its own column's set from it and writes E and X. HBM: hi read once, E and X
-// written once (24 bytes per element against 48 for the two passes).
-template <int D>
+// K1 and K2 in one pass, for sets of at most D indices and n = blockDim.x <= 1024
+// columns: one workgroup per (set, instance), each thread one column. The d values
+// of the thread's column in the set's rows are loaded once into registers; the
+// set's rows are then done G at a time: each thread forms D1 and Y = hi + D1 of the
+// G rows at its column, the workgroup publishes those G rows of Y in LDS, and each
+// thread mixes the columns of its own column's set for the G rows at once (one
+// load of each coefficient and index for G rows) and writes E and X. HBM: hi read
+// once, E and X written once (24 bytes per element against 48 for the two passes).
+template <int D, int G>
 __global__ __launch_bounds__(1024) void k_blk_norm(int64_t n, Blocks bk, const float2 *__restrict__ Lm,
                                                    const float2 *__restrict__ Rm, const float2 *__restrict__ hi,
                                                    float2 *__restrict__ Xo, float2 *__restrict__ Eo) {
-  extern __shared__ float2 yrow[];  // 2 x n
-  const int c = threadIdx.x;        // blockDim.x == n
+  extern __shared__ float2 ys[];  // G x n: Y of the current rows
+  const int c = threadIdx.x;      // blockDim.x == n
   const size_t nn = (size_t)n * n;
   const size_t inst = blockIdx.y;
   const int b = blockIdx.x;
@@ -336,42 +362,61 @@ __global__ __launch_bounds__(1024) void k_blk_norm(int64_t n, Blocks bk, const f
   const float2 *H = hi + inst * nn + c;
   float2 *X = Xo + inst * nn + c;
   float2 *E = Eo + inst * nn + c;
-  // the set of this thread's column: its column of T_cb^-1 - I and where the set's
-  // indices are
+  // the set of this thread's column: its column of T_cb^-1 - I (zero above cl) and
+  // the set's indices
   const int cb = bk.of[c], cl = bk.loc[c];
   const int cs0 = bk.start[cb], cd = bk.start[cb + 1] - cs0;
-  const float2 *Rc = Rm + bk.mat[cb] + (int64_t)cl * cd;  // zero above cl
+  const float2 *Rc = Rm + bk.mat[cb] + (int64_t)cl * cd;
   const int32_t *cidx = bk.idx + cs0;
   float2 x[D];
 #pragma unroll
   for (int k = 0; k < D; ++k) x[k] = k < d ? H[(size_t)rows[k] * n] : make_float2(0.f, 0.f);
 #pragma unroll
-  for (int l = 0; l < D; ++l) {
-    if (l < d) {
-      const float2 *Lr = L + (size_t)l * d;
-      float dr = 0.f, di = 0.f;
+  for (int g0 = 0; g0 < D; g0 += G) {
+    if (g0 < d) {
+      float2 dl[G], yo[G];
 #pragma unroll
-      for (int k = l; k < D; ++k) {
-        if (k < d) {
-          const float2 m = Lr[k];
-          dr += m.x * x[k].x - m.y * x[k].y;
-          di += m.x * x[k].y + m.y * x[k].x;
+      for (int r = 0; r < G; ++r) {
+        const int l = g0 + r;
+        float dr = 0.f, di = 0.f;
+        if (l < d) {
+          const float2 *Lr = L + (size_t)l * d;
+#pragma unroll
+          for (int k = l; k < D; ++k) {
+            if (k < d) {
+              const float2 m = Lr[k];
+              dr += m.x * x[k].x - m.y * x[k].y;
+              di += m.x * x[k].y + m.y * x[k].x;
+            }
+          }
         }
+        dl[r] = make_float2(dr, di);
+        yo[r] = make_float2(x[l].x + dr, x[l].y + di);
+        ys[r * n + c] = yo[r];
       }
-      const float yr = x[l].x + dr, yi = x[l].y + di;
-      float2 *yb = yrow + (l & 1) * n;
-      yb[c] = make_float2(yr, yi);
       __syncthreads();
-      float ar = 0.f, ai = 0.f;
+      float ar[G], ai[G];
+#pragma unroll
+      for (int r = 0; r < G; ++r) ar[r] = ai[r] = 0.f;
       for (int k = cl; k < cd; ++k) {
         const float2 m = Rc[k];
-        const float2 y = yb[cidx[k]];
-        ar += y.x * m.x - y.y * m.y;
-        ai += y.x * m.y + y.y * m.x;
+        const int src = cidx[k];
+#pragma unroll
+        for (int r = 0; r < G; ++r) {
+          const float2 y = ys[r * n + src];
+          ar[r] += y.x * m.x - y.y * m.y;
+          ai[r] += y.x * m.y + y.y * m.x;
+        }
       }
-      const size_t g = (size_t)rows[l] * n;
-      E[g] = make_float2(dr + ar, di + ai);
-      X[g] = make_float2(yr + ar, yi + ai);
+#pragma unroll
+      for (int r = 0; r < G; ++r) {
+        if (g0 + r < d) {
+          const size_t g = (size_t)rows[g0 + r] * n;
+          E[g] = make_float2(dl[r].x + ar[r], dl[r].y + ai[r]);
+          X[g] = make_float2(yo[r].x + ar[r], yo[r].y + ai[r]);
+        }
+      }
+      __syncthreads();  // the next rows overwrite ys
     }
   }
 }
@@ -472,6 +517,13 @@ __device__ __forceinline__ bool row_elem_xcd(int64_t n, int &i, int &j) {
   j = (int)((b - r * cpr) * 256u + threadIdx.x);
   return j < n;
 }
+
+// Largest number of instances one thread of the global sweep updates (4 or 8).
+#ifndef DQH_KRAUS_IB
+#define DQH_KRAUS_IB 4
+#endif
+constexpr int kKrausIB = DQH_KRAUS_IB;
+static_assert(kKrausIB == 4 || kKrausIB == 8, "DQH_KRAUS_IB must be 4 or 8");
 
 // X at one element (i, j) of IB instances (instance stride nn; nib of them valid,
 // the rest read the last valid instance and are discarded by the caller). A
@@ -688,7 +740,7 @@ extern "C" int dqh_mesolve_rouchon1_dia(const dqh_dia_rouchon1 *pb, void *stream
   // block form of the normalisation: strip width so that a strip of n elements fits
   // 64 KiB of LDS
   const bool blocks = p.nblk > 0;
-  const int bw = n <= 1024 ? 8 : (n <= 2048 ? 4 : 2);
+  const int bw = std::min(dia::kBlkWMax, n <= 1024 ? 8 : (n <= 2048 ? 4 : 2));
   if (blocks && (n > 4096 || p.nblk > n || p.blk_maxd < 1 || p.blk_maxd > DQH_DIA_MAXBLK || p.ninst > 65535 || !p.blk_of || !p.blk_loc || !p.blk_start || !p.blk_idx || !p.blk_mat || !p.Lm[0] ||
                  !p.Rm[0] || (p.nsets > 1 && (!p.Lm[1] || !p.Rm[1]))))
     return fail(DQH_BAD_ARG, "dqh_mesolve_rouchon1_dia: block normalisation needs n <= 4096, 1 <= blk_maxd <= DQH_DIA_MAXBLK, ninst <= 65535 and every block array");
@@ -697,11 +749,11 @@ extern "C" int dqh_mesolve_rouchon1_dia(const dqh_dia_rouchon1 *pb, void *stream
                  reinterpret_cast<const int64_t *>(p.blk_mat)};
   const dim3 gblk((unsigned)(n / bw), (unsigned)p.ninst);
   const dim3 gleft((unsigned)((n + 255) / 256), (unsigned)std::max<int64_t>(p.nblk, 1), (unsigned)p.ninst);
-  // both passes in one kernel when a workgroup can hold every column (n <= 1024)
-  // and the sets are small enough for their column values to stay in registers
-  const bool fused = blocks && n <= 1024 && p.blk_maxd <= 32;
+  // one pass (k_blk_norm) when a workgroup holds every column and the sets are
+  // small enough for their column values to stay in registers
+  const bool fused = blocks && dia::kNormFuse && n <= 1024 && p.blk_maxd <= 32;
   const dim3 gnorm((unsigned)std::max<int64_t>(p.nblk, 1), (unsigned)p.ninst);
-  const size_t norm_lds = 2 * (size_t)n * sizeof(float2);
+  const size_t norm_lds = (size_t)dia::kNormRows * n * sizeof(float2);
   const size_t blk_lds = (size_t)n * bw * sizeof(float2);
   const dim3 gsolve((unsigned)(n / panel), (unsigned)p.ninst);
   const dim3 gtile((unsigned)((n / diat::kTI) * (n / diat::kTJ)), (unsigned)p.ninst);
@@ -728,8 +780,8 @@ extern "C" int dqh_mesolve_rouchon1_dia(const dqh_dia_rouchon1 *pb, void *stream
   hipEvent_t ev[3 * 8] = {};
   for (int64_t k = 0; k < 3 * t_count; ++k)
     if (hipEventCreate(&ev[k]) != hipSuccess) return fail(DQH_HIP_ERROR, "dqh_mesolve_rouchon1_dia: hipEventCreate");
-  // global sweep: IB instances per thread (4 from a batch of 4 onwards)
-  const int kib = p.ninst >= 4 ? 4 : (p.ninst >= 2 ? 2 : 1);
+  // global sweep: IB instances per thread (up to dia::kKrausIB)
+  const int kib = p.ninst >= dia::kKrausIB ? dia::kKrausIB : (p.ninst >= 4 ? 4 : (p.ninst >= 2 ? 2 : 1));
   const dim3 gkr(gel.x, (unsigned)((p.ninst + kib - 1) / kib));
   auto sweep = [&](const dqh_dia_ops &ops, const dia::Saves &sv) {
     if (halo == 8)
@@ -741,6 +793,9 @@ extern "C" int dqh_mesolve_rouchon1_dia(const dqh_dia_rouchon1 *pb, void *stream
     else if (halo == 32)
       hipLaunchKernelGGL(dia::k_kraus_tiled<32>, gtile, dim3(diat::kThreads), 0, st, n, p.n_true, ops, X, E, hi, lo,
                          states, p.nsave, sv);
+    else if (kib == 8)
+      hipLaunchKernelGGL(dia::k_kraus<8>, gkr, dim3(256), 0, st, n, p.n_true, p.ninst, ops, X, E, hi, lo, states,
+                         p.nsave, sv);
     else if (kib == 4)
       hipLaunchKernelGGL(dia::k_kraus<4>, gkr, dim3(256), 0, st, n, p.n_true, p.ninst, ops, X, E, hi, lo, states,
                          p.nsave, sv);
@@ -764,11 +819,14 @@ extern "C" int dqh_mesolve_rouchon1_dia(const dqh_dia_rouchon1 *pb, void *stream
       const float2 *Rm = reinterpret_cast<const float2 *>(p.Rm[set]);
       if (fused) {
         if (p.blk_maxd <= 8)
-          hipLaunchKernelGGL(dia::k_blk_norm<8>, gnorm, dim3((unsigned)n), norm_lds, st, n, bk, Lm, Rm, hi, X, E);
+          hipLaunchKernelGGL((dia::k_blk_norm<8, dia::kNormRows>), gnorm, dim3((unsigned)n), norm_lds, st, n, bk, Lm,
+                             Rm, hi, X, E);
         else if (p.blk_maxd <= 16)
-          hipLaunchKernelGGL(dia::k_blk_norm<16>, gnorm, dim3((unsigned)n), norm_lds, st, n, bk, Lm, Rm, hi, X, E);
+          hipLaunchKernelGGL((dia::k_blk_norm<16, dia::kNormRows>), gnorm, dim3((unsigned)n), norm_lds, st, n, bk, Lm,
+                             Rm, hi, X, E);
         else
-          hipLaunchKernelGGL(dia::k_blk_norm<32>, gnorm, dim3((unsigned)n), norm_lds, st, n, bk, Lm, Rm, hi, X, E);
+          hipLaunchKernelGGL((dia::k_blk_norm<32, dia::kNormRows>), gnorm, dim3((unsigned)n), norm_lds, st, n, bk, Lm,
+                             Rm, hi, X, E);
       } else if (p.blk_maxd <= 8)
         hipLaunchKernelGGL(dia::k_blk_left<8>, gleft, dim3(256), 0, st, n, bk, Lm, hi, X);
       else if (p.blk_maxd <= 16)
@@ -780,11 +838,11 @@ extern "C" int dqh_mesolve_rouchon1_dia(const dqh_dia_rouchon1 *pb, void *stream
       if (fused)
         ;
       else if (bw == 8)
-        hipLaunchKernelGGL(dia::k_blk_right<8>, gblk, dim3(256), blk_lds, st, n, bk, Rm, hi, X, E);
+        hipLaunchKernelGGL(dia::k_blk_right<8>, gblk, dim3(dia::kBlkThreads), blk_lds, st, n, bk, Rm, hi, X, E);
       else if (bw == 4)
-        hipLaunchKernelGGL(dia::k_blk_right<4>, gblk, dim3(256), blk_lds, st, n, bk, Rm, hi, X, E);
+        hipLaunchKernelGGL(dia::k_blk_right<4>, gblk, dim3(dia::kBlkThreads), blk_lds, st, n, bk, Rm, hi, X, E);
       else
-        hipLaunchKernelGGL(dia::k_blk_right<2>, gblk, dim3(256), blk_lds, st, n, bk, Rm, hi, X, E);
+        hipLaunchKernelGGL(dia::k_blk_right<2>, gblk, dim3(dia::kBlkThreads), blk_lds, st, n, bk, Rm, hi, X, E);
     } else if (panel == 32) {
       hipLaunchKernelGGL(dia::k_lsolve<32>, gsolve, dim3(dia::kThreads), 0, st, 0, n, Vp, Wn, hi, X, X);
       hipLaunchKernelGGL(dia::k_lsolve<32>, gsolve, dim3(dia::kThreads), 0, st, 1, n, Vp, Wn, hi, X, E);

@@ -443,8 +443,8 @@ __device__ __forceinline__ void rhs_resident(const dqh_me_inst &p, float2 *lds) 
   }
 }
 
-template <int NP, int NW>
-__device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh, float2 *lds) {
+template <int NP, int NW, bool HD>
+__device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh, float2 *lds, double2 *dg) {
   using R = RCfg<NP, NW>;
   constexpr int NE = R::NEU;
   const int n = (int)p.n, nL = (int)p.nL, ntsave = (int)p.ntsave, nseg = (int)p.nseg;
@@ -477,6 +477,30 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
   for (int q = 0; q < NE; ++q) {
     if (ij[q] >= 0) yg[threadIdx.x + R::NTH * q] = ybase[ij_row(ij[q]) * NP + ij_col(ij[q])];
   }
+
+  // FP64 diagonal of the generator (hdiag): Hnh then holds the off-diagonal part,
+  // and every evaluation adds (d_i + conj(d_j)) ys[i][j] in FP64 to the FP32
+  // products, the stage input ys given in FP64 by ysq(q). Published to the
+  // workgroup by the barriers at the start of the first vector field.
+  if constexpr (HD) {
+    const double2 *hdg = gp<const double2>(p.hdiag);
+    for (int i = threadIdx.x; i < NP; i += R::NTH) dg[i] = hdg[i];
+  }
+  auto add_diag = [&](float2 (&out)[NE], auto ysq) {
+#pragma unroll
+    for (int q = 0; q < NE; ++q) {
+      if (ij[q] >= 0) {
+        const double2 a = dg[ij_row(ij[q])], b = dg[ij_col(ij[q])];
+        const double gr = a.x + b.x, gi = a.y - b.y;
+        const double2 y = ysq(q);
+        out[q] = make_float2((float)((double)out[q].x + (gr * y.x - gi * y.y)),
+                             (float)((double)out[q].y + (gr * y.y + gi * y.x)));
+      }
+      // one element at a time: keeps the FP64 temporaries of the elements from
+      // being live together (the stages already fill the register file)
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  };
 
   float2 k[7][NE];
   float2 tmp[NE];
@@ -511,6 +535,7 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
   }
   res_write_ys_upper<NP, NW>(lds, ij, tmp);
   res_rhs_upper<NP, NW>(Hnh, D, nL, hasH, lds, ij, k[0]);
+  if constexpr (HD) add_diag(k[0], yload);
   ++nrhs;
 
   // initial step (Hairer-Norsett-Wanner II.4), scale atol + rtol |y0|; sums over
@@ -539,6 +564,11 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
     if (smooth) res_update_pwc<NP, NW>(p, seg, t + h0);
     res_write_ys_upper<NP, NW>(lds, ij, tmp);
     res_rhs_upper<NP, NW>(Hnh, D, nL, hasH, lds, ij, k[1]);
+    if constexpr (HD)
+      add_diag(k[1], [&](int q) {
+        const double2 yq = yload(q);
+        return make_double2(yq.x + h0 * k[0][q].x, yq.y + h0 * k[0][q].y);
+      });
     ++nrhs;
     double s2 = 0;
 #pragma unroll
@@ -576,6 +606,7 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
       if (smooth) res_update_pwc<NP, NW>(p, seg, t);
       res_write_ys_upper<NP, NW>(lds, ij, tmp);
       res_rhs_upper<NP, NW>(Hnh, D, nL, hasH, lds, ij, k[0]);
+      if constexpr (HD) add_diag(k[0], yload);
       ++nrhs;
       need_k0 = false;
     }
@@ -623,6 +654,10 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
         for (int j = 0; j < 6; ++j) a0 += (j < s) ? rk.A[s][j] : 0.0;
         c[0] = h * a0;
       }
+      // the stage input: its FP32 rounding goes to LDS for the products, and with
+      // hdiag the rounding residual stays in registers, so that the FP64 diagonal
+      // term after the products sees the FP64 input (hi from LDS + lo)
+      float2 ylo[HD ? NE : 1];
 #pragma unroll
       for (int q = 0; q < NE; ++q) {
         const double2 yq = with_y ? yload(q) : make_double2(0.0, 0.0);
@@ -633,10 +668,16 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
           im += c[j] * k[j][q].y;
         }
         tmp[q] = make_float2((float)re, (float)im);
+        if constexpr (HD) ylo[q] = make_float2((float)(re - (double)tmp[q].x), (float)(im - (double)tmp[q].y));
       }
       if (smooth) res_update_pwc<NP, NW>(p, seg, t + rk.C[s] * h);
       res_write_ys_upper<NP, NW>(lds, ij, tmp);
       res_rhs_upper<NP, NW>(Hnh, D, nL, hasH, lds, ij, tmp);
+      if constexpr (HD)
+        add_diag(tmp, [&](int q) {
+          const float2 hi = lds[ij_row(ij[q]) * R::LDP + ij_col(ij[q])];
+          return make_double2((double)hi.x + ylo[q].x, (double)hi.y + ylo[q].y);
+        });
       ++nrhs;
 #pragma unroll
       for (int j = 1; j < 7; ++j) {

@@ -138,6 +138,11 @@ class _Launch:
         self.t_launch = t_launch
         self.host_out = host_out  # numpy view of the pinned outputs (persistent launches)
         self.answered = [False] * len(reqs)
+        if host_out is not None:
+            # the completion words stats[7] of every request, polled in one gather
+            self.words = host_out.view(np.int64)
+            self.done_at = np.array([(o[2] + 56) // 8 for o in outs], dtype=np.int64)
+            self.pending = np.ones(len(reqs), dtype=bool)
         self.event = None  # completion event (recorded after the kernel)
         self.ev0 = None  # timing event recorded before the launch
         self.ready = None  # set by the worker thread of a grid-wide launch
@@ -209,15 +214,17 @@ def _pack_host(reqs: list, base: int, staging: np.ndarray, descs: np.ndarray, la
             d['step_log'], d['step_log_cap'] = out_base + ob + lo['log'], r.step_log_cap
 
 
-def pack_and_launch(reqs: list, stream, grid: bool = False):
+def pack_and_launch(reqs: list, stream, grid: bool = False, times: dict | None = None):
     """Pack uneven requests and launch them on ``stream``: the persistent Tsit5 kernel
     with outputs in pinned host memory and per-request completion words, or (``grid``)
     a callable that runs the grid-wide Tsit5 on them with device outputs (requests of
     one grid launch share their padded dimension). Returns (tensors kept alive,
     per-request output layout, pinned output view or None, completion event or the
-    callable, device bytes held)."""
+    callable, device bytes held). ``times`` (optional) accumulates the host seconds
+    spent on buffers ('alloc'), packing ('pack') and the copy and launches ('launch')."""
     import torch
 
+    t_start = time.perf_counter()
     dev = torch.device('cuda', torch.cuda.current_device())
     B = len(reqs)
     lays = [_layout(r.n, len(r.Ls), len(r.tsave), len(r.exp_ops), r.step_log_cap) for r in reqs]
@@ -247,8 +254,13 @@ def pack_and_launch(reqs: list, stream, grid: bool = False):
         for b in range(B):
             a = int(out_off[b]) + lays[b][3]['stats']
             host_out[a : a + 64] = 0
+    t_alloc = time.perf_counter()
     _pack_host(reqs, arena.data_ptr(), staging, descs, lays, in_off, out_base, out_off, scratch.data_ptr(), w_off,
                signal=not grid)
+    t_pack = time.perf_counter()
+    if times is not None:
+        times['alloc'] = times.get('alloc', 0.0) + (t_alloc - t_start)
+        times['pack'] = times.get('pack', 0.0) + (t_pack - t_alloc)
     npd = int(descs['np'][0])
     if grid and np.any(descs['np'] != npd):
         raise ValueError('a grid-wide launch needs requests of one padded dimension')
@@ -281,11 +293,15 @@ def pack_and_launch(reqs: list, stream, grid: bool = False):
                 return ev
 
             keep = [oarena, arena, scratch, staging_t, gscratch, flags, h_descs]
+            if times is not None:
+                times['launch'] = times.get('launch', 0.0) + (time.perf_counter() - t_pack)
             return keep, outs, None, run, nbytes + nb
         _native.check(lib.dqh_mesolve_tsit5_dense(c_p(d_descs_ptr), B, c_p(stream.cuda_stream)),
                       'dqh_mesolve_tsit5_dense')
         ev = torch.cuda.Event(enable_timing=True)
         ev.record(stream)
+    if times is not None:
+        times['launch'] = times.get('launch', 0.0) + (time.perf_counter() - t_pack)
     return [oarena, arena, scratch, staging_t], outs, host_out, ev, nbytes
 
 
@@ -365,6 +381,9 @@ class Server:
         self._next_stream = 0
         self.stats = {'launches': 0, 'requests': 0, 'max_inflight_bytes': 0, 'max_concurrent_launches': 0,
                       'answered_before_launch_end': 0}
+        # host seconds of the scheduler thread: buffers, packing, copy + launches, polling
+        # and retirement
+        self.host_s = {'alloc': 0.0, 'pack': 0.0, 'launch': 0.0, 'retire': 0.0}
         # device-busy accounting: a timing event before and after every launch,
         # measured from a reference event recorded now
         self._t_ref = torch.cuda.Event(enable_timing=True)
@@ -451,7 +470,7 @@ class Server:
         ev0 = torch.cuda.Event(enable_timing=True)
         ev0.record(stream)
         try:
-            tensors, outs, host_out, ev, total = pack_and_launch(batch, stream, grid=grid)
+            tensors, outs, host_out, ev, total = pack_and_launch(batch, stream, grid=grid, times=self.host_s)
         except Exception as e:  # answer the batch with the error, keep serving
             for r in batch:
                 r.future.set_exception(e)
@@ -502,16 +521,20 @@ class Server:
         for L in list(self._inflight):
             if L.host_out is not None:
                 # per request: stats[7] is set by the kernel once that request is done
-                for b, out in enumerate(L.outs):
-                    if not L.answered[b] and L.host_out[out[2] + 56 : out[2] + 64].view(np.int64)[0] == 1:
+                # (all words of the launch read in one gather)
+                newly = np.flatnonzero(L.pending & (L.words[L.done_at] == 1))
+                if len(newly):
+                    before_end = not L.done()
+                    for b in newly.tolist():
+                        L.pending[b] = False
                         L.answered[b] = True
                         self.stats['requests'] += 1
                         # the copy out of pinned memory runs on a host thread (the
                         # launch object keeps the pinned buffer alive until then)
-                        _pool().submit(self._answer_pinned, L, b, out)
-                        progress = True
-                        if not L.done():
+                        _pool().submit(self._answer_pinned, L, b, L.outs[b])
+                        if before_end:
                             self.stats['answered_before_launch_end'] += 1
+                    progress = True
             finished = (L.ready is not None and L.ready.is_set() and L.error is not None) or L.done()
             if not finished:
                 continue
@@ -548,7 +571,9 @@ class Server:
                     got = True
             except queue.Empty:
                 pass
+            t0 = time.perf_counter()
             retired = self._retire()
+            self.host_s['retire'] += time.perf_counter() - t0
             self._admit()
             if not (got or retired):
                 time.sleep(0.0002)

@@ -88,6 +88,16 @@ typedef struct dqh_me_inst {
   uint64_t mod_coef;  /* const double2[nmod*mod_nint*4]: c0..c3 per interval        */
   double mod_t0, mod_dt;
   int64_t mod_nint;
+  /* optional (0: off), resident Tsit5 body only (np <= 64, nL <= 3, no
+   * DQH_FLAG_GLOBAL / DQH_FLAG_LMOD): the diagonal of the constant generator part
+   * in FP64, with Hnh holding its off-diagonal part. Every vector-field evaluation
+   * then adds (d_i + conj(d_j)) rho_s[i][j] in FP64 to the FP32 products, so the
+   * largest single terms (the diagonal energies and decay rates) carry no FP32
+   * rounding. Only dqh_mesolve_tsit5_dense_uniform_fp64diag takes such
+   * descriptors: dqh_mesolve_tsit5_dense and dqh_mesolve_tsit5_dense_uniform set
+   * their status to DQH_UNSUPPORTED, dqh_mesolve_tsit5_dense_grid returns
+   * DQH_BAD_ARG. */
+  uint64_t hdiag;     /* const double2[np] */
 } dqh_me_inst;
 
 #define DQH_MAX_MOD 4
@@ -144,6 +154,13 @@ int dqh_mesolve_rouchon1_dense(const dqh_me_inst *d_insts, int64_t ninst, void *
  * that holds only the LDS/register-resident body of that size. An instance that
  * does not match gets status DQH_UNSUPPORTED in its stats and is not solved. */
 int dqh_mesolve_tsit5_dense_uniform(const dqh_me_inst *d_insts, int64_t ninst, int64_t np, void *stream);
+
+/* dqh_mesolve_tsit5_dense_uniform for descriptors that all carry hdiag (the
+ * generator's diagonal in FP64, Hnh its off-diagonal part): every vector-field
+ * evaluation adds the diagonal term in FP64, so the largest single terms carry no
+ * FP32 rounding. Same solve as the reference's Tsit5 mesolve (diffrax_integrator.py
+ * :101-147 with the vector field of :319-323), at lower rounding error. */
+int dqh_mesolve_tsit5_dense_uniform_fp64diag(const dqh_me_inst *d_insts, int64_t ninst, int64_t np, void *stream);
 
 /* The same Tsit5 solves spread over the whole grid: every stage of every
  * instance is a set of grid-wide launches (MFMA tiles of the two GEMM phases, an
@@ -228,7 +245,7 @@ typedef struct dqh_dia_rouchon1 {
    * in ascending order), the Cholesky factor of S is block-diagonal on those sets, and
    * T^-dag rho T^-1 is two memory-bound passes: a pass applying T_b^-dag - I to the
    * rows of each set, and a row-strip pass applying T_b^-1 - I to its columns (one
-   * fused pass, a workgroup per set, when n <= 1024 and every set has at most 32
+   * pass, a workgroup per set, when n <= 1024 and every set has at most 32
    * indices). Needs n <= 4096. */
   int64_t nblk;        /* number of index sets (0: banded form) */
   int64_t blk_maxd;    /* largest set size (<= DQH_DIA_MAXBLK) */

@@ -212,3 +212,40 @@ def test_tsit5_kerr_matches_fp64_replay_of_its_own_steps():
         saved, _ = tsit5.integrate_steps(f, insts[b].rho0, tsave, out['step_ends'][b])
         err = np.linalg.norm(out['states'][b] - np.stack(saved), axis=(-2, -1)).max()
         assert err <= TOL, (b, err)
+
+
+@pytest.mark.parametrize('fp64_diag', [True, False])
+def test_tsit5_kerr_fp64_diagonal_against_fp64_replay(fp64_diag):
+    """The driven Kerr batch (n = 64, eps2 up to 4, t in [0, 2]) with and without the
+    generator's diagonal in FP64 (hdiag on the single-size resident kernel), each
+    against the complex128 oracle replaying its own accepted steps: both within the
+    1e-5 Frobenius bar at every save time."""
+    import os
+    import sys
+
+    from dynamiqs_amd.integrators import dense
+    from oracle import tsit5
+
+    sys.path.insert(0, os.path.dirname(os.path.dirname(__file__)))
+    import bench
+
+    insts, eps2 = bench.kerr_instances(32, 64)
+    pick = [int(np.argmax(eps2)), int(np.argmin(eps2))]
+    insts = [insts[i] for i in pick]
+    tsave = np.linspace(0.0, 2.0, 41)
+    plan = dense.build_plan('tsit5', insts, tsave, rtol=1e-6, atol=1e-6, step_log_cap=20000, fp64_diag=fp64_diag)
+    assert plan.fp64_diag == fp64_diag and plan.uniform_np == 64
+    plan.run()
+    out = plan.results()
+    assert np.all(out['status'] == 0)
+    for b, inst in enumerate(insts):
+        Hnh = -1j * inst.H - 0.5 * sum(L.conj().T @ L for L in inst.Ls)
+
+        def f(t, y, tseg, Hnh=Hnh, Ls=inst.Ls):
+            tmp = Hnh @ y + 0.5 * sum(L @ y @ L.conj().T for L in Ls)
+            return tmp + tmp.conj().T
+
+        saved, _ = tsit5.integrate_steps(f, inst.rho0, tsave, out['step_ends'][b])
+        err = np.linalg.norm(out['states'][b] - np.stack(saved), axis=(-2, -1)).max()
+        print(f'fp64_diag={fp64_diag} eps2={eps2[pick[b]]:.3f} steps={len(out["step_ends"][b])} replay err {err:.3e}')
+        assert err <= TOL, (b, err)

@@ -45,7 +45,16 @@ def test_descriptors_point_into_plan_allocations(host_device, kind, K, pwc):
         base = plan.arena.tensors[0].data_ptr()
         off = int(plan.descs[1]['Hnh']) - base
         host = plan.arena.tensors[0].numpy()
-        got = host[off : off + 32 * 32 * 8].view(np.complex64).reshape(32, 32)[:n, :n]
+        got = host[off : off + 32 * 32 * 8].view(np.complex64).reshape(32, 32)[:n, :n].astype(np.complex128)
+        if kind == 'tsit5':
+            # the single-size resident kernel takes the diagonal in FP64 (hdiag, exact)
+            # and Hnh holds the rest
+            assert plan.fp64_diag and plan.uniform_np == 32
+            offd = int(plan.descs[1]['hdiag']) - base
+            dg = host[offd : offd + 32 * 16].view(np.complex128)
+            assert not np.any(np.diag(got)) and not np.any(dg[n:])
+            np.testing.assert_array_equal(dg[:n], np.diag(want))
+            got = got + np.diag(dg[:n])
         np.testing.assert_allclose(got, want, rtol=1e-6, atol=1e-5)
 
 
@@ -127,3 +136,37 @@ def test_shared_kraus_sets_built_once_and_guarded(host_device, monkeypatch):
                                key_rho=()) for b in range(3)]
     with pytest.raises(NotImplementedError, match='exceed'):
         dense.build_plan('rouchon1', distinct, tsave, dt=0.01)
+
+
+def test_fp64_diagonal_split_leaves_caller_generator_intact(host_device):
+    """An explicit generator part (Instance.Hnh, as the adjoint's backward solve
+    passes) at a dimension that needs no padding: the plan stages its off-diagonal
+    part and its diagonal in FP64 without writing into the caller's array."""
+    rng = np.random.default_rng(4)
+    n = 16
+    H = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    Hnh = -1j * (H + H.conj().T) - np.diag(rng.uniform(0.5, 2.0, size=n))
+    keep = Hnh.copy()
+    inst = dense.Instance(H=np.zeros((n, n), dtype=complex), Ls=[0.1 * np.eye(n, k=1)], rho0=np.eye(n) / n, Hnh=Hnh)
+    plan = dense.build_plan('tsit5', [inst], np.array([0.0, 1.0]), form='resident')
+    np.testing.assert_array_equal(Hnh, keep)
+    assert plan.fp64_diag and plan.uniform_np == 16
+    host = plan.arena.tensors[0].numpy()
+    base = plan.arena.tensors[0].data_ptr()
+    off = int(plan.descs[0]['Hnh']) - base
+    offd = int(plan.descs[0]['hdiag']) - base
+    got = host[off : off + n * n * 8].view(np.complex64).reshape(n, n)
+    dg = host[offd : offd + n * 16].view(np.complex128)
+    np.testing.assert_array_equal(dg, np.diag(keep))
+    np.testing.assert_array_equal(got, (keep - np.diag(np.diag(keep))).astype(np.complex64))
+
+
+def test_grid_plan_carries_no_fp64_diagonal(host_device, monkeypatch):
+    """The grid-wide Tsit5 does not take the FP64 diagonal: a plan forced onto it
+    stages the full generator part and leaves hdiag at zero."""
+    monkeypatch.setattr(torch, 'zeros', lambda *a, **k: torch.empty(*a, **{kk: v for kk, v in k.items() if kk != 'pin_memory'}).zero_())
+    rng = np.random.default_rng(5)
+    insts = [_inst(rng, 20, 1) for _ in range(2)]
+    plan = dense.build_plan('tsit5', insts, np.array([0.0, 0.5, 1.0]), form='resident', grid_tsit5=True)
+    assert plan.grid is not None and not plan.fp64_diag
+    assert np.all(plan.descs['hdiag'] == 0)
