@@ -68,6 +68,7 @@ FLAG_DOPRI5 = 8
 FLAG_EULER = 16
 FLAG_LMOD = 32
 FLAG_SIGNAL = 64
+FLAG_DOPRI8 = 128
 
 STATUS_OK = 0
 STATUS_MAX_STEPS = -3

@@ -16,6 +16,7 @@
 #include "dqh_common.h"
 #include "dqh_mfma.h"
 #include "dqh_tsit5_tableau.h"
+#include "dqh_dopri8_tableau.h"
 
 namespace dqh {
 
@@ -554,6 +555,215 @@ __device__ __forceinline__ void tsit5_global(const dqh_me_inst &p, double *sh) {
   }
 }
 
+// ------------------------------------------------------------ Dopri8 body
+// Stage slot j of the 13-stage pair: 0..6 in the Tsit5 stage buffers, 7..12 after
+// the Hcur slot (work needs (16 + nL) np^2 elements; no DQH_FLAG_LMOD).
+__device__ __forceinline__ float2 *k8(const View &v, float2 *w, int j) {
+  return j < 7 ? v.kbase + (size_t)j * v.nn : w + (size_t)(10 + v.nL + j - 7) * v.nn;
+}
+
+// Adaptive Dormand-Prince 8(7) (DQH_FLAG_DOPRI8; the reference's dx.Dopri8() in
+// diffrax_integrator.py:101-147): the same vector field, FP64 state and stage
+// combination, RMS error norm and integral controller as the Tsit5 body, with the
+// exponent 1/8 (error order 8) and 13 evaluations per step (no FSAL). Steps land on
+// the save times (and on PWC breakpoints and t1), so every saved state is a step end
+// of the 8th-order solution and no continuous extension is needed.
+template <int NW>
+__device__ __forceinline__ void rk8_global(const dqh_me_inst &p, double *sh) {
+  const RK8Tab &rk = kRK8;
+  View v;
+  make_view(v, p);
+  float2 *w = gp<float2>(p.work);
+  const double *tsave = gp<const double>(p.tsave);
+  const double *ptimes = gp<const double>(p.pwc_times);
+  int64_t *stats = gp<int64_t>(p.stats);
+  const double rtol = p.rtol, atol = p.atol;
+  const double inv_nn = 1.0 / ((double)v.n * v.n);
+  const double t_end = p.t1;
+  const bool hasH = (v.flags & DQH_FLAG_HAS_HNH) != 0;
+  const bool smooth = p.nmod > 0;
+
+  double t = p.t0;
+  int cur = 0;
+  int isave = 0;
+  int64_t nsteps = 0, nacc = 0, nrej = 0, nrhs = 0;
+  int status = 0;
+  int seg = -2;
+  const double w1[1] = {0.0};
+
+  while (isave < v.ntsave && tsave[isave] <= t) {
+    wg_save<NW>(v, p, v.y[cur], 0.0, w1, 0, isave, sh);
+    ++isave;
+  }
+  if (v.npwc > 0 || smooth) {
+    seg = v.npwc > 0 ? pwc_segment(ptimes, v.nseg, t) : -1;
+    wg_update_pwc<NW>(v, p, seg, t);
+  }
+  wg_round_state<NW>(v, v.y[cur]);
+  wg_rhs<NW>(v, v.D, v.C, hasH, k8(v, w, 0));
+  ++nrhs;
+
+  // initial step (Hairer-Norsett-Wanner II.4), exponent 1/8
+  double dt;
+  {
+    double s0 = 0, s1 = 0;
+    for (size_t e = threadIdx.x; e < v.nn; e += NW * 64) {
+      const double2 y0 = v.y[cur][e];
+      const double sc = atol + rtol * sqrt(y0.x * y0.x + y0.y * y0.y);
+      const float2 f0 = k8(v, w, 0)[e];
+      s0 += (y0.x * y0.x + y0.y * y0.y) / (sc * sc);
+      s1 += ((double)f0.x * f0.x + (double)f0.y * f0.y) / (sc * sc);
+    }
+    const double d0 = sqrt(block_sum<NW>(s0, sh) * inv_nn);
+    const double d1 = sqrt(block_sum<NW>(s1, sh) * inv_nn);
+    const double h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
+    for (size_t e = threadIdx.x; e < v.nn; e += NW * 64) {
+      const double2 y0 = v.y[cur][e];
+      const float2 f0 = k8(v, w, 0)[e];
+      v.ys[e] = make_float2((float)(y0.x + h0 * f0.x), (float)(y0.y + h0 * f0.y));
+    }
+    __syncthreads();
+    if (smooth) wg_update_pwc<NW>(v, p, seg, t + h0);
+    wg_rhs<NW>(v, v.D, v.C, hasH, k8(v, w, 1));
+    ++nrhs;
+    double s2 = 0;
+    for (size_t e = threadIdx.x; e < v.nn; e += NW * 64) {
+      const double2 y0 = v.y[cur][e];
+      const double sc = atol + rtol * sqrt(y0.x * y0.x + y0.y * y0.y);
+      const double dr = (double)k8(v, w, 1)[e].x - k8(v, w, 0)[e].x, di = (double)k8(v, w, 1)[e].y - k8(v, w, 0)[e].y;
+      s2 += (dr * dr + di * di) / (sc * sc);
+    }
+    const double d2 = sqrt(block_sum<NW>(s2, sh) * inv_nn) / h0;
+    const double md = fmax(d1, d2);
+    const double h1 = md <= 1e-15 ? fmax(1e-6, h0 * 1e-3) : ctrl_pow(0.01 / md, 1.0 / 8.0);
+    dt = fmin(100.0 * h0, h1);
+  }
+
+  bool need_k0 = false;
+  while (t < t_end) {
+    if (nsteps >= p.max_steps) {
+      status = DQH_MAX_STEPS;
+      break;
+    }
+    if (v.npwc > 0) {
+      const int sg = pwc_segment(ptimes, v.nseg, t);
+      if (sg != seg) {
+        seg = sg;
+        if (!smooth) wg_update_pwc<NW>(v, p, seg, t);
+      }
+    }
+    if (need_k0) {  // k_0 = f(y) at the start of every step (the pair has no FSAL stage)
+      wg_round_state<NW>(v, v.y[cur]);
+      if (smooth) wg_update_pwc<NW>(v, p, seg, t);
+      wg_rhs<NW>(v, v.D, v.C, hasH, k8(v, w, 0));
+      ++nrhs;
+    }
+    need_k0 = true;
+    // clip the step to the next save time, breakpoint or the end time after t
+    double target = t_end;
+    if (isave < v.ntsave && tsave[isave] < target) target = tsave[isave];
+    if (v.npwc > 0) {
+      const int m = pwc_segment(ptimes, v.nseg, t) + 1;
+      if (m < v.nseg && ptimes[m] < target) target = ptimes[m];
+    }
+    double h = dt;
+    double t_new = t + h;
+    bool on_target = false;
+    if (snaps_to(t, h, target)) {
+      h = target - t;
+      t_new = target;
+      on_target = true;
+    }
+    if (!(h > 0.0) || !(h > 1e-14 * fmax(1.0, fabs(t)))) {
+      status = DQH_NONFINITE;
+      break;
+    }
+    const double2 *y = v.y[cur];
+    double2 *ynew = v.y[1 - cur];
+    for (int st = 1; st < kRK8Stages; ++st) {
+      for (size_t e = threadIdx.x; e < v.nn; e += NW * 64) {
+        double re = y[e].x, im = y[e].y;
+        for (int j = 0; j < st; ++j) {
+          const double c = rk.a[st][j];
+          if (c != 0.0) {
+            const float2 kj = k8(v, w, j)[e];
+            re += h * c * kj.x;
+            im += h * c * kj.y;
+          }
+        }
+        v.ys[e] = make_float2((float)re, (float)im);
+      }
+      __syncthreads();
+      if (smooth) wg_update_pwc<NW>(v, p, seg, t + rk.c[st] * h);
+      wg_rhs<NW>(v, v.D, v.C, hasH, k8(v, w, st));
+      ++nrhs;
+    }
+    // y1 (8th order) and the error against the 7th-order solution, in FP64
+    double ssum = 0;
+    for (size_t e = threadIdx.x; e < v.nn; e += NW * 64) {
+      const double2 a = y[e];
+      double yr = a.x, yi = a.y, er = 0, ei = 0;
+      for (int j = 0; j < kRK8Stages; ++j) {
+        const float2 kj = k8(v, w, j)[e];
+        yr += h * rk.b[j] * kj.x;
+        yi += h * rk.b[j] * kj.y;
+        er += rk.e[j] * kj.x;
+        ei += rk.e[j] * kj.y;
+      }
+      er *= h;
+      ei *= h;
+      ynew[e] = make_double2(yr, yi);
+      const double sc = atol + rtol * fmax(sqrt(a.x * a.x + a.y * a.y), sqrt(yr * yr + yi * yi));
+      ssum += (er * er + ei * ei) / (sc * sc);
+    }
+    const double norm = sqrt(block_sum<NW>(ssum, sh) * inv_nn);
+    const bool finite = isfinite(norm);
+    const bool keep = finite && norm < 1.0;
+    double factor;
+    if (!finite)
+      factor = p.factormin;
+    else if (norm == 0.0)
+      factor = p.factormax;
+    else
+      factor = p.safety * ctrl_pow(norm, -1.0 / 8.0);
+    factor = fmin(fmax(factor, keep ? 1.0 : p.factormin), p.factormax);
+    ++nsteps;
+    if (keep) {
+      if (threadIdx.x == 0 && nacc < p.step_log_cap) gp<double>(p.step_log)[nacc] = t_new;
+      ++nacc;
+      t = on_target ? target : t_new;
+      cur = 1 - cur;
+      while (isave < v.ntsave && tsave[isave] <= t) {
+        wg_save<NW>(v, p, v.y[cur], 0.0, w1, 0, isave, sh);
+        ++isave;
+      }
+    } else {
+      ++nrej;
+      need_k0 = false;  // k_0 = f(y) still holds
+    }
+    dt = factor * h;
+  }
+  while (isave < v.ntsave) {
+    wg_save<NW>(v, p, v.y[cur], 0.0, w1, 0, isave, sh);
+    ++isave;
+  }
+  wg_write_final<NW>(v, p, v.y[cur]);
+  if (!(v.flags & DQH_FLAG_SAVE_STATES)) {
+    float2 *stp = gp<float2>(p.states);
+    for (size_t e = threadIdx.x; e < v.nn; e += NW * 64) {
+      const int i = (int)(e / v.np), j = (int)(e % v.np);
+      if (i < v.n && j < v.n) stp[(size_t)i * v.n + j] = to_f2(v.y[cur][e]);
+    }
+  }
+  if (threadIdx.x == 0) {
+    stats[0] = nsteps;
+    stats[1] = nacc;
+    stats[2] = nrej;
+    stats[3] = status;
+    stats[4] = nrhs;
+  }
+}
+
 // -------------------------------------------------------- Rouchon1 kernel
 template <int NW>
 __device__ __forceinline__ void rouchon1_global(const dqh_me_inst &p, double *sh) {
@@ -640,9 +850,13 @@ __global__ __launch_bounds__(kThreads) void k_tsit5(const dqh_me_inst *__restric
   const dqh_me_inst &p = insts[blockIdx.x];
   const int np = (int)p.np;
   const bool res = p.nL <= kResMaxL && !(p.flags & (DQH_FLAG_GLOBAL | DQH_FLAG_LMOD));
-  if (p.hdiag != 0) {  // the FP64 diagonal runs on k_tsit5_res<NP, true> only
+  if (p.hdiag != 0 || ((p.flags & DQH_FLAG_DOPRI8) && (p.flags & DQH_FLAG_LMOD))) {
+    // the FP64 diagonal runs on k_tsit5_res<NP, true> only; Dopri8 takes no
+    // modulated jump operators
     if (threadIdx.x == 0) gp<int64_t>(p.stats)[3] = DQH_UNSUPPORTED;
-  } else if (res && np == 16)
+  } else if (p.flags & DQH_FLAG_DOPRI8)
+    rk8_global<kWaves>(p, sh);
+  else if (res && np == 16)
     tsit5_resident<16, kWaves, false>(p, sh, lds, dg);
   else if (res && np == 32)
     tsit5_resident<32, kWaves, false>(p, sh, lds, dg);

@@ -536,6 +536,8 @@ extern "C" int dqh_mesolve_tsit5_dense_grid(const dqh_me_inst *d_insts, const dq
     if (p.npwc != 0 || p.nmod != 0 || p.nL < 0 || p.nL > 8 || p.ntsave < 1 || !p.work || !p.y || !p.tsave || !p.stats ||
         !p.states || !p.final_state || (p.nL > 0 && (!p.Dops || !p.Cops)))
       return fail(DQH_BAD_ARG, "dqh_mesolve_tsit5_dense_grid: constant generators only (npwc == nmod == 0), nL <= 8");
+    if (p.flags & DQH_FLAG_DOPRI8)
+      return fail(DQH_BAD_ARG, "dqh_mesolve_tsit5_dense_grid: Dopri8 instances run on dqh_mesolve_tsit5_dense");
     if (p.hdiag != 0)
       return fail(DQH_BAD_ARG, "dqh_mesolve_tsit5_dense_grid: descriptors with hdiag (FP64 diagonal) run on "
                                "dqh_mesolve_tsit5_dense_uniform_fp64diag only");

@@ -411,9 +411,11 @@ def build_plan(
     per instance: True, False, or 'auto' (when the batch is too small to fill the
     GPU with one workgroup per instance and n >= GRID_MIN_N).
 
-    ``tableau`` is the 5(4) pair of the adaptive kinds: 'tsit5' (``Tsit5``) or
-    'dopri5' (``Dopri5``, ``DQH_FLAG_DOPRI5``); both use the same controller,
-    error norm and saving.
+    ``tableau`` is the embedded pair of the adaptive kinds: 'tsit5' (``Tsit5``),
+    'dopri5' (``Dopri5``, ``DQH_FLAG_DOPRI5``), both with the same controller, error
+    norm and dense-output saving, or 'dopri8' (``Dopri8``, ``DQH_FLAG_DOPRI8``: the
+    Dormand-Prince 8(7) pair on the global-memory form, exponent 1/8, steps landing
+    on the save times).
 
     ``rouchon_order`` selects the fixed-step map of kind 'rouchon1': 1, 2, 3 for
     Rouchon1/2/3 (Kraus operators at the step midpoint, optionally normalised) and
@@ -427,8 +429,11 @@ def build_plan(
     FP64 oracle on stiff problems such as the driven Kerr batch)."""
     import torch
 
-    if tableau not in ('tsit5', 'dopri5'):
+    if tableau not in ('tsit5', 'dopri5', 'dopri8'):
         raise ValueError(f'unknown Runge-Kutta pair {tableau!r}')
+    dopri8 = tableau == 'dopri8' and kind == 'tsit5'
+    if dopri8 and any(lm is not None for inst in instances for lm in (inst.Lmod or [])):
+        raise NotImplementedError('Dopri8 with time-dependent jump operators is not supported.')
     lmod_any = any(lm is not None for inst in instances for lm in (inst.Lmod or []))
     if (any(inst.mod for inst in instances) or lmod_any) and kind != 'rouchon1':
         if max(len(_hmods(inst)) for inst in instances) > _native.MAX_MOD:
@@ -466,10 +471,12 @@ def build_plan(
     if form not in ('auto', 'resident', 'global'):
         raise ValueError("form must be 'auto', 'resident' or 'global'")
     use_global = form == 'global'
-    if form == 'auto' and kind in ('tsit5', 'rhs'):
+    if form == 'auto' and kind in ('tsit5', 'rhs') and not dopri8:
         from .. import autotune
 
         use_global = autotune.dense_global(npad, K, B)
+    if dopri8:  # the 13-stage pair runs on the global-memory form only
+        use_global = True
     descs = np.zeros(B, dtype=_native.ME_INST_DTYPE)
 
     p_tsave = arena.put(tsave)
@@ -512,7 +519,7 @@ def build_plan(
     # the FP64 diagonal goes with the single-size resident kernel (every instance of
     # the plan on it), never with the grid-wide kernels
     npwc_all = max((len(inst.pwc) for inst in instances), default=0)
-    grid_plan = kind == 'tsit5' and use_grid(grid_tsit5, n, B, npwc_all + nmod_max, nE, save_states)
+    grid_plan = kind == 'tsit5' and not dopri8 and use_grid(grid_tsit5, n, B, npwc_all + nmod_max, nE, save_states)
     hdiag_plan = (fp64_diag and kind == 'tsit5' and npad in (16, 32, 48, 64) and K <= 3 and not use_global
                   and not lmod_any and not grid_plan)
     t_stop = float(tsave[-1])
@@ -564,6 +571,8 @@ def build_plan(
             d['flags'] |= _native.FLAG_HAS_HNH
             if tableau == 'dopri5':
                 d['flags'] |= _native.FLAG_DOPRI5
+            elif tableau == 'dopri8' and kind == 'tsit5':
+                d['flags'] |= _native.FLAG_DOPRI8
             lmod = [lm is not None for lm in (inst.Lmod or [])]
             if use_global or any(lmod):
                 d['flags'] |= _native.FLAG_GLOBAL
@@ -696,6 +705,8 @@ def build_plan(
     welems = _native.work_elems(npad, nLdev, npwc_max + nmod_max)
     if lmod_any:  # per-stage copies of D and C (DQH_FLAG_LMOD)
         welems += 2 * nLdev * nn
+    if dopri8:  # stages 7..12 after the generator slot: (16 + nL) np^2
+        welems = max(welems, (16 + nLdev) * nn)
     work = arena.empty(B * welems * 8)
     y = torch.zeros((B, 2, nn, 2), dtype=torch.float64, device=dev)
     y0 = np.zeros((B, npad, npad), dtype=np.complex128)

@@ -19,7 +19,7 @@ import numpy as np
 
 from .._checks import check_hermitian, check_qarray_is_dense, check_shape, check_times
 from ..gradient import Gradient, reject_gradient
-from ..method import Dopri5, Euler, Expm, Method, Rouchon1, Rouchon2, Rouchon3, Tsit5, _DEAdaptiveStep
+from ..method import Dopri5, Dopri8, Euler, Expm, Method, Rouchon1, Rouchon2, Rouchon3, Tsit5, _DEAdaptiveStep
 from ..options import Options, check_options
 from ..qarrays import QArray, asqarray
 from ..result import AdaptiveStepInfos, FixedStepInfos, MESolveResult
@@ -28,8 +28,12 @@ from . import dense
 from . import dia as dia_path
 from .operators import decompose, resolve_callables
 
-SUPPORTED_METHODS = (Tsit5, Dopri5, Euler, Rouchon1, Rouchon2, Rouchon3, Expm)
-ADAPTIVE = (Tsit5, Dopri5)
+SUPPORTED_METHODS = (Tsit5, Dopri5, Dopri8, Euler, Rouchon1, Rouchon2, Rouchon3, Expm)
+ADAPTIVE = (Tsit5, Dopri5, Dopri8)
+
+
+def _tableau(method) -> str:
+    return 'dopri8' if isinstance(method, Dopri8) else ('dopri5' if isinstance(method, Dopri5) else 'tsit5')
 
 
 def assert_method_supported(method: Method, supported_methods) -> None:
@@ -90,6 +94,8 @@ def dia_route(method, H: TimeQArray, Ls, n: int):
     """Operators for the DIA kernels, or None when the dense path applies: Tsit5, or
     Rouchon1 with normalisation, constant un-batched DIA operators, n >= DIA_MIN_N."""
     if n < DIA_MIN_N:
+        return None
+    if isinstance(method, Dopri8):  # the 13-stage pair runs on the dense kernels only
         return None
     if not (isinstance(method, ADAPTIVE) or (isinstance(method, Rouchon1) and method.normalize)):
         return None
@@ -171,8 +177,9 @@ def mesolve(
     Same signature, batching rules, errors and result as ``dq.mesolve``
     (``dynamiqs/integrators/apis/mesolve.py:62-304``). Methods: ``Tsit5`` and
     ``Dopri5`` (adaptive; with ``Options(vectorized=True)`` the Liouvillian as a
-    superoperator, n <= 32), ``Euler``, ``Rouchon1`` and the fixed-step ``Rouchon2``
-    / ``Rouchon3``, ``Expm``. Operators: constant (dense or DIA; constant DIA
+    superoperator, n <= 32), ``Dopri8`` (adaptive 8(7) pair on the dense kernels,
+    n <= 256, steps landing on the save times), ``Euler``, ``Rouchon1`` and the
+    fixed-step ``Rouchon2`` / ``Rouchon3``, ``Expm``. Operators: constant (dense or DIA; constant DIA
     operators from n = 128 on run on the DIA kernels), piecewise-constant
     (``dq.pwc``), modulated and ``timecallable`` Hamiltonians, constant jump
     operators or one modulated term ``f(t) L``. ``gradient`` is validated as in the
@@ -209,6 +216,8 @@ def mesolve(
     Ls = [resolve_callables(L, t_first, float(tsave[-1])) for L in Ls]
 
     if options.vectorized and isinstance(method, ADAPTIVE):
+        if isinstance(method, Dopri8):
+            raise NotImplementedError('Option `vectorized=True` is supported with Tsit5 and Dopri5.')
         return _mesolve_vectorized(H, Ls, rho0, tsave, exp_ops, method, gradient, options)
 
     route = dia_route(method, H, Ls, H.shape[-1])
@@ -248,7 +257,7 @@ def mesolve(
             factormax=method.max_factor,
             max_steps=method.max_steps,
             save_states=save_all,
-            tableau='dopri5' if isinstance(method, Dopri5) else 'tsit5',
+            tableau=_tableau(method),
         )
     else:
         plan = dense.build_plan(

@@ -1,8 +1,8 @@
 """Integration methods, mirroring ``dynamiqs/method.py``.
 
 ``Tsit5`` (:438-476, defaults of ``_DEAdaptiveStep`` :121-127), ``Dopri5``,
-``Euler``, ``Rouchon1`` (:207-237), the fixed-step ``Rouchon2`` / ``Rouchon3`` and
-``Expm`` run on the GPU. The other method names of the reference exist so that the
+``Dopri8`` (mesolve), ``Euler``, ``Rouchon1`` (:207-237), the fixed-step
+``Rouchon2`` / ``Rouchon3`` and ``Expm`` run on the GPU. The other method names of the reference exist so that the
 solvers can raise the reference's ``TypeError`` for the methods they do not
 support.
 """
@@ -79,6 +79,11 @@ class Dopri5(_DEAdaptiveStep):
 
 
 class Dopri8(_DEAdaptiveStep):
+    """Dormand-Prince 8(7) adaptive ODE method (Prince & Dormand 1981, RK8(7)13M):
+    13 stages on the dense global-memory kernel (``DQH_FLAG_DOPRI8``), the same
+    controller with exponent 1/8, steps landing on the save times (the saved states
+    are step ends, not a continuous extension)."""
+
     SUPPORTED_GRADIENT = (Direct, BackwardCheckpointed, Forward)
 
 

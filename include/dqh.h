@@ -120,6 +120,11 @@ typedef struct dqh_me_inst {
  * stats in pinned host memory (device address from dqh_host_device_pointer) can
  * poll stats[7] and take each instance's results while the launch still runs. */
 #define DQH_FLAG_SIGNAL 64
+/* dqh_mesolve_tsit5_dense: integrate the instance with the Dormand-Prince 8(7) pair
+ * (13 stages, error order 8) on the global-memory form, steps landing on the save
+ * times; work needs (16 + nL) np^2 elements; not with DQH_FLAG_LMOD, not on the
+ * uniform or grid-wide entry points. */
+#define DQH_FLAG_DOPRI8 128
 
 /* Size checks for FFI mirrors. */
 int64_t dqh_sizeof_me_inst(void);

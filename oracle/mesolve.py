@@ -123,7 +123,7 @@ def mesolve(
     def save(y):
         return (y.copy() if save_states else None, np.array([tracemm(E, y) for E in Es], dtype=np.complex128))
 
-    if method in ('tsit5', 'dopri5'):
+    if method in ('tsit5', 'dopri5', 'dopri8'):
         if form == 'hermitian':
             rhs = lindblad.rhs_hermitian
         elif form == 'standard':
@@ -137,21 +137,14 @@ def mesolve(
             ts = _segment_time(t, tseg, jumps)
             return rhs(H(ts, t), [L(ts, t) for L in Ls], y)
 
-        saved, ylast, st = tsit5.integrate(
-            f,
-            rho0,
-            tsave,
-            t0=t0,
-            rtol=rtol,
-            atol=atol,
-            safety=safety,
-            factormin=factormin,
-            factormax=factormax,
-            max_steps=max_steps,
-            jump_ts=jumps,
-            save=save,
-            tab=tsit5.tableau(method),
-        )
+        common = dict(t0=t0, rtol=rtol, atol=atol, safety=safety, factormin=factormin, factormax=factormax,
+                      max_steps=max_steps, jump_ts=jumps, save=save)
+        if method == 'dopri8':
+            from . import dopri8
+
+            saved, ylast, st = dopri8.integrate(f, rho0, tsave, **common)
+        else:
+            saved, ylast, st = tsit5.integrate(f, rho0, tsave, tab=tsit5.tableau(method), **common)
         infos = dict(nsteps=st.nsteps, naccepted=st.naccepted, nrejected=st.nrejected, nrhs=st.nrhs)
     elif method in ('rouchon1', 'rouchon2', 'rouchon3', 'euler'):
         if dt is None:

@@ -18,8 +18,10 @@ def _args(n=3):
 
 def test_unsupported_method_type_error():
     H, Ls, psi, ts = _args()
+    with pytest.raises(TypeError, match=re.escape('Method of type `Kvaerno5` is not supported (supported method types:')):
+        dq.mesolve(H, Ls, psi, ts, method=dq.method.Kvaerno5())
     with pytest.raises(TypeError, match=re.escape('Method of type `Dopri8` is not supported (supported method types:')):
-        dq.mesolve(H, Ls, psi, ts, method=dq.method.Dopri8())
+        dq.sesolve(H, psi, ts, method=dq.method.Dopri8())
 
 
 def test_shape_errors():

@@ -884,3 +884,116 @@ def test_sesolve_euler_first_order():
         y.append(y[-1] - 0.1j * H @ y[-1])
     np.testing.assert_allclose(out['states'][1], y[2] + 0.5 * (y[3] - y[2]), atol=1e-14)
     assert out['infos']['nsteps'] == 3
+
+
+def _rooted_trees(n):
+    """Rooted trees with n nodes as sorted tuples of subtrees."""
+    import itertools
+
+    if n == 1:
+        return [()]
+    out = set()
+
+    def parts(m, cap):
+        if m == 0:
+            yield []
+            return
+        for k in range(min(m, cap), 0, -1):
+            for rest in parts(m - k, k):
+                yield [k] + rest
+
+    for p in parts(n - 1, n - 1):
+        for combo in itertools.product(*[_rooted_trees(k) for k in p]):
+            out.add(tuple(sorted(combo)))
+    return sorted(out)
+
+
+def _tree_order(t):
+    return 1 + sum(_tree_order(c) for c in t)
+
+
+def _tree_gamma(t):
+    g = _tree_order(t)
+    for c in t:
+        g *= _tree_gamma(c)
+    return g
+
+
+def test_dopri8_tableau_order_conditions():
+    """RK8(7)13M in exact rationals: every elementary-weight condition
+    b . Phi(t) = 1 / gamma(t) for the 200 rooted trees up to order 8 and, for the
+    embedded weights, the 85 up to order 7 (and not all of order 8); row sums c."""
+    from fractions import Fraction as Fr
+
+    from oracle import dopri8
+
+    A, c, b, bh = dopri8.A_FR, dopri8.C_FR, dopri8.B_FR, dopri8.BHAT_FR
+    s = dopri8.NSTAGES
+    # the published rationals satisfy the conditions to about 1e-16 (they round
+    # irrational optimal values)
+    assert all(abs(float(sum(A[i]) - c[i])) < 1e-15 for i in range(s))
+    memo = {}
+
+    def phi(t):
+        if t not in memo:
+            v = [Fr(1)] * s
+            for ch in t:
+                pc = phi(ch)
+                w = [sum(A[i][j] * pc[j] for j in range(s)) for i in range(s)]
+                v = [v[i] * w[i] for i in range(s)]
+            memo[t] = v
+        return memo[t]
+
+    def residual(w, t):
+        return abs(float(sum(wi * pi for wi, pi in zip(w, phi(t))) - Fr(1, _tree_gamma(t))))
+
+    n8 = n7 = 0
+    for q in range(1, 9):
+        for t in _rooted_trees(q):
+            assert residual(b, t) < 1e-14, (q, t)
+            n8 += 1
+            if q <= 7:
+                assert residual(bh, t) < 1e-14, (q, t)
+                n7 += 1
+    assert n8 == 200 and n7 == 85
+    assert max(residual(bh, t) for t in _rooted_trees(8)) > 1e-6  # bhat is a 7th-order pair member
+    np.testing.assert_allclose(dopri8.E, dopri8.B - np.array([float(x) for x in bh]), atol=1e-17)
+
+
+def test_dopri8_eighth_order_convergence_and_closed_form():
+    from oracle import dopri8
+
+    lam = -1.0 + 3.0j
+
+    def run(h):
+        y = 1.0 + 0j
+        for _ in range(int(round(1.0 / h))):
+            ks = [lam * y]
+            for s in range(1, dopri8.NSTAGES):
+                ks.append(lam * (y + h * sum(dopri8.A[s, j] * ks[j] for j in range(s))))
+            y = y + h * sum(dopri8.B[j] * ks[j] for j in range(dopri8.NSTAGES))
+        return abs(y - np.exp(lam))
+
+    # at least 8th order (2^8 = 256); on the linear test equation the error falls
+    # like h^9 (its stability polynomial matches exp(z) one order further)
+    r = run(0.2) / run(0.1)
+    assert 200 < r < 700, r
+    # adaptive Dopri8 on the lossy cavity against the closed form, and against the
+    # superoperator exponential on a random problem
+    H, Ls, psi0 = _ocavity(16, 1.0, 0.1, 1.0)
+    tsave = np.linspace(0, 2 * np.pi, 11)
+    res = om.mesolve(H, Ls, psi0, tsave, rtol=1e-10, atol=1e-12, method='dopri8')
+    for i, t in enumerate(tsave):
+        al = np.exp(-1j * t - 0.05 * t)
+        assert np.linalg.norm(res['states'][i] - op.coherent_dm(16, al)) < 1e-6
+    rng = np.random.default_rng(8)
+    H, Ls, rho0 = _rand_problem(rng, 6, 2)
+    H /= np.linalg.norm(H, 2)
+    Ls = [0.3 * L / np.linalg.norm(L, 2) for L in Ls]
+    tsave = np.linspace(0, 3, 7)
+    res = om.mesolve(H, Ls, rho0, tsave, rtol=1e-10, atol=1e-12, method='dopri8')
+    Lsup = lindblad.slindbladian(H, Ls)
+    for i, t in enumerate(tsave):
+        ex = lindblad.unvectorize(sla.expm(t * Lsup) @ lindblad.vectorize(rho0))
+        assert np.linalg.norm(res['states'][i] - ex) < 1e-9
+    assert res['infos']['nrhs'] >= 13 * res['infos']['naccepted']
