@@ -12,7 +12,7 @@ INCLUDE = os.path.join(os.path.dirname(HERE), 'include')
 LIB = os.path.join(HERE, 'libdqh.so')
 
 HIP_SOURCES = ['dqh_dense.hip', 'dqh_dense_grid.hip', 'dqh_dia.hip', 'dqh_dia_rhs.hip', 'dqh_dia_tsit5.hip', 'dqh_adjoint.hip',
-               'dqh_sesolve.hip', 'dqh_expm.hip']
+               'dqh_sesolve.hip', 'dqh_expm.hip', 'dqh_rouchon_adaptive.hip']
 CXX_SOURCES = ['dqh_abi.cpp']
 
 # Packed FP32 VALU forms are disabled for every HIP translation unit.

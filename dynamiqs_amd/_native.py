@@ -84,6 +84,10 @@ SYMBOLS = {
     'dqh_dense_prepare_ops': (ctypes.c_int, [_P, _I64, _P]),
     'dqh_mesolve_tsit5_dense': (ctypes.c_int, [_P, _I64, _P]),
     'dqh_mesolve_rouchon1_dense': (ctypes.c_int, [_P, _I64, _P]),
+    'dqh_sizeof_ar_inst': (_I64, []),
+    'dqh_ar_work_elems': (_I64, [_I64]),
+    'dqh_mesolve_rouchon_adaptive_dense': (ctypes.c_int, [_P, _I64, _P]),
+    'dqh_zgemm_dense': (ctypes.c_int, [_P, _P, _P, _I64, _I64, _I64, _I64, _P]),
     'dqh_mesolve_tsit5_dense_uniform': (ctypes.c_int, [_P, _I64, _I64, _P]),
     'dqh_mesolve_tsit5_dense_uniform_fp64diag': (ctypes.c_int, [_P, _I64, _I64, _P]),
     'dqh_tsit5_grid_scratch_bytes': (_I64, [_I64, _I64]),

@@ -178,8 +178,9 @@ def mesolve(
     (``dynamiqs/integrators/apis/mesolve.py:62-304``). Methods: ``Tsit5`` and
     ``Dopri5`` (adaptive; with ``Options(vectorized=True)`` the Liouvillian as a
     superoperator, n <= 32), ``Dopri8`` (adaptive 8(7) pair on the dense kernels,
-    n <= 256, steps landing on the save times), ``Euler``, ``Rouchon1`` and the
-    fixed-step ``Rouchon2`` / ``Rouchon3``, ``Expm``. Operators: constant (dense or DIA; constant DIA
+    n <= 256, steps landing on the save times), ``Euler``, ``Rouchon1``, ``Rouchon2`` /
+    ``Rouchon3`` (fixed step with ``dt``; without ``dt`` the adaptive 1-2 / 2-3 pairs in
+    FP64, n <= 64, constant jump operators), ``Expm``. Operators: constant (dense or DIA; constant DIA
     operators from n = 128 on run on the DIA kernels), piecewise-constant
     (``dq.pwc``), modulated and ``timecallable`` Hamiltonians, constant jump
     operators or one modulated term ``f(t) L``. ``gradient`` is validated as in the
@@ -201,10 +202,8 @@ def mesolve(
     assert_method_supported(method, SUPPORTED_METHODS)
     method.assert_supports_gradient(gradient)
     reject_gradient(gradient, '`dq.mesolve()`')
-    if isinstance(method, (Rouchon2, Rouchon3)) and method.dt is None:
-        raise NotImplementedError(
-            f'the adaptive {type(method).__name__} (dt=None) is not supported by the MI355X solver: set `dt`'
-        )
+    # Rouchon2() / Rouchon3() without dt: the adaptive pairs (rouchon_integrator.py:620-651)
+    adaptive_rouchon = isinstance(method, (Rouchon2, Rouchon3)) and method.dt is None
     if isinstance(method, Expm):  # propagators on MFMA applied to the initial states
         from .propagators import expm_solve
 
@@ -243,7 +242,25 @@ def mesolve(
     # save_extra sees the state at every save time, whatever save_states is
     save_all = options.save_states or options.save_extra is not None
 
-    if isinstance(method, ADAPTIVE):
+    if adaptive_rouchon:
+        from . import rouchon_adaptive
+
+        plan = rouchon_adaptive.build_plan(
+            insts,
+            tsave,
+            order=3 if isinstance(method, Rouchon3) else 2,
+            exp_ops=Es,
+            t0=t0,
+            rtol=method.rtol,
+            atol=method.atol,
+            safety=method.safety_factor,
+            factormin=method.min_factor,
+            factormax=method.max_factor,
+            max_steps=method.max_steps,
+            normalize=method.normalize,
+            save_states=save_all,
+        )
+    elif isinstance(method, ADAPTIVE):
         plan = dense.build_plan(
             'tsit5',
             insts,
@@ -273,6 +290,12 @@ def mesolve(
         )
     plan.run()
     res = plan.results()
+    if adaptive_rouchon:  # FP64 on the device; results in the dtype of the other paths
+        res['states'] = res['states'].astype(np.complex64)
+        if 'expects' in res:
+            res['expects'] = res['expects'].astype(np.complex64)
+    if np.any(res['status'] == -6):
+        raise RuntimeError('an instance was rejected by the MI355X kernel (unsupported shape or options)')
     if np.any(res['status'] == -3):
         raise max_steps_error()
     if np.any(res['status'] == -4):
@@ -295,7 +318,7 @@ def mesolve(
         x = np.asarray(x)[pindex[:, 0]].reshape(out_shape)
         return x if out_shape else x[()]
 
-    if isinstance(method, ADAPTIVE):
+    if isinstance(method, ADAPTIVE) or adaptive_rouchon:
         infos = AdaptiveStepInfos(shaped(res['nsteps']), shaped(res['naccepted']), shaped(res['nrejected']))
     else:
         infos = FixedStepInfos(shaped(res['nsteps']))
@@ -347,6 +370,12 @@ def _mesolve_vectorized(H, Ls, rho0, tsave, exp_ops, method, gradient, options) 
     )
     plan.run()
     res = plan.results()
+    if adaptive_rouchon:  # FP64 on the device; results in the dtype of the other paths
+        res['states'] = res['states'].astype(np.complex64)
+        if 'expects' in res:
+            res['expects'] = res['expects'].astype(np.complex64)
+    if np.any(res['status'] == -6):
+        raise RuntimeError('an instance was rejected by the MI355X kernel (unsupported shape or options)')
     if np.any(res['status'] == -3):
         raise max_steps_error()
     if np.any(res['status'] == -4):

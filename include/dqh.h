@@ -154,6 +154,65 @@ int dqh_mesolve_tsit5_dense(const dqh_me_inst *d_insts, int64_t ninst, void *str
  * tmp + tmp^dag with tmp = sum_k D_k rho C_k to the FP64 state. */
 int dqh_mesolve_rouchon1_dense(const dqh_me_inst *d_insts, int64_t ninst, void *stream);
 
+/* ------------------------------------------------------- adaptive Rouchon */
+/* One adaptive Rouchon solve of dense operators (n <= np <= 64, np % 16 == 0), all
+ * in FP64. Operators in ``ops`` (device complex128 [2 + nL + npwc + nmod][np][np],
+ * zero-padded): G0 = -i H0 - 1/2 sum L^dag L, LdL = sum L^dag L, the jump operators
+ * L_k, then -i P_j of the piecewise-constant terms and -i O_j of the modulated ones,
+ * so that G(tau) = G0 + sum_j c_j(seg) (-i P_j) + sum_j f_j(tau) (-i O_j). */
+typedef struct dqh_ar_inst {
+  int64_t n;           /* Hilbert dimension */
+  int64_t np;          /* padded dimension: 16, 32, 48 or 64 */
+  int64_t nL;          /* jump operators */
+  int64_t nE;          /* expectation operators */
+  int64_t ntsave;
+  int64_t order;       /* 2: Rouchon 1-2 pair (Rouchon2()), 3: Rouchon 2-3 pair (Rouchon3()) */
+  int64_t normalize;   /* 1: Cholesky normalisation of both maps */
+  int64_t flags;       /* DQH_FLAG_SAVE_STATES */
+  int64_t max_steps;
+  int64_t npwc;        /* piecewise-constant terms; pwc_times has nseg breakpoints */
+  int64_t nseg;
+  int64_t nmod;        /* modulated terms (<= DQH_MAX_MOD), tables as in dqh_me_inst */
+  int64_t mod_nint;
+  double rtol, atol, safety, factormin, factormax;
+  double t0, t1;       /* start time; end time = tsave[ntsave - 1] */
+  double dt0;          /* first trial step */
+  double mod_t0, mod_dt;
+  uint64_t ops;        /* device complex128, layout above */
+  uint64_t pwc_times;  /* device double [nseg] */
+  uint64_t pwc_vals;   /* device complex128 [npwc][nseg - 1] */
+  uint64_t mod_coef;   /* device complex128 [nmod][mod_nint][4] */
+  uint64_t EopsT;      /* device complex128 [nE][np][np]: E^T, zero-padded */
+  uint64_t tsave;      /* device double [ntsave], increasing */
+  uint64_t rho;        /* device complex128 [np][np]: rho0 on entry, final state on exit */
+  uint64_t work;       /* device complex128 [dqh_ar_work_elems(np)] */
+  uint64_t states;     /* device complex128 [ntsave][n][n] (DQH_FLAG_SAVE_STATES) */
+  uint64_t expects;    /* device complex128 [nE][ntsave] */
+  uint64_t stats;      /* device int64 [8]: nsteps, naccepted, nrejected, status */
+} dqh_ar_inst;
+
+int64_t dqh_sizeof_ar_inst(void);
+int64_t dqh_ar_work_elems(int64_t np);
+
+/* Adaptive Rouchon2 / Rouchon3 (``Rouchon2()`` / ``Rouchon3()`` without dt:
+ * MESolveAdaptiveRouchonIntegrator, rouchon_integrator.py:455-549, 585-651, with
+ * the PIDController of diffrax_integrator.py:59-72 and step_ts = tsave). Per trial
+ * step of h from t: the no-jump propagators of the embedded pair (midpoint / Euler
+ * or Bogacki-Shampine 3(2)) on dU/dt = G(tau) U, the high and low Kraus maps
+ * (KrausHeun2 / KrausEuler or KrausHeun3 / KrausHeun2, :83-302) each after its own
+ * Cholesky normalisation (:552-582), the error (rho_high - rho_low) / 2 in the RMS
+ * norm with the integral controller (exponent 1/order). Steps land on the save
+ * times and PWC breakpoints; the first trial step is dt0. One workgroup per
+ * instance; status in stats[3] (DQH_MAX_STEPS, DQH_NONFINITE, DQH_UNSUPPORTED). */
+int dqh_mesolve_rouchon_adaptive_dense(const dqh_ar_inst *d_insts, int64_t ninst, void *stream);
+
+/* complex128 products C[b] = op(A[b]) op(B[b]) for b < nbatch (np x np row-major,
+ * np % 16 == 0, np <= 64; op: 0 none, 1 conjugate transpose), on the FP64 MFMA tiles
+ * of dqh_mesolve_rouchon_adaptive_dense. No counterpart in the reference: the
+ * kernel's product exposed for its unit test. */
+int dqh_zgemm_dense(const void *A, const void *B, void *C, int64_t np, int64_t opA, int64_t opB, int64_t nbatch,
+                    void *stream);
+
 /* dqh_mesolve_tsit5_dense for a batch in which every instance has padded dimension
  * np (16, 32, 48 or 64), at most 3 jump operators and no DQH_FLAG_GLOBAL: a kernel
  * that holds only the LDS/register-resident body of that size. An instance that

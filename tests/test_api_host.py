@@ -195,13 +195,18 @@ def test_save_extra_outputs_stacking():
 
 
 def test_rouchon_method_support():
-    """Fixed-step Rouchon2 and Rouchon3 are supported; their adaptive forms
-    (dt=None) raise NotImplementedError before any device work."""
-    H, Ls, psi, ts = _args()
-    with pytest.raises(NotImplementedError, match='adaptive Rouchon2'):
-        dq.mesolve(H, Ls, psi, ts, method=dq.method.Rouchon2())
-    with pytest.raises(NotImplementedError, match='adaptive Rouchon3'):
-        dq.mesolve(H, Ls, psi, ts, method=dq.method.Rouchon3())
+    """Rouchon2 and Rouchon3 run fixed-step with dt and as the adaptive pairs
+    without; adaptive solves outside the kernel's scope (n > 64, time-dependent
+    jump operators) raise NotImplementedError before any device work."""
+    a = dq.destroy(80)
+    psi, ts = dq.fock(80, 0), np.linspace(0.0, 1.0, 3)
+    for m in (dq.method.Rouchon2(), dq.method.Rouchon3()):
+        with pytest.raises(NotImplementedError, match='n <= 64'):
+            dq.mesolve(a.dag() @ a, [a], psi, ts, method=m)
+    b = dq.destroy(3)
+    Lmod = dq.modulated(lambda t: np.cos(t), b)
+    with pytest.raises(NotImplementedError, match='constant jump operators'):
+        dq.mesolve(b.dag() @ b, [Lmod], dq.fock(3, 1), ts, method=dq.method.Rouchon2())
     m = dq.method.Rouchon2(dt=0.01, normalize=False)
     assert (m.dt, m.normalize, m.rtol, m.max_steps) == (0.01, False, 1e-6, 100_000)
 

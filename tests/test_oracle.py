@@ -997,3 +997,128 @@ def test_dopri8_eighth_order_convergence_and_closed_form():
         ex = lindblad.unvectorize(sla.expm(t * Lsup) @ lindblad.vectorize(rho0))
         assert np.linalg.norm(res['states'][i] - ex) < 1e-9
     assert res['infos']['nrhs'] >= 13 * res['infos']['naccepted']
+
+
+# ----------------------------------------------------------------- adaptive Rouchon
+
+
+def _ar_problem():
+    n = 5
+    a = op.destroy(n)
+    H = a.conj().T @ a + 0.2 * (a + a.conj().T)
+    Ls = [0.5 * a, 0.3 * a.conj().T @ a]
+    return H, Ls, op.coherent_dm(n, 0.7)
+
+
+def test_adaptive_rouchon_pair_reduces_to_fixed_step_maps():
+    """Constant operators: the pair's high maps are the fixed-step Rouchon2/3 steps
+    (pinned above), the order-2 low map is Rouchon1, and the embedded Bosh3 solution
+    has the closed form I + hG + h^2/2 G^2 + 3h^3/16 G^3 + h^4/48 G^4, i.e. second
+    order (its h^3 coefficient is not 1/6)."""
+    from oracle import rouchon_adaptive as ora
+
+    H, Ls, rho0 = _ar_problem()
+    h, n = 0.07, H.shape[0]
+    G = -1j * H - 0.5 * sum(L.conj().T @ L for L in Ls)
+    U2 = ora.propagators(ora._as_hf(H), Ls, 0.3, h, 2)
+    np.testing.assert_allclose(U2['high'][1.0], rouchon.rouchon2_propagator(H, Ls, h), atol=1e-14)
+    np.testing.assert_allclose(U2['low'], np.eye(n) + h * G, atol=1e-14)
+    U3 = ora.propagators(ora._as_hf(H), Ls, 0.3, h, 3)
+    ref3 = rouchon.rouchon3_propagators(H, Ls, h)
+    for c, U in ref3.items():
+        np.testing.assert_allclose(U3['high'][c], U, atol=1e-14)
+    G2 = G @ G
+    closed = np.eye(n) + h * G + h**2 / 2 * G2 + 3 * h**3 / 16 * G2 @ G + h**4 / 48 * G2 @ G2
+    np.testing.assert_allclose(U3['low'], closed, atol=1e-13)
+    for normalize in (True, False):
+        hi, err = ora.pair_step(H, Ls, rho0, h, 2, normalize)
+        np.testing.assert_allclose(hi, rouchon.rouchon2_step(H, Ls, rho0, h, normalize), atol=1e-14)
+        np.testing.assert_allclose(hi - 2 * err, rouchon.kraus_step_dense(H, Ls, rho0, h, normalize), atol=1e-14)
+        hi3, _ = ora.pair_step(H, Ls, rho0, h, 3, normalize)
+        np.testing.assert_allclose(hi3, rouchon.rouchon3_step(H, Ls, rho0, h, normalize), atol=1e-14)
+
+
+@pytest.mark.parametrize('order', [2, 3])
+def test_adaptive_rouchon_local_error_orders(order):
+    """Over one step of h: the error estimate scales as h^order (the low map's local
+    error) and the high map's distance to the exact step as h^(order+1)."""
+    from oracle import rouchon_adaptive as ora
+
+    H, Ls, rho0 = _ar_problem()
+    Lsup = lindblad.slindbladian(H, Ls)
+    est, loc = [], []
+    for h in (0.04, 0.02):
+        hi, err = ora.pair_step(H, Ls, rho0, h, order)
+        ex = lindblad.unvectorize(sla.expm(h * Lsup) @ lindblad.vectorize(rho0))
+        est.append(np.linalg.norm(err))
+        loc.append(np.linalg.norm(hi - ex))
+    r_est, r_loc = est[0] / est[1], loc[0] / loc[1]
+    assert 0.75 * 2**order < r_est < 1.3 * 2**order, r_est
+    assert 0.75 * 2 ** (order + 1) < r_loc < 1.3 * 2 ** (order + 1), r_loc
+
+
+@pytest.mark.parametrize('order', [2, 3])
+def test_adaptive_rouchon_converges_to_expm(order):
+    """Lossy driven oscillator: every saved state has unit trace; the error against
+    expm of the Lindbladian falls with the tolerance; steps end on the save times."""
+    from oracle import rouchon_adaptive as ora
+
+    H, Ls, rho0 = _ar_problem()
+    Lsup = lindblad.slindbladian(H, Ls)
+    tsave = np.linspace(0.0, 1.0, 6)
+    exact = [lindblad.unvectorize(sla.expm(t * Lsup) @ lindblad.vectorize(rho0)) for t in tsave]
+    errs = []
+    for tol in (1e-4, 1e-6):
+        states, st = ora.integrate(H, Ls, rho0, tsave, order=order, rtol=tol, atol=tol)
+        assert states.shape == (6, 5, 5)
+        assert np.max(np.abs(np.trace(states, axis1=1, axis2=2) - 1)) < 1e-12
+        assert abs(sum(st.dts) - 1.0) < 1e-12 and st.naccepted + st.nrejected == st.nsteps
+        errs.append(max(np.linalg.norm(s - e) for s, e in zip(states, exact)))
+    assert errs[1] < errs[0] / 5, errs
+    assert errs[1] < 1e-4, errs
+
+
+def test_adaptive_rouchon_otdqubit_closed_form(kats):
+    """The reference's OTDQubit (H = eps cos(w t) sigma_x, one jump operator
+    sqrt(gamma) sigma_x): both pairs against the closed form at the default
+    tolerances, the Hamiltonian taken at every stage time."""
+    from oracle import rouchon_adaptive as ora
+
+    k = kats['otdqubit']
+    eps, w, g = k['eps'], k['omega'], k['gamma']
+    sx = op.sigmax()
+    Hf = lambda tau, tseg: eps * np.cos(w * tau) * sx  # noqa: E731
+    tsave = np.linspace(0, k['t_end'], k['ntsave'])
+    for order in (2, 3):
+        states, _ = ora.integrate(Hf, [np.sqrt(g) * sx], op.fock_dm(2, 0), tsave, order=order)
+        for i, t in enumerate(tsave):
+            th = 2 * eps / w * np.sin(w * t)
+            eta = np.exp(-2 * g * t)
+            want = np.array([[0.5 * (1 + eta * np.cos(th)), 0.5j * eta * np.sin(th)],
+                             [-0.5j * eta * np.sin(th), 0.5 * (1 - eta * np.cos(th))]])
+            assert np.linalg.norm(states[i] - want) < 1e-4, (order, i)
+
+
+def test_adaptive_rouchon_pwc_steps_land_on_breakpoints():
+    """A piecewise-constant drive: steps end on its breakpoints and the solution
+    matches the product of exact segment propagators."""
+    from oracle import rouchon_adaptive as ora
+
+    H, Ls, rho0 = _ar_problem()
+    a = op.destroy(5)
+    X = a + a.conj().T
+    bps = np.array([0.0, 0.33, 0.71, 1.0])
+    vals = np.array([0.4, -0.8, 0.2])
+
+    def Hf(tau, tseg):
+        i = int(np.searchsorted(bps, tseg, side='right')) - 1
+        return H + (vals[i] if 0 <= i < len(vals) else 0.0) * X
+
+    states, st = ora.integrate(Hf, Ls, rho0, [0.0, 1.0], order=3, rtol=1e-8, atol=1e-8, jump_ts=bps)
+    ends = np.cumsum(st.dts)
+    for b in bps[1:-1]:
+        assert np.min(np.abs(ends - b)) < 1e-12
+    r = lindblad.vectorize(rho0)
+    for i in range(3):
+        r = sla.expm((bps[i + 1] - bps[i]) * lindblad.slindbladian(H + vals[i] * X, Ls)) @ r
+    assert np.linalg.norm(states[-1] - lindblad.unvectorize(r)) < 1e-6
