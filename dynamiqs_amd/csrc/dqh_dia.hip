@@ -287,6 +287,10 @@ constexpr int kBlkThreads = DQH_BLK_THREADS;
 #endif
 constexpr bool kNormFuse = DQH_NORM_FUSE != 0;
 constexpr int kNormRows = DQH_NORM_ROWS;
+#ifndef DQH_NORM_BATCH
+#define DQH_NORM_BATCH 4
+#endif
+constexpr int kNormBatch = DQH_NORM_BATCH;  // column coefficients loaded per batch
 static_assert(kNormRows == 4 || kNormRows == 8, "DQH_NORM_ROWS must divide the set bounds 8, 16 and 32");
 constexpr int kBlkWMax = DQH_BLK_WMAX;
 static_assert(kBlkThreads == 256 || kBlkThreads == 512 || kBlkThreads == 1024, "DQH_BLK_THREADS");
@@ -398,14 +402,25 @@ __global__ __launch_bounds__(1024) void k_blk_norm(int64_t n, Blocks bk, const f
       float ar[G], ai[G];
 #pragma unroll
       for (int r = 0; r < G; ++r) ar[r] = ai[r] = 0.f;
-      for (int k = cl; k < cd; ++k) {
-        const float2 m = Rc[k];
-        const int src = cidx[k];
+      // the column's coefficients and indices are loaded kNormBatch at a time, all
+      // issued before the first use (the loop bound differs between the threads)
+      for (int k0 = cl; k0 < cd; k0 += kNormBatch) {
+        float2 mk[kNormBatch];
+        int sk[kNormBatch];
 #pragma unroll
-        for (int r = 0; r < G; ++r) {
-          const float2 y = ys[r * n + src];
-          ar[r] += y.x * m.x - y.y * m.y;
-          ai[r] += y.x * m.y + y.y * m.x;
+        for (int u = 0; u < kNormBatch; ++u) {
+          const bool in = k0 + u < cd;
+          mk[u] = in ? Rc[k0 + u] : make_float2(0.f, 0.f);
+          sk[u] = in ? cidx[k0 + u] : c;
+        }
+#pragma unroll
+        for (int u = 0; u < kNormBatch; ++u) {
+#pragma unroll
+          for (int r = 0; r < G; ++r) {
+            const float2 y = ys[r * n + sk[u]];
+            ar[r] += y.x * mk[u].x - y.y * mk[u].y;
+            ai[r] += y.x * mk[u].y + y.y * mk[u].x;
+          }
         }
       }
 #pragma unroll
