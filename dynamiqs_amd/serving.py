@@ -140,6 +140,7 @@ class _Launch:
         self.answered = [False] * len(reqs)
         if host_out is not None:
             # the completion words stats[7] of every request, polled in one gather
+            # (set by the persistent kernel; a grid-wide launch answers at its end)
             self.words = host_out.view(np.int64)
             self.done_at = np.array([(o[2] + 56) // 8 for o in outs], dtype=np.int64)
             self.pending = np.ones(len(reqs), dtype=bool)
@@ -216,10 +217,10 @@ def _pack_host(reqs: list, base: int, staging: np.ndarray, descs: np.ndarray, la
 
 def pack_and_launch(reqs: list, stream, grid: bool = False, times: dict | None = None):
     """Pack uneven requests and launch them on ``stream``: the persistent Tsit5 kernel
-    with outputs in pinned host memory and per-request completion words, or (``grid``)
-    a callable that runs the grid-wide Tsit5 on them with device outputs (requests of
-    one grid launch share their padded dimension). Returns (tensors kept alive,
-    per-request output layout, pinned output view or None, completion event or the
+    with per-request completion words, or (``grid``) a callable that runs the
+    grid-wide Tsit5 on them (requests of one grid launch share their padded
+    dimension); outputs in pinned host memory either way. Returns (tensors kept
+    alive, per-request output layout, pinned output view, completion event or the
     callable, device bytes held). ``times`` (optional) accumulates the host seconds
     spent on buffers ('alloc'), packing ('pack') and the copy and launches ('launch')."""
     import torch
@@ -241,19 +242,18 @@ def pack_and_launch(reqs: list, stream, grid: bool = False, times: dict | None =
     with torch.cuda.stream(stream):
         arena = torch.empty(n_in + _align(dsz), dtype=torch.uint8, device=dev)
         scratch = torch.empty(max(int(w_off[-1]), 256), dtype=torch.uint8, device=dev)
-        oarena = torch.zeros(int(out_off[-1]), dtype=torch.uint8, device=dev) if grid else None
     descs = staging[n_in : n_in + dsz].view(_native.ME_INST_DTYPE)
     lib = _native.lib()
     c_p = _native.ctypes.c_void_p
-    if grid:
-        out_base, host_out = oarena.data_ptr(), None
-    else:
-        # the kernel writes every output; only the completion words must start at zero
-        oarena = torch.empty(int(out_off[-1]), dtype=torch.uint8, pin_memory=True)
-        out_base, host_out = _native.host_device_pointer(oarena.data_ptr()), oarena.numpy()
-        for b in range(B):
-            a = int(out_off[b]) + lays[b][3]['stats']
-            host_out[a : a + 64] = 0
+    # outputs in pinned host memory that the kernels write through its device
+    # address: a finished request is read on the host with no device copy and no
+    # synchronisation. The kernels write every output; only the stats words (the
+    # completion word of the persistent kernel among them) must start at zero.
+    oarena = torch.empty(int(out_off[-1]), dtype=torch.uint8, pin_memory=True)
+    out_base, host_out = _native.host_device_pointer(oarena.data_ptr()), oarena.numpy()
+    for b in range(B):
+        a = int(out_off[b]) + lays[b][3]['stats']
+        host_out[a : a + 64] = 0
     t_alloc = time.perf_counter()
     _pack_host(reqs, arena.data_ptr(), staging, descs, lays, in_off, out_base, out_off, scratch.data_ptr(), w_off,
                signal=not grid)
@@ -295,7 +295,7 @@ def pack_and_launch(reqs: list, stream, grid: bool = False, times: dict | None =
             keep = [oarena, arena, scratch, staging_t, gscratch, flags, h_descs]
             if times is not None:
                 times['launch'] = times.get('launch', 0.0) + (time.perf_counter() - t_pack)
-            return keep, outs, None, run, nbytes + nb
+            return keep, outs, host_out, run, nbytes + nb
         _native.check(lib.dqh_mesolve_tsit5_dense(c_p(d_descs_ptr), B, c_p(stream.cuda_stream)),
                       'dqh_mesolve_tsit5_dense')
         ev = torch.cuda.Event(enable_timing=True)
@@ -519,7 +519,7 @@ class Server:
     def _retire(self) -> bool:
         progress = False
         for L in list(self._inflight):
-            if L.host_out is not None:
+            if L.host_out is not None and L.ready is None:
                 # per request: stats[7] is set by the kernel once that request is done
                 # (all words of the launch read in one gather)
                 newly = np.flatnonzero(L.pending & (L.words[L.done_at] == 1))
@@ -547,18 +547,13 @@ class Server:
                 L.answered[b] = True
             self.stats['requests'] += len(rest)
             if rest:
-                try:
-                    if L.error is not None:
-                        raise L.error
-                    res = collect(L.tensors[0], [L.outs[b] for b in rest])
-                except Exception as e:
+                if L.error is not None:
                     for b in rest:
-                        L.reqs[b].future.set_exception(e)
-                    res = None
-                if res is not None:
-                    t = time.perf_counter()
-                    for b, rb in zip(rest, res):
-                        self._answer(L, b, rb, t)
+                        L.reqs[b].future.set_exception(L.error)
+                else:
+                    # read out of the pinned outputs on the host threads
+                    for b in rest:
+                        _pool().submit(self._answer_pinned, L, b, L.outs[b])
             self._used -= L.nbytes
         return progress
 
