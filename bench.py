@@ -935,11 +935,32 @@ def secondary_c4(pool_jobs, nreq: int = 1000, rate: float = 700.0, nsample: int 
         })
         if r > 0:
             sampled = {i: got[i] for i in sample}
+            loaded_lat = lat
     for i, g in sampled.items():
         H, Ls, psi, ts = reqs[i]
         pool_jobs.append((H, Ls, psi @ psi.conj().T, ts, g.step_ends, g.states))
+    # every request solved alone (one at a time, nothing else on the device): the
+    # latency floor the loaded latencies are compared with, per quartile of n
+    iso = np.zeros(nreq)
+    with Server(memory_budget=32 << 30) as srv:
+        for i, (H, Ls, psi, ts) in enumerate(reqs):
+            iso[i] = srv.submit(H, Ls, psi, ts, rtol=1e-5).result().latency_s
+    edges = np.percentile(ns, [0, 25, 50, 75, 100])
+    buckets = []
+    for q in range(4):
+        lo, hi = edges[q], edges[q + 1]
+        inb = (ns >= lo) & ((ns < hi) if q < 3 else (ns <= hi))
+        p50, p99 = float(np.percentile(loaded_lat[inb], 50)), float(np.percentile(loaded_lat[inb], 99))
+        i50, i99 = float(np.percentile(iso[inb], 50)), float(np.percentile(iso[inb], 99))
+        buckets.append({'n_range': [int(lo), int(hi)], 'p50_latency_s_at_rate': p50, 'p99_latency_s_at_rate': p99,
+                        'isolated_p50_latency_s': i50, 'isolated_p99_latency_s': i99,
+                        'p99_over_isolated_p99': p99 / i99, 'p99_over_isolated_p50': p99 / i50})
     return {'workload': f'serving mix, {nreq} requests, n log-uniform 8..256 (median {int(np.median(ns))}), K 1-3, '
                         f'T U[1,20], 50 tsave, rtol 1e-5', 'rates': rows,
+            'latency_vs_isolated': {'rate': rate, 'by_n_quartile': buckets,
+                                    'note': 'latencies of each quartile of n at the offered rate against the same '
+                                            'requests solved one at a time (T spans 1-20, so the isolated p99 is '
+                                            'the like-for-like floor of the loaded p99)'},
             'sampled_requests': sorted(int(i) for i in sample),
             'sampled_n': sorted(int(ns[i]) for i in sample)}
 

@@ -108,6 +108,11 @@ SYMBOLS = {
     'dqh_adjoint_pwc_quadrature_dia': (ctypes.c_int, [_I64, _I64, _I64, _I64, _P, _P, _P, _P, _P, _P, _P]),
     'dqh_adjoint_rate_quadrature': (ctypes.c_int, [_I64, _I64, _I64, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P]),
     'dqh_host_device_pointer': (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
+    'dqh_host_alloc': (ctypes.c_int, [_I64, _I64, ctypes.POINTER(_P)]),
+    'dqh_host_free': (ctypes.c_int, [_P]),
+    'dqh_mesolve_tsit5_dense_queue': (ctypes.c_int, [_P, _I64, _P]),
+    'dqh_wall_clock_khz': (_I64, []),
+    'dqh_memcpy_h2d_async': (ctypes.c_int, [_P, _P, _I64, _P]),
     'dqh_last_error': (ctypes.c_char_p, []),
     'dqh_device_info': (ctypes.c_int, [ctypes.c_char_p, _I64, ctypes.POINTER(_I64)]),
 }

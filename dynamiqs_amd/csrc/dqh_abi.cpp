@@ -42,3 +42,19 @@ extern "C" int dqh_host_device_pointer(void *host, void **device) {
   *device = d;
   return DQH_OK;
 }
+
+extern "C" int dqh_host_alloc(int64_t bytes, int64_t coherent, void **host) {
+  if (!host || bytes <= 0) return dqh::fail(DQH_BAD_ARG, "dqh_host_alloc: bad arguments");
+  void *h = nullptr;
+  const unsigned flags = hipHostMallocMapped | (coherent ? hipHostMallocCoherent : hipHostMallocNonCoherent);
+  const hipError_t e = hipHostMalloc(&h, (size_t)bytes, flags);
+  if (e != hipSuccess || !h) return dqh::fail(DQH_HIP_ERROR, std::string("dqh_host_alloc: ") + hipGetErrorString(e));
+  *host = h;
+  return DQH_OK;
+}
+
+extern "C" int dqh_host_free(void *host) {
+  if (!host) return DQH_OK;
+  const hipError_t e = hipHostFree(host);
+  return e == hipSuccess ? DQH_OK : dqh::fail(DQH_HIP_ERROR, std::string("dqh_host_free: ") + hipGetErrorString(e));
+}

@@ -843,11 +843,8 @@ __device__ __forceinline__ void signal_done(const dqh_me_inst &p) {
     __hip_atomic_store(gp<int64_t>(p.stats) + 7, (int64_t)1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-__global__ __launch_bounds__(kThreads) void k_tsit5(const dqh_me_inst *__restrict__ insts) {
-  __shared__ double sh[kWaves];
-  __shared__ float2 lds[kResLdsFloat2];
-  __shared__ double2 dg[64];
-  const dqh_me_inst &p = insts[blockIdx.x];
+// The body that fits one instance (called by every thread of the workgroup).
+__device__ __forceinline__ void tsit5_dispatch(const dqh_me_inst &p, double *sh, float2 *lds, double2 *dg) {
   const int np = (int)p.np;
   const bool res = p.nL <= kResMaxL && !(p.flags & (DQH_FLAG_GLOBAL | DQH_FLAG_LMOD));
   if (p.hdiag != 0 || ((p.flags & DQH_FLAG_DOPRI8) && (p.flags & DQH_FLAG_LMOD))) {
@@ -866,7 +863,142 @@ __global__ __launch_bounds__(kThreads) void k_tsit5(const dqh_me_inst *__restric
     tsit5_resident<64, kWaves, false>(p, sh, lds, dg);
   else
     tsit5_global<kWaves>(p, sh);
+}
+
+__global__ __launch_bounds__(kThreads) void k_tsit5(const dqh_me_inst *__restrict__ insts) {
+  __shared__ double sh[kWaves];
+  __shared__ float2 lds[kResLdsFloat2];
+  __shared__ double2 dg[64];
+  const dqh_me_inst &p = insts[blockIdx.x];
+  tsit5_dispatch(p, sh, lds, dg);
   signal_done(p);
+}
+
+// Operator preparation on the device (diffrax_integrator.py:319-323 builds
+// Hnh = -iH - 1/2 sum L^dag L on every call): on entry Hnh holds -iH and Dops the
+// jump operators L_k (zero-padded to np); on exit Hnh = -iH - 1/2 sum_k D_k^dag D_k,
+// the products accumulated in FP64 from the complex64 D_k (the same D_k the jump
+// term uses), and Cops = D_k^dag / 2. grid: (kPrepTiles, ninst); each workgroup
+// strides over the np^2 elements of its instance (D_k[m][i] is the same for the
+// threads of one row, D_k[m][j] coalesced).
+constexpr int kPrepTiles = 16;
+
+// Hnh <- Hnh - 1/2 sum_k D_k^dag D_k (FP64 products) and C_k = D_k^dag / 2 for the
+// elements e0, e0 + stride, ... of one instance
+__device__ __forceinline__ void prepare_elems(const dqh_me_inst &p, int e0, int stride) {
+  const int np = (int)p.np, nL = (int)p.nL;
+  const int nn = np * np;
+  float2 *Hnh = gp<float2>(p.Hnh);
+  const float2 *D = gp<const float2>(p.Dops);
+  float2 *C = gp<float2>(p.Cops);
+  for (int e = e0; e < nn; e += stride) {
+    const int i = e / np, j = e - i * np;
+    double ar = 0.0, ai = 0.0;
+    for (int k = 0; k < nL; ++k) {
+      const float2 *Dk = D + (size_t)k * nn;
+      for (int m = 0; m < np; ++m) {
+        const float2 a = Dk[(size_t)m * np + i], b = Dk[(size_t)m * np + j];
+        // conj(a) b
+        ar += (double)a.x * b.x + (double)a.y * b.y;
+        ai += (double)a.x * b.y - (double)a.y * b.x;
+      }
+      const float2 t = Dk[(size_t)j * np + i];
+      C[(size_t)k * nn + e] = make_float2(0.5f * t.x, -0.5f * t.y);
+    }
+    const float2 h = Hnh[e];
+    Hnh[e] = make_float2((float)((double)h.x - 0.5 * ar), (float)((double)h.y - 0.5 * ai));
+  }
+}
+
+__global__ __launch_bounds__(256) void k_prepare_ops(const dqh_me_inst *__restrict__ insts) {
+  prepare_elems(insts[blockIdx.y], blockIdx.x * 256 + threadIdx.x, kPrepTiles * 256);
+}
+
+// Work-queue form of k_tsit5 for serving: a persistent grid whose workgroups take
+// requests from a ring that the host keeps filling while the grid runs. A ring slot
+// (kQueueSlotWords words) names a request's staged inputs and descriptor in pinned
+// host memory and where they go in device memory: {src, dst, bytes, src_desc,
+// dst_desc}. Thread 0 claims the next published slot (head < tail, compare-and-swap
+// on head) or, with none published, sleeps and polls again; it gives up when the
+// host sets stop, when no workgroup has claimed a request for idle_ticks of the wall
+// clock, or (before a claim) once the workgroup is older than life_ticks, so every
+// workgroup exits and the launch ends regularly even under load. The workgroup then copies the inputs and the descriptor to the
+// device itself, completes Hnh and C (as dqh_dense_prepare_ops), solves the request
+// with the bodies of k_tsit5 and signals it; no host-side copy or launch is needed
+// to start a request. busy_ticks accumulates the wall-clock ticks spent per request.
+constexpr int kQueueSlotWords = DQH_QUEUE_SLOT_WORDS;
+
+__global__ __launch_bounds__(kThreads) void k_tsit5_queue(dqh_queue q) {
+  __shared__ double sh[kWaves];
+  __shared__ float2 lds[kResLdsFloat2];
+  __shared__ double2 dg[64];
+  __shared__ uint64_t slot[kQueueSlotWords];
+  int64_t *tail = gp<int64_t>(q.tail), *head = gp<int64_t>(q.head), *stop = gp<int64_t>(q.stop);
+  int64_t *last_claim = head + 1;  // wall clock of the latest claim by any workgroup
+  const uint64_t *ring = gp<const uint64_t>(q.ring);
+  const uint64_t t_birth = wall_clock64();
+  for (;;) {
+    if (threadIdx.x == 0) {
+      int64_t got = -1;
+      const uint64_t t_idle = wall_clock64();
+      for (;;) {
+        if (t_idle - t_birth > (uint64_t)q.life_ticks) break;
+        const int64_t t = __hip_atomic_load(tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+        int64_t h = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (h < t) {
+          if (__hip_atomic_compare_exchange_strong(head, &h, h + 1, __ATOMIC_ACQ_REL, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT)) {
+            got = h;
+            __hip_atomic_store(last_claim, (int64_t)wall_clock64(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+          continue;
+        }
+        if (__hip_atomic_load(stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0) break;
+        // idle: no workgroup of the queue has claimed a request for idle_ticks (a
+        // workgroup alone seeing no work under a steady load does not leave)
+        const uint64_t last = (uint64_t)__hip_atomic_load(last_claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t now = wall_clock64();
+        if (now - (last > t_idle ? last : t_idle) > (uint64_t)q.idle_ticks) break;
+        __builtin_amdgcn_s_sleep(64);
+      }
+      const uint64_t *src = ring + (size_t)(got < 0 ? 0 : (got & (q.capacity - 1))) * kQueueSlotWords;
+#pragma unroll
+      for (int w = 0; w < kQueueSlotWords; ++w)
+        slot[w] = got < 0 ? 0 : __hip_atomic_load(src + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+    __syncthreads();
+    if (slot[4] == 0) break;
+    const uint64_t t0 = wall_clock64();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
+    {
+      // staged inputs (16-byte words) and the descriptor, host -> device
+      const uint4 *src = gp<const uint4>(slot[0]);
+      uint4 *dst = gp<uint4>(slot[1]);
+      const int64_t nw = (int64_t)(slot[2] / 16);
+      for (int64_t w = threadIdx.x; w < nw; w += kThreads) dst[w] = src[w];
+      const uint64_t *sd = gp<const uint64_t>(slot[3]);
+      uint64_t *dd = gp<uint64_t>(slot[4]);
+      for (int w = threadIdx.x; w < (int)(sizeof(dqh_me_inst) / 8); w += kThreads) dd[w] = sd[w];
+    }
+    __threadfence();
+    __syncthreads();
+    // readfirstlane returns a signed int: each half goes through unsigned before
+    // widening, or a low word >= 2^31 would sign-extend over the high word
+    const uint32_t su_lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)slot[4]);
+    const uint32_t su_hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(slot[4] >> 32));
+    const uint64_t su = ((uint64_t)su_hi << 32) | (uint64_t)su_lo;
+    const dqh_me_inst &p = *gp<const dqh_me_inst>(su);
+    prepare_elems(p, threadIdx.x, kThreads);
+    __threadfence();
+    __syncthreads();
+    tsit5_dispatch(p, sh, lds, dg);
+    signal_done(p);
+    if (threadIdx.x == 0)
+      __hip_atomic_fetch_add(gp<int64_t>(q.busy_ticks), (int64_t)(wall_clock64() - t0), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();  // slot is rewritten for the next request
+  }
 }
 
 // A batch whose instances all take the resident body for one NP: the kernel holds
@@ -919,41 +1051,6 @@ __global__ __launch_bounds__(kThreads) void k_rouchon1(const dqh_me_inst *__rest
   rouchon1_global<kWaves>(p, sh);
 }
 
-// Operator preparation on the device (diffrax_integrator.py:319-323 builds
-// Hnh = -iH - 1/2 sum L^dag L on every call): on entry Hnh holds -iH and Dops the
-// jump operators L_k (zero-padded to np); on exit Hnh = -iH - 1/2 sum_k D_k^dag D_k,
-// the products accumulated in FP64 from the complex64 D_k (the same D_k the jump
-// term uses), and Cops = D_k^dag / 2. grid: (kPrepTiles, ninst); each workgroup
-// strides over the np^2 elements of its instance (D_k[m][i] is the same for the
-// threads of one row, D_k[m][j] coalesced).
-constexpr int kPrepTiles = 16;
-
-__global__ __launch_bounds__(256) void k_prepare_ops(const dqh_me_inst *__restrict__ insts) {
-  const dqh_me_inst &p = insts[blockIdx.y];
-  const int np = (int)p.np, nL = (int)p.nL;
-  const int nn = np * np;
-  float2 *Hnh = gp<float2>(p.Hnh);
-  const float2 *D = gp<const float2>(p.Dops);
-  float2 *C = gp<float2>(p.Cops);
-  for (int e = blockIdx.x * 256 + threadIdx.x; e < nn; e += kPrepTiles * 256) {
-    const int i = e / np, j = e - i * np;
-    double ar = 0.0, ai = 0.0;
-    for (int k = 0; k < nL; ++k) {
-      const float2 *Dk = D + (size_t)k * nn;
-      for (int m = 0; m < np; ++m) {
-        const float2 a = Dk[(size_t)m * np + i], b = Dk[(size_t)m * np + j];
-        // conj(a) b
-        ar += (double)a.x * b.x + (double)a.y * b.y;
-        ai += (double)a.x * b.y - (double)a.y * b.x;
-      }
-      const float2 t = Dk[(size_t)j * np + i];
-      C[(size_t)k * nn + e] = make_float2(0.5f * t.x, -0.5f * t.y);
-    }
-    const float2 h = Hnh[e];
-    Hnh[e] = make_float2((float)((double)h.x - 0.5 * ar), (float)((double)h.y - 0.5 * ai));
-  }
-}
-
 }  // namespace dqh
 
 using namespace dqh;
@@ -982,6 +1079,29 @@ extern "C" int dqh_lindblad_rhs_dense(const dqh_me_inst *d_insts, int64_t ninst,
   if (!d_insts || ninst <= 0 || ninst > (1 << 30)) return fail(DQH_BAD_ARG, "dqh_lindblad_rhs_dense: bad arguments");
   hipLaunchKernelGGL(k_rhs, dim3((unsigned)ninst), dim3(kThreads), 0, (hipStream_t)stream, d_insts);
   return launch_checked("dqh_lindblad_rhs_dense", hipGetLastError());
+}
+
+extern "C" int dqh_mesolve_tsit5_dense_queue(const dqh_queue *q, int64_t nworkgroups, void *stream) {
+  if (!q || nworkgroups <= 0 || nworkgroups > 65535 || !q->ring || !q->tail || !q->head || !q->stop ||
+      !q->busy_ticks || q->capacity <= 0 || (q->capacity & (q->capacity - 1)) || q->idle_ticks < 0 ||
+      q->life_ticks < 0)
+    return fail(DQH_BAD_ARG, "dqh_mesolve_tsit5_dense_queue: bad arguments");
+  hipLaunchKernelGGL(k_tsit5_queue, dim3((unsigned)nworkgroups), dim3(kThreads), 0, (hipStream_t)stream, *q);
+  return launch_checked("dqh_mesolve_tsit5_dense_queue", hipGetLastError());
+}
+
+extern "C" int dqh_memcpy_h2d_async(void *dst, const void *src, int64_t bytes, void *stream) {
+  if (!dst || !src || bytes < 0) return fail(DQH_BAD_ARG, "dqh_memcpy_h2d_async: bad arguments");
+  if (bytes == 0) return DQH_OK;
+  const hipError_t e = hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, (hipStream_t)stream);
+  return e == hipSuccess ? DQH_OK : fail(DQH_HIP_ERROR, std::string("dqh_memcpy_h2d_async: ") + hipGetErrorString(e));
+}
+
+extern "C" int64_t dqh_wall_clock_khz(void) {
+  int dev = 0, khz = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess)
+    return 0;
+  return khz;
 }
 
 extern "C" int dqh_mesolve_tsit5_dense(const dqh_me_inst *d_insts, int64_t ninst, void *stream) {

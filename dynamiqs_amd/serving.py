@@ -1,33 +1,37 @@
 """Continuous-batching mesolve server on one MI355X.
 
 Requests (one dense Tsit5 solve each, any Hilbert dimension n <= 256, any number
-of jump operators, own save times and tolerances) are admitted while earlier
-batches are still on the GPU: a scheduler thread packs whatever is pending into
-one launch of the persistent Tsit5 kernel (every workgroup integrates one request
-with its own adaptive step, so uneven requests share a launch), bounded by a
-device-memory budget, and launches it on the next free HIP stream. Requests of
-padded dimension above GRID_MIN_NP run on the grid-wide Tsit5 instead (each stage
-spread over many CUs, driven by a worker thread), so that one large request does
-not hold a single CU for the whole solve. Launches on different streams overlap on
-the device.
+of jump operators, own save times and tolerances) are admitted one by one while
+others are on the GPU. A scheduler thread sorts them by padded dimension:
 
-Packing: the inputs of a launch (-iH, jump operators, initial states, save
-times) and its descriptor array are written on the host into one pinned staging
-buffer and cross to the device in one copy; a small kernel then completes the
-non-Hermitian Hamiltonian -iH - 1/2 sum L^dag L and the right factors L^dag / 2
-on the device (FP64 products), so no O(n^3) work is left on the host. The FP64
-state, the stage scratch and every input stay resident in HBM for the whole solve.
+* Small requests (padded dimension below GRID_MIN_NP) go to a work queue. A
+  persistent grid of ``queue_workgroups`` workgroups
+  (``dqh_mesolve_tsit5_dense_queue``) keeps taking requests from a ring in pinned
+  host memory: the host stages a request's inputs (-iH, jump operators, initial
+  state, save times) and its descriptor in coherent pinned memory, writes a ring
+  slot and advances the tail; a free workgroup claims it, copies the inputs to HBM
+  itself, completes -iH - 1/2 sum L^dag L and L^dag / 2 (FP64 products), solves it
+  and signals it. No copy, launch or synchronisation is needed to start a request,
+  and a workgroup takes the next request as soon as it is done, so a request never
+  waits for the slowest member of a batch. The grid leaves when no request has
+  been claimed for ``idle_ms``, and its workgroups stop claiming after ``life_ms``
+  (the scheduler relaunches it while requests are outstanding).
+* Large requests run on the grid-wide Tsit5 (each stage spread over many CUs,
+  driven by a worker thread), batched by size class, on two further streams.
 
-Retirement is per request: the outputs of a persistent launch (saved states,
-expectation values, final state, statistics) live in pinned host memory that the
-kernel writes directly, and each workgroup publishes its request's completion in
-stats[7] with a system-scope release (``DQH_FLAG_SIGNAL``). The scheduler polls
-those words and answers a request as soon as its own solve is done, whatever the
-other requests of its launch are still doing; the launch's device memory returns
-to the budget when its last request is done.
+The three streams are created together once per process, so that they get
+distinct hardware queues and the persistent grid never holds a grid-wide launch
+behind it. Device memory (inputs, FP64 state, stage scratch) is bounded by a
+memory budget.
 
-The result of a request does not depend on what it was batched with: the kernel
-keeps no cross-instance state and each workgroup's arithmetic is fixed by its own
+Retirement is per request: outputs (saved states, expectation values, final
+state, statistics) live in pinned host memory that the kernels write directly, and
+each workgroup publishes its request's completion in stats[7] with a system-scope
+release (``DQH_FLAG_SIGNAL``). The scheduler polls those words and host threads
+copy each answer out as soon as its own solve is done.
+
+The result of a request does not depend on what it was served with: the kernels
+keep no cross-instance state and each workgroup's arithmetic is fixed by its own
 descriptor (see ``tests/test_gpu_serving.py``).
 
 There is no counterpart in the reference (``dq.mesolve`` is one synchronous call
@@ -99,10 +103,13 @@ def _layout(n: int, K: int, nt: int, nE: int, nlog: int = 0):
     npd = pad16(n)
     nn = npd * npd
     lay_in, off = {}, 0
-    for name, nbytes in (('H', nn * 8), ('D', max(K, 1) * nn * 8), ('C', max(K, 1) * nn * 8),
-                         ('E', max(nE, 1) * nn * 8), ('ts', nt * 8), ('y', 2 * nn * 16)):
+    # what the host fills comes first (up to the first FP64 state buffer: 'copy'
+    # bytes); C is written on the device
+    for name, nbytes in (('H', nn * 8), ('D', max(K, 1) * nn * 8), ('E', max(nE, 1) * nn * 8), ('ts', nt * 8),
+                         ('y', 2 * nn * 16), ('C', max(K, 1) * nn * 8)):
         lay_in[name] = off
         off += _align(nbytes)
+    lay_in['copy'] = lay_in['y'] + nn * 16
     size_in = off
     work = _align(_native.work_elems(npd, K, 0) * 8) if _needs_work(npd, K) else 256
     lay_out, off = {}, 0
@@ -148,8 +155,14 @@ class _Launch:
         self.ev0 = None  # timing event recorded before the launch
         self.ready = None  # set by the worker thread of a grid-wide launch
         self.error = None
+        self.queued = False  # requests published to the work queue (no launch of their own)
+        self.slots = None  # their ring slots
+        self.blocks = None  # their coherent pinned blocks (inputs, outputs)
+        self.unread = 0  # answers still to be read out of the output block
 
     def done(self) -> bool:
+        if self.queued:
+            return not self.pending.any()
         if self.ready is not None and not self.ready.is_set():
             return False
         return self.event is not None and self.event.query()
@@ -215,14 +228,18 @@ def _pack_host(reqs: list, base: int, staging: np.ndarray, descs: np.ndarray, la
             d['step_log'], d['step_log_cap'] = out_base + ob + lo['log'], r.step_log_cap
 
 
-def pack_and_launch(reqs: list, stream, grid: bool = False, times: dict | None = None):
+def pack_and_launch(reqs: list, stream, grid: bool = False, times: dict | None = None, pool=None, out_pool=None,
+                    blocks_out: list | None = None):
     """Pack uneven requests and launch them on ``stream``: the persistent Tsit5 kernel
     with per-request completion words, or (``grid``) a callable that runs the
     grid-wide Tsit5 on them (requests of one grid launch share their padded
     dimension); outputs in pinned host memory either way. Returns (tensors kept
     alive, per-request output layout, pinned output view, completion event or the
     callable, device bytes held). ``times`` (optional) accumulates the host seconds
-    spent on buffers ('alloc'), packing ('pack') and the copy and launches ('launch')."""
+    spent on buffers ('alloc'), packing ('pack') and the copy and launches ('launch').
+    With ``pool`` and ``out_pool`` (pinned pools), the staging and output buffers are
+    blocks of them (appended to ``blocks_out``: inputs, outputs) instead of new pinned
+    allocations, which can take milliseconds each."""
     import torch
 
     t_start = time.perf_counter()
@@ -235,8 +252,14 @@ def pack_and_launch(reqs: list, stream, grid: bool = False, times: dict | None =
     dsz = B * _native.ME_INST_DTYPE.itemsize
     n_in = int(in_off[-1])
     # staging: every request's inputs, then the descriptor array: one copy
-    staging_t = torch.zeros(n_in + _align(dsz), dtype=torch.uint8, pin_memory=True)
-    staging = staging_t.numpy()
+    if pool is not None:
+        in_blk = pool.alloc(n_in + _align(dsz))
+        staging_t = None
+        staging = pool.view(in_blk)
+        staging[:] = 0
+    else:
+        staging_t = torch.zeros(n_in + _align(dsz), dtype=torch.uint8, pin_memory=True)
+        staging = staging_t.numpy()
     # device buffers belong to the launch stream: the allocator's reuse and the
     # zero fill of device outputs are then ordered with the launch itself
     with torch.cuda.stream(stream):
@@ -249,8 +272,14 @@ def pack_and_launch(reqs: list, stream, grid: bool = False, times: dict | None =
     # address: a finished request is read on the host with no device copy and no
     # synchronisation. The kernels write every output; only the stats words (the
     # completion word of the persistent kernel among them) must start at zero.
-    oarena = torch.empty(int(out_off[-1]), dtype=torch.uint8, pin_memory=True)
-    out_base, host_out = _native.host_device_pointer(oarena.data_ptr()), oarena.numpy()
+    if pool is not None:
+        out_blk = out_pool.alloc(int(out_off[-1]))
+        oarena, out_base, host_out = None, out_pool.device(out_blk), out_pool.view(out_blk)
+        if blocks_out is not None:
+            blocks_out += [in_blk, out_blk]
+    else:
+        oarena = torch.empty(int(out_off[-1]), dtype=torch.uint8, pin_memory=True)
+        out_base, host_out = _native.host_device_pointer(oarena.data_ptr()), oarena.numpy()
     for b in range(B):
         a = int(out_off[b]) + lays[b][3]['stats']
         host_out[a : a + 64] = 0
@@ -269,9 +298,13 @@ def pack_and_launch(reqs: list, stream, grid: bool = False, times: dict | None =
     outs = [(int(out_off[b]) + lays[b][3]['st'], int(out_off[b]) + lays[b][3]['ex'],
              int(out_off[b]) + lays[b][3]['stats'], r.n, len(r.tsave), len(r.exp_ops),
              int(out_off[b]) + lays[b][3]['log'], r.step_log_cap) for b, r in enumerate(reqs)]
-    nbytes = int(arena.numel() + scratch.numel() + (oarena.numel() if grid else 0))
+    nbytes = int(arena.numel() + scratch.numel())  # device bytes (the outputs are in host memory)
     with torch.cuda.stream(stream):
-        arena.copy_(staging_t, non_blocking=True)
+        if staging_t is not None:
+            arena.copy_(staging_t, non_blocking=True)
+        else:
+            _native.check(lib.dqh_memcpy_h2d_async(c_p(arena.data_ptr()), c_p(staging.ctypes.data), int(staging.nbytes),
+                                                   c_p(stream.cuda_stream)), 'dqh_memcpy_h2d_async')
         _native.check(lib.dqh_dense_prepare_ops(c_p(d_descs_ptr), B, c_p(stream.cuda_stream)), 'dqh_dense_prepare_ops')
         if grid:
             nb = int(lib.dqh_tsit5_grid_scratch_bytes(npd, B))
@@ -303,6 +336,232 @@ def pack_and_launch(reqs: list, stream, grid: bool = False, times: dict | None =
     if times is not None:
         times['launch'] = times.get('launch', 0.0) + (time.perf_counter() - t_pack)
     return [oarena, arena, scratch, staging_t], outs, host_out, ev, nbytes
+
+
+# ------------------------------------------------------------------ work queue
+SLOT_WORDS = 8  # DQH_QUEUE_SLOT_WORDS
+
+
+class _DQueue(_native.ctypes.Structure):
+    """Host mirror of ``dqh_queue`` (include/dqh.h)."""
+
+    _fields_ = [('ring', _native.ctypes.c_uint64), ('capacity', _native.ctypes.c_int64),
+                ('tail', _native.ctypes.c_uint64), ('head', _native.ctypes.c_uint64),
+                ('stop', _native.ctypes.c_uint64), ('busy_ticks', _native.ctypes.c_uint64),
+                ('idle_ticks', _native.ctypes.c_int64), ('life_ticks', _native.ctypes.c_int64)]
+
+
+class _PinnedPool:
+    """Pinned, device-mapped host memory (``dqh_host_alloc``) in chunks of CHUNK bytes,
+    sub-allocated first fit (256-byte blocks) and kept for the life of the process,
+    since freeing pinned memory can wait for the device. The coherent (fine-grained)
+    pool holds what the persistent queue kernel reads while the host writes it
+    (staged inputs, ring and control words): the device never reads a stale cached
+    copy. The non-coherent pool holds kernel outputs, which the kernels publish with
+    a system-scope release and whose stores the device can combine."""
+
+    CHUNK = 256 << 20
+
+    def __init__(self, coherent: bool):
+        self.coherent = bool(coherent)
+        self._chunks = []  # [device address, uint8 view, free list of (offset, bytes)]
+        self._lock = threading.Lock()
+
+    def alloc(self, nbytes: int):
+        nbytes = _align(max(int(nbytes), 16))
+        with self._lock:
+            for ci, ch in enumerate(self._chunks):
+                free = ch[2]
+                for k, (off, n) in enumerate(free):
+                    if n >= nbytes:
+                        if n == nbytes:
+                            free.pop(k)
+                        else:
+                            free[k] = (off + nbytes, n - nbytes)
+                        return (ci, off, nbytes)
+            size = max(self.CHUNK, nbytes)
+            ptr = _native.ctypes.c_void_p()
+            _native.check(_native.lib().dqh_host_alloc(size, int(self.coherent), _native.ctypes.byref(ptr)),
+                          'dqh_host_alloc')
+            view = np.ctypeslib.as_array((_native.ctypes.c_uint8 * size).from_address(ptr.value))
+            self._chunks.append([_native.host_device_pointer(ptr.value), view, [(nbytes, size - nbytes)] if size > nbytes else []])
+            return (len(self._chunks) - 1, 0, nbytes)
+
+    def view(self, blk) -> np.ndarray:
+        ci, off, n = blk
+        return self._chunks[ci][1][off : off + n]
+
+    def device(self, blk) -> int:
+        return self._chunks[blk[0]][0] + blk[1]
+
+    def free(self, blk) -> None:
+        ci, off, n = blk
+        with self._lock:
+            free = sorted(self._chunks[ci][2] + [(off, n)])
+            merged = []
+            for o, m in free:
+                if merged and merged[-1][0] + merged[-1][1] == o:
+                    merged[-1] = (merged[-1][0], merged[-1][1] + m)
+                else:
+                    merged.append((o, m))
+            self._chunks[ci][2] = merged
+
+
+_pools = {}
+_streams = None
+
+
+def _pinned_pool(coherent: bool) -> _PinnedPool:
+    with _pool_lock:
+        if coherent not in _pools:
+            _pools[coherent] = _PinnedPool(coherent)
+        return _pools[coherent]
+
+
+def _serving_streams():
+    """The queue kernel's stream and the grid-wide launches' two streams, created
+    together once per process and shared by every Server: streams created in a row
+    get distinct hardware queues, so the persistent queue kernel shares none with a
+    grid-wide launch (a kernel waits for the earlier ones of its hardware queue)."""
+    global _streams
+    import torch
+
+    with _pool_lock:
+        if _streams is None:
+            _streams = (torch.cuda.Stream(), [torch.cuda.Stream(), torch.cuda.Stream()])
+        return _streams
+
+
+class _WorkQueue:
+    """The ring of ``dqh_mesolve_tsit5_dense_queue`` and its persistent launches.
+    ``publish`` fills ring slots and then advances tail; the scheduler relaunches the
+    grid whenever published requests are outstanding and no launch is running (its
+    workgroups leave when idle for idle_ms or after life_ms)."""
+
+    def __init__(self, capacity: int, workgroups: int, idle_ms: float, life_ms: float):
+        import torch
+
+        lib = _native.lib()
+        if capacity & (capacity - 1):
+            raise ValueError('the ring capacity must be a power of two')
+        self.capacity, self.workgroups = capacity, workgroups
+        self.pool = _pinned_pool(True)
+        self.out_pool = _pinned_pool(False)
+        nwords = capacity * SLOT_WORDS + 64
+        self.blk = self.pool.alloc(nwords * 8)
+        words = self.pool.view(self.blk).view(np.uint64)
+        words[:] = 0
+        self.ring = words[: capacity * SLOT_WORDS].reshape(capacity, SLOT_WORDS)
+        self.ctrl = words[capacity * SLOT_WORDS :]  # ctrl[0]: tail, ctrl[32]: stop (other cache lines)
+        base = self.pool.device(self.blk)
+        self.counters = torch.zeros(64, dtype=torch.int64, device=torch.device('cuda', torch.cuda.current_device()))
+        self.khz = int(lib.dqh_wall_clock_khz()) or 100_000
+        self.q = _DQueue(ring=base, capacity=capacity, tail=base + capacity * SLOT_WORDS * 8,
+                         head=self.counters.data_ptr(), stop=base + (capacity * SLOT_WORDS + 32) * 8,
+                         busy_ticks=self.counters.data_ptr() + 256, idle_ticks=int(idle_ms * self.khz),
+                         life_ticks=int(life_ms * self.khz))
+        torch.cuda.synchronize()  # the counters are zero before the first launch
+        self.tail = 0
+        self.outstanding = 0  # published and not yet completed
+        self.slot_busy = np.zeros(capacity, dtype=bool)
+        self.stream = _serving_streams()[0]
+        self.event = None
+        self.launches = 0
+        self._lock = threading.Lock()
+
+    def free_slots(self) -> int:
+        return self.capacity - self.outstanding
+
+    def publish(self, entries) -> list:
+        """Write one slot per entry (src, dst, bytes, src_desc, dst_desc) and advance
+        tail; returns the slots used."""
+        slots = []
+        for e in entries:
+            sl = self.tail & (self.capacity - 1)
+            if self.slot_busy[sl]:
+                raise RuntimeError('work-queue slot still in use')
+            self.ring[sl, :5] = e
+            self.slot_busy[sl] = True
+            slots.append(sl)
+            self.tail += 1
+        self.outstanding += len(entries)
+        # the lock's atomic operations order the staged inputs and the slots (also
+        # write-combined stores) before the tail store that publishes them
+        with self._lock:
+            self.ctrl[0] = self.tail
+        return slots
+
+    def complete(self, slot: int) -> None:
+        self.slot_busy[slot] = False
+        self.outstanding -= 1
+
+    def running(self) -> bool:
+        return self.event is not None and not self.event.query()
+
+    def launch(self) -> None:
+        import torch
+
+        with torch.cuda.stream(self.stream):
+            _native.check(_native.lib().dqh_mesolve_tsit5_dense_queue(_native.ctypes.byref(self.q), self.workgroups,
+                                                                      _native.ctypes.c_void_p(self.stream.cuda_stream)),
+                          'dqh_mesolve_tsit5_dense_queue')
+            self.event = torch.cuda.Event(enable_timing=True)
+            self.event.record(self.stream)
+        self.launches += 1
+
+    def busy_seconds(self) -> float:
+        return float(self.counters[32].item()) / (self.khz * 1e3)
+
+    def close(self) -> None:
+        with self._lock:
+            self.ctrl[32] = 1
+        if self.event is not None:
+            self.event.synchronize()
+        self.pool.free(self.blk)
+
+
+def pack_queued(reqs: list, pool: _PinnedPool, out_pool: _PinnedPool, times: dict | None = None):
+    """Stage ``reqs`` for the work queue: inputs and descriptors in coherent pinned
+    memory (``pool``), device buffers for their copies and scratch (allocated, not
+    written: the queue kernel copies the inputs itself), outputs in non-coherent
+    pinned memory (``out_pool``).
+    Returns (device tensors, [input block, output block], per-request output layout,
+    output view, ring entries, device bytes)."""
+    import torch
+
+    t_start = time.perf_counter()
+    dev = torch.device('cuda', torch.cuda.current_device())
+    B = len(reqs)
+    lays = [_layout(r.n, len(r.Ls), len(r.tsave), len(r.exp_ops), r.step_log_cap) for r in reqs]
+    in_off = np.cumsum([0] + [lay[1] for lay in lays])
+    w_off = np.cumsum([0] + [lay[2] for lay in lays])
+    out_off = np.cumsum([0] + [lay[4] for lay in lays])
+    isz = _native.ME_INST_DTYPE.itemsize
+    n_in = int(in_off[-1])
+    in_blk = pool.alloc(n_in + _align(B * isz))
+    out_blk = out_pool.alloc(int(out_off[-1]))
+    staging = pool.view(in_blk)
+    staging[:] = 0
+    host_out = out_pool.view(out_blk)
+    arena = torch.empty(n_in + _align(B * isz), dtype=torch.uint8, device=dev)
+    scratch = torch.empty(max(int(w_off[-1]), 256), dtype=torch.uint8, device=dev)
+    for b in range(B):
+        a = int(out_off[b]) + lays[b][3]['stats']
+        host_out[a : a + 64] = 0
+    t_alloc = time.perf_counter()
+    descs = staging[n_in : n_in + B * isz].view(_native.ME_INST_DTYPE)
+    _pack_host(reqs, arena.data_ptr(), staging, descs, lays, in_off, out_pool.device(out_blk), out_off,
+               scratch.data_ptr(), w_off, signal=True)
+    src, dst = pool.device(in_blk), arena.data_ptr()
+    entries = [(src + int(in_off[b]), dst + int(in_off[b]), lays[b][0]['copy'], src + n_in + b * isz,
+                dst + n_in + b * isz) for b in range(B)]
+    outs = [(int(out_off[b]) + lays[b][3]['st'], int(out_off[b]) + lays[b][3]['ex'],
+             int(out_off[b]) + lays[b][3]['stats'], r.n, len(r.tsave), len(r.exp_ops),
+             int(out_off[b]) + lays[b][3]['log'], r.step_log_cap) for b, r in enumerate(reqs)]
+    if times is not None:
+        times['queue_alloc'] = times.get('queue_alloc', 0.0) + (t_alloc - t_start)
+        times['queue_pack'] = times.get('queue_pack', 0.0) + (time.perf_counter() - t_alloc)
+    return [arena, scratch], [in_blk, out_blk], outs, host_out, entries, int(arena.numel() + scratch.numel())
 
 
 def _read_outputs(host: np.ndarray, out) -> tuple:
@@ -358,12 +617,19 @@ class Server:
 
     Args:
         memory_budget: device bytes that in-flight requests may hold together.
-        max_batch: maximum requests per launch.
-        nstreams: HIP streams launches rotate over (default priority).
-        max_inflight: maximum concurrent launches.
+        max_batch: maximum requests staged together (one work-queue batch or one
+            grid-wide launch).
+        nstreams: streams (1 or 2, default priority) the grid-wide launches rotate over.
+        max_inflight: maximum concurrent grid-wide launches.
+        queue_workgroups: workgroups of the persistent work-queue grid (one per CU;
+            the other CUs stay free for grid-wide launches).
+        queue_capacity: ring slots (a power of two): requests queued or in service.
+        idle_ms: the work-queue grid leaves after this long without a claim.
+        life_ms: its workgroups stop claiming after this long (then it is relaunched).
     """
 
-    def __init__(self, memory_budget: int = 8 << 30, max_batch: int = 64, nstreams: int = 4, max_inflight: int = 8):
+    def __init__(self, memory_budget: int = 8 << 30, max_batch: int = 64, nstreams: int = 2, max_inflight: int = 8,
+                 queue_workgroups: int = 128, queue_capacity: int = 4096, idle_ms: float = 20.0, life_ms: float = 200.0):
         import torch
 
         if not torch.cuda.is_available():
@@ -372,7 +638,10 @@ class Server:
         self.memory_budget = int(memory_budget)
         self.max_batch = int(max_batch)
         self.max_inflight = int(max_inflight)
-        self.streams = [torch.cuda.Stream() for _ in range(nstreams)]
+        self.streams = _serving_streams()[1][: max(1, int(nstreams))]
+        # small requests go to the persistent work-queue kernel (its own stream)
+        self._wq = _WorkQueue(int(queue_capacity), int(queue_workgroups), float(idle_ms), float(life_ms))
+        self._q_ev0 = None
         self._q: queue.Queue = queue.Queue()
         self._pending: list = []
         self._inflight: list = []
@@ -383,7 +652,7 @@ class Server:
                       'answered_before_launch_end': 0}
         # host seconds of the scheduler thread: buffers, packing, copy + launches, polling
         # and retirement
-        self.host_s = {'alloc': 0.0, 'pack': 0.0, 'launch': 0.0, 'retire': 0.0}
+        self.host_s = {'alloc': 0.0, 'pack': 0.0, 'queue_alloc': 0.0, 'queue_pack': 0.0, 'launch': 0.0, 'retire': 0.0}
         # device-busy accounting: a timing event before and after every launch,
         # measured from a reference event recorded now
         self._t_ref = torch.cuda.Event(enable_timing=True)
@@ -415,7 +684,13 @@ class Server:
     def close(self) -> None:
         self._stop.set()
         self._thread.join()
+        self._wq.close()
+        if self._q_ev0 is not None:
+            self._busy.append((self._q_ev0, self._wq.event))
         self.stats['busy_s'] = self.busy_seconds()
+        self.stats['queue_launches'] = self._wq.launches
+        self.stats['queue_busy_s'] = self._wq.busy_seconds()
+        self.stats['queue_workgroups'] = self._wq.workgroups
 
     def busy_seconds(self) -> float:
         """Time during which at least one launch of this server was on the device
@@ -442,60 +717,115 @@ class Server:
 
     # ------------------------------------------------------------- scheduler
     def _admit(self) -> None:
-        if not self._pending or len(self._inflight) >= self.max_inflight:
-            return
-        # fill a launch with requests of the same size class as the oldest pending one
-        # (FIFO inside the class)
-        cls = _size_class(self._pending[0])
-        batch, nbytes, keep = [], 0, []
-        for r in self._pending:
-            if len(batch) >= self.max_batch or _size_class(r) != cls:
-                keep.append(r)
-                continue
-            fb = _request_bytes(r)
-            if self._used + nbytes + fb > self.memory_budget:
-                keep.append(r)
-                continue
-            batch.append(r)
-            nbytes += fb
-        self._pending = keep
-        if not batch:
+        """Small requests (padded dimension below GRID_MIN_NP) go to the work queue as
+        soon as they are pending (ring slots and the memory budget permitting): they
+        never wait behind large ones. Large requests are batched by size class, the
+        oldest first, into a grid-wide launch when one of max_inflight is free."""
+        if not self._pending:
             return
         import torch
 
-        stream = self.streams[self._next_stream % len(self.streams)]
-        self._next_stream += 1
-        t0 = time.perf_counter()
-        grid = cls >= 1000
-        ev0 = torch.cuda.Event(enable_timing=True)
-        ev0.record(stream)
-        try:
-            tensors, outs, host_out, ev, total = pack_and_launch(batch, stream, grid=grid, times=self.host_s)
-        except Exception as e:  # answer the batch with the error, keep serving
-            for r in batch:
-                r.future.set_exception(e)
-            return
-        self._used += total
-        launch = _Launch(batch, stream, tensors, outs, total, t0, host_out)
-        launch.ev0 = ev0
-        if grid:
-            launch.ready = threading.Event()
-
-            def work(launch=launch, run=ev):
+        small, large = [], []
+        for r in self._pending:
+            (small if _size_class(r) < 1000 else large).append(r)
+        keep = []
+        if small:
+            batch, nbytes = [], 0
+            room = min(self.max_batch, self._wq.free_slots())
+            for r in small:
+                fb = _request_bytes(r)
+                if len(batch) < room and self._used + nbytes + fb <= self.memory_budget:
+                    batch.append(r)
+                    nbytes += fb
+                else:
+                    keep.append(r)
+            if batch:
+                t0 = time.perf_counter()
                 try:
-                    launch.event = run()
-                except Exception as e:  # reported when the launch retires
-                    launch.error = e
-                finally:
-                    launch.ready.set()
+                    tensors, blocks, outs, host_out, entries, total = pack_queued(batch, self._wq.pool, self._wq.out_pool,
+                                                                                  times=self.host_s)
+                except Exception as e:  # answer the batch with the error, keep serving
+                    for r in batch:
+                        r.future.set_exception(e)
+                else:
+                    launch = _Launch(batch, None, tensors, outs, total, t0, host_out)
+                    launch.queued, launch.blocks, launch.unread = True, blocks, len(batch)
+                    t1 = time.perf_counter()
+                    launch.slots = self._wq.publish(entries)
+                    self._used += total
+                    self._inflight.append(launch)
+                    self._ensure_queue()
+                    self.host_s['launch'] += time.perf_counter() - t1
+                    self._count_launch()
+        if large and sum(1 for L in self._inflight if not L.queued) < self.max_inflight:
+            cls = _size_class(large[0])
+            batch, nbytes = [], 0
+            for r in large:
+                fb = _request_bytes(r)
+                if (_size_class(r) == cls and len(batch) < self.max_batch
+                        and self._used + nbytes + fb <= self.memory_budget):
+                    batch.append(r)
+                    nbytes += fb
+                else:
+                    keep.append(r)
+            if batch:
+                t0 = time.perf_counter()
+                stream = self.streams[self._next_stream % len(self.streams)]
+                self._next_stream += 1
+                ev0 = torch.cuda.Event(enable_timing=True)
+                ev0.record(stream)
+                blocks = []
+                try:
+                    tensors, outs, host_out, ev, total = pack_and_launch(batch, stream, grid=True, times=self.host_s,
+                                                                         pool=self._wq.pool, out_pool=self._wq.out_pool,
+                                                                         blocks_out=blocks)
+                except Exception as e:  # answer the batch with the error, keep serving
+                    for blk, pl in zip(blocks, (self._wq.pool, self._wq.out_pool)):
+                        pl.free(blk)
+                    for r in batch:
+                        r.future.set_exception(e)
+                else:
+                    self._used += total
+                    launch = _Launch(batch, stream, tensors, outs, total, t0, host_out)
+                    launch.blocks, launch.unread = blocks, len(batch)
+                    launch.ev0 = ev0
+                    launch.ready = threading.Event()
 
-            threading.Thread(target=work, name='dqh-grid', daemon=True).start()
+                    def work(launch=launch, run=ev):
+                        try:
+                            launch.event = run()
+                        except Exception as e:  # reported when the launch retires
+                            launch.error = e
+                        finally:
+                            launch.ready.set()
+
+                    threading.Thread(target=work, name='dqh-grid', daemon=True).start()
+                    self._inflight.append(launch)
+                    self._count_launch()
         else:
-            launch.event = ev
-        self._inflight.append(launch)
+            keep += large
+        # FIFO order of what stays pending
+        kept = set(id(r) for r in keep)
+        self._pending = [r for r in self._pending if id(r) in kept]
+
+    def _count_launch(self) -> None:
         self.stats['launches'] += 1
         self.stats['max_inflight_bytes'] = max(self.stats['max_inflight_bytes'], self._used)
         self.stats['max_concurrent_launches'] = max(self.stats['max_concurrent_launches'], len(self._inflight))
+
+    def _ensure_queue(self) -> None:
+        """(Re)launch the work-queue grid when requests are outstanding and no launch
+        of it runs: its workgroups leave when idle or old, and a launch that has
+        ended has completed every request it took."""
+        import torch
+
+        if self._wq.outstanding == 0 or self._wq.running():
+            return
+        if self._q_ev0 is not None and self._wq.event is not None:
+            self._busy.append((self._q_ev0, self._wq.event))
+        self._q_ev0 = torch.cuda.Event(enable_timing=True)
+        self._q_ev0.record(self._wq.stream)
+        self._wq.launch()
 
     def _answer(self, L: _Launch, b: int, res, t: float) -> None:
         """Complete request b of launch L with its outputs (any thread; the scheduler
@@ -514,6 +844,13 @@ class Server:
         except Exception as e:  # pragma: no cover - answered with the error
             L.reqs[b].future.set_exception(e)
             return
+        finally:
+            if L.blocks is not None:
+                with _pool_lock:
+                    L.unread -= 1
+                    last = L.unread == 0
+                if last:  # every answer of the batch is read out: its output block is free
+                    self._wq.out_pool.free(L.blocks[1])
         self._answer(L, b, res, time.perf_counter())
 
     def _retire(self) -> bool:
@@ -524,11 +861,13 @@ class Server:
                 # (all words of the launch read in one gather)
                 newly = np.flatnonzero(L.pending & (L.words[L.done_at] == 1))
                 if len(newly):
-                    before_end = not L.done()
+                    before_end = not (L.pending.sum() == len(newly)) if L.queued else not L.done()
                     for b in newly.tolist():
                         L.pending[b] = False
                         L.answered[b] = True
                         self.stats['requests'] += 1
+                        if L.queued:
+                            self._wq.complete(L.slots[b])
                         # the copy out of pinned memory runs on a host thread (the
                         # launch object keeps the pinned buffer alive until then)
                         _pool().submit(self._answer_pinned, L, b, L.outs[b])
@@ -540,6 +879,8 @@ class Server:
                 continue
             self._inflight.remove(L)
             progress = True
+            if L.blocks is not None:  # the inputs were copied to the device: their staging block is free
+                self._wq.pool.free(L.blocks[0])
             if L.event is not None:
                 self._busy.append((L.ev0, L.event))
             rest = [b for b in range(len(L.reqs)) if not L.answered[b]]
@@ -548,6 +889,8 @@ class Server:
             self.stats['requests'] += len(rest)
             if rest:
                 if L.error is not None:
+                    if L.blocks is not None:
+                        self._wq.out_pool.free(L.blocks[1])
                     for b in rest:
                         L.reqs[b].future.set_exception(L.error)
                 else:
@@ -570,5 +913,6 @@ class Server:
             retired = self._retire()
             self.host_s['retire'] += time.perf_counter() - t0
             self._admit()
+            self._ensure_queue()
             if not (got or retired):
                 time.sleep(0.0002)

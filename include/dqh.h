@@ -149,6 +149,57 @@ int dqh_dense_prepare_ops(const dqh_me_inst *d_insts, int64_t ninst, void *strea
  * diffrax_integrator.py:101-147 with the vector field of :319-323). */
 int dqh_mesolve_tsit5_dense(const dqh_me_inst *d_insts, int64_t ninst, void *stream);
 
+/* Work queue of the serving loop: requests published by the host into a ring while a
+ * persistent grid (dqh_mesolve_tsit5_dense_queue) solves them. A ring slot is
+ * DQH_QUEUE_SLOT_WORDS uint64: {src, dst, bytes, src_desc, dst_desc, 0, 0, 0}: a
+ * request's inputs staged in pinned host memory (device address src, ``bytes`` a
+ * multiple of 16) and where they go in device memory (dst), and its dqh_me_inst in
+ * pinned host memory (src_desc) and in device memory (dst_desc). The descriptor's
+ * pointers name the device copies; as for dqh_dense_prepare_ops, Hnh holds -iH and
+ * Dops the jump operators on entry. */
+#define DQH_QUEUE_SLOT_WORDS 8
+typedef struct dqh_queue {
+  uint64_t ring;        /* device address of HOST pinned coherent uint64[capacity * DQH_QUEUE_SLOT_WORDS] */
+  int64_t capacity;     /* ring slots, a power of two */
+  uint64_t tail;        /* device address of a HOST pinned coherent int64: requests published */
+  uint64_t head;        /* device int64[2]: requests claimed and the wall clock of the latest claim (zero
+                           before the first launch, kept across launches) */
+  uint64_t stop;        /* device address of a HOST pinned coherent int64: nonzero: exit once nothing is published */
+  uint64_t busy_ticks;  /* device int64: wall-clock ticks the workgroups spent on requests */
+  int64_t idle_ticks;   /* wall-clock ticks without a published request after which a workgroup exits */
+  int64_t life_ticks;   /* wall-clock ticks after its start after which a workgroup claims no more requests */
+} dqh_queue;
+
+/* Persistent work-queue form of dqh_mesolve_tsit5_dense for continuous batching
+ * (replaces the lockstep vmap of a batch, apis/mesolve.py:307-336 and
+ * integrators/_utils.py:144-187, by requests admitted one by one). Each of the
+ * ``nworkgroups`` workgroups claims the next published slot (claimed % capacity),
+ * copies the request's inputs and descriptor to the device, completes Hnh and Cops,
+ * solves it with the bodies of dqh_mesolve_tsit5_dense and signals it
+ * (DQH_FLAG_SIGNAL), then claims the next. The host fills slot tail % capacity and
+ * then increments tail; a slot may be rewritten once its request has completed.
+ * Every workgroup exits when stop is set and nothing is published, when no workgroup
+ * has claimed a request for idle_ticks, or at its first claim after life_ticks (the launch then ends
+ * and other work queued behind it on the same hardware queue runs); requests
+ * published after that need another launch (head carries over). */
+int dqh_mesolve_tsit5_dense_queue(const dqh_queue *q, int64_t nworkgroups, void *stream);
+
+/* hipMemcpyAsync host -> device (``src`` pinned) on ``stream``: the staging copy of
+ * the serving loop's grid-wide launches, from memory of dqh_host_alloc. No
+ * counterpart in the reference (serving plumbing). */
+int dqh_memcpy_h2d_async(void *dst, const void *src, int64_t bytes, void *stream);
+
+/* Rate of the device wall clock that idle_ticks and busy_ticks count, in kHz. */
+int64_t dqh_wall_clock_khz(void);
+
+/* Pinned, device-mapped host memory (hipHostMalloc with hipHostMallocMapped and
+ * hipHostMallocCoherent or hipHostMallocNonCoherent). Coherent (fine-grained) memory
+ * is for data the host writes while a kernel reads it (the device never caches it);
+ * non-coherent memory for kernel outputs, made visible by a system-scope release.
+ * No counterpart in the reference (serving plumbing). */
+int dqh_host_alloc(int64_t bytes, int64_t coherent, void **host);
+int dqh_host_free(void *host);
+
 /* Fixed-step Rouchon1 with the Cholesky normalisation folded into the Kraus
  * operators (rouchon_integrator.py:226-269, 394-407, 552-582): each step adds
  * tmp + tmp^dag with tmp = sum_k D_k rho C_k to the FP64 state. */

@@ -170,3 +170,40 @@ def test_device_operator_preparation():
             np.testing.assert_array_equal(C[:n, :n], (0.5 * L64[k].conj().T).astype(np.complex64))
             assert not np.any(C[n:, :]) and not np.any(C[:, n:])
     del keep, outs, host_out
+
+
+def test_work_queue_relaunches_and_matches_batch_kernel():
+    """Small requests go to the persistent work-queue kernel. With a short workgroup
+    lifetime and idle time the grid exits and is relaunched many times while requests
+    arrive in waves; every result equals the same request solved by the batch kernel
+    (dqh_mesolve_tsit5_dense, the same bodies) and is answered exactly once."""
+    import time
+
+    import torch
+
+    from dynamiqs_amd import serving
+
+    rng = np.random.default_rng(17)
+    shapes = [(int(n), int(K)) for n, K in zip(rng.integers(4, 90, size=24), rng.integers(1, 4, size=24))]
+    reqs = [_random_request(rng, n, K) for n, K in shapes]
+    got = []
+    with serving.Server(memory_budget=1 << 30, max_batch=5, queue_workgroups=32, idle_ms=0.2, life_ms=0.5) as srv:
+        for w in range(4):
+            futs = [srv.submit(H, Ls, psi, ts, rtol=1e-6, atol=1e-8) for (H, Ls, psi, ts) in reqs[6 * w : 6 * w + 6]]
+            got += [f.result(timeout=120) for f in futs]
+            time.sleep(0.01)  # every workgroup leaves (idle) before the next wave
+    stats = dict(srv.stats)
+    print(f"queue launches {stats['queue_launches']}, batches {stats['launches']}, "
+          f"queue busy {stats['queue_busy_s'] * 1e3:.2f} ms", flush=True)
+    assert stats['queue_launches'] >= 4 and stats['requests'] == len(reqs)
+    assert stats['queue_busy_s'] > 0
+    sreqs = [serving.Request(H=H, Ls=Ls, rho0=psi, tsave=ts, rtol=1e-6, atol=1e-8) for (H, Ls, psi, ts) in reqs]
+    stream = torch.cuda.Stream()
+    keep, outs, host_out, ev, _ = serving.pack_and_launch(sreqs, stream)
+    ev.synchronize()
+    for b, (o, g) in enumerate(zip(outs, got)):
+        st, ex, ns, na, nr, status, _ = serving._read_outputs(host_out, o)
+        assert status == 0 and g.status == 0
+        assert (g.nsteps, g.naccepted, g.nrejected) == (ns, na, nr), b
+        np.testing.assert_allclose(g.states, st, rtol=0, atol=1e-6)
+    del keep
