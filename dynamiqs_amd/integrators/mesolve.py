@@ -370,12 +370,6 @@ def _mesolve_vectorized(H, Ls, rho0, tsave, exp_ops, method, gradient, options) 
     )
     plan.run()
     res = plan.results()
-    if adaptive_rouchon:  # FP64 on the device; results in the dtype of the other paths
-        res['states'] = res['states'].astype(np.complex64)
-        if 'expects' in res:
-            res['expects'] = res['expects'].astype(np.complex64)
-    if np.any(res['status'] == -6):
-        raise RuntimeError('an instance was rejected by the MI355X kernel (unsupported shape or options)')
     if np.any(res['status'] == -3):
         raise max_steps_error()
     if np.any(res['status'] == -4):
