@@ -921,7 +921,8 @@ __global__ __launch_bounds__(256) void k_prepare_ops(const dqh_me_inst *__restri
 // dst_desc}. Thread 0 claims the next published slot (head < tail, compare-and-swap
 // on head) or, with none published, sleeps and polls again; it gives up when the
 // host sets stop, when no workgroup has claimed a request for idle_ticks of the wall
-// clock, or (before a claim) once the workgroup is older than life_ticks, so every
+// clock, or once the workgroup is older than life_ticks (checked at every poll, so an
+// idle workgroup leaves at once and a busy one after its request), so every
 // workgroup exits and the launch ends regularly even under load. The workgroup then copies the inputs and the descriptor to the
 // device itself, completes Hnh and C (as dqh_dense_prepare_ops), solves the request
 // with the bodies of k_tsit5 and signals it; no host-side copy or launch is needed
@@ -942,7 +943,7 @@ __global__ __launch_bounds__(kThreads) void k_tsit5_queue(dqh_queue q) {
       int64_t got = -1;
       const uint64_t t_idle = wall_clock64();
       for (;;) {
-        if (t_idle - t_birth > (uint64_t)q.life_ticks) break;
+        if (wall_clock64() - t_birth > (uint64_t)q.life_ticks) break;
         const int64_t t = __hip_atomic_load(tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
         int64_t h = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (h < t) {

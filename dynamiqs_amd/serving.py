@@ -419,24 +419,30 @@ def _pinned_pool(coherent: bool) -> _PinnedPool:
 
 
 def _serving_streams():
-    """The queue kernel's stream and the grid-wide launches' two streams, created
-    together once per process and shared by every Server: streams created in a row
-    get distinct hardware queues, so the persistent queue kernel shares none with a
-    grid-wide launch (a kernel waits for the earlier ones of its hardware queue)."""
+    """The queue grid's two streams (consecutive launches alternate, so that one can
+    start while the previous drains) and the grid-wide launches' two streams,
+    created together once per process and shared by every Server: streams created
+    in a row get distinct hardware queues, so the persistent queue grid shares none
+    with a grid-wide launch (a kernel waits for the earlier ones of its hardware
+    queue)."""
     global _streams
     import torch
 
     with _pool_lock:
         if _streams is None:
-            _streams = (torch.cuda.Stream(), [torch.cuda.Stream(), torch.cuda.Stream()])
+            _streams = ([torch.cuda.Stream(), torch.cuda.Stream()], [torch.cuda.Stream(), torch.cuda.Stream()])
         return _streams
 
 
 class _WorkQueue:
     """The ring of ``dqh_mesolve_tsit5_dense_queue`` and its persistent launches.
-    ``publish`` fills ring slots and then advances tail; the scheduler relaunches the
-    grid whenever published requests are outstanding and no launch is running (its
-    workgroups leave when idle for idle_ms or after life_ms)."""
+    ``publish`` fills ring slots and then advances tail. The scheduler keeps a launch
+    of the grid running while published requests are outstanding: a launch's
+    workgroups leave when no claim was made for idle_ms or stop claiming after
+    life_ms (an idle workgroup then leaves at once, a busy one after its request), so
+    the next launch is started on the other stream once the newest one is life_ms
+    old: it claims from the same head and takes the CUs the old launch frees while
+    that one drains its last requests."""
 
     def __init__(self, capacity: int, workgroups: int, idle_ms: float, life_ms: float):
         import torch
@@ -464,8 +470,10 @@ class _WorkQueue:
         self.tail = 0
         self.outstanding = 0  # published and not yet completed
         self.slot_busy = np.zeros(capacity, dtype=bool)
-        self.stream = _serving_streams()[0]
-        self.event = None
+        self.streams = _serving_streams()[0]
+        self.life_s = float(life_ms) * 1e-3
+        self.gens = []  # (stream, start event, end event, host launch time) of launches not known to be done
+        self.event = None  # end event of the newest launch
         self.launches = 0
         self._lock = threading.Lock()
 
@@ -498,26 +506,47 @@ class _WorkQueue:
     def running(self) -> bool:
         return self.event is not None and not self.event.query()
 
+    def retire_done(self) -> list:
+        """(start, end) events of the launches that have ended since the last call."""
+        done = [g for g in self.gens if g[2].query()]
+        self.gens = [g for g in self.gens if g not in done]
+        return [(g[1], g[2]) for g in done]
+
+    def needs_launch(self) -> bool:
+        """Requests are outstanding and no running launch will keep claiming for long."""
+        if self.outstanding == 0:
+            return False
+        live = [g for g in self.gens if not g[2].query()]
+        return not live or time.perf_counter() - live[-1][3] > self.life_s
+
     def launch(self) -> None:
         import torch
 
-        with torch.cuda.stream(self.stream):
+        stream = self.streams[self.launches % len(self.streams)]
+        with torch.cuda.stream(stream):
+            ev0 = torch.cuda.Event(enable_timing=True)
+            ev0.record(stream)
             _native.check(_native.lib().dqh_mesolve_tsit5_dense_queue(_native.ctypes.byref(self.q), self.workgroups,
-                                                                      _native.ctypes.c_void_p(self.stream.cuda_stream)),
+                                                                      _native.ctypes.c_void_p(stream.cuda_stream)),
                           'dqh_mesolve_tsit5_dense_queue')
             self.event = torch.cuda.Event(enable_timing=True)
-            self.event.record(self.stream)
+            self.event.record(stream)
+        self.gens.append((stream, ev0, self.event, time.perf_counter()))
         self.launches += 1
 
     def busy_seconds(self) -> float:
         return float(self.counters[32].item()) / (self.khz * 1e3)
 
-    def close(self) -> None:
+    def close(self) -> list:
+        """Stop the grid; returns the (start, end) events of the launches not yet retired."""
         with self._lock:
             self.ctrl[32] = 1
-        if self.event is not None:
-            self.event.synchronize()
+        for g in self.gens:
+            g[2].synchronize()
+        rest = [(g[1], g[2]) for g in self.gens]
+        self.gens = []
         self.pool.free(self.blk)
+        return rest
 
 
 def pack_queued(reqs: list, pool: _PinnedPool, out_pool: _PinnedPool, times: dict | None = None):
@@ -641,7 +670,6 @@ class Server:
         self.streams = _serving_streams()[1][: max(1, int(nstreams))]
         # small requests go to the persistent work-queue kernel (its own stream)
         self._wq = _WorkQueue(int(queue_capacity), int(queue_workgroups), float(idle_ms), float(life_ms))
-        self._q_ev0 = None
         self._q: queue.Queue = queue.Queue()
         self._pending: list = []
         self._inflight: list = []
@@ -684,9 +712,7 @@ class Server:
     def close(self) -> None:
         self._stop.set()
         self._thread.join()
-        self._wq.close()
-        if self._q_ev0 is not None:
-            self._busy.append((self._q_ev0, self._wq.event))
+        self._busy += self._wq.close()
         self.stats['busy_s'] = self.busy_seconds()
         self.stats['queue_launches'] = self._wq.launches
         self.stats['queue_busy_s'] = self._wq.busy_seconds()
@@ -814,18 +840,11 @@ class Server:
         self.stats['max_concurrent_launches'] = max(self.stats['max_concurrent_launches'], len(self._inflight))
 
     def _ensure_queue(self) -> None:
-        """(Re)launch the work-queue grid when requests are outstanding and no launch
-        of it runs: its workgroups leave when idle or old, and a launch that has
-        ended has completed every request it took."""
-        import torch
-
-        if self._wq.outstanding == 0 or self._wq.running():
-            return
-        if self._q_ev0 is not None and self._wq.event is not None:
-            self._busy.append((self._q_ev0, self._wq.event))
-        self._q_ev0 = torch.cuda.Event(enable_timing=True)
-        self._q_ev0.record(self._wq.stream)
-        self._wq.launch()
+        """Keep a launch of the work-queue grid running while requests are outstanding
+        (a launch that has ended has completed every request it took)."""
+        self._busy += self._wq.retire_done()
+        if self._wq.needs_launch():
+            self._wq.launch()
 
     def _answer(self, L: _Launch, b: int, res, t: float) -> None:
         """Complete request b of launch L with its outputs (any thread; the scheduler
