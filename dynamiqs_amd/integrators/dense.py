@@ -18,7 +18,7 @@ import scipy.linalg as sla
 from .. import _native
 from .operators import spline_table
 
-MAX_NP = 256
+MAX_NP = 1024
 
 
 def pad16(n: int) -> int:
@@ -768,6 +768,8 @@ def build_plan(
 # batch leaves most CUs idle with one workgroup per instance.
 GRID_MIN_N = 128
 GRID_MAX_BATCH = 64
+# above this dimension one workgroup per instance is far too slow for any batch size
+GRID_ALWAYS_N = 256
 
 
 def use_grid(grid, n: int, B: int, npwc: int, nE: int, save_states: bool) -> bool:
@@ -775,7 +777,7 @@ def use_grid(grid, n: int, B: int, npwc: int, nE: int, save_states: bool) -> boo
     expectation values are taken from the saved states)."""
     ok = npwc == 0 and (nE == 0 or save_states)
     if grid == 'auto':
-        return ok and n >= GRID_MIN_N and B <= GRID_MAX_BATCH
+        return ok and n >= GRID_MIN_N and (B <= GRID_MAX_BATCH or n > GRID_ALWAYS_N)
     if grid and not ok:
         raise ValueError('the grid-wide Tsit5 needs constant generators and saved states for expectation values')
     return bool(grid)

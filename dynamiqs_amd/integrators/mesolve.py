@@ -178,7 +178,7 @@ def mesolve(
     (``dynamiqs/integrators/apis/mesolve.py:62-304``). Methods: ``Tsit5`` and
     ``Dopri5`` (adaptive; with ``Options(vectorized=True)`` the Liouvillian as a
     superoperator, n <= 32), ``Dopri8`` (adaptive 8(7) pair on the dense kernels,
-    n <= 256, steps landing on the save times), ``Euler``, ``Rouchon1``, ``Rouchon2`` /
+    n <= 1024, steps landing on the save times), ``Euler``, ``Rouchon1``, ``Rouchon2`` /
     ``Rouchon3`` (fixed step with ``dt``; without ``dt`` the adaptive 1-2 / 2-3 pairs in
     FP64, n <= 64, constant jump operators), ``Expm``. Operators: constant (dense or DIA; constant DIA
     operators from n = 128 on run on the DIA kernels), piecewise-constant

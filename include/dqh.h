@@ -42,7 +42,7 @@ enum dqh_status {
  * zero-padded to np x np (np = 16 * ceil(n / 16)). */
 typedef struct dqh_me_inst {
   int64_t n;          /* Hilbert-space dimension */
-  int64_t np;         /* padded dimension, multiple of 16, <= 256 */
+  int64_t np;         /* padded dimension, multiple of 16, <= 1024 */
   int64_t nL;         /* jump-operator terms in the RHS (D/C pairs) */
   int64_t nE;         /* expectation operators */
   int64_t ntsave;     /* save times */

@@ -67,3 +67,25 @@ def test_grid_equals_persistent_kernel_and_graph_equals_plain():
     assert np.array_equal(outs['graph']['nsteps'], outs['plain']['nsteps'])
     err = np.abs(outs['graph']['states'].astype(complex) - outs['persistent']['states']).max()
     assert err <= TOL, err
+
+
+def test_dense_above_256_through_mesolve():
+    """Dense operators above n = 256 (up to np = 1024): the plan takes the grid-wide
+    Tsit5 whatever the batch size, and dq.mesolve matches the complex128 oracle."""
+    import dynamiqs_amd as dq
+    from dynamiqs_amd.integrators import dense
+    from oracle import mesolve as om
+
+    rng = np.random.default_rng(300)
+    n = 300
+    insts = _random_instances(rng, n, 2, 1)
+    tsave = np.linspace(0.0, 1.0, 5)
+    plan = dense.build_plan('tsit5', insts, tsave, rtol=1e-6, atol=1e-6)
+    assert plan.grid is not None and plan.n == n
+    assert dense.use_grid('auto', 600, 128, 0, 0, True)  # large n: grid-wide for any batch
+    inst = insts[0]
+    print(f'dense n={n} through dq.mesolve', flush=True)
+    r = dq.mesolve(inst.H, inst.Ls, inst.rho0, tsave, method=dq.method.Tsit5(rtol=1e-6, atol=1e-6))
+    ref = om.mesolve(inst.H, inst.Ls, inst.rho0, tsave, method='tsit5', rtol=1e-6, atol=1e-6)
+    err = np.linalg.norm(r.states.to_numpy() - ref['states'], axis=(-2, -1)).max()
+    assert err <= TOL, err
