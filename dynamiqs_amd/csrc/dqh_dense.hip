@@ -401,7 +401,7 @@ __device__ __forceinline__ void tsit5_global(const dqh_me_inst &p, double *sh) {
     }
     const double d2 = sqrt(block_sum<NW>(s2, sh) * inv_nn) / h0;
     const double md = fmax(d1, d2);
-    const double h1 = md <= 1e-15 ? fmax(1e-6, h0 * 1e-3) : ctrl_pow(0.01 / md, 0.2);
+    const double h1 = md <= 1e-15 ? fmax(1e-6, h0 * 1e-3) : ctrl_pow(0.01 / md, kInitExp);
     dt = fmin(100.0 * h0, h1);
   }
 
@@ -603,7 +603,7 @@ __device__ __forceinline__ void rk8_global(const dqh_me_inst &p, double *sh) {
   wg_rhs<NW>(v, v.D, v.C, hasH, k8(v, w, 0));
   ++nrhs;
 
-  // initial step (Hairer-Norsett-Wanner II.4), exponent 1/8
+  // initial step (Hairer-Norsett-Wanner II.4), exponent 1/7 (the embedded order, as kInitExp)
   double dt;
   {
     double s0 = 0, s1 = 0;
@@ -635,7 +635,7 @@ __device__ __forceinline__ void rk8_global(const dqh_me_inst &p, double *sh) {
     }
     const double d2 = sqrt(block_sum<NW>(s2, sh) * inv_nn) / h0;
     const double md = fmax(d1, d2);
-    const double h1 = md <= 1e-15 ? fmax(1e-6, h0 * 1e-3) : ctrl_pow(0.01 / md, 1.0 / 8.0);
+    const double h1 = md <= 1e-15 ? fmax(1e-6, h0 * 1e-3) : ctrl_pow(0.01 / md, 1.0 / 7.0);
     dt = fmin(100.0 * h0, h1);
   }
 

@@ -406,7 +406,7 @@ __global__ __launch_bounds__(256) void k_init_dt(const dqh_me_inst *__restrict__
   const double d1 = sqrt(reduce_partials<256>(partial(g, inst, 1), g.npart, sh) * inv_nn);
   const double d2 = sqrt(reduce_partials<256>(partial(g, inst, 0), g.npart, sh) * inv_nn) / h0;
   const double md = fmax(d1, d2);
-  const double h1 = md <= 1e-15 ? fmax(1e-6, h0 * 1e-3) : ctrl_pow(0.01 / md, 0.2);
+  const double h1 = md <= 1e-15 ? fmax(1e-6, h0 * 1e-3) : ctrl_pow(0.01 / md, kInitExp);
   Ctrl o = c;
   o.dt = fmin(100.0 * h0, h1);
   const double hn = step_h(o, p.t1);

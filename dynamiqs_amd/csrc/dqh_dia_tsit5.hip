@@ -379,7 +379,7 @@ __global__ __launch_bounds__(256) void k_init_h0(V v) {
   }
 }
 
-// dt0 = min(100 h0, (0.01 / max(d1, d2))^(1/5)); control block 0 and ys_1 of the
+// dt0 = min(100 h0, (0.01 / max(d1, d2))^kInitExp); control block 0 and ys_1 of the
 // first attempt.
 __global__ __launch_bounds__(256) void k_init_dt(V v) {
   const RKTab &rk = rk_tab(v.flags);
@@ -391,7 +391,7 @@ __global__ __launch_bounds__(256) void k_init_dt(V v) {
   const double d1 = sqrt(reduce_partials<256>(partial(v, inst, 1), v.npart, sh) * v.inv_nn);
   const double d2 = sqrt(reduce_partials<256>(partial(v, inst, 0), v.npart, sh) * v.inv_nn) / h0;
   const double md = fmax(d1, d2);
-  const double h1 = md <= 1e-15 ? fmax(1e-6, h0 * 1e-3) : pow(0.01 / md, 0.2);
+  const double h1 = md <= 1e-15 ? fmax(1e-6, h0 * 1e-3) : pow(0.01 / md, kInitExp);
   Ctrl o = c;
   o.dt = fmin(100.0 * h0, h1);
   const double hn = step_h(o, v.t1);

@@ -580,7 +580,7 @@ __device__ __forceinline__ void tsit5_resident(const dqh_me_inst &p, double *sh,
     }
     const double d2 = sqrt(block_sum<NW>(s2, sh) * inv_nn) / h0;
     const double md = fmax(d1, d2);
-    const double h1 = md <= 1e-15 ? fmax(1e-6, h0 * 1e-3) : ctrl_pow(0.01 / md, 0.2);
+    const double h1 = md <= 1e-15 ? fmax(1e-6, h0 * 1e-3) : ctrl_pow(0.01 / md, kInitExp);
     dt = wave_uniform(fmin(100.0 * h0, h1));
   }
 

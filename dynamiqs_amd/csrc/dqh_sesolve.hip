@@ -285,7 +285,7 @@ __global__ __launch_bounds__(T) void k_sesolve(const dqh_me_inst *__restrict__ i
       }
       const double d2 = sqrt(bsum<T>(s2, red) * inv_n) / h0;
       const double md = fmax(d1, d2);
-      const double h1 = md <= 1e-15 ? fmax(1e-6, h0 * 1e-3) : ctrl_pow(0.01 / md, 0.2);
+      const double h1 = md <= 1e-15 ? fmax(1e-6, h0 * 1e-3) : ctrl_pow(0.01 / md, kInitExp);
       dt = fmin(100.0 * h0, h1);
     }
     bool need_k0 = false;

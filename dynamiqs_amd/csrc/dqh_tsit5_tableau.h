@@ -20,9 +20,12 @@ struct RKTab {
 };
 
 // [0] Tsit5. [1] Dopri5: J. R. Dormand, P. J. Prince, J. Comput. Appl. Math. 6 (1980)
-// 19-26, with the quartic dense output through y0, y1, f0, f1 and Shampine's
-// midpoint value (Math. Comp. 46 (1986) 135-150) written in the same b_i(theta)
-// form; its theta^4 column equals the dense-output constants of the DOPRI5 code.
+// 19-26. Its error weights are b - b_hat with Shampine's fourth-order b_hat
+// (1951/21600, 0, 22642/50085, 451/720, -12231/42400, 649/6300, 1/60; Math. Comp.
+// 46 (1986) 135-150), not the 1980 paper's (5179/57600, ..., 1/40). The dense
+// output is the quartic through y0, y1, f0, f1 and Shampine's midpoint value, in
+// the same b_i(theta) form. Its theta^4 column equals the dense-output constants
+// of the DOPRI5 code.
 static __constant__ RKTab kRK[2] = {
     {{0, 0.161, 0.327, 0.9, 0.9800255409045097, 1, 1},
      {{0, 0, 0, 0, 0, 0, 0},
@@ -49,8 +52,8 @@ static __constant__ RKTab kRK[2] = {
       {2.9525986892242035, -11.595793324188385, 9.822892851699436, -0.2908093278463649, 0, 0, 0},
       {2.8462752525252526, -10.757575757575758, 8.906422717743473, 0.2784090909090909, -0.2735313036020583, 0, 0},
       {0.09114583333333333, 0, 0.44923629829290207, 0.6510416666666666, -0.322376179245283, 0.13095238095238096, 0}},
-     {0.0012326388888888888, 0, -0.0042527702905061394, 0.03697916666666667, -0.05086379716981132,
-      0.0419047619047619, -0.025},
+     {0.0008217592592592593, 0, -0.0028351801936707596, 0.024652777777777777, -0.03390919811320755,
+      0.027936507936507936, -0.016666666666666666},
      {{1.0, -2.8535800653862835, 3.0717434641059005, -1.1270175653862835},
       {0, 0, 0, 0},
       {0, 4.023133379230305, -6.249321565289, 2.675424484351598},
@@ -59,6 +62,15 @@ static __constant__ RKTab kRK[2] = {
       {0, -1.3744241142186024, 3.272657752246729, -1.7672812570757455},
       {0, 1.3824689317781436, -3.764937863556287, 2.382468931778144}}},
 };
+
+// Initial step h1 = (0.01 / max(d1, d2))^kInitExp (Hairer-Norsett-Wanner II.4). The
+// exponent is one over the order of the embedded solution (4), while the
+// controller uses one over the order of the pair (5). This is the exponent with
+// which both pairs take exactly the step counts the reference prints for its
+// examples (README.md:75, apis/mesolve.py:225, apis/sesolve.py:164,
+// apis/floquet.py:139, docs/documentation/basics/workflow.md:88). With 1/5,
+// every Tsit5 example takes one step fewer.
+constexpr double kInitExp = 0.25;
 
 // the pair of a descriptor: Dopri5 with DQH_FLAG_DOPRI5, Tsit5 otherwise
 __device__ __forceinline__ const RKTab &rk_tab(int64_t flags) { return kRK[(flags & DQH_FLAG_DOPRI5) ? 1 : 0]; }

@@ -12,8 +12,10 @@ vendored in the reference, so this module restates the published method:
   formulae", J. Comput. Appl. Math. 7 (1981) 67-75, method RK8(7)13M: 13 stages,
   the 8th-order solution propagated (local extrapolation), the 7th-order one
   for the error estimate.
-* Error norm, integral controller and initial step as ``oracle.tsit5`` with the
-  exponent ``1/8`` (error order 8).
+* Error norm, integral controller and initial step as ``oracle.tsit5``: the
+  controller exponent is ``1/8`` (the order of the pair). The initial-step exponent
+  is ``1/7``, the order of the embedded solution, by the rule fitted to the 5(4)
+  pairs. No printed count pins it for this pair (parity unpinned).
 * Saving: steps land on the save times (and on ``jump_ts`` and ``t1``), so every
   saved state is a step end. diffrax evaluates its own continuous extension at
   the save times instead; that interpolant is not restated here, so saved values
@@ -80,7 +82,7 @@ E = np.array([float(b - bh) for b, bh in zip(B_FR, BHAT_FR)])
 
 
 def initial_step(f, t0: float, y0: np.ndarray, f0: np.ndarray, rtol: float, atol: float, stats=None) -> float:
-    """Hairer-Norsett-Wanner II.4 with the exponent 1/8."""
+    """Hairer-Norsett-Wanner II.4 with the exponent 1/7 (order of the embedded solution)."""
     scale = atol + np.abs(y0) * rtol
     d0 = rms_norm(y0 / scale)
     d1 = rms_norm(f0 / scale)
@@ -90,7 +92,7 @@ def initial_step(f, t0: float, y0: np.ndarray, f0: np.ndarray, rtol: float, atol
         stats.nrhs += 1
     d2 = rms_norm((f1 - f0) / scale) / h0
     max_d = max(d1, d2)
-    h1 = max(1e-6, h0 * 1e-3) if max_d <= 1e-15 else (0.01 / max_d) ** (1.0 / ORDER)
+    h1 = max(1e-6, h0 * 1e-3) if max_d <= 1e-15 else (0.01 / max_d) ** (1.0 / (ORDER - 1))
     return min(100 * h0, h1)
 
 

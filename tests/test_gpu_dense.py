@@ -172,6 +172,29 @@ def test_tsit5_pwc_parity():
         assert err <= TOL, (b, err)
 
 
+def test_printed_step_counts_through_mesolve_and_sesolve(kats):
+    """The reference prints these infos for its README example (``README.md:75``),
+    its mesolve docstring (``dynamiqs/integrators/apis/mesolve.py:225``) and its
+    sesolve docstring (``dynamiqs/integrators/apis/sesolve.py:164``). The
+    kernels' default Tsit5 takes exactly those step counts."""
+    import dynamiqs_amd as dq
+
+    k = kats['readme_lossy_oscillator']
+    a = dq.destroy(k['n'])
+    print('README lossy oscillator through dq.mesolve', flush=True)
+    r = dq.mesolve(k['omega'] * a.dag() @ a, [np.sqrt(k['kappa']) * a], dq.coherent(k['n'], k['alpha0']),
+                   np.linspace(0, k['T'], k['ntsave']))
+    assert str(r.infos) == f"{k['nsteps']} steps ({k['naccepted']} accepted, {k['nrejected']} rejected)"
+    d = kats['mesolve_docstring']
+    print('mesolve docstring example', flush=True)
+    r = dq.mesolve(a.dag() @ a, [a], dq.coherent(d['n'], d['alpha0']), np.linspace(0, d['T'], d['ntsave']))
+    assert str(r.infos) == f"{d['nsteps']} steps ({d['naccepted']} accepted, {d['nrejected']} rejected)"
+    s = kats['sesolve_docstring']
+    print('sesolve docstring example', flush=True)
+    r = dq.sesolve(a.dag() @ a, dq.coherent(s['n'], s['alpha0']), np.linspace(0, s['T'], s['ntsave']))
+    assert str(r.infos) == f"{s['nsteps']} steps ({s['naccepted']} accepted, {s['nrejected']} rejected)"
+
+
 def test_max_steps_raises_reference_error():
     import dynamiqs_amd as dq
 

@@ -379,12 +379,61 @@ def test_tsit5_readme_kat(kats):
     al = k['alpha0'] * np.exp(-k['kappa'] * k['T'] / 2 - 1j * k['omega'] * k['T'])
     # Tsit5 global error at rtol = atol = 1e-6 over 40 steps is ~1e-4 in Frobenius norm
     assert np.linalg.norm(r['final_state'] - op.coherent_dm(n, al)) < 2e-4
-    # printed step counts are matched to within one step (diffrax's initial-step rule is unpinned)
-    assert abs(r['infos']['nsteps'] - k['nsteps']) <= 1
-    assert r['infos']['nrejected'] == k['nrejected']
+    # the printed step counts, exactly
+    assert (r['infos']['nsteps'], r['infos']['naccepted'], r['infos']['nrejected']) == (
+        k['nsteps'], k['naccepted'], k['nrejected'])
     d = kats['mesolve_docstring']
     r = om.mesolve(a.conj().T @ a, [a], op.coherent(n, d['alpha0']), np.linspace(0, d['T'], d['ntsave']))
-    assert abs(r['infos']['nsteps'] - d['nsteps']) <= 1
+    assert (r['infos']['nsteps'], r['infos']['naccepted'], r['infos']['nrejected']) == (
+        d['nsteps'], d['naccepted'], d['nrejected'])
+
+
+def _counts(st):
+    return st.nsteps, st.naccepted, st.nrejected
+
+
+def test_printed_step_counts_of_sesolve_floquet_and_dopri5(kats):
+    """The step counts printed by the reference for its sesolve and floquet docstring
+    examples (Tsit5) and its Rabi workflow (Dopri5 at atol = 1e-8, with rejected
+    steps). Together with the two mesolve examples they fix the initial-step
+    exponent (1/4) and Dopri5's embedded weights (Shampine's). With 1/5, or with the
+    1980 embedded weights, these counts are missed: checked below by planting each
+    change in a copy of the oracle's tableau and constants."""
+    from oracle import dopri5
+    from oracle import sesolve as ose
+
+    d = kats['sesolve_docstring']
+    a = op.destroy(d['n'])
+    r = ose.sesolve(a.conj().T @ a, op.coherent(d['n'], d['alpha0']), np.linspace(0, d['T'], d['ntsave']))
+    assert (r['infos']['nsteps'], r['infos']['naccepted'], r['infos']['nrejected']) == (
+        d['nsteps'], d['naccepted'], d['nrejected'])
+
+    # floquet: the propagator from 0 to T, saved at tsave and at t0 + T
+    q = kats['floquet_docstring']
+    sx = op.sigmax()
+    ts = np.append(np.linspace(0, q['T'], q['ntsave']), q['T'])
+    fq = lambda t, y, tseg: -1j * np.cos(2 * np.pi * t) * (sx @ y)  # noqa: E731
+    _, _, st = tsit5.integrate(fq, np.eye(2, dtype=complex), ts)
+    assert _counts(st) == (q['nsteps'], q['naccepted'], q['nrejected'])
+
+    w = kats['workflow_rabi_dopri5']
+    H = w['delta'] * op.sigmaz() + w['Omega'] * sx
+    tsw = np.linspace(0.0, w['sim_time'], w['num_save'])
+    r = ose.sesolve(H, op.fock(2, 0), tsw, method='dopri5', rtol=w['rtol'], atol=w['atol'])
+    want_w = (w['nsteps'], w['naccepted'], w['nrejected'])
+    assert (r['infos']['nsteps'], r['infos']['naccepted'], r['infos']['nrejected']) == want_w
+
+    # planted: HINIT's own exponent 1/5 drops one step of the sesolve example, and the
+    # 1980 embedded weights take more accepted Dopri5 steps
+    import unittest.mock as um
+
+    with um.patch.object(tsit5, 'INIT_ORDER', 5):
+        r = ose.sesolve(a.conj().T @ a, op.coherent(d['n'], d['alpha0']), np.linspace(0, d['T'], d['ntsave']))
+        assert r['infos']['nsteps'] == d['nsteps'] - 1
+    bhat80 = np.array([5179 / 57600, 0.0, 7571 / 16695, 393 / 640, -92097 / 339200, 187 / 2100, 1 / 40])
+    with um.patch.object(dopri5, 'E', dopri5.B - bhat80):
+        r = ose.sesolve(H, op.fock(2, 0), tsw, method='dopri5', rtol=w['rtol'], atol=w['atol'])
+        assert r['infos']['naccepted'] > w['naccepted'] + 1
 
 
 def test_qubit_decay_doc_kat(kats):
