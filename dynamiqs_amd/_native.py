@@ -69,6 +69,7 @@ FLAG_EULER = 16
 FLAG_LMOD = 32
 FLAG_SIGNAL = 64
 FLAG_DOPRI8 = 128
+FLAG_LPWC = 256
 
 STATUS_OK = 0
 STATUS_MAX_STEPS = -3

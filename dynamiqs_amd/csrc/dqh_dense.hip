@@ -178,15 +178,25 @@ __device__ __forceinline__ double2 mod_eval(const dqh_me_inst &p, int j, double 
 }
 
 // time-dependent jump operators (DQH_FLAG_LMOD): D_k <- f_k(tau) L_k and
-// C_k <- conj(f_k(tau)) L_k^dag / 2 in work at (10 + nL) np^2, f_k = table nmod + k
-__device__ __forceinline__ void lmod_update(const dqh_me_inst &p, double tau, int nthreads) {
+// C_k <- conj(f_k(tau)) L_k^dag / 2 in work at (10 + nL) np^2, f_k = table nmod + k;
+// with DQH_FLAG_LPWC, f_k also carries the factor of row npwc + k of pwc_vals on the
+// segment seg of the step start (zero outside the breakpoints)
+__device__ __forceinline__ void lmod_update(const dqh_me_inst &p, double tau, int seg, int nthreads) {
   const int nL = (int)p.nL, nmod = (int)p.nmod;
   const size_t nn = (size_t)(p.np * p.np);
   float2 *Dc = gp<float2>(p.work) + (size_t)(10 + nL) * nn;
   float2 *Cc = Dc + (size_t)nL * nn;
   const float2 *D0 = gp<const float2>(p.Dops), *C0 = gp<const float2>(p.Cops);
+  const int nv = (int)p.nseg - 1;
+  const bool lpwc = (p.flags & DQH_FLAG_LPWC) != 0;
   for (int k = 0; k < nL; ++k) {
-    const double2 f = mod_eval(p, nmod + k, tau);
+    double2 f = mod_eval(p, nmod + k, tau);
+    if (lpwc) {
+      const double *vals = gp<const double>(p.pwc_vals) + 2 * (size_t)(p.npwc + k) * (size_t)(nv > 0 ? nv : 0);
+      const double cr = (seg >= 0 && seg < nv) ? vals[2 * seg] : 0.0;
+      const double ci = (seg >= 0 && seg < nv) ? vals[2 * seg + 1] : 0.0;
+      f = make_double2(f.x * cr - f.y * ci, f.x * ci + f.y * cr);
+    }
     for (size_t e = threadIdx.x; e < nn; e += nthreads) {
       const float2 d = D0[k * nn + e], c = C0[k * nn + e];
       Dc[k * nn + e] = make_float2((float)(f.x * d.x - f.y * d.y), (float)(f.x * d.y + f.y * d.x));
@@ -227,7 +237,7 @@ __device__ __forceinline__ void wg_update_pwc(const View &v, const dqh_me_inst &
     }
     v.Hcur[e] = make_float2((float)re, (float)im);
   }
-  if (p.flags & DQH_FLAG_LMOD) lmod_update(p, tau, NW * 64);
+  if (p.flags & DQH_FLAG_LMOD) lmod_update(p, tau, seg, NW * 64);
   __syncthreads();
 }
 

@@ -54,6 +54,7 @@ class Instance:
     t0: float | None = None  # per-instance start time
     mod: list = field(default_factory=list)  # [(g, op (n, n), key)]: + g(t) op, g scalar; equal keys share tables
     Lmod: list = field(default_factory=list)  # per jump operator: None (constant) or (g, key): L_k(t) = g(t) Ls[k]
+    Lpwc: list = field(default_factory=list)  # per jump operator: None or (times, values): L_k(t) = c(t) Ls[k], c PWC
 
 
 class _Arena:
@@ -363,6 +364,31 @@ def _one(t):
     return np.ones(np.shape(t)) + 0j
 
 
+def pwc_value(times, values, t: float) -> complex:
+    """values[searchsorted(times, t, 'right') - 1], zero outside [times[0], times[-1])
+    (the reference's PWC prefactor, time_qarray.py:649-659)."""
+    times = np.asarray(times, dtype=np.float64)
+    if not (times[0] <= t < times[-1]):
+        return 0j
+    return complex(np.asarray(values)[int(np.searchsorted(times, t, side='right')) - 1])
+
+
+def pwc_rows(terms, extra):
+    """Union breakpoint grid of the PWC terms [(times, values, _)] and the times in
+    ``extra``, and each term's value on every grid interval (zero outside its own
+    [times[0], times[-1])), as complex rows."""
+    grid = np.unique(np.concatenate([np.asarray(t, dtype=np.float64) for (t, _, _) in terms]
+                                    + [np.asarray(extra, dtype=np.float64)]))
+    left = grid[:-1]
+    rows = []
+    for t, v, _ in terms:
+        t = np.asarray(t, dtype=np.float64)
+        v = np.asarray(v, dtype=np.complex128)
+        idx = np.clip(np.searchsorted(t, left, side='right') - 1, 0, len(v) - 1)
+        rows.append(np.where((left >= t[0]) & (left < t[-1]), v[idx], 0.0))
+    return grid, rows
+
+
 def _hmods(inst: Instance) -> list:
     """The modulated Hamiltonian-side terms of an instance: its own, plus for every
     modulated jump operator g_k(t) L_k the term -1/2 |g_k(t)|^2 L_k^dag L_k of
@@ -432,9 +458,10 @@ def build_plan(
     if tableau not in ('tsit5', 'dopri5', 'dopri8'):
         raise ValueError(f'unknown Runge-Kutta pair {tableau!r}')
     dopri8 = tableau == 'dopri8' and kind == 'tsit5'
-    if dopri8 and any(lm is not None for inst in instances for lm in (inst.Lmod or [])):
+    lpwc_any = any(lp is not None for inst in instances for lp in (inst.Lpwc or []))
+    lmod_any = any(lm is not None for inst in instances for lm in (inst.Lmod or [])) or lpwc_any
+    if dopri8 and lmod_any:
         raise NotImplementedError('Dopri8 with time-dependent jump operators is not supported.')
-    lmod_any = any(lm is not None for inst in instances for lm in (inst.Lmod or []))
     if (any(inst.mod for inst in instances) or lmod_any) and kind != 'rouchon1':
         if max(len(_hmods(inst)) for inst in instances) > _native.MAX_MOD:
             raise NotImplementedError(
@@ -518,7 +545,7 @@ def build_plan(
     nmod_max = max((len(_hmods(inst)) for inst in instances), default=0)
     # the FP64 diagonal goes with the single-size resident kernel (every instance of
     # the plan on it), never with the grid-wide kernels
-    npwc_all = max((len(inst.pwc) for inst in instances), default=0)
+    npwc_all = max((len(inst.pwc) + sum(lp is not None for lp in (inst.Lpwc or [])) for inst in instances), default=0)
     grid_plan = kind == 'tsit5' and not dopri8 and use_grid(grid_tsit5, n, B, npwc_all + nmod_max, nE, save_states)
     hdiag_plan = (fp64_diag and kind == 'tsit5' and npad in (16, 32, 48, 64) and K <= 3 and not use_global
                   and not lmod_any and not grid_plan)
@@ -573,18 +600,20 @@ def build_plan(
                 d['flags'] |= _native.FLAG_DOPRI5
             elif tableau == 'dopri8' and kind == 'tsit5':
                 d['flags'] |= _native.FLAG_DOPRI8
-            lmod = [lm is not None for lm in (inst.Lmod or [])]
+            lpwc = [k < len(inst.Lpwc or []) and inst.Lpwc[k] is not None for k in range(K)]
+            lmod = [(k < len(inst.Lmod or []) and inst.Lmod[k] is not None) or lpwc[k] for k in range(K)]
             if use_global or any(lmod):
                 d['flags'] |= _native.FLAG_GLOBAL
             if any(lmod):
                 d['flags'] |= _native.FLAG_LMOD
+            if any(lpwc):
+                d['flags'] |= _native.FLAG_LPWC
 
             def bH128(inst=inst, lmod=lmod):
                 if inst.Hnh is not None:
                     return _pad(np.asarray(inst.Hnh, dtype=np.complex128), npad)
-                # the modulated jump operators' L^dag L terms are time dependent (in Hmod)
-                LdL = sum((_dag(L) @ L for k, L in enumerate(inst.Ls) if not (k < len(lmod) and lmod[k])),
-                          np.zeros_like(inst.H))
+                # the time-dependent jump operators' L^dag L terms are in Hmod / Hpwc
+                LdL = sum((_dag(L) @ L for k, L in enumerate(inst.Ls) if not lmod[k]), np.zeros_like(inst.H))
                 return _pad(-1j * inst.H - 0.5 * LdL, npad)
 
             hkey = ('Hnh', inst.key_H, inst.key_L, tuple(lmod))
@@ -604,7 +633,28 @@ def build_plan(
                 d['Cops'] = shared(
                     ('C', inst.key_L), lambda Ls=Ls: _pad(0.5 * np.stack([_dag(L) for L in Ls]), npad).astype(np.complex64)
                 )
-            if inst.pwc:
+            if any(lpwc):
+                # PWC jump operators c_k(t) L_k: the terms -1/2 |c_k|^2 L_k^dag L_k join the
+                # PWC Hamiltonian terms (O = -i/2 L^dag L, so that -i O = -1/2 L^dag L), and
+                # rows npwc + k of pwc_vals hold every jump operator's coefficient (1 for
+                # the others) on a grid that covers the whole solve interval
+                jterms = [(inst.Lpwc[k][0], np.abs(np.asarray(inst.Lpwc[k][1], dtype=np.complex128)) ** 2,
+                           -0.5j * (_dag(L) @ L)) for k, L in enumerate(inst.Ls) if lpwc[k]]
+                pwc_eff = list(inst.pwc) + jterms
+                span = np.array([t_tab, t_stop])
+                coef = [(inst.Lpwc[k][0], inst.Lpwc[k][1], None) if lpwc[k] else (span, np.ones(1), None)
+                        for k in range(K)]
+                grid, rows = pwc_rows(pwc_eff + coef, span)
+                d['npwc'] = len(pwc_eff)
+                d['nseg'] = len(grid)
+                npwc_max = max(npwc_max, len(pwc_eff))
+                d['Hpwc'] = shared(
+                    ('Hpwc', inst.key_H, inst.key_L, tuple(lpwc)),
+                    lambda pwc_eff=pwc_eff: _pad(np.stack([-1j * o for (_, _, o) in pwc_eff]), npad).astype(np.complex64),
+                )
+                d['pwc_times'] = shared(('ptimes', inst.key_H, inst.key_L, tuple(lpwc)), lambda grid=grid: grid)
+                d['pwc_vals'] = arena.put(np.ascontiguousarray(np.stack(rows)).view(np.float64))
+            elif inst.pwc:
                 grid, samplers = pwc_sampling(inst.pwc)
                 vals = np.stack([np.where(inside, np.asarray(v, dtype=np.complex128)[idx], 0.0)
                                  for (_, v, _), (idx, inside) in zip(inst.pwc, samplers)])
@@ -617,12 +667,13 @@ def build_plan(
                 d['pwc_times'] = shared(('ptimes', inst.key_H), lambda grid=grid: grid)
                 d['pwc_vals'] = arena.put(np.ascontiguousarray(vals).view(np.float64))
             hm = _hmods(inst)
-            if hm:
+            if hm or any(lmod):
                 # tables: the nmod Hamiltonian-side terms, then (DQH_FLAG_LMOD) one per jump
                 # operator, g_k or a constant 1, all on one grid
                 fns = [(g, key) for (g, _, key) in hm]
                 if any(lmod):
-                    fns += [inst.Lmod[k] if lmod[k] else (_one, ('one',)) for k in range(K)]
+                    fns += [inst.Lmod[k] if (k < len(inst.Lmod or []) and inst.Lmod[k] is not None)
+                            else (_one, ('one',)) for k in range(K)]
                 tabs = [mod_table(g, key) for (g, key) in fns]
                 nint = max(len(c) for (_, _, c) in tabs)
                 tabs = [spline_table(g, t_tab, t_stop, n0=nint, nmax=nint) if len(c) != nint else (a, dtt, c)
@@ -630,10 +681,11 @@ def build_plan(
                 d['nmod'] = len(hm)
                 d['mod_t0'], d['mod_dt'], d['mod_nint'] = tabs[0][0], tabs[0][1], nint
                 d['mod_coef'] = arena.put(np.ascontiguousarray(np.stack([c for (_, _, c) in tabs])).view(np.float64))
-                d['Hmod'] = shared(
-                    ('Hmod', inst.key_H, inst.key_L if any(lmod) else None),
-                    lambda hm=hm: _pad(np.stack([-1j * o for (_, o, _) in hm]), npad).astype(np.complex64),
-                )
+                if hm:
+                    d['Hmod'] = shared(
+                        ('Hmod', inst.key_H, inst.key_L if any(lmod) else None),
+                        lambda hm=hm: _pad(np.stack([-1j * o for (_, o, _) in hm]), npad).astype(np.complex64),
+                    )
         else:  # rouchon1: Kraus sets per (PWC segment, step length)
             d['nL'] = nLdev
             grid = None
@@ -641,7 +693,8 @@ def build_plan(
                 grid = np.unique(np.concatenate([np.asarray(t, dtype=np.float64) for (t, _, _) in inst.pwc]))
             # smooth (modulated) terms make every step's operators distinct: one set per step
             lmods = list(inst.Lmod or [None] * K)
-            smooth = bool(inst.mod) or any(lm is not None for lm in lmods)
+            lpwcs = list(inst.Lpwc or [None] * K)
+            smooth = bool(inst.mod) or any(lm is not None for lm in lmods) or any(lp is not None for lp in lpwcs)
             key = ('kraus', inst.key_H, inst.key_L)
             cacheable = not any(k is None for k in key[1:])
             keys = []
@@ -687,6 +740,7 @@ def build_plan(
                     for g, op, _ in inst.mod:
                         Hm = Hm + complex(g(tk)) * op
                     Lm = [L if lm is None else complex(lm[0](tk)) * L for L, lm in zip(inst.Ls, lmods)]
+                    Lm = [L if lp is None else pwc_value(lp[0], lp[1], tk) * L for L, lp in zip(Lm, lpwcs)]
                 Dk, Ck = _kraus_sets(Hm, Lm, h, normalize, rouchon_order)
                 sets_D.append(np.stack(Dk))
                 sets_C.append(np.stack(Ck))

@@ -111,12 +111,13 @@ def enumerate_instances(H: TimeQArray, Ls, rho0: QArray, cartesian: bool):
     Ht = decompose(H)
     Lts = [decompose(L) for L in Ls]
     for i, Lt in enumerate(Lts):
-        # a time-dependent jump operator must be one modulated term f(t) L (its rate
-        # |f|^2 and phase are applied at every stage time)
-        if Lt.pwc or len(Lt.mod) > 1 or (Lt.mod and Lt.const is not None and np.any(Lt.const)):
+        # a time-dependent jump operator must be one term: modulated f(t) L (its rate
+        # |f|^2 and phase are applied at every stage time) or piecewise constant c(t) L
+        nterms = len(Lt.mod) + len(Lt.pwc) + (Lt.const is not None and bool(np.any(Lt.const)))
+        if (Lt.mod or Lt.pwc) and nterms > 1:
             raise NotImplementedError(
-                'time-dependent jump operators are supported by the MI355X solver only as one modulated term'
-                ' dq.modulated(f, L)'
+                'time-dependent jump operators are supported by the MI355X solver only as one term'
+                ' dq.modulated(f, L) or dq.pwc(times, values, L)'
             )
     r = rho0.to_numpy().astype(np.complex128)
     rb = r.shape[:-2]
@@ -138,18 +139,28 @@ def enumerate_instances(H: TimeQArray, Ls, rho0: QArray, cartesian: bool):
         rho = r[ir]
         if rho.shape[-1] == 1:
             rho = rho @ rho.conj().T
-        Lmods = []
+        Lmods, Lpwcs, Lops = [], [], []
         for Lt, i in zip(Lts, iLs):
             if Lt.mod:
                 g, _, f = Lt.mod_at(i)[0]
                 Lmods.append((g, (id(f), i if np.ndim(f(0.0)) else None)))
+                Lpwcs.append(None)
+                Lops.append(Lt.mod[0][1])
+            elif Lt.pwc:
+                times, vals, o = Lt.pwc_at(i)[0]
+                Lmods.append(None)
+                Lpwcs.append((times, vals))
+                Lops.append(o)
             else:
                 Lmods.append(None)
+                Lpwcs.append(None)
+                Lops.append(Lt.const_at(i))
         insts.append(
             dense.Instance(
                 H=Ht.const_at(iH),
-                Ls=[Lt.mod[0][1] if Lt.mod else Lt.const_at(i) for Lt, i in zip(Lts, iLs)],
+                Ls=Lops,
                 Lmod=Lmods if any(m is not None for m in Lmods) else [],
+                Lpwc=Lpwcs if any(m is not None for m in Lpwcs) else [],
                 rho0=rho,
                 pwc=Ht.pwc_at(iH),
                 mod=[(g, o, (id(f), iH if np.ndim(f(0.0)) else None)) for (g, o, f) in Ht.mod_at(iH)],
@@ -183,7 +194,7 @@ def mesolve(
     FP64, n <= 64, constant jump operators), ``Expm``. Operators: constant (dense or DIA; constant DIA
     operators from n = 128 on run on the DIA kernels), piecewise-constant
     (``dq.pwc``), modulated and ``timecallable`` Hamiltonians, constant jump
-    operators or one modulated term ``f(t) L``. ``gradient`` is validated as in the
+    operators or one term ``f(t) L`` / ``pwc(times, values, L)``. ``gradient`` is validated as in the
     reference and then rejected (no autodiff here; see ``dynamiqs_amd.adjoint``).
     """
     # === convert arguments
@@ -348,7 +359,7 @@ def _mesolve_vectorized(H, Ls, rho0, tsave, exp_ops, method, gradient, options) 
             f'Option `vectorized=True` integrates the {n * n} x {n * n} Liouvillian as a matrix-vector product on the '
             f'MI355X, which supports n^2 <= {VECTORIZED_MAX_N2} (n <= 32); use the default unvectorized form.'
         )
-    if any(inst.Lmod for inst in insts):
+    if any(inst.Lmod or inst.Lpwc for inst in insts):
         raise NotImplementedError('Option `vectorized=True` is supported with constant jump operators only.')
     if options.assume_hermitian:
         for inst in insts:

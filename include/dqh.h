@@ -125,6 +125,13 @@ typedef struct dqh_me_inst {
  * times; work needs (16 + nL) np^2 elements; not with DQH_FLAG_LMOD, not on the
  * uniform or grid-wide entry points. */
 #define DQH_FLAG_DOPRI8 128
+/* with DQH_FLAG_LMOD: jump operator k is additionally scaled by the piecewise-constant
+ * coefficient in row npwc + k of pwc_vals (pwc_vals then holds npwc + nL rows). The
+ * value is taken on the PWC segment of the step start, and it is zero outside
+ * [pwc_times[0], pwc_times[nseg-1]). The -1/2 |c_k|^2 L_k^dag L_k part of the
+ * generator is among the npwc Hamiltonian terms. This is the reference's
+ * pwc(times, values, L) as a jump operator (time_qarray.py:572-666). */
+#define DQH_FLAG_LPWC 256
 
 /* Size checks for FFI mirrors. */
 int64_t dqh_sizeof_me_inst(void);

@@ -77,7 +77,7 @@ def test_unsupported_time_dependence_is_reported():
     """What the kernels cannot run raises before any device work: callable
     operators that span more than four time-dependent directions, modulated
     operators with discontinuity times, jump operators other than one modulated
-    term; a non-callable f is a TypeError (time_qarray.py:161-165)."""
+    or piecewise-constant term; a non-callable f is a TypeError (time_qarray.py:161-165)."""
     H, Ls, psi, ts = _args()
     rng = np.random.default_rng(0)
     ops = [(lambda x: x + x.conj().T)(rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))) for _ in range(5)]
@@ -89,7 +89,10 @@ def test_unsupported_time_dependence_is_reported():
     with pytest.raises(NotImplementedError, match='time-dependent jump operators'):
         dq.mesolve(H, [Ls[0] + dq.modulated(np.cos, Ls[0])], psi, ts)
     with pytest.raises(NotImplementedError, match='time-dependent jump operators'):
-        dq.mesolve(H, [dq.pwc(np.array([0.0, 1.0]), np.array([1.0]), Ls[0])], psi, ts)
+        dq.mesolve(H, [Ls[0] + dq.pwc(np.array([0.0, 1.0]), np.array([1.0]), Ls[0])], psi, ts)
+    with pytest.raises(NotImplementedError, match='time-dependent jump operators'):
+        pw = dq.pwc(np.array([0.0, 1.0]), np.array([1.0]), Ls[0])
+        dq.mesolve(H, [pw + dq.pwc(np.array([0.0, 0.5]), np.array([2.0]), Ls[0])], psi, ts)
 
     with pytest.raises(TypeError, match=re.escape('Argument `f` must be a function, but has type float.')):
         dq.modulated(1.0, H)

@@ -263,3 +263,59 @@ def test_fixed_step_methods_with_modulated_operators(method):
     err = np.linalg.norm(res.states.to_numpy() - ref, axis=(-2, -1)).max()
     print(method, 'err vs oracle', err)
     assert err < 1e-5
+
+
+def test_pwc_jump_operators_against_oracle():
+    """``dq.pwc(times, values, L)`` as a jump operator (``time_qarray.py:572-666``): the
+    kernels scale L by the coefficient of the step's PWC segment (zero outside the
+    breakpoints) and carry -1/2 |c|^2 L^dag L as a PWC Hamiltonian-side term. Alone,
+    beside a constant jump operator, with a batched coefficient next to a PWC
+    Hamiltonian term and a modulated jump operator, and on the fixed-step Rouchon1
+    kernel: each against the complex128 oracle at the same tolerances."""
+    import dynamiqs_amd as dq
+    from oracle import mesolve as om
+    from oracle import operators as op
+
+    n = 8
+    a = op.destroy(n)
+    zero = np.zeros((n, n), dtype=complex)
+    H0 = a.conj().T @ a + 0.3 * (a + a.conj().T)
+    times = np.array([0.0, 0.4, 1.1, 2.0])
+    vals = np.array([0.5, 1.2 - 0.4j, 0.0])
+    tsave = np.linspace(0.0, 2.5, 6)
+    rho0 = op.coherent_dm(n, 0.8)
+    Lc = np.sqrt(0.1) * a.conj().T @ a
+    m = dq.method.Tsit5(rtol=1e-7, atol=1e-9)
+    pw = dq.pwc(times, vals, dq.asqarray(a))
+    pw_or = om.TimeOp(zero, [om.PWCTerm(times, vals, a)])
+    for name, Ls_dq, Ls_or in [('alone', [pw], [pw_or]), ('with a constant one', [pw, dq.asqarray(Lc)], [pw_or, Lc])]:
+        print(f'PWC jump operator {name}', flush=True)
+        r = dq.mesolve(dq.asqarray(H0), Ls_dq, dq.asqarray(rho0), tsave, method=m)
+        ref = om.mesolve(H0, Ls_or, rho0, tsave, rtol=1e-7, atol=1e-9)
+        err = np.linalg.norm(r.states.to_numpy() - ref['states'], axis=(-2, -1)).max()
+        assert err <= TOL, (name, err)
+        assert abs(int(r.infos.nsteps) - ref['infos']['nsteps']) <= 2, (name, r.infos.nsteps, ref['infos']['nsteps'])
+
+    # batched PWC coefficient, a PWC Hamiltonian term and a modulated jump operator
+    vals2 = np.array([[0.5, 1.2 - 0.4j, 0.0], [0.2j, 0.7, 1.5]])
+    X = a + a.conj().T
+    htimes, hvals = np.array([0.3, 1.7]), np.array([0.6])
+    g = lambda t: 0.4 * np.cos(2.0 * t)  # noqa: E731
+    Hdq = dq.asqarray(H0) + dq.pwc(htimes, hvals, dq.asqarray(X))
+    print('batched PWC jump operator with a PWC H term and a modulated jump operator', flush=True)
+    r = dq.mesolve(Hdq, [dq.pwc(times, vals2, dq.asqarray(a)), dq.modulated(g, dq.asqarray(a.conj().T @ a))],
+                   dq.asqarray(rho0), tsave, method=m)
+    st = r.states.to_numpy()
+    assert st.shape == (2, 6, n, n)
+    Hor = om.TimeOp(H0, [om.PWCTerm(htimes, hvals, X)])
+    for b in range(2):
+        Ls_or = [om.TimeOp(zero, [om.PWCTerm(times, vals2[b], a)]), om.TimeOp(zero, [om.ModTerm(g, a.conj().T @ a)])]
+        ref = om.mesolve(Hor, Ls_or, rho0, tsave, rtol=1e-7, atol=1e-9)
+        err = np.linalg.norm(st[b] - ref['states'], axis=(-2, -1)).max()
+        assert err <= TOL, (b, err)
+
+    print('PWC jump operator on Rouchon1', flush=True)
+    r = dq.mesolve(dq.asqarray(H0), [pw], dq.asqarray(rho0), tsave, method=dq.method.Rouchon1(dt=0.01))
+    ref = om.mesolve(H0, [pw_or], rho0, tsave, method='rouchon1', dt=0.01)
+    err = np.linalg.norm(r.states.to_numpy() - ref['states'], axis=(-2, -1)).max()
+    assert err <= TOL, err
