@@ -92,7 +92,8 @@ DIA_MIN_N = 128
 
 def dia_route(method, H: TimeQArray, Ls, n: int):
     """Operators for the DIA kernels, or None when the dense path applies: Tsit5, or
-    Rouchon1 with normalisation, constant un-batched DIA operators, n >= DIA_MIN_N."""
+    Rouchon1 with normalisation, constant DIA operators, n >= DIA_MIN_N; batched
+    operators only above the dense kernels' size limit (``('batched', H, Ls)``)."""
     if n < DIA_MIN_N:
         return None
     if isinstance(method, Dopri8):  # the 13-stage pair runs on the dense kernels only
@@ -101,9 +102,15 @@ def dia_route(method, H: TimeQArray, Ls, n: int):
         return None
     Ht = decompose(H)
     Lts = [decompose(L) for L in Ls]
-    if Ht.dia is None or Ht.batch != () or any(Lt.dia is None or Lt.batch != () for Lt in Lts):
+    if Ht.dia is None or any(Lt.dia is None for Lt in Lts):
         return None
-    return Ht.const_at(()), [Lt.const_at(()) for Lt in Lts]
+    if Ht.batch == () and all(Lt.batch == () for Lt in Lts):
+        return Ht.const_at(()), [Lt.const_at(()) for Lt in Lts]
+    # batched constant DIA operators: one DIA plan per distinct operator set, where
+    # the dense kernels (one launch for the whole batch) cannot take the size
+    if n > dense.MAX_NP:
+        return 'batched', Ht, Lts
+    return None
 
 
 def enumerate_instances(H: TimeQArray, Ls, rho0: QArray, cartesian: bool):
@@ -492,18 +499,92 @@ def combine_parts(x: np.ndarray, index: np.ndarray) -> np.ndarray:
     return out
 
 
+def _dia_groups(Ht, Lts, rho0: QArray, cartesian: bool):
+    """The reference's batching rules over batched DIA operators: the output shape and,
+    per distinct operator set (H index, jump-operator indices), its dense operators and
+    the (output position, rho0 index) pairs that use it."""
+    r = rho0.to_numpy()
+    rb = r.shape[:-2]
+    shapes = [Ht.batch, *[Lt.batch for Lt in Lts], rb]
+    if cartesian:
+        out_shape = tuple(d for sh in shapes for d in sh)
+    else:
+        out_shape = tuple(np.broadcast_shapes(*shapes))
+    groups = {}
+    for pos, idx in enumerate(np.ndindex(*out_shape) if out_shape else [()]):
+        if cartesian:
+            parts, o = [], 0
+            for sh in shapes:
+                parts.append(tuple(idx[o : o + len(sh)]))
+                o += len(sh)
+        else:
+            parts = [_broadcast_index(idx, sh) for sh in shapes]
+        groups.setdefault((parts[0], tuple(parts[1:-1])), []).append((pos, parts[-1]))
+    out = []
+    for (iH, iLs), members in groups.items():
+        ops = (Ht.const_at(iH), [Lt.const_at(i) for Lt, i in zip(Lts, iLs)])
+        out.append((ops, members))
+    return out_shape, out
+
+
 def _mesolve_dia(route, rho0: QArray, tsave, exp_ops, method, gradient, options) -> MESolveResult:
     """Tsit5 (``dqh_mesolve_tsit5_dia``, per-instance adaptive steps) or Rouchon1
     (``dqh_mesolve_rouchon1_dia``, lockstep) on the DIA sweeps: the batch is the
-    batch of ``rho0``, the operators are shared."""
-    Hd, Ld = route
-    r = rho0.to_numpy().astype(np.complex128)
-    out_shape = r.shape[:-2]
+    batch of ``rho0`` for shared operators; batched operators run one plan per
+    distinct operator set (each launch is grid-wide over the n^2 elements)."""
+    r_all = rho0.to_numpy().astype(np.complex128)
+    n = r_all.shape[-2]
+    if isinstance(route[0], str) and route[0] == 'batched':
+        out_shape, groups = _dia_groups(route[1], route[2], rho0, options.cartesian_batching)
+    else:
+        out_shape = r_all.shape[:-2]
+        rb = out_shape
+        groups = [(route, [(pos, idx) for pos, idx in enumerate(np.ndindex(*rb) if rb else [()])])]
+    Es = [E.to_numpy() for E in exp_ops] if exp_ops is not None else None
+    nout = int(np.prod(out_shape)) if out_shape else 1
+    states_all = None
+    expects_all = np.zeros((nout, len(Es), len(tsave)), dtype=np.complex64) if Es is not None else None
+    infos_all = np.zeros((3, nout), dtype=np.int64)
+    plan = None
+    for (Hd, Ld), members in groups:
+        r = np.stack([r_all[ir] for (_, ir) in members])
+        st, ex, inf, plan = _dia_solve(Hd, Ld, r, tsave, Es, method, options)
+        pos = np.array([p for (p, _) in members])
+        if states_all is None:
+            states_all = np.zeros((nout, *st.shape[1:]), dtype=st.dtype)
+        states_all[pos] = st
+        if Es is not None:
+            expects_all[pos] = ex
+        infos_all[:, pos] = inf
+    states = states_all
+    extra = None
+    if options.save_extra is not None:
+        extra = save_extra_outputs(options.save_extra, states, rho0.dims, out_shape)
+    states = states.reshape(*out_shape, -1, n, n)
+    if not options.save_states:  # the final state only, nsave = 1 (save_mixin.py:62-66, mesolve.py:184-187)
+        states = states[..., -1:, :, :]
+    ysave = QArray(states, dims=rho0.dims)
+    Esave = None
+    if Es is not None:
+        Esave = expects_all.reshape(*out_shape, len(Es), len(tsave))
+
+    def shaped(x):
+        x = np.asarray(x).reshape(out_shape)
+        return x if out_shape else x[()]
+
+    if isinstance(method, ADAPTIVE):
+        infos = AdaptiveStepInfos(shaped(infos_all[0]), shaped(infos_all[1]), shaped(infos_all[2]))
+    else:
+        infos = FixedStepInfos(shaped(np.full(nout, plan.nsteps, dtype=np.int64)))
+    return MESolveResult(tsave, method, gradient, options, ysave, Esave, extra, infos)
+
+
+def _dia_solve(Hd, Ld, r: np.ndarray, tsave, Es, method, options):
+    """One DIA plan over the initial states ``r`` (m, n, n|1) with shared operators:
+    (states (m, nsave, n, n), expects (m, nE, ntsave) or None, infos (3, m), plan)."""
     n = r.shape[-2]
-    r = r.reshape(-1, *r.shape[-2:])
     if r.shape[-1] == 1:
         r = r @ np.conj(np.swapaxes(r, -1, -2))
-    Es = [E.to_numpy() for E in exp_ops] if exp_ops is not None else None
     t0 = None if options.t0 is None else float(options.t0)
     if isinstance(method, ADAPTIVE) and options.assume_hermitian:
         for x in r:
@@ -536,23 +617,12 @@ def _mesolve_dia(route, rho0: QArray, tsave, exp_ops, method, gradient, options)
         if np.any(res['status'] == -4):
             raise RuntimeError('The solver stopped on a non-finite error estimate or a vanishing step size.')
     states = combine_parts(res['states'], pindex)
-    extra = None
-    if options.save_extra is not None:
-        extra = save_extra_outputs(options.save_extra, states, rho0.dims, out_shape)
-    states = states.reshape(*out_shape, -1, n, n)
-    if not options.save_states:  # the final state only, nsave = 1 (save_mixin.py:62-66, mesolve.py:184-187)
-        states = states[..., -1:, :, :]
-    ysave = QArray(states, dims=rho0.dims)
-    Esave = None
-    if Es is not None:
-        Esave = combine_parts(res['expects'], pindex).reshape(*out_shape, len(Es), len(tsave))
-
-    def shaped(x):
-        x = np.asarray(x)[pindex[:, 0]].reshape(out_shape)
-        return x if out_shape else x[()]
-
+    m = len(pindex)
+    states = states.reshape(m, -1, n, n)
+    ex = combine_parts(res['expects'], pindex).reshape(m, len(Es), len(tsave)) if Es is not None else None
     if isinstance(method, ADAPTIVE):
-        infos = AdaptiveStepInfos(shaped(res['nsteps']), shaped(res['naccepted']), shaped(res['nrejected']))
+        inf = np.stack([np.asarray(res[k])[pindex[:, 0]] for k in ('nsteps', 'naccepted', 'nrejected')])
     else:
-        infos = FixedStepInfos(shaped(np.full(len(pindex), plan.nsteps, dtype=np.int64)))
-    return MESolveResult(tsave, method, gradient, options, ysave, Esave, extra, infos)
+        inf = np.zeros((3, m), dtype=np.int64)
+        inf[0] = plan.nsteps
+    return states, ex, inf, plan

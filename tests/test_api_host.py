@@ -404,3 +404,34 @@ def test_timecallable_low_rank_rewrite():
     for t in np.random.default_rng(3).uniform(-1.0, 2.0, 20):
         np.testing.assert_allclose(R(t).to_numpy(), f(t).to_numpy() + 0.5 * np.eye(4), atol=1e-12)
     assert isinstance(resolve_callables(dq.timecallable(lambda t: dq.sigmaz()), 0.0, 1.0), ConstantTimeQArray)
+
+
+def test_dia_batched_operator_groups():
+    """Batched DIA operators above the dense size run one DIA plan per distinct
+    operator set: the grouping follows the reference's cartesian and flat batching
+    rules (mesolve.py:307-336) and covers every output position once."""
+    from dynamiqs_amd.integrators.mesolve import _dia_groups
+    from dynamiqs_amd.integrators.operators import decompose
+    from dynamiqs_amd.time_qarray import astimeqarray
+
+    n = 6
+    a = dq.destroy(n)
+    eps, kap = np.array([0.1, 0.3]), np.array([0.05, 0.2, 0.4])
+    H = a.dag() @ a + eps[:, None, None] * (a + a.dag())
+    L = np.sqrt(kap)[:, None, None] * a
+    rho = dq.asqarray(np.stack([dq.coherent(n, 0.5).to_numpy(), dq.coherent(n, 1.0).to_numpy()]))
+    Ht, Lts = decompose(astimeqarray(H)), [decompose(astimeqarray(L))]
+    ad = a.to_numpy()
+    out, groups = _dia_groups(Ht, Lts, rho, True)
+    assert out == (2, 3, 2) and len(groups) == 6
+    assert sorted(p for _, m in groups for p, _ in m) == list(range(12))
+    for (Hd, Ld), members in groups:
+        for pos, ir in members:
+            i, j, k = np.unravel_index(pos, out)
+            assert ir == (k,)
+            np.testing.assert_allclose(Hd, ad.conj().T @ ad + eps[i] * (ad + ad.conj().T), atol=1e-6)
+            np.testing.assert_allclose(Ld[0], np.sqrt(kap[j]) * ad, atol=1e-6)
+    H2 = a.dag() @ a + np.array([0.1, 0.3, 0.5])[:, None, None] * (a + a.dag())
+    out, groups = _dia_groups(decompose(astimeqarray(H2)), Lts, dq.coherent(n, 1.0), False)
+    assert out == (3,) and len(groups) == 3
+    assert all(len(m) == 1 and m[0][1] == () for _, m in groups)

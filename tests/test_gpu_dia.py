@@ -358,3 +358,41 @@ def test_dia_rouchon1_long_run_is_stable_and_matches_dense(normalisation):
     ref = dplan.results()['states'][0]
     err = np.linalg.norm(st - ref, axis=(-2, -1)).max()
     assert err < 1e-5, err
+
+
+@pytest.mark.parametrize('method', ['tsit5', 'rouchon1'])
+def test_batched_dia_operators_above_the_dense_limit(method):
+    """Batched constant DIA operators at n = 1100, beyond the dense kernels' 1024: one
+    DIA plan per distinct operator set. Every element of the batch equals the same
+    problem solved alone through the shared-operator DIA path (pinned against the
+    oracle by the tests above), with cartesian and flat batching; shapes and infos
+    follow the reference's batching rules (mesolve.py:307-336)."""
+    import dynamiqs_amd as dq
+
+    n = 1100
+    a = dq.destroy(n)
+    eps = np.array([0.1, 0.3])
+    kap = np.array([0.05, 0.2])
+    H = a.dag() @ a + eps[:, None, None] * (a + a.dag())
+    L = np.sqrt(kap)[:, None, None] * a
+    psi = dq.coherent(n, 1.0)
+    tsave = np.linspace(0.0, 0.3, 4)
+    m = dq.method.Tsit5() if method == 'tsit5' else dq.method.Rouchon1(dt=0.01)
+    E = [dq.number(n)]
+    print(f'{method}: batched DIA n={n}, cartesian', flush=True)
+    r = dq.mesolve(H, [L], psi, tsave, exp_ops=E, method=m)
+    assert r.states.shape == (2, 2, 4, n, n) and r.expects.shape == (2, 2, 1, 4)
+    st = r.states.to_numpy()
+    for i in range(2):
+        for j in range(2):
+            print(f'{method}: alone eps={eps[i]} kappa={kap[j]}', flush=True)
+            ri = dq.mesolve(a.dag() @ a + eps[i] * (a + a.dag()), [np.sqrt(kap[j]) * a], psi, tsave, exp_ops=E, method=m)
+            assert np.max(np.abs(st[i, j] - ri.states.to_numpy())) <= 1e-7, (i, j)
+            np.testing.assert_allclose(r.expects[i, j], ri.expects, atol=1e-6)
+            if method == 'tsit5':
+                assert int(r.infos.nsteps[i, j]) == int(ri.infos.nsteps)
+    print(f'{method}: batched DIA n={n}, flat', flush=True)
+    rf = dq.mesolve(H, [L], psi, tsave, method=m, options=dq.Options(cartesian_batching=False))
+    assert rf.states.shape == (2, 4, n, n)
+    for i in range(2):
+        assert np.max(np.abs(rf.states.to_numpy()[i] - st[i, i])) <= 1e-7, i
