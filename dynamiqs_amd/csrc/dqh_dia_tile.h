@@ -160,10 +160,12 @@ struct GlobalX {
 // Out-of-range neighbours read as zero, so the diagonal index is only clamped to
 // stay inside the array.
 template <class XA>
-__device__ __forceinline__ float2 stencil(const XA &X, const dqh_dia_ops &ops, bool sand0, int64_t n, int i, int j) {
+__device__ __forceinline__ float2 stencil(const XA &X, const dqh_dia_ops &ops, bool sand0, int64_t n, int i, int j,
+                                          const float2 *D0 = nullptr) {
   float ar = 0.f, ai = 0.f;
   {
-    const float2 *D = cgp<float2>(ops.diags[0]);
+    // D0: the diagonals of operator 0 at the current time (time-dependent generators)
+    const float2 *D = D0 ? D0 : cgp<float2>(ops.diags[0]);
     const int no = ops.noff[0];
     for (int u = 0; u < no; ++u) {
       const int p = ops.off[0][u];

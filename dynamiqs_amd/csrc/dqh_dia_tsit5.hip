@@ -53,17 +53,62 @@ struct V {
   const double *tsave;
   float2 *states;
   int64_t *stats;
+  // time-dependent Hamiltonian terms (npwc + nmod > 0): hcur[inst] holds the
+  // diagonals of Hnh at the current stage time on the noff0 offsets of operator 0
+  int64_t npwc, nseg, nmod, noff0, mod_nint;
+  double mod_t0, mod_dt;
+  const double *ptimes;
+  const double *pvals;
+  const double2 *mcoef;
+  const float2 *H0;
+  const float2 *Hterms;
+  float2 *hcur;
 };
 
 __device__ __forceinline__ bool is_done(const Ctrl &c, double t1) { return c.status != 0 || !(c.t < t1); }
 // step of the attempt described by c: the proposed dt, or exactly the rest of the
 // interval when it would end within the snapping tolerance of t1
-__device__ __forceinline__ double step_h(const Ctrl &c, double t1) { return snaps_to(c.t, c.dt, t1) ? t1 - c.t : c.dt; }
-__device__ __forceinline__ double step_end(const Ctrl &c, double t1) { return snaps_to(c.t, c.dt, t1) ? t1 : c.t + c.dt; }
+// Ctrl::rot holds the slot of k_0 (bits 0-2) and kJump: the last accepted step ended on
+// a PWC breakpoint, so k_0 of the next step is evaluated anew (k_stage with s = 0)
+constexpr int64_t kJump = 8;
+
+// searchsorted(ptimes, t, 'right') - 1: the PWC segment of t (-1 before the first breakpoint)
+__device__ __forceinline__ int seg_of(const V &v, double t) {
+  int lo = 0, hi = (int)v.nseg;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (v.ptimes[mid] <= t)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo - 1;
+}
+// the end time, or the first breakpoint strictly after t when it comes before it
+__device__ __forceinline__ double target_of(const V &v, double t) {
+  if (v.npwc > 0) {
+    const int m = seg_of(v, t) + 1;
+    if (m < v.nseg && v.ptimes[m] < v.t1) return v.ptimes[m];
+  }
+  return v.t1;
+}
+__device__ __forceinline__ double step_h(const V &v, const Ctrl &c) {
+  const double tg = target_of(v, c.t);
+  return snaps_to(c.t, c.dt, tg) ? tg - c.t : c.dt;
+}
+__device__ __forceinline__ double step_end(const V &v, const Ctrl &c) {
+  const double tg = target_of(v, c.t);
+  return snaps_to(c.t, c.dt, tg) ? tg : c.t + c.dt;
+}
+// the attempt ends on a breakpoint before t1
+__device__ __forceinline__ bool ends_on_jump(const V &v, const Ctrl &c) {
+  const double tg = target_of(v, c.t);
+  return tg < v.t1 && snaps_to(c.t, c.dt, tg);
+}
 
 __device__ __forceinline__ float2 *kslot(const V &v, int j, int64_t rot, size_t inst) {
   const size_t nn = (size_t)v.n * v.n;
-  return v.k + ((size_t)((j + rot) % 7) * v.ninst + inst) * nn;
+  return v.k + ((size_t)((j + (rot & 7)) % 7) * v.ninst + inst) * nn;
 }
 __device__ __forceinline__ float2 *ysbuf(const V &v, int which, size_t inst) {
   const size_t nn = (size_t)v.n * v.n;
@@ -101,7 +146,8 @@ __device__ __forceinline__ double sq(double2 v) { return v.x * v.x + v.y * v.y; 
 // (HALO == 0, 256 threads, neighbours read through the caches). fn(i, j, f)
 // receives the vector field f at (i, j).
 template <int HALO, class Fn>
-__device__ __forceinline__ void sweep(const V &v, const dqh_dia_ops &ops, const float2 *in, float2 *sx, Fn fn) {
+__device__ __forceinline__ void sweep(const V &v, const dqh_dia_ops &ops, const float2 *in, float2 *sx, Fn fn,
+                                      const float2 *D0 = nullptr) {
   const int64_t n = v.n;
   if constexpr (HALO > 0) {
     int i0, j0;
@@ -113,13 +159,13 @@ __device__ __forceinline__ void sweep(const V &v, const dqh_dia_ops &ops, const 
     for (int m = 0; m < diat::kRowsPerThread; ++m) {
       const int ii = (threadIdx.x >> 6) + (diat::kThreads / diat::kTJ) * m;
       const int i = i0 + ii, j = j0 + jj;
-      fn(i, j, diat::stencil(diat::LdsX<HALO>::at(sx, ii, jj), ops, false, n, i, j));
+      fn(i, j, diat::stencil(diat::LdsX<HALO>::at(sx, ii, jj), ops, false, n, i, j, D0));
     }
   } else {
     int i, j;
     if (diat::row_elem(n, i, j)) {
       const diat::GlobalX X{in, nullptr, n, i, j};
-      fn(i, j, diat::stencil(X, ops, false, n, i, j));
+      fn(i, j, diat::stencil(X, ops, false, n, i, j, D0));
     }
   }
 }
@@ -143,7 +189,9 @@ __global__ __launch_bounds__(nthreads<HALO>()) void k_stage(V v, dqh_dia_ops ops
   const size_t inst = blockIdx.y;
   const Ctrl c = v.ctrl[(size_t)cur * v.ninst + inst];
   if (is_done(c, v.t1)) return;
-  const double h = step_h(c, v.t1);
+  // s = 0: k_0 = f(y) after a step that ended on a breakpoint (input: y in ys buffer 1)
+  if (s == 0 && !(c.rot & kJump)) return;
+  const double h = step_h(v, c);
   const int64_t n = v.n;
   const size_t nn = (size_t)n * n;
   const double2 *y = v.y + inst * nn;
@@ -176,7 +224,7 @@ __global__ __launch_bounds__(nthreads<HALO>()) void k_stage(V v, dqh_dia_ops ops
       const double sc = v.atol + v.rtol * fmax(sqrt(sq(yv)), sqrt(yr * yr + yi * yi));
       acc += (er * er + ei * ei) / (sc * sc);
     }
-  });
+  }, v.hcur ? v.hcur + inst * (size_t)v.noff0 * n : nullptr);
   if (s == 6) {
     const double tot = block_sum<NT>(acc, sh);
     if (threadIdx.x == 0) partial(v, inst, 0)[blockIdx.x] = tot;
@@ -195,8 +243,9 @@ __global__ __launch_bounds__(256) void k_commit(V v, int cur) {
     if (writer) *nx = c;
     return;
   }
-  const double h = step_h(c, v.t1);
-  const double t_new = step_end(c, v.t1);
+  const double h = step_h(v, c);
+  const double t_new = step_end(v, c);
+  const bool jump = ends_on_jump(v, c);
   const double norm = sqrt(reduce_partials<256>(partial(v, inst, 0), v.npart, sh) * v.inv_nn);
   const bool finite = isfinite(norm);
   const bool keep = finite && norm < 1.0;
@@ -214,18 +263,19 @@ __global__ __launch_bounds__(256) void k_commit(V v, int cur) {
   if (keep) {
     o.nacc += 1;
     o.t = t_new;
-    o.rot = (c.rot + 6) % 7;  // FSAL: k_6 of this step is k_0 of the next
+    o.rot = ((c.rot & 7) + 6) % 7 + (jump ? kJump : 0);  // FSAL: k_6 of this step is k_0 of the next
     while (is_end < v.nsave && v.tsave[is_end] <= t_new) ++is_end;
     o.isave = is_end;
   } else {
     o.nrej += 1;
+    o.rot = c.rot & 7;
   }
   o.dt = factor * h;
   if (o.t < v.t1) {
     if (o.nsteps >= v.max_steps) {
       o.status = DQH_MAX_STEPS;
     } else {
-      const double hn = step_h(o, v.t1);
+      const double hn = step_h(v, o);
       if (!(hn > 0.0) || !(hn > 1e-14 * fmax(1.0, fabs(o.t)))) o.status = DQH_NONFINITE;
     }
   }
@@ -262,9 +312,11 @@ __global__ __launch_bounds__(256) void k_commit(V v, int cur) {
       }
       *yp = y1;
       k0n = kk[6];
+      // on a breakpoint, the next step evaluates k_0 = f(y1) with the new segment's H
+      if (jump) ysbuf(v, 1, inst)[e] = make_float2((float)y1.x, (float)y1.y);
     }
     if (!is_done(o, v.t1)) {
-      const double hn = step_h(o, v.t1);
+      const double hn = step_h(v, o);
       ysbuf(v, 0, inst)[e] = make_float2((float)(y1.x + hn * rk.A[1][0] * k0n.x), (float)(y1.y + hn * rk.A[1][0] * k0n.y));
     }
   }
@@ -345,7 +397,7 @@ __global__ __launch_bounds__(nthreads<HALO>()) void k_init_rhs(V v, dqh_dia_ops 
       const double dr = (double)f.x - kq.x, di = (double)f.y - kq.y;
       a0 += (dr * dr + di * di) / (sc * sc);
     }
-  });
+  }, v.hcur ? v.hcur + inst * (size_t)v.noff0 * n : nullptr);
   const double t0s = block_sum<NT>(a0, sh);
   const double t1s = block_sum<NT>(a1, sh);
   if (threadIdx.x == 0) {
@@ -394,7 +446,7 @@ __global__ __launch_bounds__(256) void k_init_dt(V v) {
   const double h1 = md <= 1e-15 ? fmax(1e-6, h0 * 1e-3) : pow(0.01 / md, kInitExp);
   Ctrl o = c;
   o.dt = fmin(100.0 * h0, h1);
-  const double hn = step_h(o, v.t1);
+  const double hn = step_h(v, o);
   if (!(hn > 0.0) || !(hn > 1e-14 * fmax(1.0, fabs(o.t)))) o.status = DQH_NONFINITE;
   const int64_t n = v.n;
   const size_t nn = (size_t)n * n;
@@ -406,6 +458,64 @@ __global__ __launch_bounds__(256) void k_init_dt(V v) {
     ysbuf(v, 0, inst)[e] = make_float2((float)(yv.x + hn * rk.A[1][0] * kq.x), (float)(yv.y + hn * rk.A[1][0] * kq.y));
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) v.ctrl[inst] = o;
+}
+
+// f_j(tau) of modulated table j (Horner on the interval that contains tau)
+__device__ __forceinline__ double2 mod_at(const V &v, int j, double tau) {
+  int64_t i = (int64_t)floor((tau - v.mod_t0) / v.mod_dt);
+  i = i < 0 ? 0 : (i >= v.mod_nint ? v.mod_nint - 1 : i);
+  const double x = tau - (v.mod_t0 + (double)i * v.mod_dt);
+  const double2 *q = v.mcoef + ((size_t)j * v.mod_nint + i) * 4;
+  double re = q[0].x, im = q[0].y;
+#pragma unroll
+  for (int m = 1; m < 4; ++m) {
+    re = re * x + q[m].x;
+    im = im * x + q[m].y;
+  }
+  return make_double2(re, im);
+}
+
+// hcur <- diagonals of Hnh(tau) = H0 + sum_j c_j Hterms_j per instance, PWC values
+// on the segment of the step start. mode 0..6: stage mode of the attempt in control
+// block cur (0: the k_0 refresh after a breakpoint, only then); 7: t0 (k_0 of the
+// initial step); 8: t0 + h0 (the initial step's probe, h0 parked in block 1).
+// grid (ceil(noff0 n / 256), ninst)
+__global__ __launch_bounds__(256) void k_hupd(V v, int cur, int mode) {
+  const RKTab &rk = rk_tab(v.flags);
+  const size_t inst = blockIdx.y;
+  double tseg, tau;
+  if (mode <= 6) {
+    const Ctrl c = v.ctrl[(size_t)cur * v.ninst + inst];
+    if (is_done(c, v.t1) || (mode == 0 && !(c.rot & kJump))) return;
+    tseg = c.t;
+    tau = c.t + rk.C[mode] * step_h(v, c);
+  } else {
+    const Ctrl c = v.ctrl[(mode == 7 ? 0 : v.ninst) + inst];
+    if (is_done(c, v.t1)) return;
+    tseg = c.t;
+    tau = mode == 7 ? c.t : c.t + c.dt;
+  }
+  const size_t m = (size_t)v.noff0 * v.n;
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= m) return;
+  const int nv = (int)v.nseg - 1;
+  const int seg = v.npwc > 0 ? seg_of(v, tseg) : -1;
+  const float2 h0 = v.H0[e];
+  double re = h0.x, im = h0.y;
+  if (seg >= 0 && seg < nv)
+    for (int j = 0; j < (int)v.npwc; ++j) {
+      const double cr = v.pvals[2 * ((size_t)j * nv + seg)], ci = v.pvals[2 * ((size_t)j * nv + seg) + 1];
+      const float2 o = v.Hterms[(size_t)j * m + e];
+      re += cr * o.x - ci * o.y;
+      im += cr * o.y + ci * o.x;
+    }
+  for (int j = 0; j < (int)v.nmod; ++j) {
+    const double2 f = mod_at(v, j, tau);
+    const float2 o = v.Hterms[(size_t)(v.npwc + j) * m + e];
+    re += f.x * o.x - f.y * o.y;
+    im += f.x * o.y + f.y * o.x;
+  }
+  v.hcur[inst * m + e] = make_float2((float)re, (float)im);
 }
 
 // saves not reached (an instance that stopped early keeps its last state) and stats
@@ -464,19 +574,34 @@ extern "C" int64_t dqh_dia_tsit5_work_bytes(int64_t n, int64_t ninst) {
 
 namespace {
 
+// grid of k_hupd: the noff0 x n diagonal elements per instance
+inline dim3 hupd_grid(const diats::V &v) {
+  return dim3((unsigned)(((size_t)v.noff0 * v.n + 255) / 256), (unsigned)v.ninst);
+}
+
 template <int HALO>
 void launch_step(const diats::V &v, const dqh_dia_ops &ops, dim3 gstage, dim3 gflat, int cur, hipStream_t s) {
-  for (int sg = 1; sg <= 6; ++sg)
+  const bool td = v.hcur != nullptr;
+  if (v.npwc > 0) {  // k_0 anew after a breakpoint (instances that did not land on one return at once)
+    hipLaunchKernelGGL(diats::k_hupd, hupd_grid(v), dim3(256), 0, s, v, cur, 0);
+    hipLaunchKernelGGL(diats::k_stage<HALO>, gstage, dim3(diats::nthreads<HALO>()), 0, s, v, ops, 0, cur);
+  }
+  for (int sg = 1; sg <= 6; ++sg) {
+    if (td) hipLaunchKernelGGL(diats::k_hupd, hupd_grid(v), dim3(256), 0, s, v, cur, sg);
     hipLaunchKernelGGL(diats::k_stage<HALO>, gstage, dim3(diats::nthreads<HALO>()), 0, s, v, ops, sg, cur);
+  }
   hipLaunchKernelGGL(diats::k_commit, gflat, dim3(256), 0, s, v, cur);
 }
 
 template <int HALO>
 int drive(const dqh_dia_tsit5 &p, const diats::V &v, hipStream_t st, dim3 gstage, dim3 gflat) {
   const dqh_dia_ops &ops = p.ops;
+  const bool td = v.hcur != nullptr;
   hipLaunchKernelGGL(diats::k_init, gflat, dim3(256), 0, st, v);
+  if (td) hipLaunchKernelGGL(diats::k_hupd, hupd_grid(v), dim3(256), 0, st, v, 0, 7);
   hipLaunchKernelGGL(diats::k_init_rhs<HALO>, gstage, dim3(diats::nthreads<HALO>()), 0, st, v, ops, 0);
   hipLaunchKernelGGL(diats::k_init_h0, gflat, dim3(256), 0, st, v);
+  if (td) hipLaunchKernelGGL(diats::k_hupd, hupd_grid(v), dim3(256), 0, st, v, 0, 8);
   hipLaunchKernelGGL(diats::k_init_rhs<HALO>, gstage, dim3(diats::nthreads<HALO>()), 0, st, v, ops, 1);
   hipLaunchKernelGGL(diats::k_init_dt, gflat, dim3(256), 0, st, v);
   hipError_t err = hipGetLastError();
@@ -584,6 +709,14 @@ extern "C" int dqh_mesolve_tsit5_dia(const dqh_dia_tsit5 *pb, void *stream) {
   }
   if (p.variant == DQH_SWEEP_SPLIT)
     return fail(DQH_BAD_ARG, "dqh_mesolve_tsit5_dia: the split sweep applies to the Rouchon1 Kraus map only");
+  if (p.npwc < 0 || p.nmod < 0 || p.npwc + p.nmod > 64)
+    return fail(DQH_BAD_ARG, "dqh_mesolve_tsit5_dia: 0 <= npwc, nmod and npwc + nmod <= 64");
+  if (p.npwc + p.nmod > 0 && (!p.Hterms || !p.hcur))
+    return fail(DQH_BAD_ARG, "dqh_mesolve_tsit5_dia: time-dependent terms need Hterms and hcur");
+  if (p.npwc > 0 && (p.nseg < 2 || !p.pwc_times || !p.pwc_vals))
+    return fail(DQH_BAD_ARG, "dqh_mesolve_tsit5_dia: PWC terms need nseg >= 2 breakpoints and their values");
+  if (p.nmod > 0 && (!p.mod_coef || p.mod_nint < 1 || !(p.mod_dt > 0)))
+    return fail(DQH_BAD_ARG, "dqh_mesolve_tsit5_dia: modulated terms need their tables");
   const int halo = diat::resolve_halo(p.variant, p.n, diat::max_abs_offset(p.ops));
   if (halo < 0)
     return fail(DQH_BAD_ARG, "dqh_mesolve_tsit5_dia: the tiled sweep needs n % 64 == 0 and every |offset| <= its halo");
@@ -614,6 +747,20 @@ extern "C" int dqh_mesolve_tsit5_dia(const dqh_dia_tsit5 *pb, void *stream) {
   v.tsave = reinterpret_cast<const double *>(p.tsave);
   v.states = reinterpret_cast<float2 *>(p.states);
   v.stats = reinterpret_cast<int64_t *>(p.stats);
+  const bool td = p.npwc + p.nmod > 0;
+  v.npwc = td ? p.npwc : 0;
+  v.nseg = p.npwc > 0 ? p.nseg : 0;
+  v.nmod = td ? p.nmod : 0;
+  v.noff0 = p.ops.noff[0];
+  v.mod_nint = p.mod_nint;
+  v.mod_t0 = p.mod_t0;
+  v.mod_dt = p.mod_dt;
+  v.ptimes = reinterpret_cast<const double *>(p.pwc_times);
+  v.pvals = reinterpret_cast<const double *>(p.pwc_vals);
+  v.mcoef = reinterpret_cast<const double2 *>(p.mod_coef);
+  v.H0 = reinterpret_cast<const float2 *>(p.ops.diags[0]);
+  v.Hterms = td ? reinterpret_cast<const float2 *>(p.Hterms) : nullptr;
+  v.hcur = td ? reinterpret_cast<float2 *>(p.hcur) : nullptr;
   const dim3 gflat = diat::row_grid(p.n, p.ninst);
   hipStream_t st = (hipStream_t)stream;
   const dim3 gtile((unsigned)((p.n / diat::kTI) * (p.n / diat::kTJ)), (unsigned)p.ninst);

@@ -452,7 +452,31 @@ class DiaTsit5(ctypes.Structure):
         ('expects', ctypes.c_uint64),
         ('stats', ctypes.c_uint64),
         ('host_flags', ctypes.c_uint64),
+        ('npwc', ctypes.c_int64),
+        ('nseg', ctypes.c_int64),
+        ('nmod', ctypes.c_int64),
+        ('Hterms', ctypes.c_uint64),
+        ('pwc_times', ctypes.c_uint64),
+        ('pwc_vals', ctypes.c_uint64),
+        ('mod_coef', ctypes.c_uint64),
+        ('mod_t0', ctypes.c_double),
+        ('mod_dt', ctypes.c_double),
+        ('mod_nint', ctypes.c_int64),
+        ('hcur', ctypes.c_uint64),
     ]
+
+
+def _diags_on(x: np.ndarray, offs) -> np.ndarray:
+    """The diagonals of ``x`` at the offsets ``offs`` in the column-indexed DIA layout
+    (``diags[k, j] = x[j - offs[k], j]``, zero outside the matrix)."""
+    n = x.shape[-1]
+    d = np.zeros((len(offs), n), dtype=np.complex128)
+    for i, k in enumerate(offs):
+        if k >= 0:
+            d[i, k:] = np.diagonal(x, offset=k)
+        else:
+            d[i, : n + k] = np.diagonal(x, offset=k)
+    return d
 
 
 class DiaTsit5Plan:
@@ -511,10 +535,14 @@ def tsit5_plan(
     exp_ops=None,
     variant: int = SWEEP_AUTO,
     graph_steps: int = 16,
+    pwc=None,
+    mod=None,
 ) -> DiaTsit5Plan:
     """Pack a batch of Tsit5 solves that share H and Ls (dense FP64 host arrays whose
     nonzeros lie on few diagonals) and differ in rho0 (B, n, n). ``tableau='dopri5'``
-    integrates with the Dopri5 pair instead."""
+    integrates with the Dopri5 pair instead. Time-dependent Hamiltonian terms, shared
+    by the batch: ``pwc`` [(times, values (nv,), O)] (``dq.pwc``) and ``mod`` [(f, O)]
+    (``dq.modulated``, f(t) scalar), with O on few diagonals: H(t) = H + sum c_j(t) O_j."""
     import torch
 
     dev = _device()
@@ -530,7 +558,10 @@ def tsit5_plan(
     if variant == SWEEP_AUTO:
         from .. import autotune
 
-        variant = autotune.dia_sweep(H, Ls, B)
+        # the sweep form must admit every offset of the time-dependent terms too
+        Hpat = H + sum((np.abs(np.asarray(O)) for O in [o for (_, _, o) in (pwc or [])] + [o for (_, o) in (mod or [])]),
+                       np.zeros(H.shape))
+        variant = autotune.dia_sweep(Hpat, Ls, B)
     arena = _Arena(dev)
     prob = DiaTsit5()
     prob.n, prob.n_true, prob.ninst, prob.nsave = n_pad, n, B, len(tsave)
@@ -540,7 +571,38 @@ def tsit5_plan(
     prob.t0 = float(tsave[0] if t0 is None else t0)
     prob.t1 = float(tsave[-1])
     prob.flags = _tableau_flag(tableau)
-    _fill_ops(prob.ops, [_dia_of(Hnh)] + [_dia_of(L) for L in Ls], n_pad, arena)
+    pwc, mod = list(pwc or []), list(mod or [])
+    terms = [O for (_, _, O) in pwc] + [O for (_, O) in mod]
+    if terms:
+        # operator 0 on the union of the offsets of Hnh and of every term
+        offs0 = sorted(set(_dia_of(Hnh)[0]).union(*[set(_dia_of(O)[0]) for O in terms])) or [0]
+        if len(offs0) > MAXOFF:
+            raise NotImplementedError(f'DIA sweeps support at most {MAXOFF} diagonals per operator')
+        _fill_ops(prob.ops, [(offs0, _diags_on(Hnh, offs0))] + [_dia_of(L) for L in Ls], n_pad, arena)
+        Ht = np.zeros((len(terms), len(offs0), n_pad), dtype=np.complex64)
+        for j, O in enumerate(terms):
+            Ht[j, :, :n] = _diags_on(-1j * np.asarray(O, dtype=np.complex128), offs0)
+        prob.Hterms = arena.put_now(Ht)
+        prob.hcur = arena.put_now(np.zeros((B, len(offs0), n_pad), dtype=np.complex64))
+        prob.npwc, prob.nmod = len(pwc), len(mod)
+        if pwc:
+            from .dense import pwc_rows
+
+            grid, rows = pwc_rows([(t, v, None) for (t, v, _) in pwc], [])
+            prob.nseg = len(grid)
+            prob.pwc_times = arena.put_now(grid)
+            prob.pwc_vals = arena.put_now(np.ascontiguousarray(np.stack(rows)).view(np.float64))
+        if mod:
+            from .operators import spline_table
+
+            tabs = [spline_table(f, prob.t0, prob.t1) for (f, _) in mod]
+            nint = max(len(c) for (_, _, c) in tabs)
+            tabs = [spline_table(f, prob.t0, prob.t1, n0=nint, nmax=nint) if len(tab[2]) != nint else tab
+                    for (f, _), tab in zip(mod, tabs)]
+            prob.mod_t0, prob.mod_dt, prob.mod_nint = tabs[0][0], tabs[0][1], nint
+            prob.mod_coef = arena.put_now(np.ascontiguousarray(np.stack([c for (_, _, c) in tabs])).view(np.float64))
+    else:
+        _fill_ops(prob.ops, [_dia_of(Hnh)] + [_dia_of(L) for L in Ls], n_pad, arena)
     r64 = np.zeros((B, n_pad, n_pad), dtype=np.complex128)
     r64[:, :n, :n] = rho0
     y0 = torch.from_numpy(r64.view(np.float64)).to(dev)

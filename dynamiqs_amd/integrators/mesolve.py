@@ -102,7 +102,15 @@ def dia_route(method, H: TimeQArray, Ls, n: int):
         return None
     Ht = decompose(H)
     Lts = [decompose(L) for L in Ls]
-    if Ht.dia is None or any(Lt.dia is None for Lt in Lts):
+    if any(Lt.dia is None for Lt in Lts):
+        return None
+    if Ht.dia is None:
+        # PWC / modulated Hamiltonian terms on the DIA Tsit5 sweeps (the per-stage
+        # generator rebuild of dqh_mesolve_tsit5_dia) where the dense kernels cannot take
+        # the size: un-batched operators, constant jump operators
+        if (n > dense.MAX_NP and isinstance(method, ADAPTIVE) and (Ht.pwc or Ht.mod) and Ht.batch == ()
+                and all(Lt.batch == () for Lt in Lts)):
+            return 'timedep', Ht, Lts
         return None
     if Ht.batch == () and all(Lt.batch == () for Lt in Lts):
         return Ht.const_at(()), [Lt.const_at(()) for Lt in Lts]
@@ -534,8 +542,15 @@ def _mesolve_dia(route, rho0: QArray, tsave, exp_ops, method, gradient, options)
     distinct operator set (each launch is grid-wide over the n^2 elements)."""
     r_all = rho0.to_numpy().astype(np.complex128)
     n = r_all.shape[-2]
+    terms = {}
     if isinstance(route[0], str) and route[0] == 'batched':
         out_shape, groups = _dia_groups(route[1], route[2], rho0, options.cartesian_batching)
+    elif isinstance(route[0], str) and route[0] == 'timedep':
+        Ht, Lts = route[1], route[2]
+        terms = dict(pwc=Ht.pwc_at(()), mod=[(g, o) for (g, o, _) in Ht.mod_at(())])
+        out_shape = r_all.shape[:-2]
+        groups = [((Ht.const_at(()), [Lt.const_at(()) for Lt in Lts]),
+                   [(pos, idx) for pos, idx in enumerate(np.ndindex(*out_shape) if out_shape else [()])])]
     else:
         out_shape = r_all.shape[:-2]
         rb = out_shape
@@ -548,7 +563,7 @@ def _mesolve_dia(route, rho0: QArray, tsave, exp_ops, method, gradient, options)
     plan = None
     for (Hd, Ld), members in groups:
         r = np.stack([r_all[ir] for (_, ir) in members])
-        st, ex, inf, plan = _dia_solve(Hd, Ld, r, tsave, Es, method, options)
+        st, ex, inf, plan = _dia_solve(Hd, Ld, r, tsave, Es, method, options, **terms)
         pos = np.array([p for (p, _) in members])
         if states_all is None:
             states_all = np.zeros((nout, *st.shape[1:]), dtype=st.dtype)
@@ -579,7 +594,7 @@ def _mesolve_dia(route, rho0: QArray, tsave, exp_ops, method, gradient, options)
     return MESolveResult(tsave, method, gradient, options, ysave, Esave, extra, infos)
 
 
-def _dia_solve(Hd, Ld, r: np.ndarray, tsave, Es, method, options):
+def _dia_solve(Hd, Ld, r: np.ndarray, tsave, Es, method, options, pwc=None, mod=None):
     """One DIA plan over the initial states ``r`` (m, n, n|1) with shared operators:
     (states (m, nsave, n, n), expects (m, nE, ntsave) or None, infos (3, m), plan)."""
     n = r.shape[-2]
@@ -606,6 +621,8 @@ def _dia_solve(Hd, Ld, r: np.ndarray, tsave, Es, method, options):
             max_steps=method.max_steps,
             exp_ops=Es,
             tableau='dopri5' if isinstance(method, Dopri5) else 'tsit5',
+            pwc=pwc,
+            mod=mod,
         )
     else:
         plan = dia_path.build_plan(Hd, Ld, r, tsave, method.dt, t0=t0, exp_ops=Es)

@@ -435,6 +435,25 @@ typedef struct dqh_dia_tsit5 {
   uint64_t expects;     /* device c64[ninst][nE][nsave] */
   uint64_t stats;       /* device int64[ninst][8]: nsteps, naccepted, nrejected, status, nrhs */
   uint64_t host_flags;  /* HOST pinned int64[2]: completion counts read back by the host loop */
+  /* optional time-dependent Hamiltonian terms (npwc + nmod = 0: a constant H).
+   * H(t) = H0 + sum_j c_j(t) O_j: before every stage the kernels rebuild the diagonals
+   * of Hnh(t) = ops.diags[0] + sum_j c_j(t) Hterms[j] per instance into hcur (each
+   * term holds -i O_j on the noff[0] offsets of operator 0). PWC coefficients
+   * (time_qarray.py:572-666) take their value on the segment of the step start
+   * (zero outside [pwc_times[0], pwc_times[nseg-1])), and steps land on the
+   * breakpoints, after which the first stage is evaluated anew. Modulated
+   * coefficients come from piecewise-cubic tables laid out as in dqh_me_inst,
+   * evaluated at the stage time. */
+  int64_t npwc;         /* PWC terms (first in Hterms) */
+  int64_t nseg;         /* breakpoints in pwc_times */
+  int64_t nmod;         /* modulated terms (after the PWC ones in Hterms) */
+  uint64_t Hterms;      /* device c64[npwc + nmod][noff[0]][n] */
+  uint64_t pwc_times;   /* device double[nseg], increasing */
+  uint64_t pwc_vals;    /* device double[npwc][nseg - 1][2]: complex coefficient per segment */
+  uint64_t mod_coef;    /* device double2[nmod][mod_nint][4] */
+  double mod_t0, mod_dt;
+  int64_t mod_nint;
+  uint64_t hcur;        /* device c64[ninst][noff[0]][n] scratch */
 } dqh_dia_tsit5;
 
 int64_t dqh_sizeof_dia_tsit5(void);
