@@ -5,8 +5,8 @@
 ``dq.sepropagator`` / ``dq.mepropagator``, quantum arrays
 (dense and sparse DIA), time-dependent operators (``constant``, ``pwc``,
 ``modulated``, ``timecallable``), operator/state builders, ``Options``, the
-methods ``Tsit5`` / ``Dopri5`` / ``Euler`` / ``Rouchon1`` / ``Rouchon2`` /
-``Rouchon3`` / ``Expm`` and the result classes.
+methods ``Tsit5`` / ``Dopri5`` / ``Dopri8`` / ``Euler`` / ``Rouchon1`` / ``Rouchon2`` /
+``Rouchon3`` (fixed step, or adaptive without ``dt``) / ``Expm`` and the result classes.
 The solve runs in hand-written HIP kernels (``libdqh.so``, C-ABI in
 ``include/dqh.h``) loaded with ctypes; there is no CPU fallback.
 """

@@ -1,8 +1,8 @@
 """Integration methods, mirroring ``dynamiqs/method.py``.
 
 ``Tsit5`` (:438-476, defaults of ``_DEAdaptiveStep`` :121-127), ``Dopri5``,
-``Dopri8`` (mesolve), ``Euler``, ``Rouchon1`` (:207-237), the fixed-step
-``Rouchon2`` / ``Rouchon3`` and ``Expm`` run on the GPU. The other method names of the reference exist so that the
+``Dopri8`` (mesolve), ``Euler``, ``Rouchon1`` (:207-237), ``Rouchon2`` / ``Rouchon3``
+(fixed step with ``dt``, the adaptive pairs without it) and ``Expm`` run on the GPU. The other method names of the reference exist so that the
 solvers can raise the reference's ``TypeError`` for the methods they do not
 support.
 """
