@@ -39,10 +39,11 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per batched launch of the resident Tsit5 kernel on the default workload
 # (Kerr n=64, B=256), from two rocprofv3 --pmc runs of this script (FETCH_SIZE,
 # then WRITE_SIZE) scaled by the same counters on a device copy of known size
-# (--config copy: FETCH_SIZE x 2, WRITE_SIZE x 1): 124 576 769 KiB fetched,
-# 232 674 728 KiB written (packed FP64 state writes and register spills).
-KERR_TRAFFIC_BYTES = 124_576_769 * 1024 * 2 + 232_674_728 * 1024
-KERR_TRAFFIC_SOURCE = 'rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this script, build of c347621'
+# (--config copy: FETCH_SIZE x 2, WRITE_SIZE x 1): 258 134 603 KiB fetched,
+# 294 650 707 KiB written (packed FP64 state writes and register spills of the
+# FP64-diagonal body, 292 B/lane of scratch).
+KERR_TRAFFIC_BYTES = 258_134_603 * 1024 * 2 + 294_650_707 * 1024
+KERR_TRAFFIC_SOURCE = 'rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this script, build of 3abd21b'
 
 
 def kerr_instances(B: int, n: int = 64, seed: int = 0):
