@@ -70,6 +70,7 @@ FLAG_LMOD = 32
 FLAG_SIGNAL = 64
 FLAG_DOPRI8 = 128
 FLAG_LPWC = 256
+FLAG_LPAIR = 512
 
 STATUS_OK = 0
 STATUS_MAX_STEPS = -3

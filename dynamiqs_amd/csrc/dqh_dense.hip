@@ -189,18 +189,26 @@ __device__ __forceinline__ void lmod_update(const dqh_me_inst &p, double tau, in
   const float2 *D0 = gp<const float2>(p.Dops), *C0 = gp<const float2>(p.Cops);
   const int nv = (int)p.nseg - 1;
   const bool lpwc = (p.flags & DQH_FLAG_LPWC) != 0;
-  for (int k = 0; k < nL; ++k) {
-    double2 f = mod_eval(p, nmod + k, tau);
+  const bool lpair = (p.flags & DQH_FLAG_LPAIR) != 0;
+  // coefficient q of D (q = 0) or of C (q = 1, DQH_FLAG_LPAIR): table nmod + q nL + k,
+  // times row npwc + q nL + k of pwc_vals (DQH_FLAG_LPWC)
+  auto coef = [&](int q, int k) {
+    double2 f = mod_eval(p, nmod + q * nL + k, tau);
     if (lpwc) {
-      const double *vals = gp<const double>(p.pwc_vals) + 2 * (size_t)(p.npwc + k) * (size_t)(nv > 0 ? nv : 0);
+      const double *vals = gp<const double>(p.pwc_vals) + 2 * (size_t)(p.npwc + q * nL + k) * (size_t)(nv > 0 ? nv : 0);
       const double cr = (seg >= 0 && seg < nv) ? vals[2 * seg] : 0.0;
       const double ci = (seg >= 0 && seg < nv) ? vals[2 * seg + 1] : 0.0;
       f = make_double2(f.x * cr - f.y * ci, f.x * ci + f.y * cr);
     }
+    return f;
+  };
+  for (int k = 0; k < nL; ++k) {
+    const double2 f = coef(0, k);
+    const double2 fc = lpair ? coef(1, k) : f;
     for (size_t e = threadIdx.x; e < nn; e += nthreads) {
       const float2 d = D0[k * nn + e], c = C0[k * nn + e];
       Dc[k * nn + e] = make_float2((float)(f.x * d.x - f.y * d.y), (float)(f.x * d.y + f.y * d.x));
-      Cc[k * nn + e] = make_float2((float)(f.x * c.x + f.y * c.y), (float)(f.x * c.y - f.y * c.x));
+      Cc[k * nn + e] = make_float2((float)(fc.x * c.x + fc.y * c.y), (float)(fc.x * c.y - fc.y * c.x));
     }
   }
 }

@@ -55,6 +55,13 @@ class Instance:
     mod: list = field(default_factory=list)  # [(g, op (n, n), key)]: + g(t) op, g scalar; equal keys share tables
     Lmod: list = field(default_factory=list)  # per jump operator: None (constant) or (g, key): L_k(t) = g(t) Ls[k]
     Lpwc: list = field(default_factory=list)  # per jump operator: None or (times, values): L_k(t) = c(t) Ls[k], c PWC
+    # per jump operator: None or a sum of terms [(coef, op)], coef ('const', v) | ('mod', g, key) |
+    # ('pwc', times, values); expand_jump_sums turns such instances into D/C pairs
+    Lsum: list = field(default_factory=list)
+    Cs: list | None = None  # explicit right factors C_k (default 1/2 L_k^dag)
+    Lbmod: list = field(default_factory=list)  # per D/C pair: the coefficient of C when it differs from D's
+    Lbpwc: list = field(default_factory=list)
+    expanded: bool = False  # the generator terms of the jump operators are in H / mod / pwc already
 
 
 class _Arena:
@@ -364,6 +371,89 @@ def _one(t):
     return np.ones(np.shape(t)) + 0j
 
 
+def _coef_parts(c):
+    """(constant factor, modulated (g, key) or None, PWC (times, values) or None) of a term coefficient."""
+    if c[0] == 'const':
+        return complex(c[1]), None, None
+    if c[0] == 'mod':
+        return 1.0, (c[1], c[2]), None
+    return 1.0, None, (c[1], c[2])
+
+
+def coef_value(c, t: float) -> complex:
+    """A term coefficient at time t (PWC: the reference's prefactor rule)."""
+    if c[0] == 'const':
+        return complex(c[1])
+    if c[0] == 'mod':
+        return complex(c[1](t))
+    return pwc_value(c[1], c[2], t)
+
+
+def expand_jump_sums(inst: Instance) -> Instance:
+    """Rewrite an instance whose jump operators are sums L_k(t) = sum_j c_j(t) L_j
+    (constant, modulated or PWC terms) as D/C pairs for the kernels' per-stage rebuild
+    (DQH_FLAG_LPAIR): (c_j L_j, conj(c_j) L_j^dag / 2) and, for j < l,
+    (c_j L_j, conj(c_l) L_l^dag), which the vector field's tmp + tmp^dag completes to
+    L rho L^dag. The products -1/2 conj(c_j) c_l L_j^dag L_l go into the constant
+    generator, the modulated terms (product functions) or the PWC terms (products on
+    the merged breakpoints). A sum that mixes modulated and PWC terms raises."""
+    from dataclasses import replace
+
+    if inst.Hnh is not None:
+        raise NotImplementedError('sums of jump-operator terms with an explicit generator are not supported')
+    K = len(inst.Ls)
+    per = []
+    for k in range(K):
+        if inst.Lsum and inst.Lsum[k] is not None:
+            per.append(list(inst.Lsum[k]))
+        elif inst.Lmod and inst.Lmod[k] is not None:
+            per.append([(('mod',) + tuple(inst.Lmod[k]), inst.Ls[k])])
+        elif inst.Lpwc and inst.Lpwc[k] is not None:
+            per.append([(('pwc',) + tuple(inst.Lpwc[k]), inst.Ls[k])])
+        else:
+            per.append([(('const', 1.0), inst.Ls[k])])
+    Hc = -1j * np.asarray(inst.H, dtype=np.complex128)
+    mod, pwc = list(inst.mod), list(inst.pwc)
+    D, C, amod, apwc, bmod, bpwc = [], [], [], [], [], []
+    for terms in per:
+        parts = [(_coef_parts(c), np.asarray(L, dtype=np.complex128)) for c, L in terms]
+        if any(m is not None for (_, m, _), _ in parts) and any(q is not None for (_, _, q), _ in parts):
+            raise NotImplementedError('a jump operator that sums modulated and piecewise-constant terms is not supported')
+        J = len(parts)
+        for j in range(J):
+            for l in range(j, J):
+                (vj, mj, pj), Lj = parts[j]
+                (vl, ml, pl), Ll = parts[l]
+                D.append(vj * Lj)
+                C.append(np.conj(vl) * (0.5 if j == l else 1.0) * _dag(Ll))
+                amod.append(mj)
+                apwc.append(pj)
+                bmod.append(ml)
+                bpwc.append(pl)
+        for j in range(J):
+            for l in range(J):
+                (vj, mj, pj), Lj = parts[j]
+                (vl, ml, pl), Ll = parts[l]
+                O = np.conj(vj) * vl * (_dag(Lj) @ Ll)
+                if mj is None and pj is None and ml is None and pl is None:
+                    Hc = Hc - 0.5 * O
+                elif pj is None and pl is None:
+                    # -1/2 conj(g_j) g_l O as a modulated term: -i op with op = -i/2 O
+                    def g(t, mj=mj, ml=ml):
+                        a = np.conj(mj[0](t)) if mj is not None else 1.0
+                        return a * (ml[0](t) if ml is not None else 1.0)
+
+                    mod.append((g, -0.5j * O, None))
+                else:
+                    rows_in = [(q[0], np.conj(q[1]) if side == 0 else q[1], None)
+                               for side, q in ((0, pj), (1, pl)) if q is not None]
+                    grid, rows = pwc_rows(rows_in, [])
+                    vals = rows[0] * (rows[1] if len(rows) > 1 else 1.0)
+                    pwc.append((grid, vals, -0.5j * O))
+    return replace(inst, Ls=D, Cs=C, Hnh=Hc, mod=mod, pwc=pwc, Lmod=amod, Lpwc=apwc, Lbmod=bmod, Lbpwc=bpwc,
+                   Lsum=[], expanded=True)
+
+
 def pwc_value(times, values, t: float) -> complex:
     """values[searchsorted(times, t, 'right') - 1], zero outside [times[0], times[-1])
     (the reference's PWC prefactor, time_qarray.py:649-659)."""
@@ -395,6 +485,8 @@ def _hmods(inst: Instance) -> list:
     -1/2 sum L^dag L, written as |g_k|^2 times O = -i/2 L_k^dag L_k (the kernels add
     -i O times the coefficient)."""
     out = list(inst.mod)
+    if inst.expanded:
+        return out
     for L, lm in zip(inst.Ls, inst.Lmod or []):
         if lm is not None:
             g, key = lm
@@ -458,8 +550,11 @@ def build_plan(
     if tableau not in ('tsit5', 'dopri5', 'dopri8'):
         raise ValueError(f'unknown Runge-Kutta pair {tableau!r}')
     dopri8 = tableau == 'dopri8' and kind == 'tsit5'
-    lpwc_any = any(lp is not None for inst in instances for lp in (inst.Lpwc or []))
-    lmod_any = any(lm is not None for inst in instances for lm in (inst.Lmod or [])) or lpwc_any
+    if kind in ('tsit5', 'rhs'):
+        instances = [expand_jump_sums(i) if any(x is not None for x in (i.Lsum or [])) else i for i in instances]
+    lpwc_any = any(lp is not None for inst in instances for lp in list(inst.Lpwc or []) + list(inst.Lbpwc or []))
+    lmod_any = (any(lm is not None for inst in instances for lm in (inst.Lmod or [])) or lpwc_any
+                or any(inst.expanded for inst in instances))
     if dopri8 and lmod_any:
         raise NotImplementedError('Dopri8 with time-dependent jump operators is not supported.')
     if (any(inst.mod for inst in instances) or lmod_any) and kind != 'rouchon1':
@@ -601,13 +696,17 @@ def build_plan(
             elif tableau == 'dopri8' and kind == 'tsit5':
                 d['flags'] |= _native.FLAG_DOPRI8
             lpwc = [k < len(inst.Lpwc or []) and inst.Lpwc[k] is not None for k in range(K)]
-            lmod = [(k < len(inst.Lmod or []) and inst.Lmod[k] is not None) or lpwc[k] for k in range(K)]
+            lbpwc = [k < len(inst.Lbpwc or []) and inst.Lbpwc[k] is not None for k in range(K)]
+            lmod = [(k < len(inst.Lmod or []) and inst.Lmod[k] is not None) or lpwc[k] or inst.expanded
+                    for k in range(K)]
             if use_global or any(lmod):
                 d['flags'] |= _native.FLAG_GLOBAL
             if any(lmod):
                 d['flags'] |= _native.FLAG_LMOD
-            if any(lpwc):
+            if any(lpwc) or any(lbpwc):
                 d['flags'] |= _native.FLAG_LPWC
+            if inst.expanded:
+                d['flags'] |= _native.FLAG_LPAIR
 
             def bH128(inst=inst, lmod=lmod):
                 if inst.Hnh is not None:
@@ -628,22 +727,31 @@ def build_plan(
                 d['hdiag'] = shared(hkey + ('diag',), lambda bH128=bH128: np.ascontiguousarray(np.diag(bH128())))
             else:
                 d['Hnh'] = shared(hkey, lambda bH128=bH128: bH128().astype(np.complex64))
-            if K:
+            if K and inst.Cs is not None:  # D/C pairs of expanded jump-operator sums
+                d['Dops'] = shared(('Dpairs', inst.key_L), lambda Ls=Ls: _pad(np.stack(Ls), npad).astype(np.complex64))
+                d['Cops'] = shared(('Cpairs', inst.key_L),
+                                   lambda Cs=inst.Cs: _pad(np.stack(Cs), npad).astype(np.complex64))
+            elif K:
                 d['Dops'] = shared(('D', inst.key_L), lambda Ls=Ls: _pad(np.stack(Ls), npad).astype(np.complex64))
                 d['Cops'] = shared(
                     ('C', inst.key_L), lambda Ls=Ls: _pad(0.5 * np.stack([_dag(L) for L in Ls]), npad).astype(np.complex64)
                 )
-            if any(lpwc):
+            if any(lpwc) or any(lbpwc):
                 # PWC jump operators c_k(t) L_k: the terms -1/2 |c_k|^2 L_k^dag L_k join the
                 # PWC Hamiltonian terms (O = -i/2 L^dag L, so that -i O = -1/2 L^dag L), and
                 # rows npwc + k of pwc_vals hold every jump operator's coefficient (1 for
-                # the others) on a grid that covers the whole solve interval
-                jterms = [(inst.Lpwc[k][0], np.abs(np.asarray(inst.Lpwc[k][1], dtype=np.complex128)) ** 2,
-                           -0.5j * (_dag(L) @ L)) for k, L in enumerate(inst.Ls) if lpwc[k]]
+                # the others) on a grid that covers the whole solve interval; expanded sums
+                # carry their generator terms already and add the rows of the C factors
+                jterms = [] if inst.expanded else [
+                    (inst.Lpwc[k][0], np.abs(np.asarray(inst.Lpwc[k][1], dtype=np.complex128)) ** 2,
+                     -0.5j * (_dag(L) @ L)) for k, L in enumerate(inst.Ls) if lpwc[k]]
                 pwc_eff = list(inst.pwc) + jterms
                 span = np.array([t_tab, t_stop])
                 coef = [(inst.Lpwc[k][0], inst.Lpwc[k][1], None) if lpwc[k] else (span, np.ones(1), None)
                         for k in range(K)]
+                if inst.expanded:
+                    coef += [(inst.Lbpwc[k][0], inst.Lbpwc[k][1], None) if lbpwc[k] else (span, np.ones(1), None)
+                             for k in range(K)]
                 grid, rows = pwc_rows(pwc_eff + coef, span)
                 d['npwc'] = len(pwc_eff)
                 d['nseg'] = len(grid)
@@ -674,6 +782,8 @@ def build_plan(
                 if any(lmod):
                     fns += [inst.Lmod[k] if (k < len(inst.Lmod or []) and inst.Lmod[k] is not None)
                             else (_one, ('one',)) for k in range(K)]
+                if inst.expanded:
+                    fns += [inst.Lbmod[k] if inst.Lbmod[k] is not None else (_one, ('one',)) for k in range(K)]
                 tabs = [mod_table(g, key) for (g, key) in fns]
                 nint = max(len(c) for (_, _, c) in tabs)
                 tabs = [spline_table(g, t_tab, t_stop, n0=nint, nmax=nint) if len(c) != nint else (a, dtt, c)
@@ -694,7 +804,9 @@ def build_plan(
             # smooth (modulated) terms make every step's operators distinct: one set per step
             lmods = list(inst.Lmod or [None] * K)
             lpwcs = list(inst.Lpwc or [None] * K)
-            smooth = bool(inst.mod) or any(lm is not None for lm in lmods) or any(lp is not None for lp in lpwcs)
+            lsums = list(inst.Lsum or [None] * K)
+            smooth = (bool(inst.mod) or any(lm is not None for lm in lmods) or any(lp is not None for lp in lpwcs)
+                      or any(ls is not None for ls in lsums))
             key = ('kraus', inst.key_H, inst.key_L)
             cacheable = not any(k is None for k in key[1:])
             keys = []
@@ -741,6 +853,8 @@ def build_plan(
                         Hm = Hm + complex(g(tk)) * op
                     Lm = [L if lm is None else complex(lm[0](tk)) * L for L, lm in zip(inst.Ls, lmods)]
                     Lm = [L if lp is None else pwc_value(lp[0], lp[1], tk) * L for L, lp in zip(Lm, lpwcs)]
+                    Lm = [L if ls is None else sum(coef_value(c, tk) * np.asarray(o, dtype=np.complex128) for c, o in ls)
+                          for L, ls in zip(Lm, lsums)]
                 Dk, Ck = _kraus_sets(Hm, Lm, h, normalize, rouchon_order)
                 sets_D.append(np.stack(Dk))
                 sets_C.append(np.stack(Ck))

@@ -126,13 +126,13 @@ def enumerate_instances(H: TimeQArray, Ls, rho0: QArray, cartesian: bool):
     Ht = decompose(H)
     Lts = [decompose(L) for L in Ls]
     for i, Lt in enumerate(Lts):
-        # a time-dependent jump operator must be one term: modulated f(t) L (its rate
-        # |f|^2 and phase are applied at every stage time) or piecewise constant c(t) L
-        nterms = len(Lt.mod) + len(Lt.pwc) + (Lt.const is not None and bool(np.any(Lt.const)))
-        if (Lt.mod or Lt.pwc) and nterms > 1:
+        # a time-dependent jump operator is one modulated or PWC term, or a sum of
+        # constant and modulated terms, or of constant and PWC terms (expanded into D/C
+        # pairs by the dense plan)
+        if Lt.mod and Lt.pwc:
             raise NotImplementedError(
-                'time-dependent jump operators are supported by the MI355X solver only as one term'
-                ' dq.modulated(f, L) or dq.pwc(times, values, L)'
+                'time-dependent jump operators are supported by the MI355X solver as sums of constant and modulated'
+                ' terms, or of constant and piecewise-constant terms, not both kinds'
             )
     r = rho0.to_numpy().astype(np.complex128)
     rb = r.shape[:-2]
@@ -154,8 +154,23 @@ def enumerate_instances(H: TimeQArray, Ls, rho0: QArray, cartesian: bool):
         rho = r[ir]
         if rho.shape[-1] == 1:
             rho = rho @ rho.conj().T
-        Lmods, Lpwcs, Lops = [], [], []
+        Lmods, Lpwcs, Lsums, Lops = [], [], [], []
         for Lt, i in zip(Lts, iLs):
+            nterms = len(Lt.mod) + len(Lt.pwc) + (Lt.const is not None and bool(np.any(Lt.const)))
+            if (Lt.mod or Lt.pwc) and nterms > 1:
+                terms = []
+                if Lt.const is not None and np.any(Lt.const):
+                    terms.append((('const', 1.0), Lt.const_at(i)))
+                for g, o, f in Lt.mod_at(i):
+                    terms.append((('mod', g, (id(f), i if np.ndim(f(0.0)) else None)), o))
+                for times, vals, o in Lt.pwc_at(i):
+                    terms.append((('pwc', times, vals), o))
+                Lmods.append(None)
+                Lpwcs.append(None)
+                Lsums.append(terms)
+                Lops.append(terms[0][1])
+                continue
+            Lsums.append(None)
             if Lt.mod:
                 g, _, f = Lt.mod_at(i)[0]
                 Lmods.append((g, (id(f), i if np.ndim(f(0.0)) else None)))
@@ -176,6 +191,7 @@ def enumerate_instances(H: TimeQArray, Ls, rho0: QArray, cartesian: bool):
                 Ls=Lops,
                 Lmod=Lmods if any(m is not None for m in Lmods) else [],
                 Lpwc=Lpwcs if any(m is not None for m in Lpwcs) else [],
+                Lsum=Lsums if any(m is not None for m in Lsums) else [],
                 rho0=rho,
                 pwc=Ht.pwc_at(iH),
                 mod=[(g, o, (id(f), iH if np.ndim(f(0.0)) else None)) for (g, o, f) in Ht.mod_at(iH)],
@@ -209,7 +225,7 @@ def mesolve(
     FP64, n <= 64, constant jump operators), ``Expm``. Operators: constant (dense or DIA; constant DIA
     operators from n = 128 on run on the DIA kernels), piecewise-constant
     (``dq.pwc``), modulated and ``timecallable`` Hamiltonians, constant jump
-    operators or one term ``f(t) L`` / ``pwc(times, values, L)``. ``gradient`` is validated as in the
+    operators, modulated or PWC terms and their sums with constant terms. ``gradient`` is validated as in the
     reference and then rejected (no autodiff here; see ``dynamiqs_amd.adjoint``).
     """
     # === convert arguments
@@ -374,7 +390,7 @@ def _mesolve_vectorized(H, Ls, rho0, tsave, exp_ops, method, gradient, options) 
             f'Option `vectorized=True` integrates the {n * n} x {n * n} Liouvillian as a matrix-vector product on the '
             f'MI355X, which supports n^2 <= {VECTORIZED_MAX_N2} (n <= 32); use the default unvectorized form.'
         )
-    if any(inst.Lmod or inst.Lpwc for inst in insts):
+    if any(inst.Lmod or inst.Lpwc or inst.Lsum for inst in insts):
         raise NotImplementedError('Option `vectorized=True` is supported with constant jump operators only.')
     if options.assume_hermitian:
         for inst in insts:

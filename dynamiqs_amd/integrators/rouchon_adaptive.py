@@ -155,7 +155,7 @@ def build_plan(instances: list, tsave, *, order: int, exp_ops=None, t0: float | 
             ' fixed-step method'
         )
     for inst in instances:
-        if any(lm is not None for lm in list(inst.Lmod or []) + list(inst.Lpwc or [])):
+        if any(lm is not None for lm in list(inst.Lmod or []) + list(inst.Lpwc or []) + list(inst.Lsum or [])):
             raise NotImplementedError('the adaptive Rouchon methods take constant jump operators only')
         if inst.Hnh is not None or inst.tsave is not None:
             raise NotImplementedError('explicit generators and per-instance save times are not supported here')

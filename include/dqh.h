@@ -132,6 +132,14 @@ typedef struct dqh_me_inst {
  * generator is among the npwc Hamiltonian terms. This is the reference's
  * pwc(times, values, L) as a jump operator (time_qarray.py:572-666). */
 #define DQH_FLAG_LPWC 256
+/* with DQH_FLAG_LMOD: the right factors take coefficients of their own, C_k <- conj(g_k) C_k
+ * with g_k from table nmod + nL + k (and, with DQH_FLAG_LPWC, row npwc + nL + k of
+ * pwc_vals), while D_k keeps f_k. A jump operator that is a sum of terms
+ * sum_j c_j(t) L_j expands into pairs (c_j L_j, conj(c_j) L_j^dag / 2) and cross pairs
+ * (c_j L_j, conj(c_l) L_l^dag) for j < l, since the vector field adds tmp + tmp^dag
+ * (time_qarray.py:836-956 as a jump operator). The -1/2 L^dag L products are generator
+ * terms of the descriptor. */
+#define DQH_FLAG_LPAIR 512
 
 /* Size checks for FFI mirrors. */
 int64_t dqh_sizeof_me_inst(void);

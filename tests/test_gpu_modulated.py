@@ -319,3 +319,39 @@ def test_pwc_jump_operators_against_oracle():
     ref = om.mesolve(H0, [pw_or], rho0, tsave, method='rouchon1', dt=0.01)
     err = np.linalg.norm(r.states.to_numpy() - ref['states'], axis=(-2, -1)).max()
     assert err <= TOL, err
+
+
+def test_jump_operator_sums_against_oracle():
+    """Jump operators that sum a constant and a modulated term, and a constant and a
+    PWC term (``time_qarray.py:836-956`` as jump operators): the kernels rebuild the
+    D/C pairs of the sum at every stage (DQH_FLAG_LPAIR) and carry the products of
+    -1/2 L^dag L as generator terms. Tsit5 and Rouchon1 against the complex128 oracle
+    at the same tolerances / step."""
+    import dynamiqs_amd as dq
+    from oracle import mesolve as om
+    from oracle import operators as op
+
+    n = 8
+    a = op.destroy(n)
+    ad = a.conj().T
+    H0 = ad @ a + 0.3 * (a + ad)
+    g = lambda t: 0.5 * np.cos(2.0 * t)  # noqa: E731
+    times, vals = np.array([0.0, 0.5, 1.4]), np.array([0.8, 0.3 - 0.5j])
+    rho0 = op.coherent_dm(n, 0.8)
+    tsave = np.linspace(0.0, 2.0, 5)
+    A, N = dq.asqarray(a), dq.asqarray(ad @ a)
+    Ls_dq = [np.sqrt(0.2) * A + dq.modulated(g, N), 0.4 * N + dq.pwc(times, vals, A)]
+    Ls_or = [om.TimeOp(np.sqrt(0.2) * a, [om.ModTerm(g, ad @ a)]), om.TimeOp(0.4 * ad @ a, [om.PWCTerm(times, vals, a)])]
+    # rtol 1e-6 / atol 1e-8: at 1e-7 / 1e-9 the error estimate of the FP32 stages (four D/C
+    # pairs here) nears its rounding floor and the step counts drift by a few steps
+    print('sums of jump-operator terms, Tsit5', flush=True)
+    r = dq.mesolve(dq.asqarray(H0), Ls_dq, dq.asqarray(rho0), tsave, method=dq.method.Tsit5(rtol=1e-6, atol=1e-8))
+    ref = om.mesolve(H0, Ls_or, rho0, tsave, rtol=1e-6, atol=1e-8)
+    err = np.linalg.norm(r.states.to_numpy() - ref['states'], axis=(-2, -1)).max()
+    assert err <= TOL, err
+    assert abs(int(r.infos.nsteps) - ref['infos']['nsteps']) <= 2, (r.infos.nsteps, ref['infos']['nsteps'])
+    print('sums of jump-operator terms, Rouchon1', flush=True)
+    r = dq.mesolve(dq.asqarray(H0), Ls_dq, dq.asqarray(rho0), tsave, method=dq.method.Rouchon1(dt=0.01))
+    ref = om.mesolve(H0, Ls_or, rho0, tsave, method='rouchon1', dt=0.01)
+    err = np.linalg.norm(r.states.to_numpy() - ref['states'], axis=(-2, -1)).max()
+    assert err <= TOL, err
