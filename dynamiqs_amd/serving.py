@@ -17,7 +17,7 @@ others are on the GPU. A scheduler thread sorts them by padded dimension:
   been claimed for ``idle_ms``, and its workgroups stop claiming after ``life_ms``
   (the scheduler relaunches it while requests are outstanding).
 * Large requests run on the grid-wide Tsit5 (each stage spread over many CUs,
-  driven by a worker thread), batched by size class, on two further streams.
+  driven by a worker thread), batched by buckets of padded dimension, on two further streams.
 
 The three streams are created together once per process, so that they get
 distinct hardware queues and the persistent grid never holds a grid-wide launch
@@ -95,12 +95,13 @@ def _align(nbytes: int) -> int:
     return (nbytes + 255) // 256 * 256
 
 
-def _layout(n: int, K: int, nt: int, nE: int, nlog: int = 0):
+def _layout(n: int, K: int, nt: int, nE: int, nlog: int = 0, npd: int | None = None):
     """Byte offsets of one request's buffers, 256-byte aligned: the staged inputs
     (copied from the host, the FP64 state buffers last), the device scratch of the
-    stages (never copied) and the outputs. Returns (input offsets, input bytes,
-    scratch bytes, output offsets, output bytes)."""
-    npd = pad16(n)
+    stages (never copied) and the outputs. ``npd``: the padded dimension the request
+    runs at (default pad16(n)). Returns (input offsets, input bytes, scratch bytes,
+    output offsets, output bytes)."""
+    npd = npd or pad16(n)
     nn = npd * npd
     lay_in, off = {}, 0
     # what the host fills comes first (up to the first FP64 state buffer: 'copy'
@@ -120,16 +121,20 @@ def _layout(n: int, K: int, nt: int, nE: int, nlog: int = 0):
     return lay_in, size_in, work, lay_out, off
 
 
-def footprint_bytes(n: int, K: int, ntsave: int, nE: int = 0, nlog: int = 0) -> int:
+def footprint_bytes(n: int, K: int, ntsave: int, nE: int = 0, nlog: int = 0, npd: int | None = None) -> int:
     """Device bytes one request holds while in flight (inputs, state and scratch,
-    and its outputs when they are on the device)."""
-    _, size_in, work, _, size_out = _layout(n, K, ntsave, nE, nlog)
+    and its outputs when they are on the device), at padded dimension ``npd``."""
+    _, size_in, work, _, size_out = _layout(n, K, ntsave, nE, nlog, npd)
     return size_in + work + size_out
 
 
 # Requests with a padded dimension above this run on the grid-wide Tsit5 (every
 # stage spread over many CUs) instead of one workgroup per request.
 GRID_MIN_NP = 96
+# Under a deep backlog, a grid-wide launch takes the requests of one exact padded
+# dimension when at least this many are pending; otherwise those of its bucket
+# (_grid_bucket), padded to it.
+EXACT_CLASS_MIN = 8
 
 
 class _Launch:
@@ -190,7 +195,7 @@ def _pack_host(reqs: list, base: int, staging: np.ndarray, descs: np.ndarray, la
     outputs and the scratch will have."""
     for b, r in enumerate(reqs):
         n, K, nt, nE = r.n, len(r.Ls), len(r.tsave), len(r.exp_ops)
-        npd = pad16(n)
+        npd = _launch_np(r)
         nn = npd * npd
         li, _, _, lo, _ = lays[b]
         ib, ob = int(in_off[b]), int(out_off[b])
@@ -245,7 +250,7 @@ def pack_and_launch(reqs: list, stream, grid: bool = False, times: dict | None =
     t_start = time.perf_counter()
     dev = torch.device('cuda', torch.cuda.current_device())
     B = len(reqs)
-    lays = [_layout(r.n, len(r.Ls), len(r.tsave), len(r.exp_ops), r.step_log_cap) for r in reqs]
+    lays = [_layout(r.n, len(r.Ls), len(r.tsave), len(r.exp_ops), r.step_log_cap, _launch_np(r)) for r in reqs]
     in_off = np.cumsum([0] + [lay[1] for lay in lays])
     w_off = np.cumsum([0] + [lay[2] for lay in lays])
     out_off = np.cumsum([0] + [lay[4] for lay in lays])
@@ -561,7 +566,7 @@ def pack_queued(reqs: list, pool: _PinnedPool, out_pool: _PinnedPool, times: dic
     t_start = time.perf_counter()
     dev = torch.device('cuda', torch.cuda.current_device())
     B = len(reqs)
-    lays = [_layout(r.n, len(r.Ls), len(r.tsave), len(r.exp_ops), r.step_log_cap) for r in reqs]
+    lays = [_layout(r.n, len(r.Ls), len(r.tsave), len(r.exp_ops), r.step_log_cap, _launch_np(r)) for r in reqs]
     in_off = np.cumsum([0] + [lay[1] for lay in lays])
     w_off = np.cumsum([0] + [lay[2] for lay in lays])
     out_off = np.cumsum([0] + [lay[4] for lay in lays])
@@ -624,20 +629,37 @@ def status_error(status: int, nsteps: int, naccepted: int, nrejected: int) -> Ex
 def _request_bytes(r) -> int:
     """Device bytes a request holds while in flight, with its share of the grid-wide
     Tsit5 scratch when it runs grid-wide."""
-    nb = footprint_bytes(r.n, len(r.Ls), len(r.tsave), len(r.exp_ops), r.step_log_cap)
     npd = pad16(r.n)
     if npd > GRID_MIN_NP:
-        nb += int(_native.lib().dqh_tsit5_grid_scratch_bytes(npd, 1))
-    return nb
+        npd = _grid_bucket(npd)
+        nb = footprint_bytes(r.n, len(r.Ls), len(r.tsave), len(r.exp_ops), r.step_log_cap, npd=npd)
+        return nb + int(_native.lib().dqh_tsit5_grid_scratch_bytes(npd, 1))
+    return footprint_bytes(r.n, len(r.Ls), len(r.tsave), len(r.exp_ops), r.step_log_cap)
+
+
+def _grid_bucket(npd: int) -> int:
+    """Padded dimension a grid-wide request runs at: 128, 160, 192, 224 or 256, then
+    multiples of 64. Requests of one bucket share a launch (the grid-wide kernels take one padded
+    dimension per launch); the extra padding holds exact zeros, and each request keeps
+    its own n for the error norm and the outputs."""
+    for b in (128, 160, 192, 224, 256):
+        if npd <= b:
+            return b
+    return min((npd + 63) // 64 * 64, MAX_NP)
+
+
+def _launch_np(r) -> int:
+    """The padded dimension of a request in its launch (set when a grid-wide batch is formed)."""
+    return getattr(r, 'np_launch', None) or pad16(r.n)
 
 
 def _size_class(r) -> int:
     """Launch grouping: small requests by size band (a persistent launch holds its
-    memory until its slowest request is done), grid-wide requests by exact padded
-    dimension."""
+    memory until its slowest request is done), grid-wide requests by their bucket of
+    padded dimensions."""
     npd = pad16(r.n)
     if npd > GRID_MIN_NP:
-        return 1000 + npd
+        return 1000 + _grid_bucket(npd)
     return 0 if npd <= 32 else (1 if npd <= 64 else 2)
 
 
@@ -745,7 +767,7 @@ class Server:
     def _admit(self) -> None:
         """Small requests (padded dimension below GRID_MIN_NP) go to the work queue as
         soon as they are pending (ring slots and the memory budget permitting): they
-        never wait behind large ones. Large requests are batched by size class, the
+        never wait behind large ones. Large requests are batched by bucket (_grid_bucket), the
         oldest first, into a grid-wide launch when one of max_inflight is free."""
         if not self._pending:
             return
@@ -784,17 +806,26 @@ class Server:
                     self.host_s['launch'] += time.perf_counter() - t1
                     self._count_launch()
         if large and sum(1 for L in self._inflight if not L.queued) < self.max_inflight:
-            cls = _size_class(large[0])
+            # with a deep backlog (more than four full launches), the oldest request's exact
+            # padded dimension, which wastes no work on padding; otherwise its bucket:
+            # fewer, fuller launches that keep the latency of a steady stream low
+            exact = pad16(large[0].n)
+            if len(large) > 4 * self.max_batch and sum(1 for r in large if pad16(r.n) == exact) >= EXACT_CLASS_MIN:
+                launch_np, member = exact, (lambda r: pad16(r.n) == exact)
+            else:
+                launch_np = _grid_bucket(exact)
+                member = lambda r: _grid_bucket(pad16(r.n)) == launch_np  # noqa: E731
             batch, nbytes = [], 0
             for r in large:
                 fb = _request_bytes(r)
-                if (_size_class(r) == cls and len(batch) < self.max_batch
-                        and self._used + nbytes + fb <= self.memory_budget):
+                if member(r) and len(batch) < self.max_batch and self._used + nbytes + fb <= self.memory_budget:
                     batch.append(r)
                     nbytes += fb
                 else:
                     keep.append(r)
             if batch:
+                for r in batch:  # one padded dimension per grid-wide launch
+                    r.np_launch = launch_np
                 t0 = time.perf_counter()
                 stream = self.streams[self._next_stream % len(self.streams)]
                 self._next_stream += 1
