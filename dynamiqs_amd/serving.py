@@ -677,14 +677,29 @@ class Server:
         queue_capacity: ring slots (a power of two): requests queued or in service.
         idle_ms: the work-queue grid leaves after this long without a claim.
         life_ms: its workgroups stop claiming after this long (then it is relaunched).
+            Work queued behind the persistent kernel in the same hardware queue waits
+            up to this long, so it is kept short.
+
+    The server's four streams (two for the work queue, two grid-wide) and the
+    process's other streams should each have a hardware queue of their own: with
+    HIP's default of 4 (``GPU_MAX_HW_QUEUES``) a grid-wide launch can share one with
+    the persistent kernel and wait for its end. Set ``GPU_MAX_HW_QUEUES=16`` before the
+    process initialises HIP (``bench.py`` does); a warning is issued otherwise.
     """
 
     def __init__(self, memory_budget: int = 8 << 30, max_batch: int = 64, nstreams: int = 2, max_inflight: int = 8,
-                 queue_workgroups: int = 128, queue_capacity: int = 4096, idle_ms: float = 20.0, life_ms: float = 200.0):
+                 queue_workgroups: int = 128, queue_capacity: int = 4096, idle_ms: float = 20.0, life_ms: float = 50.0):
+        import os
+        import warnings
+
         import torch
 
         if not torch.cuda.is_available():
             raise RuntimeError('dynamiqs_amd.serving needs an MI355X GPU')
+        if int(os.environ.get('GPU_MAX_HW_QUEUES', '4')) < 8:
+            warnings.warn('dynamiqs_amd.serving: fewer than 8 hardware queues (GPU_MAX_HW_QUEUES); grid-wide launches '
+                          'may wait behind the persistent work-queue kernel. Set GPU_MAX_HW_QUEUES=16 before HIP starts.',
+                          stacklevel=2)
         _native.lib()
         self.memory_budget = int(memory_budget)
         self.max_batch = int(max_batch)
