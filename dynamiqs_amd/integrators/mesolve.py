@@ -108,8 +108,7 @@ def dia_route(method, H: TimeQArray, Ls, n: int):
         # PWC / modulated Hamiltonian terms on the DIA Tsit5 sweeps (the per-stage
         # generator rebuild of dqh_mesolve_tsit5_dia) where the dense kernels cannot take
         # the size: un-batched operators, constant jump operators
-        if (n > dense.MAX_NP and isinstance(method, ADAPTIVE) and (Ht.pwc or Ht.mod) and Ht.batch == ()
-                and all(Lt.batch == () for Lt in Lts)):
+        if n > dense.MAX_NP and isinstance(method, ADAPTIVE) and (Ht.pwc or Ht.mod):
             return 'timedep', Ht, Lts
         return None
     if Ht.batch == () and all(Lt.batch == () for Lt in Lts):
@@ -547,7 +546,10 @@ def _dia_groups(Ht, Lts, rho0: QArray, cartesian: bool):
     out = []
     for (iH, iLs), members in groups.items():
         ops = (Ht.const_at(iH), [Lt.const_at(i) for Lt, i in zip(Lts, iLs)])
-        out.append((ops, members))
+        terms = {}
+        if Ht.pwc or Ht.mod:  # time-dependent Hamiltonian terms of this batch element
+            terms = dict(pwc=Ht.pwc_at(iH), mod=[(g, o) for (g, o, _) in Ht.mod_at(iH)])
+        out.append((ops, members, terms))
     return out_shape, out
 
 
@@ -558,26 +560,19 @@ def _mesolve_dia(route, rho0: QArray, tsave, exp_ops, method, gradient, options)
     distinct operator set (each launch is grid-wide over the n^2 elements)."""
     r_all = rho0.to_numpy().astype(np.complex128)
     n = r_all.shape[-2]
-    terms = {}
-    if isinstance(route[0], str) and route[0] == 'batched':
+    if isinstance(route[0], str):  # 'batched' or 'timedep': one plan per batch element of the operators
         out_shape, groups = _dia_groups(route[1], route[2], rho0, options.cartesian_batching)
-    elif isinstance(route[0], str) and route[0] == 'timedep':
-        Ht, Lts = route[1], route[2]
-        terms = dict(pwc=Ht.pwc_at(()), mod=[(g, o) for (g, o, _) in Ht.mod_at(())])
-        out_shape = r_all.shape[:-2]
-        groups = [((Ht.const_at(()), [Lt.const_at(()) for Lt in Lts]),
-                   [(pos, idx) for pos, idx in enumerate(np.ndindex(*out_shape) if out_shape else [()])])]
     else:
         out_shape = r_all.shape[:-2]
         rb = out_shape
-        groups = [(route, [(pos, idx) for pos, idx in enumerate(np.ndindex(*rb) if rb else [()])])]
+        groups = [(route, [(pos, idx) for pos, idx in enumerate(np.ndindex(*rb) if rb else [()])], {})]
     Es = [E.to_numpy() for E in exp_ops] if exp_ops is not None else None
     nout = int(np.prod(out_shape)) if out_shape else 1
     states_all = None
     expects_all = np.zeros((nout, len(Es), len(tsave)), dtype=np.complex64) if Es is not None else None
     infos_all = np.zeros((3, nout), dtype=np.int64)
     plan = None
-    for (Hd, Ld), members in groups:
+    for (Hd, Ld), members, terms in groups:
         r = np.stack([r_all[ir] for (_, ir) in members])
         st, ex, inf, plan = _dia_solve(Hd, Ld, r, tsave, Es, method, options, **terms)
         pos = np.array([p for (p, _) in members])

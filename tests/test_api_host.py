@@ -420,8 +420,9 @@ def test_dia_batched_operator_groups():
     ad = a.to_numpy()
     out, groups = _dia_groups(Ht, Lts, rho, True)
     assert out == (2, 3, 2) and len(groups) == 6
-    assert sorted(p for _, m in groups for p, _ in m) == list(range(12))
-    for (Hd, Ld), members in groups:
+    assert sorted(p for _, m, _ in groups for p, _ in m) == list(range(12))
+    for (Hd, Ld), members, terms in groups:
+        assert terms == {}
         for pos, ir in members:
             i, j, k = np.unravel_index(pos, out)
             assert ir == (k,)
@@ -430,7 +431,7 @@ def test_dia_batched_operator_groups():
     H2 = a.dag() @ a + np.array([0.1, 0.3, 0.5])[:, None, None] * (a + a.dag())
     out, groups = _dia_groups(decompose(astimeqarray(H2)), Lts, dq.coherent(n, 1.0), False)
     assert out == (3,) and len(groups) == 3
-    assert all(len(m) == 1 and m[0][1] == () for _, m in groups)
+    assert all(len(m) == 1 and m[0][1] == () for _, m, _ in groups)
 
 
 def test_jump_operator_sums_expand_to_the_lindbladian():
