@@ -33,7 +33,8 @@ import numpy as np
 # The serving measurement runs four streams of its own beside a persistent kernel; with
 # HIP's default of 4 hardware queues a grid-wide launch can share one with that kernel
 # and wait for its end. Set before anything initialises HIP (every GPU use is below).
-os.environ.setdefault('GPU_MAX_HW_QUEUES', '16')
+if int(os.environ.get('GPU_MAX_HW_QUEUES', '4')) < 16:  # raised, never lowered
+    os.environ['GPU_MAX_HW_QUEUES'] = '16'
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:

@@ -8,7 +8,8 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 # the serving tests run the persistent work-queue kernel beside grid-wide launches on
 # streams of their own: one hardware queue each (set before HIP starts)
-os.environ.setdefault('GPU_MAX_HW_QUEUES', '16')
+if int(os.environ.get('GPU_MAX_HW_QUEUES', '4')) < 16:  # raised, never lowered
+    os.environ['GPU_MAX_HW_QUEUES'] = '16'
 
 
 def pytest_configure(config):
