@@ -821,11 +821,11 @@ class Server:
                     self.host_s['launch'] += time.perf_counter() - t1
                     self._count_launch()
         if large and sum(1 for L in self._inflight if not L.queued) < self.max_inflight:
-            # with a deep backlog (more than four full launches), the oldest request's exact
+            # with a deep backlog (more than two full launches), the oldest request's exact
             # padded dimension, which wastes no work on padding; otherwise its bucket:
             # fewer, fuller launches that keep the latency of a steady stream low
             exact = pad16(large[0].n)
-            if len(large) > 4 * self.max_batch and sum(1 for r in large if pad16(r.n) == exact) >= EXACT_CLASS_MIN:
+            if len(large) > 2 * self.max_batch and sum(1 for r in large if pad16(r.n) == exact) >= EXACT_CLASS_MIN:
                 launch_np, member = exact, (lambda r: pad16(r.n) == exact)
             else:
                 launch_np = _grid_bucket(exact)
