@@ -174,9 +174,10 @@ def test_tsit5_pwc_parity():
 
 def test_printed_step_counts_through_mesolve_and_sesolve(kats):
     """The reference prints these infos for its README example (``README.md:75``),
-    its mesolve docstring (``dynamiqs/integrators/apis/mesolve.py:225``) and its
-    sesolve docstring (``dynamiqs/integrators/apis/sesolve.py:164``). The
-    kernels' default Tsit5 takes exactly those step counts."""
+    its mesolve docstring (``dynamiqs/integrators/apis/mesolve.py:225``), its
+    sesolve docstring (``dynamiqs/integrators/apis/sesolve.py:164``) and its
+    workflow tutorial (Dopri5, ``docs/documentation/basics/workflow.md:88``). The
+    kernels take exactly those step counts."""
     import dynamiqs_amd as dq
 
     k = kats['readme_lossy_oscillator']
@@ -193,6 +194,13 @@ def test_printed_step_counts_through_mesolve_and_sesolve(kats):
     print('sesolve docstring example', flush=True)
     r = dq.sesolve(a.dag() @ a, dq.coherent(s['n'], s['alpha0']), np.linspace(0, s['T'], s['ntsave']))
     assert str(r.infos) == f"{s['nsteps']} steps ({s['naccepted']} accepted, {s['nrejected']} rejected)"
+    # the workflow tutorial's Rabi run with Dopri5 at atol 1e-8 (docs/documentation/basics/workflow.md:88)
+    w = kats['workflow_rabi_dopri5']
+    print('workflow tutorial, Dopri5', flush=True)
+    r = dq.sesolve(w['delta'] * dq.sigmaz() + w['Omega'] * dq.sigmax(), dq.fock(2, 0),
+                   np.linspace(0.0, w['sim_time'], w['num_save']),
+                   method=dq.method.Dopri5(rtol=w['rtol'], atol=w['atol']))
+    assert str(r.infos) == f"{w['nsteps']} steps ({w['naccepted']} accepted, {w['nrejected']} rejected)"
 
 
 def test_max_steps_raises_reference_error():
